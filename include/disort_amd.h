@@ -1,0 +1,159 @@
+/*
+ * disort_amd.h -- C ABI of the MI355X DISORT library (libdisort_amd.so).
+ *
+ * The reference (DisORT++, NewStrangeWorlds/DisORT) has no C ABI: its hot path
+ * is reached through C++ classes and a pybind11 module.  Each entry point
+ * below names the reference interface it replaces; the C++ facade in
+ * disort_amd/csrc/host (DisortFluxSolver<N>, solveFluxSpectralBands,
+ * solveSpectralBands) and the pybind11 module `disortpp` sit on top of it.
+ *
+ * Conventions: plain pointers and sizes, no exceptions cross the boundary,
+ * every function returns DISORT_OK (0) or a DISORT_E_* code and leaves a
+ * message in disort_last_error(ctx).  A context owns one HIP stream on one
+ * device; calls on one context are not re-entrant (as a reference solver
+ * object is not: src/FluxSolver.hpp:80-194 keeps per-solve scratch).
+ */
+#ifndef DISORT_AMD_H
+#define DISORT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes -------------------------------------------------------- */
+enum {
+  DISORT_OK = 0,
+  DISORT_E_INVALID = 1,     /* input fails the reference's validate() rules     */
+  DISORT_E_UNSUPPORTED = 2, /* configuration outside what the GPU path solves   */
+  DISORT_E_NUMERIC = 3,     /* an instance hit a numerical failure (see status) */
+  DISORT_E_HIP = 4,         /* HIP runtime error                                 */
+  DISORT_E_NOMEM = 5
+};
+
+/* Per-instance status codes written to disort_flux_batch.status[i]. */
+enum {
+  DISORT_ST_OK = 0,
+  DISORT_ST_EIGEN = 1       /* neither symmetrisation of the layer eigenproblem
+                               is positive definite (degenerate layer)          */
+};
+
+/* ---- batch flags (uniform over a batch) ---------------------------------- */
+enum {
+  DISORT_F_THERMAL = 1,       /* DisortFluxConfig::use_thermal_emission      */
+  DISORT_F_SPHERICAL = 2,     /* use_spherical_beam (not on the GPU path yet) */
+  DISORT_F_DELTA_M_PLUS = 4,  /* use_delta_m_plus                            */
+  DISORT_F_DIFFUSION_BC = 8   /* use_diffusion_lower_bc                      */
+};
+
+/* ---- per-instance scalar block: scal[i*DISORT_NSCAL + slot] ------------- */
+enum {
+  DISORT_S_BEAM_FLUX = 0, /* direct_beam_flux       */
+  DISORT_S_BEAM_MU = 1,   /* direct_beam_mu         */
+  DISORT_S_ISO_TOP = 2,   /* isotropic_flux_top     */
+  DISORT_S_ISO_BOT = 3,   /* isotropic_flux_bottom  */
+  DISORT_S_TEMP_TOP = 4,  /* temperature_top        */
+  DISORT_S_TEMP_BOT = 5,  /* temperature_bottom     */
+  DISORT_S_EMIS_TOP = 6,  /* emissivity_top         */
+  DISORT_S_ALBEDO = 7,    /* surface_albedo         */
+  DISORT_S_WN_LO = 8,     /* wavenumber_low         */
+  DISORT_S_WN_HI = 9,     /* wavenumber_high        */
+  DISORT_S_RADIUS = 10,   /* bottom_radius          */
+  DISORT_NSCAL = 12
+};
+
+/* ---- output fields: out[field*total_levels + level] ---------------------- */
+/* Same order as the members of FluxResult (src/FluxResult.hpp:14-22). */
+enum {
+  DISORT_O_FLUX_DIRECT_BEAM = 0,
+  DISORT_O_FLUX_DOWN = 1,
+  DISORT_O_FLUX_UP = 2,
+  DISORT_O_FLUX_TAU_DIVERGENCE = 3,
+  DISORT_O_MEAN_INTENSITY = 4,
+  DISORT_O_MEAN_INTENSITY_DOWN = 5,
+  DISORT_O_MEAN_INTENSITY_UP = 6,
+  DISORT_O_MEAN_INTENSITY_DIRECT_BEAM = 7,
+  DISORT_NOUT = 8
+};
+
+enum { DISORT_MEM_HOST = 0, DISORT_MEM_DEVICE = 1 };
+
+/*
+ * A batch of independent flux-only DISORT instances (one instance = one
+ * (wavenumber band x column) solve of DisortFluxSolver<N>::solve,
+ * src/FluxSolver.cpp:23-171).  Layouts are structure-of-arrays with
+ * per-instance offsets, so stream count and layer count may differ between
+ * instances ("ragged" batches).  Instance i owns
+ *   layers  dtau[lay_off[i] .. lay_off[i]+nlyr[i])       (same for ssa)
+ *   levels  temper[lay_off[i]+i .. +nlyr[i]+1)            (same for alt, out)
+ *   moments pmom[mom_off[i] + lc*pstride[i] + k], k = 0..pstride[i]-1
+ * Arrays are indexed from the top of the atmosphere (index_from_bottom is
+ * handled by the host facade).  pstride[i] >= nstr[i]+1 (>= nstr[i]+2 with
+ * DISORT_F_DELTA_M_PLUS), mirroring phase_function_moments[lc][0..nmomNstr()].
+ */
+typedef struct disort_flux_batch {
+  int64_t n;
+  int32_t flags;
+  int32_t reserved;
+  const int32_t* nstr;
+  const int32_t* nlyr;
+  const int32_t* pstride;
+  const int64_t* lay_off;
+  const int64_t* mom_off;
+  const double* dtau;
+  const double* ssa;
+  const double* pmom;
+  const double* temper; /* NULL unless DISORT_F_THERMAL */
+  const double* alt;    /* NULL unless DISORT_F_SPHERICAL */
+  const double* scal;
+  double* out;
+  int64_t total_levels; /* sum over i of nlyr[i]+1 */
+  int32_t* status;      /* [n], may be NULL */
+} disort_flux_batch;
+
+typedef struct disort_ctx disort_ctx;
+
+/* Context: one device, one stream, grow-only device scratch. */
+int disort_ctx_create(int device, disort_ctx** ctx);
+int disort_ctx_destroy(disort_ctx* ctx);
+const char* disort_last_error(const disort_ctx* ctx);
+void* disort_ctx_stream(disort_ctx* ctx); /* the hipStream_t the kernels run on */
+int disort_sync(disort_ctx* ctx);
+
+/*
+ * Batched flux solve.  Replaces the OpenMP band loop of
+ * solveFluxSpectralBandsImpl<N> (src/python_bindings.cpp:170-190) and the
+ * per-instance DisortFluxSolver<N>::solve (src/FluxSolver.cpp:23-171); with
+ * DISORT_MEM_HOST the pointers are host memory (copied in and out
+ * synchronously), with DISORT_MEM_DEVICE they are device memory and the call
+ * is asynchronous on the context stream.  Supported on the GPU: nstr in
+ * {4,6,...,16}, plane-parallel beam, thermal emission, Lambertian surface,
+ * Delta-M / Delta-M+, layer cut-off, diffusion lower boundary.
+ */
+int disort_flux_solve(disort_ctx* ctx, const disort_flux_batch* batch, int memspace);
+
+/* Device memory helpers for callers without their own allocator (ctypes). */
+int disort_device_alloc(disort_ctx* ctx, size_t bytes, void** ptr);
+int disort_device_free(disort_ctx* ctx, void* ptr);
+int disort_memcpy_h2d(disort_ctx* ctx, void* dst, const void* src, size_t bytes);
+int disort_memcpy_d2h(disort_ctx* ctx, void* dst, const void* src, size_t bytes);
+
+/*
+ * Per-stage device timings of the most recent disort_flux_solve on this
+ * context, in milliseconds, measured with hipEvents on the context stream
+ * when timing is enabled.  Stages: 0 column set-up, 1 layer eigen-system,
+ * 2 boundary solve, 3 flux assembly.  Returns the number of stages written.
+ */
+int disort_set_timing(disort_ctx* ctx, int enable);
+int disort_stage_times(disort_ctx* ctx, double* ms, int max_stages);
+
+/* Library version string. */
+const char* disort_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DISORT_AMD_H */

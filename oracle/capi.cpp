@@ -1,0 +1,164 @@
+// C entry points of the oracle for ctypes (tests and bench.py's cpu_baseline).
+// TEST INFRASTRUCTURE ONLY (see oracle.hpp).
+#include <chrono>
+#include <cstring>
+#include <exception>
+#include <stdexcept>
+#include <string>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/disort_amd.h"
+#include "oracle.hpp"
+
+namespace {
+void put_err(char* err, int errlen, const char* msg) {
+  if (err && errlen > 0) {
+    std::strncpy(err, msg, errlen - 1);
+    err[errlen - 1] = 0;
+  }
+}
+
+oracle::FluxConfig from_batch(const disort_flux_batch* b, int64_t i) {
+  oracle::FluxConfig c;
+  c.nstr = b->nstr[i];
+  c.nlyr = b->nlyr[i];
+  c.nmom = b->pstride[i] - 1;  // rows hold nmom+1 = pstride entries (>= nstr+1)
+  c.thermal = (b->flags & DISORT_F_THERMAL) != 0;
+  c.spherical = (b->flags & DISORT_F_SPHERICAL) != 0;
+  c.deltam_plus = (b->flags & DISORT_F_DELTA_M_PLUS) != 0;
+  c.diffusion_bc = (b->flags & DISORT_F_DIFFUSION_BC) != 0;
+  const double* s = b->scal + i * DISORT_NSCAL;
+  c.beam_flux = s[DISORT_S_BEAM_FLUX];
+  c.beam_mu = s[DISORT_S_BEAM_MU];
+  c.iso_top = s[DISORT_S_ISO_TOP];
+  c.iso_bot = s[DISORT_S_ISO_BOT];
+  c.temp_top = s[DISORT_S_TEMP_TOP];
+  c.temp_bot = s[DISORT_S_TEMP_BOT];
+  c.emis_top = s[DISORT_S_EMIS_TOP];
+  c.albedo = s[DISORT_S_ALBEDO];
+  c.wn_lo = s[DISORT_S_WN_LO];
+  c.wn_hi = s[DISORT_S_WN_HI];
+  c.radius = s[DISORT_S_RADIUS];
+  const int64_t lo = b->lay_off[i], vo = b->lay_off[i] + i;
+  c.dtau.assign(b->dtau + lo, b->dtau + lo + c.nlyr);
+  c.ssa.assign(b->ssa + lo, b->ssa + lo + c.nlyr);
+  c.pmom.assign(b->pmom + b->mom_off[i], b->pmom + b->mom_off[i] + (size_t)c.nlyr * b->pstride[i]);
+  if (c.thermal && b->temper) c.temper.assign(b->temper + vo, b->temper + vo + c.nlyr + 1);
+  if (c.spherical && b->alt) c.alt.assign(b->alt + vo, b->alt + vo + c.nlyr + 1);
+  return c;
+}
+}  // namespace
+
+extern "C" {
+
+int oracle_planck(double lo, double hi, double t, double* out, char* err, int errlen) {
+  try {
+    *out = oracle::planck_band(lo, hi, t);
+    return 0;
+  } catch (const std::exception& e) {
+    put_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+int oracle_planck_deriv(double lo, double hi, double t, double* out, char* err, int errlen) {
+  try {
+    *out = oracle::planck_band_deriv(lo, hi, t);
+    return 0;
+  } catch (const std::exception& e) {
+    put_err(err, errlen, e.what());
+    return 1;
+  }
+}
+
+void oracle_legendre_flat(int nmu, int m, int maxmu, int twonm1, const double* mu, double* ylm) {
+  oracle::legendre_flat(nmu, m, maxmu, twonm1, mu, ylm);
+}
+
+int oracle_gauss(int nn, double* cmu, double* cwt) {
+  try {
+    oracle::gauss_quadrature(nn, cmu, cwt);
+    return 0;
+  } catch (...) {
+    return 1;
+  }
+}
+
+void oracle_band_factor(double* abd, int lda, int n, int ml, int mu, int* ipvt) {
+  oracle::band_factor(abd, lda, n, ml, mu, ipvt);
+}
+
+void oracle_band_solve(const double* abd, int lda, int n, int ml, int mu, const int* ipvt, double* b) {
+  oracle::band_solve(abd, lda, n, ml, mu, ipvt, b);
+}
+
+void oracle_sym_eigen(int n, const double* a, double* evals, double* evecs) {
+  oracle::sym_eigen_jacobi(n, a, evals, evecs);
+}
+
+int oracle_cholesky(int n, const double* a, double* l) { return oracle::cholesky_lower(n, a, l) ? 0 : 1; }
+
+double oracle_chapman(int lc, double tau_frac, const double* zd, const double* dtau, int nlyr, double r, double mu0) {
+  return oracle::chapman(lc, tau_frac, zd, dtau, nlyr, r, mu0);
+}
+
+/* One instance of the batch; out is [8][nlyr+1].  Returns 0, 1 (invalid
+ * argument) or 2 (runtime error), with the message in err. */
+int oracle_flux_one(const disort_flux_batch* b, int64_t i, int index_from_bottom, double* out, char* err, int errlen) {
+  try {
+    oracle::FluxConfig c = from_batch(b, i);
+    c.index_from_bottom = index_from_bottom != 0;
+    oracle::FluxOutput o = oracle::flux_solve(c);
+    const int nl = c.nlyr + 1;
+    for (int f = 0; f < 8; ++f) std::memcpy(out + (size_t)f * nl, o.f[f].data(), sizeof(double) * nl);
+    return 0;
+  } catch (const std::invalid_argument& e) {
+    put_err(err, errlen, e.what());
+    return 1;
+  } catch (const std::exception& e) {
+    put_err(err, errlen, e.what());
+    return 2;
+  }
+}
+
+/* Whole batch into b->out ([8][total_levels]) with an OpenMP static loop over
+ * instances -- the structure of solveFluxSpectralBandsImpl<N>
+ * (src/python_bindings.cpp:170-190).  Returns the number of failed instances;
+ * *seconds receives the wall time of the loop. */
+int64_t oracle_flux_batch(const disort_flux_batch* b, int nthreads, double* seconds) {
+  int64_t failed = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+#pragma omp parallel for schedule(static) reduction(+ : failed)
+  for (int64_t i = 0; i < b->n; ++i) {
+    try {
+      oracle::FluxConfig c = from_batch(b, i);
+      oracle::FluxOutput o = oracle::flux_solve(c);
+      const int64_t vo = b->lay_off[i] + i;
+      for (int f = 0; f < 8; ++f)
+        for (int l = 0; l <= c.nlyr; ++l) b->out[f * b->total_levels + vo + l] = o.f[f][l];
+      if (b->status) b->status[i] = 0;
+    } catch (...) {
+      ++failed;
+      if (b->status) b->status[i] = 1;
+    }
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return failed;
+}
+
+int oracle_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) and, where it was
+built, to the reference's own Eigen-free numerics (oracle/_ref).
+
+Test infrastructure: the oracle is the checker here, never the thing tested.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from disort_amd.batch import NOUT, FluxBatch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_ORACLE = os.path.join(ROOT, "oracle", "liboracle.so")
+_REF = os.path.join(ROOT, "oracle", "_ref", "libref_parts.so")
+
+_lib = None
+_ref = None
+
+D = C.c_double
+PD = C.POINTER(C.c_double)
+PI = C.POINTER(C.c_int)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_ORACLE):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        l = C.CDLL(_ORACLE)
+        l.oracle_planck.argtypes = [D, D, D, PD, C.c_char_p, C.c_int]
+        l.oracle_planck_deriv.argtypes = [D, D, D, PD, C.c_char_p, C.c_int]
+        l.oracle_legendre_flat.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        l.oracle_gauss.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        l.oracle_band_factor.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        l.oracle_band_solve.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        l.oracle_sym_eigen.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+        l.oracle_cholesky.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        l.oracle_chapman.argtypes = [C.c_int, D, C.c_void_p, C.c_void_p, C.c_int, D, D]
+        l.oracle_chapman.restype = D
+        l.oracle_flux_one.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_char_p, C.c_int]
+        l.oracle_flux_batch.argtypes = [C.c_void_p, C.c_int, PD]
+        l.oracle_flux_batch.restype = C.c_int64
+        _lib = l
+    return _lib
+
+
+def ref():
+    """The reference's Planck/Legendre/BandSolver, or None when not built."""
+    global _ref
+    if _ref is None and os.path.exists(_REF):
+        r = C.CDLL(_REF)
+        r.ref_planck2.argtypes = [D, D, D, PD]
+        r.ref_planck_deriv2.argtypes = [D, D, D, PD]
+        r.ref_legendre_flat.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        r.ref_band_factor.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p]
+        r.ref_band_solve.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        _ref = r
+    return _ref
+
+
+class OracleError(Exception):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+def planck(lo, hi, t):
+    out = D()
+    err = C.create_string_buffer(256)
+    rc = lib().oracle_planck(lo, hi, t, C.byref(out), err, 256)
+    if rc:
+        raise OracleError(rc, err.value.decode())
+    return out.value
+
+
+def planck_deriv(lo, hi, t):
+    out = D()
+    err = C.create_string_buffer(256)
+    rc = lib().oracle_planck_deriv(lo, hi, t, C.byref(out), err, 256)
+    if rc:
+        raise OracleError(rc, err.value.decode())
+    return out.value
+
+
+def gauss(nn):
+    cmu = np.zeros(2 * nn)
+    cwt = np.zeros(2 * nn)
+    assert lib().oracle_gauss(nn, cmu.ctypes.data, cwt.ctypes.data) == 0
+    return cmu, cwt
+
+
+def legendre_flat(mu, m, twonm1, ylm=None):
+    mu = np.ascontiguousarray(mu, dtype=np.float64)
+    nmu = mu.shape[0]
+    if ylm is None:
+        ylm = np.zeros((twonm1 + 2, nmu))
+    lib().oracle_legendre_flat(nmu, m, nmu - 1, twonm1, mu.ctypes.data, ylm.ctypes.data)
+    return ylm
+
+
+def flux_one(batch: FluxBatch, i: int, index_from_bottom=False):
+    """Oracle solve of instance i; returns an [8, nlyr+1] array."""
+    L = int(batch.nlyr[i])
+    out = np.zeros((NOUT, L + 1))
+    err = C.create_string_buffer(512)
+    st = batch.c_struct()
+    rc = lib().oracle_flux_one(C.byref(st), i, int(index_from_bottom), out.ctypes.data, err, 512)
+    if rc:
+        raise OracleError(rc, err.value.decode())
+    return out
+
+
+def flux_batch(batch: FluxBatch, nthreads=0):
+    """Oracle solve of the whole batch into batch.out; returns seconds."""
+    st = batch.c_struct()
+    secs = D()
+    failed = lib().oracle_flux_batch(C.byref(st), nthreads, C.byref(secs))
+    if failed:
+        raise OracleError(2, f"{failed} oracle instances failed")
+    return secs.value

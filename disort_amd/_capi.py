@@ -1,0 +1,217 @@
+"""ctypes binding of libdisort_amd.so (the C ABI in include/disort_amd.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` into
+``disort_amd/lib/libdisort_amd.so``.  There is no CPU fallback: if the
+library is missing or no GPU is present, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .batch import NOUT, CFluxBatch, FluxBatch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libdisort_amd.so")
+
+DISORT_OK = 0
+DISORT_E_INVALID = 1
+DISORT_E_UNSUPPORTED = 2
+DISORT_E_NUMERIC = 3
+MEM_HOST = 0
+MEM_DEVICE = 1
+
+_lib = None
+
+
+class DisortError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[disort code {code}] {msg}")
+        self.code = code
+
+
+class DisortInvalid(DisortError, ValueError):
+    pass
+
+
+def lib():
+    """Load libdisort_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build the HIP library first (__graft_entry__.build())"
+            )
+        l = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        l.disort_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+        l.disort_ctx_destroy.argtypes = [vp]
+        l.disort_last_error.argtypes = [vp]
+        l.disort_last_error.restype = C.c_char_p
+        l.disort_ctx_stream.argtypes = [vp]
+        l.disort_ctx_stream.restype = vp
+        l.disort_sync.argtypes = [vp]
+        l.disort_flux_solve.argtypes = [vp, vp, C.c_int]
+        l.disort_flux_plan_create.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_int32, C.POINTER(vp)]
+        l.disort_flux_plan_destroy.argtypes = [vp, vp]
+        l.disort_flux_execute.argtypes = [vp, vp, vp]
+        l.disort_flux_status.argtypes = [vp, vp, vp]
+        l.disort_device_alloc.argtypes = [vp, C.c_size_t, C.POINTER(vp)]
+        l.disort_device_free.argtypes = [vp, vp]
+        l.disort_memcpy_h2d.argtypes = [vp, vp, vp, C.c_size_t]
+        l.disort_memcpy_d2h.argtypes = [vp, vp, vp, C.c_size_t]
+        l.disort_set_timing.argtypes = [vp, C.c_int]
+        l.disort_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.c_int]
+        l.disort_version.restype = C.c_char_p
+        _lib = l
+    return _lib
+
+
+def _check(ctx_ptr, rc):
+    if rc != DISORT_OK:
+        msg = lib().disort_last_error(ctx_ptr).decode(errors="replace")
+        if rc == DISORT_E_INVALID:
+            raise DisortInvalid(rc, msg)
+        raise DisortError(rc, msg)
+
+
+class Context:
+    """One device, one HIP stream (disort_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._ptr = C.c_void_p()
+        rc = lib().disort_ctx_create(device, C.byref(self._ptr))
+        if rc != DISORT_OK:
+            raise DisortError(rc, "disort_ctx_create failed (is a GPU visible?)")
+
+    @property
+    def ptr(self):
+        return self._ptr
+
+    def close(self):
+        if self._ptr:
+            lib().disort_ctx_destroy(self._ptr)
+            self._ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, on: bool):
+        lib().disort_set_timing(self._ptr, int(on))
+
+    def stage_times(self):
+        ms = (C.c_double * 4)()
+        n = lib().disort_stage_times(self._ptr, ms, 4)
+        return list(ms[:n])
+
+    def sync(self):
+        _check(self._ptr, lib().disort_sync(self._ptr))
+
+    def solve(self, batch: FluxBatch) -> FluxBatch:
+        """Solve a host batch in place (batch.out, batch.status)."""
+        st = batch.c_struct()
+        rc = lib().disort_flux_solve(self._ptr, C.byref(st), MEM_HOST)
+        _check(self._ptr, rc)
+        return batch
+
+    # -- device helpers -----------------------------------------------------
+    def alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        _check(self._ptr, lib().disort_device_alloc(self._ptr, max(int(nbytes), 1), C.byref(p)))
+        return p.value
+
+    def free(self, p: int):
+        lib().disort_device_free(self._ptr, C.c_void_p(p))
+
+    def h2d(self, dst: int, arr: np.ndarray):
+        arr = np.ascontiguousarray(arr)
+        _check(self._ptr, lib().disort_memcpy_h2d(self._ptr, C.c_void_p(dst), arr.ctypes.data, arr.nbytes))
+
+    def d2h(self, arr: np.ndarray, src: int):
+        _check(self._ptr, lib().disort_memcpy_d2h(self._ptr, arr.ctypes.data, C.c_void_p(src), arr.nbytes))
+
+
+class DeviceFluxBatch:
+    """A FluxBatch resident in device memory plus its plan, for repeated
+    asynchronous solves (disort_flux_plan_create / disort_flux_execute)."""
+
+    def __init__(self, ctx: Context, batch: FluxBatch):
+        self.ctx = ctx
+        self.host = batch
+        self._bufs = []
+        l = lib()
+        self.plan = C.c_void_p()
+        rc = l.disort_flux_plan_create(
+            ctx.ptr,
+            batch.n,
+            batch.nstr.ctypes.data,
+            batch.nlyr.ctypes.data,
+            batch.pstride.ctypes.data,
+            int(batch.flags),
+            C.byref(self.plan),
+        )
+        _check(ctx.ptr, rc)
+
+        def up(a):
+            if a is None:
+                return None
+            p = ctx.alloc(a.nbytes)
+            self._bufs.append(p)
+            ctx.h2d(p, a)
+            return p
+
+        s = CFluxBatch()
+        s.n = batch.n
+        s.flags = int(batch.flags)
+        s.nstr = up(batch.nstr)
+        s.nlyr = up(batch.nlyr)
+        s.pstride = up(batch.pstride)
+        s.lay_off = up(batch.lay_off)
+        s.mom_off = up(batch.mom_off)
+        s.dtau = up(batch.dtau)
+        s.ssa = up(batch.ssa)
+        s.pmom = up(batch.pmom)
+        s.temper = up(batch.temper)
+        s.alt = None
+        s.scal = up(batch.scal)
+        self.out_ptr = ctx.alloc(batch.out.nbytes)
+        self._bufs.append(self.out_ptr)
+        s.out = self.out_ptr
+        s.total_levels = int(batch.total_levels)
+        s.status = None
+        self.cstruct = s
+
+    def execute(self):
+        """Enqueue one solve on the context stream (asynchronous)."""
+        _check(self.ctx.ptr, lib().disort_flux_execute(self.ctx.ptr, self.plan, C.byref(self.cstruct)))
+
+    def download(self) -> FluxBatch:
+        self.ctx.sync()
+        self.ctx.d2h(self.host.out, self.out_ptr)
+        st = np.zeros(self.host.n, np.int32)
+        _check(self.ctx.ptr, lib().disort_flux_status(self.ctx.ptr, self.plan, st.ctypes.data))
+        self.host.status[:] = st
+        return self.host
+
+    def close(self):
+        if self.plan:
+            lib().disort_flux_plan_destroy(self.ctx.ptr, self.plan)
+            self.plan = C.c_void_p()
+        for p in self._bufs:
+            self.ctx.free(p)
+        self._bufs = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def version() -> str:
+    return lib().disort_version().decode()

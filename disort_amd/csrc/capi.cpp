@@ -1,0 +1,515 @@
+// Host implementation of the C ABI declared in include/disort_amd.h.
+//
+// Replaces the OpenMP band loop of solveFluxSpectralBandsImpl<N>
+// (src/python_bindings.cpp:170-190): instances are grouped into buckets of
+// equal stream count, each bucket is solved by four kernel launches on the
+// context stream (flux_kernels.hip), scratch lives in grow-only device
+// buffers owned by the context.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/disort_amd.h"
+#include "flux_args.hpp"
+#include "flux_layout.hpp"
+
+namespace dsa {
+bool launch_flux_bucket(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev);
+}
+
+using dsa::FluxArgs;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct disort_flux_plan;
+
+struct disort_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool timing = false;
+  hipEvent_t ev[5] = {};
+  double stage_ms[4] = {0, 0, 0, 0};
+  std::map<std::string, DevBuf> bufs;
+  std::map<int, double*> tables;  // per stream count
+};
+
+namespace {
+
+#define HIPCHK(ctx, call)                                                              \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) {                                                            \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                  \
+      return DISORT_E_HIP;                                                             \
+    }                                                                                  \
+  } while (0)
+
+int fail(disort_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+// Grow-only named device buffer.
+int get_buf(disort_ctx* ctx, const char* name, size_t bytes, void** out) {
+  DevBuf& b = ctx->bufs[name];
+  if (b.bytes < bytes) {
+    if (b.p) {
+      HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+      HIPCHK(ctx, hipFree(b.p));
+      b.p = nullptr;
+      b.bytes = 0;
+    }
+    const size_t want = std::max<size_t>(bytes, 256);
+    HIPCHK(ctx, hipMalloc(&b.p, want));
+    b.bytes = want;
+  }
+  *out = b.p;
+  return DISORT_OK;
+}
+
+// Double-Gauss quadrature on (0,1): Newton iteration on P_n (tolerance
+// 3e-14, at most 10 steps) as in src/FluxSolver.cpp:257-290, so that nodes
+// and weights agree with the reference to the last bits.
+bool gauss_nodes(int n, std::vector<double>& cmu, std::vector<double>& cwt) {
+  cmu.assign(2 * n, 0.0);
+  cwt.assign(2 * n, 0.0);
+  for (int i = 1; i <= n; ++i) {
+    double z = std::cos(M_PI * (i - 0.25) / (n + 0.5)), z1, dp;
+    int it = 0;
+    do {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      dp = n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / dp;
+      if (++it > 10) return false;
+    } while (std::abs(z - z1) > 3.0e-14);
+    cmu[i - 1] = 0.5 * (1.0 + z);
+    cwt[i - 1] = 1.0 / ((1.0 - z * z) * dp * dp);
+  }
+  for (int i = 0; i < n; ++i) {
+    cmu[n + i] = -cmu[i];
+    cwt[n + i] = cwt[i];
+  }
+  return true;
+}
+
+int table_for(disort_ctx* ctx, int nstr, double** out) {
+  auto it = ctx->tables.find(nstr);
+  if (it != ctx->tables.end()) {
+    *out = it->second;
+    return DISORT_OK;
+  }
+  const int nn = nstr / 2;
+  std::vector<double> cmu, cwt;
+  if (!gauss_nodes(nn, cmu, cwt)) return fail(ctx, DISORT_E_NUMERIC, "gaussianQuadrature: Failed to converge");
+  dsa::TabOff to(nstr);
+  std::vector<double> tab(to.size(), 0.0);
+  for (int k = 0; k < nstr; ++k) {
+    tab[to.cmu() + k] = cmu[k];
+    tab[to.cwt() + k] = cwt[k];
+  }
+  for (int r = 0; r < nn; ++r) {
+    const double sd = std::sqrt(cwt[r] * cmu[r]);
+    tab[to.sd() + r] = sd;
+    tab[to.sdi() + r] = 1.0 / sd;
+    // Legendre P_l(mu_r) by upward recurrence (src/Legendre.cpp:73-90).
+    std::vector<double> p(nstr, 0.0);
+    p[0] = 1.0;
+    if (nstr > 1) p[1] = cmu[r];
+    for (int l = 2; l < nstr; ++l) p[l] = ((2.0 * l - 1.0) * cmu[r] * p[l - 1] - (l - 1.0) * p[l - 2]) / l;
+    const double sc = std::sqrt(cwt[r] / cmu[r]);
+    for (int l = 0; l < nstr; ++l) {
+      tab[to.P() + l * nn + r] = p[l];
+      tab[to.Q() + l * nn + r] = p[l] * sc;
+    }
+  }
+  double* d = nullptr;
+  HIPCHK(ctx, hipMalloc(&d, tab.size() * sizeof(double)));
+  HIPCHK(ctx, hipMemcpy(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  ctx->tables[nstr] = d;
+  *out = d;
+  return DISORT_OK;
+}
+
+bool gpu_supported_nstr(int n) { return n >= 4 && n <= 16 && n % 2 == 0; }
+
+// Host-side validation of one instance, with the reference's messages
+// (src/DisortFluxConfig.cpp:43-143).
+std::string validate_instance(const disort_flux_batch* b, int64_t i) {
+  const int L = b->nlyr[i], N = b->nstr[i], P = b->pstride[i];
+  auto at = [&](const std::string& m) { return "instance " + std::to_string(i) + ": " + m; };
+  if (L <= 0) return at("num_layers must be > 0");
+  if (N < 4 || N % 2 != 0) return at("num_streams must be even and >= 4");
+  if (P < N + 1) return at("phase_function_moments[0] size < nmomNstr()+1");
+  if ((b->flags & DISORT_F_DELTA_M_PLUS) && P < N + 2)
+    return at("use_delta_m_plus requires num_phase_func_moments >= num_streams + 1");
+  const int64_t lo = b->lay_off[i], vo = lo + i;
+  for (int lc = 0; lc < L; ++lc) {
+    if (b->dtau[lo + lc] < 0.0) return at("delta_tau[" + std::to_string(lc) + "] must be >= 0");
+    const double w = b->ssa[lo + lc];
+    if (w < 0.0 || w > 1.0) return at("single_scat_albedo[" + std::to_string(lc) + "] must be in [0, 1]");
+  }
+  const double* s = b->scal + i * DISORT_NSCAL;
+  if ((b->flags & DISORT_F_DIFFUSION_BC) && !(b->flags & DISORT_F_THERMAL))
+    return at("use_diffusion_lower_bc requires use_thermal_emission = true");
+  if (s[DISORT_S_BEAM_FLUX] < 0.0) return at("direct_beam_flux must be >= 0");
+  if (s[DISORT_S_BEAM_FLUX] > 0.0 && (s[DISORT_S_BEAM_MU] <= 0.0 || s[DISORT_S_BEAM_MU] > 1.0))
+    return at("direct_beam_mu must be in (0, 1] when direct_beam_flux > 0");
+  if (s[DISORT_S_ALBEDO] < 0.0 || s[DISORT_S_ALBEDO] > 1.0) return at("surface_albedo must be in [0, 1]");
+  if (b->flags & DISORT_F_THERMAL) {
+    if (!b->temper) return at("temperature size < num_layers + 1");
+    for (int l = 0; l <= L; ++l)
+      if (b->temper[vo + l] < 0.0) return at("temperature[" + std::to_string(l) + "] must be >= 0");
+    if (s[DISORT_S_WN_LO] < 0.0 || s[DISORT_S_WN_LO] > s[DISORT_S_WN_HI])
+      return at("wavenumber: need 0 <= wavenumber_low <= wavenumber_high");
+    if (s[DISORT_S_TEMP_TOP] < 0.0 || s[DISORT_S_TEMP_BOT] < 0.0)
+      return at("temperature_top and temperature_bottom must be >= 0");
+    if (s[DISORT_S_EMIS_TOP] < 0.0 || s[DISORT_S_EMIS_TOP] > 1.0) return at("emissivity_top must be in [0, 1]");
+  }
+  return std::string();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Plans: bucket schedule computed from host-side shapes.
+// ---------------------------------------------------------------------------
+struct Bucket {
+  int nstr = 0;
+  int64_t nb = 0, n_tasks = 0, n_lv = 0;
+  int32_t* d_inst = nullptr;
+  int64_t* d_task0 = nullptr;
+  int32_t* d_t_inst = nullptr;
+  int32_t* d_t_lc = nullptr;
+  int32_t* d_v_bk = nullptr;
+  int32_t* d_v_lu = nullptr;
+};
+
+struct disort_flux_plan {
+  int64_t n = 0, total_layers = 0, total_levels = 0;
+  int32_t flags = 0;
+  std::vector<int64_t> lay_off, mom_off;
+  int64_t* d_lay_off = nullptr;
+  int64_t* d_mom_off = nullptr;
+  int32_t* d_nstr = nullptr;
+  int32_t* d_nlyr = nullptr;
+  int32_t* d_pstride = nullptr;
+  std::vector<Bucket> buckets;
+  int64_t max_tasks = 0, max_eig = 0, max_panel = 0, max_cvec = 0;
+  std::vector<void*> owned;
+};
+
+extern "C" {
+
+const char* disort_version(void) { return "disort_amd 0.1 (gfx950)"; }
+
+int disort_ctx_create(int device, disort_ctx** out) {
+  if (!out) return DISORT_E_INVALID;
+  auto ctx = std::make_unique<disort_ctx>();
+  ctx->device = device;
+  HIPCHK(ctx, hipSetDevice(device));
+  HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  for (auto& e : ctx->ev) HIPCHK(ctx, hipEventCreate(&e));
+  *out = ctx.release();
+  return DISORT_OK;
+}
+
+int disort_ctx_destroy(disort_ctx* ctx) {
+  if (!ctx) return DISORT_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& kv : ctx->bufs)
+    if (kv.second.p) (void)hipFree(kv.second.p);
+  for (auto& kv : ctx->tables) (void)hipFree(kv.second);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return DISORT_OK;
+}
+
+const char* disort_last_error(const disort_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void* disort_ctx_stream(disort_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int disort_sync(disort_ctx* ctx) {
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DISORT_OK;
+}
+
+int disort_set_timing(disort_ctx* ctx, int enable) {
+  ctx->timing = enable != 0;
+  return DISORT_OK;
+}
+
+int disort_stage_times(disort_ctx* ctx, double* ms, int max_stages) {
+  const int n = std::min(max_stages, 4);
+  for (int k = 0; k < n; ++k) ms[k] = ctx->stage_ms[k];
+  return n;
+}
+
+int disort_device_alloc(disort_ctx* ctx, size_t bytes, void** ptr) {
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  HIPCHK(ctx, hipMalloc(ptr, std::max<size_t>(bytes, 1)));
+  return DISORT_OK;
+}
+
+int disort_device_free(disort_ctx* ctx, void* ptr) {
+  HIPCHK(ctx, hipFree(ptr));
+  return DISORT_OK;
+}
+
+int disort_memcpy_h2d(disort_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DISORT_OK;
+}
+
+int disort_memcpy_d2h(disort_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  HIPCHK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return DISORT_OK;
+}
+
+int disort_flux_plan_destroy(disort_ctx* ctx, disort_flux_plan* plan) {
+  if (!plan) return DISORT_OK;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  for (void* p : plan->owned) (void)hipFree(p);
+  delete plan;
+  return DISORT_OK;
+}
+
+int disort_flux_plan_create(disort_ctx* ctx, int64_t n, const int32_t* nstr, const int32_t* nlyr,
+                            const int32_t* pstride, int32_t flags, disort_flux_plan** out) {
+  if (!ctx || !out || n <= 0 || !nstr || !nlyr || !pstride) return fail(ctx, DISORT_E_INVALID, "invalid plan arguments");
+  if (flags & DISORT_F_SPHERICAL)
+    return fail(ctx, DISORT_E_UNSUPPORTED, "use_spherical_beam is not supported by the GPU flux path");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  auto plan = std::make_unique<disort_flux_plan>();
+  plan->n = n;
+  plan->flags = flags;
+  plan->lay_off.resize(n);
+  plan->mom_off.resize(n);
+  int64_t lay = 0, mom = 0;
+  std::map<int, std::vector<int32_t>> groups;
+  for (int64_t i = 0; i < n; ++i) {
+    if (nlyr[i] <= 0) return fail(ctx, DISORT_E_INVALID, "instance " + std::to_string(i) + ": num_layers must be > 0");
+    if (nstr[i] < 4 || nstr[i] % 2)
+      return fail(ctx, DISORT_E_INVALID, "instance " + std::to_string(i) + ": num_streams must be even and >= 4");
+    if (!gpu_supported_nstr(nstr[i]))
+      return fail(ctx, DISORT_E_UNSUPPORTED,
+                  "instance " + std::to_string(i) + ": num_streams=" + std::to_string(nstr[i]) +
+                      " is not yet supported by the GPU flux path (4..16)");
+    if (pstride[i] < nstr[i] + 1 + ((flags & DISORT_F_DELTA_M_PLUS) ? 1 : 0))
+      return fail(ctx, DISORT_E_INVALID, "instance " + std::to_string(i) + ": phase moment rows too short");
+    plan->lay_off[i] = lay;
+    plan->mom_off[i] = mom;
+    lay += nlyr[i];
+    mom += (int64_t)nlyr[i] * pstride[i];
+    groups[nstr[i]].push_back((int32_t)i);
+  }
+  plan->total_layers = lay;
+  plan->total_levels = lay + n;
+  auto upload = [&](const void* src, size_t bytes, void** dst) -> int {
+    HIPCHK(ctx, hipMalloc(dst, std::max<size_t>(bytes, 8)));
+    plan->owned.push_back(*dst);
+    HIPCHK(ctx, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return DISORT_OK;
+  };
+  int rc;
+  if ((rc = upload(plan->lay_off.data(), n * sizeof(int64_t), (void**)&plan->d_lay_off))) return rc;
+  if ((rc = upload(plan->mom_off.data(), n * sizeof(int64_t), (void**)&plan->d_mom_off))) return rc;
+  if ((rc = upload(nstr, n * sizeof(int32_t), (void**)&plan->d_nstr))) return rc;
+  if ((rc = upload(nlyr, n * sizeof(int32_t), (void**)&plan->d_nlyr))) return rc;
+  if ((rc = upload(pstride, n * sizeof(int32_t), (void**)&plan->d_pstride))) return rc;
+  for (auto& kv : groups) {
+    Bucket bk;
+    bk.nstr = kv.first;
+    const auto& inst = kv.second;
+    bk.nb = (int64_t)inst.size();
+    std::vector<int64_t> task0(bk.nb);
+    std::vector<int32_t> t_inst, t_lc, v_bk, v_lu;
+    for (int64_t k = 0; k < bk.nb; ++k) {
+      const int32_t i = inst[k];
+      task0[k] = (int64_t)t_inst.size();
+      for (int lc = 0; lc < nlyr[i]; ++lc) {
+        t_inst.push_back(i);
+        t_lc.push_back(lc);
+      }
+      for (int lu = 0; lu <= nlyr[i]; ++lu) {
+        v_bk.push_back((int32_t)k);
+        v_lu.push_back(lu);
+      }
+    }
+    bk.n_tasks = (int64_t)t_inst.size();
+    bk.n_lv = (int64_t)v_bk.size();
+    if ((rc = upload(inst.data(), inst.size() * 4, (void**)&bk.d_inst))) return rc;
+    if ((rc = upload(task0.data(), task0.size() * 8, (void**)&bk.d_task0))) return rc;
+    if ((rc = upload(t_inst.data(), t_inst.size() * 4, (void**)&bk.d_t_inst))) return rc;
+    if ((rc = upload(t_lc.data(), t_lc.size() * 4, (void**)&bk.d_t_lc))) return rc;
+    if ((rc = upload(v_bk.data(), v_bk.size() * 4, (void**)&bk.d_v_bk))) return rc;
+    if ((rc = upload(v_lu.data(), v_lu.size() * 4, (void**)&bk.d_v_lu))) return rc;
+    plan->max_tasks = std::max(plan->max_tasks, bk.n_tasks);
+    plan->max_eig = std::max<int64_t>(plan->max_eig, (int64_t)dsa::EigOff(bk.nstr).size() * bk.n_tasks);
+    plan->max_panel = std::max<int64_t>(plan->max_panel, (int64_t)dsa::panel_stride(bk.nstr) * bk.n_tasks);
+    plan->max_cvec = std::max<int64_t>(plan->max_cvec, (int64_t)bk.nstr * bk.n_tasks);
+    plan->buckets.push_back(bk);
+  }
+  *out = plan.release();
+  return DISORT_OK;
+}
+
+// Launch a planned solve on device-resident batch arrays (asynchronous).
+// The batch's lay_off/mom_off are ignored: the plan's offsets are used.
+int disort_flux_execute(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b) {
+  if (!ctx || !plan || !b) return fail(ctx, DISORT_E_INVALID, "null argument");
+  if (b->n != plan->n) return fail(ctx, DISORT_E_INVALID, "batch size differs from the plan");
+  if ((b->flags & DISORT_F_THERMAL) && !b->temper) return fail(ctx, DISORT_E_INVALID, "thermal batch without temperatures");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  FluxArgs a{};
+  a.flags = b->flags;
+  a.nstr = plan->d_nstr;
+  a.nlyr = plan->d_nlyr;
+  a.pstride = plan->d_pstride;
+  a.lay_off = plan->d_lay_off;
+  a.mom_off = plan->d_mom_off;
+  a.dtau = b->dtau;
+  a.ssa = b->ssa;
+  a.pmom = b->pmom;
+  a.temper = b->temper;
+  a.scal = b->scal;
+  a.out = b->out;
+  a.total_levels = plan->total_levels;
+  a.total_layers = plan->total_layers;
+  int rc;
+  void* p;
+  if ((rc = get_buf(ctx, "inst_d", plan->n * dsa::kInstD * sizeof(double), &p))) return rc;
+  a.inst_d = (double*)p;
+  if ((rc = get_buf(ctx, "inst_i", plan->n * dsa::kInstI * sizeof(int32_t), &p))) return rc;
+  a.inst_i = (int32_t*)p;
+  if ((rc = get_buf(ctx, "lay_d", plan->total_layers * dsa::kLayD * sizeof(double), &p))) return rc;
+  a.lay_d = (double*)p;
+  if ((rc = get_buf(ctx, "lev_d", plan->total_levels * dsa::kLevD * sizeof(double), &p))) return rc;
+  a.lev_d = (double*)p;
+  if ((rc = get_buf(ctx, "eig", plan->max_eig * sizeof(double), &p))) return rc;
+  a.eig = (double*)p;
+  if ((rc = get_buf(ctx, "panel", plan->max_panel * sizeof(double), &p))) return rc;
+  a.panel = (double*)p;
+  if ((rc = get_buf(ctx, "cvec", plan->max_cvec * sizeof(double), &p))) return rc;
+  a.cvec = (double*)p;
+  for (int k = 0; k < 4; ++k) ctx->stage_ms[k] = 0.0;
+  for (const Bucket& bk : plan->buckets) {
+    double* tab = nullptr;
+    if ((rc = table_for(ctx, bk.nstr, &tab))) return rc;
+    a.tab = tab;
+    a.nb = (int32_t)bk.nb;
+    a.b_inst = bk.d_inst;
+    a.b_task0 = bk.d_task0;
+    a.n_tasks = bk.n_tasks;
+    a.t_inst = bk.d_t_inst;
+    a.t_lc = bk.d_t_lc;
+    a.n_lvtasks = bk.n_lv;
+    a.v_bk = bk.d_v_bk;
+    a.v_lu = bk.d_v_lu;
+    if (!dsa::launch_flux_bucket(bk.nstr, a, ctx->stream, ctx->timing ? ctx->ev : nullptr))
+      return fail(ctx, DISORT_E_UNSUPPORTED, "no kernels for num_streams=" + std::to_string(bk.nstr));
+    HIPCHK(ctx, hipGetLastError());
+    if (ctx->timing) {
+      HIPCHK(ctx, hipEventSynchronize(ctx->ev[4]));
+      for (int k = 0; k < 4; ++k) {
+        float ms = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
+        ctx->stage_ms[k] += ms;
+      }
+    }
+  }
+  return DISORT_OK;
+}
+
+// Per-instance status codes of the last execute (copied to host).
+int disort_flux_status(disort_ctx* ctx, disort_flux_plan* plan, int32_t* status_host) {
+  void* p = nullptr;
+  int rc;
+  if ((rc = get_buf(ctx, "inst_i", plan->n * dsa::kInstI * sizeof(int32_t), &p))) return rc;
+  std::vector<int32_t> ii(plan->n * dsa::kInstI);
+  HIPCHK(ctx, hipMemcpyAsync(ii.data(), p, ii.size() * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int64_t i = 0; i < plan->n; ++i) status_host[i] = ii[i * dsa::kInstI + dsa::II_STATUS];
+  return DISORT_OK;
+}
+
+int disort_flux_solve(disort_ctx* ctx, const disort_flux_batch* b, int memspace) {
+  if (!ctx || !b) return fail(ctx, DISORT_E_INVALID, "null argument");
+  if (memspace == DISORT_MEM_DEVICE)
+    return fail(ctx, DISORT_E_INVALID, "device-resident batches go through disort_flux_plan_create/disort_flux_execute");
+  if (b->n <= 0) return DISORT_OK;
+  for (int64_t i = 0; i < b->n; ++i) {
+    const std::string m = validate_instance(b, i);
+    if (!m.empty()) return fail(ctx, DISORT_E_INVALID, m);
+  }
+  disort_flux_plan* plan = nullptr;
+  int rc = disort_flux_plan_create(ctx, b->n, b->nstr, b->nlyr, b->pstride, b->flags, &plan);
+  if (rc) return rc;
+  std::unique_ptr<disort_flux_plan, void (*)(disort_flux_plan*)> guard(plan, [](disort_flux_plan* p) {
+    for (void* q : p->owned) (void)hipFree(q);
+    delete p;
+  });
+  const int64_t TL = plan->total_layers, TV = plan->total_levels;
+  int64_t mom_total = 0;
+  for (int64_t i = 0; i < b->n; ++i) mom_total += (int64_t)b->nlyr[i] * b->pstride[i];
+  struct Up {
+    const char* name;
+    const void* src;
+    size_t bytes;
+    void** dst;
+  };
+  disort_flux_batch d = *b;
+  void *p_dtau, *p_ssa, *p_pmom, *p_temp = nullptr, *p_scal, *p_out;
+  std::vector<Up> ups = {{"in_dtau", b->dtau, (size_t)TL * 8, &p_dtau},
+                         {"in_ssa", b->ssa, (size_t)TL * 8, &p_ssa},
+                         {"in_pmom", b->pmom, (size_t)mom_total * 8, &p_pmom},
+                         {"in_scal", b->scal, (size_t)b->n * DISORT_NSCAL * 8, &p_scal}};
+  if (b->flags & DISORT_F_THERMAL) ups.push_back({"in_temp", b->temper, (size_t)TV * 8, &p_temp});
+  for (auto& u : ups) {
+    if ((rc = get_buf(ctx, u.name, u.bytes, u.dst))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(*u.dst, u.src, u.bytes, hipMemcpyHostToDevice, ctx->stream));
+  }
+  if ((rc = get_buf(ctx, "out", (size_t)DISORT_NOUT * TV * 8, &p_out))) return rc;
+  d.dtau = (const double*)p_dtau;
+  d.ssa = (const double*)p_ssa;
+  d.pmom = (const double*)p_pmom;
+  d.temper = (const double*)p_temp;
+  d.scal = (const double*)p_scal;
+  d.out = (double*)p_out;
+  d.total_levels = TV;
+  if ((rc = disort_flux_execute(ctx, plan, &d))) return rc;
+  HIPCHK(ctx, hipMemcpyAsync(b->out, p_out, (size_t)DISORT_NOUT * TV * 8, hipMemcpyDeviceToHost, ctx->stream));
+  std::vector<int32_t> st(b->n);
+  if ((rc = disort_flux_status(ctx, plan, st.data()))) return rc;
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  int64_t bad = 0;
+  for (int64_t i = 0; i < b->n; ++i) {
+    if (b->status) b->status[i] = st[i];
+    bad += st[i] != 0;
+  }
+  if (bad) return fail(ctx, DISORT_E_NUMERIC, std::to_string(bad) + " instance(s) failed (see status)");
+  return DISORT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,46 @@
+// Kernel argument block for the flux kernels (device pointers only).
+#pragma once
+#include <cstdint>
+
+namespace dsa {
+
+struct FluxArgs {
+  // Batch (device copies of disort_flux_batch arrays).
+  int32_t flags;
+  int32_t pad0;
+  const int32_t* nstr;
+  const int32_t* nlyr;
+  const int32_t* pstride;
+  const int64_t* lay_off;
+  const int64_t* mom_off;
+  const double* dtau;
+  const double* ssa;
+  const double* pmom;
+  const double* temper;
+  const double* scal;
+  double* out;
+  int64_t total_levels;
+  int64_t total_layers;
+  // Scratch shared by all buckets.
+  double* inst_d;   // [n][kInstD]
+  int32_t* inst_i;  // [n][kInstI]
+  double* lay_d;    // [kLayD][total_layers]
+  double* lev_d;    // [kLevD][total_levels]
+  // Bucket (instances with the same stream count).
+  int32_t nb;                // instances in the bucket
+  int32_t pad1;
+  const int32_t* b_inst;     // [nb] batch instance index
+  const int64_t* b_task0;    // [nb] first layer task of the instance
+  int64_t n_tasks;           // layer tasks (sum of nlyr over the bucket)
+  const int32_t* t_inst;     // [n_tasks] batch instance index
+  const int32_t* t_lc;       // [n_tasks] layer index
+  int64_t n_lvtasks;         // level tasks (sum of nlyr+1)
+  const int32_t* v_bk;       // [n_lvtasks] bucket-local instance index
+  const int32_t* v_lu;       // [n_lvtasks] level index
+  double* eig;               // [EigOff::size][n_tasks]
+  double* panel;             // [n_tasks][panel_stride]
+  double* cvec;              // [n_tasks][N]
+  const double* tab;         // TabOff layout for this N
+};
+
+}  // namespace dsa

@@ -1,0 +1,968 @@
+// gfx950 kernels of the batched flux-only DISORT solve.
+//
+// One instance = one DisortFluxSolver<N>::solve (src/FluxSolver.cpp:23-171).
+// The solve is split into four launches over a bucket of instances that share
+// the stream count N = 2*NN:
+//   k_column  one wavefront per instance: Delta-M / Delta-M+ scaling, scaled
+//             and unscaled optical-depth prefix sums, absorption cut-off,
+//             beam transmission, Planck function per level
+//             (src/FluxSolver.cpp:309-452, :104-122; src/Planck.cpp:35-169).
+//   k_layer   one lane per (instance, layer): the reduced NN x NN eigenproblem
+//             (src/FluxSolver.cpp:459-588) solved in registers by Cholesky
+//             symmetrisation and cyclic Jacobi, then the beam and thermal
+//             particular solutions (:595-643, :745-831) expressed through
+//             that eigen-decomposition instead of N x N LU solves.
+//   k_boundary 16 lanes per instance: the (N*L)-unknown boundary system of
+//             setMatrix/solve0 (:838-1131) eliminated layer by layer with
+//             Householder QR on 3NN x N panels (rows: the NN equations
+//             carried from above plus the N continuity equations of the next
+//             interface), then block back substitution.
+//   k_flux    one lane per (instance, level): intensities at the quadrature
+//             angles and the eight flux / mean-intensity outputs (:1138-1287).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/disort_amd.h"
+#include "flux_args.hpp"
+#include "flux_layout.hpp"
+
+namespace dsa {
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kEps = 2.220446049250313e-16;     // DBL_EPSILON
+constexpr double kLogDblMax = 709.782712893384;    // log(DBL_MAX)
+
+// ---------------------------------------------------------------------------
+// Band-integrated Planck function (W m^-2 sr^-1 over [lo, hi] cm^-1), the
+// algorithm of src/Planck.cpp:35-169 (Simpson refinement for narrow bands,
+// power series below v = 1.5, exponential series above), with the constants
+// of src/Planck.hpp:10-22.  Arguments are validated on the host.
+// ---------------------------------------------------------------------------
+__device__ inline double planck_shape(double v) { return v * v * v / (exp(v) - 1.0); }
+
+__device__ double planck_band(double lo, double hi, double t) {
+  const double c2 = 1.438786, sigma = 5.67032e-8;
+  if (t < 1.e-4) return 0.0;
+  if (hi == lo) {
+    const double arg = exp(-c2 * hi / t);
+    return 1.1911e-8 * hi * hi * hi * arg / (1.0 - arg);
+  }
+  const double sigdpi = sigma / kPi;
+  const double conc = 15.0 / (kPi * kPi * kPi * kPi);
+  const double t4 = t * t * t * t;
+  const double v0 = c2 * lo / t, v1 = c2 * hi / t;
+  if (v0 > kEps && v1 < kLogDblMax && (hi - lo) / hi < 1.e-2) {
+    const double span = v1 - v0;
+    const double ends = planck_shape(v0) + planck_shape(v1);
+    double prev = 0.0;
+    for (int n = 1; n <= 10; ++n) {
+      const double h = span / (2.0 * n);
+      double acc = ends;
+      for (int k = 1; k <= 2 * n - 1; ++k) acc += 2.0 * (1 + k % 2) * planck_shape(v0 + k * h);
+      acc *= h * (1.0 / 3.0);
+      if (fabs((acc - prev) / acc) <= 1.e-6) return sigdpi * t4 * conc * acc;
+      prev = acc;
+    }
+    return sigdpi * t4 * conc * prev;
+  }
+  const double a1 = 1.0 / 3.0, a2 = -1.0 / 8.0, a3 = 1.0 / 60.0;
+  const double a4 = -1.0 / 5040.0, a5 = 1.0 / 272160.0, a6 = -1.0 / 13305600.0;
+  const double vcp[7] = {10.25, 5.7, 3.9, 2.9, 2.3, 1.9, 0.0};
+  double p[2] = {0.0, 0.0}, d[2] = {0.0, 0.0};
+  const double v[2] = {v0, v1};
+  int nsmall = 0;
+  for (int i = 0; i < 2; ++i) {
+    if (v[i] < 1.5) {
+      ++nsmall;
+      const double vs = v[i] * v[i];
+      p[i] = conc * vs * v[i] * (a1 + v[i] * (a2 + v[i] * (a3 + vs * (a4 + vs * (a5 + vs * a6)))));
+    } else {
+      int terms = 1;
+      while (v[i] < vcp[terms - 1]) ++terms;
+      const double ex = exp(-v[i]);
+      double exm = 1.0, s = 0.0;
+      for (int m = 1; m <= terms; ++m) {
+        const double mv = m * v[i];
+        exm *= ex;
+        s += exm * (6.0 + mv * (6.0 + mv * (3.0 + mv))) / (double)(m * m * m * m);
+      }
+      d[i] = s * conc;
+    }
+  }
+  double ans;
+  if (nsmall == 2) ans = p[1] - p[0];
+  else if (nsmall == 1) ans = 1.0 - p[0] - d[1];
+  else ans = d[0] - d[1];
+  return ans * (sigdpi * t4);
+}
+
+// Broadcast lane `src` of each row of 16 lanes to the whole row (DPP row_newbcast).
+template <int SRC>
+__device__ __forceinline__ double row_bcast(double x) {
+  static_assert(SRC >= 0 && SRC < 16, "row lane");
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0x150 + SRC, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0x150 + SRC, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Compile-time loop helper: f(integral_constant<int, I>) for I in [B, E).
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+__host__ __device__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }  // i >= j
+
+// ===========================================================================
+// k_column: per-instance set-up (one wavefront per instance).
+// ===========================================================================
+template <int N>
+__global__ __launch_bounds__(64) void k_column(FluxArgs a) {
+  const int bk = blockIdx.x;
+  if (bk >= a.nb) return;
+  const int lane = threadIdx.x;
+  const int i = a.b_inst[bk];
+  const int L = a.nlyr[i];
+  const int P = a.pstride[i];
+  const int64_t lo = a.lay_off[i], vo = lo + i;
+  const double* pm = a.pmom + a.mom_off[i];
+  const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
+  const bool thermal = (a.flags & DISORT_F_THERMAL) != 0;
+  const bool want_dmp = (a.flags & DISORT_F_DELTA_M_PLUS) != 0;
+  const int64_t TL = a.total_layers, TV = a.total_levels;
+
+  // src/FluxSolver.cpp:56-81: delta-M when any pmom[N] != 0; Delta-M+ only if
+  // every layer has pmom[N] >= 1e-4 and pmom[N+1] >= 0.7 pmom[N].
+  int any_dm = 0, dmp_bad = 0;
+  for (int lc = lane; lc < L; lc += 64) {
+    const double pN = pm[(int64_t)lc * P + N];
+    any_dm |= (pN != 0.0);
+    if (want_dmp) {
+      const double pN1 = pm[(int64_t)lc * P + N + 1];
+      dmp_bad |= (pN < 1e-4 || pN1 < 0.7 * pN);
+    }
+  }
+  const bool dmp = want_dmp && !__any(dmp_bad);
+  const bool deltam = !dmp && __any(any_dm);
+
+  // Per-layer scaling (src/FluxSolver.cpp:333-389).
+  for (int lc = lane; lc < L; lc += 64) {
+    const int64_t g = lo + lc;
+    const double w = a.ssa[g], dt = a.dtau[g];
+    double f = 0.0, sig2 = -1.0, ws = w, dts = dt;
+    if (deltam || dmp) {
+      const double pN = pm[(int64_t)lc * P + N];
+      sig2 = 0.0;
+      f = pN;
+      if (dmp && pm[(int64_t)lc * P + N - 1] != pN) {
+        const double pN1 = pm[(int64_t)lc * P + N + 1];
+        if (pN != pN1) {
+          const double M = (double)N;
+          sig2 = ((M + 1.0) * (M + 1.0) - M * M) / (log(pN * pN) - log(pN1 * pN1));
+          f = exp(M * M / (2.0 * sig2)) * pN;
+        }
+      }
+      ws = w * (1.0 - f) / (1.0 - f * w);
+      dts = (1.0 - f * w) * dt;
+    }
+    a.lay_d[LD_DTAU_S * TL + g] = dts;
+    a.lay_d[LD_SSA_S * TL + g] = ws;
+    a.lay_d[LD_F * TL + g] = f;
+    a.lay_d[LD_SIG2 * TL + g] = sig2;
+  }
+  __syncthreads();
+
+  const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
+  if (lane == 0) {
+    // Prefix sums, absorption cut-off and beam transmission (:311-331, :392-435).
+    double taus = 0.0, tauc = 0.0, abst = 0.0;
+    int ncut = L;
+    a.lev_d[VD_TAU_S * TV + vo] = 0.0;
+    a.lev_d[VD_TAUC * TV + vo] = 0.0;
+    for (int lc = 0; lc < L; ++lc) {
+      const int64_t g = lo + lc;
+      if (abst < 10.0) ncut = lc + 1;
+      abst += (1.0 - a.ssa[g]) * a.dtau[g];
+      a.lay_d[LD_TAU_TOP * TL + g] = taus;
+      taus += a.lay_d[LD_DTAU_S * TL + g];
+      tauc += a.dtau[g];
+      a.lev_d[VD_TAU_S * TV + vo + lc + 1] = taus;
+      a.lev_d[VD_TAUC * TV + vo + lc + 1] = tauc;
+    }
+    // Transmission as seen by the boundary conditions: exp(-tau'/mu0) down to
+    // the absorption cut-off level and zero below it, as a freshly constructed
+    // reference solver leaves it (beam_transmission_ is only filled for
+    // lc < ncut before the cut-off decision, src/FluxSolver.cpp:417-421).
+    double bt_prev = (F0 > 0.0) ? 1.0 : 0.0;
+    for (int lc = 0; lc < L; ++lc) {
+      const int64_t g = lo + lc;
+      double bt = 0.0;
+      if (F0 > 0.0 && lc < ncut) bt = exp(-a.lev_d[VD_TAU_S * TV + vo + lc + 1] / mu0);
+      a.lay_d[LD_BT_TOP * TL + g] = bt_prev;
+      a.lay_d[LD_BT_BOT * TL + g] = bt;
+      bt_prev = bt;
+    }
+    const bool lyrcut = (abst >= 10.0 && !thermal && L > 1);
+    const int ncut_f = lyrcut ? ncut : L;
+    a.inst_i[i * kInstI + II_NCUT] = ncut_f;
+    a.inst_i[i * kInstI + II_LYRCUT] = lyrcut ? 1 : 0;
+    a.inst_i[i * kInstI + II_STATUS] = 0;
+    a.inst_d[i * kInstD + ID_BTRANS_NCUT] = (ncut_f == 0) ? bt_prev : a.lay_d[LD_BT_BOT * TL + lo + ncut_f - 1];
+  }
+  // Planck function per level (src/FluxSolver.cpp:104-122, :748-752).
+  if (thermal) {
+    const double wlo = sc[DISORT_S_WN_LO], whi = sc[DISORT_S_WN_HI];
+    for (int l = lane; l <= L; l += 64) a.lev_d[VD_PLANCK * TV + vo + l] = planck_band(wlo, whi, a.temper[vo + l]);
+    if (lane == 0) {
+      a.inst_d[i * kInstD + ID_PL_TOP] = planck_band(wlo, whi, sc[DISORT_S_TEMP_TOP]) * sc[DISORT_S_EMIS_TOP];
+      const double pb = planck_band(wlo, whi, sc[DISORT_S_TEMP_BOT]);
+      a.inst_d[i * kInstD + ID_PL_BOT] = pb;
+      double pbd = 0.0;
+      if (a.flags & DISORT_F_DIFFUSION_BC) {
+        const double bl = planck_band(wlo, whi, a.temper[vo + L - 1]);
+        const double dl = a.lay_d[LD_DTAU_S * TL + lo + L - 1];
+        if (dl > 0.0) pbd = (pb - bl) / dl;
+      }
+      a.inst_d[i * kInstD + ID_PL_BOT_DERIV] = pbd;
+    }
+  } else if (lane == 0) {
+    a.inst_d[i * kInstD + ID_PL_TOP] = 0.0;
+    a.inst_d[i * kInstD + ID_PL_BOT] = 0.0;
+    a.inst_d[i * kInstD + ID_PL_BOT_DERIV] = 0.0;
+  }
+}
+
+// ===========================================================================
+// k_layer: eigen-system and particular solutions, one lane per layer.
+// ===========================================================================
+template <int NN>
+struct LayerMath {
+  static constexpr int N = 2 * NN;
+  static constexpr int NT = NN * (NN + 1) / 2;
+  double L[NT];       // Cholesky factor of -B_s (packed lower)
+  double V[NN][NN];   // eigenvectors of S = L^T (-A_s) L (columns)
+  double lam[NN];     // eigenvalues of S (= k^2)
+
+  __device__ void lsolve(double* x) const {  // x <- L^{-1} x
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      double s = x[r];
+#pragma unroll
+      for (int m = 0; m < r; ++m) s -= L[tri(r, m)] * x[m];
+      x[r] = s / L[tri(r, r)];
+    }
+  }
+  __device__ void ltsolve(double* x) const {  // x <- L^{-T} x
+#pragma unroll
+    for (int r = NN - 1; r >= 0; --r) {
+      double s = x[r];
+#pragma unroll
+      for (int m = r + 1; m < NN; ++m) s -= L[tri(m, r)] * x[m];
+      x[r] = s / L[tri(r, r)];
+    }
+  }
+  __device__ void lmul(double* x) const {  // x <- L x
+#pragma unroll
+    for (int r = NN - 1; r >= 0; --r) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = 0; m <= r; ++m) s += L[tri(r, m)] * x[m];
+      x[r] = s;
+    }
+  }
+  __device__ void ltmul(double* x) const {  // x <- L^T x
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int m = r; m < NN; ++m) s += L[tri(m, r)] * x[m];
+      x[r] = s;
+    }
+  }
+  // x <- V diag(d) V^T x
+  __device__ void vdv(double* x, const double* d) const {
+    double y[NN];
+#pragma unroll
+    for (int j = 0; j < NN; ++j) {
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < NN; ++r) s += V[r][j] * x[r];
+      y[j] = s * d[j];
+    }
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int j = 0; j < NN; ++j) s += V[r][j] * y[j];
+      x[r] = s;
+    }
+  }
+};
+
+template <int NN>
+__device__ void jacobi_eigen(double (&S)[NN * (NN + 1) / 2], double (&V)[NN][NN], double (&lam)[NN]) {
+#pragma unroll
+  for (int r = 0; r < NN; ++r)
+#pragma unroll
+    for (int c = 0; c < NN; ++c) V[r][c] = (r == c) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 16; ++sweep) {
+    double off = 0.0, dia = 0.0;
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      dia += S[tri(r, r)] * S[tri(r, r)];
+#pragma unroll
+      for (int c = 0; c < r; ++c) off += S[tri(r, c)] * S[tri(r, c)];
+    }
+    const int done = (off == 0.0) || (off <= 1e-36 * dia);
+    if (__all(done)) break;
+#pragma unroll
+    for (int p = 0; p < NN - 1; ++p) {
+#pragma unroll
+      for (int q = p + 1; q < NN; ++q) {
+        const double apq = S[tri(q, p)];
+        const double app = S[tri(p, p)], aqq = S[tri(q, q)];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double at = fabs(theta);
+        double t = (at > 1e150) ? 0.5 / at : 1.0 / (at + sqrt(theta * theta + 1.0));
+        t = (theta < 0.0) ? -t : t;
+        t = (apq == 0.0) ? 0.0 : t;
+        const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
+        for (int k = 0; k < NN; ++k) {
+          if (k == p || k == q) continue;
+          const int ip = (k > p) ? tri(k, p) : tri(p, k);
+          const int iq = (k > q) ? tri(k, q) : tri(q, k);
+          const double akp = S[ip], akq = S[iq];
+          S[ip] = c * akp - s * akq;
+          S[iq] = s * akp + c * akq;
+        }
+        S[tri(p, p)] = app - t * apq;
+        S[tri(q, q)] = aqq + t * apq;
+        S[tri(q, p)] = 0.0;
+#pragma unroll
+        for (int k = 0; k < NN; ++k) {
+          const double vkp = V[k][p], vkq = V[k][q];
+          V[k][p] = c * vkp - s * vkq;
+          V[k][q] = s * vkp + c * vkq;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NN; ++r) lam[r] = S[tri(r, r)];
+}
+
+template <int NN>
+__global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
+  constexpr int N = 2 * NN;
+  constexpr int NT = NN * (NN + 1) / 2;
+  const int64_t t = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (t >= a.n_tasks) return;
+  const int i = a.t_inst[t];
+  const int lc = a.t_lc[t];
+  const int64_t g = a.lay_off[i] + lc;
+  const int64_t v = a.lay_off[i] + i + lc;  // level at the layer top
+  const int P = a.pstride[i];
+  const double* pm = a.pmom + a.mom_off[i] + (int64_t)lc * P;
+  const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
+  const int64_t TL = a.total_layers, TV = a.total_levels, T = a.n_tasks;
+  const double* tb = a.tab;
+  constexpr TabOff to(N);
+  constexpr EigOff eo(N);
+
+  const double ws = a.lay_d[LD_SSA_S * TL + g];
+  const double f = a.lay_d[LD_F * TL + g];
+  const double sig2 = a.lay_d[LD_SIG2 * TL + g];
+
+  // Scaled phase coefficients chi'_l (src/FluxSolver.cpp:342-388).
+  double chi[N];
+#pragma unroll
+  for (int l = 0; l < N; ++l) {
+    const double pl = pm[l];
+    double c;
+    if (sig2 < 0.0) c = (2.0 * l + 1.0) * ws * pl;
+    else if (sig2 == 0.0) c = (2.0 * l + 1.0) * ws * (pl - f) / (1.0 - f);
+    else c = (2.0 * l + 1.0) * ws * (pl - f * exp(-(double)(l * l) / (2.0 * sig2))) / (1.0 - f);
+    chi[l] = c;
+  }
+
+  // Beam source sums b+ + b- and b+ - b- over the positive nodes
+  // (right-hand side of src/FluxSolver.cpp:628-632 split by parity of l).
+  const double F0 = sc[DISORT_S_BEAM_FLUX];
+  const double mu0 = sc[DISORT_S_BEAM_MU];
+  const bool beam = F0 > 0.0;
+  double bs[NN], bd[NN];
+  {
+    double p0[N];
+    const double x = -mu0;
+    p0[0] = 1.0;
+    if (N > 1) p0[1] = x;
+#pragma unroll
+    for (int l = 2; l < N; ++l) p0[l] = ((2.0 * l - 1.0) * x * p0[l - 1] - (l - 1.0) * p0[l - 2]) / l;
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      double se = 0.0, so = 0.0;
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        const double term = chi[l] * tb[to.P() + l * NN + r] * p0[l];
+        if (l % 2 == 0) se += term;
+        else so += term;
+      }
+      bs[r] = F0 * se / (2.0 * kPi);
+      bd[r] = F0 * so / (2.0 * kPi);
+    }
+  }
+
+  // Symmetric forms A_s = S_d (alpha+beta) S_d^-1 and -B_s (:494-512):
+  // A_s(i,k) = sum_{l even} chi_l Q_l(i) Q_l(k) - delta_ik / mu_i, and the
+  // odd-l sum for -B_s, with Q_l(i) = P_l(mu_i) sqrt(w_i / mu_i).
+  double As[NT], nB[NT];
+#pragma unroll
+  for (int r = 0; r < NN; ++r)
+#pragma unroll
+    for (int c = 0; c <= r; ++c) {
+      double se = 0.0, so = 0.0;
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        const double q = chi[l] * tb[to.Q() + l * NN + r] * tb[to.Q() + l * NN + c];
+        if (l % 2 == 0) se += q;
+        else so += q;
+      }
+      const double dg = (r == c) ? 1.0 / tb[to.cmu() + r] : 0.0;
+      As[tri(r, c)] = se - dg;
+      nB[tri(r, c)] = dg - so;
+    }
+
+  LayerMath<NN> m;
+  // Cholesky of -B_s (positive definite for omega' <= 1).
+  int ok = 1;
+#pragma unroll
+  for (int c = 0; c < NN; ++c) {
+    double x = nB[tri(c, c)];
+#pragma unroll
+    for (int k = 0; k < c; ++k) x -= m.L[tri(c, k)] * m.L[tri(c, k)];
+    ok &= (x > 0.0);
+    const double lcc = sqrt(x > 0.0 ? x : 1.0);
+    m.L[tri(c, c)] = lcc;
+#pragma unroll
+    for (int r = c + 1; r < NN; ++r) {
+      double s = nB[tri(r, c)];
+#pragma unroll
+      for (int k = 0; k < c; ++k) s -= m.L[tri(r, k)] * m.L[tri(c, k)];
+      m.L[tri(r, c)] = s / lcc;
+    }
+  }
+  if (!ok) atomicOr(&a.inst_i[i * kInstI + II_STATUS], DISORT_ST_EIGEN);
+
+  // S = L^T (-A_s) L, column by column.
+  double S[NT];
+#pragma unroll
+  for (int c = 0; c < NN; ++c) {
+    double tc[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = c; k < NN; ++k) s -= ((r >= k) ? As[tri(r, k)] : As[tri(k, r)]) * m.L[tri(k, c)];
+      tc[r] = s;
+    }
+#pragma unroll
+    for (int r = 0; r <= c; ++r) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = r; k < NN; ++k) s += m.L[tri(k, r)] * tc[k];
+      S[tri(c, r)] = s;
+    }
+  }
+  jacobi_eigen<NN>(S, m.V, m.lam);
+
+  // Eigenvalues k = sqrt|lambda| and the G+/- blocks (:560-587):
+  // u = S_d^-1 L^-T w is the eigenvector of (alpha+beta)(alpha-beta);
+  // (alpha-beta) u / k = -S_d^-1 L w / k.
+  double kv[NN];
+#pragma unroll
+  for (int j = 0; j < NN; ++j) {
+    kv[j] = sqrt(fabs(m.lam[j]));
+    a.eig[(eo.k() + j) * T + t] = kv[j];
+    double u[NN], y[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) u[r] = y[r] = m.V[r][j];
+    m.ltsolve(u);
+    m.lmul(y);
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      const double sdi = tb[to.sdi() + r];
+      const double uu = sdi * u[r];
+      const double tt = -sdi * y[r] / kv[j];
+      a.eig[(eo.gp() + r + j * NN) * T + t] = 0.5 * (tt + uu);
+      a.eig[(eo.gm() + r + j * NN) * T + t] = 0.5 * (tt - uu);
+    }
+  }
+
+  // Particular solutions in the reference's angle order: index NN+r is the
+  // upward node mu_r, index NN-1-r the downward node -mu_r.
+  double zb[N], z0[N], z1[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) zb[r] = z0[r] = z1[r] = 0.0;
+
+  if (beam) {
+    // (I + diag(mu)/mu0 - C) x = b in sum/difference form: with s = x+ + x-,
+    // d = x+ - x-, ((alpha+beta)(alpha-beta) - 1/mu0^2) d = (p + mu0 (alpha+beta) q)/mu0
+    // and s = mu0((alpha-beta) d - q), p = -D^-1 (b+ + b-), q = -D^-1 (b+ - b-).
+    double mub = mu0;
+    const double mu0sq = mu0 * mu0;
+#pragma unroll
+    for (int j = 0; j < NN; ++j)
+      if (fabs(1.0 - mu0sq * kv[j] * kv[j]) < 1000.0 * kEps) mub = 0.999 * mu0;  // dither (:604-618)
+    double p[NN], q[NN], w[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      const double cm = tb[to.cmu() + r];
+      p[r] = -bs[r] / cm;
+      q[r] = -bd[r] / cm;
+      w[r] = tb[to.sd() + r] * q[r];
+    }
+    // (alpha+beta) q = -S_d^-1 L^-T V Lambda V^T L^-1 S_d q
+    m.lsolve(w);
+    m.vdv(w, m.lam);
+    m.ltsolve(w);
+    double rr[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      const double apbq = -tb[to.sdi() + r] * w[r];
+      rr[r] = (p[r] + mub * apbq) / mub;
+      rr[r] *= tb[to.sd() + r];
+    }
+    // d = S_d^-1 L^-T V (Lambda - 1/mu0^2)^-1 V^T L^T S_d r
+    double dinv[NN];
+    const double im2 = 1.0 / (mub * mub);
+#pragma unroll
+    for (int j = 0; j < NN; ++j) dinv[j] = 1.0 / (m.lam[j] - im2);
+    m.ltmul(rr);
+    m.vdv(rr, dinv);
+    m.ltsolve(rr);
+    double d[NN], e[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      d[r] = tb[to.sdi() + r] * rr[r];
+      e[r] = tb[to.sd() + r] * d[r];
+    }
+    // (alpha-beta) d = -S_d^-1 L L^T S_d d
+    m.ltmul(e);
+    m.lmul(e);
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      const double ambd = -tb[to.sdi() + r] * e[r];
+      const double s = mub * (ambd - q[r]);
+      zb[NN + r] = 0.5 * (s + d[r]);
+      zb[NN - 1 - r] = 0.5 * (s - d[r]);
+    }
+  }
+
+  const double taut = a.lev_d[VD_TAU_S * TV + v];
+  const double taub = a.lev_d[VD_TAU_S * TV + v + 1];
+  double xr0 = 0.0, xr1 = 0.0;
+  if (a.flags & DISORT_F_THERMAL) {
+    // (src/FluxSolver.cpp:755-776) B(tau) = xr0 + xr1 tau;
+    // (I - C) z1 = (1-omega') xr1, (I - C) z0 = (1-omega') xr0 + mu z1.
+    const double pt = a.lev_d[VD_PLANCK * TV + v], pb = a.lev_d[VD_PLANCK * TV + v + 1];
+    const double dts = a.lay_d[LD_DTAU_S * TL + g];
+    if (dts > 1e-4) xr1 = (pb - pt) / dts;
+    xr0 = pt - xr1 * taut;
+    // F^-1 (2*1) with F = I - X - Y = -D (alpha+beta):
+    //   S_d^-1 L V Lambda^-1 V^T L^T S_d D^-1 (2*1)
+    double base[NN], ilam[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      base[r] = tb[to.sd() + r] * (2.0 / tb[to.cmu() + r]);
+      ilam[r] = 1.0 / m.lam[r];
+    }
+    m.ltmul(base);
+    m.vdv(base, ilam);
+    m.lmul(base);
+#pragma unroll
+    for (int r = 0; r < NN; ++r) base[r] *= tb[to.sdi() + r];
+    const double c1 = (1.0 - ws) * xr1, c0 = (1.0 - ws) * xr0;
+    double h[NN], dd[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      h[r] = 0.5 * c1 * base[r];  // z1 at +mu_r and -mu_r
+      dd[r] = tb[to.sd() + r] * h[r];
+    }
+    // d0 = 2 S_d^-1 L^-T L^-1 S_d z1h  (solves H d0 = 2 D z1h, H = -D (alpha-beta))
+    m.lsolve(dd);
+    m.ltsolve(dd);
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      const double d0 = 2.0 * tb[to.sdi() + r] * dd[r];
+      const double s0 = c0 * base[r];
+      z1[NN + r] = h[r];
+      z1[NN - 1 - r] = h[r];
+      z0[NN + r] = 0.5 * (s0 + d0);
+      z0[NN - 1 - r] = 0.5 * (s0 - d0);
+    }
+  }
+
+  const double btt = a.lay_d[LD_BT_TOP * TL + g], btb = a.lay_d[LD_BT_BOT * TL + g];
+  const double fact = beam ? exp(-taub / mu0) : 0.0;
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    a.eig[(eo.ztop() + r) * T + t] = zb[r] * btt + z0[r] + z1[r] * taut;
+    a.eig[(eo.zbot() + r) * T + t] = zb[r] * btb + z0[r] + z1[r] * taub;
+    a.eig[(eo.zfl() + r) * T + t] = zb[r] * fact + z0[r] + z1[r] * taub;
+  }
+  a.eig[eo.pls() * T + t] = xr0 + xr1 * taub;
+}
+
+// ===========================================================================
+// k_boundary: block QR elimination of the boundary-value problem.
+// 16 lanes per instance, 4 instances per 64-lane workgroup.  Lane j owns
+// column j of the left (current layer) block, column j of the right (next
+// layer) block, and a redundant copy of the right-hand side.
+// ===========================================================================
+template <int NN>
+struct Panel {
+  static constexpr int N = 2 * NN;
+  static constexpr int R = 3 * NN;
+  double A0[R], A1[R], A2[R];
+};
+
+// Column j of the eigenvector matrix G of task t (reference ordering).
+template <int NN>
+__device__ __forceinline__ void load_gcol(const FluxArgs& a, int64_t t, int j, double (&gcol)[2 * NN]) {
+  constexpr EigOff eo(2 * NN);
+  const int64_t T = a.n_tasks;
+  const bool up = j >= NN;
+  const int jp = up ? j - NN : NN - 1 - j;
+#pragma unroll
+  for (int r = 0; r < NN; ++r) {
+    const double gp = a.eig[(eo.gp() + r + jp * NN) * T + t];
+    const double gm = a.eig[(eo.gm() + r + jp * NN) * T + t];
+    gcol[NN + r] = up ? gp : -gm;
+    gcol[NN - 1 - r] = up ? gm : -gp;
+  }
+}
+
+template <int NN>
+__global__ __launch_bounds__(64) void k_boundary(FluxArgs a) {
+  constexpr int N = 2 * NN;
+  constexpr int R = 3 * NN;
+  constexpr EigOff eo(N);
+  constexpr TabOff to(N);
+  constexpr int PS = panel_stride(N);
+  static_assert(N <= 16, "one 16-lane row per instance");
+  const int grp = threadIdx.x >> 4;
+  const int lane = threadIdx.x & 15;
+  const int bk = blockIdx.x * 4 + grp;
+  if (bk >= a.nb) return;
+  const int i = a.b_inst[bk];
+  const int64_t t0 = a.b_task0[bk];
+  const int64_t lo = a.lay_off[i];
+  const int64_t T = a.n_tasks, TL = a.total_layers;
+  const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
+  const double* tb = a.tab;
+  const int M = a.inst_i[i * kInstI + II_NCUT];
+  const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
+  const bool diffusion = (a.flags & DISORT_F_DIFFUSION_BC) != 0;
+  const bool thermal = (a.flags & DISORT_F_THERMAL) != 0;
+  const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
+  const double alb = sc[DISORT_S_ALBEDO];
+  const int j = (lane < N) ? lane : 0;  // idle lanes mirror column 0, results discarded
+  const bool live = lane < N;
+
+  Panel<NN> pn;
+#pragma unroll
+  for (int r = 0; r < R; ++r) pn.A0[r] = pn.A1[r] = pn.A2[r] = 0.0;
+
+  // Exponential factor of mode column j over layer task tt:
+  // Etop (mode value at the layer top) and Ebot (at the bottom).
+  auto efac = [&](int64_t tt, double& etop, double& ebot) {
+    const int jp = (j >= NN) ? j - NN : NN - 1 - j;
+    const double k = a.eig[(eo.k() + jp) * T + tt];
+    const double dts = a.lay_d[LD_DTAU_S * TL + lo + (tt - t0)];
+    const double e = exp(-k * dts);
+    etop = (j < NN) ? e : 1.0;
+    ebot = (j < NN) ? 1.0 : e;
+  };
+
+  for (int ip = 0; ip < M; ++ip) {
+    const int64_t tt = t0 + ip;
+    const bool last = (ip == M - 1);
+    double g[N], etop, ebot;
+    load_gcol<NN>(a, tt, j, g);
+    efac(tt, etop, ebot);
+    if (ip == 0) {
+      // Top boundary: downward intensities at tau=0 (:890-907, :976-990).
+      const double top = sc[DISORT_S_ISO_TOP] + a.inst_d[i * kInstD + ID_PL_TOP];
+#pragma unroll
+      for (int r = 0; r < NN; ++r) {
+        pn.A0[r] = live ? g[NN - 1 - r] * etop : 0.0;
+        pn.A1[r] = 0.0;
+        pn.A2[r] = top - a.eig[(eo.ztop() + NN - 1 - r) * T + tt];
+      }
+    }
+    if (!last) {
+      // Continuity at the interface below layer ip (:851-887, :1094-1118).
+      double gn[N], etn, ebn;
+      load_gcol<NN>(a, tt + 1, j, gn);
+      efac(tt + 1, etn, ebn);
+#pragma unroll
+      for (int r = 0; r < N; ++r) {
+        pn.A0[NN + r] = live ? g[r] * ebot : 0.0;
+        pn.A1[NN + r] = live ? -gn[r] * etn : 0.0;
+        pn.A2[NN + r] = a.eig[(eo.ztop() + r) * T + tt + 1] - a.eig[(eo.zbot() + r) * T + tt];
+      }
+    } else {
+      // Bottom boundary: Lambertian reflection of the downward field (:909-959, :992-1091).
+      const bool refl = !(lyrcut || diffusion);
+      double rs = 0.0;
+      if (refl) {
+#pragma unroll
+        for (int k = 0; k < NN; ++k) rs += tb[to.cwt() + k] * tb[to.cmu() + k] * alb * g[NN - 1 - k];
+      }
+      double zrefl = 0.0;
+      if (refl) {
+#pragma unroll
+        for (int k = 0; k < NN; ++k)
+          zrefl += tb[to.cwt() + k] * tb[to.cmu() + k] * alb * a.eig[(eo.zbot() + NN - 1 - k) * T + tt];
+      }
+      const double btn = a.inst_d[i * kInstD + ID_BTRANS_NCUT];
+      const double plb = a.inst_d[i * kInstD + ID_PL_BOT];
+      const double plbd = a.inst_d[i * kInstD + ID_PL_BOT_DERIV];
+#pragma unroll
+      for (int r = 0; r < NN; ++r) {
+        pn.A0[NN + r] = live ? (g[NN + r] - 2.0 * rs) * ebot : 0.0;
+        pn.A1[NN + r] = 0.0;
+        double s = -a.eig[(eo.zbot() + NN + r) * T + tt];
+        if (diffusion) {
+          s += plb + tb[to.cmu() + r] * plbd;
+        } else {
+          if (!lyrcut) s += 2.0 * zrefl;
+          if (F0 > 0.0 && !lyrcut) s += alb * mu0 * F0 * btn / kPi;
+          s += sc[DISORT_S_ISO_BOT];
+          if (thermal) s += (1.0 - alb) * plb;
+        }
+        pn.A2[NN + r] = s;
+      }
+#pragma unroll
+      for (int r = N; r < R; ++r) pn.A0[r] = pn.A1[r] = pn.A2[r] = 0.0;
+    }
+
+    // Householder QR of the left block, applied to the right block and rhs.
+    static_for<0, N>([&](auto cc) {
+      constexpr int c = decltype(cc)::value;
+      double tail = 0.0;
+#pragma unroll
+      for (int r = c + 1; r < R; ++r) tail += pn.A0[r] * pn.A0[r];
+      const double x0 = pn.A0[c];
+      const double nrm = sqrt(tail + x0 * x0);
+      const double alpha = (x0 >= 0.0) ? -nrm : nrm;
+      const double vc = x0 - alpha;
+      const double vv = tail + vc * vc;
+      const double beta = (vv > 0.0) ? 2.0 / vv : 0.0;
+      double bv[R];
+      bv[c] = row_bcast<c>(vc);
+#pragma unroll
+      for (int r = c + 1; r < R; ++r) bv[r] = row_bcast<c>(pn.A0[r]);
+      const double bb = row_bcast<c>(beta);
+      const double ba = row_bcast<c>(alpha);
+      double w0 = 0.0, w1 = 0.0, w2 = 0.0;
+#pragma unroll
+      for (int r = c; r < R; ++r) {
+        w0 += bv[r] * pn.A0[r];
+        w1 += bv[r] * pn.A1[r];
+        w2 += bv[r] * pn.A2[r];
+      }
+      w0 *= bb;
+      w1 *= bb;
+      w2 *= bb;
+#pragma unroll
+      for (int r = c; r < R; ++r) {
+        pn.A0[r] -= w0 * bv[r];
+        pn.A1[r] -= w1 * bv[r];
+        pn.A2[r] -= w2 * bv[r];
+      }
+      if (lane == c) {
+        pn.A0[c] = ba;
+#pragma unroll
+        for (int r = c + 1; r < R; ++r) pn.A0[r] = 0.0;
+      }
+    });
+
+    double* pp = a.panel + tt * PS;
+    if (live) {
+#pragma unroll
+      for (int r = 0; r < N; ++r) {
+        pp[j * N + r] = pn.A0[r];
+        pp[N * N + j * N + r] = pn.A1[r];
+      }
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int r = 0; r < N; ++r) pp[2 * N * N + r] = pn.A2[r];
+    }
+    // Equations left over for the next layer's unknowns.
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      pn.A0[r] = pn.A1[N + r];
+      pn.A1[r] = 0.0;
+      pn.A2[r] = pn.A2[N + r];
+    }
+  }
+
+  // Back substitution, lane = row k of each panel.
+  const int k = j;
+  double xn = 0.0;  // this lane's entry of the next layer's constants
+  for (int ip = M - 1; ip >= 0; --ip) {
+    const int64_t tt = t0 + ip;
+    const double* pp = a.panel + tt * PS;
+    double v = pp[2 * N * N + k];
+    if (ip < M - 1) {
+      static_for<0, N>([&](auto cj) {
+        constexpr int c = decltype(cj)::value;
+        v -= pp[N * N + c * N + k] * row_bcast<c>(xn);
+      });
+    }
+    const double dg = pp[k * N + k];
+    double xk = 0.0;
+    static_for<0, N>([&](auto cj) {
+      constexpr int c = N - 1 - decltype(cj)::value;
+      const double qv = v / dg;
+      const double xc = row_bcast<c>(qv);
+      if (k == c) xk = xc;
+      v -= pp[c * N + k] * xc;
+    });
+    xn = xk;
+    if (live) a.cvec[tt * N + k] = xk;
+  }
+}
+
+// ===========================================================================
+// k_flux: fluxes and mean intensities at the layer boundaries.
+// ===========================================================================
+template <int NN>
+__global__ __launch_bounds__(64) void k_flux(FluxArgs a) {
+  constexpr int N = 2 * NN;
+  constexpr EigOff eo(N);
+  constexpr TabOff to(N);
+  const int64_t vt = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (vt >= a.n_lvtasks) return;
+  const int bk = a.v_bk[vt];
+  const int lu = a.v_lu[vt];
+  const int i = a.b_inst[bk];
+  const int64_t t0 = a.b_task0[bk];
+  const int64_t lo = a.lay_off[i], vo = lo + i;
+  const int64_t T = a.n_tasks, TL = a.total_layers, TV = a.total_levels, OV = a.total_levels;
+  const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
+  const double* tb = a.tab;
+  const int ncut = a.inst_i[i * kInstI + II_NCUT];
+  const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
+  int lyu = (lu == 0) ? 0 : lu - 1;
+  double o[DISORT_NOUT];
+#pragma unroll
+  for (int f = 0; f < DISORT_NOUT; ++f) o[f] = 0.0;
+  if (!(lyrcut && lyu > ncut - 1)) {
+    if (lyu >= ncut) lyu = ncut - 1;
+    const int64_t tt = t0 + lyu;
+    const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
+    const double ut = a.lev_d[VD_TAU_S * TV + vo + lu];
+    const double dts = a.lay_d[LD_DTAU_S * TL + lo + lyu];
+    // Mode amplitudes at this level: C_j e_j.
+    double yd[NN], yu[NN];
+#pragma unroll
+    for (int jp = 0; jp < NN; ++jp) {
+      const double e = exp(-a.eig[(eo.k() + jp) * T + tt] * dts);
+      const double cd = a.cvec[tt * N + NN - 1 - jp];  // mode with eigenvalue -k_jp
+      const double cu = a.cvec[tt * N + NN + jp];      // mode with eigenvalue +k_jp
+      yd[jp] = cd * ((lu == 0) ? e : 1.0);
+      yu[jp] = cu * ((lu == 0) ? 1.0 : e);
+    }
+    const int zoff = (lu == 0) ? eo.ztop() : eo.zfl();
+    double mi = 0.0, mid = 0.0, miu = 0.0, fldn = 0.0, flup = 0.0;
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      double iu = a.eig[(zoff + NN + r) * T + tt];
+      double id = a.eig[(zoff + NN - 1 - r) * T + tt];
+#pragma unroll
+      for (int jp = 0; jp < NN; ++jp) {
+        const double gp = a.eig[(eo.gp() + r + jp * NN) * T + tt];
+        const double gm = a.eig[(eo.gm() + r + jp * NN) * T + tt];
+        iu += gp * yu[jp] - gm * yd[jp];
+        id += gm * yu[jp] - gp * yd[jp];
+      }
+      const double w = tb[to.cwt() + r], cm = tb[to.cmu() + r];
+      mi += w * id + w * iu;
+      mid += w * id;
+      miu += w * iu;
+      fldn += w * id * cm;
+      flup += w * iu * cm;
+    }
+    double fact = 0.0, dirint = 0.0, fldir = 0.0, fdir = 0.0;
+    if (F0 > 0.0) {
+      const double tauc = a.lev_d[VD_TAUC * TV + vo + lu];
+      fact = exp(-ut / mu0);
+      fdir = mu0 * F0 * exp(-tauc / mu0);
+      dirint = F0 * fact;
+      fldir = mu0 * F0 * fact;
+    }
+    flup *= 2.0 * kPi;
+    fldn *= 2.0 * kPi;
+    const double fdn = (fldn + fldir) - fdir;
+    mi = (2.0 * kPi * mi + dirint) / (4.0 * kPi);
+    const double midir = dirint / (4.0 * kPi);
+    mid = (2.0 * kPi * mid) / (4.0 * kPi);
+    miu = (2.0 * kPi * miu) / (4.0 * kPi);
+    double pls = 0.0;
+    if (a.flags & DISORT_F_THERMAL)
+      pls = (lu == 0) ? a.lev_d[VD_PLANCK * TV + vo] : a.eig[eo.pls() * T + tt];
+    const double div = (1.0 - a.ssa[lo + lyu]) * 4.0 * kPi * (mi - pls);
+    o[DISORT_O_FLUX_DIRECT_BEAM] = fdir;
+    o[DISORT_O_FLUX_DOWN] = fdn;
+    o[DISORT_O_FLUX_UP] = flup;
+    o[DISORT_O_FLUX_TAU_DIVERGENCE] = div;
+    o[DISORT_O_MEAN_INTENSITY] = mi;
+    o[DISORT_O_MEAN_INTENSITY_DOWN] = mid;
+    o[DISORT_O_MEAN_INTENSITY_UP] = miu;
+    o[DISORT_O_MEAN_INTENSITY_DIRECT_BEAM] = midir;
+  }
+#pragma unroll
+  for (int f = 0; f < DISORT_NOUT; ++f) a.out[f * OV + vo + lu] = o[f];
+}
+
+}  // namespace
+
+// Host-callable launchers, one set per stream count.
+template <int NN>
+static void launch_all(const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
+  constexpr int N = 2 * NN;
+  if (ev) (void)hipEventRecord(ev[0], s);
+  hipLaunchKernelGGL(k_column<N>, dim3(a.nb), dim3(64), 0, s, a);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  hipLaunchKernelGGL(k_layer<NN>, dim3((unsigned)((a.n_tasks + 63) / 64)), dim3(64), 0, s, a);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  hipLaunchKernelGGL(k_boundary<NN>, dim3((unsigned)((a.nb + 3) / 4)), dim3(64), 0, s, a);
+  if (ev) (void)hipEventRecord(ev[3], s);
+  hipLaunchKernelGGL(k_flux<NN>, dim3((unsigned)((a.n_lvtasks + 63) / 64)), dim3(64), 0, s, a);
+  if (ev) (void)hipEventRecord(ev[4], s);
+}
+
+bool launch_flux_bucket(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
+  switch (nstr) {
+    case 4: launch_all<2>(a, s, ev); return true;
+    case 6: launch_all<3>(a, s, ev); return true;
+    case 8: launch_all<4>(a, s, ev); return true;
+    case 10: launch_all<5>(a, s, ev); return true;
+    case 12: launch_all<6>(a, s, ev); return true;
+    case 14: launch_all<7>(a, s, ev); return true;
+    case 16: launch_all<8>(a, s, ev); return true;
+    default: return false;
+  }
+}
+
+}  // namespace dsa

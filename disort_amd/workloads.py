@@ -1,0 +1,111 @@
+"""Seeded synthetic workloads of the flux path (counter-based Philox RNG).
+
+c1  single column: nlyr=4, nstr=16, Henyey-Greenstein g=0.8 (17 moments),
+    delta_tau=[0.1,0.5,1,2], single_scat_albedo=0.9, Lambertian albedo 0.1,
+    beam flux pi at mu0=0.5, no thermal emission (README quick start of the
+    reference, README.md:57-84).
+c2  spectral bands: 10,000 bands [100+b, 101+b] cm^-1 x one column, nlyr=60,
+    nstr=16, thermal emission with level temperatures linear from 200 K (top)
+    to 300 K (surface); per band and layer delta_tau = 10^U(-4,1),
+    single_scat_albedo ~ U(0,0.999), HG g ~ U(0,0.9); per band albedo
+    ~ U(0,0.3); one beam cosine mu0 ~ U(0.1,1) for the column (the spectral
+    driver cannot override mu0 per band, src/python_bindings.cpp:51-60);
+    beam flux pi; temperature_bottom 300 K, temperature_top 0 K,
+    emissivity_top 1 (choices not fixed by the workload statement).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .batch import F_THERMAL, FluxBatch, scalars, uniform_batch
+
+C2_SEED = 20261203
+
+
+def rng(seed: int) -> np.random.Generator:
+    return np.random.Generator(np.random.Philox(seed))
+
+
+def hg_moments(g: np.ndarray, nmom_plus1: int) -> np.ndarray:
+    """pmom_k = g^k, k = 0..nmom_plus1-1 (src/PhaseFunction.cpp:6-18)."""
+    g = np.asarray(g, dtype=np.float64)
+    k = np.arange(nmom_plus1, dtype=np.float64)
+    return np.power(g[..., None], k)
+
+
+def c1() -> FluxBatch:
+    nstr, L = 16, 4
+    dtau = np.array([[0.1, 0.5, 1.0, 2.0]])
+    ssa = np.full((1, L), 0.9)
+    pmom = hg_moments(np.full((1, L), 0.8), nstr + 1)
+    return uniform_batch(dtau, ssa, pmom, nstr, scalars(1, beam_flux=np.pi, beam_mu=0.5, albedo=0.1))
+
+
+def c2(n_bands: int = 10000, nlyr: int = 60, nstr: int = 16, seed: int = C2_SEED, first_band: int = 0) -> FluxBatch:
+    """Bands first_band .. first_band+n_bands-1 of the C2 workload.
+
+    Every band's draws depend only on (seed, band index), so any subset of
+    bands reproduces the same instances as the full 10,000-band run.
+    """
+    dtau = np.empty((n_bands, nlyr))
+    ssa = np.empty((n_bands, nlyr))
+    g = np.empty((n_bands, nlyr))
+    alb = np.empty(n_bands)
+    for k in range(n_bands):
+        r = rng(seed + 1 + first_band + k)
+        dtau[k] = 10.0 ** r.uniform(-4.0, 1.0, nlyr)
+        ssa[k] = r.uniform(0.0, 0.999, nlyr)
+        g[k] = r.uniform(0.0, 0.9, nlyr)
+        alb[k] = r.uniform(0.0, 0.3)
+    mu0 = rng(seed).uniform(0.1, 1.0)
+    b = np.arange(first_band, first_band + n_bands, dtype=np.float64)
+    wn_lo, wn_hi = 100.0 + b, 101.0 + b
+    temper = np.tile(np.linspace(200.0, 300.0, nlyr + 1), (n_bands, 1))
+    sc = scalars(
+        n_bands,
+        beam_flux=np.pi,
+        beam_mu=mu0,
+        albedo=alb,
+        temp_top=0.0,
+        temp_bot=300.0,
+        emis_top=1.0,
+        wn_lo=wn_lo,
+        wn_hi=wn_hi,
+    )
+    return uniform_batch(dtau, ssa, hg_moments(g, nstr + 1), nstr, sc, flags=F_THERMAL, temper=temper)
+
+
+def random_flux_batch(
+    n: int,
+    nlyr: int,
+    nstr: int,
+    seed: int,
+    thermal: bool = True,
+    beam: bool = True,
+    flags_extra: int = 0,
+    pstride: int | None = None,
+) -> FluxBatch:
+    """Small C2-distributed batch for parity tests (any nstr / nlyr)."""
+    r = rng(seed)
+    dtau = 10.0 ** r.uniform(-4.0, 1.0, (n, nlyr))
+    ssa = r.uniform(0.0, 0.999, (n, nlyr))
+    g = r.uniform(0.0, 0.9, (n, nlyr))
+    P = pstride if pstride is not None else nstr + 1
+    pmom = hg_moments(g, P)
+    alb = r.uniform(0.0, 0.3, n)
+    mu0 = r.uniform(0.1, 1.0, n)
+    wn = r.uniform(100.0, 3000.0, n)
+    temper = np.tile(np.linspace(200.0, 300.0, nlyr + 1), (n, 1)) + r.uniform(-5, 5, (n, nlyr + 1))
+    sc = scalars(
+        n,
+        beam_flux=np.pi if beam else 0.0,
+        beam_mu=mu0,
+        albedo=alb,
+        temp_top=0.0,
+        temp_bot=300.0,
+        emis_top=1.0,
+        wn_lo=wn,
+        wn_hi=wn + 1.0,
+    )
+    flags = (F_THERMAL if thermal else 0) | flags_extra
+    return uniform_batch(dtau, ssa, pmom, nstr, sc, flags=flags, temper=temper if thermal else None)

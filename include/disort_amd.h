@@ -123,16 +123,35 @@ void* disort_ctx_stream(disort_ctx* ctx); /* the hipStream_t the kernels run on 
 int disort_sync(disort_ctx* ctx);
 
 /*
- * Batched flux solve.  Replaces the OpenMP band loop of
+ * Batched flux solve on host arrays.  Replaces the OpenMP band loop of
  * solveFluxSpectralBandsImpl<N> (src/python_bindings.cpp:170-190) and the
- * per-instance DisortFluxSolver<N>::solve (src/FluxSolver.cpp:23-171); with
- * DISORT_MEM_HOST the pointers are host memory (copied in and out
- * synchronously), with DISORT_MEM_DEVICE they are device memory and the call
- * is asynchronous on the context stream.  Supported on the GPU: nstr in
- * {4,6,...,16}, plane-parallel beam, thermal emission, Lambertian surface,
- * Delta-M / Delta-M+, layer cut-off, diffusion lower boundary.
+ * per-instance DisortFluxSolver<N>::solve (src/FluxSolver.cpp:23-171).
+ * memspace must be DISORT_MEM_HOST: inputs are validated with the rules of
+ * DisortFluxConfig::validate (src/DisortFluxConfig.cpp:43-143), copied in,
+ * solved, and the outputs and per-instance status copied back before return.
+ * Supported on the GPU: nstr in {4,6,...,16}, plane-parallel beam, thermal
+ * emission, Lambertian surface, Delta-M / Delta-M+, layer cut-off, diffusion
+ * lower boundary.
  */
 int disort_flux_solve(disort_ctx* ctx, const disort_flux_batch* batch, int memspace);
+
+/*
+ * Device-resident path.  A plan holds the bucket schedule (instances grouped
+ * by stream count, layer and level task lists) computed once from host-side
+ * shapes; disort_flux_execute then only launches kernels, asynchronously on
+ * the context stream, reading and writing device arrays.  The batch's
+ * lay_off/mom_off are ignored (the plan's prefix offsets are used).  Values
+ * of device arrays are not validated.  The same split as constructing a
+ * DisortFluxSolver<N> once and calling solve() per band
+ * (src/python_bindings.cpp:177-188).
+ */
+typedef struct disort_flux_plan disort_flux_plan;
+int disort_flux_plan_create(disort_ctx* ctx, int64_t n, const int32_t* nstr, const int32_t* nlyr,
+                            const int32_t* pstride, int32_t flags, disort_flux_plan** plan);
+int disort_flux_plan_destroy(disort_ctx* ctx, disort_flux_plan* plan);
+int disort_flux_execute(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* device_batch);
+/* Per-instance DISORT_ST_* codes of the last execute with this plan (host array [n]). */
+int disort_flux_status(disort_ctx* ctx, disort_flux_plan* plan, int32_t* status_host);
 
 /* Device memory helpers for callers without their own allocator (ctypes). */
 int disort_device_alloc(disort_ctx* ctx, size_t bytes, void** ptr);

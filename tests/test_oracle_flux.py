@@ -1,0 +1,123 @@
+"""Pins the oracle's flux solver: the reference's own expected values
+(DisORT++ self-regression, C-DISORT disotest tables), closed forms and
+invariants of the radiative-transfer solution."""
+import numpy as np
+import pytest
+from conftest import case_batch, max_rel_err
+
+import oracle_lib as ol
+from disort_amd import workloads as W
+from disort_amd.batch import F_THERMAL, OUT_FIELDS, scalars, uniform_batch
+
+FIELD = {name: k for k, name in enumerate(OUT_FIELDS)}
+
+
+def test_reference_cases(reference_cases):
+    for case in reference_cases:
+        out = ol.flux_one(case_batch(case), 0)
+        for field, levels in case["expect"].items():
+            for lev, expect in levels.items():
+                got = out[FIELD[field], int(lev)]
+                if expect == 0.0:
+                    assert abs(got) < 1e-6, (case["name"], field, lev, got)
+                else:
+                    assert got == pytest.approx(expect, rel=case["rtol"]), (case["name"], field, lev, got, expect)
+
+
+def test_beer_law_and_no_diffuse():
+    L = 5
+    dtau = np.array([[0.3, 0.1, 1.2, 0.7, 2.0]])
+    b = uniform_batch(dtau, np.zeros((1, L)), W.hg_moments(np.zeros((1, L)), 17), 16,
+                      scalars(1, beam_flux=2.0, beam_mu=0.4))
+    out = ol.flux_one(b, 0)
+    tau = np.concatenate([[0.0], np.cumsum(dtau[0])])
+    np.testing.assert_allclose(out[0], 0.4 * 2.0 * np.exp(-tau / 0.4), rtol=1e-14)
+    assert np.abs(out[1]).max() < 1e-14 and np.abs(out[2]).max() < 1e-14
+
+
+def test_isothermal_blackbody_cavity():
+    # Uniform temperature everywhere, black boundaries: I = B isotropically.
+    L, t0, lo, hi = 4, 250.0, 600.0, 606.0
+    rng = np.random.Generator(np.random.Philox(8))
+    b = uniform_batch(rng.uniform(0.1, 2.0, (1, L)), rng.uniform(0.0, 0.95, (1, L)),
+                      W.hg_moments(rng.uniform(0, 0.8, (1, L)), 17), 16,
+                      scalars(1, temp_top=t0, temp_bot=t0, emis_top=1.0, albedo=0.0, wn_lo=lo, wn_hi=hi),
+                      flags=F_THERMAL, temper=np.full((1, L + 1), t0))
+    out = ol.flux_one(b, 0)
+    B = ol.planck(lo, hi, t0)
+    np.testing.assert_allclose(out[2], np.pi * B, rtol=1e-12)  # flux_up
+    np.testing.assert_allclose(out[1], np.pi * B, rtol=1e-12)  # flux_down
+    np.testing.assert_allclose(out[4], B, rtol=1e-12)          # mean intensity
+    assert np.abs(out[3]).max() < 1e-10 * B                    # no divergence
+
+
+def test_conservative_net_flux_constant():
+    L = 5
+    b = uniform_batch(np.array([[0.2, 0.5, 1.0, 0.1, 0.7]]), np.ones((1, L)),
+                      W.hg_moments(np.full((1, L), 0.6), 17), 16,
+                      scalars(1, beam_flux=np.pi, beam_mu=0.6, albedo=0.3))
+    out = ol.flux_one(b, 0)
+    net = out[0] + out[1] - out[2]
+    np.testing.assert_allclose(net, net[0], rtol=1e-7)
+
+
+def test_layer_splitting_invariance():
+    # Splitting every layer in two identical halves leaves the fluxes at the
+    # original levels unchanged (the 11a/11b check, tests/test_solver.cpp:2473-2595).
+    rng = np.random.Generator(np.random.Philox(4))
+    L = 4
+    dtau = rng.uniform(0.05, 1.5, (1, L))
+    ssa = rng.uniform(0.1, 0.95, (1, L))
+    g = rng.uniform(0.1, 0.8, (1, L))
+    sc = scalars(1, beam_flux=1.0, beam_mu=0.45, albedo=0.2, iso_top=0.1)
+    b1 = uniform_batch(dtau, ssa, W.hg_moments(g, 17), 16, sc)
+    b2 = uniform_batch(np.repeat(dtau / 2, 2, axis=1), np.repeat(ssa, 2, axis=1),
+                       W.hg_moments(np.repeat(g, 2, axis=1), 17), 16, sc)
+    o1 = ol.flux_one(b1, 0)
+    o2 = ol.flux_one(b2, 0)[:, ::2]
+    assert max_rel_err(o2, o1) < 1e-11
+
+
+def test_index_from_bottom_reverses():
+    b = W.random_flux_batch(1, 6, 8, seed=9)
+    top = ol.flux_one(b, 0)
+    rev = W.random_flux_batch(1, 6, 8, seed=9)
+    rev.dtau[:] = rev.dtau[::-1]
+    rev.ssa[:] = rev.ssa[::-1]
+    rev.pmom[:] = rev.pmom.reshape(6, -1)[::-1].reshape(-1)
+    rev.temper[:] = rev.temper[::-1]
+    bot = ol.flux_one(rev, 0, index_from_bottom=True)
+    np.testing.assert_array_equal(bot[:, ::-1], top)
+
+
+def test_batch_equals_single_instances():
+    b = W.random_flux_batch(5, 9, 12, seed=2)
+    ol.flux_batch(b, nthreads=2)
+    for i in range(b.n):
+        np.testing.assert_array_equal(b.out[:, b.level_slice(i)], ol.flux_one(b, i))
+
+
+def test_c2_workload_oracle_sanity():
+    b = W.c2(3)
+    ol.flux_batch(b, nthreads=1)
+    assert np.isfinite(b.out).all()
+    # Direct beam at the top is mu0 * pi for every band.
+    mu0 = b.scal[1]
+    np.testing.assert_allclose(b.out[0, [b.level_slice(i).start for i in range(3)]], mu0 * np.pi, rtol=1e-15)
+
+
+@pytest.mark.parametrize(
+    "mutate, msg",
+    [
+        (lambda b: b.dtau.__setitem__(0, -1.0), "delta_tau[0] must be >= 0"),
+        (lambda b: b.ssa.__setitem__(1, 1.5), "single_scat_albedo[1] must be in [0, 1]"),
+        (lambda b: b.scal.__setitem__(7, 2.0), "surface_albedo must be in [0, 1]"),
+        (lambda b: b.scal.__setitem__(1, 0.0), "direct_beam_mu must be in (0, 1]"),
+    ],
+)
+def test_validation_messages(mutate, msg):
+    b = W.random_flux_batch(1, 3, 4, seed=1)
+    mutate(b)
+    with pytest.raises(ol.OracleError) as e:
+        ol.flux_one(b, 0)
+    assert msg in str(e.value)

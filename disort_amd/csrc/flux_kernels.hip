@@ -380,49 +380,32 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
   const double f = a.lay_d[LD_F * TL + g];
   const double sig2 = a.lay_d[LD_SIG2 * TL + g];
 
-  // Scaled phase coefficients chi'_l (src/FluxSolver.cpp:342-388).
-  double chi[N];
+  // Scaled phase coefficients chi'_l (src/FluxSolver.cpp:342-388).  They are
+  // evaluated twice (phase matrix, beam source) rather than kept live across
+  // the eigen-solve, to bound register pressure.
+  auto scaled_coeffs = [&](double (&chi)[N]) {
 #pragma unroll
-  for (int l = 0; l < N; ++l) {
-    const double pl = pm[l];
-    double c;
-    if (sig2 < 0.0) c = (2.0 * l + 1.0) * ws * pl;
-    else if (sig2 == 0.0) c = (2.0 * l + 1.0) * ws * (pl - f) / (1.0 - f);
-    else c = (2.0 * l + 1.0) * ws * (pl - f * exp(-(double)(l * l) / (2.0 * sig2))) / (1.0 - f);
-    chi[l] = c;
-  }
+    for (int l = 0; l < N; ++l) {
+      const double pl = pm[l];
+      double c;
+      if (sig2 < 0.0) c = (2.0 * l + 1.0) * ws * pl;
+      else if (sig2 == 0.0) c = (2.0 * l + 1.0) * ws * (pl - f) / (1.0 - f);
+      else c = (2.0 * l + 1.0) * ws * (pl - f * exp(-(double)(l * l) / (2.0 * sig2))) / (1.0 - f);
+      chi[l] = c;
+    }
+  };
 
-  // Beam source sums b+ + b- and b+ - b- over the positive nodes
-  // (right-hand side of src/FluxSolver.cpp:628-632 split by parity of l).
   const double F0 = sc[DISORT_S_BEAM_FLUX];
   const double mu0 = sc[DISORT_S_BEAM_MU];
   const bool beam = F0 > 0.0;
-  double bs[NN], bd[NN];
-  {
-    double p0[N];
-    const double x = -mu0;
-    p0[0] = 1.0;
-    if (N > 1) p0[1] = x;
-#pragma unroll
-    for (int l = 2; l < N; ++l) p0[l] = ((2.0 * l - 1.0) * x * p0[l - 1] - (l - 1.0) * p0[l - 2]) / l;
-#pragma unroll
-    for (int r = 0; r < NN; ++r) {
-      double se = 0.0, so = 0.0;
-#pragma unroll
-      for (int l = 0; l < N; ++l) {
-        const double term = chi[l] * tb[to.P() + l * NN + r] * p0[l];
-        if (l % 2 == 0) se += term;
-        else so += term;
-      }
-      bs[r] = F0 * se / (2.0 * kPi);
-      bd[r] = F0 * so / (2.0 * kPi);
-    }
-  }
 
   // Symmetric forms A_s = S_d (alpha+beta) S_d^-1 and -B_s (:494-512):
   // A_s(i,k) = sum_{l even} chi_l Q_l(i) Q_l(k) - delta_ik / mu_i, and the
   // odd-l sum for -B_s, with Q_l(i) = P_l(mu_i) sqrt(w_i / mu_i).
   double As[NT], nB[NT];
+  {
+  double chi[N];
+  scaled_coeffs(chi);
 #pragma unroll
   for (int r = 0; r < NN; ++r)
 #pragma unroll
@@ -438,6 +421,7 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
       As[tri(r, c)] = se - dg;
       nB[tri(r, c)] = dg - so;
     }
+  }
 
   LayerMath<NN> m;
   // Cholesky of -B_s (positive definite for omega' <= 1).
@@ -480,7 +464,14 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
       S[tri(c, r)] = s;
     }
   }
+  // Park L in the (not yet written) particular-solution slots of this task
+  // while the Jacobi sweep holds S and V in registers.
+  static_assert(NT <= 3 * N, "L must fit in the ztop/zbot/zfl slots");
+#pragma unroll
+  for (int q = 0; q < NT; ++q) a.eig[(eo.ztop() + q) * T + t] = m.L[q];
   jacobi_eigen<NN>(S, m.V, m.lam);
+#pragma unroll
+  for (int q = 0; q < NT; ++q) m.L[q] = a.eig[(eo.ztop() + q) * T + t];
 
   // Eigenvalues k = sqrt|lambda| and the G+/- blocks (:560-587):
   // u = S_d^-1 L^-T w is the eigenvector of (alpha+beta)(alpha-beta);
@@ -510,6 +501,32 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
   double zb[N], z0[N], z1[N];
 #pragma unroll
   for (int r = 0; r < N; ++r) zb[r] = z0[r] = z1[r] = 0.0;
+
+  // Beam source sums b+ + b- and b+ - b- over the positive nodes
+  // (right-hand side of src/FluxSolver.cpp:628-632 split by parity of l).
+  double bs[NN], bd[NN];
+  {
+    double chi[N];
+    scaled_coeffs(chi);
+    double p0[N];
+    const double x = -mu0;
+    p0[0] = 1.0;
+    if (N > 1) p0[1] = x;
+#pragma unroll
+    for (int l = 2; l < N; ++l) p0[l] = ((2.0 * l - 1.0) * x * p0[l - 1] - (l - 1.0) * p0[l - 2]) / l;
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      double se = 0.0, so = 0.0;
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        const double term = chi[l] * tb[to.P() + l * NN + r] * p0[l];
+        if (l % 2 == 0) se += term;
+        else so += term;
+      }
+      bs[r] = F0 * se / (2.0 * kPi);
+      bd[r] = F0 * so / (2.0 * kPi);
+    }
+  }
 
   if (beam) {
     // (I + diag(mu)/mu0 - C) x = b in sum/difference form: with s = x+ + x-,
@@ -575,17 +592,57 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
     const double dts = a.lay_d[LD_DTAU_S * TL + g];
     if (dts > 1e-4) xr1 = (pb - pt) / dts;
     xr0 = pt - xr1 * taut;
-    // F^-1 (2*1) with F = I - X - Y = -D (alpha+beta):
-    //   S_d^-1 L V Lambda^-1 V^T L^T S_d D^-1 (2*1)
-    double base[NN], ilam[NN];
+    // base = F^-1 (2*1) with F = I - X - Y = -D (alpha+beta)
+    //      = S_d^-1 (-A_s)^-1 S_d D^-1 (2*1), through a Cholesky factor of
+    // -A_s (conditioned like the reference's LU of I - C, about 1/(1-omega'),
+    // unlike the eigen-decomposition of S whose small eigenvalues are only
+    // known to eps * k_max^2).
+    double base[NN];
+    {
+      double R[NT];
+      {
+        double chi[N];
+        scaled_coeffs(chi);
 #pragma unroll
-    for (int r = 0; r < NN; ++r) {
-      base[r] = tb[to.sd() + r] * (2.0 / tb[to.cmu() + r]);
-      ilam[r] = 1.0 / m.lam[r];
+        for (int r = 0; r < NN; ++r)
+#pragma unroll
+          for (int c = 0; c <= r; ++c) {
+            double se = 0.0;
+#pragma unroll
+            for (int l = 0; l < N; l += 2) se += chi[l] * tb[to.Q() + l * NN + r] * tb[to.Q() + l * NN + c];
+            R[tri(r, c)] = ((r == c) ? 1.0 / tb[to.cmu() + r] : 0.0) - se;
+          }
+      }
+#pragma unroll
+      for (int c = 0; c < NN; ++c) {
+        double x = R[tri(c, c)];
+#pragma unroll
+        for (int k = 0; k < c; ++k) x -= R[tri(c, k)] * R[tri(c, k)];
+        const double rcc = sqrt(x > 0.0 ? x : 1.0);
+        R[tri(c, c)] = rcc;
+#pragma unroll
+        for (int r = c + 1; r < NN; ++r) {
+          double s2 = R[tri(r, c)];
+#pragma unroll
+          for (int k = 0; k < c; ++k) s2 -= R[tri(r, k)] * R[tri(c, k)];
+          R[tri(r, c)] = s2 / rcc;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < NN; ++r) {
+        double s2 = tb[to.sd() + r] * (2.0 / tb[to.cmu() + r]);
+#pragma unroll
+        for (int k = 0; k < r; ++k) s2 -= R[tri(r, k)] * base[k];
+        base[r] = s2 / R[tri(r, r)];
+      }
+#pragma unroll
+      for (int r = NN - 1; r >= 0; --r) {
+        double s2 = base[r];
+#pragma unroll
+        for (int k = r + 1; k < NN; ++k) s2 -= R[tri(k, r)] * base[k];
+        base[r] = s2 / R[tri(r, r)];
+      }
     }
-    m.ltmul(base);
-    m.vdv(base, ilam);
-    m.lmul(base);
 #pragma unroll
     for (int r = 0; r < NN; ++r) base[r] *= tb[to.sdi() + r];
     const double c1 = (1.0 - ws) * xr1, c0 = (1.0 - ws) * xr0;
@@ -621,10 +678,25 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
 }
 
 // ===========================================================================
-// k_boundary: block QR elimination of the boundary-value problem.
+// k_boundary: elimination of the boundary-value problem by LU with partial
+// pivoting, one layer panel at a time.
+//
+// The (N*L)-unknown band system of setMatrix/solve0 (src/FluxSolver.cpp:
+// 838-1131) has, for the constants C_i of layer i, the equations carried
+// from above (NN rows in C_i) and the N continuity equations of the
+// interface below layer i (rows in C_i and C_{i+1}).  Eliminating C_i from
+// these 3NN rows with partial pivoting -- the pivot sequence LINPACK's
+// xGBFA would choose on the band (src/BandSolver.hpp:29-126) up to ties --
+// leaves N pivot rows (the U rows of layer i) and NN rows in C_{i+1} that
+// are carried to the next panel.
+//
 // 16 lanes per instance, 4 instances per 64-lane workgroup.  Lane j owns
-// column j of the left (current layer) block, column j of the right (next
-// layer) block, and a redundant copy of the right-hand side.
+// column j of the left block (C_i), column j of the right block (C_{i+1})
+// and a copy of the right-hand side.  Rows live in 3NN fixed slots: pivot
+// rows are marked used instead of being swapped, their values are picked by
+// a select chain, and the next panel refills the used slots with the new
+// interface rows (row = rank of the slot among the used ones), loaded from
+// memory at computed addresses -- so no register is ever indexed dynamically.
 // ===========================================================================
 template <int NN>
 struct Panel {
@@ -633,20 +705,76 @@ struct Panel {
   double A0[R], A1[R], A2[R];
 };
 
-// Column j of the eigenvector matrix G of task t (reference ordering).
+template <int SRC>
+__device__ __forceinline__ int row_bcast_i(int x) {
+  return __builtin_amdgcn_mov_dpp(x, 0x150 + SRC, 0xF, 0xF, false);
+}
+
+// Entry G(q, j) of the eigenvector matrix of task t (reference ordering),
+// with a run-time row q.
 template <int NN>
-__device__ __forceinline__ void load_gcol(const FluxArgs& a, int64_t t, int j, double (&gcol)[2 * NN]) {
+__device__ __forceinline__ double g_entry(const FluxArgs& a, int64_t t, int q, int j) {
   constexpr EigOff eo(2 * NN);
-  const int64_t T = a.n_tasks;
   const bool up = j >= NN;
   const int jp = up ? j - NN : NN - 1 - j;
+  const bool lower = q >= NN;
+  const int r = lower ? q - NN : NN - 1 - q;
+  // lower & up: gp; upper & up: gm; lower & down: -gm; upper & down: -gp
+  const int f = (lower == up) ? eo.gp() : eo.gm();
+  const double v = a.eig[(f + r + jp * NN) * (int64_t)a.n_tasks + t];
+  return up ? v : -v;
+}
+
+// One panel: N steps of partial-pivot elimination over the candidate slots
+// (those not in `used`).  Pivot rows are written to pp as the U rows of this
+// layer in pivot order: [c][j] left block, N*N + [c][j] right block, 2N*N + c
+// right-hand side.  Returns the mask of pivot slots.
+template <int NN>
+__device__ __forceinline__ unsigned lu_panel(Panel<NN>& pn, unsigned used, int lane, bool live, double* pp) {
+  constexpr int N = 2 * NN;
+  constexpr int R = 3 * NN;
+  static_for<0, N>([&](auto cc) {
+    constexpr int c = decltype(cc)::value;
+    // Pivot search in column c (meaningful in lane c): first largest |a|.
+    int p = -1;
+    double best = -1.0;
 #pragma unroll
-  for (int r = 0; r < NN; ++r) {
-    const double gp = a.eig[(eo.gp() + r + jp * NN) * T + t];
-    const double gm = a.eig[(eo.gm() + r + jp * NN) * T + t];
-    gcol[NN + r] = up ? gp : -gm;
-    gcol[NN - 1 - r] = up ? gm : -gp;
-  }
+    for (int r = 0; r < R; ++r) {
+      const bool cand = !((used >> r) & 1u);
+      const double mag = fabs(pn.A0[r]);
+      const bool take = cand && (mag > best);
+      best = take ? mag : best;
+      p = take ? r : p;
+    }
+    p = row_bcast_i<c>(p);
+    // Pivot-row entries of this lane's columns.
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool hit = (r == p);
+      v0 = hit ? pn.A0[r] : v0;
+      v1 = hit ? pn.A1[r] : v1;
+      v2 = hit ? pn.A2[r] : v2;
+    }
+    // Multipliers of column c (lane c), broadcast to the row.
+    const double piv = v0;
+    const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
+    used |= 1u << p;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool elim = !((used >> r) & 1u);
+      const double l = row_bcast<c>(elim ? pn.A0[r] * inv : 0.0);
+      pn.A0[r] -= l * v0;
+      pn.A1[r] -= l * v1;
+      pn.A2[r] -= l * v2;
+    }
+    if (live) {
+      pp[c * N + lane] = v0;
+      pp[N * N + c * N + lane] = v1;
+    }
+    if (lane == 0) pp[2 * N * N + c] = v2;
+  });
+  return used;
 }
 
 template <int NN>
@@ -657,6 +785,7 @@ __global__ __launch_bounds__(64) void k_boundary(FluxArgs a) {
   constexpr TabOff to(N);
   constexpr int PS = panel_stride(N);
   static_assert(N <= 16, "one 16-lane row per instance");
+  static_assert(R <= 32, "slot mask is 32 bits");
   const int grp = threadIdx.x >> 4;
   const int lane = threadIdx.x & 15;
   const int bk = blockIdx.x * 4 + grp;
@@ -676,147 +805,98 @@ __global__ __launch_bounds__(64) void k_boundary(FluxArgs a) {
   const int j = (lane < N) ? lane : 0;  // idle lanes mirror column 0, results discarded
   const bool live = lane < N;
 
-  Panel<NN> pn;
-#pragma unroll
-  for (int r = 0; r < R; ++r) pn.A0[r] = pn.A1[r] = pn.A2[r] = 0.0;
-
-  // Exponential factor of mode column j over layer task tt:
-  // Etop (mode value at the layer top) and Ebot (at the bottom).
+  // Mode column j over layer task tt: value at the layer top / bottom.
   auto efac = [&](int64_t tt, double& etop, double& ebot) {
     const int jp = (j >= NN) ? j - NN : NN - 1 - j;
     const double k = a.eig[(eo.k() + jp) * T + tt];
-    const double dts = a.lay_d[LD_DTAU_S * TL + lo + (tt - t0)];
-    const double e = exp(-k * dts);
+    const double e = exp(-k * a.lay_d[LD_DTAU_S * TL + lo + (tt - t0)]);
     etop = (j < NN) ? e : 1.0;
     ebot = (j < NN) ? 1.0 : e;
   };
 
-  for (int ip = 0; ip < M; ++ip) {
-    const int64_t tt = t0 + ip;
-    const bool last = (ip == M - 1);
-    double g[N], etop, ebot;
-    load_gcol<NN>(a, tt, j, g);
-    efac(tt, etop, ebot);
-    if (ip == 0) {
-      // Top boundary: downward intensities at tau=0 (:890-907, :976-990).
-      const double top = sc[DISORT_S_ISO_TOP] + a.inst_d[i * kInstD + ID_PL_TOP];
+  Panel<NN> pn;
+  // Top boundary in slots 0..NN-1: downward intensities at tau=0
+  // (:890-907, :976-990).  They are stored as if carried from a previous
+  // panel (right block A1), which the first panel moves into the left block;
+  // slots NN.. are free (marked used) for the first interface.
+  {
+    double etop, ebot;
+    efac(t0, etop, ebot);
+    const double top = sc[DISORT_S_ISO_TOP] + a.inst_d[i * kInstD + ID_PL_TOP];
 #pragma unroll
-      for (int r = 0; r < NN; ++r) {
-        pn.A0[r] = live ? g[NN - 1 - r] * etop : 0.0;
-        pn.A1[r] = 0.0;
-        pn.A2[r] = top - a.eig[(eo.ztop() + NN - 1 - r) * T + tt];
-      }
-    }
-    if (!last) {
-      // Continuity at the interface below layer ip (:851-887, :1094-1118).
-      double gn[N], etn, ebn;
-      load_gcol<NN>(a, tt + 1, j, gn);
-      efac(tt + 1, etn, ebn);
-#pragma unroll
-      for (int r = 0; r < N; ++r) {
-        pn.A0[NN + r] = live ? g[r] * ebot : 0.0;
-        pn.A1[NN + r] = live ? -gn[r] * etn : 0.0;
-        pn.A2[NN + r] = a.eig[(eo.ztop() + r) * T + tt + 1] - a.eig[(eo.zbot() + r) * T + tt];
-      }
-    } else {
-      // Bottom boundary: Lambertian reflection of the downward field (:909-959, :992-1091).
-      const bool refl = !(lyrcut || diffusion);
-      double rs = 0.0;
-      if (refl) {
-#pragma unroll
-        for (int k = 0; k < NN; ++k) rs += tb[to.cwt() + k] * tb[to.cmu() + k] * alb * g[NN - 1 - k];
-      }
-      double zrefl = 0.0;
-      if (refl) {
-#pragma unroll
-        for (int k = 0; k < NN; ++k)
-          zrefl += tb[to.cwt() + k] * tb[to.cmu() + k] * alb * a.eig[(eo.zbot() + NN - 1 - k) * T + tt];
-      }
-      const double btn = a.inst_d[i * kInstD + ID_BTRANS_NCUT];
-      const double plb = a.inst_d[i * kInstD + ID_PL_BOT];
-      const double plbd = a.inst_d[i * kInstD + ID_PL_BOT_DERIV];
-#pragma unroll
-      for (int r = 0; r < NN; ++r) {
-        pn.A0[NN + r] = live ? (g[NN + r] - 2.0 * rs) * ebot : 0.0;
-        pn.A1[NN + r] = 0.0;
-        double s = -a.eig[(eo.zbot() + NN + r) * T + tt];
-        if (diffusion) {
-          s += plb + tb[to.cmu() + r] * plbd;
-        } else {
-          if (!lyrcut) s += 2.0 * zrefl;
-          if (F0 > 0.0 && !lyrcut) s += alb * mu0 * F0 * btn / kPi;
-          s += sc[DISORT_S_ISO_BOT];
-          if (thermal) s += (1.0 - alb) * plb;
-        }
-        pn.A2[NN + r] = s;
-      }
-#pragma unroll
-      for (int r = N; r < R; ++r) pn.A0[r] = pn.A1[r] = pn.A2[r] = 0.0;
-    }
-
-    // Householder QR of the left block, applied to the right block and rhs.
-    static_for<0, N>([&](auto cc) {
-      constexpr int c = decltype(cc)::value;
-      double tail = 0.0;
-#pragma unroll
-      for (int r = c + 1; r < R; ++r) tail += pn.A0[r] * pn.A0[r];
-      const double x0 = pn.A0[c];
-      const double nrm = sqrt(tail + x0 * x0);
-      const double alpha = (x0 >= 0.0) ? -nrm : nrm;
-      const double vc = x0 - alpha;
-      const double vv = tail + vc * vc;
-      const double beta = (vv > 0.0) ? 2.0 / vv : 0.0;
-      double bv[R];
-      bv[c] = row_bcast<c>(vc);
-#pragma unroll
-      for (int r = c + 1; r < R; ++r) bv[r] = row_bcast<c>(pn.A0[r]);
-      const double bb = row_bcast<c>(beta);
-      const double ba = row_bcast<c>(alpha);
-      double w0 = 0.0, w1 = 0.0, w2 = 0.0;
-#pragma unroll
-      for (int r = c; r < R; ++r) {
-        w0 += bv[r] * pn.A0[r];
-        w1 += bv[r] * pn.A1[r];
-        w2 += bv[r] * pn.A2[r];
-      }
-      w0 *= bb;
-      w1 *= bb;
-      w2 *= bb;
-#pragma unroll
-      for (int r = c; r < R; ++r) {
-        pn.A0[r] -= w0 * bv[r];
-        pn.A1[r] -= w1 * bv[r];
-        pn.A2[r] -= w2 * bv[r];
-      }
-      if (lane == c) {
-        pn.A0[c] = ba;
-#pragma unroll
-        for (int r = c + 1; r < R; ++r) pn.A0[r] = 0.0;
-      }
-    });
-
-    double* pp = a.panel + tt * PS;
-    if (live) {
-#pragma unroll
-      for (int r = 0; r < N; ++r) {
-        pp[j * N + r] = pn.A0[r];
-        pp[N * N + j * N + r] = pn.A1[r];
-      }
-    }
-    if (lane == 0) {
-#pragma unroll
-      for (int r = 0; r < N; ++r) pp[2 * N * N + r] = pn.A2[r];
-    }
-    // Equations left over for the next layer's unknowns.
-#pragma unroll
-    for (int r = 0; r < NN; ++r) {
-      pn.A0[r] = pn.A1[N + r];
-      pn.A1[r] = 0.0;
-      pn.A2[r] = pn.A2[N + r];
+    for (int r = 0; r < R; ++r) {
+      const bool tbc = r < NN;
+      pn.A0[r] = 0.0;
+      pn.A1[r] = (tbc && live) ? g_entry<NN>(a, t0, tbc ? NN - 1 - r : 0, j) * etop : 0.0;
+      pn.A2[r] = tbc ? top - a.eig[(eo.ztop() + (tbc ? NN - 1 - r : 0)) * T + t0] : 0.0;
     }
   }
+  unsigned used = ((1u << R) - 1u) & ~((1u << NN) - 1u);
 
-  // Back substitution, lane = row k of each panel.
+  // Interior panels: continuity at the interface below layer ip (:851-887, :1094-1118).
+  for (int ip = 0; ip < M - 1; ++ip) {
+    const int64_t tt = t0 + ip;
+    double etop, ebot, etn, ebn;
+    efac(tt, etop, ebot);
+    efac(tt + 1, etn, ebn);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool fill = (used >> r) & 1u;
+      const int q = fill ? __popc(used & ((1u << r) - 1u)) : 0;  // interface row of this slot
+      const double gi = live ? g_entry<NN>(a, tt, q, j) : 0.0;
+      const double gn = live ? g_entry<NN>(a, tt + 1, q, j) : 0.0;
+      const double rhs = a.eig[(eo.ztop() + q) * T + tt + 1] - a.eig[(eo.zbot() + q) * T + tt];
+      const double carried = pn.A1[r];
+      pn.A0[r] = fill ? gi * ebot : carried;
+      pn.A1[r] = fill ? -gn * etn : 0.0;
+      pn.A2[r] = fill ? rhs : pn.A2[r];
+    }
+    used = lu_panel<NN>(pn, 0u, lane, live, a.panel + tt * PS);
+  }
+
+  // Last panel: carried equations plus the bottom boundary, Lambertian
+  // reflection of the downward field (:909-959, :992-1091).  Only NN of the
+  // free slots take bottom rows; the others stay out of the pivot search.
+  {
+    const int64_t tt = t0 + M - 1;
+    double etop, ebot;
+    efac(tt, etop, ebot);
+    const bool refl = !(lyrcut || diffusion);
+    double rs = 0.0, zrefl = 0.0;
+#pragma unroll
+    for (int k = 0; k < NN; ++k) {
+      const double wk = tb[to.cwt() + k] * tb[to.cmu() + k] * alb;
+      rs += wk * (live ? g_entry<NN>(a, tt, NN - 1 - k, j) : 0.0);
+      zrefl += wk * a.eig[(eo.zbot() + NN - 1 - k) * T + tt];
+    }
+    rs = refl ? rs : 0.0;
+    const double btn = a.inst_d[i * kInstD + ID_BTRANS_NCUT];
+    const double plb = a.inst_d[i * kInstD + ID_PL_BOT];
+    const double plbd = a.inst_d[i * kInstD + ID_PL_BOT_DERIV];
+    double extra = sc[DISORT_S_ISO_BOT];
+    if (refl) extra += 2.0 * zrefl;
+    if (F0 > 0.0 && !lyrcut) extra += alb * mu0 * F0 * btn / kPi;
+    if (thermal) extra += (1.0 - alb) * plb;
+    unsigned excluded = 0u;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const bool fill = (used >> r) & 1u;
+      const int q = fill ? __popc(used & ((1u << r) - 1u)) : 0;
+      const bool bot = fill && q < NN;
+      const int t = bot ? q : 0;
+      const double gb = live ? g_entry<NN>(a, tt, NN + t, j) : 0.0;
+      const double zb = -a.eig[(eo.zbot() + NN + t) * T + tt];
+      const double carried = pn.A1[r];
+      pn.A0[r] = bot ? (gb - 2.0 * rs) * ebot : (fill ? 0.0 : carried);
+      pn.A1[r] = 0.0;
+      pn.A2[r] = bot ? zb + (diffusion ? plb + tb[to.cmu() + t] * plbd : extra) : (fill ? 0.0 : pn.A2[r]);
+      excluded |= (fill && !bot) ? (1u << r) : 0u;
+    }
+    lu_panel<NN>(pn, excluded, lane, live, a.panel + tt * PS);
+  }
+
+  // Back substitution over the layers, lane = U row k of each panel:
+  // x_k = (r_k - sum_{c>k} U(k,c) x_c - sum_c V(k,c) xnext_c) / U(k,k).
   const int k = j;
   double xn = 0.0;  // this lane's entry of the next layer's constants
   for (int ip = M - 1; ip >= 0; --ip) {
@@ -826,17 +906,16 @@ __global__ __launch_bounds__(64) void k_boundary(FluxArgs a) {
     if (ip < M - 1) {
       static_for<0, N>([&](auto cj) {
         constexpr int c = decltype(cj)::value;
-        v -= pp[N * N + c * N + k] * row_bcast<c>(xn);
+        v -= pp[N * N + k * N + c] * row_bcast<c>(xn);
       });
     }
     const double dg = pp[k * N + k];
     double xk = 0.0;
     static_for<0, N>([&](auto cj) {
       constexpr int c = N - 1 - decltype(cj)::value;
-      const double qv = v / dg;
-      const double xc = row_bcast<c>(qv);
-      if (k == c) xk = xc;
-      v -= pp[c * N + k] * xc;
+      const double xc = row_bcast<c>(v / dg);
+      xk = (k == c) ? xc : xk;
+      v -= pp[k * N + c] * xc;
     });
     xn = xk;
     if (live) a.cvec[tt * N + k] = xk;
@@ -874,34 +953,37 @@ __global__ __launch_bounds__(64) void k_flux(FluxArgs a) {
     const double ut = a.lev_d[VD_TAU_S * TV + vo + lu];
     const double dts = a.lay_d[LD_DTAU_S * TL + lo + lyu];
     // Mode amplitudes at this level: C_j e_j.
-    double yd[NN], yu[NN];
+    // Intensity at the quadrature angles: I = G (C o e) + Z, accumulated one
+    // mode pair at a time (mode NN-1-jp has eigenvalue -k_jp, mode NN+jp +k_jp).
+    const int zoff = (lu == 0) ? eo.ztop() : eo.zfl();
+    double iu[NN], id[NN];
 #pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      iu[r] = a.eig[(zoff + NN + r) * T + tt];
+      id[r] = a.eig[(zoff + NN - 1 - r) * T + tt];
+    }
+#pragma unroll 1
     for (int jp = 0; jp < NN; ++jp) {
       const double e = exp(-a.eig[(eo.k() + jp) * T + tt] * dts);
-      const double cd = a.cvec[tt * N + NN - 1 - jp];  // mode with eigenvalue -k_jp
-      const double cu = a.cvec[tt * N + NN + jp];      // mode with eigenvalue +k_jp
-      yd[jp] = cd * ((lu == 0) ? e : 1.0);
-      yu[jp] = cu * ((lu == 0) ? 1.0 : e);
+      const double yd = a.cvec[tt * N + NN - 1 - jp] * ((lu == 0) ? e : 1.0);
+      const double yu = a.cvec[tt * N + NN + jp] * ((lu == 0) ? 1.0 : e);
+#pragma unroll
+      for (int r = 0; r < NN; ++r) {
+        const double gp = a.eig[(eo.gp() + r + jp * NN) * T + tt];
+        const double gm = a.eig[(eo.gm() + r + jp * NN) * T + tt];
+        iu[r] += gp * yu - gm * yd;
+        id[r] += gm * yu - gp * yd;
+      }
     }
-    const int zoff = (lu == 0) ? eo.ztop() : eo.zfl();
     double mi = 0.0, mid = 0.0, miu = 0.0, fldn = 0.0, flup = 0.0;
 #pragma unroll
     for (int r = 0; r < NN; ++r) {
-      double iu = a.eig[(zoff + NN + r) * T + tt];
-      double id = a.eig[(zoff + NN - 1 - r) * T + tt];
-#pragma unroll
-      for (int jp = 0; jp < NN; ++jp) {
-        const double gp = a.eig[(eo.gp() + r + jp * NN) * T + tt];
-        const double gm = a.eig[(eo.gm() + r + jp * NN) * T + tt];
-        iu += gp * yu[jp] - gm * yd[jp];
-        id += gm * yu[jp] - gp * yd[jp];
-      }
       const double w = tb[to.cwt() + r], cm = tb[to.cmu() + r];
-      mi += w * id + w * iu;
-      mid += w * id;
-      miu += w * iu;
-      fldn += w * id * cm;
-      flup += w * iu * cm;
+      mi += w * id[r] + w * iu[r];
+      mid += w * id[r];
+      miu += w * iu[r];
+      fldn += w * id[r] * cm;
+      flup += w * iu[r] * cm;
     }
     double fact = 0.0, dirint = 0.0, fldir = 0.0, fdir = 0.0;
     if (F0 > 0.0) {

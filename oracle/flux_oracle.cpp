@@ -22,7 +22,7 @@
 
 #include "oracle.hpp"
 
-namespace oracle {
+namespace ORACLE_NS {
 
 void validate_flux_config(const FluxConfig& c) {
   // src/DisortFluxConfig.cpp:43-143
@@ -67,39 +67,39 @@ struct FluxSolver {
   int N, NN, L, ncut = 0;
   bool deltam = false, deltam_plus = false, lyrcut = false;
 
-  std::vector<double> cmu, cwt, sd, sdinv, ylmc, ylm0;
-  std::vector<double> dtaus, taus, ssas, fwd, gl;     // scaled optical properties
-  std::vector<double> chap, chtau, btrans;            // beam attenuation
-  std::vector<double> kk, gc, cc;                     // eigen-system and phase matrix per layer
-  std::vector<double> zb, zbs, xba, z0, z1, xr0, xr1; // particular solutions
-  double pl_top = 0, pl_bot = 0, pl_bot_deriv = 0;
-  std::vector<double> LL;                             // integration constants
+  std::vector<Real> cmu, cwt, sd, sdinv, ylmc, ylm0;
+  std::vector<Real> dtaus, taus, ssas, fwd, gl;     // scaled optical properties
+  std::vector<Real> chap, chtau, btrans;            // beam attenuation
+  std::vector<Real> kk, gc, cc;                     // eigen-system and phase matrix per layer
+  std::vector<Real> zb, zbs, xba, z0, z1, xr0, xr1; // particular solutions
+  Real pl_top = 0, pl_bot = 0, pl_bot_deriv = 0;
+  std::vector<Real> LL;                             // integration constants
 
   explicit FluxSolver(const FluxConfig& cfg) : c(cfg), N(cfg.nstr), NN(cfg.nstr / 2), L(cfg.nlyr) {}
 
-  double& G(int lc, int r, int col) { return gc[(size_t)lc * N * N + r + (size_t)col * N]; }
-  double& CC(int lc, int r, int col) { return cc[(size_t)lc * N * N + r + (size_t)col * N]; }
-  double& GL(int k, int lc) { return gl[(size_t)lc * N + k]; }
-  double Y(int l, int i) const { return ylmc[(size_t)l * N + i]; }
+  Real& G(int lc, int r, int col) { return gc[(size_t)lc * N * N + r + (size_t)col * N]; }
+  Real& CC(int lc, int r, int col) { return cc[(size_t)lc * N * N + r + (size_t)col * N]; }
+  Real& GL(int k, int lc) { return gl[(size_t)lc * N + k]; }
+  Real Y(int l, int i) const { return ylmc[(size_t)l * N + i]; }
 
   // src/FluxSolver.cpp:309-452
   void set_up() {
-    const double abscut = 10.0;
-    std::vector<double> tauc(L + 1, 0.0);
+    const Real abscut = 10.0;
+    std::vector<Real> tauc(L + 1, 0.0);
     for (int lc = 0; lc < L; ++lc) tauc[lc + 1] = tauc[lc] + c.dtau[lc];
     dtaus.assign(L, 0.0);
     taus.assign(L + 1, 0.0);
     ssas.assign(L, 0.0);
     fwd.assign(L, 0.0);
     gl.assign((size_t)L * N, 0.0);
-    double abstau = 0.0;
+    Real abstau = 0.0;
     ncut = L;
     for (int lc = 0; lc < L; ++lc) {
       if (abstau < abscut) ncut = lc + 1;
       abstau += (1.0 - c.ssa[lc]) * c.dtau[lc];
-      const double w = c.ssa[lc];
-      double f;
-      auto plain_delta_m = [&](double fr) {
+      const Real w = c.ssa[lc];
+      Real f;
+      auto plain_delta_m = [&](Real fr) {
         ssas[lc] = w * (1.0 - fr) / (1.0 - fr * w);
         dtaus[lc] = (1.0 - fr * w) * c.dtau[lc];
         taus[lc + 1] = taus[lc] + dtaus[lc];
@@ -115,21 +115,21 @@ struct FluxSolver {
         f = c.pm(N, lc);
         plain_delta_m(f);
       } else {
-        const double pm = c.pm(N, lc), pm1 = c.pm(N + 1, lc);
+        const Real pm = c.pm(N, lc), pm1 = c.pm(N + 1, lc);
         if (pm == pm1) {
           f = pm;
           plain_delta_m(f);
         } else {
           // Delta-M+ (Lin et al. 2018): Gaussian tail fit through moments N, N+1.
-          const double M = static_cast<double>(N);
-          const double sig2 = ((M + 1.0) * (M + 1.0) - M * M) / (std::log(pm * pm) - std::log(pm1 * pm1));
-          const double cf = std::exp(M * M / (2.0 * sig2));
+          const Real M = static_cast<Real>(N);
+          const Real sig2 = ((M + 1.0) * (M + 1.0) - M * M) / (std::log(pm * pm) - std::log(pm1 * pm1));
+          const Real cf = std::exp(M * M / (2.0 * sig2));
           f = cf * pm;
           ssas[lc] = w * (1.0 - f) / (1.0 - f * w);
           dtaus[lc] = (1.0 - f * w) * c.dtau[lc];
           taus[lc + 1] = taus[lc] + dtaus[lc];
           for (int k = 0; k < N; ++k) {
-            const double kd = static_cast<double>(k);
+            const Real kd = static_cast<Real>(k);
             GL(k, lc) = (2.0 * k + 1.0) * ssas[lc] * (c.pm(k, lc) - f * std::exp(-kd * kd / (2.0 * sig2))) / (1.0 - f);
           }
         }
@@ -146,8 +146,8 @@ struct FluxSolver {
       if (c.spherical) {
         // The reference's beam_mu<0 branch is unreachable after validation.
         for (int lc = 0; lc < ncut; ++lc) {
-          const double taup_mid = taus[lc] + dtaus[lc] / 2.0;
-          const double ch_mid = chapman(lc + 1, 0.5, c.alt.data(), dtaus.data(), L, c.radius, c.beam_mu);
+          const Real taup_mid = taus[lc] + dtaus[lc] / 2.0;
+          const Real ch_mid = chapman(lc + 1, 0.5, c.alt.data(), dtaus.data(), L, c.radius, c.beam_mu);
           chap[lc] = taup_mid / ch_mid;
           chtau[lc + 1] = chapman(lc + 1, 0.0, c.alt.data(), dtaus.data(), L, c.radius, c.beam_mu);
           btrans[lc + 1] = std::exp(-chtau[lc + 1]);
@@ -175,10 +175,10 @@ struct FluxSolver {
 
   // src/FluxSolver.cpp:459-588
   void solve_eigen(int lc) {
-    std::vector<double> top((size_t)NN * N);  // cc_top, col-major NN x N
+    std::vector<Real> top((size_t)NN * N);  // cc_top, col-major NN x N
     for (int j = 0; j < N; ++j)
       for (int i = 0; i < NN; ++i) {
-        double s = 0.0;
+        Real s = 0.0;
         for (int l = 0; l < N; ++l) s += (GL(l, lc) * Y(l, i)) * Y(l, j);
         top[i + (size_t)j * NN] = s * 0.5 * cwt[j];
       }
@@ -189,9 +189,9 @@ struct FluxSolver {
         CC(lc, NN + i, j) = top[i + (size_t)(NN + j) * NN];
         CC(lc, NN + i, NN + j) = top[i + (size_t)j * NN];
       }
-    std::vector<double> amb(NN * NN), apb(NN * NN);
+    std::vector<Real> amb(NN * NN), apb(NN * NN);
     for (int i = 0; i < NN; ++i) {
-      const double inv = 1.0 / cmu[i];
+      const Real inv = 1.0 / cmu[i];
       for (int j = 0; j < NN; ++j) {
         amb[i + j * NN] = inv * (top[i + j * NN] - top[i + (NN + j) * NN]);
         apb[i + j * NN] = inv * (top[i + j * NN] + top[i + (NN + j) * NN]);
@@ -199,33 +199,33 @@ struct FluxSolver {
       amb[i + i * NN] -= inv;
       apb[i + i * NN] -= inv;
     }
-    std::vector<double> as(NN * NN), nbs(NN * NN), nas(NN * NN);
+    std::vector<Real> as(NN * NN), nbs(NN * NN), nas(NN * NN);
     for (int i = 0; i < NN; ++i)
       for (int j = 0; j < NN; ++j) {
         as[i + j * NN] = sd[i] * apb[i + j * NN] * sdinv[j];
         nbs[i + j * NN] = -sd[i] * amb[i + j * NN] * sdinv[j];
         nas[i + j * NN] = -as[i + j * NN];
       }
-    std::vector<double> lw(NN * NN), S(NN * NN, 0.0), tmp(NN * NN, 0.0), ev(NN), W(NN * NN), vec(NN * NN);
+    std::vector<Real> lw(NN * NN), S(NN * NN, 0.0), tmp(NN * NN, 0.0), ev(NN), W(NN * NN), vec(NN * NN);
     if (cholesky_lower(NN, nbs.data(), lw.data())) {
       // S = L^T (-A_s) L ; eigenvectors of M: u = D^{-1} L^{-T} w
       for (int j = 0; j < NN; ++j)
         for (int i = 0; i < NN; ++i) {
-          double s = 0.0;
+          Real s = 0.0;
           for (int k = 0; k < NN; ++k) s += nas[i + k * NN] * lw[k + j * NN];
           tmp[i + j * NN] = s;
         }
       for (int j = 0; j < NN; ++j)
         for (int i = 0; i < NN; ++i) {
-          double s = 0.0;
+          Real s = 0.0;
           for (int k = 0; k < NN; ++k) s += lw[k + i * NN] * tmp[k + j * NN];
           S[i + j * NN] = s;
         }
       sym_eigen_jacobi(NN, S.data(), ev.data(), W.data());
       for (int col = 0; col < NN; ++col) {
-        std::vector<double> v(NN);
+        std::vector<Real> v(NN);
         for (int i = NN - 1; i >= 0; --i) {  // L^T v = w (upper-triangular)
-          double s = W[i + col * NN];
+          Real s = W[i + col * NN];
           for (int k = i + 1; k < NN; ++k) s -= lw[k + i * NN] * v[k];
           v[i] = s / lw[i + i * NN];
         }
@@ -235,43 +235,43 @@ struct FluxSolver {
       // -A_s = R R^T: T = R^T (-B_s) R, u = D^{-1} R w.
       for (int j = 0; j < NN; ++j)
         for (int i = 0; i < NN; ++i) {
-          double s = 0.0;
+          Real s = 0.0;
           for (int k = 0; k < NN; ++k) s += nbs[i + k * NN] * lw[k + j * NN];
           tmp[i + j * NN] = s;
         }
       for (int j = 0; j < NN; ++j)
         for (int i = 0; i < NN; ++i) {
-          double s = 0.0;
+          Real s = 0.0;
           for (int k = 0; k < NN; ++k) s += lw[k + i * NN] * tmp[k + j * NN];
           S[i + j * NN] = s;
         }
       sym_eigen_jacobi(NN, S.data(), ev.data(), W.data());
       for (int col = 0; col < NN; ++col)
         for (int j = 0; j < NN; ++j) {
-          double s = 0.0;
+          Real s = 0.0;
           for (int k = 0; k <= j; ++k) s += lw[j + k * NN] * W[k + col * NN];
           vec[j + col * NN] = sdinv[j] * s;
         }
     } else {
       throw std::runtime_error("oracle: neither -B_s nor -A_s is positive definite (degenerate layer)");
     }
-    std::vector<double> rk(NN);
+    std::vector<Real> rk(NN);
     for (int iq = 0; iq < NN; ++iq) {
       rk[iq] = std::sqrt(std::abs(ev[iq]));
       kk[(size_t)lc * N + iq + NN] = rk[iq];
       kk[(size_t)lc * N + NN - 1 - iq] = -rk[iq];
     }
-    std::vector<double> sum(NN * NN);  // (alpha-beta) u / k
+    std::vector<Real> sum(NN * NN);  // (alpha-beta) u / k
     for (int j = 0; j < NN; ++j)
       for (int i = 0; i < NN; ++i) {
-        double s = 0.0;
+        Real s = 0.0;
         for (int k = 0; k < NN; ++k) s += amb[i + k * NN] * vec[k + j * NN];
         sum[i + j * NN] = s / rk[j];
       }
     for (int jq = 0; jq < NN; ++jq)
       for (int iq = 0; iq < NN; ++iq) {
-        double plus = sum[iq + jq * NN];
-        const double minus = vec[iq + jq * NN];
+        Real plus = sum[iq + jq * NN];
+        const Real minus = vec[iq + jq * NN];
         G(lc, NN + iq, NN + jq) = 0.5 * (plus + minus);
         G(lc, NN - iq - 1, NN + jq) = 0.5 * (plus - minus);
         plus = -plus;
@@ -283,16 +283,16 @@ struct FluxSolver {
   // src/FluxSolver.cpp:595-643
   void beam_source() {
     ylm0.assign((size_t)N + 1, 0.0);
-    const double mu = -c.beam_mu;
+    const Real mu = -c.beam_mu;
     legendre_flat(1, 0, 0, N - 1, &mu, ylm0.data());
-    const double thresh = 1000.0 * std::numeric_limits<double>::epsilon();
-    std::vector<double> A((size_t)N * N), b(N);
+    const Real thresh = 1000.0 * std::numeric_limits<double>::epsilon();
+    std::vector<Real> A((size_t)N * N), b(N);
     DenseLU lu;
     for (int lc = 0; lc < L; ++lc) {
-      double bmu = c.beam_mu;
-      const double mu0sq = c.beam_mu * c.beam_mu;
+      Real bmu = c.beam_mu;
+      const Real mu0sq = c.beam_mu * c.beam_mu;
       for (int iq = 0; iq < NN; ++iq) {
-        const double k = kk[(size_t)lc * N + iq + NN];
+        const Real k = kk[(size_t)lc * N + iq + NN];
         if (std::abs(1.0 - mu0sq * k * k) < thresh) {
           bmu = 0.999 * c.beam_mu;
           break;
@@ -302,7 +302,7 @@ struct FluxSolver {
         for (int i = 0; i < N; ++i) A[i + (size_t)j * N] = -CC(lc, i, j);
       for (int i = 0; i < N; ++i) A[i + (size_t)i * N] += 1.0 + cmu[i] / bmu;
       for (int iq = 0; iq < N; ++iq) {
-        double s = 0.0;
+        Real s = 0.0;
         for (int k = 0; k < N; ++k) s += GL(k, lc) * Y(k, iq) * ylm0[k];
         b[iq] = c.beam_flux * s / (4.0 * M_PI);
       }
@@ -318,20 +318,20 @@ struct FluxSolver {
   // src/FluxSolver.cpp:650-738
   void beam_source_spherical() {
     ylm0.assign((size_t)N + 1, 0.0);
-    const double mu = -c.beam_mu;
+    const Real mu = -c.beam_mu;
     legendre_flat(1, 0, 0, N - 1, &mu, ylm0.data());
-    const double big = std::sqrt(std::numeric_limits<double>::max()) / 1.0e10;
-    std::vector<double> A((size_t)N * N), zj(N), xb0(N), xb1(N);
+    const Real big = std::sqrt(std::numeric_limits<double>::max()) / 1.0e10;
+    std::vector<Real> A((size_t)N * N), zj(N), xb0(N), xb1(N);
     DenseLU lu;
     for (int lc = 0; lc < L; ++lc) {
       for (int iq = 0; iq < N; ++iq) {
-        double s = 0.0;
+        Real s = 0.0;
         for (int k = 0; k < N; ++k) s += GL(k, lc) * Y(k, iq) * ylm0[k];
         zj[iq] = c.beam_flux * s / (4.0 * M_PI);
       }
-      const double q0a = std::exp(-chtau[lc]);
-      const double q2a = std::exp(-chtau[lc + 1]);
-      double x = 1.0 / chap[lc];
+      const Real q0a = std::exp(-chtau[lc]);
+      const Real q2a = std::exp(-chtau[lc + 1]);
+      Real x = 1.0 / chap[lc];
       if (std::abs(x) > big && taus[lc + 1] > 1.0) x = 0.0;
       if (std::abs(x * taus[lc + 1]) > std::log(big)) x = 0.0;
       if (std::abs(x) > 1.0e-5)
@@ -341,13 +341,13 @@ struct FluxSolver {
             break;
           }
       xba[lc] = x;
-      const double dt = taus[lc + 1] - taus[lc];
+      const Real dt = taus[lc + 1] - taus[lc];
       for (int iq = 0; iq < N; ++iq) {
-        const double q0 = q0a * zj[iq], q2 = q2a * zj[iq];
+        const Real q0 = q0a * zj[iq], q2 = q2a * zj[iq];
         xb1[iq] = (q2 * std::exp(x * taus[lc + 1]) - q0 * std::exp(x * taus[lc])) / dt;
         xb0[iq] = q0 * std::exp(x * taus[lc]) - xb1[iq] * taus[lc];
       }
-      auto build = [&](double xv) {
+      auto build = [&](Real xv) {
         for (int j = 0; j < N; ++j)
           for (int i = 0; i < N; ++i) A[i + (size_t)j * N] = -CC(lc, i, j);
         for (int i = 0; i < N; ++i) A[i + (size_t)i * N] += 1.0 + xv * cmu[i];
@@ -361,9 +361,9 @@ struct FluxSolver {
         build(x);
         lu.factor(N, A.data());
       }
-      std::vector<double> s1 = xb1;
+      std::vector<Real> s1 = xb1;
       lu.solve(s1.data());
-      std::vector<double> s0(N);
+      std::vector<Real> s0(N);
       for (int iq = 0; iq < N; ++iq) s0[iq] = xb0[iq] + cmu[iq] * s1[iq];
       lu.solve(s0.data());
       for (int iq = 0; iq < NN; ++iq) {
@@ -377,18 +377,18 @@ struct FluxSolver {
 
   // src/FluxSolver.cpp:745-831 (Jacobian derivatives not restated)
   void thermal_source() {
-    std::vector<double> pk(L + 1);
+    std::vector<Real> pk(L + 1);
     for (int l = 0; l <= L; ++l) pk[l] = planck_band(c.wn_lo, c.wn_hi, c.temper[l]);
-    std::vector<double> A((size_t)N * N), b1(N), b0(N);
+    std::vector<Real> A((size_t)N * N), b1(N), b0(N);
     DenseLU lu;
     for (int lc = 0; lc < L; ++lc) {
       for (int j = 0; j < N; ++j)
         for (int i = 0; i < N; ++i) A[i + (size_t)j * N] = -CC(lc, i, j);
       for (int i = 0; i < N; ++i) A[i + (size_t)i * N] += 1.0;
       lu.factor(N, A.data());
-      double s1 = 0.0;
+      Real s1 = 0.0;
       if (dtaus[lc] > 1e-4) s1 = (pk[lc + 1] - pk[lc]) / dtaus[lc];
-      const double s0 = pk[lc] - s1 * taus[lc];
+      const Real s0 = pk[lc] - s1 * taus[lc];
       xr0[lc] = s0;
       xr1[lc] = s1;
       for (int i = 0; i < N; ++i) b1[i] = (1.0 - ssas[lc]) * s1;
@@ -405,7 +405,7 @@ struct FluxSolver {
   }
 
   // Beam particular solution (with its attenuation) evaluated at scaled depth t in layer lc.
-  double beam_at(int lc, int iq, double t, double plane_trans) {
+  Real beam_at(int lc, int iq, Real t, Real plane_trans) {
     if (c.spherical) return std::exp(-xba[lc] * t) * (zb[(size_t)lc * N + iq] + zbs[(size_t)lc * N + iq] * t);
     return zb[(size_t)lc * N + iq] * plane_trans;
   }
@@ -416,9 +416,9 @@ struct FluxSolver {
     const int lda = 3 * ncd + 1;
     const int nshift = lda - 2 * N + 1;
     const int nn = N * L;
-    std::vector<double> band((size_t)lda * nn, 0.0);
-    auto B = [&](int r, int col) -> double& { return band[r + (size_t)col * lda]; };
-    std::vector<double> work(N);
+    std::vector<Real> band((size_t)lda * nn, 0.0);
+    auto B = [&](int r, int col) -> Real& { return band[r + (size_t)col * lda]; };
+    std::vector<Real> work(N);
     int ncol = 0;
     for (int lc = 0; lc < ncut; ++lc) {
       std::fill(work.begin(), work.end(), 0.0);
@@ -448,7 +448,7 @@ struct FluxSolver {
     // Top boundary: no diffuse downward radiation beyond the given sources.
     int jcol = 0;
     for (int iq = 0; iq < NN; ++iq) {
-      const double expa = std::exp(kk[iq] * taus[1]);
+      const Real expa = std::exp(kk[iq] * taus[1]);
       int irow = nshift - jcol + NN - 1;
       for (int jq = NN - 1; jq >= 0; --jq) B(irow++, jcol) = G(0, jq, iq) * expa;
       ++jcol;
@@ -470,7 +470,7 @@ struct FluxSolver {
         if (!refl) {
           B(irow, nncol) = G(lb, jq, iq);
         } else {
-          double rs = 0.0;
+          Real rs = 0.0;
           for (int k = 0; k < NN; ++k) rs += cwt[k] * cmu[k] * c.albedo * G(lb, NN - 1 - k, iq);
           B(irow, nncol) = G(lb, jq, iq) - 2.0 * rs;
         }
@@ -481,12 +481,12 @@ struct FluxSolver {
     for (int iq = NN; iq < N; ++iq) {
       ++nncol;
       int irow = nshift - jcol + N - 1;
-      const double expa = std::exp(kk[(size_t)lb * N + N - iq - 1] * dtaus[lb]);
+      const Real expa = std::exp(kk[(size_t)lb * N + N - iq - 1] * dtaus[lb]);
       for (int jq = NN; jq < N; ++jq) {
         if (!refl) {
           B(irow, nncol) = G(lb, jq, iq) * expa;
         } else {
-          double rs = 0.0;
+          Real rs = 0.0;
           for (int k = 0; k < NN; ++k) rs += cwt[k] * cmu[k] * c.albedo * G(lb, NN - 1 - k, iq);
           B(irow, nncol) = (G(lb, jq, iq) - 2.0 * rs) * expa;
         }
@@ -497,19 +497,19 @@ struct FluxSolver {
 
     // Right-hand side.
     const int nc = N * ncut;
-    std::vector<double> b(nc, 0.0);
+    std::vector<Real> b(nc, 0.0);
     const bool beam = c.beam_flux > 0.0;
     for (int iq = 0; iq < NN; ++iq) {
-      double s = 0.0;
+      Real s = 0.0;
       if (beam) s -= zb[NN - iq - 1];
       if (c.thermal) s -= z0[NN - iq - 1];
       s += c.iso_top + pl_top;
       b[iq] = s;
     }
-    const double tb = taus[ncut];
+    const Real tb = taus[ncut];
     if (c.diffusion_bc) {
       for (int iq = 0; iq < NN; ++iq) {
-        double s = 0.0;
+        Real s = 0.0;
         if (beam) s -= beam_at(lb, NN + iq, tb, btrans[ncut]);
         if (c.thermal) {
           s -= z0[(size_t)lb * N + NN + iq];
@@ -520,11 +520,11 @@ struct FluxSolver {
       }
     } else {
       for (int iq = 0; iq < NN; ++iq) {
-        double s = 0.0;
+        Real s = 0.0;
         if (!lyrcut) {
-          double rs = 0.0;
+          Real rs = 0.0;
           for (int jq = 0; jq < NN; ++jq) {
-            double ps = 0.0;
+            Real ps = 0.0;
             if (beam) ps += beam_at(lb, NN - 1 - jq, tb, btrans[ncut]);
             if (c.thermal) ps += z0[(size_t)lb * N + NN - 1 - jq] + z1[(size_t)lb * N + NN - 1 - jq] * tb;
             rs += cwt[jq] * cmu[jq] * c.albedo * ps;
@@ -547,7 +547,7 @@ struct FluxSolver {
     for (int lc = 1; lc < ncut; ++lc) {
       const int ip = lc * N - NN;
       for (int iq = 0; iq < N; ++iq) {
-        double s = 0.0;
+        Real s = 0.0;
         if (beam) {
           s += beam_at(lc, iq, taus[lc], btrans[lc]);
           s -= beam_at(lc - 1, iq, taus[lc], btrans[lc]);
@@ -569,8 +569,8 @@ struct FluxSolver {
   // src/FluxSolver.cpp:1138-1287
   void fluxes(FluxOutput& out) {
     for (auto& v : out.f) v.assign(L + 1, 0.0);
-    double tauc = 0.0;
-    std::vector<double> ec(N), inten(N);
+    Real tauc = 0.0;
+    std::vector<Real> ec(N), inten(N);
     for (int lu = 0; lu <= L; ++lu) {
       int lyu = (lu == 0) ? 0 : lu - 1;
       if (lyrcut && lyu > ncut - 1) {
@@ -578,8 +578,8 @@ struct FluxSolver {
         continue;
       }
       if (lyu >= ncut) lyu = ncut - 1;
-      const double ut = taus[lu];
-      double fact = 0.0, dirint = 0.0, fldir = 0.0, fdirect = 0.0;
+      const Real ut = taus[lu];
+      Real fact = 0.0, dirint = 0.0, fldir = 0.0, fdirect = 0.0;
       if (c.beam_flux > 0.0) {
         if (c.spherical) {
           fact = std::exp(-ut / chap[lyu]);
@@ -595,7 +595,7 @@ struct FluxSolver {
       for (int jq = 0; jq < NN; ++jq) ec[jq] = LL[off + jq] * std::exp(-kk[(size_t)lyu * N + jq] * (ut - taus[lyu + 1]));
       for (int jq = NN; jq < N; ++jq) ec[jq] = LL[off + jq] * std::exp(-kk[(size_t)lyu * N + jq] * (ut - taus[lyu]));
       for (int i = 0; i < N; ++i) {
-        double s = 0.0;
+        Real s = 0.0;
         for (int j = 0; j < N; ++j) s += G(lyu, i, j) * ec[j];
         inten[i] = s;
       }
@@ -603,32 +603,32 @@ struct FluxSolver {
         for (int i = 0; i < N; ++i) inten[i] += beam_at(lyu, i, ut, fact);
       if (c.thermal)
         for (int i = 0; i < N; ++i) inten[i] += z0[(size_t)lyu * N + i] + z1[(size_t)lyu * N + i] * ut;
-      double mi = 0.0, mid = 0.0, miu = 0.0, fldn = 0.0, flup = 0.0;
+      Real mi = 0.0, mid = 0.0, miu = 0.0, fldn = 0.0, flup = 0.0;
       for (int iq = 0; iq < NN; ++iq) {
         const int ci = NN - iq - 1;
-        const double v = inten[iq], w = cwt[ci];
+        const Real v = inten[iq], w = cwt[ci];
         mi += w * v;
         mid += w * v;
         fldn += w * v * cmu[ci];
       }
       for (int iq = NN; iq < N; ++iq) {
         const int ci = iq - NN;
-        const double v = inten[iq], w = cwt[ci];
+        const Real v = inten[iq], w = cwt[ci];
         mi += w * v;
         miu += w * v;
         flup += w * v * cmu[ci];
       }
       flup *= 2.0 * M_PI;
       fldn *= 2.0 * M_PI;
-      const double fdntot = fldn + fldir;
-      const double fdn = fdntot - fdirect;
+      const Real fdntot = fldn + fldir;
+      const Real fdn = fdntot - fdirect;
       mi = (2.0 * M_PI * mi + dirint) / (4.0 * M_PI);
-      const double midir = dirint / (4.0 * M_PI);
+      const Real midir = dirint / (4.0 * M_PI);
       mid = (2.0 * M_PI * mid) / (4.0 * M_PI);
       miu = (2.0 * M_PI * miu) / (4.0 * M_PI);
-      double pls = 0.0;
+      Real pls = 0.0;
       if (c.thermal) pls = xr0[lyu] + xr1[lyu] * ut;
-      const double div = (1.0 - c.ssa[lyu]) * 4.0 * M_PI * (mi - pls);
+      const Real div = (1.0 - c.ssa[lyu]) * 4.0 * M_PI * (mi - pls);
       out.f[0][lu] = fdirect;
       out.f[1][lu] = fdn;
       out.f[2][lu] = flup;
@@ -649,7 +649,7 @@ struct FluxSolver {
     deltam_plus = c.deltam_plus;
     if (deltam_plus) {
       for (int lc = 0; lc < L; ++lc) {
-        const double pm = c.pm(N, lc), pm1 = c.pm(N + 1, lc);
+        const Real pm = c.pm(N, lc), pm1 = c.pm(N + 1, lc);
         if (pm < 1e-4 || pm1 < 0.7 * pm) { deltam_plus = false; break; }
       }
       if (deltam_plus) deltam = false;
@@ -660,7 +660,7 @@ struct FluxSolver {
       pl_top = planck_band(c.wn_lo, c.wn_hi, c.temp_top) * c.emis_top;
       pl_bot = planck_band(c.wn_lo, c.wn_hi, c.temp_bot);
       if (c.diffusion_bc) {
-        const double bl = planck_band(c.wn_lo, c.wn_hi, c.temper[L - 1]);
+        const Real bl = planck_band(c.wn_lo, c.wn_hi, c.temper[L - 1]);
         if (dtaus[L - 1] > 0.0) pl_bot_deriv = (pl_bot - bl) / dtaus[L - 1];
       }
     }
@@ -715,4 +715,4 @@ FluxOutput flux_solve(const FluxConfig& cin) {
   return out;
 }
 
-}  // namespace oracle
+}  // namespace ORACLE_NS

@@ -25,34 +25,45 @@
 //        transparent medium), energy conservation at omega=1.
 #pragma once
 #include <cstdint>
+
+// The oracle is compiled twice: in FP64 (namespace oracle) and in x87
+// extended precision (namespace oracle_ld, 64-bit significand) to serve as a
+// higher-precision truth for judging FP64 rounding differences.
+#ifndef ORACLE_REAL
+#define ORACLE_REAL double
+#endif
+#ifndef ORACLE_NS
+#define ORACLE_NS oracle
+#endif
 #include <string>
 #include <vector>
 
-namespace oracle {
+namespace ORACLE_NS {
+using Real = ORACLE_REAL;
 
 // ---- numerics (restating src/Planck.cpp, src/Legendre.cpp, ...) ----------
-double planck_band(double wnumlo, double wnumhi, double temp);        // planckFunction2
-double planck_band_deriv(double wnumlo, double wnumhi, double temp);  // planckFunctionDeriv2
-void legendre_flat(int nmu, int m, int maxmu, int twonm1, const double* mu, double* ylm);
-void gauss_quadrature(int nn, double* cmu /*[2nn]*/, double* cwt /*[2nn]*/);
-double chapman(int lc, double tau_frac, const double* level_altitudes,
-               const double* dtau_lyr, int num_layers, double r, double mu0);
+Real planck_band(Real wnumlo, Real wnumhi, Real temp);        // planckFunction2
+Real planck_band_deriv(Real wnumlo, Real wnumhi, Real temp);  // planckFunctionDeriv2
+void legendre_flat(int nmu, int m, int maxmu, int twonm1, const Real* mu, Real* ylm);
+void gauss_quadrature(int nn, Real* cmu /*[2nn]*/, Real* cwt /*[2nn]*/);
+Real chapman(int lc, Real tau_frac, const Real* level_altitudes,
+               const Real* dtau_lyr, int num_layers, Real r, Real mu0);
 
 // LINPACK-semantics band LU (column-major, lda rows, 1-based ipvt).
-void band_factor(double* abd, int lda, int n, int ml, int mu, int* ipvt);
-void band_solve(const double* abd, int lda, int n, int ml, int mu, const int* ipvt, double* b);
+void band_factor(Real* abd, int lda, int n, int ml, int mu, int* ipvt);
+void band_solve(const Real* abd, int lda, int n, int ml, int mu, const int* ipvt, Real* b);
 
 // Dense helpers, column-major n x n.
-bool cholesky_lower(int n, const double* a, double* l);  // a = l l^T; false if not SPD
-void sym_eigen_jacobi(int n, const double* a, double* evals, double* evecs); // ascending
+bool cholesky_lower(int n, const Real* a, Real* l);  // a = l l^T; false if not SPD
+void sym_eigen_jacobi(int n, const Real* a, Real* evals, Real* evecs); // ascending
 // Partial-pivot LU of a (in place) + solve of one right-hand side.
 struct DenseLU {
   int n = 0;
-  std::vector<double> lu;  // column-major
+  std::vector<Real> lu;  // column-major
   std::vector<int> piv;
-  void factor(int n, const double* a);
-  void solve(double* b) const;
-  double frob_norm_of_input = 0.0;
+  void factor(int n, const Real* a);
+  void solve(Real* b) const;
+  Real frob_norm_of_input = 0.0;
 };
 
 // ---- flux solver (restating src/FluxSolver.cpp DisortFluxSolver<N>) -------
@@ -60,25 +71,25 @@ struct FluxConfig {
   int nlyr = 0, nstr = 0, nmom = 0;           // pmom rows hold max(nmom,nstr)+1 entries
   bool thermal = false, spherical = false, deltam_plus = false;
   bool diffusion_bc = false, index_from_bottom = false, jacobian = false;
-  double beam_flux = 0, beam_mu = 0.5, iso_top = 0, iso_bot = 0;
-  double temp_top = 0, temp_bot = 0, emis_top = 1, albedo = 0;
-  double wn_lo = 0, wn_hi = 0, radius = 0;
-  std::vector<double> dtau, ssa, temper, alt;
-  std::vector<double> pmom;                   // [nlyr][pstride]
+  Real beam_flux = 0, beam_mu = 0.5, iso_top = 0, iso_bot = 0;
+  Real temp_top = 0, temp_bot = 0, emis_top = 1, albedo = 0;
+  Real wn_lo = 0, wn_hi = 0, radius = 0;
+  std::vector<Real> dtau, ssa, temper, alt;
+  std::vector<Real> pmom;                   // [nlyr][pstride]
   int pstride() const { return (nmom > nstr ? nmom : nstr) + 1; }
-  double pm(int k, int lc) const { return pmom[(size_t)lc * pstride() + k]; }
+  Real pm(int k, int lc) const { return pmom[(size_t)lc * pstride() + k]; }
 };
 
 struct FluxOutput {
   // flux_direct_beam, flux_down, flux_up, flux_tau_divergence, mean_intensity,
   // mean_intensity_down, mean_intensity_up, mean_intensity_direct_beam
-  std::vector<double> f[8];
+  std::vector<Real> f[8];
   // Temperature Jacobians [6][(nlyr+1)*(nlyr+2)] when requested.
-  std::vector<double> jac[6];
+  std::vector<Real> jac[6];
 };
 
 // Throws std::invalid_argument / std::runtime_error with the reference's messages.
 void validate_flux_config(const FluxConfig& c);
 FluxOutput flux_solve(const FluxConfig& c);
 
-}  // namespace oracle
+}  // namespace ORACLE_NS

@@ -41,6 +41,7 @@ def lib():
         l.oracle_chapman.argtypes = [C.c_int, D, C.c_void_p, C.c_void_p, C.c_int, D, D]
         l.oracle_chapman.restype = D
         l.oracle_flux_one.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_char_p, C.c_int]
+        l.oracle_ld_flux_one.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_char_p, C.c_int]
         l.oracle_flux_batch.argtypes = [C.c_void_p, C.c_int, PD]
         l.oracle_flux_batch.restype = C.c_int64
         _lib = l
@@ -108,6 +109,19 @@ def flux_one(batch: FluxBatch, i: int, index_from_bottom=False):
     err = C.create_string_buffer(512)
     st = batch.c_struct()
     rc = lib().oracle_flux_one(C.byref(st), i, int(index_from_bottom), out.ctypes.data, err, 512)
+    if rc:
+        raise OracleError(rc, err.value.decode())
+    return out
+
+
+def flux_one_extended(batch: FluxBatch, i: int):
+    """The same solve in x87 extended precision (64-bit significand),
+    rounded to FP64: a higher-precision truth for rounding comparisons."""
+    L = int(batch.nlyr[i])
+    out = np.zeros((NOUT, L + 1))
+    err = C.create_string_buffer(512)
+    st = batch.c_struct()
+    rc = lib().oracle_ld_flux_one(C.byref(st), i, out.ctypes.data, err, 512)
     if rc:
         raise OracleError(rc, err.value.decode())
     return out

@@ -14,7 +14,7 @@ import numpy as np
 from .batch import NOUT, CFluxBatch, FluxBatch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "lib", "libdisort_amd.so")
+LIB_PATH = os.environ.get("DISORT_AMD_LIB") or os.path.join(_HERE, "lib", "libdisort_amd.so")
 
 DISORT_OK = 0
 DISORT_E_INVALID = 1

@@ -747,29 +747,29 @@ __device__ __forceinline__ unsigned lu_panel(Panel<NN>& pn, unsigned used, int l
       p = take ? r : p;
     }
     p = row_bcast_i<c>(p);
-    // Pivot-row entries of this lane's columns.
+    // Pivot-row entries of this lane's columns (one-hot sum over the slots).
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const bool hit = (r == p);
-      v0 = hit ? pn.A0[r] : v0;
-      v1 = hit ? pn.A1[r] : v1;
-      v2 = hit ? pn.A2[r] : v2;
+      const double hit = (r == p) ? 1.0 : 0.0;
+      v0 = fma(hit, pn.A0[r], v0);
+      v1 = fma(hit, pn.A1[r], v1);
+      v2 = fma(hit, pn.A2[r], v2);
     }
-    // Multipliers of column c (lane c), broadcast to the row.
-    const double piv = v0;
-    const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
+    // Multipliers of column c (lane c), broadcast to the row.  Every slot is
+    // eliminated: the pivot rows of this panel are already stored and are
+    // refilled by the next panel, so changing them is harmless.
+    const double inv = (v0 != 0.0) ? 1.0 / (v0 != 0.0 ? v0 : 1.0) : 0.0;
     used |= 1u << p;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const bool elim = !((used >> r) & 1u);
-      const double l = row_bcast<c>(elim ? pn.A0[r] * inv : 0.0);
+      const double l = row_bcast<c>(pn.A0[r] * inv);
       pn.A0[r] -= l * v0;
       pn.A1[r] -= l * v1;
       pn.A2[r] -= l * v2;
     }
     if (live) {
-      pp[c * N + lane] = v0;
+      pp[c * N + lane] = (lane == c) ? inv : v0;  // diagonal stored as its reciprocal
       pp[N * N + c * N + lane] = v1;
     }
     if (lane == 0) pp[2 * N * N + c] = v2;
@@ -778,7 +778,7 @@ __device__ __forceinline__ unsigned lu_panel(Panel<NN>& pn, unsigned used, int l
 }
 
 template <int NN>
-__global__ __launch_bounds__(64) void k_boundary(FluxArgs a) {
+__global__ __launch_bounds__(64, 2) void k_boundary(FluxArgs a) {
   constexpr int N = 2 * NN;
   constexpr int R = 3 * NN;
   constexpr EigOff eo(N);
@@ -909,11 +909,11 @@ __global__ __launch_bounds__(64) void k_boundary(FluxArgs a) {
         v -= pp[N * N + k * N + c] * row_bcast<c>(xn);
       });
     }
-    const double dg = pp[k * N + k];
+    const double rdg = pp[k * N + k];  // 1 / U(k,k)
     double xk = 0.0;
     static_for<0, N>([&](auto cj) {
       constexpr int c = N - 1 - decltype(cj)::value;
-      const double xc = row_bcast<c>(v / dg);
+      const double xc = row_bcast<c>(v * rdg);
       xk = (k == c) ? xc : xk;
       v -= pp[k * N + c] * xc;
     });

@@ -219,6 +219,15 @@ extern "C" {
 
 const char* disort_version(void) { return "disort_amd 0.1 (gfx950)"; }
 
+int disort_quadrature(int nstr, double* cmu, double* cwt) {
+  if (nstr < 4 || nstr % 2 != 0 || !cmu || !cwt) return DISORT_E_INVALID;
+  std::vector<double> m, w;
+  if (!gauss_nodes(nstr / 2, m, w)) return DISORT_E_NUMERIC;
+  std::copy(m.begin(), m.end(), cmu);
+  std::copy(w.begin(), w.end(), cwt);
+  return DISORT_OK;
+}
+
 int disort_ctx_create(int device, disort_ctx** out) {
   if (!out) return DISORT_E_INVALID;
   auto ctx = std::make_unique<disort_ctx>();

@@ -1,0 +1,711 @@
+// disortpp.cpp -- DisORT++ class API over the gfx950 C ABI (libdisort_amd.so).
+//
+// Every solve is packed into one structure-of-arrays batch (disort_flux_batch)
+// and handed to disort_flux_solve; a spectral call over B bands is a single
+// batch of B instances instead of the reference's OpenMP band loop
+// (src/python_bindings.cpp:130-211).
+#include "disortpp.hpp"
+
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <mutex>
+
+#include "../../include/disort_amd.h"
+
+namespace disortpp {
+
+// ---------------------------------------------------------------------------
+// Phase functions (src/PhaseFunction.cpp:6-116)
+// ---------------------------------------------------------------------------
+namespace phase_function {
+namespace {
+template <class F>
+void for_layers(int nlyr, int lc, F&& f) {
+  const int lo = lc < 0 ? 0 : lc, hi = lc < 0 ? nlyr - 1 : lc;
+  for (int l = lo; l <= hi; ++l) f(l);
+}
+}  // namespace
+
+void fillHenyeyGreenstein(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, double g, int lc) {
+  for_layers(nlyr, lc, [&](int l) {
+    double gk = 1.0;  // successive products, as the reference forms g^k
+    for (int k = 0; k <= nmom_nstr; ++k, gk *= g) m[l][k] = gk;
+  });
+}
+
+void fillIsotropic(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, int lc) {
+  for_layers(nlyr, lc, [&](int l) {
+    for (int k = 0; k <= nmom_nstr; ++k) m[l][k] = k == 0 ? 1.0 : 0.0;
+  });
+}
+
+void fillRayleigh(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, int lc) {
+  for_layers(nlyr, lc, [&](int l) {
+    for (int k = 0; k <= nmom_nstr; ++k) m[l][k] = k == 0 ? 1.0 : (k == 2 ? 0.1 : 0.0);
+  });
+}
+
+void fillPhaseFunction(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, PhaseFunction type, double g,
+                       int lc) {
+  switch (type) {
+    case PhaseFunction::Isotropic: return fillIsotropic(m, nmom_nstr, nlyr, lc);
+    case PhaseFunction::Rayleigh: return fillRayleigh(m, nmom_nstr, nlyr, lc);
+    case PhaseFunction::HenyeyGreenstein: return fillHenyeyGreenstein(m, nmom_nstr, nlyr, g, lc);
+    case PhaseFunction::HazeGarciaSiewert:
+    case PhaseFunction::CloudGarciaSiewert:
+      throw std::invalid_argument(
+          "Garcia-Siewert tabulated phase functions are not bundled with this library; "
+          "set phase_function_moments directly");
+  }
+  throw std::invalid_argument("unknown phase function type");
+}
+}  // namespace phase_function
+
+namespace {
+
+void check_phase_alloc(const std::vector<std::vector<double>>& m, int nlyr, int nmom_nstr) {
+  if (static_cast<int>(m.size()) < nlyr) throw std::invalid_argument("phase_function_moments not allocated");
+  for (int l = 0; l < nlyr; ++l)
+    if (static_cast<int>(m[l].size()) < nmom_nstr + 1) throw std::invalid_argument("phase_function_moments not allocated");
+}
+
+// ---------------------------------------------------------------------------
+// The GPU behind the facade: one context, calls serialised.
+// ---------------------------------------------------------------------------
+struct Gpu {
+  std::mutex mu;
+  disort_ctx* ctx = nullptr;
+  int device = -1;
+  ~Gpu() {
+    if (ctx) disort_ctx_destroy(ctx);
+  }
+};
+
+Gpu& gpu() {
+  static Gpu g;
+  return g;
+}
+
+int default_device() {
+  const char* e = std::getenv("DISORT_AMD_DEVICE");
+  return e ? std::atoi(e) : 0;
+}
+
+// The per-instance record of one flux problem, in top-down order.
+struct Instance {
+  int nlyr, nstr, pstride;
+  bool reverse;
+  const std::vector<double>* dtau;
+  const std::vector<double>* ssa;
+  const std::vector<std::vector<double>>* pmom;
+  const std::vector<double>* temper;
+  const std::vector<double>* alt;
+  double scal[DISORT_NSCAL];
+  bool pmom0_one;  // DisortSolver sets pmom[0] = 1 (src/DisortSolver.cpp:669)
+};
+
+// Packs instances that share the batch-uniform flags into SoA arrays,
+// applying index_from_bottom by reversal (src/IndexConversion.hpp:40-51):
+// whole vectors are reversed, then the first nlyr (+1) entries are used.
+struct Packed {
+  int32_t flags = 0;
+  std::vector<int32_t> nstr, nlyr, pstride, status;
+  std::vector<int64_t> lay_off, mom_off;
+  std::vector<double> dtau, ssa, pmom, temper, alt, scal, out;
+  int64_t layers = 0, levels = 0, moments = 0;
+
+  void add(const Instance& in) {
+    const int L = in.nlyr, P = in.pstride;
+    nstr.push_back(in.nstr);
+    nlyr.push_back(L);
+    pstride.push_back(P);
+    lay_off.push_back(layers);
+    mom_off.push_back(moments);
+    auto pick = [&](const std::vector<double>& v, int count, std::vector<double>& dst) {
+      const int64_t sz = static_cast<int64_t>(v.size());
+      for (int j = 0; j < count; ++j) dst.push_back(in.reverse ? v[sz - 1 - j] : v[j]);
+    };
+    pick(*in.dtau, L, dtau);
+    pick(*in.ssa, L, ssa);
+    const int64_t rows = static_cast<int64_t>(in.pmom->size());
+    for (int l = 0; l < L; ++l) {
+      const auto& row = (*in.pmom)[in.reverse ? rows - 1 - l : l];
+      for (int k = 0; k < P; ++k) pmom.push_back(k == 0 && in.pmom0_one ? 1.0 : row[k]);
+    }
+    if (flags & DISORT_F_THERMAL) pick(*in.temper, L + 1, temper);
+    if (flags & DISORT_F_SPHERICAL) pick(*in.alt, L + 1, alt);
+    scal.insert(scal.end(), in.scal, in.scal + DISORT_NSCAL);
+    layers += L;
+    levels += L + 1;
+    moments += static_cast<int64_t>(L) * P;
+  }
+
+  // Solves the batch on the GPU; throws the reference's exception types.
+  void solve() {
+    const int64_t n = static_cast<int64_t>(nstr.size());
+    if (n == 0) return;
+    out.assign(static_cast<size_t>(DISORT_NOUT * levels), 0.0);
+    status.assign(static_cast<size_t>(n), 0);
+    disort_flux_batch b{};
+    b.n = n;
+    b.flags = flags;
+    b.nstr = nstr.data();
+    b.nlyr = nlyr.data();
+    b.pstride = pstride.data();
+    b.lay_off = lay_off.data();
+    b.mom_off = mom_off.data();
+    b.dtau = dtau.data();
+    b.ssa = ssa.data();
+    b.pmom = pmom.data();
+    b.temper = (flags & DISORT_F_THERMAL) ? temper.data() : nullptr;
+    b.alt = (flags & DISORT_F_SPHERICAL) ? alt.data() : nullptr;
+    b.scal = scal.data();
+    b.out = out.data();
+    b.total_levels = levels;
+    b.status = status.data();
+
+    Gpu& g = gpu();
+    std::lock_guard<std::mutex> lock(g.mu);
+    if (!g.ctx) {
+      const int dev = g.device >= 0 ? g.device : default_device();
+      if (disort_ctx_create(dev, &g.ctx) != DISORT_OK) {
+        g.ctx = nullptr;
+        throw std::runtime_error("disortpp: cannot create a GPU context on device " + std::to_string(dev));
+      }
+      g.device = dev;
+    }
+    const int rc = disort_flux_solve(g.ctx, &b, DISORT_MEM_HOST);
+    if (rc == DISORT_OK) return;
+    std::string msg = disort_last_error(g.ctx);
+    if (n == 1 && msg.rfind("instance 0: ", 0) == 0) msg = msg.substr(12);
+    if (rc == DISORT_E_INVALID) throw std::invalid_argument(msg);
+    throw std::runtime_error("disortpp: " + msg);
+  }
+
+  // Copies instance i's outputs into the eight result vectors (reversed
+  // when the instance was indexed from the bottom).
+  template <class R>
+  void unpack(int64_t i, bool reverse, R& r) const {
+    const int L = nlyr[i];
+    const int64_t v0 = lay_off[i] + i;
+    std::vector<double>* f[DISORT_NOUT] = {&r.flux_direct_beam, &r.flux_down, &r.flux_up,
+                                           &r.flux_tau_divergence, &r.mean_intensity, &r.mean_intensity_down,
+                                           &r.mean_intensity_up, &r.mean_intensity_direct_beam};
+    for (int q = 0; q < DISORT_NOUT; ++q) {
+      f[q]->resize(L + 1);
+      const double* src = out.data() + q * levels + v0;
+      for (int l = 0; l <= L; ++l) (*f[q])[reverse ? L - l : l] = src[l];
+    }
+  }
+};
+
+int32_t batch_flags(bool thermal, bool spherical, bool dmp, bool diffusion) {
+  return (thermal ? DISORT_F_THERMAL : 0) | (spherical ? DISORT_F_SPHERICAL : 0) |
+         (dmp ? DISORT_F_DELTA_M_PLUS : 0) | (diffusion ? DISORT_F_DIFFUSION_BC : 0);
+}
+
+void fill_scal(double* s, double beam, double mu0, double iso_top, double iso_bot, double t_top, double t_bot,
+               double emis, double albedo, double wlo, double whi, double radius) {
+  for (int k = 0; k < DISORT_NSCAL; ++k) s[k] = 0.0;
+  s[DISORT_S_BEAM_FLUX] = beam;
+  s[DISORT_S_BEAM_MU] = mu0;
+  s[DISORT_S_ISO_TOP] = iso_top;
+  s[DISORT_S_ISO_BOT] = iso_bot;
+  s[DISORT_S_TEMP_TOP] = t_top;
+  s[DISORT_S_TEMP_BOT] = t_bot;
+  s[DISORT_S_EMIS_TOP] = emis;
+  s[DISORT_S_ALBEDO] = albedo;
+  s[DISORT_S_WN_LO] = wlo;
+  s[DISORT_S_WN_HI] = whi;
+  s[DISORT_S_RADIUS] = radius;
+}
+
+Instance flux_instance(const DisortFluxConfig& c) {
+  Instance in{};
+  in.nlyr = c.num_layers;
+  in.nstr = c.num_streams;
+  in.pstride = c.nmomNstr() + 1;
+  in.reverse = c.index_from_bottom;
+  in.dtau = &c.delta_tau;
+  in.ssa = &c.single_scat_albedo;
+  in.pmom = &c.phase_function_moments;
+  in.temper = &c.temperature;
+  in.alt = &c.level_altitudes;
+  in.pmom0_one = false;
+  fill_scal(in.scal, c.direct_beam_flux, c.direct_beam_mu, c.isotropic_flux_top, c.isotropic_flux_bottom,
+            c.temperature_top, c.temperature_bottom, c.emissivity_top, c.surface_albedo, c.wavenumber_low,
+            c.wavenumber_high, c.bottom_radius);
+  return in;
+}
+
+int32_t flux_flags(const DisortFluxConfig& c) {
+  return batch_flags(c.use_thermal_emission, c.use_spherical_beam, c.use_delta_m_plus, c.use_diffusion_lower_bc);
+}
+
+// Validation in the order the reference's solve() runs it: on the
+// top-down view (after reversal), then the GPU-path limits.
+void validate_flux(const DisortFluxConfig& c) {
+  if (c.index_from_bottom) {
+    DisortFluxConfig r = c;
+    std::reverse(r.delta_tau.begin(), r.delta_tau.end());
+    std::reverse(r.single_scat_albedo.begin(), r.single_scat_albedo.end());
+    std::reverse(r.phase_function_moments.begin(), r.phase_function_moments.end());
+    std::reverse(r.temperature.begin(), r.temperature.end());
+    std::reverse(r.level_altitudes.begin(), r.level_altitudes.end());
+    r.validate();
+  } else {
+    c.validate();
+  }
+  if (c.compute_temperature_jacobian)
+    throw std::runtime_error("disortpp: compute_temperature_jacobian is not available on the GPU path");
+}
+
+}  // namespace
+
+void setDevice(int device) {
+  Gpu& g = gpu();
+  std::lock_guard<std::mutex> lock(g.mu);
+  if (g.ctx && g.device != device) {
+    disort_ctx_destroy(g.ctx);
+    g.ctx = nullptr;
+  }
+  g.device = device;
+}
+
+bool isSupportedStreamCount(int n) {
+  return n == 4 || n == 6 || n == 8 || n == 10 || n == 12 || n == 14 || n == 16 || n == 32 || n == 64;
+}
+
+// ---------------------------------------------------------------------------
+// DisortFluxConfig (src/DisortFluxConfig.cpp:9-143)
+// ---------------------------------------------------------------------------
+DisortFluxConfig::DisortFluxConfig(int nlyr, int nstr, int nmom)
+    : num_layers(nlyr), num_streams(nstr), num_phase_func_moments(nmom) {
+  if (num_layers <= 0) throw std::invalid_argument("num_layers must be > 0");
+  if (num_streams < 4 || num_streams % 2 != 0) throw std::invalid_argument("num_streams must be even and >= 4");
+  if (num_phase_func_moments < 0) throw std::invalid_argument("num_phase_func_moments must be >= 0");
+}
+
+void DisortFluxConfig::allocate() {
+  delta_tau.resize(num_layers, 0.0);
+  single_scat_albedo.resize(num_layers, 0.0);
+  phase_function_moments.resize(num_layers);
+  for (auto& row : phase_function_moments) row.resize(nmomNstr() + 1, 0.0);
+  if (use_thermal_emission) temperature.resize(num_layers + 1, 0.0);
+  if (use_spherical_beam) level_altitudes.resize(num_layers + 1, 0.0);
+}
+
+void DisortFluxConfig::validate() const {
+  auto bad = [](const std::string& m) { throw std::invalid_argument(m); };
+  if (num_layers <= 0) bad("num_layers must be > 0");
+  if (num_streams < 4 || num_streams % 2 != 0) bad("num_streams must be even and >= 4");
+  if (num_phase_func_moments < 0) bad("num_phase_func_moments must be >= 0");
+  if (static_cast<int>(delta_tau.size()) < num_layers)
+    bad("delta_tau size (" + std::to_string(delta_tau.size()) + ") < num_layers (" + std::to_string(num_layers) + ")");
+  if (static_cast<int>(single_scat_albedo.size()) < num_layers) bad("single_scat_albedo size < num_layers");
+  if (static_cast<int>(phase_function_moments.size()) < num_layers) bad("phase_function_moments size < num_layers");
+  for (int lc = 0; lc < num_layers; ++lc)
+    if (static_cast<int>(phase_function_moments[lc].size()) < nmomNstr() + 1)
+      bad("phase_function_moments[" + std::to_string(lc) + "] size < nmomNstr()+1");
+  for (int lc = 0; lc < num_layers; ++lc) {
+    if (delta_tau[lc] < 0.0) bad("delta_tau[" + std::to_string(lc) + "] must be >= 0");
+    if (single_scat_albedo[lc] < 0.0 || single_scat_albedo[lc] > 1.0)
+      bad("single_scat_albedo[" + std::to_string(lc) + "] must be in [0, 1]");
+  }
+  if (use_diffusion_lower_bc && !use_thermal_emission) bad("use_diffusion_lower_bc requires use_thermal_emission = true");
+  if (direct_beam_flux < 0.0) bad("direct_beam_flux must be >= 0");
+  if (direct_beam_flux > 0.0 && (direct_beam_mu <= 0.0 || direct_beam_mu > 1.0))
+    bad("direct_beam_mu must be in (0, 1] when direct_beam_flux > 0");
+  if (surface_albedo < 0.0 || surface_albedo > 1.0) bad("surface_albedo must be in [0, 1]");
+  if (use_thermal_emission) {
+    if (static_cast<int>(temperature.size()) < num_layers + 1) bad("temperature size < num_layers + 1");
+    for (int l = 0; l <= num_layers; ++l)
+      if (temperature[l] < 0.0) bad("temperature[" + std::to_string(l) + "] must be >= 0");
+    if (wavenumber_low < 0.0 || wavenumber_low > wavenumber_high)
+      bad("wavenumber: need 0 <= wavenumber_low <= wavenumber_high");
+    if (temperature_top < 0.0 || temperature_bottom < 0.0) bad("temperature_top and temperature_bottom must be >= 0");
+    if (emissivity_top < 0.0 || emissivity_top > 1.0) bad("emissivity_top must be in [0, 1]");
+  }
+  if (use_spherical_beam) {
+    if (bottom_radius <= 0.0) bad("bottom_radius must be > 0 when use_spherical_beam is true");
+    if (static_cast<int>(level_altitudes.size()) < num_layers + 1) bad("level_altitudes size < num_layers + 1");
+  }
+  if (use_delta_m_plus && num_phase_func_moments < num_streams + 1)
+    bad("use_delta_m_plus requires num_phase_func_moments >= num_streams + 1");
+}
+
+void DisortFluxConfig::setHenyeyGreenstein(double g, int lc) {
+  check_phase_alloc(phase_function_moments, num_layers, nmomNstr());
+  phase_function::fillHenyeyGreenstein(phase_function_moments, nmomNstr(), num_layers, g, lc);
+}
+void DisortFluxConfig::setIsotropic(int lc) {
+  check_phase_alloc(phase_function_moments, num_layers, nmomNstr());
+  phase_function::fillIsotropic(phase_function_moments, nmomNstr(), num_layers, lc);
+}
+void DisortFluxConfig::setRayleigh(int lc) {
+  check_phase_alloc(phase_function_moments, num_layers, nmomNstr());
+  phase_function::fillRayleigh(phase_function_moments, nmomNstr(), num_layers, lc);
+}
+void DisortFluxConfig::setHazeGarciaSiewert(int lc) { setPhaseFunction(PhaseFunction::HazeGarciaSiewert, 0.0, lc); }
+void DisortFluxConfig::setCloudGarciaSiewert(int lc) { setPhaseFunction(PhaseFunction::CloudGarciaSiewert, 0.0, lc); }
+void DisortFluxConfig::setPhaseFunction(PhaseFunction type, double g, int lc) {
+  check_phase_alloc(phase_function_moments, num_layers, nmomNstr());
+  phase_function::fillPhaseFunction(phase_function_moments, nmomNstr(), num_layers, type, g, lc);
+}
+
+void FluxResult::allocate(int num_layers) {
+  for (auto* v : {&flux_direct_beam, &flux_down, &flux_up, &flux_tau_divergence, &mean_intensity,
+                  &mean_intensity_down, &mean_intensity_up, &mean_intensity_direct_beam})
+    v->assign(num_layers + 1, 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// Flux solves
+// ---------------------------------------------------------------------------
+std::vector<FluxResult> solveFluxBatch(const std::vector<DisortFluxConfig>& configs) {
+  std::vector<FluxResult> results(configs.size());
+  for (const auto& c : configs) validate_flux(c);
+  // One GPU batch per distinct set of batch-uniform flags.
+  std::vector<int32_t> seen;
+  for (const auto& c : configs)
+    if (std::find(seen.begin(), seen.end(), flux_flags(c)) == seen.end()) seen.push_back(flux_flags(c));
+  for (int32_t fl : seen) {
+    Packed p;
+    p.flags = fl;
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < configs.size(); ++i)
+      if (flux_flags(configs[i]) == fl) {
+        p.add(flux_instance(configs[i]));
+        idx.push_back(i);
+      }
+    p.solve();
+    for (size_t j = 0; j < idx.size(); ++j) p.unpack(static_cast<int64_t>(j), configs[idx[j]].index_from_bottom, results[idx[j]]);
+  }
+  return results;
+}
+
+template <int NStr>
+FluxResult DisortFluxSolver<NStr>::solve(const DisortFluxConfig& config) {
+  validate_flux(config);
+  if (config.num_streams != NStr)
+    throw std::invalid_argument("DisortFluxSolver<" + std::to_string(NStr) + ">::solve: config.num_streams=" +
+                                std::to_string(config.num_streams) + " does not match template NStr");
+  Packed p;
+  p.flags = flux_flags(config);
+  p.add(flux_instance(config));
+  p.solve();
+  FluxResult r;
+  p.unpack(0, config.index_from_bottom, r);
+  return r;
+}
+
+template class DisortFluxSolver<4>;
+template class DisortFluxSolver<6>;
+template class DisortFluxSolver<8>;
+template class DisortFluxSolver<10>;
+template class DisortFluxSolver<12>;
+template class DisortFluxSolver<14>;
+template class DisortFluxSolver<16>;
+template class DisortFluxSolver<32>;
+template class DisortFluxSolver<64>;
+
+namespace {
+void unsupported_streams(int nstr) {
+  throw std::invalid_argument("Unsupported num_streams=" + std::to_string(nstr) +
+                              ". Supported values: 4, 6, 8, 10, 12, 14, 16, 32, 64");
+}
+
+// The per-band overrides of applyOverrides (src/python_bindings.cpp:73-82).
+template <class Cfg, class Set>
+void apply_overrides(Cfg& c, const SpectralOverrides& ov, size_t i, Set&& set_scalars) {
+  if (!ov.delta_tau.empty()) c.delta_tau = ov.delta_tau.at(i);
+  if (!ov.single_scat_albedo.empty()) c.single_scat_albedo = ov.single_scat_albedo.at(i);
+  if (!ov.phase_function_moments.empty()) c.phase_function_moments = ov.phase_function_moments.at(i);
+  set_scalars(c);
+}
+}  // namespace
+
+std::vector<FluxResult> solveFluxSpectralBands(DisortFluxConfig& config,
+                                               const std::vector<std::pair<double, double>>& bands,
+                                               const SpectralOverrides& ov) {
+  if (!isSupportedStreamCount(config.num_streams)) unsupported_streams(config.num_streams);
+  std::vector<DisortFluxConfig> cfgs(bands.size(), config);
+  for (size_t i = 0; i < bands.size(); ++i) {
+    DisortFluxConfig& c = cfgs[i];
+    c.wavenumber_low = bands[i].first;
+    c.wavenumber_high = bands[i].second;
+    apply_overrides(c, ov, i, [&](DisortFluxConfig& x) {
+      if (!ov.surface_albedo.empty()) x.surface_albedo = ov.surface_albedo.at(i);
+      if (!ov.emissivity_top.empty()) x.emissivity_top = ov.emissivity_top.at(i);
+      if (!ov.direct_beam_flux.empty()) x.direct_beam_flux = ov.direct_beam_flux.at(i);
+      if (!ov.isotropic_flux_top.empty()) x.isotropic_flux_top = ov.isotropic_flux_top.at(i);
+      if (!ov.isotropic_flux_bottom.empty()) x.isotropic_flux_bottom = ov.isotropic_flux_bottom.at(i);
+    });
+  }
+  return solveFluxBatch(cfgs);
+}
+
+std::vector<FluxResult> solveFluxSpectral(DisortFluxConfig& config, const std::vector<double>& wavenumbers,
+                                          const SpectralOverrides& ov) {
+  std::vector<std::pair<double, double>> bands;
+  bands.reserve(wavenumbers.size());
+  for (double w : wavenumbers) bands.emplace_back(w, w);
+  return solveFluxSpectralBands(config, bands, ov);
+}
+
+// ---------------------------------------------------------------------------
+// DisortConfig (src/DisortConfig.cpp:9-249)
+// ---------------------------------------------------------------------------
+DisortConfig::DisortConfig(int nlyr, int nstr, int nmom) : num_layers(nlyr), num_phase_func_moments(nmom), num_streams(nstr) {
+  if (num_layers <= 0) throw std::invalid_argument("num_layers must be positive");
+  if (num_streams < 4 || num_streams % 2 != 0) throw std::invalid_argument("num_streams must be even and >= 4");
+  if (num_phase_func_moments < 0) throw std::invalid_argument("num_phase_func_moments must be non-negative");
+}
+
+void DisortConfig::allocate() {
+  delta_tau.resize(num_layers, 0.0);
+  single_scat_albedo.resize(num_layers, 0.0);
+  phase_function_moments.resize(num_layers);
+  for (auto& row : phase_function_moments) row.resize(nmomNstr() + 1, 0.0);
+  if (flags.use_thermal_emission) temperature.resize(num_layers + 1, 0.0);
+  if (num_user_tau > 0) tau_user.resize(num_user_tau, 0.0);
+  if (num_user_mu > 0) mu_user.resize(num_user_mu, 0.0);
+  if (!flags.comp_only_fluxes && num_phi > 0) phi_user.resize(num_phi, 0.0);
+  if (flags.use_spherical_beam) {
+    if (bottom_radius <= 0.0) throw std::invalid_argument("bottom_radius must be positive when use_spherical_beam=true");
+    level_altitudes.resize(num_layers + 1, 0.0);
+  }
+  if (flags.intensity_corr_buras && !flags.intensity_corr_nakajima && num_phase_func_angles > 0) {
+    mu_phase_function.resize(num_phase_func_angles, 0.0);
+    phase_function.assign(num_layers, std::vector<double>(num_phase_func_angles, 0.0));
+  }
+}
+
+void DisortConfig::validate() const {
+  auto bad = [](const std::string& m) { throw std::invalid_argument(m); };
+  if (num_layers <= 0) bad("num_layers must be positive");
+  if (num_streams < 4) bad("num_streams must be >= 4");
+  if (num_streams % 2 != 0) bad("num_streams must be even");
+  if (num_phase_func_moments < 0) bad("num_phase_func_moments must be non-negative");
+  if (delta_tau.size() < static_cast<size_t>(num_layers)) bad("delta_tau array is too small");
+  if (single_scat_albedo.size() < static_cast<size_t>(num_layers)) bad("single_scat_albedo array is too small");
+  if (static_cast<int>(phase_function_moments.size()) < num_layers)
+    bad("phase_function_moments must have at least num_layers layers");
+  for (int lc = 0; lc < num_layers; ++lc)
+    if (static_cast<int>(phase_function_moments[lc].size()) < nmomNstr() + 1)
+      bad("phase_function_moments[" + std::to_string(lc) + "] must have at least nmomNstr()+1 elements");
+  for (int lc = 0; lc < num_layers; ++lc) {
+    if (delta_tau[lc] < 0.0) bad("delta_tau[" + std::to_string(lc) + "] is negative");
+    if (single_scat_albedo[lc] < 0.0 || single_scat_albedo[lc] > 1.0)
+      bad("single_scat_albedo[" + std::to_string(lc) + "] must be in [0, 1]");
+  }
+  if (flags.use_thermal_emission) {
+    if (temperature.size() < static_cast<size_t>(num_layers + 1))
+      bad("temperature array is too small (need num_layers+1 when use_thermal_emission=true)");
+    for (int l = 0; l <= num_layers; ++l)
+      if (temperature[l] < 0.0) bad("temperature[" + std::to_string(l) + "] is negative");
+    if (wavenumber_low < 0.0 || wavenumber_high < wavenumber_low) bad("Invalid wavenumber range for Planck function");
+  }
+  if (flags.use_diffusion_lower_bc) {
+    if (!flags.use_thermal_emission) bad("use_diffusion_lower_bc requires use_thermal_emission = true");
+    if (!flags.use_lambertian_surface)
+      bad("use_diffusion_lower_bc requires use_lambertian_surface = true (BRDF is incompatible)");
+  }
+  if (bc.direct_beam_flux < 0.0) bad("direct_beam_flux must be non-negative");
+  if ((bc.direct_beam_mu <= 0.0 || bc.direct_beam_mu > 1.0) && bc.direct_beam_flux > 0.0)
+    bad("direct_beam_mu must be in (0, 1] when direct_beam_flux > 0");
+  if (flags.use_lambertian_surface && (bc.surface_albedo < 0.0 || bc.surface_albedo > 1.0))
+    bad("surface_albedo must be in [0, 1] when use_lambertian_surface=true");
+  if (flags.use_user_mu) {
+    if (num_user_mu <= 0) bad("num_user_mu must be positive when use_user_mu=true");
+    if (mu_user.size() < static_cast<size_t>(num_user_mu)) bad("mu_user array is too small");
+    for (int iu = 0; iu < num_user_mu; ++iu) {
+      if (std::abs(mu_user[iu]) > 1.0) bad("mu_user[" + std::to_string(iu) + "] out of range [-1, 1]");
+      if (iu > 0 && mu_user[iu] <= mu_user[iu - 1]) bad("mu_user must be in increasing order");
+    }
+  }
+  if (flags.use_spherical_beam) {
+    if (bottom_radius <= 0.0) bad("bottom_radius must be positive when use_spherical_beam=true");
+    if (level_altitudes.size() < static_cast<size_t>(num_layers + 1))
+      bad("level_altitudes array is too small (need num_layers+1 when use_spherical_beam=true)");
+  }
+  if (!flags.comp_only_fluxes && flags.ibcnd != BoundaryConditionType::Special) {
+    if (num_phi <= 0) bad("num_phi must be positive when comp_only_fluxes=false");
+    if (phi_user.size() < static_cast<size_t>(num_phi)) bad("phi_user array is too small");
+    for (int j = 0; j < num_phi; ++j)
+      if (phi_user[j] < 0.0 || phi_user[j] > 360.0) bad("phi_user[" + std::to_string(j) + "] must be in [0, 360]");
+  }
+  if (flags.use_delta_m_plus && num_phase_func_moments < num_streams + 1)
+    bad("use_delta_m_plus=true requires num_phase_func_moments >= num_streams + 1 (need moments M and M+1)");
+  if (accuracy_fourier_series < 0.0) bad("accuracy_fourier_series must be non-negative (0 = disabled)");
+  if (flags.use_thermal_emission) {
+    if (wavenumber_low < 0.0 || wavenumber_high <= wavenumber_low)
+      bad("use_thermal_emission=true requires 0 <= wavenumber_low < wavenumber_high");
+    if (bc.temperature_top < 0.0 || bc.temperature_bottom < 0.0)
+      bad("use_thermal_emission=true requires non-negative temperature_top and temperature_bottom");
+    if (bc.emissivity_top < 0.0 || bc.emissivity_top > 1.0)
+      bad("use_thermal_emission=true requires 0 <= emissivity_top <= 1");
+    if (static_cast<int>(temperature.size()) < num_layers + 1)
+      bad("use_thermal_emission=true requires temperature array of size num_layers+1");
+  }
+}
+
+void DisortConfig::setHenyeyGreenstein(double g, int lc) { setPhaseFunction(PhaseFunction::HenyeyGreenstein, g, lc); }
+void DisortConfig::setIsotropic(int lc) { setPhaseFunction(PhaseFunction::Isotropic, 0.0, lc); }
+void DisortConfig::setRayleigh(int lc) { setPhaseFunction(PhaseFunction::Rayleigh, 0.0, lc); }
+void DisortConfig::setHazeGarciaSiewert(int lc) { setPhaseFunction(PhaseFunction::HazeGarciaSiewert, 0.0, lc); }
+void DisortConfig::setCloudGarciaSiewert(int lc) { setPhaseFunction(PhaseFunction::CloudGarciaSiewert, 0.0, lc); }
+void DisortConfig::setPhaseFunction(PhaseFunction type, double g, int lc) {
+  check_phase_alloc(phase_function_moments, num_layers, nmomNstr());
+  phase_function::fillPhaseFunction(phase_function_moments, nmomNstr(), num_layers, type, g, lc);
+}
+
+// ---------------------------------------------------------------------------
+// DisortSolver (src/DisortSolver.cpp:21-399): the GPU path covers the
+// flux-only, Lambertian, general-BC problem at the layer boundaries.
+// ---------------------------------------------------------------------------
+namespace {
+
+void reverse_general_inputs(DisortConfig& c) {
+  std::reverse(c.delta_tau.begin(), c.delta_tau.end());
+  std::reverse(c.single_scat_albedo.begin(), c.single_scat_albedo.end());
+  std::reverse(c.phase_function_moments.begin(), c.phase_function_moments.end());
+  std::reverse(c.temperature.begin(), c.temperature.end());
+  std::reverse(c.level_altitudes.begin(), c.level_altitudes.end());
+}
+
+void require_gpu_subset(const DisortConfig& c) {
+  auto no = [](const std::string& what) {
+    throw std::runtime_error("disortpp: " + what + " is not available on the GPU path (flux-only, Lambertian, "
+                             "general boundary conditions at the layer boundaries)");
+  };
+  if (!c.flags.comp_only_fluxes) no("intensity output (comp_only_fluxes=false)");
+  if (!c.flags.use_lambertian_surface) no("a BRDF surface");
+  if (c.flags.ibcnd != BoundaryConditionType::General) no("SPECIAL_BC");
+  if (c.flags.use_user_tau) no("use_user_tau");
+  if (c.flags.compute_temperature_jacobian) no("compute_temperature_jacobian");
+}
+
+// Everything DisortSolver::solve does to the config before the solve
+// proper (src/DisortSolver.cpp:26-66, :641-669, :842-850); on return the
+// config holds what the reference leaves in it.
+void prepare_general(DisortConfig& c, std::vector<double>& cmu) {
+  if (c.delta_tau.empty()) c.allocate();
+  const bool rev = c.flags.index_from_bottom;
+  if (rev) reverse_general_inputs(c);
+  c.validate();
+  require_gpu_subset(c);
+  if (!isSupportedStreamCount(c.num_streams)) unsupported_streams(c.num_streams);
+  const int L = c.num_layers, N = c.num_streams, NN = N / 2;
+  if (!c.flags.use_user_tau) {
+    c.num_user_tau = L + 1;
+    c.tau_user.resize(L + 1);
+    double t = 0.0;
+    c.tau_user[0] = 0.0;
+    for (int l = 0; l < L; ++l) c.tau_user[l + 1] = (t += c.delta_tau[l]);
+  }
+  if (!c.flags.use_user_mu || c.flags.comp_only_fluxes) {
+    c.num_user_mu = N;
+    c.mu_user.resize(N);
+    std::vector<double> wt(N);
+    cmu.resize(N);
+    if (disort_quadrature(N, cmu.data(), wt.data()) != DISORT_OK)
+      throw std::runtime_error("gaussianQuadrature: Failed to converge");
+    for (int iu = 0; iu < NN; ++iu) c.mu_user[iu] = -cmu[NN - 1 - iu];
+    for (int iu = NN; iu < N; ++iu) c.mu_user[iu] = cmu[iu - NN];
+  }
+  for (int l = 0; l < L; ++l) c.phase_function_moments[l][0] = 1.0;
+  if (rev) reverse_general_inputs(c);
+  // tau_user is an output grid; with index_from_bottom the reference keeps
+  // it top-down (it is not an input array), so it is not reversed here.
+}
+
+Instance general_instance(const DisortConfig& c) {
+  Instance in{};
+  in.nlyr = c.num_layers;
+  in.nstr = c.num_streams;
+  in.pstride = c.nmomNstr() + 1;
+  in.reverse = c.flags.index_from_bottom;
+  in.dtau = &c.delta_tau;
+  in.ssa = &c.single_scat_albedo;
+  in.pmom = &c.phase_function_moments;
+  in.temper = &c.temperature;
+  in.alt = &c.level_altitudes;
+  in.pmom0_one = true;
+  fill_scal(in.scal, c.bc.direct_beam_flux, c.bc.direct_beam_mu, c.bc.isotropic_flux_top, c.bc.isotropic_flux_bottom,
+            c.bc.temperature_top, c.bc.temperature_bottom, c.bc.emissivity_top, c.bc.surface_albedo, c.wavenumber_low,
+            c.wavenumber_high, c.bottom_radius);
+  return in;
+}
+
+int32_t general_flags(const DisortConfig& c) {
+  return batch_flags(c.flags.use_thermal_emission, c.flags.use_spherical_beam, c.flags.use_delta_m_plus,
+                     c.flags.use_diffusion_lower_bc);
+}
+
+DisortResult to_result(const Packed& p, int64_t i, const DisortConfig& c) {
+  DisortResult r;
+  p.unpack(i, c.flags.index_from_bottom, r);
+  r.mu_angles.assign(c.mu_user.begin(), c.mu_user.begin() + c.num_user_mu);
+  return r;
+}
+
+}  // namespace
+
+DisortResult DisortSolver::solve(DisortConfig& config) {
+  std::vector<double> cmu;
+  prepare_general(config, cmu);
+  Packed p;
+  p.flags = general_flags(config);
+  p.add(general_instance(config));
+  p.solve();
+  return to_result(p, 0, config);
+}
+
+std::vector<DisortResult> solveSpectralBands(DisortConfig config,
+                                             const std::vector<std::pair<double, double>>& bands,
+                                             const SpectralOverrides& ov) {
+  std::vector<DisortConfig> cfgs(bands.size(), config);
+  std::vector<double> cmu;
+  for (size_t i = 0; i < bands.size(); ++i) {
+    DisortConfig& c = cfgs[i];
+    c.wavenumber_low = bands[i].first;
+    c.wavenumber_high = bands[i].second;
+    apply_overrides(c, ov, i, [&](DisortConfig& x) {
+      if (!ov.surface_albedo.empty()) x.bc.surface_albedo = ov.surface_albedo.at(i);
+      if (!ov.emissivity_top.empty()) x.bc.emissivity_top = ov.emissivity_top.at(i);
+      if (!ov.direct_beam_flux.empty()) x.bc.direct_beam_flux = ov.direct_beam_flux.at(i);
+      if (!ov.isotropic_flux_top.empty()) x.bc.isotropic_flux_top = ov.isotropic_flux_top.at(i);
+      if (!ov.isotropic_flux_bottom.empty()) x.bc.isotropic_flux_bottom = ov.isotropic_flux_bottom.at(i);
+    });
+    prepare_general(c, cmu);
+  }
+  std::vector<DisortResult> results(bands.size());
+  std::vector<int32_t> seen;
+  for (const auto& c : cfgs)
+    if (std::find(seen.begin(), seen.end(), general_flags(c)) == seen.end()) seen.push_back(general_flags(c));
+  for (int32_t fl : seen) {
+    Packed p;
+    p.flags = fl;
+    std::vector<size_t> idx;
+    for (size_t i = 0; i < cfgs.size(); ++i)
+      if (general_flags(cfgs[i]) == fl) {
+        p.add(general_instance(cfgs[i]));
+        idx.push_back(i);
+      }
+    p.solve();
+    for (size_t j = 0; j < idx.size(); ++j) results[idx[j]] = to_result(p, static_cast<int64_t>(j), cfgs[idx[j]]);
+  }
+  return results;
+}
+
+std::vector<DisortResult> solveSpectral(DisortConfig config, const std::vector<double>& wavenumbers,
+                                        const SpectralOverrides& ov) {
+  std::vector<std::pair<double, double>> bands;
+  bands.reserve(wavenumbers.size());
+  for (double w : wavenumbers) bands.emplace_back(w, w);
+  return solveSpectralBands(std::move(config), bands, ov);
+}
+
+}  // namespace disortpp

@@ -4,7 +4,7 @@
  * The reference (DisORT++, NewStrangeWorlds/DisORT) has no C ABI: its hot path
  * is reached through C++ classes and a pybind11 module.  Each entry point
  * below names the reference interface it replaces; the C++ facade in
- * disort_amd/csrc/host (DisortFluxSolver<N>, solveFluxSpectralBands,
+ * disort_amd/host (DisortFluxSolver<N>, DisortSolver, solveFluxSpectralBands,
  * solveSpectralBands) and the pybind11 module `disortpp` sit on top of it.
  *
  * Conventions: plain pointers and sizes, no exceptions cross the boundary,
@@ -170,6 +170,14 @@ int disort_stage_times(disort_ctx* ctx, double* ms, int max_stages);
 
 /* Library version string. */
 const char* disort_version(void);
+
+/*
+ * Double-Gauss computational angles and weights for nstr streams, in the
+ * layout of gaussianQuadrature (src/FluxSolver.cpp:257-290): cmu[0..nstr/2)
+ * the Gauss-Legendre nodes on (0,1), cmu[nstr/2+i] = -cmu[i], weights
+ * mirrored.  Host-only; needs no context or GPU.
+ */
+int disort_quadrature(int nstr, double* cmu, double* cwt);
 
 #ifdef __cplusplus
 }

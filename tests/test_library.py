@@ -1,0 +1,72 @@
+"""The built libraries load and export what include/disort_amd.h declares,
+and the host-only entry points behave (no GPU is touched here)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "disort_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(disort_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_the_abi():
+    names = declared_functions()
+    for expected in ("disort_ctx_create", "disort_flux_solve", "disort_flux_plan_create", "disort_flux_execute",
+                     "disort_quadrature", "disort_version"):
+        assert expected in names
+
+
+def test_library_exports_every_declared_symbol():
+    from disort_amd import _capi
+
+    lib = C.CDLL(_capi.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"libdisort_amd.so lacks {missing}"
+    assert _capi.version().startswith("disort_amd")
+
+
+def test_quadrature_matches_gauss_legendre():
+    from disort_amd import _capi
+
+    lib = C.CDLL(_capi.LIB_PATH)
+    lib.disort_quadrature.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+    for nstr in (4, 6, 8, 16, 32, 64):
+        mu = np.zeros(nstr)
+        wt = np.zeros(nstr)
+        assert lib.disort_quadrature(nstr, mu.ctypes.data, wt.ctypes.data) == 0
+        x, w = np.polynomial.legendre.leggauss(nstr // 2)
+        order = np.argsort(-x)  # nodes descend in cos, as the Newton loop finds them
+        np.testing.assert_allclose(mu[: nstr // 2], 0.5 * (1 + x[order]), rtol=0, atol=2e-15)
+        np.testing.assert_allclose(wt[: nstr // 2], 0.5 * w[order], rtol=5e-13)
+        np.testing.assert_array_equal(mu[nstr // 2 :], -mu[: nstr // 2])
+        np.testing.assert_array_equal(wt[nstr // 2 :], wt[: nstr // 2])
+    assert lib.disort_quadrature(5, mu.ctypes.data, wt.ctypes.data) == 1
+
+
+def test_facade_module_loads():
+    import disort_amd.disortpp as d
+
+    for n in (4, 6, 8, 10, 12, 14, 16, 32, 64):
+        assert hasattr(d, f"DisortFluxSolver{n}")
+    for name in ("DisortConfig", "DisortFluxConfig", "DisortResult", "FluxResult", "DisortSolver",
+                 "solve_spectral", "solve_spectral_bands", "solve_flux_spectral", "solve_flux_spectral_bands",
+                 "create_flux_solver"):
+        assert hasattr(d, name), name
+    assert d.__version__ == "1.0"
+
+
+def test_ctx_without_gpu_fails_loudly():
+    if os.path.exists("/dev/kfd"):
+        pytest.skip("a GPU driver is present")
+    from disort_amd._capi import Context, DisortError
+
+    with pytest.raises(DisortError):
+        Context(0)

@@ -22,6 +22,7 @@ DISORT_E_UNSUPPORTED = 2
 DISORT_E_NUMERIC = 3
 MEM_HOST = 0
 MEM_DEVICE = 1
+V_DEFAULT, V_LANE, V_WORKGROUP = 0, 1, 2
 
 _lib = None
 
@@ -65,6 +66,9 @@ def lib():
         l.disort_set_timing.argtypes = [vp, C.c_int]
         l.disort_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.c_int]
         l.disort_version.restype = C.c_char_p
+        l.disort_flux_plan_set_variant.argtypes = [vp, vp, C.c_int, C.c_int]
+        l.disort_flux_plan_get_variant.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
+        l.disort_quadrature.argtypes = [C.c_int, vp, vp]
         _lib = l
     return _lib
 
@@ -185,6 +189,19 @@ class DeviceFluxBatch:
         s.total_levels = int(batch.total_levels)
         s.status = None
         self.cstruct = s
+
+    def set_variant(self, nstr: int, variant: int):
+        """Kernel variant of the bucket with `nstr` streams (V_LANE / V_WORKGROUP)."""
+        _check(self.ctx.ptr, lib().disort_flux_plan_set_variant(self.ctx.ptr, self.plan, nstr, variant))
+
+    def get_variant(self, nstr: int) -> int:
+        v = C.c_int()
+        _check(self.ctx.ptr, lib().disort_flux_plan_get_variant(self.ctx.ptr, self.plan, nstr, C.byref(v)))
+        return v.value
+
+    def buckets(self):
+        """Stream counts present in the batch."""
+        return sorted(set(int(x) for x in self.host.nstr))
 
     def execute(self):
         """Enqueue one solve on the context stream (asynchronous)."""

@@ -21,7 +21,10 @@
 #include "flux_layout.hpp"
 
 namespace dsa {
+// Register kernels (flux_kernels.hip), N in {4,...,16}.
 bool launch_flux_bucket(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev);
+// Workgroup kernels (flux_wide.hip), N in {8, 16, 32, 64}.
+bool launch_flux_bucket_wg(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev);
 }
 
 using dsa::FluxArgs;
@@ -147,7 +150,11 @@ int table_for(disort_ctx* ctx, int nstr, double** out) {
   return DISORT_OK;
 }
 
-bool gpu_supported_nstr(int n) { return n >= 4 && n <= 16 && n % 2 == 0; }
+bool lane_kernels(int n) { return n >= 4 && n <= 16 && n % 2 == 0; }
+bool wg_kernels(int n) { return n == 8 || n == 16 || n == 32 || n == 64; }
+bool gpu_supported_nstr(int n) { return lane_kernels(n) || wg_kernels(n); }
+int default_variant(int n) { return lane_kernels(n) ? DISORT_V_LANE : DISORT_V_WORKGROUP; }
+bool variant_ok(int n, int v) { return (v == DISORT_V_LANE && lane_kernels(n)) || (v == DISORT_V_WORKGROUP && wg_kernels(n)); }
 
 // Host-side validation of one instance, with the reference's messages
 // (src/DisortFluxConfig.cpp:43-143).
@@ -192,6 +199,7 @@ std::string validate_instance(const disort_flux_batch* b, int64_t i) {
 // ---------------------------------------------------------------------------
 struct Bucket {
   int nstr = 0;
+  int variant = 0;
   int64_t nb = 0, n_tasks = 0, n_lv = 0;
   int32_t* d_inst = nullptr;
   int64_t* d_task0 = nullptr;
@@ -304,6 +312,30 @@ int disort_flux_plan_destroy(disort_ctx* ctx, disort_flux_plan* plan) {
   return DISORT_OK;
 }
 
+int disort_flux_plan_set_variant(disort_ctx* ctx, disort_flux_plan* plan, int nstr, int variant) {
+  if (!plan) return fail(ctx, DISORT_E_INVALID, "null plan");
+  for (Bucket& bk : plan->buckets)
+    if (bk.nstr == nstr) {
+      if (variant == DISORT_V_DEFAULT) variant = default_variant(nstr);
+      if (!variant_ok(nstr, variant))
+        return fail(ctx, DISORT_E_UNSUPPORTED,
+                    "kernel variant " + std::to_string(variant) + " has no kernels for num_streams=" + std::to_string(nstr));
+      bk.variant = variant;
+      return DISORT_OK;
+    }
+  return fail(ctx, DISORT_E_INVALID, "the plan has no bucket with num_streams=" + std::to_string(nstr));
+}
+
+int disort_flux_plan_get_variant(disort_ctx* ctx, disort_flux_plan* plan, int nstr, int* variant) {
+  if (!plan || !variant) return fail(ctx, DISORT_E_INVALID, "null argument");
+  for (Bucket& bk : plan->buckets)
+    if (bk.nstr == nstr) {
+      *variant = bk.variant;
+      return DISORT_OK;
+    }
+  return fail(ctx, DISORT_E_INVALID, "the plan has no bucket with num_streams=" + std::to_string(nstr));
+}
+
 int disort_flux_plan_create(disort_ctx* ctx, int64_t n, const int32_t* nstr, const int32_t* nlyr,
                             const int32_t* pstride, int32_t flags, disort_flux_plan** out) {
   if (!ctx || !out || n <= 0 || !nstr || !nlyr || !pstride) return fail(ctx, DISORT_E_INVALID, "invalid plan arguments");
@@ -324,7 +356,7 @@ int disort_flux_plan_create(disort_ctx* ctx, int64_t n, const int32_t* nstr, con
     if (!gpu_supported_nstr(nstr[i]))
       return fail(ctx, DISORT_E_UNSUPPORTED,
                   "instance " + std::to_string(i) + ": num_streams=" + std::to_string(nstr[i]) +
-                      " is not yet supported by the GPU flux path (4..16)");
+                      " is not supported by the GPU flux path (4, 6, ..., 16, 32, 64)");
     if (pstride[i] < nstr[i] + 1 + ((flags & DISORT_F_DELTA_M_PLUS) ? 1 : 0))
       return fail(ctx, DISORT_E_INVALID, "instance " + std::to_string(i) + ": phase moment rows too short");
     plan->lay_off[i] = lay;
@@ -350,6 +382,7 @@ int disort_flux_plan_create(disort_ctx* ctx, int64_t n, const int32_t* nstr, con
   for (auto& kv : groups) {
     Bucket bk;
     bk.nstr = kv.first;
+    bk.variant = default_variant(bk.nstr);
     const auto& inst = kv.second;
     bk.nb = (int64_t)inst.size();
     std::vector<int64_t> task0(bk.nb);
@@ -436,7 +469,10 @@ int disort_flux_execute(disort_ctx* ctx, disort_flux_plan* plan, const disort_fl
     a.n_lvtasks = bk.n_lv;
     a.v_bk = bk.d_v_bk;
     a.v_lu = bk.d_v_lu;
-    if (!dsa::launch_flux_bucket(bk.nstr, a, ctx->stream, ctx->timing ? ctx->ev : nullptr))
+    hipEvent_t* ev = ctx->timing ? ctx->ev : nullptr;
+    const bool launched = bk.variant == DISORT_V_LANE ? dsa::launch_flux_bucket(bk.nstr, a, ctx->stream, ev)
+                                                      : dsa::launch_flux_bucket_wg(bk.nstr, a, ctx->stream, ev);
+    if (!launched)
       return fail(ctx, DISORT_E_UNSUPPORTED, "no kernels for num_streams=" + std::to_string(bk.nstr));
     HIPCHK(ctx, hipGetLastError());
     if (ctx->timing) {

@@ -1,242 +1,30 @@
-// gfx950 kernels of the batched flux-only DISORT solve.
+// gfx950 kernels of the batched flux-only DISORT solve for N <= 16 streams,
+// with the whole per-lane working set in registers.
 //
 // One instance = one DisortFluxSolver<N>::solve (src/FluxSolver.cpp:23-171).
 // The solve is split into four launches over a bucket of instances that share
 // the stream count N = 2*NN:
-//   k_column  one wavefront per instance: Delta-M / Delta-M+ scaling, scaled
-//             and unscaled optical-depth prefix sums, absorption cut-off,
-//             beam transmission, Planck function per level
+//   k_column  (flux_common.hpp) one wavefront per instance: Delta-M /
+//             Delta-M+ scaling, optical-depth prefix sums, absorption
+//             cut-off, beam transmission, Planck function per level
 //             (src/FluxSolver.cpp:309-452, :104-122; src/Planck.cpp:35-169).
 //   k_layer   one lane per (instance, layer): the reduced NN x NN eigenproblem
 //             (src/FluxSolver.cpp:459-588) solved in registers by Cholesky
 //             symmetrisation and cyclic Jacobi, then the beam and thermal
 //             particular solutions (:595-643, :745-831) expressed through
-//             that eigen-decomposition instead of N x N LU solves.
+//             that eigen-decomposition and a Cholesky factor of -A_s instead
+//             of N x N LU solves.
 //   k_boundary 16 lanes per instance: the (N*L)-unknown boundary system of
-//             setMatrix/solve0 (:838-1131) eliminated layer by layer with
-//             Householder QR on 3NN x N panels (rows: the NN equations
+//             setMatrix/solve0 (:838-1131) eliminated layer by layer by LU
+//             with partial pivoting on 3NN-row panels (the NN equations
 //             carried from above plus the N continuity equations of the next
 //             interface), then block back substitution.
-//   k_flux    one lane per (instance, level): intensities at the quadrature
-//             angles and the eight flux / mean-intensity outputs (:1138-1287).
-#include <hip/hip_runtime.h>
-
-#include <cmath>
-#include <cstdint>
-
-#include "../../include/disort_amd.h"
-#include "flux_args.hpp"
-#include "flux_layout.hpp"
+//   k_flux    (flux_common.hpp) one lane per (instance, level): intensities
+//             at the quadrature angles and the eight outputs (:1138-1287).
+#include "flux_common.hpp"
 
 namespace dsa {
 namespace {
-
-constexpr double kPi = 3.14159265358979323846;
-constexpr double kEps = 2.220446049250313e-16;     // DBL_EPSILON
-constexpr double kLogDblMax = 709.782712893384;    // log(DBL_MAX)
-
-// ---------------------------------------------------------------------------
-// Band-integrated Planck function (W m^-2 sr^-1 over [lo, hi] cm^-1), the
-// algorithm of src/Planck.cpp:35-169 (Simpson refinement for narrow bands,
-// power series below v = 1.5, exponential series above), with the constants
-// of src/Planck.hpp:10-22.  Arguments are validated on the host.
-// ---------------------------------------------------------------------------
-__device__ inline double planck_shape(double v) { return v * v * v / (exp(v) - 1.0); }
-
-__device__ double planck_band(double lo, double hi, double t) {
-  const double c2 = 1.438786, sigma = 5.67032e-8;
-  if (t < 1.e-4) return 0.0;
-  if (hi == lo) {
-    const double arg = exp(-c2 * hi / t);
-    return 1.1911e-8 * hi * hi * hi * arg / (1.0 - arg);
-  }
-  const double sigdpi = sigma / kPi;
-  const double conc = 15.0 / (kPi * kPi * kPi * kPi);
-  const double t4 = t * t * t * t;
-  const double v0 = c2 * lo / t, v1 = c2 * hi / t;
-  if (v0 > kEps && v1 < kLogDblMax && (hi - lo) / hi < 1.e-2) {
-    const double span = v1 - v0;
-    const double ends = planck_shape(v0) + planck_shape(v1);
-    double prev = 0.0;
-    for (int n = 1; n <= 10; ++n) {
-      const double h = span / (2.0 * n);
-      double acc = ends;
-      for (int k = 1; k <= 2 * n - 1; ++k) acc += 2.0 * (1 + k % 2) * planck_shape(v0 + k * h);
-      acc *= h * (1.0 / 3.0);
-      if (fabs((acc - prev) / acc) <= 1.e-6) return sigdpi * t4 * conc * acc;
-      prev = acc;
-    }
-    return sigdpi * t4 * conc * prev;
-  }
-  const double a1 = 1.0 / 3.0, a2 = -1.0 / 8.0, a3 = 1.0 / 60.0;
-  const double a4 = -1.0 / 5040.0, a5 = 1.0 / 272160.0, a6 = -1.0 / 13305600.0;
-  const double vcp[7] = {10.25, 5.7, 3.9, 2.9, 2.3, 1.9, 0.0};
-  double p[2] = {0.0, 0.0}, d[2] = {0.0, 0.0};
-  const double v[2] = {v0, v1};
-  int nsmall = 0;
-  for (int i = 0; i < 2; ++i) {
-    if (v[i] < 1.5) {
-      ++nsmall;
-      const double vs = v[i] * v[i];
-      p[i] = conc * vs * v[i] * (a1 + v[i] * (a2 + v[i] * (a3 + vs * (a4 + vs * (a5 + vs * a6)))));
-    } else {
-      int terms = 1;
-      while (v[i] < vcp[terms - 1]) ++terms;
-      const double ex = exp(-v[i]);
-      double exm = 1.0, s = 0.0;
-      for (int m = 1; m <= terms; ++m) {
-        const double mv = m * v[i];
-        exm *= ex;
-        s += exm * (6.0 + mv * (6.0 + mv * (3.0 + mv))) / (double)(m * m * m * m);
-      }
-      d[i] = s * conc;
-    }
-  }
-  double ans;
-  if (nsmall == 2) ans = p[1] - p[0];
-  else if (nsmall == 1) ans = 1.0 - p[0] - d[1];
-  else ans = d[0] - d[1];
-  return ans * (sigdpi * t4);
-}
-
-// Broadcast lane `src` of each row of 16 lanes to the whole row (DPP row_newbcast).
-template <int SRC>
-__device__ __forceinline__ double row_bcast(double x) {
-  static_assert(SRC >= 0 && SRC < 16, "row lane");
-  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), 0x150 + SRC, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), 0x150 + SRC, 0xF, 0xF, false);
-  return __hiloint2double(hi, lo);
-}
-
-// Compile-time loop helper: f(integral_constant<int, I>) for I in [B, E).
-template <int B, int E, typename F>
-__device__ __forceinline__ void static_for(F&& f) {
-  if constexpr (B < E) {
-    f(std::integral_constant<int, B>{});
-    static_for<B + 1, E>(f);
-  }
-}
-
-__host__ __device__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }  // i >= j
-
-// ===========================================================================
-// k_column: per-instance set-up (one wavefront per instance).
-// ===========================================================================
-template <int N>
-__global__ __launch_bounds__(64) void k_column(FluxArgs a) {
-  const int bk = blockIdx.x;
-  if (bk >= a.nb) return;
-  const int lane = threadIdx.x;
-  const int i = a.b_inst[bk];
-  const int L = a.nlyr[i];
-  const int P = a.pstride[i];
-  const int64_t lo = a.lay_off[i], vo = lo + i;
-  const double* pm = a.pmom + a.mom_off[i];
-  const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
-  const bool thermal = (a.flags & DISORT_F_THERMAL) != 0;
-  const bool want_dmp = (a.flags & DISORT_F_DELTA_M_PLUS) != 0;
-  const int64_t TL = a.total_layers, TV = a.total_levels;
-
-  // src/FluxSolver.cpp:56-81: delta-M when any pmom[N] != 0; Delta-M+ only if
-  // every layer has pmom[N] >= 1e-4 and pmom[N+1] >= 0.7 pmom[N].
-  int any_dm = 0, dmp_bad = 0;
-  for (int lc = lane; lc < L; lc += 64) {
-    const double pN = pm[(int64_t)lc * P + N];
-    any_dm |= (pN != 0.0);
-    if (want_dmp) {
-      const double pN1 = pm[(int64_t)lc * P + N + 1];
-      dmp_bad |= (pN < 1e-4 || pN1 < 0.7 * pN);
-    }
-  }
-  const bool dmp = want_dmp && !__any(dmp_bad);
-  const bool deltam = !dmp && __any(any_dm);
-
-  // Per-layer scaling (src/FluxSolver.cpp:333-389).
-  for (int lc = lane; lc < L; lc += 64) {
-    const int64_t g = lo + lc;
-    const double w = a.ssa[g], dt = a.dtau[g];
-    double f = 0.0, sig2 = -1.0, ws = w, dts = dt;
-    if (deltam || dmp) {
-      const double pN = pm[(int64_t)lc * P + N];
-      sig2 = 0.0;
-      f = pN;
-      if (dmp && pm[(int64_t)lc * P + N - 1] != pN) {
-        const double pN1 = pm[(int64_t)lc * P + N + 1];
-        if (pN != pN1) {
-          const double M = (double)N;
-          sig2 = ((M + 1.0) * (M + 1.0) - M * M) / (log(pN * pN) - log(pN1 * pN1));
-          f = exp(M * M / (2.0 * sig2)) * pN;
-        }
-      }
-      ws = w * (1.0 - f) / (1.0 - f * w);
-      dts = (1.0 - f * w) * dt;
-    }
-    a.lay_d[LD_DTAU_S * TL + g] = dts;
-    a.lay_d[LD_SSA_S * TL + g] = ws;
-    a.lay_d[LD_F * TL + g] = f;
-    a.lay_d[LD_SIG2 * TL + g] = sig2;
-  }
-  __syncthreads();
-
-  const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
-  if (lane == 0) {
-    // Prefix sums, absorption cut-off and beam transmission (:311-331, :392-435).
-    double taus = 0.0, tauc = 0.0, abst = 0.0;
-    int ncut = L;
-    a.lev_d[VD_TAU_S * TV + vo] = 0.0;
-    a.lev_d[VD_TAUC * TV + vo] = 0.0;
-    for (int lc = 0; lc < L; ++lc) {
-      const int64_t g = lo + lc;
-      if (abst < 10.0) ncut = lc + 1;
-      abst += (1.0 - a.ssa[g]) * a.dtau[g];
-      a.lay_d[LD_TAU_TOP * TL + g] = taus;
-      taus += a.lay_d[LD_DTAU_S * TL + g];
-      tauc += a.dtau[g];
-      a.lev_d[VD_TAU_S * TV + vo + lc + 1] = taus;
-      a.lev_d[VD_TAUC * TV + vo + lc + 1] = tauc;
-    }
-    // Transmission as seen by the boundary conditions: exp(-tau'/mu0) down to
-    // the absorption cut-off level and zero below it, as a freshly constructed
-    // reference solver leaves it (beam_transmission_ is only filled for
-    // lc < ncut before the cut-off decision, src/FluxSolver.cpp:417-421).
-    double bt_prev = (F0 > 0.0) ? 1.0 : 0.0;
-    for (int lc = 0; lc < L; ++lc) {
-      const int64_t g = lo + lc;
-      double bt = 0.0;
-      if (F0 > 0.0 && lc < ncut) bt = exp(-a.lev_d[VD_TAU_S * TV + vo + lc + 1] / mu0);
-      a.lay_d[LD_BT_TOP * TL + g] = bt_prev;
-      a.lay_d[LD_BT_BOT * TL + g] = bt;
-      bt_prev = bt;
-    }
-    const bool lyrcut = (abst >= 10.0 && !thermal && L > 1);
-    const int ncut_f = lyrcut ? ncut : L;
-    a.inst_i[i * kInstI + II_NCUT] = ncut_f;
-    a.inst_i[i * kInstI + II_LYRCUT] = lyrcut ? 1 : 0;
-    a.inst_i[i * kInstI + II_STATUS] = 0;
-    a.inst_d[i * kInstD + ID_BTRANS_NCUT] = (ncut_f == 0) ? bt_prev : a.lay_d[LD_BT_BOT * TL + lo + ncut_f - 1];
-  }
-  // Planck function per level (src/FluxSolver.cpp:104-122, :748-752).
-  if (thermal) {
-    const double wlo = sc[DISORT_S_WN_LO], whi = sc[DISORT_S_WN_HI];
-    for (int l = lane; l <= L; l += 64) a.lev_d[VD_PLANCK * TV + vo + l] = planck_band(wlo, whi, a.temper[vo + l]);
-    if (lane == 0) {
-      a.inst_d[i * kInstD + ID_PL_TOP] = planck_band(wlo, whi, sc[DISORT_S_TEMP_TOP]) * sc[DISORT_S_EMIS_TOP];
-      const double pb = planck_band(wlo, whi, sc[DISORT_S_TEMP_BOT]);
-      a.inst_d[i * kInstD + ID_PL_BOT] = pb;
-      double pbd = 0.0;
-      if (a.flags & DISORT_F_DIFFUSION_BC) {
-        const double bl = planck_band(wlo, whi, a.temper[vo + L - 1]);
-        const double dl = a.lay_d[LD_DTAU_S * TL + lo + L - 1];
-        if (dl > 0.0) pbd = (pb - bl) / dl;
-      }
-      a.inst_d[i * kInstD + ID_PL_BOT_DERIV] = pbd;
-    }
-  } else if (lane == 0) {
-    a.inst_d[i * kInstD + ID_PL_TOP] = 0.0;
-    a.inst_d[i * kInstD + ID_PL_BOT] = 0.0;
-    a.inst_d[i * kInstD + ID_PL_BOT_DERIV] = 0.0;
-  }
-}
 
 // ===========================================================================
 // k_layer: eigen-system and particular solutions, one lane per layer.
@@ -710,20 +498,6 @@ __device__ __forceinline__ int row_bcast_i(int x) {
   return __builtin_amdgcn_mov_dpp(x, 0x150 + SRC, 0xF, 0xF, false);
 }
 
-// Entry G(q, j) of the eigenvector matrix of task t (reference ordering),
-// with a run-time row q.
-template <int NN>
-__device__ __forceinline__ double g_entry(const FluxArgs& a, int64_t t, int q, int j) {
-  constexpr EigOff eo(2 * NN);
-  const bool up = j >= NN;
-  const int jp = up ? j - NN : NN - 1 - j;
-  const bool lower = q >= NN;
-  const int r = lower ? q - NN : NN - 1 - q;
-  // lower & up: gp; upper & up: gm; lower & down: -gm; upper & down: -gp
-  const int f = (lower == up) ? eo.gp() : eo.gm();
-  const double v = a.eig[(f + r + jp * NN) * (int64_t)a.n_tasks + t];
-  return up ? v : -v;
-}
 
 // One panel: N steps of partial-pivot elimination over the candidate slots
 // (those not in `used`).  Pivot rows are written to pp as the U rows of this
@@ -920,101 +694,6 @@ __global__ __launch_bounds__(64, 2) void k_boundary(FluxArgs a) {
     xn = xk;
     if (live) a.cvec[tt * N + k] = xk;
   }
-}
-
-// ===========================================================================
-// k_flux: fluxes and mean intensities at the layer boundaries.
-// ===========================================================================
-template <int NN>
-__global__ __launch_bounds__(64) void k_flux(FluxArgs a) {
-  constexpr int N = 2 * NN;
-  constexpr EigOff eo(N);
-  constexpr TabOff to(N);
-  const int64_t vt = (int64_t)blockIdx.x * 64 + threadIdx.x;
-  if (vt >= a.n_lvtasks) return;
-  const int bk = a.v_bk[vt];
-  const int lu = a.v_lu[vt];
-  const int i = a.b_inst[bk];
-  const int64_t t0 = a.b_task0[bk];
-  const int64_t lo = a.lay_off[i], vo = lo + i;
-  const int64_t T = a.n_tasks, TL = a.total_layers, TV = a.total_levels, OV = a.total_levels;
-  const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
-  const double* tb = a.tab;
-  const int ncut = a.inst_i[i * kInstI + II_NCUT];
-  const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
-  int lyu = (lu == 0) ? 0 : lu - 1;
-  double o[DISORT_NOUT];
-#pragma unroll
-  for (int f = 0; f < DISORT_NOUT; ++f) o[f] = 0.0;
-  if (!(lyrcut && lyu > ncut - 1)) {
-    if (lyu >= ncut) lyu = ncut - 1;
-    const int64_t tt = t0 + lyu;
-    const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
-    const double ut = a.lev_d[VD_TAU_S * TV + vo + lu];
-    const double dts = a.lay_d[LD_DTAU_S * TL + lo + lyu];
-    // Mode amplitudes at this level: C_j e_j.
-    // Intensity at the quadrature angles: I = G (C o e) + Z, accumulated one
-    // mode pair at a time (mode NN-1-jp has eigenvalue -k_jp, mode NN+jp +k_jp).
-    const int zoff = (lu == 0) ? eo.ztop() : eo.zfl();
-    double iu[NN], id[NN];
-#pragma unroll
-    for (int r = 0; r < NN; ++r) {
-      iu[r] = a.eig[(zoff + NN + r) * T + tt];
-      id[r] = a.eig[(zoff + NN - 1 - r) * T + tt];
-    }
-#pragma unroll 1
-    for (int jp = 0; jp < NN; ++jp) {
-      const double e = exp(-a.eig[(eo.k() + jp) * T + tt] * dts);
-      const double yd = a.cvec[tt * N + NN - 1 - jp] * ((lu == 0) ? e : 1.0);
-      const double yu = a.cvec[tt * N + NN + jp] * ((lu == 0) ? 1.0 : e);
-#pragma unroll
-      for (int r = 0; r < NN; ++r) {
-        const double gp = a.eig[(eo.gp() + r + jp * NN) * T + tt];
-        const double gm = a.eig[(eo.gm() + r + jp * NN) * T + tt];
-        iu[r] += gp * yu - gm * yd;
-        id[r] += gm * yu - gp * yd;
-      }
-    }
-    double mi = 0.0, mid = 0.0, miu = 0.0, fldn = 0.0, flup = 0.0;
-#pragma unroll
-    for (int r = 0; r < NN; ++r) {
-      const double w = tb[to.cwt() + r], cm = tb[to.cmu() + r];
-      mi += w * id[r] + w * iu[r];
-      mid += w * id[r];
-      miu += w * iu[r];
-      fldn += w * id[r] * cm;
-      flup += w * iu[r] * cm;
-    }
-    double fact = 0.0, dirint = 0.0, fldir = 0.0, fdir = 0.0;
-    if (F0 > 0.0) {
-      const double tauc = a.lev_d[VD_TAUC * TV + vo + lu];
-      fact = exp(-ut / mu0);
-      fdir = mu0 * F0 * exp(-tauc / mu0);
-      dirint = F0 * fact;
-      fldir = mu0 * F0 * fact;
-    }
-    flup *= 2.0 * kPi;
-    fldn *= 2.0 * kPi;
-    const double fdn = (fldn + fldir) - fdir;
-    mi = (2.0 * kPi * mi + dirint) / (4.0 * kPi);
-    const double midir = dirint / (4.0 * kPi);
-    mid = (2.0 * kPi * mid) / (4.0 * kPi);
-    miu = (2.0 * kPi * miu) / (4.0 * kPi);
-    double pls = 0.0;
-    if (a.flags & DISORT_F_THERMAL)
-      pls = (lu == 0) ? a.lev_d[VD_PLANCK * TV + vo] : a.eig[eo.pls() * T + tt];
-    const double div = (1.0 - a.ssa[lo + lyu]) * 4.0 * kPi * (mi - pls);
-    o[DISORT_O_FLUX_DIRECT_BEAM] = fdir;
-    o[DISORT_O_FLUX_DOWN] = fdn;
-    o[DISORT_O_FLUX_UP] = flup;
-    o[DISORT_O_FLUX_TAU_DIVERGENCE] = div;
-    o[DISORT_O_MEAN_INTENSITY] = mi;
-    o[DISORT_O_MEAN_INTENSITY_DOWN] = mid;
-    o[DISORT_O_MEAN_INTENSITY_UP] = miu;
-    o[DISORT_O_MEAN_INTENSITY_DIRECT_BEAM] = midir;
-  }
-#pragma unroll
-  for (int f = 0; f < DISORT_NOUT; ++f) a.out[f * OV + vo + lu] = o[f];
 }
 
 }  // namespace

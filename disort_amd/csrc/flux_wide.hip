@@ -1,0 +1,716 @@
+// gfx950 workgroup kernels of the flux solve for wide stream counts
+// (N = 32, 64; also built for N = 8, 16 as an alternative schedule).
+//
+// The register kernels of flux_kernels.hip keep a whole NN x NN eigenproblem
+// in one lane and a 3NN-row panel in 16 lanes; at N = 32 and 64 that no
+// longer fits.  Here
+//   k_layer_wg     one workgroup per (instance, layer): the matrices of the
+//                  reduced eigenproblem (src/FluxSolver.cpp:459-588) live in
+//                  LDS; Cholesky factors are built column by column, the
+//                  symmetric eigenproblem is solved by parallel cyclic Jacobi
+//                  (round-robin pairing, NN/2 disjoint rotations per round),
+//                  and the particular solutions (:595-643, :745-831) use the
+//                  same eigen-decomposition / Cholesky formulation as the
+//                  register kernel, with one thread per vector entry.
+//   k_boundary_wg  one workgroup per instance: the panel LU with partial
+//                  pivoting of the register kernel, with one thread per
+//                  (panel row, block): rows stay in registers, the pivot row
+//                  and column go through LDS.  Pivot choices, operation order
+//                  and the stored factors are those of the register kernel,
+//                  so for N <= 16 both produce the same bits from the same
+//                  eigen-system.
+// k_column and k_flux (flux_common.hpp) are shared with the register path.
+#include "flux_common.hpp"
+
+namespace dsa {
+namespace {
+
+template <int NN>
+struct WideCfg {
+  static constexpr int N = 2 * NN;
+  static constexpr int LT = NN >= 16 ? 128 : 64;  // k_layer_wg threads
+  static constexpr int R = 3 * NN;                // panel rows
+  static constexpr int RP = (R + 63) / 64 * 64;   // rows padded to whole waves
+  static constexpr int BT = 2 * RP;               // k_boundary_wg threads
+  static constexpr int LD = NN + 1;               // LDS row pitch of NN x NN matrices
+};
+
+// ---------------------------------------------------------------------------
+// Block-cooperative vector kernels on LDS (all threads call them).
+// ---------------------------------------------------------------------------
+// x <- M^{-1} x, M lower triangular (pitch ld).  Entry r accumulates its
+// subtractions in ascending column order, then divides (as a row-oriented
+// forward substitution would).
+template <int NN>
+__device__ void blk_lsolve(const double* M, int ld, double* x) {
+  const int tid = threadIdx.x;
+  double xr = (tid < NN) ? x[tid] : 0.0;
+  for (int r = 0; r < NN; ++r) {
+    if (tid == r) {
+      xr = xr / M[r * ld + r];
+      x[r] = xr;
+    }
+    __syncthreads();
+    if (tid > r && tid < NN) xr -= M[tid * ld + r] * x[r];
+  }
+  __syncthreads();
+}
+
+// x <- M^{-T} x, M lower triangular.
+template <int NN>
+__device__ void blk_ltsolve(const double* M, int ld, double* x) {
+  const int tid = threadIdx.x;
+  double xr = (tid < NN) ? x[tid] : 0.0;
+  for (int r = NN - 1; r >= 0; --r) {
+    if (tid == r) {
+      xr = xr / M[r * ld + r];
+      x[r] = xr;
+    }
+    __syncthreads();
+    if (tid < r) xr -= M[r * ld + tid] * x[r];
+  }
+  __syncthreads();
+}
+
+// x <- M x (lower) or M^T x, through scratch y.
+template <int NN, bool TRANS>
+__device__ void blk_lmul(const double* M, int ld, double* x, double* y) {
+  const int tid = threadIdx.x;
+  if (tid < NN) {
+    double s = 0.0;
+    if (!TRANS)
+      for (int m = 0; m <= tid; ++m) s += M[tid * ld + m] * x[m];
+    else
+      for (int m = tid; m < NN; ++m) s += M[m * ld + tid] * x[m];
+    y[tid] = s;
+  }
+  __syncthreads();
+  if (tid < NN) x[tid] = y[tid];
+  __syncthreads();
+}
+
+// x <- V diag(d) V^T x (V column eigenvectors, pitch ld), scratch y.
+template <int NN>
+__device__ void blk_vdv(const double* V, int ld, const double* d, double* x, double* y) {
+  const int tid = threadIdx.x;
+  if (tid < NN) {
+    double s = 0.0;
+    for (int r = 0; r < NN; ++r) s += V[r * ld + tid] * x[r];
+    y[tid] = s * d[tid];
+  }
+  __syncthreads();
+  if (tid < NN) {
+    double s = 0.0;
+    for (int j = 0; j < NN; ++j) s += V[tid * ld + j] * y[j];
+    x[tid] = s;
+  }
+  __syncthreads();
+}
+
+// In-place Cholesky of the lower triangle of M (left-looking, column by
+// column, the operation order of the register kernel).  Returns 0 when a
+// pivot is not positive (the factor then uses 1 in its place).
+template <int NN>
+__device__ int blk_cholesky(double* M, int ld, int* flag) {
+  const int tid = threadIdx.x;
+  if (tid == 0) *flag = 1;
+  __syncthreads();
+  for (int c = 0; c < NN; ++c) {
+    if (tid == 0) {
+      double x = M[c * ld + c];
+      for (int k = 0; k < c; ++k) x -= M[c * ld + k] * M[c * ld + k];
+      if (!(x > 0.0)) *flag = 0;
+      M[c * ld + c] = sqrt(x > 0.0 ? x : 1.0);
+    }
+    __syncthreads();
+    const double lcc = M[c * ld + c];
+    for (int r = c + 1 + tid; r < NN; r += blockDim.x) {
+      double s = M[r * ld + c];
+      for (int k = 0; k < c; ++k) s -= M[r * ld + k] * M[c * ld + k];
+      M[r * ld + c] = s / lcc;
+    }
+    __syncthreads();
+  }
+  return *flag;
+}
+
+// Block-wide sum of one double per thread.
+__device__ double blk_sum(double v, double* red) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  for (int k = 0; k < nw; ++k) s += red[k];
+  return s;
+}
+
+// ===========================================================================
+// k_layer_wg: one workgroup per (instance, layer) task.
+// ===========================================================================
+template <int NN>
+__global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
+  using C = WideCfg<NN>;
+  constexpr int N = C::N, LD = C::LD, NP = NN / 2;
+  constexpr TabOff to(N);
+  constexpr EigOff eo(N);
+  static_assert(NN % 2 == 0, "round-robin pairing needs an even order");
+  __shared__ double sA[NN * LD];  // A_s, later -A_s and its Cholesky factor
+  __shared__ double sL[NN * LD];  // -B_s, then its Cholesky factor L
+  __shared__ double sS[NN * LD];  // S = L^T (-A_s) L, diagonalised in place
+  __shared__ double sV[NN * LD];  // eigenvectors (columns)
+  __shared__ double chi[N], p0[N];
+  __shared__ double v0[NN], v1[NN], v2[NN], v3[NN], lam[NN], dv[NN];
+  __shared__ double rc[NP], rs[NP], rt[NP], rapp[NP], raqq[NP], rapq[NP];
+  __shared__ int rp[NP], rq[NP];
+  __shared__ double red[C::LT / 64];
+  __shared__ int flag;
+
+  const int tid = threadIdx.x;
+  const int64_t t = blockIdx.x;
+  if (t >= a.n_tasks) return;
+  const int i = a.t_inst[t];
+  const int lc = a.t_lc[t];
+  const int64_t g = a.lay_off[i] + lc;
+  const int64_t v = a.lay_off[i] + i + lc;
+  const int P = a.pstride[i];
+  const double* pm = a.pmom + a.mom_off[i] + (int64_t)lc * P;
+  const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
+  const int64_t TL = a.total_layers, TV = a.total_levels, T = a.n_tasks;
+  const double* tb = a.tab;
+  const double ws = a.lay_d[LD_SSA_S * TL + g];
+  const double f = a.lay_d[LD_F * TL + g];
+  const double sig2 = a.lay_d[LD_SIG2 * TL + g];
+  const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
+  const bool beam = F0 > 0.0;
+
+  // Scaled phase coefficients chi'_l (src/FluxSolver.cpp:342-388) and
+  // Legendre polynomials at -mu0.
+  for (int l = tid; l < N; l += blockDim.x) {
+    const double pl = pm[l];
+    double c;
+    if (sig2 < 0.0) c = (2.0 * l + 1.0) * ws * pl;
+    else if (sig2 == 0.0) c = (2.0 * l + 1.0) * ws * (pl - f) / (1.0 - f);
+    else c = (2.0 * l + 1.0) * ws * (pl - f * exp(-(double)(l * l) / (2.0 * sig2))) / (1.0 - f);
+    chi[l] = c;
+  }
+  if (tid == 0) {
+    const double x = -mu0;
+    p0[0] = 1.0;
+    p0[1] = x;
+    for (int l = 2; l < N; ++l) p0[l] = ((2.0 * l - 1.0) * x * p0[l - 1] - (l - 1.0) * p0[l - 2]) / l;
+  }
+  __syncthreads();
+
+  // A_s and -B_s from the even / odd Legendre sums (:494-512); the lower
+  // triangle is computed and mirrored so both are exactly symmetric.
+  for (int e = tid; e < NN * NN; e += blockDim.x) {
+    const int r0 = e / NN, c0 = e % NN;
+    const int r = r0 >= c0 ? r0 : c0, c = r0 >= c0 ? c0 : r0;
+    double se = 0.0, so = 0.0;
+    for (int l = 0; l < N; ++l) {
+      const double q = chi[l] * tb[to.Q() + l * NN + r] * tb[to.Q() + l * NN + c];
+      if (l % 2 == 0) se += q;
+      else so += q;
+    }
+    const double dg = (r == c) ? 1.0 / tb[to.cmu() + r] : 0.0;
+    sA[r0 * LD + c0] = se - dg;
+    sL[r0 * LD + c0] = dg - so;
+  }
+  __syncthreads();
+  if (!blk_cholesky<NN>(sL, LD, &flag) && tid == 0) atomicOr(&a.inst_i[i * kInstI + II_STATUS], DISORT_ST_EIGEN);
+
+  // S = L^T (-A_s) L: T = -A_s L into sV, then S (lower, mirrored).
+  for (int e = tid; e < NN * NN; e += blockDim.x) {
+    const int r = e / NN, c = e % NN;
+    double s = 0.0;
+    for (int k = c; k < NN; ++k) s -= sA[r * LD + k] * sL[k * LD + c];
+    sV[r * LD + c] = s;
+  }
+  __syncthreads();
+  for (int e = tid; e < NN * NN; e += blockDim.x) {
+    const int c0 = e / NN, r0 = e % NN;
+    if (r0 <= c0) {
+      double s = 0.0;
+      for (int k = r0; k < NN; ++k) s += sL[k * LD + r0] * sV[k * LD + c0];
+      sS[c0 * LD + r0] = s;
+      sS[r0 * LD + c0] = s;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < NN * NN; e += blockDim.x) sV[e / NN * LD + e % NN] = (e / NN == e % NN) ? 1.0 : 0.0;
+  __syncthreads();
+
+  // Parallel cyclic Jacobi: each round rotates NN/2 disjoint index pairs
+  // (round-robin schedule), each sweep visits every pair once.
+  for (int sweep = 0; sweep < 40; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int e = tid; e < NN * NN; e += blockDim.x) {
+      const int r = e / NN, c = e % NN;
+      const double x = sS[r * LD + c];
+      if (r > c) off += x * x;
+      if (r == c) dia += x * x;
+    }
+    off = blk_sum(off, red);
+    dia = blk_sum(dia, red);
+    if (off == 0.0 || off <= 1e-36 * dia) break;
+    for (int round = 0; round < NN - 1; ++round) {
+      if (tid < NP) {
+        const int k = tid;
+        const int pa = (k == 0) ? 0 : 1 + (k - 1 + round) % (NN - 1);
+        const int pb = 1 + (NN - 2 - k + round) % (NN - 1);
+        const int p = pa < pb ? pa : pb, q = pa < pb ? pb : pa;
+        const double apq = sS[q * LD + p], app = sS[p * LD + p], aqq = sS[q * LD + q];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double at = fabs(theta);
+        double tt = (at > 1e150) ? 0.5 / at : 1.0 / (at + sqrt(theta * theta + 1.0));
+        tt = (theta < 0.0) ? -tt : tt;
+        tt = (apq == 0.0) ? 0.0 : tt;
+        const double cs = 1.0 / sqrt(tt * tt + 1.0);
+        rp[k] = p;
+        rq[k] = q;
+        rc[k] = cs;
+        rs[k] = tt * cs;
+        rt[k] = tt;
+        rapp[k] = app;
+        raqq[k] = aqq;
+        rapq[k] = apq;
+      }
+      __syncthreads();
+      // S <- J^T S J by 2x2 blocks (pair k rows, pair k2 columns), k >= k2;
+      // V <- V J by column pairs.
+      for (int e = tid; e < NP * NP; e += blockDim.x) {
+        const int k = e / NP, k2 = e % NP;
+        if (k2 > k) continue;
+        const int p = rp[k], q = rq[k];
+        if (k == k2) {
+          const double tt = rt[k], apq = rapq[k];
+          sS[p * LD + p] = rapp[k] - tt * apq;
+          sS[q * LD + q] = raqq[k] + tt * apq;
+          sS[q * LD + p] = 0.0;
+          sS[p * LD + q] = 0.0;
+        } else {
+          const int p2 = rp[k2], q2 = rq[k2];
+          const double c1 = rc[k], s1 = rs[k], c2 = rc[k2], s2 = rs[k2];
+          const double bpp = sS[p * LD + p2], bpq = sS[p * LD + q2];
+          const double bqp = sS[q * LD + p2], bqq = sS[q * LD + q2];
+          const double upp = c1 * bpp - s1 * bqp, upq = c1 * bpq - s1 * bqq;  // rows
+          const double uqp = s1 * bpp + c1 * bqp, uqq = s1 * bpq + c1 * bqq;
+          const double npp = c2 * upp - s2 * upq, npq = s2 * upp + c2 * upq;  // columns
+          const double nqp = c2 * uqp - s2 * uqq, nqq = s2 * uqp + c2 * uqq;
+          sS[p * LD + p2] = npp;
+          sS[p * LD + q2] = npq;
+          sS[q * LD + p2] = nqp;
+          sS[q * LD + q2] = nqq;
+          sS[p2 * LD + p] = npp;
+          sS[q2 * LD + p] = npq;
+          sS[p2 * LD + q] = nqp;
+          sS[q2 * LD + q] = nqq;
+        }
+      }
+      for (int e = tid; e < NN * NP; e += blockDim.x) {
+        const int m = e / NP, k = e % NP;
+        const int p = rp[k], q = rq[k];
+        const double vp = sV[m * LD + p], vq = sV[m * LD + q];
+        sV[m * LD + p] = rc[k] * vp - rs[k] * vq;
+        sV[m * LD + q] = rs[k] * vp + rc[k] * vq;
+      }
+      __syncthreads();
+    }
+  }
+  if (tid < NN) {
+    lam[tid] = sS[tid * LD + tid];
+    a.eig[(eo.k() + tid) * T + t] = sqrt(fabs(lam[tid]));
+  }
+  __syncthreads();
+
+  // G+/- (:560-587): u = S_d^-1 L^-T w, (alpha-beta) u / k = -S_d^-1 L w / k.
+  // U = L^-T V column by column into sS (S is no longer needed), then every
+  // entry (r, j) combines U with (L V)(r, j).
+  for (int j = tid; j < NN; j += blockDim.x) {
+    for (int r = NN - 1; r >= 0; --r) {
+      double s = sV[r * LD + j];
+      for (int m = r + 1; m < NN; ++m) s -= sL[m * LD + r] * sS[m * LD + j];
+      sS[r * LD + j] = s / sL[r * LD + r];
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < NN * NN; e += blockDim.x) {
+    const int r = e % NN, j = e / NN;
+    double y = 0.0;
+    for (int m = 0; m <= r; ++m) y += sL[r * LD + m] * sV[m * LD + j];
+    const double sdi = tb[to.sdi() + r];
+    const double uu = sdi * sS[r * LD + j];
+    const double tt = -sdi * y / sqrt(fabs(lam[j]));
+    a.eig[(eo.gp() + r + j * NN) * T + t] = 0.5 * (tt + uu);
+    a.eig[(eo.gm() + r + j * NN) * T + t] = 0.5 * (tt - uu);
+  }
+
+  // Particular solutions, reference angle order: NN+r is +mu_r, NN-1-r is -mu_r.
+  // zb (beam) in v0/v1 (upper/lower halves), z0 in v2/v3, z1 kept per thread.
+  double zbu = 0.0, zbd = 0.0, z0u = 0.0, z0d = 0.0, z1h = 0.0;
+  if (beam) {
+    // Right-hand side sums over even / odd l (src/FluxSolver.cpp:628-632).
+    double bsr = 0.0, bdr = 0.0;
+    if (tid < NN) {
+      double se = 0.0, so = 0.0;
+      for (int l = 0; l < N; ++l) {
+        const double term = chi[l] * tb[to.P() + l * NN + tid] * p0[l];
+        if (l % 2 == 0) se += term;
+        else so += term;
+      }
+      bsr = F0 * se / (2.0 * kPi);
+      bdr = F0 * so / (2.0 * kPi);
+    }
+    // Dither mu0 when 1 - mu0^2 k^2 is tiny for any mode (:604-618).
+    double mub = mu0;
+    const double mu0sq = mu0 * mu0;
+    for (int j = 0; j < NN; ++j) {
+      const double kj = sqrt(fabs(lam[j]));
+      if (fabs(1.0 - mu0sq * kj * kj) < 1000.0 * kEps) mub = 0.999 * mu0;
+    }
+    double pr = 0.0, qr = 0.0;
+    if (tid < NN) {
+      const double cm = tb[to.cmu() + tid];
+      pr = -bsr / cm;
+      qr = -bdr / cm;
+      v0[tid] = tb[to.sd() + tid] * qr;  // w
+    }
+    __syncthreads();
+    // (alpha+beta) q = -S_d^-1 L^-T V Lambda V^T L^-1 S_d q
+    blk_lsolve<NN>(sL, LD, v0);
+    blk_vdv<NN>(sV, LD, lam, v0, v1);
+    blk_ltsolve<NN>(sL, LD, v0);
+    if (tid < NN) {
+      const double apbq = -tb[to.sdi() + tid] * v0[tid];
+      const double rr = (pr + mub * apbq) / mub;
+      v2[tid] = rr * tb[to.sd() + tid];
+      dv[tid] = 1.0 / (lam[tid] - 1.0 / (mub * mub));
+    }
+    __syncthreads();
+    // d = S_d^-1 L^-T V (Lambda - 1/mu0^2)^-1 V^T L^T S_d r
+    blk_lmul<NN, true>(sL, LD, v2, v1);
+    blk_vdv<NN>(sV, LD, dv, v2, v1);
+    blk_ltsolve<NN>(sL, LD, v2);
+    double dr = 0.0;
+    if (tid < NN) {
+      dr = tb[to.sdi() + tid] * v2[tid];
+      v3[tid] = tb[to.sd() + tid] * dr;
+    }
+    __syncthreads();
+    // (alpha-beta) d = -S_d^-1 L L^T S_d d
+    blk_lmul<NN, true>(sL, LD, v3, v1);
+    blk_lmul<NN, false>(sL, LD, v3, v1);
+    if (tid < NN) {
+      const double ambd = -tb[to.sdi() + tid] * v3[tid];
+      const double s = mub * (ambd - qr);
+      zbu = 0.5 * (s + dr);
+      zbd = 0.5 * (s - dr);
+    }
+    __syncthreads();
+  }
+
+  const double taut = a.lev_d[VD_TAU_S * TV + v];
+  const double taub = a.lev_d[VD_TAU_S * TV + v + 1];
+  double xr0 = 0.0, xr1 = 0.0;
+  if (a.flags & DISORT_F_THERMAL) {
+    // B(tau) = xr0 + xr1 tau (:755-776); z1 and z0 from a Cholesky factor of
+    // -A_s (see the register kernel).
+    const double pt = a.lev_d[VD_PLANCK * TV + v], pb = a.lev_d[VD_PLANCK * TV + v + 1];
+    const double dts = a.lay_d[LD_DTAU_S * TL + g];
+    if (dts > 1e-4) xr1 = (pb - pt) / dts;
+    xr0 = pt - xr1 * taut;
+    for (int e = tid; e < NN * NN; e += blockDim.x) sA[e / NN * LD + e % NN] = -sA[e / NN * LD + e % NN];
+    __syncthreads();
+    blk_cholesky<NN>(sA, LD, &flag);
+    if (tid < NN) v0[tid] = tb[to.sd() + tid] * (2.0 / tb[to.cmu() + tid]);
+    __syncthreads();
+    blk_lsolve<NN>(sA, LD, v0);
+    blk_ltsolve<NN>(sA, LD, v0);
+    const double c1 = (1.0 - ws) * xr1, c0 = (1.0 - ws) * xr0;
+    double base = 0.0;
+    if (tid < NN) {
+      base = v0[tid] * tb[to.sdi() + tid];
+      z1h = 0.5 * c1 * base;
+      v1[tid] = tb[to.sd() + tid] * z1h;
+    }
+    __syncthreads();
+    blk_lsolve<NN>(sL, LD, v1);
+    blk_ltsolve<NN>(sL, LD, v1);
+    if (tid < NN) {
+      const double d0 = 2.0 * tb[to.sdi() + tid] * v1[tid];
+      const double s0 = c0 * base;
+      z0u = 0.5 * (s0 + d0);
+      z0d = 0.5 * (s0 - d0);
+    }
+  }
+
+  if (tid < NN) {
+    const double btt = a.lay_d[LD_BT_TOP * TL + g], btb = a.lay_d[LD_BT_BOT * TL + g];
+    const double fact = beam ? exp(-taub / mu0) : 0.0;
+    const int ru = NN + tid, rd = NN - 1 - tid;
+    a.eig[(eo.ztop() + ru) * T + t] = zbu * btt + z0u + z1h * taut;
+    a.eig[(eo.ztop() + rd) * T + t] = zbd * btt + z0d + z1h * taut;
+    a.eig[(eo.zbot() + ru) * T + t] = zbu * btb + z0u + z1h * taub;
+    a.eig[(eo.zbot() + rd) * T + t] = zbd * btb + z0d + z1h * taub;
+    a.eig[(eo.zfl() + ru) * T + t] = zbu * fact + z0u + z1h * taub;
+    a.eig[(eo.zfl() + rd) * T + t] = zbd * fact + z0d + z1h * taub;
+  }
+  if (tid == 0) a.eig[eo.pls() * T + t] = xr0 + xr1 * taub;
+}
+
+// ===========================================================================
+// k_boundary_wg: one workgroup per instance.  Thread (h, r): panel row r,
+// block h of the row (N coefficients); h = 1 also owns the right-hand side.
+// In panel ip the left block (columns of layer ip) is held by h = ip & 1, so
+// the equations carried to the next panel stay in place and the other
+// block takes the next layer's columns.
+// ===========================================================================
+template <int NN>
+__global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
+  using C = WideCfg<NN>;
+  constexpr int N = C::N, R = C::R, RP = C::RP, NWH = RP / 64;
+  constexpr EigOff eo(N);
+  constexpr TabOff to(N);
+  constexpr int PS = panel_stride(N);
+  __shared__ double prow[2 * N + 1];  // pivot row: left block, right block, rhs
+  __shared__ double colc[RP];         // pivot column
+  __shared__ double red_v[NWH];
+  __shared__ int red_i[NWH];
+  __shared__ int uflag[RP];
+  __shared__ double ecur[NN], enxt[NN], rsb[N], xs[N], xn[N];
+
+  const int tid = threadIdx.x;
+  const int h = tid / RP, r = tid % RP;
+  const bool row = r < R;
+  const int bk = blockIdx.x;
+  if (bk >= a.nb) return;
+  const int i = a.b_inst[bk];
+  const int64_t t0 = a.b_task0[bk];
+  const int64_t lo = a.lay_off[i];
+  const int64_t T = a.n_tasks, TL = a.total_layers;
+  const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
+  const double* tb = a.tab;
+  const int M = a.inst_i[i * kInstI + II_NCUT];
+  const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
+  const bool diffusion = (a.flags & DISORT_F_DIFFUSION_BC) != 0;
+  const bool thermal = (a.flags & DISORT_F_THERMAL) != 0;
+  const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
+  const double alb = sc[DISORT_S_ALBEDO];
+
+  // exp(-k_jp dtau') of layer task tt into e[jp].
+  auto layer_exp = [&](int64_t tt, double* e) {
+    for (int jp = tid; jp < NN; jp += blockDim.x)
+      e[jp] = exp(-a.eig[(eo.k() + jp) * T + tt] * a.lay_d[LD_DTAU_S * TL + lo + (tt - t0)]);
+  };
+  // Mode column j: factor at the layer top / bottom.
+  auto etop_of = [](const double* e, int j) { return j < NN ? e[NN - 1 - j] : 1.0; };
+  auto ebot_of = [](const double* e, int j) { return j < NN ? 1.0 : e[j - NN]; };
+
+  double A[N];
+  double rhs = 0.0;
+  bool used;
+
+  // Top boundary rows 0..NN-1 (:890-907, :976-990) in the left block of
+  // panel 0 (h = 0); rows NN.. are free for the first interface.
+  layer_exp(t0, ecur);
+  __syncthreads();
+  {
+    const bool tbc = row && r < NN;
+    const double top = sc[DISORT_S_ISO_TOP] + a.inst_d[i * kInstD + ID_PL_TOP];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      A[j] = (tbc && h == 0) ? g_entry<NN>(a, t0, NN - 1 - r, j) * etop_of(ecur, j) : 0.0;
+    if (h == 1) rhs = tbc ? top - a.eig[(eo.ztop() + (tbc ? NN - 1 - r : 0)) * T + t0] : 0.0;
+    used = !(row && r < NN);
+  }
+
+  // N elimination steps of one panel.  `used` rows are not candidates;
+  // `left` tells whether this thread holds the pivot-column block.
+  auto eliminate = [&](bool left, double* pp) __attribute__((always_inline)) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) {
+      double key = -1.0;
+      if (left) {
+        colc[r] = A[c];
+        key = (row && !used) ? fabs(A[c]) : -1.0;
+      }
+      int idx = r;
+      for (int off = 32; off >= 1; off >>= 1) {
+        const double k2 = __shfl_xor(key, off);
+        const int i2 = __shfl_xor(idx, off);
+        const bool take = (k2 > key) || (k2 == key && i2 < idx);
+        key = take ? k2 : key;
+        idx = take ? i2 : idx;
+      }
+      if (left && (tid & 63) == 0) {
+        red_v[r >> 6] = key;
+        red_i[r >> 6] = idx;
+      }
+      __syncthreads();
+      double best = red_v[0];
+      int p = red_i[0];
+      for (int w = 1; w < NWH; ++w) {
+        const double k2 = red_v[w];
+        const int i2 = red_i[w];
+        if (k2 > best || (k2 == best && i2 < p)) {
+          best = k2;
+          p = i2;
+        }
+      }
+      const double piv = colc[p];
+      const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
+      const double l = colc[r] * inv;
+      if (r == p) {
+        const int o = left ? 0 : N;
+#pragma unroll
+        for (int j = 0; j < N; ++j) prow[o + j] = A[j];
+        if (left) {
+#pragma unroll
+          for (int j = 0; j < N; ++j) pp[c * N + j] = (j == c) ? inv : A[j];
+        } else {
+#pragma unroll
+          for (int j = 0; j < N; ++j) pp[N * N + c * N + j] = A[j];
+        }
+        if (h == 1) {
+          prow[2 * N] = rhs;
+          pp[2 * N * N + c] = rhs;
+        }
+      }
+      __syncthreads();
+      const int o = left ? 0 : N;
+#pragma unroll
+      for (int j = 0; j < N; ++j) A[j] -= l * prow[o + j];
+      if (h == 1) rhs -= l * prow[2 * N];
+      used = used || (r == p);
+    }
+  };
+
+  // Rank of this row among the used rows (the interface row it takes).
+  auto used_rank = [&]() {
+    if (h == 0) uflag[r] = (row && used) ? 1 : 0;
+    __syncthreads();
+    int q = 0;
+    for (int k = 0; k < r; ++k) q += uflag[k];
+    __syncthreads();
+    return q;
+  };
+
+  // Interior panels: continuity at the interface below layer ip
+  // (:851-887, :1094-1118).
+  for (int ip = 0; ip < M - 1; ++ip) {
+    const int64_t tt = t0 + ip;
+    const bool left = h == (ip & 1);
+    layer_exp(tt, ecur);
+    layer_exp(tt + 1, enxt);
+    const int q = used_rank();  // includes the barrier that publishes ecur/enxt
+    const bool fill = row && used;
+    if (left) {
+      if (fill) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) A[j] = g_entry<NN>(a, tt, q, j) * ebot_of(ecur, j);
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) A[j] = fill ? -g_entry<NN>(a, tt + 1, q, j) * etop_of(enxt, j) : 0.0;
+    }
+    if (h == 1 && fill) rhs = a.eig[(eo.ztop() + q) * T + tt + 1] - a.eig[(eo.zbot() + q) * T + tt];
+    used = !row;
+    eliminate(left, a.panel + tt * PS);
+  }
+
+  // Last panel: bottom boundary, Lambertian reflection of the downward field
+  // (:909-959, :992-1091).
+  {
+    const int64_t tt = t0 + M - 1;
+    const bool left = h == ((M - 1) & 1);
+    const bool refl = !(lyrcut || diffusion);
+    layer_exp(tt, ecur);
+    for (int j = tid; j < N; j += blockDim.x) {
+      double s = 0.0;
+      for (int k = 0; k < NN; ++k) s += tb[to.cwt() + k] * tb[to.cmu() + k] * alb * g_entry<NN>(a, tt, NN - 1 - k, j);
+      rsb[j] = refl ? s : 0.0;
+    }
+    double zrefl = 0.0;
+    for (int k = 0; k < NN; ++k) zrefl += tb[to.cwt() + k] * tb[to.cmu() + k] * alb * a.eig[(eo.zbot() + NN - 1 - k) * T + tt];
+    const int q = used_rank();
+    const bool fill = row && used;
+    const bool bot = fill && q < NN;
+    const int tq = bot ? q : 0;
+    if (left) {
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        A[j] = bot ? (g_entry<NN>(a, tt, NN + tq, j) - 2.0 * rsb[j]) * ebot_of(ecur, j) : (fill ? 0.0 : A[j]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) A[j] = 0.0;
+    }
+    if (h == 1) {
+      const double btn = a.inst_d[i * kInstD + ID_BTRANS_NCUT];
+      const double plb = a.inst_d[i * kInstD + ID_PL_BOT];
+      const double plbd = a.inst_d[i * kInstD + ID_PL_BOT_DERIV];
+      double extra = sc[DISORT_S_ISO_BOT];
+      if (refl) extra += 2.0 * zrefl;
+      if (F0 > 0.0 && !lyrcut) extra += alb * mu0 * F0 * btn / kPi;
+      if (thermal) extra += (1.0 - alb) * plb;
+      const double zb = -a.eig[(eo.zbot() + NN + tq) * T + tt];
+      rhs = bot ? zb + (diffusion ? plb + tb[to.cmu() + tq] * plbd : extra) : (fill ? 0.0 : rhs);
+    }
+    used = !row || (fill && !bot);
+    eliminate(left, a.panel + tt * PS);
+  }
+
+  // Back substitution, thread k = U row k of each panel (h = 0 threads).
+  const int k = tid;
+  for (int ip = M - 1; ip >= 0; --ip) {
+    const int64_t tt = t0 + ip;
+    const double* pp = a.panel + tt * PS;
+    double vk = 0.0;
+    if (k < N) {
+      vk = pp[2 * N * N + k];
+      if (ip < M - 1)
+        for (int c = 0; c < N; ++c) vk -= pp[N * N + k * N + c] * xn[c];
+    }
+    for (int c = N - 1; c >= 0; --c) {
+      if (k == c) xs[c] = vk * pp[c * N + c];
+      __syncthreads();
+      if (k < c) vk -= pp[k * N + c] * xs[c];
+    }
+    __syncthreads();
+    if (k < N) {
+      a.cvec[tt * N + k] = xs[k];
+      xn[k] = xs[k];
+    }
+    __syncthreads();
+  }
+}
+
+template <int NN>
+void launch_wide(const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
+  using C = WideCfg<NN>;
+  constexpr int N = 2 * NN;
+  if (ev) (void)hipEventRecord(ev[0], s);
+  hipLaunchKernelGGL(k_column<N>, dim3(a.nb), dim3(64), 0, s, a);
+  if (ev) (void)hipEventRecord(ev[1], s);
+  hipLaunchKernelGGL(k_layer_wg<NN>, dim3((unsigned)a.n_tasks), dim3(C::LT), 0, s, a);
+  if (ev) (void)hipEventRecord(ev[2], s);
+  hipLaunchKernelGGL(k_boundary_wg<NN>, dim3((unsigned)a.nb), dim3(C::BT), 0, s, a);
+  if (ev) (void)hipEventRecord(ev[3], s);
+  hipLaunchKernelGGL(k_flux<NN>, dim3((unsigned)((a.n_lvtasks + 63) / 64)), dim3(64), 0, s, a);
+  if (ev) (void)hipEventRecord(ev[4], s);
+}
+
+}  // namespace
+
+bool launch_flux_bucket_wg(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
+  switch (nstr) {
+    case 8: launch_wide<4>(a, s, ev); return true;
+    case 16: launch_wide<8>(a, s, ev); return true;
+    case 32: launch_wide<16>(a, s, ev); return true;
+    case 64: launch_wide<32>(a, s, ev); return true;
+    default: return false;
+  }
+}
+
+}  // namespace dsa

@@ -109,3 +109,37 @@ def random_flux_batch(
     )
     flags = (F_THERMAL if thermal else 0) | flags_extra
     return uniform_batch(dtau, ssa, pmom, nstr, sc, flags=flags, temper=temper if thermal else None)
+
+
+C4_SEED = 20261204
+C4_STREAMS = (4, 8, 16, 32, 64)
+
+
+def c4(n: int = 50000, seed: int = C4_SEED, first: int = 0, streams=C4_STREAMS, lmin: int = 10, lmax: int = 120) -> FluxBatch:
+    """Ragged batch: instance k has nstr ~ U{streams} and nlyr ~ U{lmin..lmax},
+    beam only (flux pi, mu0 ~ U(0.1,1) per instance), Lambertian albedo
+    ~ U(0,0.3), per-layer optical properties drawn as in c2.  Instance k's
+    draws depend only on (seed, first+k), so subsets reproduce the full batch.
+    """
+    nstr = np.empty(n, np.int32)
+    nlyr = np.empty(n, np.int32)
+    dt, om, pm, sc = [], [], [], []
+    for k in range(n):
+        r = rng(seed + 1 + first + k)
+        N = int(streams[r.integers(len(streams))])
+        L = int(r.integers(lmin, lmax + 1))
+        nstr[k], nlyr[k] = N, L
+        dt.append(10.0 ** r.uniform(-4.0, 1.0, L))
+        om.append(r.uniform(0.0, 0.999, L))
+        pm.append(hg_moments(r.uniform(0.0, 0.9, L), N + 1).reshape(-1))
+        sc.append(scalars(1, beam_flux=np.pi, beam_mu=r.uniform(0.1, 1.0), albedo=r.uniform(0.0, 0.3))[0])
+    return FluxBatch(
+        flags=0,
+        nstr=nstr,
+        nlyr=nlyr,
+        pstride=nstr + 1,
+        dtau=np.concatenate(dt),
+        ssa=np.concatenate(om),
+        pmom=np.concatenate(pm),
+        scal=np.stack(sc),
+    )

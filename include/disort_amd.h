@@ -129,9 +129,10 @@ int disort_sync(disort_ctx* ctx);
  * memspace must be DISORT_MEM_HOST: inputs are validated with the rules of
  * DisortFluxConfig::validate (src/DisortFluxConfig.cpp:43-143), copied in,
  * solved, and the outputs and per-instance status copied back before return.
- * Supported on the GPU: nstr in {4,6,...,16}, plane-parallel beam, thermal
- * emission, Lambertian surface, Delta-M / Delta-M+, layer cut-off, diffusion
- * lower boundary.
+ * Supported on the GPU: nstr in {4,6,...,16,32,64} (the stream counts the
+ * reference instantiates, src/FluxSolver.cpp:1572-1580), plane-parallel
+ * beam, thermal emission, Lambertian surface, Delta-M / Delta-M+, layer
+ * cut-off, diffusion lower boundary.
  */
 int disort_flux_solve(disort_ctx* ctx, const disort_flux_batch* batch, int memspace);
 
@@ -152,6 +153,19 @@ int disort_flux_plan_destroy(disort_ctx* ctx, disort_flux_plan* plan);
 int disort_flux_execute(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* device_batch);
 /* Per-instance DISORT_ST_* codes of the last execute with this plan (host array [n]). */
 int disort_flux_status(disort_ctx* ctx, disort_flux_plan* plan, int32_t* status_host);
+
+/*
+ * Kernel variant of one stream-count bucket of a plan (no counterpart in the
+ * reference: it selects between two gfx950 implementations of the same
+ * solve).  DISORT_V_LANE: register kernels, one lane per layer and 16 lanes
+ * per instance, N in {4,...,16}.  DISORT_V_WORKGROUP: LDS kernels, one
+ * workgroup per layer and per instance, N in {8,16,32,64}.  Both give
+ * results within the parity tolerance of each other; the default is LANE
+ * where it exists.  The autotuner (disort_amd/schedule.py) picks by timing.
+ */
+enum { DISORT_V_DEFAULT = 0, DISORT_V_LANE = 1, DISORT_V_WORKGROUP = 2 };
+int disort_flux_plan_set_variant(disort_ctx* ctx, disort_flux_plan* plan, int nstr, int variant);
+int disort_flux_plan_get_variant(disort_ctx* ctx, disort_flux_plan* plan, int nstr, int* variant);
 
 /* Device memory helpers for callers without their own allocator (ctypes). */
 int disort_device_alloc(disort_ctx* ctx, size_t bytes, void** ptr);

@@ -69,6 +69,11 @@ def lib():
         l.disort_flux_plan_set_variant.argtypes = [vp, vp, C.c_int, C.c_int]
         l.disort_flux_plan_get_variant.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
         l.disort_quadrature.argtypes = [C.c_int, vp, vp]
+        l.disort_flux_plan_set_concurrent.argtypes = [vp, vp, C.c_int]
+        l.disort_flux_execute_graph.argtypes = [vp, vp, vp]
+        l.disort_flux_autotune.argtypes = [vp, vp, vp, C.c_int]
+        l.disort_flux_plan_report.argtypes = [vp]
+        l.disort_flux_plan_report.restype = C.c_char_p
         _lib = l
     return _lib
 
@@ -198,6 +203,21 @@ class DeviceFluxBatch:
         v = C.c_int()
         _check(self.ctx.ptr, lib().disort_flux_plan_get_variant(self.ctx.ptr, self.plan, nstr, C.byref(v)))
         return v.value
+
+    def set_concurrent(self, on: bool):
+        """Run stream-count buckets concurrently on side streams (or serially)."""
+        _check(self.ctx.ptr, lib().disort_flux_plan_set_concurrent(self.ctx.ptr, self.plan, int(on)))
+
+    def autotune(self, reps: int = 3) -> dict:
+        """Time every kernel variant per bucket and both schedules; keep the fastest."""
+        import json
+
+        _check(self.ctx.ptr, lib().disort_flux_autotune(self.ctx.ptr, self.plan, C.byref(self.cstruct), reps))
+        return json.loads(lib().disort_flux_plan_report(self.plan).decode())
+
+    def execute_graph(self):
+        """Enqueue one solve as a single hipGraph launch (captured on first use)."""
+        _check(self.ctx.ptr, lib().disort_flux_execute_graph(self.ctx.ptr, self.plan, C.byref(self.cstruct)))
 
     def buckets(self):
         """Stream counts present in the batch."""

@@ -200,6 +200,7 @@ std::string validate_instance(const disort_flux_batch* b, int64_t i) {
 struct Bucket {
   int nstr = 0;
   int variant = 0;
+  double tuned_ms[3] = {-1.0, -1.0, -1.0};  // per variant, from the autotuner
   int64_t nb = 0, n_tasks = 0, n_lv = 0;
   int32_t* d_inst = nullptr;
   int64_t* d_task0 = nullptr;
@@ -221,7 +222,198 @@ struct disort_flux_plan {
   std::vector<Bucket> buckets;
   int64_t max_tasks = 0, max_eig = 0, max_panel = 0, max_cvec = 0;
   std::vector<void*> owned;
+  // Schedule: buckets one after another on the context stream (default), or
+  // concurrently on side streams with bucket-private scratch.
+  bool concurrent = false;
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
+  // Captured execution (disort_flux_execute_graph) and the batch it was
+  // captured for.
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  disort_flux_batch gkey{};
+  std::vector<const void*> gbufs;  // scratch and tables the graph was captured with
+  bool gconcurrent = false;
+  std::vector<int> gvariant;
+  std::string report;  // autotuner log (JSON)
 };
+
+// ---------------------------------------------------------------------------
+// Execution of a plan.
+// ---------------------------------------------------------------------------
+namespace {
+
+void release_graph(disort_flux_plan* plan) {
+  if (plan->gexec) (void)hipGraphExecDestroy(plan->gexec);
+  if (plan->graph) (void)hipGraphDestroy(plan->graph);
+  plan->gexec = nullptr;
+  plan->graph = nullptr;
+}
+
+// Batch-level arguments and every scratch buffer the plan needs, allocated
+// before any launch (so that a capture never meets an allocation).
+struct PlanArgs {
+  FluxArgs base{};
+  std::vector<double*> eig, panel, cvec;  // per bucket
+};
+
+int prepare(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b, PlanArgs& pa) {
+  if (!ctx || !plan || !b) return fail(ctx, DISORT_E_INVALID, "null argument");
+  if (b->n != plan->n) return fail(ctx, DISORT_E_INVALID, "batch size differs from the plan");
+  if ((b->flags & DISORT_F_THERMAL) && !b->temper) return fail(ctx, DISORT_E_INVALID, "thermal batch without temperatures");
+  if (b->flags & DISORT_F_SPHERICAL)
+    return fail(ctx, DISORT_E_UNSUPPORTED, "use_spherical_beam is not supported by the GPU flux path");
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  FluxArgs& a = pa.base;
+  a = FluxArgs{};
+  a.flags = b->flags;
+  a.nstr = plan->d_nstr;
+  a.nlyr = plan->d_nlyr;
+  a.pstride = plan->d_pstride;
+  a.lay_off = plan->d_lay_off;
+  a.mom_off = plan->d_mom_off;
+  a.dtau = b->dtau;
+  a.ssa = b->ssa;
+  a.pmom = b->pmom;
+  a.temper = b->temper;
+  a.scal = b->scal;
+  a.out = b->out;
+  a.total_levels = plan->total_levels;
+  a.total_layers = plan->total_layers;
+  int rc;
+  void* p;
+  if ((rc = get_buf(ctx, "inst_d", plan->n * dsa::kInstD * sizeof(double), &p))) return rc;
+  a.inst_d = (double*)p;
+  if ((rc = get_buf(ctx, "inst_i", plan->n * dsa::kInstI * sizeof(int32_t), &p))) return rc;
+  a.inst_i = (int32_t*)p;
+  if ((rc = get_buf(ctx, "lay_d", plan->total_layers * dsa::kLayD * sizeof(double), &p))) return rc;
+  a.lay_d = (double*)p;
+  if ((rc = get_buf(ctx, "lev_d", plan->total_levels * dsa::kLevD * sizeof(double), &p))) return rc;
+  a.lev_d = (double*)p;
+  const size_t nb = plan->buckets.size();
+  pa.eig.assign(nb, nullptr);
+  pa.panel.assign(nb, nullptr);
+  pa.cvec.assign(nb, nullptr);
+  if (!plan->concurrent) {
+    if ((rc = get_buf(ctx, "eig", plan->max_eig * sizeof(double), &p))) return rc;
+    std::fill(pa.eig.begin(), pa.eig.end(), (double*)p);
+    if ((rc = get_buf(ctx, "panel", plan->max_panel * sizeof(double), &p))) return rc;
+    std::fill(pa.panel.begin(), pa.panel.end(), (double*)p);
+    if ((rc = get_buf(ctx, "cvec", plan->max_cvec * sizeof(double), &p))) return rc;
+    std::fill(pa.cvec.begin(), pa.cvec.end(), (double*)p);
+  } else {
+    // Bucket-private scratch: buffers of stream count N are named by N.
+    for (size_t k = 0; k < nb; ++k) {
+      const Bucket& bk = plan->buckets[k];
+      const std::string tag = "#" + std::to_string(bk.nstr);
+      if ((rc = get_buf(ctx, ("eig" + tag).c_str(), (size_t)dsa::EigOff(bk.nstr).size() * bk.n_tasks * 8, &p))) return rc;
+      pa.eig[k] = (double*)p;
+      if ((rc = get_buf(ctx, ("panel" + tag).c_str(), (size_t)dsa::panel_stride(bk.nstr) * bk.n_tasks * 8, &p))) return rc;
+      pa.panel[k] = (double*)p;
+      if ((rc = get_buf(ctx, ("cvec" + tag).c_str(), (size_t)bk.nstr * bk.n_tasks * 8, &p))) return rc;
+      pa.cvec[k] = (double*)p;
+    }
+    while (plan->side.size() < nb) {
+      hipStream_t st;
+      HIPCHK(ctx, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      plan->side.push_back(st);
+      hipEvent_t e;
+      HIPCHK(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      plan->join.push_back(e);
+    }
+    if (!plan->fork) HIPCHK(ctx, hipEventCreateWithFlags(&plan->fork, hipEventDisableTiming));
+  }
+  for (const Bucket& bk : plan->buckets) {
+    double* tab = nullptr;
+    if ((rc = table_for(ctx, bk.nstr, &tab))) return rc;
+  }
+  return DISORT_OK;
+}
+
+FluxArgs bucket_args(disort_ctx* ctx, const PlanArgs& pa, const Bucket& bk, size_t k) {
+  FluxArgs a = pa.base;
+  a.tab = ctx->tables[bk.nstr];
+  a.nb = (int32_t)bk.nb;
+  a.b_inst = bk.d_inst;
+  a.b_task0 = bk.d_task0;
+  a.n_tasks = bk.n_tasks;
+  a.t_inst = bk.d_t_inst;
+  a.t_lc = bk.d_t_lc;
+  a.n_lvtasks = bk.n_lv;
+  a.v_bk = bk.d_v_bk;
+  a.v_lu = bk.d_v_lu;
+  a.eig = pa.eig[k];
+  a.panel = pa.panel[k];
+  a.cvec = pa.cvec[k];
+  return a;
+}
+
+int launch_bucket(disort_ctx* ctx, const FluxArgs& a, const Bucket& bk, hipStream_t st, hipEvent_t* ev) {
+  const bool ok = bk.variant == DISORT_V_LANE ? dsa::launch_flux_bucket(bk.nstr, a, st, ev)
+                                              : dsa::launch_flux_bucket_wg(bk.nstr, a, st, ev);
+  if (!ok) return fail(ctx, DISORT_E_UNSUPPORTED, "no kernels for num_streams=" + std::to_string(bk.nstr));
+  HIPCHK(ctx, hipGetLastError());
+  return DISORT_OK;
+}
+
+// Enqueue the whole plan on the context stream (forking to side streams
+// when concurrent).  With `timed`, buckets run one after another and the
+// per-stage event times are accumulated into ctx->stage_ms.
+int launch_plan(disort_ctx* ctx, disort_flux_plan* plan, const PlanArgs& pa, bool timed) {
+  int rc;
+  for (int k = 0; k < 4; ++k) ctx->stage_ms[k] = 0.0;
+  if (!plan->concurrent || timed) {
+    for (size_t k = 0; k < plan->buckets.size(); ++k) {
+      const Bucket& bk = plan->buckets[k];
+      if ((rc = launch_bucket(ctx, bucket_args(ctx, pa, bk, k), bk, ctx->stream, timed ? ctx->ev : nullptr))) return rc;
+      if (timed) {
+        HIPCHK(ctx, hipEventSynchronize(ctx->ev[4]));
+        for (int q = 0; q < 4; ++q) {
+          float ms = 0.f;
+          HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[q], ctx->ev[q + 1]));
+          ctx->stage_ms[q] += ms;
+        }
+      }
+    }
+    return DISORT_OK;
+  }
+  HIPCHK(ctx, hipEventRecord(plan->fork, ctx->stream));
+  // Largest buckets first, so that the long ones start earliest.
+  std::vector<size_t> order(plan->buckets.size());
+  for (size_t k = 0; k < order.size(); ++k) order[k] = k;
+  std::sort(order.begin(), order.end(), [&](size_t x, size_t y) {
+    const Bucket &bx = plan->buckets[x], &by = plan->buckets[y];
+    return (double)bx.n_tasks * bx.nstr * bx.nstr > (double)by.n_tasks * by.nstr * by.nstr;
+  });
+  for (size_t k : order) {
+    hipStream_t st = plan->side[k];
+    HIPCHK(ctx, hipStreamWaitEvent(st, plan->fork, 0));
+    if ((rc = launch_bucket(ctx, bucket_args(ctx, pa, plan->buckets[k], k), plan->buckets[k], st, nullptr))) return rc;
+    HIPCHK(ctx, hipEventRecord(plan->join[k], st));
+    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, plan->join[k], 0));
+  }
+  return DISORT_OK;
+}
+
+// Every device address a captured graph bakes in besides the batch arrays.
+std::vector<const void*> scratch_key(disort_ctx* ctx, const disort_flux_plan* plan, const PlanArgs& pa) {
+  std::vector<const void*> k = {pa.base.inst_d, pa.base.inst_i, pa.base.lay_d, pa.base.lev_d};
+  for (size_t q = 0; q < pa.eig.size(); ++q) {
+    k.push_back(pa.eig[q]);
+    k.push_back(pa.panel[q]);
+    k.push_back(pa.cvec[q]);
+    k.push_back(ctx->tables[plan->buckets[q].nstr]);
+  }
+  return k;
+}
+
+bool same_batch(const disort_flux_batch& x, const disort_flux_batch& y) {
+  return x.n == y.n && x.flags == y.flags && x.dtau == y.dtau && x.ssa == y.ssa && x.pmom == y.pmom &&
+         x.temper == y.temper && x.scal == y.scal && x.out == y.out;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -307,6 +499,10 @@ int disort_memcpy_d2h(disort_ctx* ctx, void* dst, const void* src, size_t bytes)
 int disort_flux_plan_destroy(disort_ctx* ctx, disort_flux_plan* plan) {
   if (!plan) return DISORT_OK;
   if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  release_graph(plan);
+  for (hipStream_t st : plan->side) (void)hipStreamDestroy(st);
+  for (hipEvent_t e : plan->join) (void)hipEventDestroy(e);
+  if (plan->fork) (void)hipEventDestroy(plan->fork);
   for (void* p : plan->owned) (void)hipFree(p);
   delete plan;
   return DISORT_OK;
@@ -420,72 +616,117 @@ int disort_flux_plan_create(disort_ctx* ctx, int64_t n, const int32_t* nstr, con
 // Launch a planned solve on device-resident batch arrays (asynchronous).
 // The batch's lay_off/mom_off are ignored: the plan's offsets are used.
 int disort_flux_execute(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b) {
-  if (!ctx || !plan || !b) return fail(ctx, DISORT_E_INVALID, "null argument");
-  if (b->n != plan->n) return fail(ctx, DISORT_E_INVALID, "batch size differs from the plan");
-  if ((b->flags & DISORT_F_THERMAL) && !b->temper) return fail(ctx, DISORT_E_INVALID, "thermal batch without temperatures");
-  HIPCHK(ctx, hipSetDevice(ctx->device));
-  FluxArgs a{};
-  a.flags = b->flags;
-  a.nstr = plan->d_nstr;
-  a.nlyr = plan->d_nlyr;
-  a.pstride = plan->d_pstride;
-  a.lay_off = plan->d_lay_off;
-  a.mom_off = plan->d_mom_off;
-  a.dtau = b->dtau;
-  a.ssa = b->ssa;
-  a.pmom = b->pmom;
-  a.temper = b->temper;
-  a.scal = b->scal;
-  a.out = b->out;
-  a.total_levels = plan->total_levels;
-  a.total_layers = plan->total_layers;
+  PlanArgs pa;
   int rc;
-  void* p;
-  if ((rc = get_buf(ctx, "inst_d", plan->n * dsa::kInstD * sizeof(double), &p))) return rc;
-  a.inst_d = (double*)p;
-  if ((rc = get_buf(ctx, "inst_i", plan->n * dsa::kInstI * sizeof(int32_t), &p))) return rc;
-  a.inst_i = (int32_t*)p;
-  if ((rc = get_buf(ctx, "lay_d", plan->total_layers * dsa::kLayD * sizeof(double), &p))) return rc;
-  a.lay_d = (double*)p;
-  if ((rc = get_buf(ctx, "lev_d", plan->total_levels * dsa::kLevD * sizeof(double), &p))) return rc;
-  a.lev_d = (double*)p;
-  if ((rc = get_buf(ctx, "eig", plan->max_eig * sizeof(double), &p))) return rc;
-  a.eig = (double*)p;
-  if ((rc = get_buf(ctx, "panel", plan->max_panel * sizeof(double), &p))) return rc;
-  a.panel = (double*)p;
-  if ((rc = get_buf(ctx, "cvec", plan->max_cvec * sizeof(double), &p))) return rc;
-  a.cvec = (double*)p;
-  for (int k = 0; k < 4; ++k) ctx->stage_ms[k] = 0.0;
-  for (const Bucket& bk : plan->buckets) {
-    double* tab = nullptr;
-    if ((rc = table_for(ctx, bk.nstr, &tab))) return rc;
-    a.tab = tab;
-    a.nb = (int32_t)bk.nb;
-    a.b_inst = bk.d_inst;
-    a.b_task0 = bk.d_task0;
-    a.n_tasks = bk.n_tasks;
-    a.t_inst = bk.d_t_inst;
-    a.t_lc = bk.d_t_lc;
-    a.n_lvtasks = bk.n_lv;
-    a.v_bk = bk.d_v_bk;
-    a.v_lu = bk.d_v_lu;
-    hipEvent_t* ev = ctx->timing ? ctx->ev : nullptr;
-    const bool launched = bk.variant == DISORT_V_LANE ? dsa::launch_flux_bucket(bk.nstr, a, ctx->stream, ev)
-                                                      : dsa::launch_flux_bucket_wg(bk.nstr, a, ctx->stream, ev);
-    if (!launched)
-      return fail(ctx, DISORT_E_UNSUPPORTED, "no kernels for num_streams=" + std::to_string(bk.nstr));
-    HIPCHK(ctx, hipGetLastError());
-    if (ctx->timing) {
-      HIPCHK(ctx, hipEventSynchronize(ctx->ev[4]));
-      for (int k = 0; k < 4; ++k) {
-        float ms = 0.f;
-        HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[k], ctx->ev[k + 1]));
-        ctx->stage_ms[k] += ms;
-      }
-    }
-  }
+  if ((rc = prepare(ctx, plan, b, pa))) return rc;
+  return launch_plan(ctx, plan, pa, ctx->timing);
+}
+
+int disort_flux_plan_set_concurrent(disort_ctx* ctx, disort_flux_plan* plan, int on) {
+  if (!plan) return fail(ctx, DISORT_E_INVALID, "null plan");
+  plan->concurrent = on != 0;
   return DISORT_OK;
 }
+
+int disort_flux_execute_graph(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b) {
+  PlanArgs pa;
+  int rc;
+  if ((rc = prepare(ctx, plan, b, pa))) return rc;
+  std::vector<int> variants;
+  for (const Bucket& bk : plan->buckets) variants.push_back(bk.variant);
+  const std::vector<const void*> bufs = scratch_key(ctx, plan, pa);
+  if (!plan->gexec || !same_batch(plan->gkey, *b) || plan->gconcurrent != plan->concurrent || plan->gvariant != variants ||
+      plan->gbufs != bufs) {
+    release_graph(plan);
+    HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    rc = launch_plan(ctx, plan, pa, false);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(ctx, DISORT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    plan->graph = g;
+    HIPCHK(ctx, hipGraphInstantiate(&plan->gexec, g, nullptr, nullptr, 0));
+    plan->gkey = *b;
+    plan->gbufs = bufs;
+    plan->gconcurrent = plan->concurrent;
+    plan->gvariant = variants;
+  }
+  HIPCHK(ctx, hipGraphLaunch(plan->gexec, ctx->stream));
+  return DISORT_OK;
+}
+
+// Autotuner: for every bucket, time each kernel variant that exists for its
+// stream count (median of `reps` runs after one warm-up, hipEvents on the
+// context stream), keep the fastest; then time the whole plan serial versus
+// concurrent and keep the faster schedule.  Runs the real solve on the
+// given device batch (its outputs are overwritten).
+int disort_flux_autotune(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b, int reps) {
+  if (reps < 1) reps = 3;
+  PlanArgs pa;
+  int rc;
+  const bool was_concurrent = plan ? plan->concurrent : false;
+  if (plan) plan->concurrent = false;
+  if ((rc = prepare(ctx, plan, b, pa))) return rc;
+  auto time_it = [&](auto&& enqueue, double* ms_out) -> int {
+    std::vector<float> t;
+    for (int r = 0; r <= reps; ++r) {
+      HIPCHK(ctx, hipEventRecord(ctx->ev[0], ctx->stream));
+      int rc2 = enqueue();
+      if (rc2) return rc2;
+      HIPCHK(ctx, hipEventRecord(ctx->ev[1], ctx->stream));
+      HIPCHK(ctx, hipEventSynchronize(ctx->ev[1]));
+      float ms = 0.f;
+      HIPCHK(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+      if (r > 0) t.push_back(ms);
+    }
+    std::sort(t.begin(), t.end());
+    *ms_out = t[t.size() / 2];
+    return DISORT_OK;
+  };
+  std::string js = "{\"buckets\": [";
+  for (size_t k = 0; k < plan->buckets.size(); ++k) {
+    Bucket& bk = plan->buckets[k];
+    const FluxArgs a = bucket_args(ctx, pa, bk, k);
+    int best = bk.variant;
+    double best_ms = 1e300;
+    for (int v : {DISORT_V_LANE, DISORT_V_WORKGROUP}) {
+      if (!variant_ok(bk.nstr, v)) continue;
+      bk.variant = v;
+      double ms = 0.0;
+      if ((rc = time_it([&]() { return launch_bucket(ctx, a, bk, ctx->stream, nullptr); }, &ms))) return rc;
+      bk.tuned_ms[v] = ms;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+    }
+    bk.variant = best;
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%s{\"nstr\": %d, \"instances\": %lld, \"layers\": %lld, \"lane_ms\": %.6g, \"workgroup_ms\": %.6g, \"chosen\": \"%s\"}",
+                  k ? ", " : "", bk.nstr, (long long)bk.nb, (long long)bk.n_tasks, bk.tuned_ms[DISORT_V_LANE],
+                  bk.tuned_ms[DISORT_V_WORKGROUP], best == DISORT_V_LANE ? "lane" : "workgroup");
+    js += buf;
+  }
+  double serial_ms = 0.0, conc_ms = 0.0;
+  if ((rc = time_it([&]() { return launch_plan(ctx, plan, pa, false); }, &serial_ms))) return rc;
+  plan->concurrent = true;
+  PlanArgs pc;
+  if ((rc = prepare(ctx, plan, b, pc))) return rc;
+  if ((rc = time_it([&]() { return launch_plan(ctx, plan, pc, false); }, &conc_ms))) return rc;
+  plan->concurrent = plan->buckets.size() > 1 && conc_ms < serial_ms;
+  (void)was_concurrent;
+  char buf[256];
+  std::snprintf(buf, sizeof buf, "], \"serial_ms\": %.6g, \"concurrent_ms\": %.6g, \"schedule\": \"%s\"}", serial_ms,
+                conc_ms, plan->concurrent ? "concurrent" : "serial");
+  js += buf;
+  plan->report = js;
+  return DISORT_OK;
+}
+
+const char* disort_flux_plan_report(disort_flux_plan* plan) { return plan ? plan->report.c_str() : ""; }
 
 // Per-instance status codes of the last execute (copied to host).
 int disort_flux_status(disort_ctx* ctx, disort_flux_plan* plan, int32_t* status_host) {

@@ -167,6 +167,33 @@ enum { DISORT_V_DEFAULT = 0, DISORT_V_LANE = 1, DISORT_V_WORKGROUP = 2 };
 int disort_flux_plan_set_variant(disort_ctx* ctx, disort_flux_plan* plan, int nstr, int variant);
 int disort_flux_plan_get_variant(disort_ctx* ctx, disort_flux_plan* plan, int nstr, int* variant);
 
+/*
+ * Bucket schedule of a plan (no counterpart in the reference, whose band loop
+ * is one OpenMP loop per stream count, src/python_bindings.cpp:170-190):
+ * on = 0 runs the buckets one after another on the context stream; on = 1
+ * forks them onto side streams (bucket-private scratch) and joins them back,
+ * so that small buckets overlap large ones.
+ */
+int disort_flux_plan_set_concurrent(disort_ctx* ctx, disort_flux_plan* plan, int on);
+
+/*
+ * disort_flux_execute captured once into a hipGraph (all buckets, all four
+ * kernels each, with the fork/join of a concurrent schedule) and replayed by
+ * one graph launch per call.  The graph is re-captured when the device
+ * pointers of the batch, the schedule or a variant change.  No memset or
+ * allocation is recorded in the graph.
+ */
+int disort_flux_execute_graph(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* device_batch);
+
+/*
+ * Autotuner: times every kernel variant of every bucket and the serial and
+ * concurrent schedules on the given device batch (median of `reps` after a
+ * warm-up; the batch's outputs are overwritten), and keeps the fastest in
+ * the plan.  disort_flux_plan_report returns the measurements as JSON.
+ */
+int disort_flux_autotune(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* device_batch, int reps);
+const char* disort_flux_plan_report(disort_flux_plan* plan);
+
 /* Device memory helpers for callers without their own allocator (ctypes). */
 int disort_device_alloc(disort_ctx* ctx, size_t bytes, void** ptr);
 int disort_device_free(disort_ctx* ctx, void* ptr);

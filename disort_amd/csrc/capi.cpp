@@ -453,6 +453,8 @@ int disort_ctx_destroy(disort_ctx* ctx) {
   return DISORT_OK;
 }
 
+int disort_ctx_device(const disort_ctx* ctx) { return ctx ? ctx->device : -1; }
+
 const char* disort_last_error(const disort_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 void* disort_ctx_stream(disort_ctx* ctx) { return ctx ? (void*)ctx->stream : nullptr; }

@@ -1,0 +1,557 @@
+"""Learned flux surrogate with an error gate (no counterpart in the reference).
+
+A multilayer perceptron (242 -> 512 x 4 -> 183, GELU, BF16 weights and
+activations, FP32 accumulation) maps a column's optical properties to its
+fluxes.  It is trained from random initialisation by the library's own
+forward / backward / Adam kernels (csrc/mlp_kernels.hip) on columns that the
+batched FP64 solver generates, and served behind a gate:
+
+  * out-of-distribution check: every raw feature must lie in the range seen
+    in training;
+  * ensemble spread: two independently trained members; the spread of their
+    net fluxes is the per-column error proxy, and columns above a threshold
+    are solved by plain DISORT;
+  * sampled re-solve: a random fraction of the columns is always solved by
+    plain DISORT.  On a calibration set the threshold is chosen so that, among
+    the columns the gate accepts, at most `alpha` exceed the net-flux error
+    bound; at serving time the sampled columns measure the error online.
+
+Plain DISORT stays the fallback and the judge: every error quoted is against
+the FP64 GPU solver, which is itself checked against the CPU oracle.
+
+Workload (C5): 60 layers x (log10 delta_tau, single_scat_albedo, g,
+layer-mean T) + mu0 + surface albedo = 242 features; outputs flux_up,
+flux_down (diffuse) and flux_direct_beam at 61 levels = 183 values.  Column
+distributions follow the spectral workload (delta_tau = 10^U(-4,1),
+omega ~ U(0,0.999), HG g ~ U(0,0.9), albedo ~ U(0,0.3), mu0 ~ U(0.1,1),
+beam flux pi) in one fixed band, 1000-1001 cm^-1, with a linear temperature
+profile whose end points are drawn per column (top ~ U(180,220) K, surface
+~ U(280,320) K) so that the temperature features carry information.
+Net-flux relative error of a column: max over levels of
+|F_net - F_net,ref| / max(|F_net,ref|, 1e-3 mu0 F0).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi
+from .batch import F_THERMAL, FluxBatch, NSCAL, S_ALBEDO, S_BEAM_FLUX, S_BEAM_MU, scalars
+
+NLYR, NSTR = 60, 16
+NLEV = NLYR + 1
+N_FEAT = 4 * NLYR + 2  # 242
+N_OUT = 3 * NLEV       # 183
+WIDTHS = (256, 512, 512, 512, 512, 192)
+TOL = 1e-3
+C5_SEED = 20261205
+
+
+def _pad64(n: int) -> int:
+    return (n + 63) // 64 * 64
+
+
+# ---------------------------------------------------------------------------
+# Columns
+# ---------------------------------------------------------------------------
+def c5_columns(n: int, seed: int = C5_SEED) -> FluxBatch:
+    """n uniform columns (nlyr=60, nstr=16, thermal, beam) of the C5 workload."""
+    r = np.random.Generator(np.random.Philox(seed))
+    L = NLYR
+    dtau = 10.0 ** r.uniform(-4.0, 1.0, (n, L))
+    ssa = r.uniform(0.0, 0.999, (n, L))
+    g = r.uniform(0.0, 0.9, (n, L))
+    t_top = r.uniform(180.0, 220.0, n)
+    t_bot = r.uniform(280.0, 320.0, n)
+    temper = t_top[:, None] + (t_bot - t_top)[:, None] * np.linspace(0.0, 1.0, L + 1)[None, :]
+    mu0 = r.uniform(0.1, 1.0, n)
+    alb = r.uniform(0.0, 0.3, n)
+    pmom = np.power(g[..., None], np.arange(NSTR + 1, dtype=np.float64))
+    sc = scalars(n, beam_flux=np.pi, beam_mu=mu0, albedo=alb, temp_top=0.0, temp_bot=t_bot, emis_top=1.0,
+                 wn_lo=1000.0, wn_hi=1001.0)
+    return FluxBatch(flags=F_THERMAL, nstr=np.full(n, NSTR, np.int32), nlyr=np.full(n, L, np.int32),
+                     pstride=np.full(n, NSTR + 1, np.int32), dtau=dtau, ssa=ssa, pmom=pmom, scal=sc, temper=temper)
+
+
+def raw_features(b: FluxBatch) -> np.ndarray:
+    """Host copy of the feature map of the device kernel (statistics only)."""
+    n, L = b.n, NLYR
+    dtau = b.dtau.reshape(n, L)
+    f = np.empty((n, N_FEAT))
+    f[:, :L] = np.log10(dtau)
+    f[:, L:2 * L] = b.ssa.reshape(n, L)
+    f[:, 2 * L:3 * L] = b.pmom.reshape(n, L, -1)[:, :, 1]
+    t = b.temper.reshape(n, L + 1)
+    f[:, 3 * L:4 * L] = 0.5 * (t[:, :-1] + t[:, 1:])
+    s = b.scal.reshape(n, NSCAL)
+    f[:, 4 * L] = s[:, S_BEAM_MU]
+    f[:, 4 * L + 1] = s[:, S_ALBEDO]
+    return f
+
+
+# ---------------------------------------------------------------------------
+# Device memory and the network
+# ---------------------------------------------------------------------------
+class DevArray:
+    """A device allocation of the context (freed explicitly or on collection)."""
+
+    def __init__(self, ctx, nbytes: int):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        self.ptr = ctx.alloc(max(self.nbytes, 8))
+
+    @classmethod
+    def of(cls, ctx, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.nbytes)
+        ctx.h2d(d.ptr, a)
+        return d
+
+    def get(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        self.ctx.d2h(out, self.ptr)
+        return out
+
+    def free(self):
+        if self.ptr:
+            self.ctx.free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _mlp_lib():
+    l = _capi.lib()
+    if not getattr(l, "_mlp_bound", False):
+        vp = C.c_void_p
+        l.disort_mlp_create.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int, C.POINTER(vp)]
+        l.disort_mlp_destroy.argtypes = [vp, vp]
+        l.disort_mlp_last_error.argtypes = [vp]
+        l.disort_mlp_last_error.restype = C.c_char_p
+        l.disort_mlp_param_count.argtypes = [vp]
+        l.disort_mlp_param_count.restype = C.c_int64
+        for f in ("disort_mlp_set_params", "disort_mlp_get_params", "disort_mlp_get_grads"):
+            getattr(l, f).argtypes = [vp, vp, vp]
+        l.disort_mlp_forward.argtypes = [vp, vp, vp, C.c_int, vp]
+        l.disort_mlp_backward.argtypes = [vp, vp, vp]
+        l.disort_mlp_adam.argtypes = [vp, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]
+        l.disort_mlp_train_step.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
+                                            C.c_float, C.c_int, vp]
+        l.disort_surrogate_features.argtypes = [vp, C.c_int64, C.c_int64, C.c_int, C.c_int, vp, vp, vp, vp, vp,
+                                                C.c_int, vp, vp]
+        l.disort_surrogate_targets.argtypes = [vp, C.c_int64, C.c_int64, C.c_int, vp, vp, vp, C.c_int, vp]
+        l.disort_surrogate_decode.argtypes = [vp, C.c_int64, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_double,
+                                              C.c_double, vp, vp]
+        l.disort_surrogate_gather.argtypes = [vp, C.c_int64, vp, C.c_int, C.c_int, vp, vp]
+        l.disort_surrogate_scatter.argtypes = [vp, C.c_int64, vp, C.c_int, vp, C.c_int64, vp]
+        l.disort_rows_gather.argtypes = [vp, C.c_int64, vp, C.c_int64, C.c_int, vp, vp]
+        l._mlp_bound = True
+    return l
+
+
+def _ck(ctx, rc, mlp=None):
+    if rc != 0:
+        msg = _mlp_lib().disort_mlp_last_error(mlp).decode() if mlp else _capi.lib().disort_last_error(ctx.ptr).decode()
+        raise _capi.DisortError(rc, msg or "surrogate kernel call failed")
+
+
+class MLP:
+    """Handle of a device network (disort_mlp)."""
+
+    def __init__(self, ctx, widths=WIDTHS, max_batch: int = 4096):
+        self.ctx = ctx
+        self.widths = tuple(int(w) for w in widths)
+        self.max_batch = _pad64(max_batch)
+        arr = (C.c_int * len(self.widths))(*self.widths)
+        self.h = C.c_void_p()
+        _ck(ctx, _mlp_lib().disort_mlp_create(ctx.ptr, len(self.widths) - 1, arr, self.max_batch, C.byref(self.h)))
+        self.n_param = int(_mlp_lib().disort_mlp_param_count(self.h))
+
+    def layout(self):
+        offs, o = [], 0
+        for l in range(len(self.widths) - 1):
+            wo = o
+            o += self.widths[l] * self.widths[l + 1]
+            offs.append((wo, o))
+            o += self.widths[l + 1]
+        return offs
+
+    def init_params(self, seed: int, n_in: int, n_out: int) -> np.ndarray:
+        """LeCun-normal weights on the real (unpadded) rows and columns, zero elsewhere."""
+        r = np.random.Generator(np.random.Philox(seed))
+        p = np.zeros(self.n_param, np.float32)
+        w = self.widths
+        for l, (wo, _) in enumerate(self.layout()):
+            fan_in = n_in if l == 0 else w[l]
+            rows = n_in if l == 0 else w[l]
+            cols = n_out if l == len(w) - 2 else w[l + 1]
+            W = np.zeros((w[l], w[l + 1]), np.float32)
+            W[:rows, :cols] = r.normal(0.0, 1.0 / math.sqrt(fan_in), (rows, cols))
+            p[wo:wo + W.size] = W.reshape(-1)
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_param
+        _ck(self.ctx, _mlp_lib().disort_mlp_set_params(self.ctx.ptr, self.h, p.ctypes.data), self.h)
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.n_param, np.float32)
+        _ck(self.ctx, _mlp_lib().disort_mlp_get_params(self.ctx.ptr, self.h, p.ctypes.data), self.h)
+        return p
+
+    def get_grads(self) -> np.ndarray:
+        g = np.empty(self.n_param, np.float32)
+        _ck(self.ctx, _mlp_lib().disort_mlp_get_grads(self.ctx.ptr, self.h, g.ctypes.data), self.h)
+        return g
+
+    def forward(self, x_ptr: int, batch: int, y_ptr: int):
+        _ck(self.ctx, _mlp_lib().disort_mlp_forward(self.ctx.ptr, self.h, x_ptr, batch, y_ptr), self.h)
+
+    def backward(self, dy_ptr: int):
+        _ck(self.ctx, _mlp_lib().disort_mlp_backward(self.ctx.ptr, self.h, dy_ptr), self.h)
+
+    def adam(self, lr, step, b1=0.9, b2=0.999, eps=1e-8):
+        _ck(self.ctx, _mlp_lib().disort_mlp_adam(self.ctx.ptr, self.h, lr, b1, b2, eps, step), self.h)
+
+    def train_step(self, x_ptr, t_ptr, mask_ptr, batch, n_valid, lr, step, loss_ptr, b1=0.9, b2=0.999, eps=1e-8):
+        _ck(self.ctx, _mlp_lib().disort_mlp_train_step(self.ctx.ptr, self.h, x_ptr, t_ptr, mask_ptr, batch, n_valid,
+                                                       lr, b1, b2, eps, step, loss_ptr), self.h)
+
+    def close(self):
+        if self.h:
+            _mlp_lib().disort_mlp_destroy(self.ctx.ptr, self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# Normalisation, datasets, training
+# ---------------------------------------------------------------------------
+@dataclass
+class Normalizer:
+    f_shift: np.ndarray
+    f_scale: np.ndarray
+    f_lo: np.ndarray
+    f_hi: np.ndarray
+    t_shift: np.ndarray
+    t_scale: np.ndarray
+
+    @classmethod
+    def fit(cls, feats: np.ndarray, targets: np.ndarray, margin: float = 0.0):
+        w0, wl = WIDTHS[0], WIDTHS[-1]
+        fs = np.zeros(w0, np.float32)
+        fc = np.ones(w0, np.float32)
+        lo = np.full(w0, -np.inf, np.float32)
+        hi = np.full(w0, np.inf, np.float32)
+        m, s = feats.mean(0), feats.std(0)
+        fs[:N_FEAT] = m
+        fc[:N_FEAT] = 1.0 / np.where(s > 0, s, 1.0)
+        span = feats.max(0) - feats.min(0)
+        lo[:N_FEAT] = feats.min(0) - margin * span
+        hi[:N_FEAT] = feats.max(0) + margin * span
+        ts = np.zeros(wl, np.float32)
+        tc = np.ones(wl, np.float32)
+        tm, tsd = targets.mean(0), targets.std(0)
+        ts[:N_OUT] = tm
+        tc[:N_OUT] = 1.0 / np.where(tsd > 1e-30, tsd, 1.0)
+        return cls(fs, fc, lo, hi, ts, tc)
+
+    def device(self, ctx):
+        return {k: DevArray.of(ctx, getattr(self, k)) for k in ("f_shift", "f_scale", "f_lo", "f_hi", "t_shift", "t_scale")}
+
+
+def host_fluxes(b: FluxBatch) -> np.ndarray:
+    """[n, 3, nlev] flux_up, flux_down, flux_direct_beam of a solved host batch."""
+    out = b.out.reshape(8, b.n, NLEV)
+    return np.stack([out[2], out[1], out[0]], axis=1)
+
+
+def host_targets(b: FluxBatch) -> np.ndarray:
+    """[n, 183] network targets of a solved host batch (the device kernel's
+    output space before standardisation): diffuse net, diffuse sum and
+    direct beam in units of mu0*F0."""
+    f = host_fluxes(b)
+    s = b.scal.reshape(b.n, NSCAL)
+    u = s[:, S_BEAM_MU] * s[:, S_BEAM_FLUX]
+    u = np.where(u > 1e-300, u, 1.0)[:, None]
+    return np.concatenate([(f[:, 1] - f[:, 0]) / u, (f[:, 1] + f[:, 0]) / u, f[:, 2] / u], axis=1)
+
+
+class DeviceColumns:
+    """A uniform C5 batch resident on the device, with its plan."""
+
+    def __init__(self, ctx, b: FluxBatch):
+        self.ctx, self.n = ctx, b.n
+        self.dev = _capi.DeviceFluxBatch(ctx, b)
+
+    @property
+    def cstruct(self):
+        return self.dev.cstruct
+
+    def solve(self):
+        self.dev.execute()
+
+    def close(self):
+        self.dev.close()
+
+
+def build_dataset(ctx, n: int, seed: int, norm: Normalizer | None = None, chunk: int = 100_000):
+    """Solve n fresh columns on the GPU and keep standardised BF16 features and
+    FP32 targets on the device.  Returns (x, t, n, norm)."""
+    l = _mlp_lib()
+    n_pad = _pad64(n)
+    x = DevArray(ctx, n_pad * WIDTHS[0] * 2)
+    t = DevArray(ctx, n_pad * WIDTHS[-1] * 4)
+    dn = None
+    done = 0
+    k = 0
+    while done < n:
+        m = min(chunk, n - done)
+        hb = c5_columns(m, seed=seed * 1000 + k)
+        dc = DeviceColumns(ctx, hb)
+        dc.solve()
+        if norm is None:
+            out = dc.dev.download()
+            norm = Normalizer.fit(raw_features(hb), host_targets(out), margin=0.0)
+        if dn is None:
+            dn = norm.device(ctx)
+        m_pad = m if done + m < n else n_pad - done
+        _ck(ctx, l.disort_surrogate_features(ctx.ptr, m, m_pad, NLYR, NSTR + 1, C.byref(dc.cstruct),
+                                             dn["f_shift"].ptr, dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr,
+                                             WIDTHS[0], x.ptr + done * WIDTHS[0] * 2, None))
+        _ck(ctx, l.disort_surrogate_targets(ctx.ptr, m, m_pad, NLEV, C.byref(dc.cstruct), dn["t_shift"].ptr,
+                                            dn["t_scale"].ptr, WIDTHS[-1], t.ptr + done * WIDTHS[-1] * 4))
+        ctx.sync()
+        dc.close()
+        done += m
+        k += 1
+    return x, t, n, norm
+
+
+def train(ctx, mlp: MLP, x: DevArray, t: DevArray, n: int, epochs: int, batch: int = 2048, lr: float = 1e-3,
+          lr_min: float = 1e-5, seed: int = 0, n_out: int | None = None, log=None):
+    """Adam with a cosine learning-rate schedule over shuffled minibatches.
+    x: [n x w_0] BF16 rows, t: [n x w_L] FP32 rows (the network's widths);
+    the first n_out outputs enter the loss (default: all of them, or the
+    183 flux outputs for the surrogate shape)."""
+    l = _mlp_lib()
+    w_in, w_out = mlp.widths[0], mlp.widths[-1]
+    if x.nbytes < n * w_in * 2 or t.nbytes < n * w_out * 4:
+        raise ValueError("training arrays are smaller than n rows of the network's widths")
+    if n_out is None:
+        n_out = N_OUT if mlp.widths == WIDTHS else w_out
+    B = _pad64(batch)
+    if B > mlp.max_batch:
+        raise ValueError("batch exceeds the network's max_batch")
+    steps_per_epoch = n // B
+    total = epochs * steps_per_epoch
+    mask = np.zeros(w_out, np.float32)
+    mask[:n_out] = 1.0
+    dmask = DevArray.of(ctx, mask)
+    xb = DevArray(ctx, B * w_in * 2)
+    tb = DevArray(ctx, B * w_out * 4)
+    loss = DevArray(ctx, 4)
+    idx = DevArray(ctx, n * 4)
+    r = np.random.Generator(np.random.Philox(seed))
+    hist = []
+    step = 0
+    for ep in range(epochs):
+        ctx.h2d(idx.ptr, r.permutation(n).astype(np.int32))
+        ctx.h2d(loss.ptr, np.zeros(1, np.float32))
+        for s in range(steps_per_epoch):
+            ip = idx.ptr + s * B * 4
+            _ck(ctx, l.disort_rows_gather(ctx.ptr, B, ip, w_in, 2, x.ptr, xb.ptr))
+            _ck(ctx, l.disort_rows_gather(ctx.ptr, B, ip, w_out, 4, t.ptr, tb.ptr))
+            step += 1
+            cur = lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * (step - 1) / max(total, 1)))
+            mlp.train_step(xb.ptr, tb.ptr, dmask.ptr, B, n_out, cur, step, loss.ptr)
+        ep_loss = float(loss.get(np.float32, 1)[0]) / max(steps_per_epoch, 1)
+        hist.append(ep_loss)
+        if log:
+            log(f"epoch {ep + 1}/{epochs}: train loss {ep_loss:.4e}")
+    for a in (dmask, xb, tb, loss, idx):
+        a.free()
+    return hist
+
+
+def net_flux_error(flux: np.ndarray, ref: np.ndarray, mu0f0: np.ndarray, floor: float = 1e-3) -> np.ndarray:
+    """Per-column max over levels of |dF_net| / max(|F_net,ref|, floor mu0 F0);
+    flux, ref: [n, 3, nlev] (up, down, direct)."""
+    net = flux[:, 2] + flux[:, 1] - flux[:, 0]
+    rnet = ref[:, 2] + ref[:, 1] - ref[:, 0]
+    den = np.maximum(np.abs(rnet), floor * mu0f0[:, None])
+    return np.max(np.abs(net - rnet) / den, axis=1)
+
+
+def net_flux_error_beam(flux: np.ndarray, ref: np.ndarray, mu0f0: np.ndarray) -> np.ndarray:
+    """Per-column max over levels of |dF_net| / (mu0 F0): the error as a
+    fraction of the incident beam (reported beside the gate's metric)."""
+    net = flux[:, 2] + flux[:, 1] - flux[:, 0]
+    rnet = ref[:, 2] + ref[:, 1] - ref[:, 0]
+    return np.max(np.abs(net - rnet), axis=1) / mu0f0
+
+
+# ---------------------------------------------------------------------------
+# Gated serving
+# ---------------------------------------------------------------------------
+# Error metrics of the gate: (c_local, c_floor) of
+# max_levels |dF_net| / max(c_local |F_net|, c_floor mu0 F0).
+METRICS = {"local": (1.0, 1e-3), "beam": (0.0, 1.0)}
+
+
+def column_error(flux, ref, mu0f0, metric: str):
+    return net_flux_error(flux, ref, mu0f0) if metric == "local" else net_flux_error_beam(flux, ref, mu0f0)
+
+
+class GatedSurrogate:
+    """Ensemble of trained members behind the OOD + spread + sampled re-solve gate.
+
+    metric "local": |dF_net| / max(|F_net|, 1e-3 mu0 F0) (relative to the
+    local net flux); "beam": |dF_net| / (mu0 F0) (relative to the incident
+    beam).  tol is the bound the gate enforces, alpha the tolerated fraction
+    of accepted columns above it."""
+
+    def __init__(self, ctx, members, norm: Normalizer, tau: float = 0.0, alpha: float = 0.01, tol: float = TOL,
+                 metric: str = "local"):
+        self.ctx, self.members, self.norm = ctx, list(members), norm
+        self.dn = norm.device(ctx)
+        self.tau, self.alpha, self.tol, self.metric = tau, alpha, tol, metric
+
+    def predict(self, dc: DeviceColumns):
+        """Device fluxes [n, 3, nlev] (FP64), spread [n] and OOD flags [n] for a resident batch."""
+        ctx, l, n = self.ctx, _mlp_lib(), dc.n
+        n_pad = _pad64(n)
+        x = DevArray(ctx, n_pad * WIDTHS[0] * 2)
+        ood = DevArray(ctx, n * 4)
+        ys = [DevArray(ctx, n_pad * WIDTHS[-1] * 4) for _ in self.members]
+        flux = DevArray(ctx, n * 3 * NLEV * 8)
+        spread = DevArray(ctx, n * 4)
+        _ck(ctx, l.disort_surrogate_features(ctx.ptr, n, n_pad, NLYR, NSTR + 1, C.byref(dc.cstruct),
+                                             self.dn["f_shift"].ptr, self.dn["f_scale"].ptr, self.dn["f_lo"].ptr,
+                                             self.dn["f_hi"].ptr, WIDTHS[0], x.ptr, ood.ptr))
+        for m, y in zip(self.members, ys):
+            for b0 in range(0, n_pad, m.max_batch):
+                bb = min(m.max_batch, n_pad - b0)
+                m.forward(x.ptr + b0 * WIDTHS[0] * 2, bb, y.ptr + b0 * WIDTHS[-1] * 4)
+        cl, cf = METRICS[self.metric]
+        _ck(ctx, l.disort_surrogate_decode(ctx.ptr, n, NLYR, WIDTHS[-1], ys[0].ptr, ys[1].ptr if len(ys) > 1 else None,
+                                           self.dn["t_shift"].ptr, self.dn["t_scale"].ptr, C.byref(dc.cstruct),
+                                           cl, cf, flux.ptr, spread.ptr))
+        for a in [x] + ys:
+            a.free()
+        return flux, spread, ood
+
+    def calibrate(self, dc: DeviceColumns, ref_flux: np.ndarray):
+        """Threshold on the spread so that, among accepted calibration columns,
+        at most alpha exceed the tolerance.  ref_flux: [n, 3, nlev] plain solve."""
+        flux, spread, ood = self.predict(dc)
+        self.ctx.sync()
+        f = flux.get(np.float64, (dc.n, 3, NLEV))
+        sp = spread.get(np.float32, dc.n)
+        od = ood.get(np.int32, dc.n).astype(bool)
+        for a in (flux, spread, ood):
+            a.free()
+        sc = dc.dev.host.scal.reshape(dc.n, NSCAL)
+        u = sc[:, S_BEAM_MU] * sc[:, S_BEAM_FLUX]
+        erra = net_flux_error(f, ref_flux, u)
+        errb = net_flux_error_beam(f, ref_flux, u)
+        err = erra if self.metric == "local" else errb
+        ok = ~od
+        order = np.argsort(sp[ok])
+        bad = (err[ok][order] > self.tol).astype(np.float64)
+        frac_bad = np.cumsum(bad) / np.arange(1, bad.size + 1)
+        good = np.nonzero(frac_bad <= self.alpha)[0]
+        self.tau = float(sp[ok][order][good[-1]]) if good.size else 0.0
+        acc = ok & (sp <= self.tau)
+        return {"tau": self.tau, "accept_frac": float(acc.mean()),
+                "err_median_all": float(np.median(erra)), "err_p90_all": float(np.quantile(erra, 0.9)),
+                "err_beam_median": float(np.median(errb)), "err_beam_p90": float(np.quantile(errb, 0.9)),
+                "viol_frac_accepted": float((err[acc] > self.tol).mean()) if acc.any() else 0.0,
+                "viol_frac_all": float((err > self.tol).mean())}
+
+    def serve(self, dc: DeviceColumns, sample_frac: float, seed: int = 0):
+        """Gated fluxes for a resident batch: surrogate where the gate accepts,
+        plain DISORT for OOD / high-spread columns and for a random sample.
+        Returns (device flux [n,3,nlev], info)."""
+        ctx, l, n = self.ctx, _mlp_lib(), dc.n
+        flux, spread, ood = self.predict(dc)
+        sp = spread.get(np.float32, n)
+        od = ood.get(np.int32, n).astype(bool)
+        r = np.random.Generator(np.random.Philox(seed))
+        sampled = r.random(n) < sample_frac
+        fail = od | (sp > self.tau)
+        route = np.nonzero(fail | sampled)[0].astype(np.int32)
+        m = int(route.size)
+        if m:
+            didx = DevArray.of(ctx, route)
+            sub = _SubBatch(ctx, m)
+            _ck(ctx, l.disort_surrogate_gather(ctx.ptr, m, didx.ptr, NLYR, NSTR + 1, C.byref(dc.cstruct),
+                                               C.byref(sub.cstruct)))
+            sub.solve()
+            _ck(ctx, l.disort_surrogate_scatter(ctx.ptr, m, didx.ptr, NLEV, sub.cstruct.out, sub.cstruct.total_levels,
+                                                flux.ptr))
+            ctx.sync()
+            sub.close()
+            didx.free()
+        spread.free()
+        ood.free()
+        return flux, {"n": n, "n_ood": int(od.sum()), "n_spread_fail": int((sp > self.tau).sum()),
+                      "n_sampled": int(sampled.sum()), "n_plain": m, "sampled_idx": np.nonzero(sampled)[0],
+                      "accepted_idx": np.nonzero(~(fail | sampled))[0]}
+
+
+class _SubBatch:
+    """Device arrays and plan for m gathered columns (uniform C5 shape)."""
+
+    def __init__(self, ctx, m: int):
+        self.ctx, self.m = ctx, m
+        L, P = NLYR, NSTR + 1
+        self.arrs = [DevArray(ctx, m * L * 8), DevArray(ctx, m * L * 8), DevArray(ctx, m * L * P * 8),
+                     DevArray(ctx, m * (L + 1) * 8), DevArray(ctx, m * NSCAL * 8), DevArray(ctx, 8 * m * (L + 1) * 8)]
+        from .batch import CFluxBatch
+
+        nstr = np.full(m, NSTR, np.int32)
+        nlyr = np.full(m, L, np.int32)
+        pst = np.full(m, P, np.int32)
+        self.plan = C.c_void_p()
+        lib = _capi.lib()
+        rc = lib.disort_flux_plan_create(ctx.ptr, m, nstr.ctypes.data, nlyr.ctypes.data, pst.ctypes.data, F_THERMAL,
+                                         C.byref(self.plan))
+        _ck(ctx, rc)
+        s = CFluxBatch()
+        s.n, s.flags = m, F_THERMAL
+        s.dtau, s.ssa, s.pmom, s.temper, s.scal, s.out = (a.ptr for a in self.arrs)
+        s.total_levels = m * (L + 1)
+        self.cstruct = s
+
+    def solve(self):
+        _ck(self.ctx, _capi.lib().disort_flux_execute(self.ctx.ptr, self.plan, C.byref(self.cstruct)))
+
+    def close(self):
+        if self.plan:
+            _capi.lib().disort_flux_plan_destroy(self.ctx.ptr, self.plan)
+            self.plan = C.c_void_p()
+        for a in self.arrs:
+            a.free()
+
+
+def timed(ctx, fn):
+    ctx.sync()
+    t0 = time.perf_counter()
+    res = fn()
+    ctx.sync()
+    return res, time.perf_counter() - t0

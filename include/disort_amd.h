@@ -117,6 +117,7 @@ typedef struct disort_ctx disort_ctx;
 
 /* Context: one device, one stream, grow-only device scratch. */
 int disort_ctx_create(int device, disort_ctx** ctx);
+int disort_ctx_device(const disort_ctx* ctx);
 int disort_ctx_destroy(disort_ctx* ctx);
 const char* disort_last_error(const disort_ctx* ctx);
 void* disort_ctx_stream(disort_ctx* ctx); /* the hipStream_t the kernels run on */
@@ -219,6 +220,68 @@ const char* disort_version(void);
  * mirrored.  Host-only; needs no context or GPU.
  */
 int disort_quadrature(int nstr, double* cmu, double* cwt);
+
+
+/* =========================================================================
+ * Learned flux surrogate (no counterpart in the reference).  A multilayer
+ * perceptron with BF16 weights and activations and FP32 accumulation on the
+ * gfx950 matrix cores, trained by the library's own forward, backward and
+ * Adam kernels on columns the batched solver generates; plain DISORT stays
+ * the fallback and the judge of its error (disort_amd/surrogate.py holds
+ * the training loop and the gate).  All array pointers are device pointers.
+ * ========================================================================= */
+typedef struct disort_mlp disort_mlp;
+
+/* widths[0..n_layers]: layer widths, multiples of 64 (pad inputs/outputs
+ * with zero columns); max_batch a multiple of 64.  Hidden layers use GELU
+ * (tanh form), the last layer is linear.  Parameters are packed per layer
+ * as W_l [widths[l] x widths[l+1]] row-major, then b_l [widths[l+1]], FP32. */
+int disort_mlp_create(disort_ctx* ctx, int n_layers, const int* widths, int max_batch, disort_mlp** mlp);
+int disort_mlp_destroy(disort_ctx* ctx, disort_mlp* mlp);
+const char* disort_mlp_last_error(const disort_mlp* mlp);
+int64_t disort_mlp_param_count(const disort_mlp* mlp);
+int disort_mlp_set_params(disort_ctx* ctx, disort_mlp* mlp, const float* host_params); /* also resets Adam state */
+int disort_mlp_get_params(disort_ctx* ctx, disort_mlp* mlp, float* host_params);
+int disort_mlp_get_grads(disort_ctx* ctx, disort_mlp* mlp, float* host_grads);
+/* y [B x widths[L]] FP32 = network(x [B x widths[0]] BF16 bits); B a multiple of 64. */
+int disort_mlp_forward(disort_ctx* ctx, disort_mlp* mlp, const uint16_t* x, int batch, float* y);
+/* Parameter gradients for output gradient dy [B x widths[L]] (BF16 bits) of the last forward. */
+int disort_mlp_backward(disort_ctx* ctx, disort_mlp* mlp, const uint16_t* dy);
+int disort_mlp_adam(disort_ctx* ctx, disort_mlp* mlp, float lr, float beta1, float beta2, float eps, int step);
+/* forward, loss = sum mask (y - t)^2 / (B * n_valid) added to *loss_dev, backward, Adam. */
+int disort_mlp_train_step(disort_ctx* ctx, disort_mlp* mlp, const uint16_t* x, const float* targets,
+                          const float* mask, int batch, int n_valid, float lr, float beta1, float beta2, float eps,
+                          int step, float* loss_dev);
+
+/* Surrogate column kernels on a uniform device batch (n columns of nlyr
+ * layers, lay_off[i] = i*nlyr).  Features: 4*nlyr+2 per column (log10 dtau,
+ * albedo, g = pmom[1], layer-mean temperature per layer; mu0; surface
+ * albedo), standardised as (f - shift) * scale into BF16 rows of width w0
+ * (rows n..n_pad-1 zero); ood[i] = 1 when a raw feature leaves [lo, hi]. */
+int disort_surrogate_features(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlyr, int pstride,
+                              const disort_flux_batch* batch, const float* shift, const float* scale,
+                              const float* lo, const float* hi, int w0, uint16_t* x, int32_t* ood);
+/* Standardised targets in units of the incident beam mu0*F0 at nlev levels:
+ * diffuse net (down - up), diffuse sum (down + up), direct beam; from the
+ * solved batch's outputs (batch->out). */
+int disort_surrogate_targets(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlev, const disort_flux_batch* batch,
+                             const float* shift, const float* scale, int wL, float* targets);
+/* Network outputs of one or two ensemble members -> fluxes [n][3][nlev]
+ * (up, down, direct; the direct beam in closed form) and the members'
+ * net-flux spread per column: max over levels of |dFnet| divided by
+ * max(c_local |Fnet|, c_floor mu0 F0). */
+int disort_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const float* y1, const float* y2,
+                            const float* shift, const float* scale, const disort_flux_batch* batch, double c_local,
+                            double c_floor, double* flux, float* spread);
+/* Route columns idx[0..m) of src into the compact batch dst, and the plain
+ * solver's outputs for them back into the flux array. */
+int disort_surrogate_gather(disort_ctx* ctx, int64_t m, const int32_t* idx, int nlyr, int pstride,
+                            const disort_flux_batch* src, disort_flux_batch* dst);
+int disort_surrogate_scatter(disort_ctx* ctx, int64_t m, const int32_t* idx, int nlev, const double* out,
+                             int64_t total_levels, double* flux);
+/* dst row k <- src row idx[k] of a row-major matrix of 2- or 4-byte elements. */
+int disort_rows_gather(disort_ctx* ctx, int64_t m, const int32_t* idx, int64_t row_elems, int elem_bytes,
+                       const void* src, void* dst);
 
 #ifdef __cplusplus
 }

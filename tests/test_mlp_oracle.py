@@ -1,0 +1,81 @@
+"""Pins of the CPU oracle of the surrogate network (oracle/mlp_oracle.py)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+
+def test_bf16_rounding():
+    x = np.array([1.0, 1.00390625, 1.005859375, 1.0078125, -3.14159265, 0.0, 65504.0], np.float32)
+    r = mo.bf16_round(x)
+    # 1 + 2^-8 is a tie between 1 and 1 + 2^-7: ties to even -> 1.
+    assert r[1] == 1.0
+    # 1 + 1.5 * 2^-8 rounds up to 1 + 2^-7.
+    assert r[2] == np.float32(1.0078125)
+    assert r[3] == np.float32(1.0078125)
+    assert r[4] == np.float32(-3.140625)
+    assert r[5] == 0.0
+    assert np.all(np.abs(r - x) <= np.abs(x) * 2.0 ** -8)
+    b = mo.bf16_bits(np.array([1.0, -2.0], np.float32))
+    assert list(b) == [0x3F80, 0xC000]
+
+
+def test_gelu_closed_forms():
+    assert mo.gelu(0.0) == 0.0
+    z = np.linspace(-6, 6, 241)
+    # GELU(z) - GELU(-z) = z for the tanh form.
+    np.testing.assert_allclose(mo.gelu(z) - mo.gelu(-z), z, atol=1e-12)
+    h = 1e-6
+    fd = (mo.gelu(z + h) - mo.gelu(z - h)) / (2 * h)
+    np.testing.assert_allclose(mo.gelu_grad(z), fd, atol=1e-8)
+    assert mo.gelu_grad(0.0) == 0.5
+
+
+def test_backward_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    widths = [6, 5, 7, 4]
+    _, n = mo.layout(widths)
+    p = rng.normal(0, 0.5, n)
+    x = rng.normal(0, 1, (3, 6))
+    t = rng.normal(0, 1, (3, 4))
+    mask = np.array([1.0, 1.0, 0.0, 1.0])
+
+    def loss(q):
+        y, _ = mo.forward(q, widths, x, bf16=False)
+        return mo.mse(y, t, mask, 3)[0]
+
+    y, cache = mo.forward(p, widths, x, bf16=False)
+    _, dy = mo.mse(y, t, mask, 3)
+    g = mo.backward(p, widths, cache, dy, bf16=False)
+    h = 1e-6
+    fd = np.array([(loss(p + h * e) - loss(p - h * e)) / (2 * h) for e in np.eye(n)])
+    np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-9)
+
+
+def test_adam_recurrence():
+    p, m, v = np.float32([1.0]), np.float32([0.0]), np.float32([0.0])
+    g = np.float32([0.5])
+    p1, m1, v1 = mo.adam(p, g, m, v, 1e-2, 0.9, 0.999, 1e-8, 1)
+    # First step: m_hat = g, v_hat = g^2, so the step is lr * sign(g).
+    assert p1[0] == pytest.approx(1.0 - 1e-2, abs=1e-7)
+    assert m1[0] == pytest.approx(0.05) and v1[0] == pytest.approx(0.00025)
+    p2, _, _ = mo.adam(p1, g, m1, v1, 1e-2, 0.9, 0.999, 1e-8, 2)
+    assert p2[0] == pytest.approx(1.0 - 2e-2, abs=2e-7)
+
+
+def test_features_layout():
+    n, L = 2, 3
+    dtau = np.array([[1e-4, 1.0, 10.0], [0.1, 0.2, 0.3]])
+    ssa = np.full((n, L), 0.5)
+    pmom = np.zeros((n, L, 5))
+    pmom[:, :, 1] = 0.7
+    temper = np.tile(np.array([200.0, 210.0, 230.0, 260.0]), (n, 1))
+    f = mo.features(dtau, ssa, pmom, temper, np.array([0.3, 0.4]), np.array([0.1, 0.2]))
+    assert f.shape == (2, 14)
+    np.testing.assert_allclose(f[0, :3], [-4.0, 0.0, 1.0])
+    np.testing.assert_allclose(f[0, 9:12], [205.0, 220.0, 245.0])
+    assert f[1, 12] == 0.4 and f[1, 13] == 0.2

@@ -186,7 +186,7 @@ class DeviceFluxBatch:
         s.ssa = up(batch.ssa)
         s.pmom = up(batch.pmom)
         s.temper = up(batch.temper)
-        s.alt = None
+        s.alt = up(batch.alt)
         s.scal = up(batch.scal)
         self.out_ptr = ctx.alloc(batch.out.nbytes)
         self._bufs.append(self.out_ptr)

@@ -155,11 +155,13 @@ def uniform_batch(
     scal: np.ndarray,
     flags: int = 0,
     temper: np.ndarray | None = None,
+    alt: np.ndarray | None = None,
 ) -> FluxBatch:
     """Batch of n instances sharing nlyr and moment count.
 
     dtau, ssa: [n, L]; pmom: [n, L, P] with P >= nstr+1; scal: [n, NSCAL];
-    temper: [n, L+1] when thermal.
+    temper: [n, L+1] when thermal; alt: [n, L+1] level altitudes (top
+    first) with the spherical beam.
     """
     dtau = np.asarray(dtau, dtype=np.float64)
     n, L = dtau.shape
@@ -175,6 +177,7 @@ def uniform_batch(
         pmom=pmom,
         scal=np.asarray(scal, dtype=np.float64),
         temper=None if temper is None else np.asarray(temper, dtype=np.float64),
+        alt=None if alt is None else np.asarray(alt, dtype=np.float64),
     )
 
 

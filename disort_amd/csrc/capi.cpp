@@ -151,7 +151,7 @@ int table_for(disort_ctx* ctx, int nstr, double** out) {
 }
 
 bool lane_kernels(int n) { return n >= 4 && n <= 16 && n % 2 == 0; }
-bool wg_kernels(int n) { return n == 8 || n == 16 || n == 32 || n == 64; }
+bool wg_kernels(int n) { return (n >= 4 && n <= 16 && n % 2 == 0) || n == 32 || n == 64; }
 bool gpu_supported_nstr(int n) { return lane_kernels(n) || wg_kernels(n); }
 int default_variant(int n) { return lane_kernels(n) ? DISORT_V_LANE : DISORT_V_WORKGROUP; }
 bool variant_ok(int n, int v) { return (v == DISORT_V_LANE && lane_kernels(n)) || (v == DISORT_V_WORKGROUP && wg_kernels(n)); }
@@ -188,6 +188,10 @@ std::string validate_instance(const disort_flux_batch* b, int64_t i) {
     if (s[DISORT_S_TEMP_TOP] < 0.0 || s[DISORT_S_TEMP_BOT] < 0.0)
       return at("temperature_top and temperature_bottom must be >= 0");
     if (s[DISORT_S_EMIS_TOP] < 0.0 || s[DISORT_S_EMIS_TOP] > 1.0) return at("emissivity_top must be in [0, 1]");
+  }
+  if (b->flags & DISORT_F_SPHERICAL) {
+    if (s[DISORT_S_RADIUS] <= 0.0) return at("bottom_radius must be > 0 when use_spherical_beam is true");
+    if (!b->alt) return at("level_altitudes size < num_layers + 1");
   }
   return std::string();
 }
@@ -262,8 +266,10 @@ int prepare(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b,
   if (!ctx || !plan || !b) return fail(ctx, DISORT_E_INVALID, "null argument");
   if (b->n != plan->n) return fail(ctx, DISORT_E_INVALID, "batch size differs from the plan");
   if ((b->flags & DISORT_F_THERMAL) && !b->temper) return fail(ctx, DISORT_E_INVALID, "thermal batch without temperatures");
-  if (b->flags & DISORT_F_SPHERICAL)
-    return fail(ctx, DISORT_E_UNSUPPORTED, "use_spherical_beam is not supported by the GPU flux path");
+  if ((b->flags & DISORT_F_SPHERICAL) && !b->alt)
+    return fail(ctx, DISORT_E_INVALID, "spherical batch without level altitudes");
+  if ((b->flags & DISORT_F_SPHERICAL) != (plan->flags & DISORT_F_SPHERICAL))
+    return fail(ctx, DISORT_E_INVALID, "the batch's spherical flag differs from the plan's");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   FluxArgs& a = pa.base;
   a = FluxArgs{};
@@ -277,6 +283,7 @@ int prepare(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b,
   a.ssa = b->ssa;
   a.pmom = b->pmom;
   a.temper = b->temper;
+  a.alt = b->alt;
   a.scal = b->scal;
   a.out = b->out;
   a.total_levels = plan->total_levels;
@@ -515,6 +522,8 @@ int disort_flux_plan_set_variant(disort_ctx* ctx, disort_flux_plan* plan, int ns
   for (Bucket& bk : plan->buckets)
     if (bk.nstr == nstr) {
       if (variant == DISORT_V_DEFAULT) variant = default_variant(nstr);
+      if ((plan->flags & DISORT_F_SPHERICAL) && variant != DISORT_V_WORKGROUP)
+        return fail(ctx, DISORT_E_UNSUPPORTED, "the spherical beam needs the workgroup kernels");
       if (!variant_ok(nstr, variant))
         return fail(ctx, DISORT_E_UNSUPPORTED,
                     "kernel variant " + std::to_string(variant) + " has no kernels for num_streams=" + std::to_string(nstr));
@@ -537,8 +546,6 @@ int disort_flux_plan_get_variant(disort_ctx* ctx, disort_flux_plan* plan, int ns
 int disort_flux_plan_create(disort_ctx* ctx, int64_t n, const int32_t* nstr, const int32_t* nlyr,
                             const int32_t* pstride, int32_t flags, disort_flux_plan** out) {
   if (!ctx || !out || n <= 0 || !nstr || !nlyr || !pstride) return fail(ctx, DISORT_E_INVALID, "invalid plan arguments");
-  if (flags & DISORT_F_SPHERICAL)
-    return fail(ctx, DISORT_E_UNSUPPORTED, "use_spherical_beam is not supported by the GPU flux path");
   HIPCHK(ctx, hipSetDevice(ctx->device));
   auto plan = std::make_unique<disort_flux_plan>();
   plan->n = n;
@@ -580,7 +587,8 @@ int disort_flux_plan_create(disort_ctx* ctx, int64_t n, const int32_t* nstr, con
   for (auto& kv : groups) {
     Bucket bk;
     bk.nstr = kv.first;
-    bk.variant = default_variant(bk.nstr);
+    // The pseudo-spherical beam source is built by the workgroup kernels only.
+    bk.variant = (flags & DISORT_F_SPHERICAL) ? DISORT_V_WORKGROUP : default_variant(bk.nstr);
     const auto& inst = kv.second;
     bk.nb = (int64_t)inst.size();
     std::vector<int64_t> task0(bk.nb);
@@ -696,6 +704,7 @@ int disort_flux_autotune(disort_ctx* ctx, disort_flux_plan* plan, const disort_f
     double best_ms = 1e300;
     for (int v : {DISORT_V_LANE, DISORT_V_WORKGROUP}) {
       if (!variant_ok(bk.nstr, v)) continue;
+      if ((plan->flags & DISORT_F_SPHERICAL) && v != DISORT_V_WORKGROUP) continue;
       bk.variant = v;
       double ms = 0.0;
       if ((rc = time_it([&]() { return launch_bucket(ctx, a, bk, ctx->stream, nullptr); }, &ms))) return rc;
@@ -773,7 +782,9 @@ int disort_flux_solve(disort_ctx* ctx, const disort_flux_batch* b, int memspace)
                          {"in_ssa", b->ssa, (size_t)TL * 8, &p_ssa},
                          {"in_pmom", b->pmom, (size_t)mom_total * 8, &p_pmom},
                          {"in_scal", b->scal, (size_t)b->n * DISORT_NSCAL * 8, &p_scal}};
+  void* p_alt = nullptr;
   if (b->flags & DISORT_F_THERMAL) ups.push_back({"in_temp", b->temper, (size_t)TV * 8, &p_temp});
+  if (b->flags & DISORT_F_SPHERICAL) ups.push_back({"in_alt", b->alt, (size_t)TV * 8, &p_alt});
   for (auto& u : ups) {
     if ((rc = get_buf(ctx, u.name, u.bytes, u.dst))) return rc;
     HIPCHK(ctx, hipMemcpyAsync(*u.dst, u.src, u.bytes, hipMemcpyHostToDevice, ctx->stream));
@@ -783,6 +794,7 @@ int disort_flux_solve(disort_ctx* ctx, const disort_flux_batch* b, int memspace)
   d.ssa = (const double*)p_ssa;
   d.pmom = (const double*)p_pmom;
   d.temper = (const double*)p_temp;
+  d.alt = (const double*)p_alt;
   d.scal = (const double*)p_scal;
   d.out = (double*)p_out;
   d.total_levels = TV;

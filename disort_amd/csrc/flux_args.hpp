@@ -17,6 +17,7 @@ struct FluxArgs {
   const double* ssa;
   const double* pmom;
   const double* temper;
+  const double* alt;  // level altitudes (spherical beam)
   const double* scal;
   double* out;
   int64_t total_levels;

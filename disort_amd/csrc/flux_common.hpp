@@ -105,6 +105,44 @@ __device__ __forceinline__ void static_for(F&& f) {
 
 __host__ __device__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }  // i >= j
 
+// Slant optical depth of the pseudo-spherical beam from the top down to
+// fraction tau_frac above level lc of layer lc (1-based), through layers
+// of thickness dtau between level altitudes zd (top first) on a sphere of
+// radius r (src/FluxSolver.cpp:1503-1566).
+__device__ double chapman(int lc, double tau_frac, const double* zd, const double* dtau, int nlyr, double r,
+                          double mu0) {
+  const double deg = kPi / 180.0;
+  const double zenang = acos(mu0) * 180.0 / kPi;
+  const double zenrad = zenang * deg;
+  const double xp = r + zd[lc] + (zd[lc - 1] - zd[lc]) * tau_frac;
+  const double xpsinz = xp * sin(zenrad);
+  if (zenang > 90.0 && xpsinz < r) return 1.0e+20;
+  int id = lc;
+  if (zenang > 90.0)
+    for (int j = lc; j <= nlyr; ++j)
+      if (xpsinz < (zd[j - 1] + r) && xpsinz >= (zd[j] + r)) id = j;
+  double sum = 0.0;
+  for (int j = 1; j <= id; ++j) {
+    double fact = 1.0, fact2 = 1.0;
+    if (j > lc) fact = 2.0;
+    else if (j == lc && lc == id && zenang > 90.0) fact2 = -1.0;
+    const double rj = r + zd[j - 1];
+    double rjp1 = r + zd[j];
+    if (j == lc && id == lc) rjp1 = xp;
+    const double dhj = zd[j - 1] - zd[j];
+    double dsj;
+    if (id > lc && j == id) dsj = sqrt(rj * rj - xpsinz * xpsinz);
+    else dsj = sqrt(rj * rj - xpsinz * xpsinz) - fact2 * sqrt(rjp1 * rjp1 - xpsinz * xpsinz);
+    sum += dtau[j - 1] * fact * dsj / dhj;
+  }
+  if (id > lc) {
+    const double dhj = zd[lc - 1] - zd[lc];
+    const double dsj = sqrt(xp * xp - xpsinz * xpsinz) - sqrt((zd[lc] + r) * (zd[lc] + r) - xpsinz * xpsinz);
+    sum += dtau[lc - 1] * dsj / dhj;
+  }
+  return sum;
+}
+
 // ===========================================================================
 // k_column: per-instance set-up (one wavefront per instance).
 // ===========================================================================
@@ -165,8 +203,11 @@ __global__ __launch_bounds__(64) void k_column(FluxArgs a) {
   __syncthreads();
 
   const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
+  const bool spherical = (a.flags & DISORT_F_SPHERICAL) != 0 && F0 > 0.0;
+  __shared__ int s_ncut;
+  __shared__ double s_abst;
   if (lane == 0) {
-    // Prefix sums, absorption cut-off and beam transmission (:311-331, :392-435).
+    // Prefix sums and absorption cut-off (:311-331, :428-435).
     double taus = 0.0, tauc = 0.0, abst = 0.0;
     int ncut = L;
     a.lev_d[VD_TAU_S * TV + vo] = 0.0;
@@ -181,15 +222,46 @@ __global__ __launch_bounds__(64) void k_column(FluxArgs a) {
       a.lev_d[VD_TAU_S * TV + vo + lc + 1] = taus;
       a.lev_d[VD_TAUC * TV + vo + lc + 1] = tauc;
     }
-    // Transmission as seen by the boundary conditions: exp(-tau'/mu0) down to
-    // the absorption cut-off level and zero below it, as a freshly constructed
-    // reference solver leaves it (beam_transmission_ is only filled for
-    // lc < ncut before the cut-off decision, src/FluxSolver.cpp:417-421).
+    s_ncut = ncut;
+    s_abst = abst;
+  }
+  __syncthreads();
+  const int ncut = s_ncut;
+  // Beam path factor per layer (:392-421): mu0, or with the spherical beam
+  // the Chapman factor at the layer middle and the slant depth at the layer
+  // bottom, filled for lc < ncut only (zero below, as in a freshly
+  // constructed reference solver).
+  for (int lc = lane; lc < L; lc += 64) {
+    const int64_t g = lo + lc;
+    double ch = mu0, cht = 0.0;
+    if (spherical) {
+      ch = 0.0;
+      if (lc < ncut) {
+        const double* zd = a.alt + vo;
+        const double* dts = a.lay_d + LD_DTAU_S * TL + lo;
+        const double rad = sc[DISORT_S_RADIUS];
+        const double taup_mid = a.lay_d[LD_TAU_TOP * TL + g] + dts[lc] / 2.0;
+        ch = taup_mid / chapman(lc + 1, 0.5, zd, dts, L, rad, mu0);
+        cht = chapman(lc + 1, 0.0, zd, dts, L, rad, mu0);
+      }
+    }
+    a.lay_d[LD_CH * TL + g] = ch;
+    a.lay_d[LD_CHTAU * TL + g] = cht;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    // Transmission as seen by the boundary conditions: exp(-tau'/mu0) (or
+    // exp(-slant depth)) down to the absorption cut-off level and zero below
+    // it, as a freshly constructed reference solver leaves it
+    // (beam_transmission_ is only filled for lc < ncut before the cut-off
+    // decision, src/FluxSolver.cpp:417-421).
+    const double abst = s_abst;
     double bt_prev = (F0 > 0.0) ? 1.0 : 0.0;
     for (int lc = 0; lc < L; ++lc) {
       const int64_t g = lo + lc;
       double bt = 0.0;
-      if (F0 > 0.0 && lc < ncut) bt = exp(-a.lev_d[VD_TAU_S * TV + vo + lc + 1] / mu0);
+      if (F0 > 0.0 && lc < ncut)
+        bt = spherical ? exp(-a.lay_d[LD_CHTAU * TL + g]) : exp(-a.lev_d[VD_TAU_S * TV + vo + lc + 1] / mu0);
       a.lay_d[LD_BT_TOP * TL + g] = bt_prev;
       a.lay_d[LD_BT_BOT * TL + g] = bt;
       bt_prev = bt;
@@ -305,9 +377,11 @@ __global__ __launch_bounds__(64) void k_flux(FluxArgs a) {
     }
     double fact = 0.0, dirint = 0.0, fldir = 0.0, fdir = 0.0;
     if (F0 > 0.0) {
+      // Path factor mu0, or the layer's Chapman factor (:1184-1194).
+      const double ch = a.lay_d[LD_CH * TL + lo + lyu];
       const double tauc = a.lev_d[VD_TAUC * TV + vo + lu];
-      fact = exp(-ut / mu0);
-      fdir = mu0 * F0 * exp(-tauc / mu0);
+      fact = exp(-ut / ch);
+      fdir = fabs(mu0) * F0 * exp(-tauc / ch);
       dirint = F0 * fact;
       fldir = mu0 * F0 * fact;
     }

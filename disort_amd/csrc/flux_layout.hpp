@@ -41,7 +41,10 @@ enum LayD {
   LD_TAU_TOP = 4,  // scaled cumulative depth at the layer top
   LD_BT_TOP = 5,   // beam transmission used by the boundary conditions
   LD_BT_BOT = 6,
-  kLayD = 7
+  LD_CH = 7,       // beam path factor of the layer: mu0, or the Chapman factor
+                   // tau'_mid / CH(tau'_mid) with the spherical beam
+  LD_CHTAU = 8,    // spherical beam: slant optical depth at the layer bottom
+  kLayD = 9
 };
 // Per-level record (doubles), SoA: field*total_levels + global_level.
 enum LevD {

@@ -1,5 +1,6 @@
-// gfx950 workgroup kernels of the flux solve for wide stream counts
-// (N = 32, 64; also built for N = 8, 16 as an alternative schedule).
+// gfx950 workgroup kernels of the flux solve: the only kernels for the wide
+// stream counts N = 32, 64 and for the pseudo-spherical beam, and an
+// alternative schedule for N = 4 ... 16.
 //
 // The register kernels of flux_kernels.hip keep a whole NN x NN eigenproblem
 // in one lane and a 3NN-row panel in 16 lanes; at N = 32 and 64 that no
@@ -152,10 +153,11 @@ __device__ double blk_sum(double v, double* red) {
 template <int NN>
 __global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
   using C = WideCfg<NN>;
-  constexpr int N = C::N, LD = C::LD, NP = NN / 2;
+  // Round-robin pairing over NNE = NN rounded up to even players; with odd
+  // NN the extra player is a dummy and its pairs rotate nothing.
+  constexpr int N = C::N, LD = C::LD, NNE = NN + (NN & 1), NP = NNE / 2;
   constexpr TabOff to(N);
   constexpr EigOff eo(N);
-  static_assert(NN % 2 == 0, "round-robin pairing needs an even order");
   __shared__ double sA[NN * LD];  // A_s, later -A_s and its Cholesky factor
   __shared__ double sL[NN * LD];  // -B_s, then its Cholesky factor L
   __shared__ double sS[NN * LD];  // S = L^T (-A_s) L, diagonalised in place
@@ -255,63 +257,82 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
     off = blk_sum(off, red);
     dia = blk_sum(dia, red);
     if (off == 0.0 || off <= 1e-36 * dia) break;
-    for (int round = 0; round < NN - 1; ++round) {
+    for (int round = 0; round < NNE - 1; ++round) {
       if (tid < NP) {
         const int k = tid;
-        const int pa = (k == 0) ? 0 : 1 + (k - 1 + round) % (NN - 1);
-        const int pb = 1 + (NN - 2 - k + round) % (NN - 1);
+        const int pa = (k == 0) ? 0 : 1 + (k - 1 + round) % (NNE - 1);
+        const int pb = 1 + (NNE - 2 - k + round) % (NNE - 1);
         const int p = pa < pb ? pa : pb, q = pa < pb ? pb : pa;
-        const double apq = sS[q * LD + p], app = sS[p * LD + p], aqq = sS[q * LD + q];
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double at = fabs(theta);
-        double tt = (at > 1e150) ? 0.5 / at : 1.0 / (at + sqrt(theta * theta + 1.0));
-        tt = (theta < 0.0) ? -tt : tt;
-        tt = (apq == 0.0) ? 0.0 : tt;
-        const double cs = 1.0 / sqrt(tt * tt + 1.0);
         rp[k] = p;
         rq[k] = q;
-        rc[k] = cs;
-        rs[k] = tt * cs;
-        rt[k] = tt;
-        rapp[k] = app;
-        raqq[k] = aqq;
-        rapq[k] = apq;
+        if (q < NN) {
+          const double apq = sS[q * LD + p], app = sS[p * LD + p], aqq = sS[q * LD + q];
+          const double theta = (aqq - app) / (2.0 * apq);
+          const double at = fabs(theta);
+          double tt = (at > 1e150) ? 0.5 / at : 1.0 / (at + sqrt(theta * theta + 1.0));
+          tt = (theta < 0.0) ? -tt : tt;
+          tt = (apq == 0.0) ? 0.0 : tt;
+          const double cs = 1.0 / sqrt(tt * tt + 1.0);
+          rc[k] = cs;
+          rs[k] = tt * cs;
+          rt[k] = tt;
+          rapp[k] = app;
+          raqq[k] = aqq;
+          rapq[k] = apq;
+        } else {  // pair with the dummy player: identity
+          rc[k] = 1.0;
+          rs[k] = 0.0;
+          rt[k] = 0.0;
+        }
       }
       __syncthreads();
       // S <- J^T S J by 2x2 blocks (pair k rows, pair k2 columns), k >= k2;
-      // V <- V J by column pairs.
+      // V <- V J by column pairs.  Indices >= NN (the dummy) are absent.
       for (int e = tid; e < NP * NP; e += blockDim.x) {
         const int k = e / NP, k2 = e % NP;
         if (k2 > k) continue;
         const int p = rp[k], q = rq[k];
+        const bool qd = q >= NN;
         if (k == k2) {
-          const double tt = rt[k], apq = rapq[k];
-          sS[p * LD + p] = rapp[k] - tt * apq;
-          sS[q * LD + q] = raqq[k] + tt * apq;
-          sS[q * LD + p] = 0.0;
-          sS[p * LD + q] = 0.0;
+          if (!qd) {
+            const double tt = rt[k], apq = rapq[k];
+            sS[p * LD + p] = rapp[k] - tt * apq;
+            sS[q * LD + q] = raqq[k] + tt * apq;
+            sS[q * LD + p] = 0.0;
+            sS[p * LD + q] = 0.0;
+          }
         } else {
           const int p2 = rp[k2], q2 = rq[k2];
+          const bool q2d = q2 >= NN;
           const double c1 = rc[k], s1 = rs[k], c2 = rc[k2], s2 = rs[k2];
-          const double bpp = sS[p * LD + p2], bpq = sS[p * LD + q2];
-          const double bqp = sS[q * LD + p2], bqq = sS[q * LD + q2];
+          const double bpp = sS[p * LD + p2];
+          const double bpq = q2d ? 0.0 : sS[p * LD + q2];
+          const double bqp = qd ? 0.0 : sS[q * LD + p2];
+          const double bqq = (qd || q2d) ? 0.0 : sS[q * LD + q2];
           const double upp = c1 * bpp - s1 * bqp, upq = c1 * bpq - s1 * bqq;  // rows
           const double uqp = s1 * bpp + c1 * bqp, uqq = s1 * bpq + c1 * bqq;
           const double npp = c2 * upp - s2 * upq, npq = s2 * upp + c2 * upq;  // columns
           const double nqp = c2 * uqp - s2 * uqq, nqq = s2 * uqp + c2 * uqq;
           sS[p * LD + p2] = npp;
-          sS[p * LD + q2] = npq;
-          sS[q * LD + p2] = nqp;
-          sS[q * LD + q2] = nqq;
           sS[p2 * LD + p] = npp;
-          sS[q2 * LD + p] = npq;
-          sS[p2 * LD + q] = nqp;
-          sS[q2 * LD + q] = nqq;
+          if (!q2d) {
+            sS[p * LD + q2] = npq;
+            sS[q2 * LD + p] = npq;
+          }
+          if (!qd) {
+            sS[q * LD + p2] = nqp;
+            sS[p2 * LD + q] = nqp;
+          }
+          if (!qd && !q2d) {
+            sS[q * LD + q2] = nqq;
+            sS[q2 * LD + q] = nqq;
+          }
         }
       }
       for (int e = tid; e < NN * NP; e += blockDim.x) {
         const int m = e / NP, k = e % NP;
         const int p = rp[k], q = rq[k];
+        if (q >= NN) continue;
         const double vp = sV[m * LD + p], vq = sV[m * LD + q];
         sV[m * LD + p] = rc[k] * vp - rs[k] * vq;
         sV[m * LD + q] = rs[k] * vp + rc[k] * vq;
@@ -348,21 +369,103 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
   }
 
   // Particular solutions, reference angle order: NN+r is +mu_r, NN-1-r is -mu_r.
-  // zb (beam) in v0/v1 (upper/lower halves), z0 in v2/v3, z1 kept per thread.
-  double zbu = 0.0, zbd = 0.0, z0u = 0.0, z0d = 0.0, z1h = 0.0;
-  if (beam) {
-    // Right-hand side sums over even / odd l (src/FluxSolver.cpp:628-632).
-    double bsr = 0.0, bdr = 0.0;
+  const bool thermal = (a.flags & DISORT_F_THERMAL) != 0;
+  const bool spherical = beam && (a.flags & DISORT_F_SPHERICAL) != 0;
+  const double taut = a.lev_d[VD_TAU_S * TV + v];
+  const double taub = a.lev_d[VD_TAU_S * TV + v + 1];
+
+  // R = Cholesky factor of -A_s (in sA), for the thermal source and the
+  // unattenuated spherical source.
+  if (thermal || spherical) {
+    for (int e = tid; e < NN * NN; e += blockDim.x) sA[e / NN * LD + e % NN] = -sA[e / NN * LD + e % NN];
+    __syncthreads();
+    blk_cholesky<NN>(sA, LD, &flag);
+  }
+
+  // Solves (I + xba diag(mu) - C) z = b for z_up (+mu_r) and z_dn (-mu_r),
+  // given this thread's bs = b_up + b_dn and bd = b_up - b_dn (thread r < NN).
+  // With s = z_up + z_dn, d = z_up - z_dn:
+  //   ((alpha+beta)(alpha-beta) - xba^2) d = xba p + (alpha+beta) q,
+  //   s = ((alpha-beta) d - q) / xba,   p = -bs/mu, q = -bd/mu,
+  // through the eigen-decomposition; for xba = 0 directly,
+  //   d = (alpha-beta)^-1 q,  s = (alpha+beta)^-1 p  (Cholesky factors L, R).
+  auto solve_ab = [&](double bs, double bd, bool xba_zero, double mub, double& zu, double& zd) {
+    double pr = 0.0, qr = 0.0;
     if (tid < NN) {
-      double se = 0.0, so = 0.0;
-      for (int l = 0; l < N; ++l) {
-        const double term = chi[l] * tb[to.P() + l * NN + tid] * p0[l];
-        if (l % 2 == 0) se += term;
-        else so += term;
-      }
-      bsr = F0 * se / (2.0 * kPi);
-      bdr = F0 * so / (2.0 * kPi);
+      const double cm = tb[to.cmu() + tid];
+      pr = -bs / cm;
+      qr = -bd / cm;
     }
+    double sr = 0.0, dr = 0.0;
+    if (xba_zero) {
+      if (tid < NN) {
+        v0[tid] = tb[to.sd() + tid] * qr;
+        v1[tid] = tb[to.sd() + tid] * pr;
+      }
+      __syncthreads();
+      blk_lsolve<NN>(sL, LD, v0);
+      blk_ltsolve<NN>(sL, LD, v0);
+      blk_lsolve<NN>(sA, LD, v1);
+      blk_ltsolve<NN>(sA, LD, v1);
+      if (tid < NN) {
+        dr = -tb[to.sdi() + tid] * v0[tid];
+        sr = -tb[to.sdi() + tid] * v1[tid];
+      }
+    } else {
+      if (tid < NN) v0[tid] = tb[to.sd() + tid] * qr;  // w
+      __syncthreads();
+      // (alpha+beta) q = -S_d^-1 L^-T V Lambda V^T L^-1 S_d q
+      blk_lsolve<NN>(sL, LD, v0);
+      blk_vdv<NN>(sV, LD, lam, v0, v1);
+      blk_ltsolve<NN>(sL, LD, v0);
+      if (tid < NN) {
+        const double apbq = -tb[to.sdi() + tid] * v0[tid];
+        const double rr = (pr + mub * apbq) / mub;
+        v2[tid] = rr * tb[to.sd() + tid];
+        dv[tid] = 1.0 / (lam[tid] - 1.0 / (mub * mub));
+      }
+      __syncthreads();
+      // d = S_d^-1 L^-T V (Lambda - xba^2)^-1 V^T L^T S_d r
+      blk_lmul<NN, true>(sL, LD, v2, v1);
+      blk_vdv<NN>(sV, LD, dv, v2, v1);
+      blk_ltsolve<NN>(sL, LD, v2);
+      if (tid < NN) {
+        dr = tb[to.sdi() + tid] * v2[tid];
+        v3[tid] = tb[to.sd() + tid] * dr;
+      }
+      __syncthreads();
+      // (alpha-beta) d = -S_d^-1 L L^T S_d d
+      blk_lmul<NN, true>(sL, LD, v3, v1);
+      blk_lmul<NN, false>(sL, LD, v3, v1);
+      if (tid < NN) {
+        const double ambd = -tb[to.sdi() + tid] * v3[tid];
+        sr = mub * (ambd - qr);
+      }
+    }
+    zu = 0.5 * (sr + dr);
+    zd = 0.5 * (sr - dr);
+    __syncthreads();
+  };
+
+  // Beam source sums over even / odd l (src/FluxSolver.cpp:628-632):
+  // b_up + b_dn = bsr, b_up - b_dn = bdr.
+  double bsr = 0.0, bdr = 0.0;
+  if (beam && tid < NN) {
+    double se = 0.0, so = 0.0;
+    for (int l = 0; l < N; ++l) {
+      const double term = chi[l] * tb[to.P() + l * NN + tid] * p0[l];
+      if (l % 2 == 0) se += term;
+      else so += term;
+    }
+    bsr = F0 * se / (2.0 * kPi);
+    bdr = F0 * so / (2.0 * kPi);
+  }
+
+  // Beam particular solution in the form e^{-xba tau} (zu0 + zu1 tau) (up)
+  // and likewise (down); plane-parallel: xba = 1/mu0, no slope, and the
+  // exponential factors come from the beam transmission.
+  double zbu = 0.0, zbd = 0.0, s1u = 0.0, s1d = 0.0, xba = 0.0;
+  if (beam && !spherical) {
     // Dither mu0 when 1 - mu0^2 k^2 is tiny for any mode (:604-618).
     double mub = mu0;
     const double mu0sq = mu0 * mu0;
@@ -370,60 +473,69 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
       const double kj = sqrt(fabs(lam[j]));
       if (fabs(1.0 - mu0sq * kj * kj) < 1000.0 * kEps) mub = 0.999 * mu0;
     }
-    double pr = 0.0, qr = 0.0;
+    solve_ab(bsr, bdr, false, mub, zbu, zbd);
+  } else if (spherical) {
+    // Pseudo-spherical source (src/FluxSolver.cpp:650-738): xba = 1/CH with
+    // its dithers, linear-in-tau fit of e^{xba tau} times the attenuated
+    // source between the layer's top and bottom, two solves.
+    const double big = sqrt(1.79769313486231571e308) / 1.0e10;
+    xba = 1.0 / a.lay_d[LD_CH * TL + g];
+    if (fabs(xba) > big && taub > 1.0) xba = 0.0;
+    if (fabs(xba * taub) > log(big)) xba = 0.0;
+    if (fabs(xba) > 1.0e-5) {
+      for (int iq = 0; iq < NN; ++iq)
+        if (fabs((fabs(xba) - 1.0 / tb[to.cmu() + iq]) / xba) < 0.05) {
+          xba *= 1.001;
+          break;
+        }
+    }
+    // Frobenius-norm dither of I + xba diag(mu) - C (:719-724).
+    double fro = 0.0;
+    for (int e = tid; e < N * N; e += blockDim.x) {
+      const int i0 = e / N, j0 = e % N;
+      const int ri = i0 < NN ? i0 : i0 - NN, rj = j0 < NN ? j0 : j0 - NN;
+      const double mi = i0 < NN ? tb[to.cmu() + ri] : -tb[to.cmu() + ri];
+      double c = 0.0;
+      for (int l = 0; l < N; ++l) {
+        const double sgn = ((l & 1) && ((i0 < NN) != (j0 < NN))) ? -1.0 : 1.0;
+        c += chi[l] * tb[to.P() + l * NN + ri] * tb[to.P() + l * NN + rj] * sgn;
+      }
+      c *= 0.5 * tb[to.cwt() + rj];
+      const double aij = ((i0 == j0) ? 1.0 + xba * mi : 0.0) - c;
+      fro += aij * aij;
+    }
+    fro = blk_sum(fro, red);
+    if (1.0 / sqrt(fro) < 1.0e-4) {
+      if (xba == 0.0) xba = 5.0e-9;
+      xba *= 1.00000005;
+    }
+    const double q0a = exp(-((lc == 0) ? 0.0 : a.lay_d[LD_CHTAU * TL + g - 1]));
+    const double q2a = exp(-a.lay_d[LD_CHTAU * TL + g]);
+    const double et = exp(xba * taut), eb = exp(xba * taub), dtl = taub - taut;
+    // Source at the up / down nodes: zj = (bs +/- bd) / 2.
+    const double zju = 0.5 * (bsr + bdr), zjd = 0.5 * (bsr - bdr);
+    const double x1u = (q2a * zju * eb - q0a * zju * et) / dtl, x1d = (q2a * zjd * eb - q0a * zjd * et) / dtl;
+    const double x0u = q0a * zju * et - x1u * taut, x0d = q0a * zjd * et - x1d * taut;
+    const bool xz = (xba == 0.0);
+    const double mubs = xz ? 0.0 : 1.0 / xba;
+    solve_ab(x1u + x1d, x1u - x1d, xz, mubs, s1u, s1d);
+    double r0u = 0.0, r0d = 0.0;
     if (tid < NN) {
       const double cm = tb[to.cmu() + tid];
-      pr = -bsr / cm;
-      qr = -bdr / cm;
-      v0[tid] = tb[to.sd() + tid] * qr;  // w
+      r0u = x0u + cm * s1u;
+      r0d = x0d - cm * s1d;
     }
-    __syncthreads();
-    // (alpha+beta) q = -S_d^-1 L^-T V Lambda V^T L^-1 S_d q
-    blk_lsolve<NN>(sL, LD, v0);
-    blk_vdv<NN>(sV, LD, lam, v0, v1);
-    blk_ltsolve<NN>(sL, LD, v0);
-    if (tid < NN) {
-      const double apbq = -tb[to.sdi() + tid] * v0[tid];
-      const double rr = (pr + mub * apbq) / mub;
-      v2[tid] = rr * tb[to.sd() + tid];
-      dv[tid] = 1.0 / (lam[tid] - 1.0 / (mub * mub));
-    }
-    __syncthreads();
-    // d = S_d^-1 L^-T V (Lambda - 1/mu0^2)^-1 V^T L^T S_d r
-    blk_lmul<NN, true>(sL, LD, v2, v1);
-    blk_vdv<NN>(sV, LD, dv, v2, v1);
-    blk_ltsolve<NN>(sL, LD, v2);
-    double dr = 0.0;
-    if (tid < NN) {
-      dr = tb[to.sdi() + tid] * v2[tid];
-      v3[tid] = tb[to.sd() + tid] * dr;
-    }
-    __syncthreads();
-    // (alpha-beta) d = -S_d^-1 L L^T S_d d
-    blk_lmul<NN, true>(sL, LD, v3, v1);
-    blk_lmul<NN, false>(sL, LD, v3, v1);
-    if (tid < NN) {
-      const double ambd = -tb[to.sdi() + tid] * v3[tid];
-      const double s = mub * (ambd - qr);
-      zbu = 0.5 * (s + dr);
-      zbd = 0.5 * (s - dr);
-    }
-    __syncthreads();
+    solve_ab(r0u + r0d, r0u - r0d, xz, mubs, zbu, zbd);
   }
 
-  const double taut = a.lev_d[VD_TAU_S * TV + v];
-  const double taub = a.lev_d[VD_TAU_S * TV + v + 1];
-  double xr0 = 0.0, xr1 = 0.0;
-  if (a.flags & DISORT_F_THERMAL) {
-    // B(tau) = xr0 + xr1 tau (:755-776); z1 and z0 from a Cholesky factor of
-    // -A_s (see the register kernel).
+  double xr0 = 0.0, xr1 = 0.0, z0u = 0.0, z0d = 0.0, z1h = 0.0;
+  if (thermal) {
+    // B(tau) = xr0 + xr1 tau (:755-776); z1 and z0 from the Cholesky factor
+    // R of -A_s (see the register kernel).
     const double pt = a.lev_d[VD_PLANCK * TV + v], pb = a.lev_d[VD_PLANCK * TV + v + 1];
     const double dts = a.lay_d[LD_DTAU_S * TL + g];
     if (dts > 1e-4) xr1 = (pb - pt) / dts;
     xr0 = pt - xr1 * taut;
-    for (int e = tid; e < NN * NN; e += blockDim.x) sA[e / NN * LD + e % NN] = -sA[e / NN * LD + e % NN];
-    __syncthreads();
-    blk_cholesky<NN>(sA, LD, &flag);
     if (tid < NN) v0[tid] = tb[to.sd() + tid] * (2.0 / tb[to.cmu() + tid]);
     __syncthreads();
     blk_lsolve<NN>(sA, LD, v0);
@@ -447,15 +559,30 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
   }
 
   if (tid < NN) {
-    const double btt = a.lay_d[LD_BT_TOP * TL + g], btb = a.lay_d[LD_BT_BOT * TL + g];
-    const double fact = beam ? exp(-taub / mu0) : 0.0;
     const int ru = NN + tid, rd = NN - 1 - tid;
-    a.eig[(eo.ztop() + ru) * T + t] = zbu * btt + z0u + z1h * taut;
-    a.eig[(eo.ztop() + rd) * T + t] = zbd * btt + z0d + z1h * taut;
-    a.eig[(eo.zbot() + ru) * T + t] = zbu * btb + z0u + z1h * taub;
-    a.eig[(eo.zbot() + rd) * T + t] = zbd * btb + z0d + z1h * taub;
-    a.eig[(eo.zfl() + ru) * T + t] = zbu * fact + z0u + z1h * taub;
-    a.eig[(eo.zfl() + rd) * T + t] = zbd * fact + z0d + z1h * taub;
+    double btu, btd, bbu, bbd, bfu, bfd;  // beam part at the top / bottom (BC form) / bottom (flux form)
+    if (spherical) {
+      const double at = exp(-xba * taut), ab = exp(-xba * taub);
+      btu = at * (zbu + s1u * taut);
+      btd = at * (zbd + s1d * taut);
+      bbu = bfu = ab * (zbu + s1u * taub);
+      bbd = bfd = ab * (zbd + s1d * taub);
+    } else {
+      const double btt = a.lay_d[LD_BT_TOP * TL + g], btb = a.lay_d[LD_BT_BOT * TL + g];
+      const double fact = beam ? exp(-taub / mu0) : 0.0;
+      btu = zbu * btt;
+      btd = zbd * btt;
+      bbu = zbu * btb;
+      bbd = zbd * btb;
+      bfu = zbu * fact;
+      bfd = zbd * fact;
+    }
+    a.eig[(eo.ztop() + ru) * T + t] = btu + z0u + z1h * taut;
+    a.eig[(eo.ztop() + rd) * T + t] = btd + z0d + z1h * taut;
+    a.eig[(eo.zbot() + ru) * T + t] = bbu + z0u + z1h * taub;
+    a.eig[(eo.zbot() + rd) * T + t] = bbd + z0d + z1h * taub;
+    a.eig[(eo.zfl() + ru) * T + t] = bfu + z0u + z1h * taub;
+    a.eig[(eo.zfl() + rd) * T + t] = bfd + z0d + z1h * taub;
   }
   if (tid == 0) a.eig[eo.pls() * T + t] = xr0 + xr1 * taub;
 }
@@ -705,7 +832,12 @@ void launch_wide(const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
 
 bool launch_flux_bucket_wg(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
   switch (nstr) {
+    case 4: launch_wide<2>(a, s, ev); return true;
+    case 6: launch_wide<3>(a, s, ev); return true;
     case 8: launch_wide<4>(a, s, ev); return true;
+    case 10: launch_wide<5>(a, s, ev); return true;
+    case 12: launch_wide<6>(a, s, ev); return true;
+    case 14: launch_wide<7>(a, s, ev); return true;
     case 16: launch_wide<8>(a, s, ev); return true;
     case 32: launch_wide<16>(a, s, ev); return true;
     case 64: launch_wide<32>(a, s, ev); return true;

@@ -43,7 +43,7 @@ enum {
 /* ---- batch flags (uniform over a batch) ---------------------------------- */
 enum {
   DISORT_F_THERMAL = 1,       /* DisortFluxConfig::use_thermal_emission      */
-  DISORT_F_SPHERICAL = 2,     /* use_spherical_beam (not on the GPU path yet) */
+  DISORT_F_SPHERICAL = 2,     /* use_spherical_beam (workgroup kernels)      */
   DISORT_F_DELTA_M_PLUS = 4,  /* use_delta_m_plus                            */
   DISORT_F_DIFFUSION_BC = 8   /* use_diffusion_lower_bc                      */
 };
@@ -131,9 +131,10 @@ int disort_sync(disort_ctx* ctx);
  * DisortFluxConfig::validate (src/DisortFluxConfig.cpp:43-143), copied in,
  * solved, and the outputs and per-instance status copied back before return.
  * Supported on the GPU: nstr in {4,6,...,16,32,64} (the stream counts the
- * reference instantiates, src/FluxSolver.cpp:1572-1580), plane-parallel
- * beam, thermal emission, Lambertian surface, Delta-M / Delta-M+, layer
- * cut-off, diffusion lower boundary.
+ * reference instantiates, src/FluxSolver.cpp:1572-1580), plane-parallel and
+ * pseudo-spherical beam (level altitudes in batch->alt, radius in
+ * scal[DISORT_S_RADIUS]), thermal emission, Lambertian surface, Delta-M /
+ * Delta-M+, layer cut-off, diffusion lower boundary.
  */
 int disort_flux_solve(disort_ctx* ctx, const disort_flux_batch* batch, int memspace);
 
@@ -160,7 +161,8 @@ int disort_flux_status(disort_ctx* ctx, disort_flux_plan* plan, int32_t* status_
  * reference: it selects between two gfx950 implementations of the same
  * solve).  DISORT_V_LANE: register kernels, one lane per layer and 16 lanes
  * per instance, N in {4,...,16}.  DISORT_V_WORKGROUP: LDS kernels, one
- * workgroup per layer and per instance, N in {8,16,32,64}.  Both give
+ * workgroup per layer and per instance, every N; the only variant for
+ * N = 32, 64 and for the spherical beam.  Both give
  * results within the parity tolerance of each other; the default is LANE
  * where it exists.  The autotuner (disort_amd/schedule.py) picks by timing.
  */

@@ -55,10 +55,11 @@ def test_workgroup_variant_at_small_n(ctx, nstr):
 def test_variant_rejected_where_no_kernels(ctx):
     from disort_amd._capi import V_LANE, V_WORKGROUP, DeviceFluxBatch, DisortError
 
-    b = W.random_flux_batch(2, 4, 6, seed=1)
+    b = W.random_flux_batch(2, 4, 32, seed=1)
     dev = DeviceFluxBatch(ctx, b)
+    assert dev.get_variant(32) == V_WORKGROUP
     with pytest.raises(DisortError):
-        dev.set_variant(6, V_WORKGROUP)
+        dev.set_variant(32, V_LANE)  # no register kernels at 32 streams
     with pytest.raises(DisortError):
         dev.set_variant(8, V_LANE)  # no bucket with 8 streams
     dev.close()

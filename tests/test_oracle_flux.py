@@ -121,3 +121,35 @@ def test_validation_messages(mutate, msg):
     with pytest.raises(ol.OracleError) as e:
         ol.flux_one(b, 0)
     assert msg in str(e.value)
+
+
+def spherical_case(spherical=True):
+    """Two-layer Earth atmosphere of the reference's spherical-geometry test
+    (tests/test_solver.cpp:4459-4532): altitudes 60/30/0 km, radius 6371 km,
+    HG g=0.8, beam 1 at mu0=0.5, albedo 0.1, 16 streams."""
+    from disort_amd.batch import F_SPHERICAL
+
+    return uniform_batch(np.array([[0.5, 1.0]]), np.array([[0.9, 0.8]]), W.hg_moments(np.full((1, 2), 0.8), 17), 16,
+                         scalars(1, beam_flux=1.0, beam_mu=0.5, albedo=0.1, radius=6371.0),
+                         flags=F_SPHERICAL if spherical else 0, alt=np.array([[60.0, 30.0, 0.0]]))
+
+
+def test_spherical_geometry_cdisort_values():
+    # C-DISORT values quoted by the reference test at 1e-3 relative.
+    out = ol.flux_one(spherical_case(), 0)
+    assert out[FIELD["flux_direct_beam"], 0] == pytest.approx(0.5, rel=1e-3)
+    assert out[FIELD["flux_up"], 0] == pytest.approx(8.80100177e-02, rel=1e-3)
+    assert out[FIELD["flux_direct_beam"], 2] == pytest.approx(2.55423483e-02, rel=1e-3)
+    assert out[FIELD["flux_down"], 2] == pytest.approx(1.99378710e-01, rel=1e-3)
+    assert out[FIELD["flux_up"], 2] == pytest.approx(2.25146963e-02, rel=1e-3)
+    # The plane-parallel direct beam at the bottom is 2.48935342e-02 (same test).
+    pp = ol.flux_one(spherical_case(spherical=False), 0)
+    assert pp[FIELD["flux_direct_beam"], 2] == pytest.approx(2.48935342e-02, rel=1e-6)
+
+
+def test_spherical_large_radius_tends_to_plane_parallel():
+    b = spherical_case()
+    b.scal.reshape(1, -1)[0, 10] = 6.371e9  # radius: curvature negligible
+    sp = ol.flux_one(b, 0)
+    pp = ol.flux_one(spherical_case(spherical=False), 0)
+    np.testing.assert_allclose(sp, pp, rtol=1e-6, atol=1e-12)

@@ -61,6 +61,7 @@ enum {
   DISORT_S_WN_LO = 8,     /* wavenumber_low         */
   DISORT_S_WN_HI = 9,     /* wavenumber_high        */
   DISORT_S_RADIUS = 10,   /* bottom_radius          */
+  DISORT_S_BEAM_PHI = 11, /* direct_beam_phi [deg]; general solver only */
   DISORT_NSCAL = 12
 };
 
@@ -222,6 +223,57 @@ const char* disort_version(void);
  * mirrored.  Host-only; needs no context or GPU.
  */
 int disort_quadrature(int nstr, double* cmu, double* cwt);
+
+/* =========================================================================
+ * General solver: DisortSolver::solve (src/DisortSolver.cpp:21-399) for a
+ * batch of columns -- azimuthal Fourier modes, intensities at user polar
+ * angles, azimuths and optical depths, Lambertian or RPV surface,
+ * plane-parallel or pseudo-spherical beam, thermal emission, the
+ * Nakajima-Tanaka intensity correction and the BRDF beam-reflection
+ * correction.  Shapes are uniform over a batch (one DisortConfig with
+ * per-column optical properties, as solveSpectralBands varies them,
+ * src/python_bindings.cpp:96-128).
+ * ========================================================================= */
+enum {
+  DISORT_G_USER_TAU = 16,       /* flags.use_user_tau                          */
+  DISORT_G_USER_MU = 32,        /* flags.use_user_mu                           */
+  DISORT_G_ONLY_FLUXES = 64,    /* flags.comp_only_fluxes                      */
+  DISORT_G_BRDF_RPV = 128,      /* use_lambertian_surface = false, BrdfType::RPV */
+  DISORT_G_CORR_BURAS = 256,    /* flags.intensity_corr_buras                  */
+  DISORT_G_CORR_NAKAJIMA = 512  /* flags.intensity_corr_nakajima               */
+};
+/* RPV parameter block rpv[i*DISORT_NRPV + slot] (src/DisortConfig.hpp RpvBrdfSpec). */
+enum {
+  DISORT_RPV_RHO0 = 0, DISORT_RPV_K = 1, DISORT_RPV_THETA = 2, DISORT_RPV_SIGMA = 3,
+  DISORT_RPV_T1 = 4, DISORT_RPV_T2 = 5, DISORT_RPV_SCALE = 6, DISORT_NRPV = 7
+};
+
+typedef struct disort_general_batch {
+  int64_t n;            /* columns                                               */
+  int32_t flags;        /* DISORT_F_* | DISORT_G_*                               */
+  int32_t nstr;         /* num_streams                                           */
+  int32_t nlyr;         /* num_layers                                            */
+  int32_t pstride;      /* moments per layer row: max(nmom, nstr) + 1            */
+  int32_t ntau;         /* output levels: num_user_tau, or nlyr+1 without USER_TAU */
+  int32_t numu;         /* num_user_mu, or nstr without USER_MU / with ONLY_FLUXES */
+  int32_t nphi;         /* num_phi (0 with ONLY_FLUXES)                          */
+  int32_t reserved;
+  double accuracy;      /* accuracy_fourier_series (0 = all nstr modes)          */
+  const double* dtau;   /* [n][nlyr]                                             */
+  const double* ssa;    /* [n][nlyr]                                             */
+  const double* pmom;   /* [n][nlyr][pstride]                                    */
+  const double* temper; /* [n][nlyr+1] with DISORT_F_THERMAL, else NULL          */
+  const double* alt;    /* [n][nlyr+1] with DISORT_F_SPHERICAL, else NULL        */
+  const double* scal;   /* [n][DISORT_NSCAL]                                     */
+  const double* rpv;    /* [n][DISORT_NRPV] with DISORT_G_BRDF_RPV, else NULL    */
+  const double* tau_user; /* [n][ntau] with DISORT_G_USER_TAU, else NULL         */
+  const double* mu_user;  /* [numu], increasing, with DISORT_G_USER_MU           */
+  const double* phi_user; /* [nphi] degrees                                      */
+  double* flux;         /* out [DISORT_NOUT][n][ntau], FluxResult field order    */
+  double* inten;        /* out [n][ntau][numu][nphi] (DisortResult::intensity)   */
+  double* inten_avg;    /* out [n][ntau][numu] (intensity_azimuth_avg)           */
+  int32_t* status;      /* out [n], may be NULL                                  */
+} disort_general_batch;
 
 
 /* =========================================================================

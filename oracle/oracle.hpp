@@ -23,6 +23,15 @@
 //        tests/test_solver.cpp:5238-5434), C-DISORT disotest tables
 //        (9a/9b, 6b/6c, thermal T1-T4), closed forms (Beer's law,
 //        transparent medium), energy conservation at omega=1.
+//   general_solve   : C-DISORT intensity tables quoted by the reference's
+//        tests (IntensityValidation 1a-3b, disotest 7b/7c/9c/10a, spherical),
+//        Nakajima-Tanaka corrected tables (IntensityCorrectionHG3a/3b), RPV
+//        tables (RpvBRDF1, TransparentBrdfRpv, MultiLayerBrdfRpv) -- 23 cases
+//        in tests/golden/intensity_cases.json; m=0 fluxes equal flux_solve's,
+//        azimuthal symmetry I(phi) = I(360-phi), the azimuth mean equals the
+//        m=0 term, the accuracy_fourier_series truncation, index_from_bottom.
+//   rpv_reflectivity: reciprocity rho(mu1,mu2,phi) = rho(mu2,mu1,phi), and
+//        the transparent-medium closed form (mu0 F0/pi rho at the top).
 #pragma once
 #include <cstdint>
 
@@ -91,5 +100,37 @@ struct FluxOutput {
 // Throws std::invalid_argument / std::runtime_error with the reference's messages.
 void validate_flux_config(const FluxConfig& c);
 FluxOutput flux_solve(const FluxConfig& c);
+
+// ---- general solver (restating src/DisortSolver.cpp DisortSolver) ---------
+// RPV parameters in the order rho0, k, theta, sigma, t1, t2, scale
+// (src/DisortConfig.hpp RpvBrdfSpec).
+enum { BRDF_NONE = 0, BRDF_RPV = 1 };
+struct GeneralConfig {
+  int nlyr = 0, nstr = 0, nmom = 0;
+  bool thermal = false, spherical = false, deltam_plus = false, diffusion_bc = false;
+  bool use_user_tau = false, use_user_mu = false, only_fluxes = false;
+  bool lambertian = true, corr_buras = false, corr_nakajima = false, index_from_bottom = false;
+  int brdf = BRDF_NONE;
+  Real rpv[7] = {0, 0, 0, 0, 0, 0, 1};
+  Real beam_flux = 0, beam_mu = 0.5, beam_phi = 0, iso_top = 0, iso_bot = 0;
+  Real temp_top = 0, temp_bot = 0, emis_top = 1, albedo = 0;
+  Real wn_lo = 0, wn_hi = 0, radius = 0, accur = 0;
+  std::vector<Real> dtau, ssa, temper, alt, pmom;  // pmom [nlyr][pstride]
+  std::vector<Real> tau_user, mu_user, phi_user;
+  int pstride() const { return (nmom > nstr ? nmom : nstr) + 1; }
+  Real pm(int k, int lc) const { return pmom[(size_t)lc * pstride() + k]; }
+};
+
+struct GeneralOutput {
+  int ntau = 0, numu = 0, nphi = 0, modes = 0;  // modes = Fourier modes computed
+  std::vector<Real> f[8];                      // [ntau] each, FluxResult field order
+  std::vector<Real> inten;                     // [ntau][numu][nphi]
+  std::vector<Real> inten_avg;                 // [ntau][numu]
+  std::vector<Real> mu_angles;                 // [numu]
+};
+
+void validate_general_config(const GeneralConfig& c);
+GeneralOutput general_solve(const GeneralConfig& c);
+Real rpv_reflectivity(const Real* rpv, Real mu1, Real mu2, Real phi);
 
 }  // namespace ORACLE_NS

@@ -194,9 +194,222 @@ def numerics_ref():
     return dict(planck=planck, legendre=legendre, band=band)
 
 
+def _hg_rows(g, P):
+    """HG moments as the reference fills them (running product g^k)."""
+    row, gk = [], 1.0
+    for _ in range(P):
+        row.append(gk)
+        gk *= g
+    return row
+
+
+def _iso_rows(P):
+    return [1.0] + [0.0] * (P - 1)
+
+
+def _ray_rows(P):
+    return [1.0 if k == 0 else (0.1 if k == 2 else 0.0) for k in range(P)]
+
+
+def intensity_cases():
+    """General-solver cases transcribed from the reference's tests
+    (tests/test_solver.cpp, line ranges per case).  Each expectation is
+    (quantity, indices, value, tolerance) with the tolerance the reference
+    test uses: relative ('rtol') or absolute ('atol')."""
+    pi = np.pi
+    cases = []
+
+    def add(name, src, nlyr, nstr, nmom, dtau, ssa, pmom, flags, scal, expect, accuracy=0.0, tau_user=None,
+            mu_user=None, phi_user=None, temper=None, alt=None, rpv=None):
+        P = max(nmom, nstr) + 1
+        assert all(len(r) == P for r in pmom), name
+        cases.append(dict(name=name, source=src, nlyr=nlyr, nstr=nstr, nmom=nmom, pstride=P, dtau=list(dtau),
+                          ssa=list(ssa), pmom=[list(r) for r in pmom], flags=flags, scal=scal,
+                          accuracy=accuracy, tau_user=tau_user, mu_user=mu_user, phi_user=phi_user,
+                          temper=temper, alt=alt, rpv=rpv, expect=expect))
+
+    def I(iu, lu, j, v, rtol=None, atol=None):
+        return dict(q="intensity", iu=iu, lu=lu, j=j, v=v, rtol=rtol, atol=atol)
+
+    def F(field, lu, v, rtol=None, atol=None):
+        return dict(q=field, lu=lu, v=v, rtol=rtol, atol=atol)
+
+    umu6 = [-1.0, -0.5, -0.1, 0.1, 0.5, 1.0]
+
+    def six(top_up, bot_dn, rtol, zero=True, top_dn=None):
+        e = []
+        for iu, v in zip((3, 4, 5), top_up):
+            e.append(I(iu, 0, 0, v, rtol=rtol))
+        for iu, v in zip((0, 1, 2), bot_dn):
+            e.append(I(iu, 1, 0, v, rtol=rtol))
+        if zero:
+            for iu in (0, 1, 2):
+                e.append(I(iu, 0, 0, 0.0 if top_dn is None else top_dn, atol=1e-10 if top_dn is None else 1e-5))
+            if top_dn is None:
+                for iu in (3, 4, 5):
+                    e.append(I(iu, 1, 0, 0.0, atol=1e-10))
+        return e
+
+    # --- C-DISORT intensity tables, no correction (IntensityValidation*) ---
+    iso = dict(beam_mu=0.1, albedo=0.0)
+    for name, src, tau, w, fb, itop, top_up, bot_dn, rtol, zero in [
+        ("iv_1a", "tests/test_solver.cpp:2613-2670", 0.03125, 0.2, pi / 0.1, 0.0,
+         [1.17770662e-01, 2.64170381e-02, 1.34041302e-02], [1.33826267e-02, 2.63323514e-02, 1.15897886e-01], 5e-3, True),
+        ("iv_1b", "tests/test_solver.cpp:2672-2726", 0.03125, 1.0, pi / 0.1, 0.0,
+         [6.22883782e-01, 1.39762939e-01, 7.09191639e-02], [7.08109342e-02, 1.39336700e-01, 6.13457861e-01], 5e-3, True),
+        ("iv_1d", "tests/test_solver.cpp:2848-2868", 32.0, 0.2, pi / 0.1, 0.0,
+         [2.62972304e-01, 9.06966927e-02, 5.02852820e-02], [], 5e-3, False),
+        ("iv_1e", "tests/test_solver.cpp:2870-2884", 32.0, 1.0, pi / 0.1, 0.0,
+         [1.93320904e+00, 1.02731911e+00, 7.97198966e-01], [2.71316407e-02, 1.87804598e-02, 1.16385133e-02], 5e-3, False),
+    ]:
+        add(name, src, 1, 16, 0, [tau], [w], [_iso_rows(17)], ["user_mu"], dict(iso, beam_flux=fb, iso_top=itop),
+            six(top_up, bot_dn, rtol, zero), accuracy=1e-4, mu_user=umu6, phi_user=[0.0])
+    add("iv_1c", "tests/test_solver.cpp:2826-2846", 1, 16, 0, [0.03125], [0.99], [_iso_rows(17)], ["user_mu"],
+        dict(iso, beam_flux=0.0, iso_top=1.0),
+        six([1.33177353e-01, 2.99878840e-02, 1.52233390e-02], [9.84446844e-01, 9.69362629e-01, 8.63945617e-01],
+            1e-3, True, top_dn=1.0) + [I(iu, 1, 0, 0.0, atol=1e-10) for iu in (3, 4, 5)],
+        accuracy=1e-4, mu_user=umu6, phi_user=[0.0])
+    add("iv_1f", "tests/test_solver.cpp:2886-2903", 1, 16, 0, [32.0], [0.99], [_iso_rows(17)], ["user_mu"],
+        dict(iso, beam_flux=0.0, iso_top=1.0),
+        six([8.77510240e-01, 8.15135758e-01, 7.52714753e-01], [1.86840458e-03, 1.26492283e-03, 7.79280391e-04],
+            5e-3, True, top_dn=1.0),
+        accuracy=1e-4, mu_user=umu6, phi_user=[0.0])
+    add("iv_2a", "tests/test_solver.cpp:2728-2789", 1, 16, 0, [0.2], [0.5], [_ray_rows(17)], ["user_mu"],
+        dict(beam_flux=1.0, beam_mu=0.08),
+        six([2.82015706e-02, 7.33147775e-03, 2.43710005e-03], [2.27937786e-03, 6.84949135e-03, 1.46808276e-02], 5e-3),
+        accuracy=1e-4, mu_user=umu6, phi_user=[0.0])
+    umu_r = [-0.981986, -0.538263, -0.018014, 0.018014, 0.538263, 0.981986]
+    for name, src, tau, w, top_up, bot_dn, zero in [
+        ("iv_2a_disotest", "tests/test_solver.cpp:2933-2951", 0.2, 0.5,
+         [1.61795654e-01, 2.11501379e-02, 7.86712687e-03], [7.71896562e-03, 2.00777929e-02, 2.57684687e-02], True),
+        ("iv_2b", "tests/test_solver.cpp:2953-2965", 0.2, 1.0,
+         [3.47677916e-01, 4.87120142e-02, 1.89386842e-02], [1.86027204e-02, 4.64061388e-02, 6.77603294e-02], False),
+        ("iv_2c", "tests/test_solver.cpp:2967-2980", 5.0, 0.5,
+         [1.62565965e-01, 2.45785751e-02, 1.01498340e-02], [1.70004351e-04, 3.97168092e-05, 1.32472007e-05], False),
+        ("iv_2d", "tests/test_solver.cpp:2982-2994", 5.0, 1.0,
+         [3.64009637e-01, 8.26992712e-02, 4.92369811e-02], [1.05949663e-02, 7.69337480e-03, 3.79276359e-03], False),
+    ]:
+        add(name, src, 1, 16, 0, [tau], [w], [_ray_rows(17)], ["user_mu"], dict(beam_flux=pi, beam_mu=0.080442),
+            six(top_up, bot_dn, 5e-3, zero), accuracy=1e-4, mu_user=umu_r, phi_user=[0.0])
+    for name, src, tau, top_up, bot_dn in [
+        ("iv_3a", "tests/test_solver.cpp:3024-3038", 1.0,
+         [1.51211231e-01, 1.01866361e-01, 3.66538569e-02], [2.67923725e+00, 2.68485407e-01, 2.13997993e-01]),
+        ("iv_3b", "tests/test_solver.cpp:3040-3055", 8.0,
+         [3.79792564e-01, 5.20401891e-01, 4.89946294e-01], [6.66732911e-01, 4.22352708e-01, 2.36362270e-01]),
+    ]:
+        add(name, src, 1, 16, 0, [tau], [1.0], [_hg_rows(0.75, 17)], ["user_mu"], dict(beam_flux=pi, beam_mu=1.0),
+            six(top_up, bot_dn, 1e-2, False), accuracy=1e-4, mu_user=umu6, phi_user=[0.0])
+
+    # --- Nakajima-Tanaka corrected tables (IntensityCorrectionHG3a/3b) ---
+    for name, src, tau, top_up, bot_dn in [
+        ("ic_hg3a", "tests/test_solver.cpp:3895-3910", 1.0,
+         [1.57317307e-01, 1.00094999e-01, 5.51969895e-02], [2.94697018e+00, 2.60364310e-01, 2.09966872e-01]),
+        ("ic_hg3b", "tests/test_solver.cpp:3912-3927", 8.0,
+         [3.85898755e-01, 5.18534731e-01, 5.11459930e-01], [6.68756301e-01, 4.22340072e-01, 2.36358326e-01]),
+    ]:
+        add(name, src, 1, 16, 16, [tau], [1.0], [_hg_rows(0.75, 17)],
+            ["user_mu", "user_tau", "corr_buras", "corr_nakajima"], dict(beam_flux=pi, beam_mu=1.0),
+            six(top_up, bot_dn, 5e-4, False), tau_user=[0.0, tau], mu_user=umu6, phi_user=[0.0])
+
+    # --- Thermal intensities (disotest 7b, 7c, 9c, 10a) ---
+    add("disotest_7b", "tests/test_solver.cpp:3415-3472", 1, 16, 0, [100.0], [0.95], [_hg_rows(0.75, 17)],
+        ["thermal", "user_tau", "user_mu"], dict(beam_flux=0.0, beam_mu=0.5, wn_lo=2702.99, wn_hi=2703.01),
+        [F("flux_up", 0, 1.10948697e-06, rtol=1e-2), F("flux_tau_divergence", 0, 8.23218837e-08, rtol=1e-2),
+         F("flux_down", 1, 2.07786355e-05, rtol=1e-2), I(1, 0, 0, 4.65744239e-07, rtol=1e-2),
+         I(0, 1, 0, 7.52310868e-06, rtol=1e-2)],
+        tau_user=[0.0, 100.0], mu_user=[-1.0, 1.0], phi_user=[0.0], temper=[200.0, 300.0])
+    add("disotest_7c", "tests/test_solver.cpp:3474-3544", 1, 12, 0, [1.0], [0.5], [_hg_rows(0.8, 13)],
+        ["thermal", "user_tau", "user_mu"],
+        dict(beam_flux=200.0, beam_mu=0.5, iso_top=100.0, temp_top=100.0, temp_bot=320.0, emis_top=1.0,
+             wn_lo=0.0, wn_hi=50000.0),
+        [F("flux_direct_beam", 0, 1.0e2, rtol=1e-2), F("flux_down", 0, 3.19829585e+02, rtol=1e-2),
+         F("flux_up", 0, 4.29571753e+02, rtol=1e-2), F("flux_tau_divergence", 0, -8.04270275e+01, rtol=1e-2),
+         F("flux_direct_beam", 1, 3.67879441e+01, rtol=1e-2), F("flux_down", 1, 3.54099239e+02, rtol=1e-2),
+         F("flux_up", 1, 4.47017662e+02, rtol=1e-2), F("flux_direct_beam", 2, 1.35335283e+01, rtol=1e-2),
+         F("flux_down", 2, 3.01333770e+02, rtol=1e-2), F("flux_up", 2, 5.94576146e+02, rtol=1e-2),
+         I(0, 0, 0, 1.01804919e+02, rtol=1e-2), I(2, 0, 0, 1.41637169e+02, rtol=1e-2),
+         I(3, 0, 0, 1.48415776e+02, rtol=1e-2), I(2, 2, 0, 1.89259465e+02, rtol=1e-2),
+         I(3, 2, 0, 1.89259465e+02, rtol=1e-2), I(2, 0, 1, 1.27777845e+02, rtol=1e-2),
+         I(3, 0, 1, 1.48415776e+02, rtol=1e-2)],
+        tau_user=[0.0, 0.5, 1.0], mu_user=[-1.0, -0.1, 0.1, 1.0], phi_user=[0.0, 90.0], temper=[300.0, 200.0])
+
+    def nine_c(name, src, nstr, utau, umu, phi, expect):
+        add(name, src, 6, nstr, 0, [float(l + 1) for l in range(6)], [0.65 + l * 0.05 for l in range(6)],
+            [_hg_rows((l + 1) / 7.0, nstr + 1) for l in range(6)], ["thermal", "user_tau", "user_mu"],
+            dict(beam_flux=pi, beam_mu=0.5, iso_top=1.0, albedo=0.5, temp_bot=700.0, temp_top=550.0, emis_top=1.0,
+                 wn_lo=999.0, wn_hi=1000.0),
+            expect, tau_user=utau, mu_user=umu, phi_user=phi, temper=[600.0 + 10.0 * l for l in range(7)])
+
+    nine_c("disotest_9c", "tests/test_solver.cpp:3309-3360,3546-3595", 8, [0.0, 1.05, 2.1, 6.0, 21.0],
+           [-1.0, -0.2, 0.2, 1.0], [60.0, 120.0, 180.0],
+           [F("flux_direct_beam", 0, 1.57079633e+00, rtol=1e-2), F("flux_down", 0, 6.09217377e+00, rtol=1e-2),
+            F("flux_up", 0, 4.68413572e+00, rtol=1e-2), F("flux_direct_beam", 1, 1.92354108e-01, rtol=1e-2),
+            F("flux_down", 1, 4.97279304e+00, rtol=1e-2), F("flux_up", 1, 4.24381392e+00, rtol=1e-2),
+            F("flux_direct_beam", 2, 2.35549970e-02, rtol=1e-2), F("flux_down", 2, 4.46616368e+00, rtol=1e-2),
+            F("flux_up", 2, 4.16941147e+00, rtol=1e-2), F("flux_down", 3, 4.22730788e+00, rtol=1e-2),
+            F("flux_up", 3, 4.30666805e+00, rtol=1e-2), F("flux_down", 4, 4.73766598e+00, rtol=1e-2),
+            F("flux_up", 4, 5.11524388e+00, rtol=1e-2),
+            I(0, 0, 0, 1.93919914e+00, rtol=1e-2), I(2, 0, 0, 1.61854966e+00, rtol=1e-2),
+            I(3, 0, 0, 1.43872527e+00, rtol=1e-2), I(0, 2, 0, 1.48510758e+00, rtol=1e-2),
+            I(1, 2, 0, 1.35008578e+00, rtol=1e-2), I(2, 4, 0, 1.62823270e+00, rtol=1e-2),
+            I(3, 4, 0, 1.62823270e+00, rtol=1e-2), I(2, 0, 1, 1.57894559e+00, rtol=1e-2),
+            I(3, 0, 1, 1.43872527e+00, rtol=1e-2), I(0, 0, 2, 1.93919914e+00, rtol=1e-2),
+            I(2, 0, 2, 1.56558604e+00, rtol=1e-2)])
+    nine_c("disotest_10a", "tests/test_solver.cpp:3309-3360,3597-3637", 4, [0.0, 2.1, 21.0],
+           [-0.788675129, -0.211324871, 0.211324871, 0.788675129], [60.0, 120.0],
+           [F("flux_direct_beam", 0, 1.57079633e+00, rtol=1e-2), F("flux_down", 0, 6.09217377e+00, rtol=1e-2),
+            F("flux_up", 0, 4.68841229e+00, rtol=1e-2), F("flux_direct_beam", 1, 2.35549970e-02, rtol=1e-2),
+            F("flux_down", 1, 4.47000071e+00, rtol=1e-2), F("flux_up", 1, 4.17037311e+00, rtol=1e-2),
+            F("flux_down", 2, 4.73856448e+00, rtol=1e-2), F("flux_up", 2, 5.11569312e+00, rtol=1e-2),
+            I(0, 0, 0, 1.93919914e+00, rtol=1e-2), I(2, 0, 0, 1.61511864e+00, rtol=1e-2),
+            I(3, 0, 0, 1.46918463e+00, rtol=1e-2), I(0, 1, 0, 1.44581337e+00, rtol=1e-2),
+            I(2, 1, 0, 1.33080189e+00, rtol=1e-2), I(3, 1, 0, 1.32690167e+00, rtol=1e-2),
+            I(2, 2, 0, 1.62837570e+00, rtol=1e-2), I(3, 2, 0, 1.62837570e+00, rtol=1e-2),
+            I(2, 0, 1, 1.57636831e+00, rtol=1e-2), I(3, 0, 1, 1.45769864e+00, rtol=1e-2)])
+
+    # --- RPV surface (RpvBRDF1, TransparentBrdfRpv, MultiLayerBrdfRpv) ---
+    add("rpv_1", "tests/test_solver.cpp:4133-4189", 1, 16, 16, [0.3], [0.9], [_iso_rows(17)],
+        ["user_tau", "user_mu", "rpv"], dict(beam_flux=1.0, beam_mu=0.6),
+        [I(3, 0, 0, 9.33168983e-02, rtol=1e-3), I(4, 0, 0, 5.20589621e-02, rtol=1e-3),
+         I(5, 0, 0, 4.53763077e-02, rtol=1e-3), I(0, 1, 0, 2.37650510e-02, rtol=1e-3),
+         I(1, 1, 0, 4.10887262e-02, rtol=1e-3), I(2, 1, 0, 8.22310683e-02, rtol=1e-3)],
+        tau_user=[0.0, 0.3], mu_user=umu6, phi_user=[0.0], rpv=[0.1, 0.9, -0.2, 0.0, 0.0, 0.0, 1.0])
+    mu30 = float(np.cos(pi / 6.0))
+    rpv_veg = [0.027, 0.647, -0.169, 0.0, 0.0, 0.0, 1.0]
+    tr = [F("flux_direct_beam", 0, 8.66025404e-01, rtol=1e-3), F("flux_up", 0, 4.84172253e-02, rtol=1e-3)]
+    for j, vals in enumerate([[1.49228151e-02, 1.24765511e-02, 1.07957108e-02, 1.56340156e-02],
+                              [1.89187154e-02, 1.59930084e-02, 1.41192980e-02, 1.56340156e-02],
+                              [2.50577685e-02, 2.15197688e-02, 2.00132506e-02, 1.56340156e-02]]):
+        tr += [I(iu, 0, j, v, rtol=1e-3) for iu, v in enumerate(vals)]
+    add("rpv_transparent", "tests/test_solver.cpp:4827-4862,4928-4958", 1, 16, 16, [1e-10], [0.0], [_iso_rows(17)],
+        ["user_tau", "user_mu", "rpv"], dict(beam_flux=1.0, beam_mu=mu30), tr,
+        tau_user=[0.0], mu_user=[0.1, 0.2, 0.5, 1.0], phi_user=[0.0, 90.0, 180.0], rpv=rpv_veg)
+    ml = [F("flux_direct_beam", 0, 8.66025404e-01, rtol=1e-3), F("flux_up", 0, 1.94651642e-01, rtol=1e-3),
+          F("flux_up", 1, 7.85885055e-02, rtol=1e-3), F("flux_down", 1, 1.51472115e-01, rtol=1e-3),
+          F("flux_up", 2, 4.12358568e-02, rtol=1e-3), F("flux_down", 2, 2.99006899e-01, rtol=1e-3)]
+    for (lu, j), vals in {(0, 0): [1.09772501e-01, 9.81705389e-02, 6.23665968e-02, 4.84158344e-02],
+                          (2, 0): [1.51962027e-02, 1.25279793e-02, 1.03952134e-02, 1.26614890e-02],
+                          (0, 2): [1.17340949e-01, 1.07569692e-01, 7.88376901e-02, 4.84158344e-02]}.items():
+        ml += [I(iu, lu, j, v, rtol=1e-3) for iu, v in enumerate(vals)]
+    add("rpv_multilayer", "tests/test_solver.cpp:5000-5049,5129-5162", 2, 16, 16, [0.32, 0.32], [1.0, 1.0],
+        [_ray_rows(17), _hg_rows(0.7, 17)], ["user_tau", "user_mu", "rpv"], dict(beam_flux=1.0, beam_mu=mu30), ml,
+        tau_user=[0.0, 0.32, 0.64], mu_user=[0.1, 0.2, 0.5, 1.0], phi_user=[0.0, 90.0, 180.0], rpv=rpv_veg)
+
+    # --- Pseudo-spherical beam through the general solver (SphericalGeometry) ---
+    add("spherical_general", "tests/test_solver.cpp:4459-4538", 2, 16, 0, [0.5, 1.0], [0.9, 0.8],
+        [_hg_rows(0.8, 17)] * 2, ["user_mu", "spherical"], dict(beam_flux=1.0, beam_mu=0.5, albedo=0.1, radius=6371.0),
+        [F("flux_direct_beam", 0, 5.0e-01, rtol=1e-3), F("flux_up", 0, 8.80100177e-02, rtol=1e-3),
+         F("flux_direct_beam", 2, 2.55423483e-02, rtol=1e-3), F("flux_down", 2, 1.99378710e-01, rtol=1e-3),
+         F("flux_up", 2, 2.25146963e-02, rtol=1e-3)],
+        mu_user=[-0.5, 0.1, 0.5, 1.0], phi_user=[0.0], alt=[60.0, 30.0, 0.0])
+    return cases
+
+
 if __name__ == "__main__":
     with open(os.path.join(HERE, "reference_cases.json"), "w") as f:
         json.dump(reference_cases(), f, indent=1)
     with open(os.path.join(HERE, "numerics_ref.json"), "w") as f:
         json.dump(numerics_ref(), f)
-    print("wrote reference_cases.json, numerics_ref.json")
+    with open(os.path.join(HERE, "intensity_cases.json"), "w") as f:
+        json.dump(intensity_cases(), f, indent=1)
+    print("wrote reference_cases.json, numerics_ref.json, intensity_cases.json")

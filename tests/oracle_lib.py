@@ -44,6 +44,10 @@ def lib():
         l.oracle_ld_flux_one.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_char_p, C.c_int]
         l.oracle_flux_batch.argtypes = [C.c_void_p, C.c_int, PD]
         l.oracle_flux_batch.restype = C.c_int64
+        l.oracle_general_one.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         PI, C.c_char_p, C.c_int]
+        l.oracle_rpv.argtypes = [C.c_void_p, D, D, D]
+        l.oracle_rpv.restype = D
         _lib = l
     return _lib
 
@@ -135,3 +139,35 @@ def flux_batch(batch: FluxBatch, nthreads=0):
     if failed:
         raise OracleError(2, f"{failed} oracle instances failed")
     return secs.value
+
+
+def general_one(gb, i, index_from_bottom=False):
+    """Oracle solve of column i of a disort_amd.general.GeneralBatch.
+    Returns (fluxes [8][ntau], intensity [ntau][numu][nphi],
+    azimuthal average [ntau][numu], modes computed)."""
+    flux = np.zeros((NOUT, gb.ntau))
+    inten = np.zeros((gb.ntau, gb.numu, max(gb.nphi, 0)))
+    avg = np.zeros((gb.ntau, gb.numu))
+    modes = C.c_int(0)
+    err = C.create_string_buffer(512)
+    st = gb.c_struct()
+    rc = lib().oracle_general_one(C.byref(st), i, int(index_from_bottom), flux.ctypes.data, inten.ctypes.data,
+                                  avg.ctypes.data, C.byref(modes), err, 512)
+    if rc:
+        raise OracleError(rc, err.value.decode())
+    return flux, inten, avg, modes.value
+
+
+def general_all(gb):
+    """Oracle solve of every column; fills gb.flux / gb.inten / gb.inten_avg."""
+    for i in range(gb.n):
+        f, u, a, _ = general_one(gb, i)
+        gb.flux[:, i] = f
+        gb.inten[i] = u
+        gb.inten_avg[i] = a
+    return gb
+
+
+def rpv(params, mu1, mu2, phi):
+    p = np.ascontiguousarray(params, dtype=np.float64)
+    return lib().oracle_rpv(p.ctypes.data, mu1, mu2, phi)

@@ -1,0 +1,147 @@
+"""Pins of the general-solver oracle (oracle/general_oracle.cpp,
+DisortSolver::solve): the reference's own intensity tables
+(tests/golden/intensity_cases.json, transcribed from tests/test_solver.cpp)
+and properties the mathematics fixes."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from disort_amd import general as G
+from disort_amd.batch import NSCAL, OUT_FIELDS, scalars, uniform_batch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CASES = json.load(open(os.path.join(HERE, "golden", "intensity_cases.json")))
+FLAG = {"thermal": 1, "spherical": 2, "delta_m_plus": 4, "user_tau": G.G_USER_TAU, "user_mu": G.G_USER_MU,
+        "only_fluxes": G.G_ONLY_FLUXES, "rpv": G.G_BRDF_RPV, "corr_buras": G.G_CORR_BURAS,
+        "corr_nakajima": G.G_CORR_NAKAJIMA}
+
+
+def case_batch(c, n=1):
+    """GeneralBatch of n identical columns from a fixture case."""
+    flags = 0
+    for f in c["flags"]:
+        flags |= FLAG[f]
+    sc = scalars(1, **{k: v for k, v in c["scal"].items() if k != "beam_phi"})
+    sc[0, G.S_BEAM_PHI] = c["scal"].get("beam_phi", 0.0)
+    rep = lambda a: None if a is None else np.repeat(np.asarray(a, np.float64)[None], n, axis=0)
+    return G.GeneralBatch(
+        flags=flags, nstr=c["nstr"], dtau=rep(c["dtau"]), ssa=rep(c["ssa"]), pmom=rep(c["pmom"]),
+        scal=np.repeat(sc, n, axis=0), temper=rep(c["temper"]), alt=rep(c["alt"]), rpv=rep(c["rpv"]),
+        tau_user=rep(c["tau_user"]), mu_user=c["mu_user"], phi_user=c["phi_user"], accuracy=c["accuracy"])
+
+
+def check_expect(c, flux, inten, label=""):
+    for e in c["expect"]:
+        if e["q"] == "intensity":
+            got = inten[e["lu"], e["iu"], e["j"]]
+            where = f"I(iu={e['iu']}, lu={e['lu']}, j={e['j']})"
+        else:
+            got = flux[OUT_FIELDS.index(e["q"]), e["lu"]]
+            where = f"{e['q']}[{e['lu']}]"
+        tol = e["atol"] if e["atol"] is not None else e["rtol"] * abs(e["v"])
+        assert abs(got - e["v"]) <= tol, f"{label}{c['name']} {where}: {got!r} vs {e['v']!r} (tol {tol:g})"
+
+
+@pytest.mark.parametrize("c", CASES, ids=[c["name"] for c in CASES])
+def test_reference_tables(c):
+    gb = case_batch(c)
+    flux, inten, avg, modes = ol.general_one(gb, 0)
+    check_expect(c, flux, inten)
+    assert np.all(np.isfinite(inten)) and np.all(np.isfinite(flux))
+
+
+def test_fluxes_match_flux_solver():
+    """At m = 0 the general solver's fluxes are the flux-only solver's
+    (src/DisortSolver.cpp:1980-2124 vs src/FluxSolver.cpp:1138-1287)."""
+    r = np.random.default_rng(5)
+    L, nstr, P = 7, 8, 10
+    dtau = r.uniform(0.01, 2.0, (1, L))
+    ssa = r.uniform(0.1, 0.99, (1, L))
+    g = r.uniform(0.0, 0.8, L)
+    pmom = (g[:, None] ** np.arange(P)[None, :])[None]
+    temper = np.linspace(220, 290, L + 1)[None]
+    kw = dict(beam_flux=2.0, beam_mu=0.37, albedo=0.3, iso_top=0.1, temp_top=150.0, temp_bot=295.0,
+              wn_lo=600.0, wn_hi=650.0)
+    fb = uniform_batch(dtau, ssa, pmom, nstr, scalars(1, **kw), flags=1, temper=temper)
+    ref = ol.flux_one(fb, 0)
+    sc = scalars(1, **kw)
+    gb = G.GeneralBatch(flags=1 | G.G_USER_MU, nstr=nstr, dtau=dtau, ssa=ssa, pmom=pmom, scal=sc, temper=temper,
+                        mu_user=[-0.9, -0.3, 0.2, 0.7], phi_user=[0.0, 45.0])
+    flux, inten, avg, modes = ol.general_one(gb, 0)
+    assert modes == nstr
+    for f, name in enumerate(OUT_FIELDS):
+        np.testing.assert_allclose(flux[f], ref[f], rtol=1e-12, atol=1e-14, err_msg=name)
+
+
+def test_azimuthal_symmetry_and_average():
+    """With the beam at phi0 = 0, I(phi) = I(360 - phi); the phi-average of
+    the Fourier sum over a uniform 2*nstr-point azimuth grid is the m = 0
+    term."""
+    c = next(x for x in CASES if x["name"] == "rpv_multilayer")
+    c = dict(c, phi_user=[float(p) for p in np.arange(32) * (360.0 / 32)], flags=["user_tau", "user_mu", "rpv"])
+    gb = case_batch(c)
+    flux, inten, avg, modes = ol.general_one(gb, 0)
+    assert modes == 16
+    for j in range(1, 32):
+        np.testing.assert_allclose(inten[:, :, j], inten[:, :, 32 - j], rtol=1e-11, atol=1e-15)
+    np.testing.assert_allclose(inten.mean(axis=2), avg, rtol=1e-11, atol=1e-15)
+
+
+def test_transparent_rpv_is_reconstructed_brdf():
+    """A transparent, non-scattering layer over an RPV surface returns, at the
+    top and for upward angles, mu0 F0 / pi times the 16-term Fourier
+    reconstruction of the BRDF (the bottom-boundary term of
+    src/DisortSolver.cpp:3218-3224).  The reconstruction converges to the
+    exact BRDF, checked here at 2e-3 of its value."""
+    c = next(x for x in CASES if x["name"] == "rpv_transparent")
+    gb = case_batch(c)
+    flux, inten, avg, modes = ol.general_one(gb, 0)
+    mu0 = c["scal"]["beam_mu"]
+    for iu, mu in enumerate(c["mu_user"]):
+        for j, phi in enumerate(c["phi_user"]):
+            exact = mu0 / np.pi * ol.rpv(c["rpv"], mu, mu0, np.radians(phi))
+            assert inten[0, iu, j] == pytest.approx(exact, rel=2e-3)
+
+
+def test_rpv_reciprocity():
+    p = [0.027, 0.647, -0.169, 0.0, 0.0, 0.0, 1.0]
+    for mu1, mu2, phi in [(0.2, 0.7, 0.3), (0.9, 0.15, 2.0), (0.5, 0.5, 1.0)]:
+        assert ol.rpv(p, mu1, mu2, phi) == pytest.approx(ol.rpv(p, mu2, mu1, phi), rel=1e-13)
+
+
+def test_accuracy_shortcut_agrees():
+    """accuracy_fourier_series > 0 stops the mode loop after two modes that
+    each change every intensity by at most that fraction
+    (src/DisortSolver.cpp:336-346); the truncated series stays within a few
+    times that accuracy of the full one (tests/test_solver.cpp:3083-3108)."""
+    for name in ("disotest_9c", "rpv_multilayer"):
+        c = next(x for x in CASES if x["name"] == name)
+        full = ol.general_one(case_batch(dict(c, accuracy=0.0)), 0)
+        for acc in (1e-2, 1e-3):
+            part = ol.general_one(case_batch(dict(c, accuracy=acc)), 0)
+            assert part[3] < full[3] == c["nstr"]
+            nz = np.abs(full[1]) > 1e-30
+            np.testing.assert_allclose(part[1][nz], full[1][nz], rtol=acc)
+
+
+def test_index_from_bottom_reverses():
+    c = next(x for x in CASES if x["name"] == "disotest_9c")
+    gb = case_batch(c)
+    f0, u0, a0, _ = ol.general_one(gb, 0)
+    total = sum(c["dtau"])
+    rev = dict(c, dtau=c["dtau"][::-1], ssa=c["ssa"][::-1], pmom=c["pmom"][::-1], temper=c["temper"][::-1],
+               tau_user=[total - t for t in c["tau_user"][::-1]])
+    f1, u1, a1, _ = ol.general_one(case_batch(rev), 0, index_from_bottom=True)
+    np.testing.assert_allclose(f1[:, ::-1], f0, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(u1[::-1], u0, rtol=1e-10, atol=1e-12)
+
+
+def test_validation_messages():
+    c = next(x for x in CASES if x["name"] == "iv_1a")
+    with pytest.raises(ol.OracleError, match="mu_user must be in increasing order"):
+        ol.general_one(case_batch(dict(c, mu_user=[0.5, 0.1, 0.2, 0.3, 0.4, 0.6])), 0)
+    with pytest.raises(ol.OracleError, match=r"phi_user\[0\] must be in \[0, 360\]"):
+        ol.general_one(case_batch(dict(c, phi_user=[400.0])), 0)

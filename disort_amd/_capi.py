@@ -74,6 +74,7 @@ def lib():
         l.disort_flux_autotune.argtypes = [vp, vp, vp, C.c_int]
         l.disort_flux_plan_report.argtypes = [vp]
         l.disort_flux_plan_report.restype = C.c_char_p
+        l.disort_general_solve.argtypes = [vp, vp, C.c_int]
         _lib = l
     return _lib
 
@@ -114,8 +115,8 @@ class Context:
         lib().disort_set_timing(self._ptr, int(on))
 
     def stage_times(self):
-        ms = (C.c_double * 4)()
-        n = lib().disort_stage_times(self._ptr, ms, 4)
+        ms = (C.c_double * 8)()
+        n = lib().disort_stage_times(self._ptr, ms, 8)
         return list(ms[:n])
 
     def sync(self):
@@ -127,6 +128,14 @@ class Context:
         rc = lib().disort_flux_solve(self._ptr, C.byref(st), MEM_HOST)
         _check(self._ptr, rc)
         return batch
+
+    def general_solve(self, gb):
+        """Solve a host disort_amd.general.GeneralBatch in place (gb.flux,
+        gb.inten, gb.inten_avg, gb.status)."""
+        st = gb.c_struct()
+        rc = lib().disort_general_solve(self._ptr, C.byref(st), MEM_HOST)
+        _check(self._ptr, rc)
+        return gb
 
     # -- device helpers -----------------------------------------------------
     def alloc(self, nbytes: int) -> int:

@@ -19,6 +19,7 @@
 #include "../../include/disort_amd.h"
 #include "flux_args.hpp"
 #include "flux_layout.hpp"
+#include "ctx.hpp"
 
 namespace dsa {
 // Register kernels (flux_kernels.hip), N in {4,...,16}.
@@ -29,34 +30,7 @@ bool launch_flux_bucket_wg(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_
 
 using dsa::FluxArgs;
 
-struct DevBuf {
-  void* p = nullptr;
-  size_t bytes = 0;
-};
-
-struct disort_flux_plan;
-
-struct disort_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
-  bool timing = false;
-  hipEvent_t ev[5] = {};
-  double stage_ms[4] = {0, 0, 0, 0};
-  std::map<std::string, DevBuf> bufs;
-  std::map<int, double*> tables;  // per stream count
-};
-
-namespace {
-
-#define HIPCHK(ctx, call)                                                              \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess) {                                                            \
-      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                  \
-      return DISORT_E_HIP;                                                             \
-    }                                                                                  \
-  } while (0)
+namespace dsa_host {
 
 int fail(disort_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->err = msg;
@@ -150,9 +124,16 @@ int table_for(disort_ctx* ctx, int nstr, double** out) {
   return DISORT_OK;
 }
 
+bool gpu_supported_nstr(int n) { return (n >= 4 && n <= 16 && n % 2 == 0) || n == 32 || n == 64; }
+
+}  // namespace dsa_host
+
+using namespace dsa_host;
+
+namespace {
+
 bool lane_kernels(int n) { return n >= 4 && n <= 16 && n % 2 == 0; }
 bool wg_kernels(int n) { return (n >= 4 && n <= 16 && n % 2 == 0) || n == 32 || n == 64; }
-bool gpu_supported_nstr(int n) { return lane_kernels(n) || wg_kernels(n); }
 int default_variant(int n) { return lane_kernels(n) ? DISORT_V_LANE : DISORT_V_WORKGROUP; }
 bool variant_ok(int n, int v) { return (v == DISORT_V_LANE && lane_kernels(n)) || (v == DISORT_V_WORKGROUP && wg_kernels(n)); }
 
@@ -370,6 +351,7 @@ int launch_bucket(disort_ctx* ctx, const FluxArgs& a, const Bucket& bk, hipStrea
 int launch_plan(disort_ctx* ctx, disort_flux_plan* plan, const PlanArgs& pa, bool timed) {
   int rc;
   for (int k = 0; k < 4; ++k) ctx->stage_ms[k] = 0.0;
+  ctx->n_stages = 4;
   if (!plan->concurrent || timed) {
     for (size_t k = 0; k < plan->buckets.size(); ++k) {
       const Bucket& bk = plan->buckets[k];
@@ -442,6 +424,7 @@ int disort_ctx_create(int device, disort_ctx** out) {
   HIPCHK(ctx, hipSetDevice(device));
   HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   for (auto& e : ctx->ev) HIPCHK(ctx, hipEventCreate(&e));
+  for (auto& e : ctx->gev) HIPCHK(ctx, hipEventCreate(&e));
   *out = ctx.release();
   return DISORT_OK;
 }
@@ -455,6 +438,9 @@ int disort_ctx_destroy(disort_ctx* ctx) {
   for (auto& kv : ctx->tables) (void)hipFree(kv.second);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->gev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& kv : ctx->mode_tables) (void)hipFree(kv.second);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return DISORT_OK;
@@ -477,8 +463,8 @@ int disort_set_timing(disort_ctx* ctx, int enable) {
 }
 
 int disort_stage_times(disort_ctx* ctx, double* ms, int max_stages) {
-  const int n = std::min(max_stages, 4);
-  for (int k = 0; k < n; ++k) ms[k] = ctx->stage_ms[k];
+  const int n = std::min(max_stages, ctx->n_stages);
+  for (int k = 0; k < n; ++k) ms[k] = (ctx->n_stages == 8) ? ctx->gstage_ms[k] : ctx->stage_ms[k];
   return n;
 }
 

@@ -271,6 +271,7 @@ __global__ __launch_bounds__(64) void k_column(FluxArgs a) {
     a.inst_i[i * kInstI + II_NCUT] = ncut_f;
     a.inst_i[i * kInstI + II_LYRCUT] = lyrcut ? 1 : 0;
     a.inst_i[i * kInstI + II_STATUS] = 0;
+    a.inst_i[i * kInstI + II_SCALING] = (deltam ? 1 : 0) | (dmp ? 2 : 0);
     a.inst_d[i * kInstD + ID_BTRANS_NCUT] = (ncut_f == 0) ? bt_prev : a.lay_d[LD_BT_BOT * TL + lo + ncut_f - 1];
   }
   // Planck function per level (src/FluxSolver.cpp:104-122, :748-752).

@@ -30,7 +30,8 @@ enum InstD {
   kInstD = 4
 };
 // Per-instance record (ints), index inst*kInstI + field.
-enum InstI { II_NCUT = 0, II_LYRCUT = 1, II_STATUS = 2, kInstI = 4 };
+// II_SCALING: bit 0 Delta-M applied, bit 1 Delta-M+ applied.
+enum InstI { II_NCUT = 0, II_LYRCUT = 1, II_STATUS = 2, II_SCALING = 3, kInstI = 4 };
 
 // Per-layer record (doubles), SoA: field*total_layers + global_layer.
 enum LayD {

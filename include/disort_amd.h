@@ -275,6 +275,31 @@ typedef struct disort_general_batch {
   int32_t* status;      /* out [n], may be NULL                                  */
 } disort_general_batch;
 
+/*
+ * Solve a general batch.  Replaces DisortSolver::solve per column
+ * (src/DisortSolver.cpp:21-399) and the band loop of solveSpectralBands
+ * (src/python_bindings.cpp:96-128).  memspace DISORT_MEM_HOST: arrays are
+ * host arrays, validated with the rules and messages of
+ * DisortConfig::validate (src/DisortConfig.cpp:72-249), copied in, solved,
+ * copied out.  DISORT_MEM_DEVICE: array pointers are device pointers (not
+ * validated; mu_user, phi_user and rpv are read back once), outputs stay on
+ * the device, the call returns after the launches are queued on the context
+ * stream.  Supported on the GPU: nstr in {4,...,16,32,64}; user optical
+ * depths, polar angles and azimuths; Lambertian or RPV surface;
+ * plane-parallel or pseudo-spherical beam; thermal emission; Delta-M /
+ * Delta-M+; layer cut-off; diffusion lower boundary; accuracy_fourier_series;
+ * with DISORT_G_CORR_BURAS the Nakajima-Tanaka correction
+ * (src/DisortSolver.cpp:3403-3524) and the BRDF beam-reflection correction
+ * (:3751-3808).  Not here: SPECIAL_BC, the Hapke / Cox-Munk / Ross-Li BRDFs,
+ * the Buras-Emde correction (it needs tabulated phase functions), temperature
+ * Jacobians.  All modes of every column are solved in parallel; the
+ * accuracy_fourier_series cut is applied when they are summed, which gives
+ * the reference's result.  Stage times (disort_stage_times, 8 stages):
+ * set-up, BRDF beam terms, layers, boundary, fluxes, user coefficients, user
+ * intensities, mode sum and corrections.
+ */
+int disort_general_solve(disort_ctx* ctx, const disort_general_batch* batch, int memspace);
+
 
 /* =========================================================================
  * Learned flux surrogate (no counterpart in the reference).  A multilayer

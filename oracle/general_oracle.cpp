@@ -6,8 +6,9 @@
 // TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 //
 // Not restated (the caller gets std::invalid_argument): SPECIAL_BC
-// (computeAlbTrans), the Hapke / Cox-Munk / Ross-Li BRDFs, the Buras-Emde
-// correction (tabulated phase functions), temperature Jacobians.
+// (computeAlbTrans), the Hapke / Cox-Munk / Ross-Li BRDFs, temperature
+// Jacobians; the Buras-Emde correction needs tabulated phase functions,
+// which this interface does not carry.
 // The eigenproblem uses the same symmetric route as flux_oracle.cpp (see the
 // deviations listed there).
 #include <algorithm>
@@ -1245,10 +1246,10 @@ struct GeneralSolver {
       }
     }
     const bool corint = c.corr_buras && deltam && c.beam_flux > 0.0 && scat_yes && !c.only_fluxes;
-    if (corint) {
-      if (!c.corr_nakajima) throw std::invalid_argument("oracle: the Buras-Emde correction is not restated");
-      nt_correction(out);
-    }
+    // Nakajima-Tanaka unless a tabulated phase function selects Buras-Emde
+    // (src/DisortSolver.cpp:369-377); this interface carries none
+    // (num_phase_func_angles = 0), so corr_nakajima does not change the result.
+    if (corint) nt_correction(out);
     if (c.corr_buras && !deltam_plus && !c.lambertian && !c.only_fluxes && c.use_user_mu && c.beam_flux > 0.0)
       brdf_correction(out);
     return out;

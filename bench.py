@@ -275,15 +275,104 @@ def bench_c5(args) -> dict:
     return line
 
 
+def bench_c3(args) -> dict:
+    """C3: the general solver (azimuthal modes, user intensities, RPV, pseudo-
+    spherical beam, Nakajima-Tanaka) on 4,096 columns resident in HBM."""
+    import ctypes as C
+
+    import numpy as np
+
+    from disort_amd import flops, general as G, workloads
+    from disort_amd._capi import Context
+
+    n = args.columns
+    gb = workloads.c3(n)
+    ctx = Context(0)
+    dev = G.DeviceGeneralBatch(ctx, gb)
+    for _ in range(args.warmup):
+        dev.execute()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev.execute()
+    ctx.sync()
+    sec = (time.perf_counter() - t0) / args.steps
+    ctx.set_timing(True)
+    dev.execute()
+    stage = ctx.stage_times()
+    ctx.set_timing(False)
+    out = dev.download()
+    assert (out.status == 0).all(), "non-zero column status"
+    names = ["setup", "k_gsurf_b", "k_glayer", "k_gboundary", "k_gflux", "k_gucoef", "k_guser", "k_gaccum+corrections"]
+    stage_ms = dict(zip(names, stage))
+    # Layers above the absorption cut-off per column (the boundary problem's size).
+    abst = np.cumsum((1.0 - gb.ssa) * gb.dtau, axis=1)
+    ncut = np.minimum((abst < 10.0).sum(axis=1) + 1, gb.nlyr)
+    model = {"k_glayer": 0.0, "k_gboundary": 0.0, "k_guser": 0.0}
+    for nc_ in np.unique(ncut):
+        cnt = int((ncut == nc_).sum())
+        for k, v in flops.general_column_flops(gb.nstr, gb.nlyr, gb.numu, gb.ntau, int(nc_)).items():
+            model[k] += cnt * v
+    fp = {k: model[k] / (stage_ms[k] * 1e-3) / 1e12 for k in model}
+    dom = max(model, key=lambda k: stage_ms[k])
+    line = {
+        "metric": "DISORT general-solver columns/sec (C3: nlyr=80, nstr=32, 32 modes, 16x8 user angles, fp64)",
+        "value": n / sec,
+        "unit": "columns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * sec,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Philox draws, disort_amd.workloads.c3)",
+        "config": {"workload": f"C3 general solver: {n} columns x nlyr=80 x nstr=32, 32 Fourier modes, "
+                               "16 polar x 8 azimuth user angles at 81 levels, RPV, pseudo-spherical, NT",
+                   "columns": n, "nlyr": gb.nlyr, "nstr": gb.nstr, "numu": gb.numu, "nphi": gb.nphi,
+                   "ntau": gb.ntau, "mean_ncut": float(ncut.mean()),
+                   "path": "disort_general_solve on device arrays (DISORT_MEM_DEVICE)"},
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": fp[dom], "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": fp[dom] / PEAK_FP64_TFLOPS, "traffic": None},
+        "kernel_fp64_tflops": fp,
+        "stage_ms": stage_ms,
+    }
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+
+        l = oracle_lib.lib()
+        l.oracle_general_batch.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+        l.oracle_general_batch.restype = C.c_int64
+        threads = cpu_threads()
+        sample = min(n, args.cpu_columns)
+        ref = workloads.c3(sample)
+        st = ref.c_struct()
+        secs = C.c_double()
+        l.oracle_general_batch(C.byref(st), 0, min(sample, threads), threads, C.byref(secs))  # warm-up
+        failed = l.oracle_general_batch(C.byref(st), 0, sample, threads, C.byref(secs))
+        k = min(sample, 8)
+        err = float(np.max(np.abs(out.inten[:k] - ref.inten[:k]) / (np.abs(ref.inten[:k]) + 1e-12 * np.abs(ref.inten[:k]).max())))
+        line["cpu_baseline"] = {"value": (sample - failed) / secs.value, "unit": "columns/s", "cores": threads,
+                                "kind": "port", "sample": f"first {sample} C3 columns, oracle general solver, "
+                                f"OpenMP static loop over columns ({secs.value:.2f} s)",
+                                "max_rel_diff_vs_gpu_first_8_columns": err}
+    dev.close()
+    return line
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--bands", type=int, default=10000)
-    ap.add_argument("--workload", choices=("c2", "c4", "c5"), default="c2",
-                    help="c2: the headline spectral flux solve (default); c4: ragged autotuned batch; "
-                         "c5: learned surrogate behind the error gate")
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
+                    help="c2: the headline spectral flux solve (default); c3: the general solver with "
+                         "intensities; c4: ragged autotuned batch; c5: learned surrogate behind the error gate")
+    ap.add_argument("--columns", type=int, default=4096, help="C3 columns")
+    ap.add_argument("--cpu-columns", type=int, default=64, help="C3 columns timed on the CPU")
     ap.add_argument("--train-columns", type=int, default=2097152, help="C5 training columns")
     ap.add_argument("--epochs", type=int, default=20, help="C5 training epochs per member")
     ap.add_argument("--serve-columns", type=int, default=1048576, help="C5 fresh columns served")
@@ -299,6 +388,9 @@ def main() -> int:
             file=sys.stderr,
         )
         return 2
+    if args.workload == "c3":
+        print(json.dumps(bench_c3(args)))
+        return 0
     if args.workload == "c4":
         print(json.dumps(bench_c4(args)))
         return 0

@@ -58,3 +58,36 @@ def instance_bytes(nstr: int, nlyr: int, pstride: int, thermal: bool = True) -> 
     inputs = 8 * (2 * L + L * pstride + (L + 1 if thermal else 0) + 12)  # dtau, ssa, pmom, T, scalars
     outputs = 8 * 8 * (L + 1)
     return inputs + outputs
+
+
+def general_mode_flops(nstr: int, nlyr: int, m: int, numu: int, ntau: int, ncut: int | None = None,
+                       thermal: bool = False, beam: bool = True) -> dict:
+    """Work of one azimuthal mode m of one column of the general solver
+    (DisortSolver::solve, src/DisortSolver.cpp:218-350), by GPU kernel.
+    Legendre sums run over l = m..N-1; the boundary problem over the ncut
+    layers above the absorption cut-off; user intensities integrate on
+    average half of the layers per (level, angle)."""
+    N, L = nstr, nlyr
+    NN = N // 2
+    A = N - m  # active Legendre orders
+    ncut = L if ncut is None else ncut
+    n = N * ncut
+    w = 3 * NN - 1
+    layer = L * (2 * NN * NN * A + 3 * NN * N            # A_s, -B_s from the scaled tables
+                 + (50.0 / 3.0) * NN**3 + 12 * NN * NN   # reduced eigenproblem
+                 + ((2.0 / 3.0) * N**3 + 5 * N * N + N if beam else 0.0)
+                 + ((2.0 / 3.0) * N**3 + 5 * N * N + 3 * N if thermal and m == 0 else 0.0)
+                 + 2 * A * N * NN + 2 * numu * N * A      # eigenvectors at the user angles
+                 + 2 * 3 * A * NN + 2 * 3 * numu * A)     # sources at the user angles
+    boundary = ncut * 2 * N * N + 2 * n * w * w + 6 * n * w
+    user = L * numu * (N * 10.0) + ntau * numu * ((L / 2.0) * 8.0 + N * 12.0)
+    return {"k_glayer": float(layer), "k_gboundary": float(boundary), "k_guser": float(user)}
+
+
+def general_column_flops(nstr: int, nlyr: int, numu: int, ntau: int, ncut: int | None = None,
+                         thermal: bool = False, beam: bool = True) -> dict:
+    tot = {"k_glayer": 0.0, "k_gboundary": 0.0, "k_guser": 0.0}
+    for m in range(nstr):
+        for k, v in general_mode_flops(nstr, nlyr, m, numu, ntau, ncut, thermal, beam).items():
+            tot[k] += v
+    return tot

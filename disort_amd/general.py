@@ -163,3 +163,51 @@ class GeneralBatch:
 
     def fluxes(self, i: int) -> dict:
         return {name: self.flux[f, i].copy() for f, name in enumerate(OUT_FIELDS)}
+
+
+class DeviceGeneralBatch:
+    """A GeneralBatch uploaded once to HBM; execute() solves it with
+    disort_general_solve(..., DISORT_MEM_DEVICE), asynchronously on the
+    context stream; download() copies the outputs back."""
+
+    def __init__(self, ctx, gb: GeneralBatch):
+        from . import _capi
+
+        self.ctx, self.gb, self._bufs = ctx, gb, []
+        st = gb.c_struct()
+
+        def up(a):
+            if a is None:
+                return None
+            p = ctx.alloc(a.nbytes)
+            self._bufs.append(p)
+            ctx.h2d(p, np.ascontiguousarray(a))
+            return p
+
+        for name in ("dtau", "ssa", "pmom", "temper", "alt", "scal", "rpv", "tau_user", "mu_user", "phi_user"):
+            setattr(st, name, up(getattr(gb, name)))
+        for name in ("flux", "inten", "inten_avg", "status"):
+            a = getattr(gb, name)
+            p = ctx.alloc(max(a.nbytes, 8))
+            self._bufs.append(p)
+            setattr(st, name, p)
+        self.cstruct = st
+        self._capi = _capi
+
+    def execute(self):
+        c = self._capi
+        rc = c.lib().disort_general_solve(self.ctx.ptr, C.byref(self.cstruct), c.MEM_DEVICE)
+        c._check(self.ctx.ptr, rc)
+
+    def download(self) -> GeneralBatch:
+        self.ctx.sync()
+        for name in ("flux", "inten", "inten_avg", "status"):
+            a = getattr(self.gb, name)
+            if a.nbytes:
+                self.ctx.d2h(a, getattr(self.cstruct, name))
+        return self.gb
+
+    def close(self):
+        for p in self._bufs:
+            self.ctx.free(p)
+        self._bufs = []

@@ -5,6 +5,7 @@
 #include <exception>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #ifdef _OPENMP
 #include <omp.h>
@@ -214,6 +215,36 @@ int oracle_general_one(const disort_general_batch* b, int64_t i, int index_from_
     put_err(err, errlen, e.what());
     return 2;
   }
+}
+
+/* Columns [c0, c0+cnt) of a general batch with an OpenMP static loop, the
+ * structure of solveSpectralBands (src/python_bindings.cpp:96-128); outputs
+ * into the batch's host arrays.  Returns the number of failed columns;
+ * *seconds receives the wall time of the loop. */
+int64_t oracle_general_batch(const disort_general_batch* b, int64_t c0, int64_t cnt, int nthreads, double* seconds) {
+  int64_t failed = 0;
+  const auto t0 = std::chrono::steady_clock::now();
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  const int64_t per = (int64_t)b->ntau * b->numu;
+#pragma omp parallel for schedule(static) reduction(+ : failed)
+  for (int64_t i = c0; i < c0 + cnt; ++i) {
+    std::vector<double> flux((size_t)DISORT_NOUT * b->ntau);
+    char err[256];
+    int modes = 0;
+    const int rc = oracle_general_one(b, i, 0, flux.data(), b->inten ? b->inten + i * per * b->nphi : nullptr,
+                                      b->inten_avg ? b->inten_avg + i * per : nullptr, &modes, err, 256);
+    if (rc) {
+      ++failed;
+      continue;
+    }
+    for (int f = 0; f < DISORT_NOUT; ++f)
+      for (int l = 0; l < b->ntau; ++l) b->flux[((int64_t)f * b->n + i) * b->ntau + l] = flux[(size_t)f * b->ntau + l];
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return failed;
 }
 
 double oracle_rpv(const double* rpv, double mu1, double mu2, double phi) {

@@ -167,8 +167,12 @@ __global__ __launch_bounds__(64) void k_gsurf_b(GenArgs a) {
 // ===========================================================================
 // k_glayer: one workgroup per (column, mode, layer) task.
 // ===========================================================================
+// One wavefront per task: block-wide barriers are then wave-local and cost
+// next to nothing, and the whole task stays within one SIMD.
+constexpr int kGLT = 64;
+
 template <int NN>
-__global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
+__global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   using C = WideCfg<NN>;
   constexpr int N = C::N, LD = C::LD, NNE = NN + (NN & 1), NP = NNE / 2;
   constexpr TabOff to(N);
@@ -180,12 +184,13 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
   double* const sV = sbuf + 3 * NN * LD;
   __shared__ double sGp[NN * NN], sGm[NN * NN];
   __shared__ double chi[N], p0[N];
+  __shared__ double sP[N * NN], qn[NN], cw[NN];  // Y_l^m(mu_r) for l >= m (zero below), sqrt(w_r / mu_r), w_r
   __shared__ double v0[NN], v1[NN], v2[NN], v3[NN], lam[NN], dv[NN];
   __shared__ double zr_u[NN], zr_d[NN], z0_u[NN], z0_d[NN], z1_h[NN];
   __shared__ double psi[3 * N];
   __shared__ double rc[NP], rs[NP], rt[NP], rapp[NP], raqq[NP], rapq[NP];
   __shared__ int rp[NP], rq[NP];
-  __shared__ double red[C::LT / 64];
+  __shared__ double red[kGLT / 64];
   __shared__ int flag;
 
   const int tid = threadIdx.x;
@@ -203,7 +208,6 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
   const int64_t TL = a.n * L, TV = a.n * (L + 1);
   const double* tb = a.tab;
   const double* Pm = a.ylm + (int64_t)m * N * NN;  // Y_l^m(mu_r), [l][r]
-  const double* Qm = a.ylq + (int64_t)m * N * NN;
   const double ws = a.lay_d[LD_SSA_S * TL + g];
   const double f = a.lay_d[LD_F * TL + g];
   const double sig2 = a.lay_d[LD_SIG2 * TL + g];
@@ -220,6 +224,11 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
     else if (sig2 == 0.0) c = (2.0 * l + 1.0) * ws * (pl - f) / (1.0 - f);
     else c = (2.0 * l + 1.0) * ws * (pl - f * exp(-(double)(l * l) / (2.0 * sig2))) / (1.0 - f);
     chi[l] = c;
+  }
+  for (int e = tid; e < N * NN; e += blockDim.x) sP[e] = (e / NN >= m) ? Pm[e] : 0.0;
+  for (int r = tid; r < NN; r += blockDim.x) {
+    qn[r] = tb[to.sd() + r] / tb[to.cmu() + r];
+    cw[r] = tb[to.cwt() + r];
   }
   // Y_l^m(-mu0) by the recurrence of src/Legendre.cpp:67-120 (the diagonal
   // term as the product the reference builds over successive modes).
@@ -244,21 +253,61 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
   }
   __syncthreads();
 
-  // A_s and -B_s at mode m: sums over l >= m split by the parity of l - m.
+  // Pseudo-spherical beam: xba = 1/CH with its dithers (src/DisortSolver.cpp:
+  // 1442-1456); the Frobenius-norm dither (:1486-1497) follows below.
+  const bool spherical = beam && (a.flags & DISORT_F_SPHERICAL) != 0;
+  const double taut = a.lev_d[VD_TAU_S * TV + v];
+  const double taub = a.lev_d[VD_TAU_S * TV + v + 1];
+  double xba = 0.0;
+  if (spherical) {
+    const double big = sqrt(1.79769313486231571e308) / 1.0e10;
+    xba = 1.0 / a.lay_d[LD_CH * TL + g];
+    if (fabs(xba) > big && taub > 1.0) xba = 0.0;
+    if (fabs(xba * taub) > log(big)) xba = 0.0;
+    if (fabs(xba) > 1.0e-5) {
+      for (int iq = 0; iq < NN; ++iq)
+        if (fabs((fabs(xba) - 1.0 / tb[to.cmu() + iq]) / xba) < 0.05) {
+          xba *= 1.001;
+          break;
+        }
+    }
+  }
+
+  // A_s and -B_s at mode m from the sums over l >= m split by the parity of
+  // l - m, e(r,c) = sum_even chi Y_r Y_c and o(r,c) = sum_odd: with
+  // q_r = sqrt(w_r/mu_r), A_s = q_r q_c e - 1/mu_r delta, -B_s = 1/mu_r delta
+  // - q_r q_c o.  The phase matrix C has the blocks w_c (e +/- o) / 2, from
+  // which the Frobenius norm of I + xba diag(mu) - C is accumulated.
+  double fro = 0.0;
   for (int e = tid; e < NN * NN; e += blockDim.x) {
     const int r0 = e / NN, c0 = e % NN;
     const int r = r0 >= c0 ? r0 : c0, c = r0 >= c0 ? c0 : r0;
     double se = 0.0, so = 0.0;
     for (int l = m; l < N; ++l) {
-      const double q = chi[l] * Qm[l * NN + r] * Qm[l * NN + c];
+      const double q = chi[l] * sP[l * NN + r] * sP[l * NN + c];
       if (((l - m) & 1) == 0) se += q;
       else so += q;
     }
+    const double qq = qn[r] * qn[c];
     const double dg = (r == c) ? 1.0 / tb[to.cmu() + r] : 0.0;
-    sA[r0 * LD + c0] = se - dg;
-    sL[r0 * LD + c0] = dg - so;
+    sA[r0 * LD + c0] = qq * se - dg;
+    sL[r0 * LD + c0] = dg - qq * so;
+    if (spherical) {
+      const double hw = 0.5 * tb[to.cwt() + c0];
+      const double tl = hw * (se + so), tr = hw * (se - so);
+      const double du = (r0 == c0) ? 1.0 + xba * tb[to.cmu() + r0] : 0.0;
+      const double dd = (r0 == c0) ? 1.0 - xba * tb[to.cmu() + r0] : 0.0;
+      fro += (du - tl) * (du - tl) + (dd - tl) * (dd - tl) + 2.0 * tr * tr;
+    }
   }
   __syncthreads();
+  if (spherical) {
+    fro = blk_sum(fro, red);
+    if (1.0 / sqrt(fro) < 1.0e-4) {
+      if (xba == 0.0) xba = 5.0e-9;
+      xba *= 1.00000005;
+    }
+  }
   if (!blk_cholesky<NN>(sL, LD, &flag) && tid == 0) atomicOr(&a.status[i], DISORT_ST_EIGEN);
 
   for (int e = tid; e < NN * NN; e += blockDim.x) {
@@ -382,16 +431,20 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
 
   // G+/- (src/DisortSolver.cpp:1110-1138).
   for (int j = tid; j < NN; j += blockDim.x) {
+#pragma unroll 1
     for (int r = NN - 1; r >= 0; --r) {
       double s = sV[r * LD + j];
+#pragma unroll 1
       for (int mm = r + 1; mm < NN; ++mm) s -= sL[mm * LD + r] * sS[mm * LD + j];
       sS[r * LD + j] = s / sL[r * LD + r];
     }
   }
   __syncthreads();
+#pragma unroll 1
   for (int e = tid; e < NN * NN; e += blockDim.x) {
     const int r = e % NN, j = e / NN;
     double y = 0.0;
+#pragma unroll 1
     for (int mm = 0; mm <= r; ++mm) y += sL[r * LD + mm] * sV[mm * LD + j];
     const double sdi = tb[to.sdi() + r];
     const double uu = sdi * sS[r * LD + j];
@@ -399,14 +452,11 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
     const double gp = 0.5 * (tt + uu), gmv = 0.5 * (tt - uu);
     E[ge.gp() + r + j * NN] = gp;
     E[ge.gm() + r + j * NN] = gmv;
-    sGp[r + j * NN] = gp;
-    sGm[r + j * NN] = gmv;
+    sGp[r * NN + j] = gp;  // row-major: lanes of the interpolation read along j
+    sGm[r * NN + j] = gmv;
   }
 
   const bool thermal = (a.flags & DISORT_F_THERMAL) != 0 && m == 0;
-  const bool spherical = beam && (a.flags & DISORT_F_SPHERICAL) != 0;
-  const double taut = a.lev_d[VD_TAU_S * TV + v];
-  const double taub = a.lev_d[VD_TAU_S * TV + v + 1];
 
   if (thermal || spherical) {
     for (int e = tid; e < NN * NN; e += blockDim.x) sA[e / NN * LD + e % NN] = -sA[e / NN * LD + e % NN];
@@ -476,7 +526,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
   if (beam && tid < NN) {
     double se = 0.0, so = 0.0;
     for (int l = m; l < N; ++l) {
-      const double term = chi[l] * Pm[l * NN + tid] * p0[l];
+      const double term = chi[l] * sP[l * NN + tid] * p0[l];
       if (((l - m) & 1) == 0) se += term;
       else so += term;
     }
@@ -484,7 +534,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
     bdr = fm * F0 * so / (2.0 * kPi);
   }
 
-  double zbu = 0.0, zbd = 0.0, s1u = 0.0, s1d = 0.0, xba = 0.0;
+  double zbu = 0.0, zbd = 0.0, s1u = 0.0, s1d = 0.0;
   if (beam && !spherical) {
     double mub = mu0;
     const double mu0sq = mu0 * mu0;
@@ -494,37 +544,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
     }
     solve_ab(bsr, bdr, false, mub, zbu, zbd);
   } else if (spherical) {
-    // src/DisortSolver.cpp:1397-1531.
-    const double big = sqrt(1.79769313486231571e308) / 1.0e10;
-    xba = 1.0 / a.lay_d[LD_CH * TL + g];
-    if (fabs(xba) > big && taub > 1.0) xba = 0.0;
-    if (fabs(xba * taub) > log(big)) xba = 0.0;
-    if (fabs(xba) > 1.0e-5) {
-      for (int iq = 0; iq < NN; ++iq)
-        if (fabs((fabs(xba) - 1.0 / tb[to.cmu() + iq]) / xba) < 0.05) {
-          xba *= 1.001;
-          break;
-        }
-    }
-    double fro = 0.0;
-    for (int e = tid; e < N * N; e += blockDim.x) {
-      const int i0 = e / N, j0 = e % N;
-      const int ri = i0 < NN ? i0 : i0 - NN, rj = j0 < NN ? j0 : j0 - NN;
-      const double mi = i0 < NN ? tb[to.cmu() + ri] : -tb[to.cmu() + ri];
-      double c = 0.0;
-      for (int l = m; l < N; ++l) {
-        const double sgn = (((l - m) & 1) && ((i0 < NN) != (j0 < NN))) ? -1.0 : 1.0;
-        c += chi[l] * Pm[l * NN + ri] * Pm[l * NN + rj] * sgn;
-      }
-      c *= 0.5 * tb[to.cwt() + rj];
-      const double aij = ((i0 == j0) ? 1.0 + xba * mi : 0.0) - c;
-      fro += aij * aij;
-    }
-    fro = blk_sum(fro, red);
-    if (1.0 / sqrt(fro) < 1.0e-4) {
-      if (xba == 0.0) xba = 5.0e-9;
-      xba *= 1.00000005;
-    }
+    // src/DisortSolver.cpp:1397-1531 (xba and its dithers above).
     const double q0a = exp(-((lc == 0) ? 0.0 : a.lay_d[LD_CHTAU * TL + g - 1]));
     const double q2a = exp(-a.lay_d[LD_CHTAU * TL + g]);
     const double et = exp(xba * taut), eb = exp(xba * taub), dtl = taub - taut;
@@ -624,6 +644,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
   const double* Yu = a.ylu + (int64_t)m * N * numu;  // [l][iu]
   double* W = sbuf;                                   // [l][col], N*N <= 4*NN*LD
   // W(l, col) = chi_l / 2 sum_jq w_jq Y_l^m(mu_jq) G(row(jq), col).
+#pragma unroll 1
   for (int e = tid; e < N * N; e += blockDim.x) {
     const int l = e / N, col = e % N;
     double s = 0.0;
@@ -631,9 +652,10 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
       const double sgn = ((l - m) & 1) ? -1.0 : 1.0;
       const bool up = col >= NN;
       const int jp = up ? col - NN : NN - 1 - col;
+#pragma unroll 2
       for (int r = 0; r < NN; ++r) {
-        const double w = tb[to.cwt() + r] * Pm[l * NN + r];
-        const double gp = sGp[r + jp * NN], gmv = sGm[r + jp * NN];
+        const double w = cw[r] * sP[l * NN + r];
+        const double gp = sGp[r * NN + jp], gmv = sGm[r * NN + jp];
         s += w * (up ? gp + sgn * gmv : -(gmv + sgn * gp));
       }
     }
@@ -645,8 +667,9 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
     double s = 0.0;
     if (l >= m && (k == 0 ? beam : thermal)) {
       const double sgn = ((l - m) & 1) ? -1.0 : 1.0;
+#pragma unroll 2
       for (int r = 0; r < NN; ++r) {
-        const double w = tb[to.cwt() + r] * Pm[l * NN + r];
+        const double w = cw[r] * sP[l * NN + r];
         const double xu = (k == 0) ? zr_u[r] : (k == 1 ? z0_u[r] : z1_h[r]);
         const double xd = (k == 0) ? zr_d[r] : (k == 1 ? z0_d[r] : z1_h[r]);
         s += w * (xu + sgn * xd);
@@ -659,6 +682,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
   for (int e = tid; e < numu * N; e += blockDim.x) {
     const int iu = e / N, col = e % N;
     double s = 0.0;
+#pragma unroll 8
     for (int l = m; l < N; ++l) s += Yu[l * numu + iu] * W[l * N + col];
     gu[e] = s;
   }
@@ -666,6 +690,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_glayer(GenArgs a) {
   const double fact = fm * F0 / (4.0 * kPi);
   for (int iu = tid; iu < numu; iu += blockDim.x) {
     double sb = 0.0, s0 = 0.0, s1 = 0.0;
+#pragma unroll 4
     for (int l = m; l < N; ++l) {
       const double y = Yu[l * numu + iu];
       sb += y * (psi[l] + fact * chi[l] * p0[l]);
@@ -1496,7 +1521,7 @@ void general_chunk_n(const GenArgs& a, const GenLaunch& g, hipStream_t s) {
   rec(0);
   if (g.brdf) hipLaunchKernelGGL(k_gsurf_b<NN>, dim3(ncm), dim3(64), 0, s, a);
   rec(1);
-  hipLaunchKernelGGL(k_glayer<NN>, dim3((unsigned)a.n_tasks), dim3(C::LT), 0, s, a);
+  hipLaunchKernelGGL(k_glayer<NN>, dim3((unsigned)a.n_tasks), dim3(kGLT), 0, s, a);
   rec(2);
   hipLaunchKernelGGL(k_gboundary<NN>, dim3(ncm), dim3(C::BT), 0, s, a);
   rec(3);

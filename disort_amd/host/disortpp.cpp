@@ -6,6 +6,7 @@
 // (src/python_bindings.cpp:130-211).
 #include "disortpp.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
@@ -562,29 +563,48 @@ void DisortConfig::setPhaseFunction(PhaseFunction type, double g, int lc) {
 }
 
 // ---------------------------------------------------------------------------
-// DisortSolver (src/DisortSolver.cpp:21-399): the GPU path covers the
-// flux-only, Lambertian, general-BC problem at the layer boundaries.
+// DisortSolver (src/DisortSolver.cpp:21-399).  Flux-only, Lambertian,
+// general-BC problems at the layer boundaries go through the flux path
+// (disort_flux_solve); everything else the GPU solves -- intensities, user
+// optical depths, the RPV surface, the intensity corrections -- through
+// disort_general_solve.
 // ---------------------------------------------------------------------------
 namespace {
 
+// src/IndexConversion.hpp:16-38 (an involution: applying it twice restores).
 void reverse_general_inputs(DisortConfig& c) {
   std::reverse(c.delta_tau.begin(), c.delta_tau.end());
   std::reverse(c.single_scat_albedo.begin(), c.single_scat_albedo.end());
   std::reverse(c.phase_function_moments.begin(), c.phase_function_moments.end());
   std::reverse(c.temperature.begin(), c.temperature.end());
   std::reverse(c.level_altitudes.begin(), c.level_altitudes.end());
+  if (c.flags.use_user_tau && !c.tau_user.empty()) {
+    double total = 0.0;
+    for (double d : c.delta_tau) total += d;
+    for (double& t : c.tau_user) t = total - t;
+    std::reverse(c.tau_user.begin(), c.tau_user.end());
+  }
+}
+
+bool flux_route(const DisortConfig& c) {
+  return c.flags.comp_only_fluxes && c.flags.use_lambertian_surface && c.flags.ibcnd == BoundaryConditionType::General &&
+         !c.flags.use_user_tau && !c.flags.compute_temperature_jacobian;
 }
 
 void require_gpu_subset(const DisortConfig& c) {
   auto no = [](const std::string& what) {
-    throw std::runtime_error("disortpp: " + what + " is not available on the GPU path (flux-only, Lambertian, "
-                             "general boundary conditions at the layer boundaries)");
+    throw std::runtime_error("disortpp: " + what + " is not available on the GPU path");
   };
-  if (!c.flags.comp_only_fluxes) no("intensity output (comp_only_fluxes=false)");
-  if (!c.flags.use_lambertian_surface) no("a BRDF surface");
   if (c.flags.ibcnd != BoundaryConditionType::General) no("SPECIAL_BC");
-  if (c.flags.use_user_tau) no("use_user_tau");
   if (c.flags.compute_temperature_jacobian) no("compute_temperature_jacobian");
+  if (!c.flags.use_lambertian_surface) {
+    if (c.flags.brdf_type != BrdfType::RPV) no("a BRDF other than RPV");
+    if (!c.brdf.rpv) throw std::runtime_error("bidirReflectivity: RPV BRDF selected but rpv spec is null");
+  }
+  if (c.flags.intensity_corr_buras && !c.flags.intensity_corr_nakajima && c.num_phase_func_angles > 0 &&
+      !c.flags.comp_only_fluxes)
+    no("the Buras-Emde intensity correction (tabulated phase functions)");
+  if (c.flags.output_fourier_expansion) no("output_fourier_expansion");
 }
 
 // Everything DisortSolver::solve does to the config before the solve
@@ -617,8 +637,6 @@ void prepare_general(DisortConfig& c, std::vector<double>& cmu) {
   }
   for (int l = 0; l < L; ++l) c.phase_function_moments[l][0] = 1.0;
   if (rev) reverse_general_inputs(c);
-  // tau_user is an output grid; with index_from_bottom the reference keeps
-  // it top-down (it is not an input array), so it is not reversed here.
 }
 
 Instance general_instance(const DisortConfig& c) {
@@ -651,16 +669,168 @@ DisortResult to_result(const Packed& p, int64_t i, const DisortConfig& c) {
   return r;
 }
 
+// Full-solver batch: columns of one shape (layers, streams, moments, user
+// grids, flags) packed into a disort_general_batch.
+int32_t full_flags(const DisortConfig& c) {
+  int32_t f = general_flags(c);
+  if (c.flags.use_user_tau) f |= DISORT_G_USER_TAU;
+  if (c.flags.use_user_mu) f |= DISORT_G_USER_MU;
+  if (c.flags.comp_only_fluxes) f |= DISORT_G_ONLY_FLUXES;
+  if (!c.flags.use_lambertian_surface) f |= DISORT_G_BRDF_RPV;
+  if (c.flags.intensity_corr_buras) f |= DISORT_G_CORR_BURAS;
+  if (c.flags.intensity_corr_nakajima) f |= DISORT_G_CORR_NAKAJIMA;
+  return f;
+}
+
+struct GeneralPack {
+  int32_t flags = 0;
+  int N = 0, L = 0, P = 0, ntau = 0, numu = 0, nphi = 0;
+  double accuracy = 0.0;
+  std::vector<double> dtau, ssa, pmom, temper, alt, scal, rpv, tau_user, mu_user, phi_user, flux, inten, avg;
+  std::vector<int32_t> status;
+  int64_t n = 0;
+
+  // Whether config c can join this batch.
+  bool fits(const DisortConfig& c) const {
+    if (n == 0) return true;
+    return full_flags(c) == flags && c.num_streams == N && c.num_layers == L && c.nmomNstr() + 1 == P &&
+           c.num_user_tau == ntau && c.num_user_mu == numu && (c.flags.comp_only_fluxes ? 0 : c.num_phi) == nphi &&
+           c.accuracy_fourier_series == accuracy &&
+           std::equal(mu_user.begin(), mu_user.end(), c.mu_user.begin()) &&
+           std::equal(phi_user.begin(), phi_user.end(), c.phi_user.begin());
+  }
+
+  void add(const DisortConfig& cin) {
+    DisortConfig c = cin;
+    if (c.flags.index_from_bottom) reverse_general_inputs(c);
+    if (n == 0) {
+      flags = full_flags(c);
+      N = c.num_streams;
+      L = c.num_layers;
+      P = c.nmomNstr() + 1;
+      ntau = c.num_user_tau;
+      numu = c.num_user_mu;
+      nphi = c.flags.comp_only_fluxes ? 0 : c.num_phi;
+      accuracy = c.accuracy_fourier_series;
+      mu_user.assign(c.mu_user.begin(), c.mu_user.begin() + numu);
+      phi_user.assign(c.phi_user.begin(), c.phi_user.begin() + nphi);
+    }
+    dtau.insert(dtau.end(), c.delta_tau.begin(), c.delta_tau.begin() + L);
+    ssa.insert(ssa.end(), c.single_scat_albedo.begin(), c.single_scat_albedo.begin() + L);
+    for (int l = 0; l < L; ++l)
+      for (int k = 0; k < P; ++k) pmom.push_back(k == 0 ? 1.0 : c.phase_function_moments[l][k]);
+    if (flags & DISORT_F_THERMAL) temper.insert(temper.end(), c.temperature.begin(), c.temperature.begin() + L + 1);
+    if (flags & DISORT_F_SPHERICAL)
+      alt.insert(alt.end(), c.level_altitudes.begin(), c.level_altitudes.begin() + L + 1);
+    if (flags & DISORT_G_USER_TAU) tau_user.insert(tau_user.end(), c.tau_user.begin(), c.tau_user.begin() + ntau);
+    double s[DISORT_NSCAL];
+    fill_scal(s, c.bc.direct_beam_flux, c.bc.direct_beam_mu, c.bc.isotropic_flux_top, c.bc.isotropic_flux_bottom,
+              c.bc.temperature_top, c.bc.temperature_bottom, c.bc.emissivity_top, c.bc.surface_albedo,
+              c.wavenumber_low, c.wavenumber_high, c.bottom_radius);
+    s[DISORT_S_BEAM_PHI] = c.bc.direct_beam_phi;
+    scal.insert(scal.end(), s, s + DISORT_NSCAL);
+    if (flags & DISORT_G_BRDF_RPV) {
+      const RpvBrdfSpec& r = *c.brdf.rpv;
+      const double v[DISORT_NRPV] = {r.rho0, r.k, r.theta, r.sigma, r.t1, r.t2, r.scale};
+      rpv.insert(rpv.end(), v, v + DISORT_NRPV);
+    }
+    ++n;
+  }
+
+  void solve() {
+    if (n == 0) return;
+    flux.assign(static_cast<size_t>(DISORT_NOUT) * n * ntau, 0.0);
+    inten.assign(static_cast<size_t>(n) * ntau * numu * std::max(nphi, 0), 0.0);
+    avg.assign(static_cast<size_t>(n) * ntau * numu, 0.0);
+    status.assign(static_cast<size_t>(n), 0);
+    disort_general_batch b{};
+    b.n = n;
+    b.flags = flags;
+    b.nstr = N;
+    b.nlyr = L;
+    b.pstride = P;
+    b.ntau = ntau;
+    b.numu = numu;
+    b.nphi = nphi;
+    b.accuracy = accuracy;
+    b.dtau = dtau.data();
+    b.ssa = ssa.data();
+    b.pmom = pmom.data();
+    b.temper = temper.empty() ? nullptr : temper.data();
+    b.alt = alt.empty() ? nullptr : alt.data();
+    b.scal = scal.data();
+    b.rpv = rpv.empty() ? nullptr : rpv.data();
+    b.tau_user = tau_user.empty() ? nullptr : tau_user.data();
+    b.mu_user = mu_user.data();
+    b.phi_user = phi_user.empty() ? nullptr : phi_user.data();
+    b.flux = flux.data();
+    b.inten = inten.empty() ? nullptr : inten.data();
+    b.inten_avg = avg.data();
+    b.status = status.data();
+    Gpu& g = gpu();
+    std::lock_guard<std::mutex> lock(g.mu);
+    if (!g.ctx) {
+      const int dev = g.device >= 0 ? g.device : default_device();
+      if (disort_ctx_create(dev, &g.ctx) != DISORT_OK) {
+        g.ctx = nullptr;
+        throw std::runtime_error("disortpp: cannot create a GPU context on device " + std::to_string(dev));
+      }
+      g.device = dev;
+    }
+    const int rc = disort_general_solve(g.ctx, &b, DISORT_MEM_HOST);
+    if (rc == DISORT_OK) return;
+    std::string msg = disort_last_error(g.ctx);
+    if (n == 1 && msg.rfind("column 0: ", 0) == 0) msg = msg.substr(10);
+    if (rc == DISORT_E_INVALID) throw std::invalid_argument(msg);
+    throw std::runtime_error("disortpp: " + msg);
+  }
+
+  // Column i as a DisortResult (levels reversed for index_from_bottom,
+  // src/IndexConversion.hpp:70-97).
+  DisortResult unpack(int64_t i, const DisortConfig& c) const {
+    DisortResult r;
+    const bool rev = c.flags.index_from_bottom;
+    std::vector<double>* f[DISORT_NOUT] = {&r.flux_direct_beam, &r.flux_down, &r.flux_up,
+                                           &r.flux_tau_divergence, &r.mean_intensity, &r.mean_intensity_down,
+                                           &r.mean_intensity_up, &r.mean_intensity_direct_beam};
+    for (int q = 0; q < DISORT_NOUT; ++q) {
+      f[q]->resize(ntau);
+      const double* src = flux.data() + (static_cast<size_t>(q) * n + i) * ntau;
+      for (int l = 0; l < ntau; ++l) (*f[q])[rev ? ntau - 1 - l : l] = src[l];
+    }
+    r.mu_angles.assign(c.mu_user.begin(), c.mu_user.begin() + numu);
+    if (!c.flags.comp_only_fluxes) {
+      r.intensity.assign(ntau, std::vector<std::vector<double>>(numu, std::vector<double>(nphi, 0.0)));
+      r.intensity_azimuth_avg.assign(ntau, std::vector<double>(numu, 0.0));
+      for (int lu = 0; lu < ntau; ++lu) {
+        const int lo = rev ? ntau - 1 - lu : lu;
+        for (int iu = 0; iu < numu; ++iu) {
+          const size_t base = (static_cast<size_t>(i) * ntau + lu) * numu + iu;
+          r.intensity_azimuth_avg[lo][iu] = avg[base];
+          for (int j = 0; j < nphi; ++j) r.intensity[lo][iu][j] = inten[base * nphi + j];
+        }
+      }
+    }
+    return r;
+  }
+};
+
 }  // namespace
 
 DisortResult DisortSolver::solve(DisortConfig& config) {
   std::vector<double> cmu;
   prepare_general(config, cmu);
-  Packed p;
-  p.flags = general_flags(config);
-  p.add(general_instance(config));
-  p.solve();
-  return to_result(p, 0, config);
+  if (flux_route(config)) {
+    Packed p;
+    p.flags = general_flags(config);
+    p.add(general_instance(config));
+    p.solve();
+    return to_result(p, 0, config);
+  }
+  GeneralPack g;
+  g.add(config);
+  g.solve();
+  return g.unpack(0, config);
 }
 
 std::vector<DisortResult> solveSpectralBands(DisortConfig config,
@@ -682,20 +852,38 @@ std::vector<DisortResult> solveSpectralBands(DisortConfig config,
     prepare_general(c, cmu);
   }
   std::vector<DisortResult> results(bands.size());
+  // Flux-only bands: one flux batch per batch-uniform flag set.
   std::vector<int32_t> seen;
   for (const auto& c : cfgs)
-    if (std::find(seen.begin(), seen.end(), general_flags(c)) == seen.end()) seen.push_back(general_flags(c));
+    if (flux_route(c) && std::find(seen.begin(), seen.end(), general_flags(c)) == seen.end())
+      seen.push_back(general_flags(c));
   for (int32_t fl : seen) {
     Packed p;
     p.flags = fl;
     std::vector<size_t> idx;
     for (size_t i = 0; i < cfgs.size(); ++i)
-      if (general_flags(cfgs[i]) == fl) {
+      if (flux_route(cfgs[i]) && general_flags(cfgs[i]) == fl) {
         p.add(general_instance(cfgs[i]));
         idx.push_back(i);
       }
     p.solve();
     for (size_t j = 0; j < idx.size(); ++j) results[idx[j]] = to_result(p, static_cast<int64_t>(j), cfgs[idx[j]]);
+  }
+  // The rest: full-solver batches of equal shape.
+  std::vector<bool> done(cfgs.size(), false);
+  for (size_t i = 0; i < cfgs.size(); ++i) done[i] = flux_route(cfgs[i]);
+  for (size_t i0 = 0; i0 < cfgs.size(); ++i0) {
+    if (done[i0]) continue;
+    GeneralPack g;
+    std::vector<size_t> idx;
+    for (size_t i = i0; i < cfgs.size(); ++i)
+      if (!done[i] && g.fits(cfgs[i])) {
+        g.add(cfgs[i]);
+        idx.push_back(i);
+        done[i] = true;
+      }
+    g.solve();
+    for (size_t j = 0; j < idx.size(); ++j) results[idx[j]] = g.unpack(static_cast<int64_t>(j), cfgs[idx[j]]);
   }
   return results;
 }

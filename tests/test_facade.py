@@ -95,19 +95,25 @@ def test_index_from_bottom_validates_in_top_down_order():
 
 
 def test_general_solver_rejects_what_the_gpu_path_does_not_cover():
+    """Checked before any GPU work: SPECIAL_BC, temperature Jacobians, BRDFs
+    other than RPV, an RPV surface without parameters."""
     cfg = d.DisortConfig(4, 16, 0)
-    cfg.flags.use_lambertian_surface = True
     cfg.allocate()
     cfg.delta_tau = [0.1, 0.5, 1.0, 2.0]
     cfg.single_scat_albedo = [0.9] * 4
     cfg.set_henyey_greenstein(0.8)
     cfg.num_phi = 1
     cfg.phi_user = [0.0]
-    with pytest.raises(RuntimeError, match="intensity output"):
-        d.DisortSolver().solve(cfg)
-    cfg.flags.comp_only_fluxes = True
     cfg.flags.use_lambertian_surface = False
-    with pytest.raises(RuntimeError, match="BRDF"):
+    cfg.flags.brdf_type = d.BrdfType.Hapke
+    with pytest.raises(RuntimeError, match="a BRDF other than RPV"):
+        d.DisortSolver().solve(cfg)
+    cfg.flags.brdf_type = d.BrdfType.RPV
+    with pytest.raises(RuntimeError, match="rpv spec is null"):
+        d.DisortSolver().solve(cfg)
+    cfg.flags.use_lambertian_surface = True
+    cfg.flags.ibcnd = d.BoundaryConditionType.Special
+    with pytest.raises(RuntimeError, match="SPECIAL_BC"):
         d.DisortSolver().solve(cfg)
 
 

@@ -146,3 +146,105 @@ def test_ragged_flux_batch(d):
     for c, r in zip(cfgs, batch):
         single = d.create_flux_solver(c.num_streams).solve(c)
         check(as_array(r), as_array(single), f"ragged batch nstr={c.num_streams} nlyr={c.num_layers}")
+
+
+# --- DisortSolver with intensities (the general GPU path) -------------------
+def iv_config(d, tau, w, beam, rpv=None, nmom=0):
+    """IntensityValidation-style single layer (tests/test_solver.cpp:2613-2670)."""
+    c = d.DisortConfig(1, 16, nmom)
+    c.bc.direct_beam_flux = beam
+    c.bc.direct_beam_mu = 0.1 if rpv is None else 0.6
+    c.flags.use_user_mu = True
+    c.accuracy_fourier_series = 1e-4
+    c.num_user_mu = 6
+    c.num_phi = 1
+    if rpv is not None:
+        c.flags.use_user_tau = True
+        c.num_user_tau = 2
+        c.flags.use_lambertian_surface = False
+        c.flags.brdf_type = d.BrdfType.RPV
+        c.accuracy_fourier_series = 0.0
+    c.allocate()
+    c.set_isotropic(0)
+    c.delta_tau = [tau]
+    c.single_scat_albedo = [w]
+    c.mu_user = [-1.0, -0.5, -0.1, 0.1, 0.5, 1.0]
+    c.phi_user = [0.0]
+    if rpv is not None:
+        c.tau_user = [0.0, tau]
+        spec = d.RpvBrdfSpec()
+        spec.rho0, spec.k, spec.theta = rpv
+        c.brdf.set_rpv(spec)
+    return c
+
+
+def test_general_solver_intensity_table_1a(d):
+    c = iv_config(d, 0.03125, 0.2, math.pi / 0.1)
+    r = d.DisortSolver().solve(c)
+    assert r.num_user_mu() == 6 and r.num_phi() == 1 and r.num_user_tau() == 2
+    # C-DISORT values quoted by tests/test_solver.cpp:2654-2669 (5e-3).
+    for (iu, lu), v in {(3, 0): 1.17770662e-01, (4, 0): 2.64170381e-02, (5, 0): 1.34041302e-02,
+                        (0, 1): 1.33826267e-02, (1, 1): 2.63323514e-02, (2, 1): 1.15897886e-01}.items():
+        assert r.intensities(iu, lu, 0) == pytest.approx(v, rel=5e-3)
+    for iu in range(3):
+        assert abs(r.intensities(iu, 0, 0)) < 1e-10 and abs(r.intensities(iu + 3, 1, 0)) < 1e-10
+
+
+def test_general_solver_rpv_table(d):
+    """RpvBRDF1 (tests/test_solver.cpp:4133-4189), 1e-3."""
+    c = iv_config(d, 0.3, 0.9, 1.0, rpv=(0.1, 0.9, -0.2), nmom=16)
+    r = d.DisortSolver().solve(c)
+    for (iu, lu), v in {(3, 0): 9.33168983e-02, (4, 0): 5.20589621e-02, (5, 0): 4.53763077e-02,
+                        (0, 1): 2.37650510e-02, (1, 1): 4.10887262e-02, (2, 1): 8.22310683e-02}.items():
+        assert r.intensities(iu, lu, 0) == pytest.approx(v, rel=1e-3)
+
+
+def general_cfg(d, L=5, seed=3, rev=False):
+    r = np.random.default_rng(seed)
+    c = d.DisortConfig(L, 8, 0)
+    c.flags.use_user_mu = True
+    c.flags.use_user_tau = True
+    c.flags.use_thermal_emission = True
+    c.flags.index_from_bottom = rev
+    c.num_user_mu, c.num_user_tau, c.num_phi = 4, 3, 3
+    c.allocate()
+    dt = r.uniform(0.1, 1.0, L)
+    c.delta_tau = list(dt[::-1] if rev else dt)
+    w = r.uniform(0.3, 0.95, L)
+    c.single_scat_albedo = list(w[::-1] if rev else w)
+    g = r.uniform(0.1, 0.8, L)
+    for lc in range(L):
+        c.set_henyey_greenstein(float(g[L - 1 - lc] if rev else g[lc]), lc)
+    t = np.linspace(220.0, 290.0, L + 1)
+    c.temperature = list(t[::-1] if rev else t)
+    c.wavenumber_low, c.wavenumber_high = 600.0, 700.0
+    c.bc.temperature_bottom, c.bc.temperature_top = 295.0, 0.0
+    c.bc.direct_beam_flux, c.bc.direct_beam_mu, c.bc.surface_albedo = math.pi, 0.55, 0.2
+    tot = float(dt.sum())
+    tu = [0.0, 0.4 * tot, tot]
+    c.tau_user = [tot - x for x in tu[::-1]] if rev else tu
+    c.mu_user = [-0.8, -0.3, 0.4, 0.9]
+    c.phi_user = [0.0, 60.0, 180.0]
+    return c
+
+
+def test_general_solver_index_from_bottom(d):
+    top = d.DisortSolver().solve(general_cfg(d))
+    bot = d.DisortSolver().solve(general_cfg(d, rev=True))
+    np.testing.assert_allclose(np.array(bot.intensity)[::-1], np.array(top.intensity), rtol=1e-10, atol=1e-13)
+    np.testing.assert_allclose(bot.flux_up[::-1], top.flux_up, rtol=1e-10, atol=1e-13)
+
+
+def test_spectral_bands_with_intensities(d):
+    """solve_spectral_bands on the full solver: one GPU batch, each band
+    equal to a single DisortSolver solve."""
+    c = general_cfg(d)
+    bands = [(600.0 + 10 * k, 610.0 + 10 * k) for k in range(4)]
+    out = d.solve_spectral_bands(c, bands, surface_albedo=[0.0, 0.1, 0.2, 0.3])
+    for k, (lo, hi) in enumerate(bands):
+        one = general_cfg(d)
+        one.wavenumber_low, one.wavenumber_high = lo, hi
+        one.bc.surface_albedo = 0.1 * k
+        ref = d.DisortSolver().solve(one)
+        np.testing.assert_allclose(np.array(out[k].intensity), np.array(ref.intensity), rtol=1e-12, atol=1e-15)
+        np.testing.assert_allclose(out[k].flux_up, ref.flux_up, rtol=1e-12, atol=1e-15)

@@ -362,6 +362,20 @@ def bench_c3(args) -> dict:
     return line
 
 
+def copy_probe(mib: int) -> None:
+    """One elementwise FP64 copy of `mib` MiB on cuda:0: its FETCH_SIZE and
+    WRITE_SIZE counters calibrate the counters of the kernels that follow."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(f"copy probe: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+    del src, dst
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -380,6 +394,9 @@ def main() -> int:
     ap.add_argument("--instances", type=int, default=50000, help="C4 batch size")
     ap.add_argument("--cpu-sample", type=int, default=2000, help="C4 instances timed on the CPU")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="first run one FP64 copy of this many MiB (reads and writes that many bytes), "
+                         "the calibration kernel of HBM counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -388,6 +405,8 @@ def main() -> int:
             file=sys.stderr,
         )
         return 2
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
     if args.workload == "c3":
         print(json.dumps(bench_c3(args)))
         return 0

@@ -295,7 +295,7 @@ int prepare(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b,
     for (size_t k = 0; k < nb; ++k) {
       const Bucket& bk = plan->buckets[k];
       const std::string tag = "#" + std::to_string(bk.nstr);
-      if ((rc = get_buf(ctx, ("eig" + tag).c_str(), (size_t)dsa::EigOff(bk.nstr).size() * bk.n_tasks * 8, &p))) return rc;
+      if ((rc = get_buf(ctx, ("eig" + tag).c_str(), (size_t)dsa::EigOff(bk.nstr).stride() * bk.n_tasks * 8, &p))) return rc;
       pa.eig[k] = (double*)p;
       if ((rc = get_buf(ctx, ("panel" + tag).c_str(), (size_t)dsa::panel_stride(bk.nstr) * bk.n_tasks * 8, &p))) return rc;
       pa.panel[k] = (double*)p;
@@ -600,7 +600,7 @@ int disort_flux_plan_create(disort_ctx* ctx, int64_t n, const int32_t* nstr, con
     if ((rc = upload(v_bk.data(), v_bk.size() * 4, (void**)&bk.d_v_bk))) return rc;
     if ((rc = upload(v_lu.data(), v_lu.size() * 4, (void**)&bk.d_v_lu))) return rc;
     plan->max_tasks = std::max(plan->max_tasks, bk.n_tasks);
-    plan->max_eig = std::max<int64_t>(plan->max_eig, (int64_t)dsa::EigOff(bk.nstr).size() * bk.n_tasks);
+    plan->max_eig = std::max<int64_t>(plan->max_eig, (int64_t)dsa::EigOff(bk.nstr).stride() * bk.n_tasks);
     plan->max_panel = std::max<int64_t>(plan->max_panel, (int64_t)dsa::panel_stride(bk.nstr) * bk.n_tasks);
     plan->max_cvec = std::max<int64_t>(plan->max_cvec, (int64_t)bk.nstr * bk.n_tasks);
     plan->buckets.push_back(bk);

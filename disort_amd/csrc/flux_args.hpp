@@ -38,7 +38,7 @@ struct FluxArgs {
   int64_t n_lvtasks;         // level tasks (sum of nlyr+1)
   const int32_t* v_bk;       // [n_lvtasks] bucket-local instance index
   const int32_t* v_lu;       // [n_lvtasks] level index
-  double* eig;               // [EigOff::size][n_tasks]
+  double* eig;               // [n_tasks][EigOff::stride]
   double* panel;             // [n_tasks][panel_stride]
   double* cvec;              // [n_tasks][N]
   const double* tab;         // TabOff layout for this N

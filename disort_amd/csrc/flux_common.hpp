@@ -94,6 +94,43 @@ __device__ __forceinline__ double row_bcast(double x) {
   return __hiloint2double(hi, lo);
 }
 
+// x of the lane given by DPP control CTRL (a pattern within rows of 16 lanes).
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// x of lane (this lane xor M), M < 32 (ds_swizzle bit mode: and 0x1f, xor M).
+template <int M>
+__device__ __forceinline__ double swz_xor(double x) {
+  static_assert(M > 0 && M < 32, "swizzle within 32 lanes");
+  const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(x), (M << 10) | 0x1F);
+  const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(x), (M << 10) | 0x1F);
+  return __hiloint2double(hi, lo);
+}
+
+// Sum of x over aligned groups of L lanes (L a power of two <= 32); every
+// lane of the group gets the same value (the butterfly adds commute).
+template <int L>
+__device__ __forceinline__ double group_sum(double x) {
+  if constexpr (L > 1) x += swz_xor<1>(x);
+  if constexpr (L > 2) x += swz_xor<2>(x);
+  if constexpr (L > 4) x += swz_xor<4>(x);
+  if constexpr (L > 8) x += swz_xor<8>(x);
+  if constexpr (L > 16) x += swz_xor<16>(x);
+  return x;
+}
+
+// LDS ordering inside one wavefront: LDS instructions of a wavefront execute
+// in order, so only the compiler must not move loads and stores across.
+__device__ __forceinline__ void wave_lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Compile-time loop helper: f(integral_constant<int, I>) for I in [B, E).
 template <int B, int E, typename F>
 __device__ __forceinline__ void static_for(F&& f) {
@@ -309,26 +346,42 @@ __device__ __forceinline__ double g_entry(const FluxArgs& a, int64_t t, int q, i
   const int r = lower ? q - NN : NN - 1 - q;
   // lower & up: gp; upper & up: gm; lower & down: -gm; upper & down: -gp
   const int f = (lower == up) ? eo.gp() : eo.gm();
-  const double v = a.eig[(f + r + jp * NN) * (int64_t)a.n_tasks + t];
+  const double v = a.eig[eo.at(t, f + r + jp * NN)];
   return up ? v : -v;
 }
 
 // ===========================================================================
 // k_flux: fluxes and mean intensities at the layer boundaries.
+//
+// A group of LPL lanes per level (LPL = NN rounded up to a power of two),
+// lane r owning the quadrature angle pair (+mu_r, -mu_r): it reads row r of
+// the layer's gp/gm blocks (consecutive lanes, consecutive addresses), and
+// the per-angle terms of the fluxes are summed over the group.
 // ===========================================================================
+template <int NN>
+struct FluxLanes {
+  static constexpr int LPL = NN <= 2 ? 2 : NN <= 4 ? 4 : NN <= 8 ? 8 : NN <= 16 ? 16 : 32;
+  static constexpr int LPW = 64 / LPL;  // levels per wavefront
+  static_assert(NN <= 32, "at most 32 angle pairs");
+};
+
 template <int NN>
 __global__ __launch_bounds__(64) void k_flux(FluxArgs a) {
   constexpr int N = 2 * NN;
   constexpr EigOff eo(N);
   constexpr TabOff to(N);
-  const int64_t vt = (int64_t)blockIdx.x * 64 + threadIdx.x;
-  if (vt >= a.n_lvtasks) return;
+  constexpr int LPL = FluxLanes<NN>::LPL, LPW = FluxLanes<NN>::LPW;
+  __shared__ double sY[LPW * 2 * LPL];
+  const int lv = threadIdx.x / LPL;
+  const int r = threadIdx.x % LPL;
+  const int64_t vt = (int64_t)blockIdx.x * LPW + lv;
+  if (vt >= a.n_lvtasks) return;  // whole groups leave together
   const int bk = a.v_bk[vt];
   const int lu = a.v_lu[vt];
   const int i = a.b_inst[bk];
   const int64_t t0 = a.b_task0[bk];
   const int64_t lo = a.lay_off[i], vo = lo + i;
-  const int64_t T = a.n_tasks, TL = a.total_layers, TV = a.total_levels, OV = a.total_levels;
+  const int64_t TL = a.total_layers, TV = a.total_levels, OV = a.total_levels;
   const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
   const double* tb = a.tab;
   const int ncut = a.inst_i[i * kInstI + II_NCUT];
@@ -340,42 +393,45 @@ __global__ __launch_bounds__(64) void k_flux(FluxArgs a) {
   if (!(lyrcut && lyu > ncut - 1)) {
     if (lyu >= ncut) lyu = ncut - 1;
     const int64_t tt = t0 + lyu;
+    const double* rec = a.eig + eo.at(tt, 0);
     const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
     const double ut = a.lev_d[VD_TAU_S * TV + vo + lu];
     const double dts = a.lay_d[LD_DTAU_S * TL + lo + lyu];
-    // Mode amplitudes at this level: C_j e_j.
-    // Intensity at the quadrature angles: I = G (C o e) + Z, accumulated one
-    // mode pair at a time (mode NN-1-jp has eigenvalue -k_jp, mode NN+jp +k_jp).
-    const int zoff = (lu == 0) ? eo.ztop() : eo.zfl();
-    double iu[NN], id[NN];
-#pragma unroll
-    for (int r = 0; r < NN; ++r) {
-      iu[r] = a.eig[(zoff + NN + r) * T + tt];
-      id[r] = a.eig[(zoff + NN - 1 - r) * T + tt];
-    }
-#pragma unroll 1
-    for (int jp = 0; jp < NN; ++jp) {
-      const double e = exp(-a.eig[(eo.k() + jp) * T + tt] * dts);
-      const double yd = a.cvec[tt * N + NN - 1 - jp] * ((lu == 0) ? e : 1.0);
-      const double yu = a.cvec[tt * N + NN + jp] * ((lu == 0) ? 1.0 : e);
-#pragma unroll
-      for (int r = 0; r < NN; ++r) {
-        const double gp = a.eig[(eo.gp() + r + jp * NN) * T + tt];
-        const double gm = a.eig[(eo.gm() + r + jp * NN) * T + tt];
-        iu[r] += gp * yu - gm * yd;
-        id[r] += gm * yu - gp * yd;
+    const bool act = r < NN;
+    const int rr = act ? r : 0;
+    // Mode amplitudes at this level, C_j e_j, one mode pair per lane (mode
+    // NN-1-jp has eigenvalue -k_jp, mode NN+jp +k_jp): y[jp] downward,
+    // y[LPL+jp] upward.
+    double* y = sY + lv * 2 * LPL;
+    {
+      const double e = exp(-rec[eo.k() + rr] * dts);
+      const double yd = a.cvec[tt * N + NN - 1 - rr] * ((lu == 0) ? e : 1.0);
+      const double yu = a.cvec[tt * N + NN + rr] * ((lu == 0) ? 1.0 : e);
+      wave_lds_order();
+      if (act) {
+        y[r] = yd;
+        y[LPL + r] = yu;
       }
+      wave_lds_order();
     }
-    double mi = 0.0, mid = 0.0, miu = 0.0, fldn = 0.0, flup = 0.0;
-#pragma unroll
-    for (int r = 0; r < NN; ++r) {
-      const double w = tb[to.cwt() + r], cm = tb[to.cmu() + r];
-      mi += w * id[r] + w * iu[r];
-      mid += w * id[r];
-      miu += w * iu[r];
-      fldn += w * id[r] * cm;
-      flup += w * iu[r] * cm;
+    // Intensity at the quadrature angles: I = G (C o e) + Z.
+    const int zoff = (lu == 0) ? eo.ztop() : eo.zfl();
+    double iu = rec[zoff + NN + rr], id = rec[zoff + NN - 1 - rr];
+    constexpr int kUnroll = NN <= 16 ? NN : 8;
+#pragma unroll kUnroll
+    for (int jp = 0; jp < NN; ++jp) {
+      const double yd = y[jp], yu = y[LPL + jp];
+      const double gp = rec[eo.gp() + rr + jp * NN];
+      const double gm = rec[eo.gm() + rr + jp * NN];
+      iu += gp * yu - gm * yd;
+      id += gm * yu - gp * yd;
     }
+    const double w = act ? tb[to.cwt() + rr] : 0.0, cm = tb[to.cmu() + rr];
+    double mi = group_sum<LPL>(w * id + w * iu);
+    double mid = group_sum<LPL>(w * id);
+    double miu = group_sum<LPL>(w * iu);
+    double fldn = group_sum<LPL>(w * id * cm);
+    double flup = group_sum<LPL>(w * iu * cm);
     double fact = 0.0, dirint = 0.0, fldir = 0.0, fdir = 0.0;
     if (F0 > 0.0) {
       // Path factor mu0, or the layer's Chapman factor (:1184-1194).
@@ -395,7 +451,7 @@ __global__ __launch_bounds__(64) void k_flux(FluxArgs a) {
     miu = (2.0 * kPi * miu) / (4.0 * kPi);
     double pls = 0.0;
     if (a.flags & DISORT_F_THERMAL)
-      pls = (lu == 0) ? a.lev_d[VD_PLANCK * TV + vo] : a.eig[eo.pls() * T + tt];
+      pls = (lu == 0) ? a.lev_d[VD_PLANCK * TV + vo] : rec[eo.pls()];
     const double div = (1.0 - a.ssa[lo + lyu]) * 4.0 * kPi * (mi - pls);
     o[DISORT_O_FLUX_DIRECT_BEAM] = fdir;
     o[DISORT_O_FLUX_DOWN] = fdn;
@@ -406,8 +462,17 @@ __global__ __launch_bounds__(64) void k_flux(FluxArgs a) {
     o[DISORT_O_MEAN_INTENSITY_UP] = miu;
     o[DISORT_O_MEAN_INTENSITY_DIRECT_BEAM] = midir;
   }
+  if (r == 0) {
 #pragma unroll
-  for (int f = 0; f < DISORT_NOUT; ++f) a.out[f * OV + vo + lu] = o[f];
+    for (int f = 0; f < DISORT_NOUT; ++f) a.out[f * OV + vo + lu] = o[f];
+  }
+}
+
+// Launch of k_flux over the level tasks of a bucket.
+template <int NN>
+inline void launch_flux_levels(const FluxArgs& a, hipStream_t s) {
+  constexpr int LPW = FluxLanes<NN>::LPW;
+  hipLaunchKernelGGL(k_flux<NN>, dim3((unsigned)((a.n_lvtasks + LPW - 1) / LPW)), dim3(64), 0, s, a);
 }
 
 }  // namespace

@@ -14,13 +14,15 @@
 //             particular solutions (:595-643, :745-831) expressed through
 //             that eigen-decomposition and a Cholesky factor of -A_s instead
 //             of N x N LU solves.
-//   k_boundary 16 lanes per instance: the (N*L)-unknown boundary system of
+//   k_boundary one lane per panel row (16 or 32 lanes per instance): the
+//             (N*L)-unknown boundary system of
 //             setMatrix/solve0 (:838-1131) eliminated layer by layer by LU
 //             with partial pivoting on 3NN-row panels (the NN equations
 //             carried from above plus the N continuity equations of the next
 //             interface), then block back substitution.
-//   k_flux    (flux_common.hpp) one lane per (instance, level): intensities
-//             at the quadrature angles and the eight outputs (:1138-1287).
+//   k_flux    (flux_common.hpp) a lane group per (instance, level), one lane
+//             per quadrature angle pair: intensities at the quadrature
+//             angles and the eight outputs (:1138-1287).
 #include "flux_common.hpp"
 
 namespace dsa {
@@ -159,7 +161,7 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
   const int P = a.pstride[i];
   const double* pm = a.pmom + a.mom_off[i] + (int64_t)lc * P;
   const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
-  const int64_t TL = a.total_layers, TV = a.total_levels, T = a.n_tasks;
+  const int64_t TL = a.total_layers, TV = a.total_levels;
   const double* tb = a.tab;
   constexpr TabOff to(N);
   constexpr EigOff eo(N);
@@ -256,10 +258,10 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
   // while the Jacobi sweep holds S and V in registers.
   static_assert(NT <= 3 * N, "L must fit in the ztop/zbot/zfl slots");
 #pragma unroll
-  for (int q = 0; q < NT; ++q) a.eig[(eo.ztop() + q) * T + t] = m.L[q];
+  for (int q = 0; q < NT; ++q) a.eig[eo.at(t, eo.ztop() + q)] = m.L[q];
   jacobi_eigen<NN>(S, m.V, m.lam);
 #pragma unroll
-  for (int q = 0; q < NT; ++q) m.L[q] = a.eig[(eo.ztop() + q) * T + t];
+  for (int q = 0; q < NT; ++q) m.L[q] = a.eig[eo.at(t, eo.ztop() + q)];
 
   // Eigenvalues k = sqrt|lambda| and the G+/- blocks (:560-587):
   // u = S_d^-1 L^-T w is the eigenvector of (alpha+beta)(alpha-beta);
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
 #pragma unroll
   for (int j = 0; j < NN; ++j) {
     kv[j] = sqrt(fabs(m.lam[j]));
-    a.eig[(eo.k() + j) * T + t] = kv[j];
+    a.eig[eo.at(t, eo.k() + j)] = kv[j];
     double u[NN], y[NN];
 #pragma unroll
     for (int r = 0; r < NN; ++r) u[r] = y[r] = m.V[r][j];
@@ -279,8 +281,8 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
       const double sdi = tb[to.sdi() + r];
       const double uu = sdi * u[r];
       const double tt = -sdi * y[r] / kv[j];
-      a.eig[(eo.gp() + r + j * NN) * T + t] = 0.5 * (tt + uu);
-      a.eig[(eo.gm() + r + j * NN) * T + t] = 0.5 * (tt - uu);
+      a.eig[eo.at(t, eo.gp() + r + j * NN)] = 0.5 * (tt + uu);
+      a.eig[eo.at(t, eo.gm() + r + j * NN)] = 0.5 * (tt - uu);
     }
   }
 
@@ -458,11 +460,11 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
   const double fact = beam ? exp(-taub / mu0) : 0.0;
 #pragma unroll
   for (int r = 0; r < N; ++r) {
-    a.eig[(eo.ztop() + r) * T + t] = zb[r] * btt + z0[r] + z1[r] * taut;
-    a.eig[(eo.zbot() + r) * T + t] = zb[r] * btb + z0[r] + z1[r] * taub;
-    a.eig[(eo.zfl() + r) * T + t] = zb[r] * fact + z0[r] + z1[r] * taub;
+    a.eig[eo.at(t, eo.ztop() + r)] = zb[r] * btt + z0[r] + z1[r] * taut;
+    a.eig[eo.at(t, eo.zbot() + r)] = zb[r] * btb + z0[r] + z1[r] * taub;
+    a.eig[eo.at(t, eo.zfl() + r)] = zb[r] * fact + z0[r] + z1[r] * taub;
   }
-  a.eig[eo.pls() * T + t] = xr0 + xr1 * taub;
+  a.eig[eo.at(t, eo.pls())] = xr0 + xr1 * taub;
 }
 
 // ===========================================================================
@@ -478,96 +480,88 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
 // leaves N pivot rows (the U rows of layer i) and NN rows in C_{i+1} that
 // are carried to the next panel.
 //
-// 16 lanes per instance, 4 instances per 64-lane workgroup.  Lane j owns
-// column j of the left block (C_i), column j of the right block (C_{i+1})
-// and a copy of the right-hand side.  Rows live in 3NN fixed slots: pivot
-// rows are marked used instead of being swapped, their values are picked by
-// a select chain, and the next panel refills the used slots with the new
-// interface rows (row = rank of the slot among the used ones), loaded from
-// memory at computed addresses -- so no register is ever indexed dynamically.
+// One lane per panel row ("slot"): a group of LPI lanes (whole 16-lane DPP
+// rows) per instance, 64/LPI instances per wavefront, one wavefront per
+// workgroup.  A lane holds its row's left block (N coefficients of C_i),
+// right block (N coefficients of C_{i+1}) and right-hand side.  Step c of a
+// panel: a group max of |a_rc| over the candidate rows (DPP rotations),
+// the first lane attaining it (ballot) is the pivot; it publishes its row in
+// LDS and every candidate row subtracts its multiple.  Pivot rows stay in
+// their lanes, are marked used, are written out at the end of the panel as
+// the U rows of the layer, and the next panel refills the used slots with
+// the new interface rows (row q = rank of the slot among the used ones).
+// Slots are ordered as rows of the band, so "first largest" picks the row
+// LINPACK picks.  The layer-task records are contiguous, so a row fill reads
+// whole cache lines.
 // ===========================================================================
 template <int NN>
-struct Panel {
+struct RowCfg {
   static constexpr int N = 2 * NN;
   static constexpr int R = 3 * NN;
-  double A0[R], A1[R], A2[R];
+  static constexpr int LPI = (R <= 16) ? 16 : 32;  // lanes per instance
+  static constexpr int G = 64 / LPI;               // instances per wavefront
+  static constexpr int W = 2 * N + 2;              // pivot-row record in LDS
+  static_assert(R <= 32, "one slot per lane of a 32-lane group");
 };
 
-template <int SRC>
-__device__ __forceinline__ int row_bcast_i(int x) {
-  return __builtin_amdgcn_mov_dpp(x, 0x150 + SRC, 0xF, 0xF, false);
+// Maximum of x over the LPI lanes of the caller's group (every lane gets it).
+// Rotations by 8, 4, 2, 1 within a 16-lane row cover the row; a swizzle with
+// lane xor 16 joins the two rows of a 32-lane group.  fmax ignores NaN.
+template <int LPI>
+__device__ __forceinline__ double group_max(double x) {
+  x = fmax(x, dpp_f64<0x128>(x));  // row_ror:8
+  x = fmax(x, dpp_f64<0x124>(x));  // row_ror:4
+  x = fmax(x, dpp_f64<0x122>(x));  // row_ror:2
+  x = fmax(x, dpp_f64<0x121>(x));  // row_ror:1
+  if constexpr (LPI == 32) x = fmax(x, swz_xor<16>(x));
+  return x;
 }
 
-
-// One panel: N steps of partial-pivot elimination over the candidate slots
-// (those not in `used`).  Pivot rows are written to pp as the U rows of this
-// layer in pivot order: [c][j] left block, N*N + [c][j] right block, 2N*N + c
-// right-hand side.  Returns the mask of pivot slots.
+// Row q of the reference's N x N eigenvector matrix G of layer-task record
+// rec (see EigOff): out[j] = G(q, j).
 template <int NN>
-__device__ __forceinline__ unsigned lu_panel(Panel<NN>& pn, unsigned used, int lane, bool live, double* pp) {
-  constexpr int N = 2 * NN;
-  constexpr int R = 3 * NN;
-  static_for<0, N>([&](auto cc) {
-    constexpr int c = decltype(cc)::value;
-    // Pivot search in column c (meaningful in lane c): first largest |a|.
-    int p = -1;
-    double best = -1.0;
+__device__ __forceinline__ void g_row(const double* rec, int q, double* out) {
+  constexpr EigOff eo(2 * NN);
+  const bool lower = q >= NN;
+  const int r = lower ? q - NN : NN - 1 - q;
+  const double* up = rec + (lower ? eo.gp() : eo.gm()) + r;  // columns NN + j'
+  const double* dn = rec + (lower ? eo.gm() : eo.gp()) + r;  // columns NN - 1 - j'
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const bool cand = !((used >> r) & 1u);
-      const double mag = fabs(pn.A0[r]);
-      const bool take = cand && (mag > best);
-      best = take ? mag : best;
-      p = take ? r : p;
-    }
-    p = row_bcast_i<c>(p);
-    // Pivot-row entries of this lane's columns (one-hot sum over the slots).
-    double v0 = 0.0, v1 = 0.0, v2 = 0.0;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const double hit = (r == p) ? 1.0 : 0.0;
-      v0 = fma(hit, pn.A0[r], v0);
-      v1 = fma(hit, pn.A1[r], v1);
-      v2 = fma(hit, pn.A2[r], v2);
-    }
-    // Multipliers of column c (lane c), broadcast to the row.  Every slot is
-    // eliminated: the pivot rows of this panel are already stored and are
-    // refilled by the next panel, so changing them is harmless.
-    const double inv = (v0 != 0.0) ? 1.0 / (v0 != 0.0 ? v0 : 1.0) : 0.0;
-    used |= 1u << p;
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const double l = row_bcast<c>(pn.A0[r] * inv);
-      pn.A0[r] -= l * v0;
-      pn.A1[r] -= l * v1;
-      pn.A2[r] -= l * v2;
-    }
-    if (live) {
-      pp[c * N + lane] = (lane == c) ? inv : v0;  // diagonal stored as its reciprocal
-      pp[N * N + c * N + lane] = v1;
-    }
-    if (lane == 0) pp[2 * N * N + c] = v2;
-  });
-  return used;
+  for (int jp = 0; jp < NN; ++jp) {
+    out[NN + jp] = up[jp * NN];
+    out[NN - 1 - jp] = -dn[jp * NN];
+  }
+}
+
+// Bits of the caller's group in a wavefront ballot.
+template <int LPI>
+__device__ __forceinline__ unsigned group_bits(unsigned long long bal, int g) {
+  const unsigned long long m = (LPI == 32) ? 0xffffffffull : 0xffffull;
+  return (unsigned)((bal >> (g * LPI)) & m);
 }
 
 template <int NN>
-__global__ __launch_bounds__(64, 2) void k_boundary(FluxArgs a) {
-  constexpr int N = 2 * NN;
-  constexpr int R = 3 * NN;
+__global__ __launch_bounds__(64, 3) void k_boundary(FluxArgs a) {
+  using C = RowCfg<NN>;
+  constexpr int N = C::N, R = C::R, LPI = C::LPI, G = C::G, W = C::W;
   constexpr EigOff eo(N);
   constexpr TabOff to(N);
   constexpr int PS = panel_stride(N);
-  static_assert(N <= 16, "one 16-lane row per instance");
-  static_assert(R <= 32, "slot mask is 32 bits");
-  const int grp = threadIdx.x >> 4;
-  const int lane = threadIdx.x & 15;
-  const int bk = blockIdx.x * 4 + grp;
-  if (bk >= a.nb) return;
+  constexpr int ES = eo.stride();
+  __shared__ double sRow[G * W];     // pivot row of the current step, per group
+  __shared__ double sE[G * 2 * NN];  // e^{-k dtau} of the two layers of a panel
+  __shared__ double sRS[G * N];      // bottom reflection row
+
+  const int g = threadIdx.x / LPI;
+  const int s = threadIdx.x % LPI;  // slot
+  const bool slot = s < R;
+  const int bk0 = blockIdx.x * G + g;
+  const bool live = bk0 < a.nb;
+  const int bk = live ? bk0 : a.nb - 1;  // a spare group repeats the last instance, stores nothing
   const int i = a.b_inst[bk];
   const int64_t t0 = a.b_task0[bk];
   const int64_t lo = a.lay_off[i];
-  const int64_t T = a.n_tasks, TL = a.total_layers;
+  const int64_t TL = a.total_layers;
   const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
   const double* tb = a.tab;
   const int M = a.inst_i[i * kInstI + II_NCUT];
@@ -576,56 +570,108 @@ __global__ __launch_bounds__(64, 2) void k_boundary(FluxArgs a) {
   const bool thermal = (a.flags & DISORT_F_THERMAL) != 0;
   const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
   const double alb = sc[DISORT_S_ALBEDO];
-  const int j = (lane < N) ? lane : 0;  // idle lanes mirror column 0, results discarded
-  const bool live = lane < N;
+  double* const sr = sRow + g * W;
+  double* const se = sE + g * 2 * NN;
 
-  // Mode column j over layer task tt: value at the layer top / bottom.
-  auto efac = [&](int64_t tt, double& etop, double& ebot) {
-    const int jp = (j >= NN) ? j - NN : NN - 1 - j;
-    const double k = a.eig[(eo.k() + jp) * T + tt];
-    const double e = exp(-k * a.lay_d[LD_DTAU_S * TL + lo + (tt - t0)]);
-    etop = (j < NN) ? e : 1.0;
-    ebot = (j < NN) ? 1.0 : e;
+  // e^{-k_j' dtau} of layer ta in se[j'] and of layer tb in se[NN + j'].
+  auto exps = [&](int64_t ta, int64_t tb2) {
+    wave_lds_order();
+    if (s < 2 * NN) {
+      const int64_t t = (s < NN) ? ta : tb2;
+      const int jp = (s < NN) ? s : s - NN;
+      se[s] = exp(-a.eig[eo.at(t, eo.k() + jp)] * a.lay_d[LD_DTAU_S * TL + lo + (t - t0)]);
+    }
+    wave_lds_order();
   };
 
-  Panel<NN> pn;
-  // Top boundary in slots 0..NN-1: downward intensities at tau=0
-  // (:890-907, :976-990).  They are stored as if carried from a previous
-  // panel (right block A1), which the first panel moves into the left block;
-  // slots NN.. are free (marked used) for the first interface.
-  {
-    double etop, ebot;
-    efac(t0, etop, ebot);
-    const double top = sc[DISORT_S_ISO_TOP] + a.inst_d[i * kInstD + ID_PL_TOP];
+  double A0[N], A1[N], rhs = 0.0;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const bool tbc = r < NN;
-      pn.A0[r] = 0.0;
-      pn.A1[r] = (tbc && live) ? g_entry<NN>(a, t0, tbc ? NN - 1 - r : 0, j) * etop : 0.0;
-      pn.A2[r] = tbc ? top - a.eig[(eo.ztop() + (tbc ? NN - 1 - r : 0)) * T + t0] : 0.0;
-    }
+  for (int j = 0; j < N; ++j) A0[j] = A1[j] = 0.0;
+
+  // Top boundary in slots 0..NN-1: downward intensities at tau=0
+  // (:890-907, :976-990), stored as if carried from a previous panel (right
+  // block), which the first panel moves into the left block.  Slots NN..R-1
+  // are free (used) for the first interface.
+  exps(t0, t0);
+  if (s < NN) {
+    g_row<NN>(a.eig + t0 * ES, NN - 1 - s, A1);
+#pragma unroll
+    for (int j = 0; j < NN; ++j) A1[j] *= se[NN - 1 - j];  // e at the layer top for the decaying modes
+    rhs = sc[DISORT_S_ISO_TOP] + a.inst_d[i * kInstD + ID_PL_TOP] - a.eig[eo.at(t0, eo.ztop() + NN - 1 - s)];
   }
-  unsigned used = ((1u << R) - 1u) & ~((1u << NN) - 1u);
+  bool used = slot && s >= NN;
+
+  // N elimination steps over the candidate rows; pivot rows are written to
+  // pp as the U rows of the layer in pivot order: [c][j] left block
+  // (diagonal as its reciprocal), N*N + [c][j] right block, 2N*N + c rhs.
+  auto lu_panel = [&](bool excluded, double* pp) {
+    int myc = -1;
+    double myinv = 0.0;
+    static_for<0, N>([&](auto cc) {
+      constexpr int c = decltype(cc)::value;
+      const bool cand = slot && !used && !excluded;
+      const double key = cand ? fabs(A0[c]) : -1.0;
+      const double m = group_max<LPI>(key);
+      const int p = __ffs(group_bits<LPI>(__ballot(key == m), g)) - 1;
+      wave_lds_order();
+      if (s == p) {
+#pragma unroll
+        for (int k = c; k < N; ++k) sr[k] = A0[k];
+#pragma unroll
+        for (int k = 0; k < N; ++k) sr[N + k] = A1[k];
+        sr[2 * N] = rhs;
+      }
+      wave_lds_order();
+      const double piv = sr[c];
+      const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
+      if (s == p) {
+        used = true;
+        myc = c;
+        myinv = inv;
+      } else if (cand) {
+        const double l = A0[c] * inv;
+#pragma unroll
+        for (int k = c + 1; k < N; ++k) A0[k] -= l * sr[k];
+#pragma unroll
+        for (int k = 0; k < N; ++k) A1[k] -= l * sr[N + k];
+        rhs -= l * sr[2 * N];
+      }
+    });
+    if (live && myc >= 0) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        pp[myc * N + j] = (j == myc) ? myinv : A0[j];
+        pp[N * N + myc * N + j] = A1[j];
+      }
+      pp[2 * N * N + myc] = rhs;
+    }
+  };
 
   // Interior panels: continuity at the interface below layer ip (:851-887, :1094-1118).
   for (int ip = 0; ip < M - 1; ++ip) {
     const int64_t tt = t0 + ip;
-    double etop, ebot, etn, ebn;
-    efac(tt, etop, ebot);
-    efac(tt + 1, etn, ebn);
+    exps(tt, tt + 1);
+    const unsigned ub = group_bits<LPI>(__ballot(used), g);
+    if (used) {
+      const int q = __popc(ub & ((1u << s) - 1u));  // interface row of this slot
+      g_row<NN>(a.eig + tt * ES, q, A0);
+      g_row<NN>(a.eig + (tt + 1) * ES, q, A1);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const bool fill = (used >> r) & 1u;
-      const int q = fill ? __popc(used & ((1u << r) - 1u)) : 0;  // interface row of this slot
-      const double gi = live ? g_entry<NN>(a, tt, q, j) : 0.0;
-      const double gn = live ? g_entry<NN>(a, tt + 1, q, j) : 0.0;
-      const double rhs = a.eig[(eo.ztop() + q) * T + tt + 1] - a.eig[(eo.zbot() + q) * T + tt];
-      const double carried = pn.A1[r];
-      pn.A0[r] = fill ? gi * ebot : carried;
-      pn.A1[r] = fill ? -gn * etn : 0.0;
-      pn.A2[r] = fill ? rhs : pn.A2[r];
+      for (int j = 0; j < NN; ++j) {
+        A0[NN + j] *= se[j];              // bottom of layer tt: growing modes decay
+        A1[j] = -A1[j] * se[2 * NN - 1 - j];  // top of layer tt+1
+        A1[NN + j] = -A1[NN + j];
+      }
+      rhs = a.eig[eo.at(tt + 1, eo.ztop() + q)] - a.eig[eo.at(tt, eo.zbot() + q)];
+      used = false;
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        A0[j] = A1[j];
+        A1[j] = 0.0;
+      }
     }
-    used = lu_panel<NN>(pn, 0u, lane, live, a.panel + tt * PS);
+    lu_panel(false, a.panel + tt * PS);
   }
 
   // Last panel: carried equations plus the bottom boundary, Lambertian
@@ -633,17 +679,23 @@ __global__ __launch_bounds__(64, 2) void k_boundary(FluxArgs a) {
   // free slots take bottom rows; the others stay out of the pivot search.
   {
     const int64_t tt = t0 + M - 1;
-    double etop, ebot;
-    efac(tt, etop, ebot);
+    const double* rec = a.eig + tt * ES;
+    exps(tt, tt);
     const bool refl = !(lyrcut || diffusion);
-    double rs = 0.0, zrefl = 0.0;
+    // Reflection row rs_j = sum_k w_k G(NN-1-k, j), one column per lane.
+    if (s < N) {
+      double rs = 0.0;
 #pragma unroll
-    for (int k = 0; k < NN; ++k) {
-      const double wk = tb[to.cwt() + k] * tb[to.cmu() + k] * alb;
-      rs += wk * (live ? g_entry<NN>(a, tt, NN - 1 - k, j) : 0.0);
-      zrefl += wk * a.eig[(eo.zbot() + NN - 1 - k) * T + tt];
+      for (int k = 0; k < NN; ++k) {
+        const double wk = tb[to.cwt() + k] * tb[to.cmu() + k] * alb;
+        rs += wk * g_entry<NN>(a, tt, NN - 1 - k, s);
+      }
+      sRS[g * N + s] = refl ? rs : 0.0;
     }
-    rs = refl ? rs : 0.0;
+    wave_lds_order();
+    double zrefl = 0.0;
+#pragma unroll
+    for (int k = 0; k < NN; ++k) zrefl += tb[to.cwt() + k] * tb[to.cmu() + k] * alb * rec[eo.zbot() + NN - 1 - k];
     const double btn = a.inst_d[i * kInstD + ID_BTRANS_NCUT];
     const double plb = a.inst_d[i * kInstD + ID_PL_BOT];
     const double plbd = a.inst_d[i * kInstD + ID_PL_BOT_DERIV];
@@ -651,27 +703,33 @@ __global__ __launch_bounds__(64, 2) void k_boundary(FluxArgs a) {
     if (refl) extra += 2.0 * zrefl;
     if (F0 > 0.0 && !lyrcut) extra += alb * mu0 * F0 * btn / kPi;
     if (thermal) extra += (1.0 - alb) * plb;
-    unsigned excluded = 0u;
+    const unsigned ub = group_bits<LPI>(__ballot(used), g);
+    const int q = __popc(ub & ((1u << s) - 1u));
+    const bool bot = used && q < NN;
+    const bool excluded = used && !bot;
+    if (bot) {
+      g_row<NN>(rec, NN + q, A0);
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const bool fill = (used >> r) & 1u;
-      const int q = fill ? __popc(used & ((1u << r) - 1u)) : 0;
-      const bool bot = fill && q < NN;
-      const int t = bot ? q : 0;
-      const double gb = live ? g_entry<NN>(a, tt, NN + t, j) : 0.0;
-      const double zb = -a.eig[(eo.zbot() + NN + t) * T + tt];
-      const double carried = pn.A1[r];
-      pn.A0[r] = bot ? (gb - 2.0 * rs) * ebot : (fill ? 0.0 : carried);
-      pn.A1[r] = 0.0;
-      pn.A2[r] = bot ? zb + (diffusion ? plb + tb[to.cmu() + t] * plbd : extra) : (fill ? 0.0 : pn.A2[r]);
-      excluded |= (fill && !bot) ? (1u << r) : 0u;
+      for (int j = 0; j < N; ++j) {
+        A0[j] = (A0[j] - 2.0 * sRS[g * N + j]) * ((j < NN) ? 1.0 : se[j - NN]);
+        A1[j] = 0.0;
+      }
+      rhs = -rec[eo.zbot() + NN + q] + (diffusion ? plb + tb[to.cmu() + q] * plbd : extra);
+      used = false;
+    } else if (!used) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        A0[j] = A1[j];
+        A1[j] = 0.0;
+      }
     }
-    lu_panel<NN>(pn, excluded, lane, live, a.panel + tt * PS);
+    lu_panel(excluded, a.panel + tt * PS);
   }
 
-  // Back substitution over the layers, lane = U row k of each panel:
+  // Back substitution over the layers, lane = U row k of each panel (the
+  // first 16-lane row of the group):
   // x_k = (r_k - sum_{c>k} U(k,c) x_c - sum_c V(k,c) xnext_c) / U(k,k).
-  const int k = j;
+  const int k = (s < N) ? s : 0;
   double xn = 0.0;  // this lane's entry of the next layer's constants
   for (int ip = M - 1; ip >= 0; --ip) {
     const int64_t tt = t0 + ip;
@@ -692,7 +750,7 @@ __global__ __launch_bounds__(64, 2) void k_boundary(FluxArgs a) {
       v -= pp[k * N + c] * xc;
     });
     xn = xk;
-    if (live) a.cvec[tt * N + k] = xk;
+    if (live && s < N) a.cvec[tt * N + k] = xk;
   }
 }
 
@@ -707,9 +765,9 @@ static void launch_all(const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
   if (ev) (void)hipEventRecord(ev[1], s);
   hipLaunchKernelGGL(k_layer<NN>, dim3((unsigned)((a.n_tasks + 63) / 64)), dim3(64), 0, s, a);
   if (ev) (void)hipEventRecord(ev[2], s);
-  hipLaunchKernelGGL(k_boundary<NN>, dim3((unsigned)((a.nb + 3) / 4)), dim3(64), 0, s, a);
+  hipLaunchKernelGGL(k_boundary<NN>, dim3((unsigned)((a.nb + RowCfg<NN>::G - 1) / RowCfg<NN>::G)), dim3(64), 0, s, a);
   if (ev) (void)hipEventRecord(ev[3], s);
-  hipLaunchKernelGGL(k_flux<NN>, dim3((unsigned)((a.n_lvtasks + 63) / 64)), dim3(64), 0, s, a);
+  launch_flux_levels<NN>(a, s);
   if (ev) (void)hipEventRecord(ev[4], s);
 }
 

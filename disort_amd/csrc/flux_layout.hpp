@@ -55,9 +55,10 @@ enum LevD {
   kLevD = 3
 };
 
-// Per-layer output of the eigen kernel (doubles), SoA over bucket tasks:
-// field*n_tasks + task.  gp/gm are column-major NN x NN: entry (a, j') at
-// a + j'*NN.  The N x N eigenvector matrix of the reference
+// Per-layer output of the eigen kernel (doubles): one contiguous record per
+// layer task, task*stride() + field, so that the boundary solve reads a
+// layer's eigen-system as whole cache lines.  gp/gm are column-major NN x NN:
+// entry (a, j') at a + j'*NN.  The N x N eigenvector matrix of the reference
 // (src/FluxSolver.cpp:574-586) is
 //   G(NN+a, NN+j') =  gp(a,j')   G(NN-1-a, NN+j')     =  gm(a,j')
 //   G(NN+a, NN-1-j') = -gm(a,j') G(NN-1-a, NN-1-j')   = -gp(a,j')
@@ -73,6 +74,9 @@ struct EigOff {
   __host__ __device__ constexpr int zfl() const { return zbot() + n; }            // [N] at bottom (flux form)
   __host__ __device__ constexpr int pls() const { return zfl() + n; }             // Planck source at bottom
   __host__ __device__ constexpr int size() const { return pls() + 1; }
+  // Record pitch, even so that records start on 16-byte boundaries.
+  __host__ __device__ constexpr int stride() const { return (size() + 1) & ~1; }
+  __host__ __device__ constexpr int64_t at(int64_t task, int field) const { return task * stride() + field; }
 };
 
 // Boundary-solve factors per panel (doubles), AoS per task:

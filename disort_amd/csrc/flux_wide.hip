@@ -58,7 +58,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
   const int P = a.pstride[i];
   const double* pm = a.pmom + a.mom_off[i] + (int64_t)lc * P;
   const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
-  const int64_t TL = a.total_layers, TV = a.total_levels, T = a.n_tasks;
+  const int64_t TL = a.total_layers, TV = a.total_levels;
   const double* tb = a.tab;
   const double ws = a.lay_d[LD_SSA_S * TL + g];
   const double f = a.lay_d[LD_F * TL + g];
@@ -221,7 +221,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
   }
   if (tid < NN) {
     lam[tid] = sS[tid * LD + tid];
-    a.eig[(eo.k() + tid) * T + t] = sqrt(fabs(lam[tid]));
+    a.eig[eo.at(t, eo.k() + tid)] = sqrt(fabs(lam[tid]));
   }
   __syncthreads();
 
@@ -243,8 +243,8 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
     const double sdi = tb[to.sdi() + r];
     const double uu = sdi * sS[r * LD + j];
     const double tt = -sdi * y / sqrt(fabs(lam[j]));
-    a.eig[(eo.gp() + r + j * NN) * T + t] = 0.5 * (tt + uu);
-    a.eig[(eo.gm() + r + j * NN) * T + t] = 0.5 * (tt - uu);
+    a.eig[eo.at(t, eo.gp() + r + j * NN)] = 0.5 * (tt + uu);
+    a.eig[eo.at(t, eo.gm() + r + j * NN)] = 0.5 * (tt - uu);
   }
 
   // Particular solutions, reference angle order: NN+r is +mu_r, NN-1-r is -mu_r.
@@ -456,14 +456,14 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
       bfu = zbu * fact;
       bfd = zbd * fact;
     }
-    a.eig[(eo.ztop() + ru) * T + t] = btu + z0u + z1h * taut;
-    a.eig[(eo.ztop() + rd) * T + t] = btd + z0d + z1h * taut;
-    a.eig[(eo.zbot() + ru) * T + t] = bbu + z0u + z1h * taub;
-    a.eig[(eo.zbot() + rd) * T + t] = bbd + z0d + z1h * taub;
-    a.eig[(eo.zfl() + ru) * T + t] = bfu + z0u + z1h * taub;
-    a.eig[(eo.zfl() + rd) * T + t] = bfd + z0d + z1h * taub;
+    a.eig[eo.at(t, eo.ztop() + ru)] = btu + z0u + z1h * taut;
+    a.eig[eo.at(t, eo.ztop() + rd)] = btd + z0d + z1h * taut;
+    a.eig[eo.at(t, eo.zbot() + ru)] = bbu + z0u + z1h * taub;
+    a.eig[eo.at(t, eo.zbot() + rd)] = bbd + z0d + z1h * taub;
+    a.eig[eo.at(t, eo.zfl() + ru)] = bfu + z0u + z1h * taub;
+    a.eig[eo.at(t, eo.zfl() + rd)] = bfd + z0d + z1h * taub;
   }
-  if (tid == 0) a.eig[eo.pls() * T + t] = xr0 + xr1 * taub;
+  if (tid == 0) a.eig[eo.at(t, eo.pls())] = xr0 + xr1 * taub;
 }
 
 // ===========================================================================
@@ -495,7 +495,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
   const int i = a.b_inst[bk];
   const int64_t t0 = a.b_task0[bk];
   const int64_t lo = a.lay_off[i];
-  const int64_t T = a.n_tasks, TL = a.total_layers;
+  const int64_t TL = a.total_layers;
   const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
   const double* tb = a.tab;
   const int M = a.inst_i[i * kInstI + II_NCUT];
@@ -508,7 +508,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
   // exp(-k_jp dtau') of layer task tt into e[jp].
   auto layer_exp = [&](int64_t tt, double* e) {
     for (int jp = tid; jp < NN; jp += blockDim.x)
-      e[jp] = exp(-a.eig[(eo.k() + jp) * T + tt] * a.lay_d[LD_DTAU_S * TL + lo + (tt - t0)]);
+      e[jp] = exp(-a.eig[eo.at(tt, eo.k() + jp)] * a.lay_d[LD_DTAU_S * TL + lo + (tt - t0)]);
   };
   // Mode column j: factor at the layer top / bottom.
   auto etop_of = [](const double* e, int j) { return j < NN ? e[NN - 1 - j] : 1.0; };
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
 #pragma unroll
     for (int j = 0; j < N; ++j)
       A[j] = (tbc && h == 0) ? g_entry<NN>(a, t0, NN - 1 - r, j) * etop_of(ecur, j) : 0.0;
-    if (h == 1) rhs = tbc ? top - a.eig[(eo.ztop() + (tbc ? NN - 1 - r : 0)) * T + t0] : 0.0;
+    if (h == 1) rhs = tbc ? top - a.eig[eo.at(t0, eo.ztop() + (tbc ? NN - 1 - r : 0))] : 0.0;
     used = !(row && r < NN);
   }
 
@@ -621,7 +621,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
 #pragma unroll
       for (int j = 0; j < N; ++j) A[j] = fill ? -g_entry<NN>(a, tt + 1, q, j) * etop_of(enxt, j) : 0.0;
     }
-    if (h == 1 && fill) rhs = a.eig[(eo.ztop() + q) * T + tt + 1] - a.eig[(eo.zbot() + q) * T + tt];
+    if (h == 1 && fill) rhs = a.eig[eo.at(tt + 1, eo.ztop() + q)] - a.eig[eo.at(tt, eo.zbot() + q)];
     used = !row;
     eliminate(left, a.panel + tt * PS);
   }
@@ -639,7 +639,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
       rsb[j] = refl ? s : 0.0;
     }
     double zrefl = 0.0;
-    for (int k = 0; k < NN; ++k) zrefl += tb[to.cwt() + k] * tb[to.cmu() + k] * alb * a.eig[(eo.zbot() + NN - 1 - k) * T + tt];
+    for (int k = 0; k < NN; ++k) zrefl += tb[to.cwt() + k] * tb[to.cmu() + k] * alb * a.eig[eo.at(tt, eo.zbot() + NN - 1 - k)];
     const int q = used_rank();
     const bool fill = row && used;
     const bool bot = fill && q < NN;
@@ -660,7 +660,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
       if (refl) extra += 2.0 * zrefl;
       if (F0 > 0.0 && !lyrcut) extra += alb * mu0 * F0 * btn / kPi;
       if (thermal) extra += (1.0 - alb) * plb;
-      const double zb = -a.eig[(eo.zbot() + NN + tq) * T + tt];
+      const double zb = -a.eig[eo.at(tt, eo.zbot() + NN + tq)];
       rhs = bot ? zb + (diffusion ? plb + tb[to.cmu() + tq] * plbd : extra) : (fill ? 0.0 : rhs);
     }
     used = !row || (fill && !bot);
@@ -703,7 +703,7 @@ void launch_wide(const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
   if (ev) (void)hipEventRecord(ev[2], s);
   hipLaunchKernelGGL(k_boundary_wg<NN>, dim3((unsigned)a.nb), dim3(C::BT), 0, s, a);
   if (ev) (void)hipEventRecord(ev[3], s);
-  hipLaunchKernelGGL(k_flux<NN>, dim3((unsigned)((a.n_lvtasks + 63) / 64)), dim3(64), 0, s, a);
+  launch_flux_levels<NN>(a, s);
   if (ev) (void)hipEventRecord(ev[4], s);
 }
 

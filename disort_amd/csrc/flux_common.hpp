@@ -123,6 +123,16 @@ __device__ __forceinline__ double group_sum(double x) {
   return x;
 }
 
+// Maximum of x over a 16-lane DPP row (every lane gets it): rotations by 8,
+// 4, 2, 1 cover the row.  fmax ignores NaN.
+__device__ __forceinline__ double row_max(double x) {
+  x = fmax(x, dpp_f64<0x128>(x));  // row_ror:8
+  x = fmax(x, dpp_f64<0x124>(x));  // row_ror:4
+  x = fmax(x, dpp_f64<0x122>(x));  // row_ror:2
+  x = fmax(x, dpp_f64<0x121>(x));  // row_ror:1
+  return x;
+}
+
 // LDS ordering inside one wavefront: LDS instructions of a wavefront execute
 // in order, so only the compiler must not move loads and stores across.
 __device__ __forceinline__ void wave_lds_order() {

@@ -504,15 +504,12 @@ struct RowCfg {
   static_assert(R <= 32, "one slot per lane of a 32-lane group");
 };
 
-// Maximum of x over the LPI lanes of the caller's group (every lane gets it).
-// Rotations by 8, 4, 2, 1 within a 16-lane row cover the row; a swizzle with
-// lane xor 16 joins the two rows of a 32-lane group.  fmax ignores NaN.
+// Maximum of x over the LPI lanes of the caller's group (every lane gets it):
+// the 16-lane row maximum, joined across the two rows of a 32-lane group by
+// a swizzle with lane xor 16.
 template <int LPI>
 __device__ __forceinline__ double group_max(double x) {
-  x = fmax(x, dpp_f64<0x128>(x));  // row_ror:8
-  x = fmax(x, dpp_f64<0x124>(x));  // row_ror:4
-  x = fmax(x, dpp_f64<0x122>(x));  // row_ror:2
-  x = fmax(x, dpp_f64<0x121>(x));  // row_ror:1
+  x = row_max(x);
   if constexpr (LPI == 32) x = fmax(x, swz_xor<16>(x));
   return x;
 }

@@ -16,10 +16,10 @@
 //   k_boundary_wg  one workgroup per instance: the panel LU with partial
 //                  pivoting of the register kernel, with one thread per
 //                  (panel row, block): rows stay in registers, the pivot row
-//                  and column go through LDS.  Pivot choices, operation order
-//                  and the stored factors are those of the register kernel,
-//                  so for N <= 16 both produce the same bits from the same
-//                  eigen-system.
+//                  and column go through LDS, the elimination steps are a
+//                  run-time loop.  Pivot choices and operation order are those
+//                  of the register kernel, so for N <= 16 both produce the
+//                  same integration constants from the same eigen-system.
 // k_column and k_flux (flux_common.hpp) are shared with the register path.
 #include "wg_common.hpp"
 
@@ -480,8 +480,8 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
   constexpr EigOff eo(N);
   constexpr TabOff to(N);
   constexpr int PS = panel_stride(N);
-  __shared__ double prow[2 * N + 1];  // pivot row: left block, right block, rhs
-  __shared__ double colc[RP];         // pivot column
+  __shared__ double prow[2 * N + 2];  // pivot row: left block (shifted), right block, rhs
+  __shared__ double colc[2][RP];      // pivot-column entry of every row, by step parity
   __shared__ double red_v[NWH];
   __shared__ int red_i[NWH];
   __shared__ int uflag[RP];
@@ -526,70 +526,94 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
     const bool tbc = row && r < NN;
     const double top = sc[DISORT_S_ISO_TOP] + a.inst_d[i * kInstD + ID_PL_TOP];
 #pragma unroll
-    for (int j = 0; j < N; ++j)
-      A[j] = (tbc && h == 0) ? g_entry<NN>(a, t0, NN - 1 - r, j) * etop_of(ecur, j) : 0.0;
+    for (int j = 0; j < N; ++j) A[j] = 0.0;
+    if (tbc && h == 0) {
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j) A[j] = g_entry<NN>(a, t0, NN - 1 - r, j) * etop_of(ecur, j);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
     if (h == 1) rhs = tbc ? top - a.eig[eo.at(t0, eo.ztop() + (tbc ? NN - 1 - r : 0))] : 0.0;
     used = !(row && r < NN);
   }
 
-  // N elimination steps of one panel.  `used` rows are not candidates;
-  // `left` tells whether this thread holds the pivot-column block.
+  // N elimination steps of one panel, a run-time loop (the unrolled form
+  // does not fit the instruction cache at N = 64).  The block holding the
+  // pivot columns shifts left by one column per step, so the pivot column
+  // is always A[0] and registers are indexed statically; `left` tells
+  // whether this thread holds that block.  `used` rows are not candidates.
+  // Barriers: partial maxima published, pivot row published.
   auto eliminate = [&](bool left, double* pp) __attribute__((always_inline)) {
-#pragma unroll
+#pragma unroll 1
     for (int c = 0; c < N; ++c) {
-      double key = -1.0;
+      const int b = c & 1;
       if (left) {
-        colc[r] = A[c];
-        key = (row && !used) ? fabs(A[c]) : -1.0;
-      }
-      int idx = r;
-      for (int off = 32; off >= 1; off >>= 1) {
-        const double k2 = __shfl_xor(key, off);
-        const int i2 = __shfl_xor(idx, off);
-        const bool take = (k2 > key) || (k2 == key && i2 < idx);
-        key = take ? k2 : key;
-        idx = take ? i2 : idx;
-      }
-      if (left && (tid & 63) == 0) {
-        red_v[r >> 6] = key;
-        red_i[r >> 6] = idx;
+        colc[b][r] = A[0];
+        const double key = (row && !used) ? fabs(A[0]) : -1.0;
+        double m = row_max(key);
+        m = fmax(m, swz_xor<16>(m));
+        const double m0 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(m), 0),
+                                           __builtin_amdgcn_readlane(__double2loint(m), 0));
+        const double m1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(m), 32),
+                                           __builtin_amdgcn_readlane(__double2loint(m), 32));
+        m = fmax(m0, m1);
+        const unsigned long long bal = __ballot(key == m);
+        if ((tid & 63) == 0) {
+          red_v[r >> 6] = m;
+          red_i[r >> 6] = (r & ~63) + __ffsll((long long)bal) - 1;
+        }
       }
       __syncthreads();
       double best = red_v[0];
       int p = red_i[0];
+#pragma unroll
       for (int w = 1; w < NWH; ++w) {
-        const double k2 = red_v[w];
-        const int i2 = red_i[w];
-        if (k2 > best || (k2 == best && i2 < p)) {
-          best = k2;
-          p = i2;
+        if (red_v[w] > best) {  // strictly larger: ties keep the earlier row
+          best = red_v[w];
+          p = red_i[w];
         }
       }
-      const double piv = colc[p];
-      const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
-      const double l = colc[r] * inv;
-      if (r == p) {
+      if (row && r == p) {
         const int o = left ? 0 : N;
 #pragma unroll
         for (int j = 0; j < N; ++j) prow[o + j] = A[j];
-        if (left) {
-#pragma unroll
-          for (int j = 0; j < N; ++j) pp[c * N + j] = (j == c) ? inv : A[j];
-        } else {
-#pragma unroll
-          for (int j = 0; j < N; ++j) pp[N * N + c * N + j] = A[j];
-        }
-        if (h == 1) {
-          prow[2 * N] = rhs;
-          pp[2 * N * N + c] = rhs;
-        }
+        if (h == 1) prow[2 * N] = rhs;
       }
       __syncthreads();
-      const int o = left ? 0 : N;
+      const double piv = prow[0];
+      const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
+      const double l = (row && !used && r != p) ? colc[b][r] * inv : 0.0;
+      // Chunks of 8 columns, kept apart by scheduling barriers, so that at
+      // most one chunk of the pivot row is live in registers.
+      if (left) {
 #pragma unroll
-      for (int j = 0; j < N; ++j) A[j] -= l * prow[o + j];
+        for (int j0 = 0; j0 < N - 1; j0 += 8) {
+#pragma unroll
+          for (int j = j0; j < j0 + 8 && j < N - 1; ++j) A[j] = A[j + 1] - l * prow[j + 1];
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        A[N - 1] = 0.0;
+      } else {
+#pragma unroll
+        for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+          for (int j = j0; j < j0 + 8 && j < N; ++j) A[j] -= l * prow[N + j];
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
       if (h == 1) rhs -= l * prow[2 * N];
       used = used || (r == p);
+      // U row c (left block with the diagonal as its reciprocal, right
+      // block, rhs), stored by the first wavefront from the LDS copy.
+      if (tid < 64) {
+        for (int t = tid; t < N; t += 64) {
+          pp[c * N + t] = (t < c) ? 0.0 : ((t == c) ? inv : prow[t - c]);
+          pp[N * N + c * N + t] = prow[N + t];
+        }
+        if (tid == 0) pp[2 * N * N + c] = prow[2 * N];
+      }
     }
   };
 
@@ -612,14 +636,27 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
     layer_exp(tt + 1, enxt);
     const int q = used_rank();  // includes the barrier that publishes ecur/enxt
     const bool fill = row && used;
+    // Fills in chunks of 8 columns (scheduling barriers bound the loads in
+    // flight, and so the registers, next to the live row).
     if (left) {
       if (fill) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) A[j] = g_entry<NN>(a, tt, q, j) * ebot_of(ecur, j);
+        for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+          for (int j = j0; j < j0 + 8 && j < N; ++j) A[j] = g_entry<NN>(a, tt, q, j) * ebot_of(ecur, j);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    } else if (fill) {
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j) A[j] = -g_entry<NN>(a, tt + 1, q, j) * etop_of(enxt, j);
+        __builtin_amdgcn_sched_barrier(0);
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < N; ++j) A[j] = fill ? -g_entry<NN>(a, tt + 1, q, j) * etop_of(enxt, j) : 0.0;
+      for (int j = 0; j < N; ++j) A[j] = 0.0;
     }
     if (h == 1 && fill) rhs = a.eig[eo.at(tt + 1, eo.ztop() + q)] - a.eig[eo.at(tt, eo.zbot() + q)];
     used = !row;
@@ -645,9 +682,18 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
     const bool bot = fill && q < NN;
     const int tq = bot ? q : 0;
     if (left) {
+      if (bot) {
 #pragma unroll
-      for (int j = 0; j < N; ++j)
-        A[j] = bot ? (g_entry<NN>(a, tt, NN + tq, j) - 2.0 * rsb[j]) * ebot_of(ecur, j) : (fill ? 0.0 : A[j]);
+        for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+          for (int j = j0; j < j0 + 8 && j < N; ++j)
+            A[j] = (g_entry<NN>(a, tt, NN + tq, j) - 2.0 * rsb[j]) * ebot_of(ecur, j);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      } else if (fill) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) A[j] = 0.0;
+      }
     } else {
 #pragma unroll
       for (int j = 0; j < N; ++j) A[j] = 0.0;
@@ -667,7 +713,8 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
     eliminate(left, a.panel + tt * PS);
   }
 
-  // Back substitution, thread k = U row k of each panel (h = 0 threads).
+  // Back substitution, lane k = U row k of each panel, first wavefront only.
+  if (tid >= 64) return;
   const int k = tid;
   for (int ip = M - 1; ip >= 0; --ip) {
     const int64_t tt = t0 + ip;
@@ -680,15 +727,15 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
     }
     for (int c = N - 1; c >= 0; --c) {
       if (k == c) xs[c] = vk * pp[c * N + c];
-      __syncthreads();
+      wave_lds_order();
       if (k < c) vk -= pp[k * N + c] * xs[c];
     }
-    __syncthreads();
+    wave_lds_order();
     if (k < N) {
       a.cvec[tt * N + k] = xs[k];
       xn[k] = xs[k];
     }
-    __syncthreads();
+    wave_lds_order();
   }
 }
 

@@ -716,8 +716,8 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
   constexpr TabOff to(N);
   constexpr GE ge(N);
   constexpr int PS = panel_stride(N);
-  __shared__ double prow[2 * N + 1];
-  __shared__ double colc[RP];
+  __shared__ double prow[2 * N + 2];  // pivot row: left block (shifted), right block, rhs
+  __shared__ double colc[2][RP];      // pivot-column entry of every row, by step parity
   __shared__ double red_v[NWH];
   __shared__ int red_i[NWH];
   __shared__ int uflag[RP];
@@ -767,67 +767,94 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
     const bool tbc = row && r < NN;
     const double top = (m == 0) ? sc[DISORT_S_ISO_TOP] + a.inst_d[i * kInstD + ID_PL_TOP] : 0.0;
 #pragma unroll
-    for (int j = 0; j < N; ++j) A[j] = (tbc && h == 0) ? ge_G<NN>(eig(t0), NN - 1 - r, j) * etop_of(ecur, j) : 0.0;
+    for (int j = 0; j < N; ++j) A[j] = 0.0;
+    if (tbc && h == 0) {
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j) A[j] = ge_G<NN>(eig(t0), NN - 1 - r, j) * etop_of(ecur, j);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
     if (h == 1) rhs = tbc ? top - eig(t0)[ge.ztop() + (tbc ? NN - 1 - r : 0)] : 0.0;
     used = !(row && r < NN);
   }
 
+  // N elimination steps of one panel, a run-time loop (the unrolled form
+  // does not fit the instruction cache at N = 64).  The block holding the
+  // pivot columns shifts left by one column per step, so the pivot column
+  // is always A[0] and registers are indexed statically; `left` tells
+  // whether this thread holds that block.  `used` rows are not candidates.
+  // Barriers: partial maxima published, pivot row published.
   auto eliminate = [&](bool left, double* pp) __attribute__((always_inline)) {
-#pragma unroll
+#pragma unroll 1
     for (int c = 0; c < N; ++c) {
-      double key = -1.0;
+      const int b = c & 1;
       if (left) {
-        colc[r] = A[c];
-        key = (row && !used) ? fabs(A[c]) : -1.0;
-      }
-      int idx = r;
-      for (int off = 32; off >= 1; off >>= 1) {
-        const double k2 = __shfl_xor(key, off);
-        const int i2 = __shfl_xor(idx, off);
-        const bool take = (k2 > key) || (k2 == key && i2 < idx);
-        key = take ? k2 : key;
-        idx = take ? i2 : idx;
-      }
-      if (left && (tid & 63) == 0) {
-        red_v[r >> 6] = key;
-        red_i[r >> 6] = idx;
+        colc[b][r] = A[0];
+        const double key = (row && !used) ? fabs(A[0]) : -1.0;
+        double m = row_max(key);
+        m = fmax(m, swz_xor<16>(m));
+        const double m0 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(m), 0),
+                                           __builtin_amdgcn_readlane(__double2loint(m), 0));
+        const double m1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(m), 32),
+                                           __builtin_amdgcn_readlane(__double2loint(m), 32));
+        m = fmax(m0, m1);
+        const unsigned long long bal = __ballot(key == m);
+        if ((tid & 63) == 0) {
+          red_v[r >> 6] = m;
+          red_i[r >> 6] = (r & ~63) + __ffsll((long long)bal) - 1;
+        }
       }
       __syncthreads();
       double best = red_v[0];
       int p = red_i[0];
+#pragma unroll
       for (int w = 1; w < NWH; ++w) {
-        const double k2 = red_v[w];
-        const int i2 = red_i[w];
-        if (k2 > best || (k2 == best && i2 < p)) {
-          best = k2;
-          p = i2;
+        if (red_v[w] > best) {  // strictly larger: ties keep the earlier row
+          best = red_v[w];
+          p = red_i[w];
         }
       }
-      const double piv = colc[p];
-      const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
-      const double l = colc[r] * inv;
-      if (r == p) {
+      if (row && r == p) {
         const int o = left ? 0 : N;
 #pragma unroll
         for (int j = 0; j < N; ++j) prow[o + j] = A[j];
-        if (left) {
-#pragma unroll
-          for (int j = 0; j < N; ++j) pp[c * N + j] = (j == c) ? inv : A[j];
-        } else {
-#pragma unroll
-          for (int j = 0; j < N; ++j) pp[N * N + c * N + j] = A[j];
-        }
-        if (h == 1) {
-          prow[2 * N] = rhs;
-          pp[2 * N * N + c] = rhs;
-        }
+        if (h == 1) prow[2 * N] = rhs;
       }
       __syncthreads();
-      const int o = left ? 0 : N;
+      const double piv = prow[0];
+      const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
+      const double l = (row && !used && r != p) ? colc[b][r] * inv : 0.0;
+      // Chunks of 8 columns, kept apart by scheduling barriers, so that at
+      // most one chunk of the pivot row is live in registers.
+      if (left) {
 #pragma unroll
-      for (int j = 0; j < N; ++j) A[j] -= l * prow[o + j];
+        for (int j0 = 0; j0 < N - 1; j0 += 8) {
+#pragma unroll
+          for (int j = j0; j < j0 + 8 && j < N - 1; ++j) A[j] = A[j + 1] - l * prow[j + 1];
+          __builtin_amdgcn_sched_barrier(0);
+        }
+        A[N - 1] = 0.0;
+      } else {
+#pragma unroll
+        for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+          for (int j = j0; j < j0 + 8 && j < N; ++j) A[j] -= l * prow[N + j];
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
       if (h == 1) rhs -= l * prow[2 * N];
       used = used || (r == p);
+      // U row c (left block with the diagonal as its reciprocal, right
+      // block, rhs), stored by the first wavefront from the LDS copy.
+      if (tid < 64) {
+        for (int t = tid; t < N; t += 64) {
+          pp[c * N + t] = (t < c) ? 0.0 : ((t == c) ? inv : prow[t - c]);
+          pp[N * N + c * N + t] = prow[N + t];
+        }
+        if (tid == 0) pp[2 * N * N + c] = prow[2 * N];
+      }
     }
   };
 
@@ -847,14 +874,27 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
     layer_exp(tt + 1, enxt);
     const int q = used_rank();
     const bool fill = row && used;
+    // Fills in chunks of 8 columns (scheduling barriers bound the loads in
+    // flight, and so the registers, next to the live row).
     if (left) {
       if (fill) {
 #pragma unroll
-        for (int j = 0; j < N; ++j) A[j] = ge_G<NN>(eig(tt), q, j) * ebot_of(ecur, j);
+        for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+          for (int j = j0; j < j0 + 8 && j < N; ++j) A[j] = ge_G<NN>(eig(tt), q, j) * ebot_of(ecur, j);
+          __builtin_amdgcn_sched_barrier(0);
+        }
+      }
+    } else if (fill) {
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j) A[j] = -ge_G<NN>(eig(tt + 1), q, j) * etop_of(enxt, j);
+        __builtin_amdgcn_sched_barrier(0);
       }
     } else {
 #pragma unroll
-      for (int j = 0; j < N; ++j) A[j] = fill ? -ge_G<NN>(eig(tt + 1), q, j) * etop_of(enxt, j) : 0.0;
+      for (int j = 0; j < N; ++j) A[j] = 0.0;
     }
     if (h == 1 && fill) rhs = eig(tt + 1)[ge.ztop() + q] - eig(tt)[ge.zbot() + q];
     used = !row;
@@ -874,17 +914,22 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
     const bool bot = fill && q < NN;
     const int tq = bot ? q : 0;
     if (left) {
+      if (bot) {
 #pragma unroll
-      for (int j = 0; j < N; ++j) {
-        double v = 0.0;
-        if (bot) {
-          double rs = 0.0;
-          if (couple)
-            for (int k = 0; k < NN; ++k)
-              rs += tb[to.cwt() + k] * tb[to.cmu() + k] * rq(tq, k) * ge_G<NN>(Eb, NN - 1 - k, j);
-          v = (ge_G<NN>(Eb, NN + tq, j) - (1.0 + d0) * rs) * ebot_of(ecur, j);
+        for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+          for (int j = j0; j < j0 + 8 && j < N; ++j) {
+            double rs = 0.0;
+            if (couple)
+              for (int k = 0; k < NN; ++k)
+                rs += tb[to.cwt() + k] * tb[to.cmu() + k] * rq(tq, k) * ge_G<NN>(Eb, NN - 1 - k, j);
+            A[j] = (ge_G<NN>(Eb, NN + tq, j) - (1.0 + d0) * rs) * ebot_of(ecur, j);
+          }
+          __builtin_amdgcn_sched_barrier(0);
         }
-        A[j] = bot ? v : (fill ? 0.0 : A[j]);
+      } else if (fill) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) A[j] = 0.0;
       }
     } else {
 #pragma unroll
@@ -926,6 +971,11 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
     eliminate(left, a.panel + tt * PS);
   }
 
+  // Layers below the cut-off carry no solution.
+  for (int e = tid; e < (L - M) * N; e += blockDim.x) a.cvec[(t0 + M) * N + e] = 0.0;
+
+  // Back substitution, lane k = U row k of each panel, first wavefront only.
+  if (tid >= 64) return;
   const int k = tid;
   for (int ip = M - 1; ip >= 0; --ip) {
     const int64_t tt = t0 + ip;
@@ -938,18 +988,16 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
     }
     for (int c = N - 1; c >= 0; --c) {
       if (k == c) xs[c] = vk * pp[c * N + c];
-      __syncthreads();
+      wave_lds_order();
       if (k < c) vk -= pp[k * N + c] * xs[c];
     }
-    __syncthreads();
+    wave_lds_order();
     if (k < N) {
       a.cvec[tt * N + k] = xs[k];
       xn[k] = xs[k];
     }
-    __syncthreads();
+    wave_lds_order();
   }
-  // Layers below the cut-off carry no solution.
-  for (int e = tid; e < (L - M) * N; e += blockDim.x) a.cvec[(t0 + M) * N + e] = 0.0;
 }
 
 // ===========================================================================

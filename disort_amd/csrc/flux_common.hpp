@@ -133,6 +133,16 @@ __device__ __forceinline__ double row_max(double x) {
   return x;
 }
 
+// Maximum of x over the aligned 32-lane half of the wavefront, given x
+// uniform within each 16-lane row (a row maximum).  v_permlane16_swap with
+// both operands x returns the even rows of x in its first result and the odd
+// rows in its second, in every lane of each row pair, without the LDS path.
+__device__ __forceinline__ double row_pair_max(double x) {
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(x), (unsigned)__double2loint(x), false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(x), (unsigned)__double2hiint(x), false, false);
+  return fmax(__hiloint2double((int)hi[0], (int)lo[0]), __hiloint2double((int)hi[1], (int)lo[1]));
+}
+
 // LDS ordering inside one wavefront: LDS instructions of a wavefront execute
 // in order, so only the compiler must not move loads and stores across.
 __device__ __forceinline__ void wave_lds_order() {

@@ -506,11 +506,11 @@ struct RowCfg {
 
 // Maximum of x over the LPI lanes of the caller's group (every lane gets it):
 // the 16-lane row maximum, joined across the two rows of a 32-lane group by
-// a swizzle with lane xor 16.
+// a permlane16 swap.
 template <int LPI>
 __device__ __forceinline__ double group_max(double x) {
   x = row_max(x);
-  if constexpr (LPI == 32) x = fmax(x, swz_xor<16>(x));
+  if constexpr (LPI == 32) x = row_pair_max(x);
   return x;
 }
 

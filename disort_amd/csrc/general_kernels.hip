@@ -794,7 +794,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
         colc[b][r] = A[0];
         const double key = (row && !used) ? fabs(A[0]) : -1.0;
         double m = row_max(key);
-        m = fmax(m, swz_xor<16>(m));
+        m = row_pair_max(m);
         const double m0 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(m), 0),
                                            __builtin_amdgcn_readlane(__double2loint(m), 0));
         const double m1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(m), 32),

@@ -177,14 +177,17 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   constexpr int N = C::N, LD = C::LD, NNE = NN + (NN & 1), NP = NNE / 2;
   constexpr TabOff to(N);
   constexpr GE ge(N);
+  // LDS holds the dense NN x NN work matrices and short vectors; the Y_l^m
+  // table (shared by every task of the mode) and G+/- (just written to the
+  // task record) are read through the caches, to keep the footprint small
+  // enough for more wavefronts per CU.
   __shared__ double sbuf[4 * NN * LD];  // A_s | -B_s | S | V, later the interpolation matrix W
   double* const sA = sbuf;
   double* const sL = sbuf + NN * LD;
   double* const sS = sbuf + 2 * NN * LD;
   double* const sV = sbuf + 3 * NN * LD;
-  __shared__ double sGp[NN * NN], sGm[NN * NN];
   __shared__ double chi[N], p0[N];
-  __shared__ double sP[N * NN], qn[NN], cw[NN];  // Y_l^m(mu_r) for l >= m (zero below), sqrt(w_r / mu_r), w_r
+  __shared__ double qn[NN], cw[NN];  // sqrt(w_r / mu_r), w_r
   __shared__ double v0[NN], v1[NN], v2[NN], v3[NN], lam[NN], dv[NN];
   __shared__ double zr_u[NN], zr_d[NN], z0_u[NN], z0_d[NN], z1_h[NN];
   __shared__ double psi[3 * N];
@@ -225,7 +228,6 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
     else c = (2.0 * l + 1.0) * ws * (pl - f * exp(-(double)(l * l) / (2.0 * sig2))) / (1.0 - f);
     chi[l] = c;
   }
-  for (int e = tid; e < N * NN; e += blockDim.x) sP[e] = (e / NN >= m) ? Pm[e] : 0.0;
   for (int r = tid; r < NN; r += blockDim.x) {
     qn[r] = tb[to.sd() + r] / tb[to.cmu() + r];
     cw[r] = tb[to.cwt() + r];
@@ -284,7 +286,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
     const int r = r0 >= c0 ? r0 : c0, c = r0 >= c0 ? c0 : r0;
     double se = 0.0, so = 0.0;
     for (int l = m; l < N; ++l) {
-      const double q = chi[l] * sP[l * NN + r] * sP[l * NN + c];
+      const double q = chi[l] * Pm[l * NN + r] * Pm[l * NN + c];
       if (((l - m) & 1) == 0) se += q;
       else so += q;
     }
@@ -452,8 +454,6 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
     const double gp = 0.5 * (tt + uu), gmv = 0.5 * (tt - uu);
     E[ge.gp() + r + j * NN] = gp;
     E[ge.gm() + r + j * NN] = gmv;
-    sGp[r * NN + j] = gp;  // row-major: lanes of the interpolation read along j
-    sGm[r * NN + j] = gmv;
   }
 
   const bool thermal = (a.flags & DISORT_F_THERMAL) != 0 && m == 0;
@@ -526,7 +526,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   if (beam && tid < NN) {
     double se = 0.0, so = 0.0;
     for (int l = m; l < N; ++l) {
-      const double term = chi[l] * sP[l * NN + tid] * p0[l];
+      const double term = chi[l] * Pm[l * NN + tid] * p0[l];
       if (((l - m) & 1) == 0) se += term;
       else so += term;
     }
@@ -654,8 +654,8 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
       const int jp = up ? col - NN : NN - 1 - col;
 #pragma unroll 2
       for (int r = 0; r < NN; ++r) {
-        const double w = cw[r] * sP[l * NN + r];
-        const double gp = sGp[r * NN + jp], gmv = sGm[r * NN + jp];
+        const double w = cw[r] * Pm[l * NN + r];
+        const double gp = E[ge.gp() + r + jp * NN], gmv = E[ge.gm() + r + jp * NN];
         s += w * (up ? gp + sgn * gmv : -(gmv + sgn * gp));
       }
     }
@@ -669,7 +669,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
       const double sgn = ((l - m) & 1) ? -1.0 : 1.0;
 #pragma unroll 2
       for (int r = 0; r < NN; ++r) {
-        const double w = cw[r] * sP[l * NN + r];
+        const double w = cw[r] * Pm[l * NN + r];
         const double xu = (k == 0) ? zr_u[r] : (k == 1 ? z0_u[r] : z1_h[r]);
         const double xd = (k == 0) ? zr_d[r] : (k == 1 ? z0_d[r] : z1_h[r]);
         s += w * (xu + sgn * xd);

@@ -181,11 +181,16 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   // table (shared by every task of the mode) and G+/- (just written to the
   // task record) are read through the caches, to keep the footprint small
   // enough for more wavefronts per CU.
-  __shared__ double sbuf[4 * NN * LD];  // A_s | -B_s | S | V, later the interpolation matrix W
+  // Three NN x NN slots: A_s, then S (formed once T = -A_s L is held in the
+  // third slot), then U = L^-T V, then the Cholesky factor of -A_s (rebuilt
+  // when a thermal or spherical source needs it) | -B_s, then L | T, then
+  // V.  The interpolation matrix W takes two of them, half its columns at a
+  // time.
+  __shared__ double sbuf[3 * NN * LD];
   double* const sA = sbuf;
+  double* const sS = sbuf;
   double* const sL = sbuf + NN * LD;
-  double* const sS = sbuf + 2 * NN * LD;
-  double* const sV = sbuf + 3 * NN * LD;
+  double* const sV = sbuf + 2 * NN * LD;
   __shared__ double chi[N], p0[N];
   __shared__ double qn[NN], cw[NN];  // sqrt(w_r / mu_r), w_r
   __shared__ double v0[NN], v1[NN], v2[NN], v3[NN], lam[NN], dv[NN];
@@ -459,7 +464,20 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   const bool thermal = (a.flags & DISORT_F_THERMAL) != 0 && m == 0;
 
   if (thermal || spherical) {
-    for (int e = tid; e < NN * NN; e += blockDim.x) sA[e / NN * LD + e % NN] = -sA[e / NN * LD + e % NN];
+    // -A_s again (the slot held S and U meanwhile), entry by entry as built
+    // above, then its Cholesky factor.
+    __syncthreads();
+    for (int e = tid; e < NN * NN; e += blockDim.x) {
+      const int r0 = e / NN, c0 = e % NN;
+      const int r = r0 >= c0 ? r0 : c0, c = r0 >= c0 ? c0 : r0;
+      double se = 0.0;
+      for (int l = m; l < N; ++l) {
+        const double q = chi[l] * Pm[l * NN + r] * Pm[l * NN + c];
+        if (((l - m) & 1) == 0) se += q;
+      }
+      const double dg = (r == c) ? 1.0 / tb[to.cmu() + r] : 0.0;
+      sA[r0 * LD + c0] = -(qn[r] * qn[c] * se - dg);
+    }
     __syncthreads();
     blk_cholesky<NN>(sA, LD, &flag);
   }
@@ -642,25 +660,6 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   __syncthreads();
   const int numu = a.numu;
   const double* Yu = a.ylu + (int64_t)m * N * numu;  // [l][iu]
-  double* W = sbuf;                                   // [l][col], N*N <= 4*NN*LD
-  // W(l, col) = chi_l / 2 sum_jq w_jq Y_l^m(mu_jq) G(row(jq), col).
-#pragma unroll 1
-  for (int e = tid; e < N * N; e += blockDim.x) {
-    const int l = e / N, col = e % N;
-    double s = 0.0;
-    if (l >= m) {
-      const double sgn = ((l - m) & 1) ? -1.0 : 1.0;
-      const bool up = col >= NN;
-      const int jp = up ? col - NN : NN - 1 - col;
-#pragma unroll 2
-      for (int r = 0; r < NN; ++r) {
-        const double w = cw[r] * Pm[l * NN + r];
-        const double gp = E[ge.gp() + r + jp * NN], gmv = E[ge.gm() + r + jp * NN];
-        s += w * (up ? gp + sgn * gmv : -(gmv + sgn * gp));
-      }
-    }
-    W[e] = s * (0.5 * chi[l]);
-  }
   // psi for the beam and thermal sources.
   for (int e = tid; e < 3 * N; e += blockDim.x) {
     const int k = e / N, l = e % N;
@@ -677,14 +676,38 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
     }
     psi[e] = 0.5 * chi[l] * s;
   }
-  __syncthreads();
+  // W(l, col) = chi_l / 2 sum_jq w_jq Y_l^m(mu_jq) G(row(jq), col), then
+  // G_u = Y_u W, half of the N columns at a time (W[l][col - cb] in sbuf).
+  double* W = sbuf;
   double* gu = a.gu + t * (int64_t)numu * N;
-  for (int e = tid; e < numu * N; e += blockDim.x) {
-    const int iu = e / N, col = e % N;
-    double s = 0.0;
+#pragma unroll 1
+  for (int cb = 0; cb < N; cb += NN) {
+#pragma unroll 1
+    for (int e = tid; e < N * NN; e += blockDim.x) {
+      const int l = e / NN, col = cb + e % NN;
+      double s = 0.0;
+      if (l >= m) {
+        const double sgn = ((l - m) & 1) ? -1.0 : 1.0;
+        const bool up = col >= NN;
+        const int jp = up ? col - NN : NN - 1 - col;
+#pragma unroll 2
+        for (int r = 0; r < NN; ++r) {
+          const double w = cw[r] * Pm[l * NN + r];
+          const double gp = E[ge.gp() + r + jp * NN], gmv = E[ge.gm() + r + jp * NN];
+          s += w * (up ? gp + sgn * gmv : -(gmv + sgn * gp));
+        }
+      }
+      W[e] = s * (0.5 * chi[l]);
+    }
+    __syncthreads();
+    for (int e = tid; e < numu * NN; e += blockDim.x) {
+      const int iu = e / NN, cc = e % NN;
+      double s = 0.0;
 #pragma unroll 8
-    for (int l = m; l < N; ++l) s += Yu[l * numu + iu] * W[l * N + col];
-    gu[e] = s;
+      for (int l = m; l < N; ++l) s += Yu[l * numu + iu] * W[l * NN + cc];
+      gu[iu * N + cb + cc] = s;
+    }
+    __syncthreads();
   }
   double* us = a.usrc + t * (int64_t)kUSrc * numu;
   const double fact = fm * F0 / (4.0 * kPi);

@@ -1023,6 +1023,259 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
   }
 }
 
+// One-wavefront form of k_gboundary for 3NN <= 64 panel rows: lane r holds
+// both blocks of panel row r (Ac, the block of the pivot columns, shifted
+// left by one column per step; An, the next layer's columns) and the
+// right-hand side, so every synchronisation is wave-local.  Same pivots,
+// operations and stored factors as k_gboundary.
+template <int NN>
+__global__ __launch_bounds__(64) void k_gboundary1(GenArgs a) {
+  constexpr int N = 2 * NN, R = 3 * NN;
+  static_assert(R <= 64, "one panel row per lane");
+  constexpr TabOff to(N);
+  constexpr GE ge(N);
+  constexpr int PS = panel_stride(N);
+  __shared__ double prow[2 * N + 2];  // pivot row: Ac (shifted), An, rhs
+  __shared__ double ecur[NN], enxt[NN], xs[N], xn[N];
+
+  const int r = threadIdx.x;
+  const bool row = r < R;
+  const int64_t bk = blockIdx.x;
+  if (bk >= a.nc * a.nm) return;
+  const int64_t cl = bk / a.nm;
+  const int m = (int)(bk % a.nm);
+  const int64_t i = a.c0 + cl;
+  const int L = a.nlyr;
+  const int64_t t0 = bk * L;
+  const int64_t lo = i * L, TL = a.n * L;
+  const double* sc = a.scal + i * DISORT_NSCAL;
+  const double* tb = a.tab;
+  const int M = a.inst_i[i * kInstI + II_NCUT];
+  const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
+  const bool diffusion = (a.flags & DISORT_F_DIFFUSION_BC) != 0;
+  const bool thermal = (a.flags & DISORT_F_THERMAL) != 0;
+  const bool lamb = (a.flags & DISORT_G_BRDF_RPV) == 0;
+  const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
+  const double alb = sc[DISORT_S_ALBEDO];
+  const double d0 = (m == 0) ? 1.0 : 0.0;
+  const double* sd = lamb ? nullptr : a.srf_d + ((int64_t)a.rpv_set[i] * a.nm + m) * surf_d_size(NN, a.numu);
+  const double* sb = lamb ? nullptr : a.srf_b + (cl * a.nm + m) * surf_b_size(NN, a.numu);
+  auto rq = [&](int iq, int jq) { return lamb ? (m == 0 ? alb : 0.0) : sd[iq * NN + jq]; };
+  auto rq0 = [&](int iq) { return lamb ? (m == 0 ? alb : 0.0) : sb[iq]; };
+  auto eig = [&](int64_t tt) { return a.eig + tt * ge.size(); };
+  auto layer_exp = [&](int64_t tt, double* e) {
+    if (r < NN) e[r] = exp(-eig(tt)[ge.k() + r] * a.lay_d[LD_DTAU_S * TL + lo + (tt - t0)]);
+  };
+  auto etop_of = [](const double* e, int j) { return j < NN ? e[NN - 1 - j] : 1.0; };
+  auto ebot_of = [](const double* e, int j) { return j < NN ? 1.0 : e[j - NN]; };
+  // Rank of this row among the used rows.
+  auto used_rank = [&](bool used) {
+    const unsigned long long b = __ballot(row && used);
+    return __popcll(b & ((1ull << r) - 1ull));
+  };
+
+  double Ac[N], An[N];
+  double rhs = 0.0;
+  bool used;
+
+  // Top boundary rows 0..NN-1 (:890-907, :976-990), carried into panel 0.
+  layer_exp(t0, ecur);
+  wave_lds_order();
+  {
+    const bool tbc = row && r < NN;
+    const double top = (m == 0) ? sc[DISORT_S_ISO_TOP] + a.inst_d[i * kInstD + ID_PL_TOP] : 0.0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) An[j] = 0.0;
+    if (tbc) {
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j) An[j] = ge_G<NN>(eig(t0), NN - 1 - r, j) * etop_of(ecur, j);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    rhs = tbc ? top - eig(t0)[ge.ztop() + (tbc ? NN - 1 - r : 0)] : 0.0;
+    used = !tbc;
+  }
+
+  // N elimination steps of one panel (run-time loop; the pivot column is
+  // always Ac[0]).
+  auto eliminate = [&](double* pp) __attribute__((always_inline)) {
+#pragma unroll 1
+    for (int c = 0; c < N; ++c) {
+      const double key = (row && !used) ? fabs(Ac[0]) : -1.0;
+      double mx = row_pair_max(row_max(key));
+      const double m0 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(mx), 0),
+                                         __builtin_amdgcn_readlane(__double2loint(mx), 0));
+      const double m1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(mx), 32),
+                                         __builtin_amdgcn_readlane(__double2loint(mx), 32));
+      mx = fmax(m0, m1);
+      const int p = __ffsll((long long)__ballot(key == mx)) - 1;
+      wave_lds_order();
+      if (row && r == p) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          prow[j] = Ac[j];
+          prow[N + j] = An[j];
+        }
+        prow[2 * N] = rhs;
+      }
+      wave_lds_order();
+      const double piv = prow[0];
+      const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
+      const double l = (row && !used && r != p) ? Ac[0] * inv : 0.0;
+#pragma unroll
+      for (int j0 = 0; j0 < N - 1; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N - 1; ++j) Ac[j] = Ac[j + 1] - l * prow[j + 1];
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      Ac[N - 1] = 0.0;
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j) An[j] -= l * prow[N + j];
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      rhs -= l * prow[2 * N];
+      used = used || (r == p);
+      // U row c: left block with the diagonal as its reciprocal, right block, rhs.
+      for (int t = r; t < N; t += 64) {
+        pp[c * N + t] = (t < c) ? 0.0 : ((t == c) ? inv : prow[t - c]);
+        pp[N * N + c * N + t] = prow[N + t];
+      }
+      if (r == 0) pp[2 * N * N + c] = prow[2 * N];
+    }
+  };
+
+  // Interior panels: continuity at the interface below layer ip.
+  for (int ip = 0; ip < M - 1; ++ip) {
+    const int64_t tt = t0 + ip;
+    layer_exp(tt, ecur);
+    layer_exp(tt + 1, enxt);
+    wave_lds_order();
+    const int q = used_rank(used);
+    const bool fill = row && used;
+    if (fill) {
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j) {
+          Ac[j] = ge_G<NN>(eig(tt), q, j) * ebot_of(ecur, j);
+          An[j] = -ge_G<NN>(eig(tt + 1), q, j) * etop_of(enxt, j);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      rhs = eig(tt + 1)[ge.ztop() + q] - eig(tt)[ge.zbot() + q];
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        Ac[j] = An[j];
+        An[j] = 0.0;
+      }
+    }
+    used = !row;
+    eliminate(a.panel + tt * PS);
+  }
+
+  // Bottom boundary: reflection of the downward field, coupled at m = 0 for
+  // a Lambertian surface and at every mode for a BRDF.
+  {
+    const int64_t tt = t0 + M - 1;
+    const double* Eb = eig(tt);
+    const bool couple = !(lyrcut || diffusion || (lamb && m > 0));
+    layer_exp(tt, ecur);
+    wave_lds_order();
+    const int q = used_rank(used);
+    const bool fill = row && used;
+    const bool bot = fill && q < NN;
+    const int tq = bot ? q : 0;
+    if (bot) {
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j) {
+          double rs = 0.0;
+          if (couple)
+            for (int k = 0; k < NN; ++k)
+              rs += tb[to.cwt() + k] * tb[to.cmu() + k] * rq(tq, k) * ge_G<NN>(Eb, NN - 1 - k, j);
+          Ac[j] = (ge_G<NN>(Eb, NN + tq, j) - (1.0 + d0) * rs) * ebot_of(ecur, j);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    } else if (fill) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) Ac[j] = 0.0;
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) Ac[j] = An[j];
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) An[j] = 0.0;
+    {
+      const double btn = a.inst_d[i * kInstD + ID_BTRANS_NCUT];
+      double v = 0.0;
+      if (m == 0) {
+        const double plb = a.inst_d[i * kInstD + ID_PL_BOT];
+        if (diffusion) {
+          v = -Eb[ge.zbot() + NN + tq] + plb + tb[to.cmu() + tq] * a.inst_d[i * kInstD + ID_PL_BOT_DERIV];
+        } else {
+          if (!lyrcut) {
+            double rs = 0.0;
+            for (int k = 0; k < NN; ++k) rs += tb[to.cwt() + k] * tb[to.cmu() + k] * rq(tq, k) * Eb[ge.zbot() + NN - 1 - k];
+            v += 2.0 * rs;
+          }
+          v -= Eb[ge.zbot() + NN + tq];
+          if (F0 > 0.0 && !lyrcut) v += rq0(tq) * mu0 * F0 * btn / kPi;
+          v += sc[DISORT_S_ISO_BOT];
+          if (thermal) {
+            const double eq = lamb ? 1.0 - alb : sd[NN * NN + a.numu * NN + tq];
+            v += eq * plb;
+          }
+        }
+      } else if (F0 > 0.0) {
+        if (lyrcut || lamb) {
+          v = -Eb[ge.zbot() + NN + tq];
+        } else {
+          double s = 0.0;
+          for (int k = 0; k < NN; ++k) s += tb[to.cwt() + k] * tb[to.cmu() + k] * rq(tq, k) * Eb[ge.zq() + NN - 1 - k];
+          v = s + rq0(tq) * mu0 * F0 / kPi * btn - Eb[ge.zq() + NN + tq];
+        }
+      }
+      rhs = bot ? v : (fill ? 0.0 : rhs);
+    }
+    used = !row || (fill && !bot);
+    eliminate(a.panel + tt * PS);
+  }
+
+  // Layers below the cut-off carry no solution.
+  for (int e = r; e < (L - M) * N; e += 64) a.cvec[(t0 + M) * N + e] = 0.0;
+
+  // Back substitution, lane k = U row k of each panel.
+  const int k = r;
+  for (int ip = M - 1; ip >= 0; --ip) {
+    const int64_t tt = t0 + ip;
+    const double* pp = a.panel + tt * PS;
+    double vk = 0.0;
+    if (k < N) {
+      vk = pp[2 * N * N + k];
+      if (ip < M - 1)
+        for (int c = 0; c < N; ++c) vk -= pp[N * N + k * N + c] * xn[c];
+    }
+    for (int c = N - 1; c >= 0; --c) {
+      if (k == c) xs[c] = vk * pp[c * N + c];
+      wave_lds_order();
+      if (k < c) vk -= pp[k * N + c] * xs[c];
+    }
+    wave_lds_order();
+    if (k < N) {
+      a.cvec[tt * N + k] = xs[k];
+      xn[k] = xs[k];
+    }
+    wave_lds_order();
+  }
+}
+
 // ===========================================================================
 // k_gflux: fluxes at the user levels (src/DisortSolver.cpp:1980-2124),
 // thread per (chunk column, level), from the m = 0 tasks.
@@ -1594,7 +1847,8 @@ void general_chunk_n(const GenArgs& a, const GenLaunch& g, hipStream_t s) {
   rec(1);
   hipLaunchKernelGGL(k_glayer<NN>, dim3((unsigned)a.n_tasks), dim3(kGLT), 0, s, a);
   rec(2);
-  hipLaunchKernelGGL(k_gboundary<NN>, dim3(ncm), dim3(C::BT), 0, s, a);
+  if constexpr (3 * NN <= 64) hipLaunchKernelGGL(k_gboundary1<NN>, dim3(ncm), dim3(64), 0, s, a);
+  else hipLaunchKernelGGL(k_gboundary<NN>, dim3(ncm), dim3(C::BT), 0, s, a);
   rec(3);
   hipLaunchKernelGGL(k_gflux<NN>, dim3((unsigned)((a.nc * a.ntau + 63) / 64)), dim3(64), 0, s, a);
   rec(4);

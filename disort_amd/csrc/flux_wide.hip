@@ -30,17 +30,22 @@ namespace {
 // k_layer_wg: one workgroup per (instance, layer) task.
 // ===========================================================================
 template <int NN>
-__global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
+__global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(FluxArgs a) {
   using C = WideCfg<NN>;
   // Round-robin pairing over NNE = NN rounded up to even players; with odd
   // NN the extra player is a dummy and its pairs rotate nothing.
   constexpr int N = C::N, LD = C::LD, NNE = NN + (NN & 1), NP = NNE / 2;
   constexpr TabOff to(N);
   constexpr EigOff eo(N);
-  __shared__ double sA[NN * LD];  // A_s, later -A_s and its Cholesky factor
-  __shared__ double sL[NN * LD];  // -B_s, then its Cholesky factor L
-  __shared__ double sS[NN * LD];  // S = L^T (-A_s) L, diagonalised in place
-  __shared__ double sV[NN * LD];  // eigenvectors (columns)
+  // Three NN x NN slots: A_s, then S = L^T (-A_s) L (diagonalised in
+  // place; formed once T = -A_s L is held in the third slot), then U, then
+  // the Cholesky factor of -A_s (rebuilt when a thermal or spherical source
+  // needs it) | -B_s, then its Cholesky factor L | T, then the eigenvectors.
+  __shared__ double sbuf[3 * NN * LD];
+  double* const sA = sbuf;
+  double* const sS = sbuf;
+  double* const sL = sbuf + NN * LD;
+  double* const sV = sbuf + 2 * NN * LD;
   __shared__ double chi[N], p0[N];
   __shared__ double v0[NN], v1[NN], v2[NN], v3[NN], lam[NN], dv[NN];
   __shared__ double rc[NP], rs[NP], rt[NP], rapp[NP], raqq[NP], rapq[NP];
@@ -256,7 +261,20 @@ __global__ __launch_bounds__(WideCfg<NN>::LT) void k_layer_wg(FluxArgs a) {
   // R = Cholesky factor of -A_s (in sA), for the thermal source and the
   // unattenuated spherical source.
   if (thermal || spherical) {
-    for (int e = tid; e < NN * NN; e += blockDim.x) sA[e / NN * LD + e % NN] = -sA[e / NN * LD + e % NN];
+    // -A_s again (the slot held S and U meanwhile), entry by entry as built
+    // above, then its Cholesky factor.
+    __syncthreads();
+    for (int e = tid; e < NN * NN; e += blockDim.x) {
+      const int r0 = e / NN, c0 = e % NN;
+      const int r = r0 >= c0 ? r0 : c0, c = r0 >= c0 ? c0 : r0;
+      double se = 0.0;
+      for (int l = 0; l < N; ++l) {
+        const double q = chi[l] * tb[to.Q() + l * NN + r] * tb[to.Q() + l * NN + c];
+        if (l % 2 == 0) se += q;
+      }
+      const double dg = (r == c) ? 1.0 / tb[to.cmu() + r] : 0.0;
+      sA[r0 * LD + c0] = -(se - dg);
+    }
     __syncthreads();
     blk_cholesky<NN>(sA, LD, &flag);
   }

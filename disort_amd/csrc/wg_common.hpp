@@ -12,6 +12,7 @@ template <int NN>
 struct WideCfg {
   static constexpr int N = 2 * NN;
   static constexpr int LT = NN >= 32 ? 256 : 64;  // k_layer_wg threads: one wavefront (wave-local barriers) up to N = 32, four at N = 64
+  static constexpr int LW = NN >= 32 ? 5 : (NN >= 16 ? 3 : 1);  // k_layer_wg waves per SIMD asked for
   static constexpr int R = 3 * NN;                // panel rows
   static constexpr int RP = (R + 63) / 64 * 64;   // rows padded to whole waves
   static constexpr int BT = 2 * RP;               // k_boundary_wg threads

@@ -492,7 +492,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
 // block takes the next layer's columns.
 // ===========================================================================
 template <int NN>
-__global__ __launch_bounds__(WideCfg<NN>::BT) void k_boundary_wg(FluxArgs a) {
+__global__ __launch_bounds__(WideCfg<NN>::BT, 2) void k_boundary_wg(FluxArgs a) {
   using C = WideCfg<NN>;
   constexpr int N = C::N, R = C::R, RP = C::RP, NWH = RP / 64;
   constexpr EigOff eo(N);

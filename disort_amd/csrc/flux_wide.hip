@@ -757,6 +757,229 @@ __global__ __launch_bounds__(WideCfg<NN>::BT, 2) void k_boundary_wg(FluxArgs a) 
   }
 }
 
+// One-wavefront form of k_boundary_wg (launched for N <= 16; it works up to
+// 3NN <= 64 panel rows, but at N = 32 the two-block form is faster): lane r holds
+// both blocks of panel row r (Ac, the block of the pivot columns, shifted
+// left by one column per step; An, the next layer's columns) and the
+// right-hand side, so every synchronisation is wave-local.  Same pivots,
+// operations and stored factors as k_boundary_wg.
+template <int NN>
+__global__ __launch_bounds__(64) void k_boundary_wg1(FluxArgs a) {
+  constexpr int N = 2 * NN, R = 3 * NN;
+  static_assert(R <= 64, "one panel row per lane");
+  constexpr EigOff eo(N);
+  constexpr TabOff to(N);
+  constexpr int PS = panel_stride(N);
+  __shared__ double prow[2 * N + 2];  // pivot row: Ac (shifted), An, rhs
+  __shared__ double ecur[NN], enxt[NN], rsb[N], xs[N], xn[N];
+
+  const int r = threadIdx.x;
+  const bool row = r < R;
+  const int bk = blockIdx.x;
+  if (bk >= a.nb) return;
+  const int i = a.b_inst[bk];
+  const int64_t t0 = a.b_task0[bk];
+  const int64_t lo = a.lay_off[i];
+  const int64_t TL = a.total_layers;
+  const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
+  const double* tb = a.tab;
+  const int M = a.inst_i[i * kInstI + II_NCUT];
+  const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
+  const bool diffusion = (a.flags & DISORT_F_DIFFUSION_BC) != 0;
+  const bool thermal = (a.flags & DISORT_F_THERMAL) != 0;
+  const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
+  const double alb = sc[DISORT_S_ALBEDO];
+
+  auto layer_exp = [&](int64_t tt, double* e) {
+    if (r < NN) e[r] = exp(-a.eig[eo.at(tt, eo.k() + r)] * a.lay_d[LD_DTAU_S * TL + lo + (tt - t0)]);
+  };
+  auto etop_of = [](const double* e, int j) { return j < NN ? e[NN - 1 - j] : 1.0; };
+  auto ebot_of = [](const double* e, int j) { return j < NN ? 1.0 : e[j - NN]; };
+  auto used_rank = [&](bool used) {
+    const unsigned long long b = __ballot(row && used);
+    return __popcll(b & ((1ull << r) - 1ull));
+  };
+
+  double Ac[N], An[N];
+  double rhs = 0.0;
+  bool used;
+
+  // Top boundary rows 0..NN-1 (:890-907, :976-990), carried into panel 0.
+  layer_exp(t0, ecur);
+  wave_lds_order();
+  {
+    const bool tbc = row && r < NN;
+    const double top = sc[DISORT_S_ISO_TOP] + a.inst_d[i * kInstD + ID_PL_TOP];
+#pragma unroll
+    for (int j = 0; j < N; ++j) An[j] = 0.0;
+    if (tbc) {
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j) An[j] = g_entry<NN>(a, t0, NN - 1 - r, j) * etop_of(ecur, j);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    rhs = tbc ? top - a.eig[eo.at(t0, eo.ztop() + (tbc ? NN - 1 - r : 0))] : 0.0;
+    used = !tbc;
+  }
+
+  // N elimination steps of one panel (run-time loop; the pivot column is
+  // always Ac[0]).
+  auto eliminate = [&](double* pp) __attribute__((always_inline)) {
+#pragma unroll 1
+    for (int c = 0; c < N; ++c) {
+      const double key = (row && !used) ? fabs(Ac[0]) : -1.0;
+      double mx = row_pair_max(row_max(key));
+      const double m0 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(mx), 0),
+                                         __builtin_amdgcn_readlane(__double2loint(mx), 0));
+      const double m1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(mx), 32),
+                                         __builtin_amdgcn_readlane(__double2loint(mx), 32));
+      mx = fmax(m0, m1);
+      const int p = __ffsll((long long)__ballot(key == mx)) - 1;
+      wave_lds_order();
+      if (row && r == p) {
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+          prow[j] = Ac[j];
+          prow[N + j] = An[j];
+        }
+        prow[2 * N] = rhs;
+      }
+      wave_lds_order();
+      const double piv = prow[0];
+      const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
+      const double l = (row && !used && r != p) ? Ac[0] * inv : 0.0;
+#pragma unroll
+      for (int j0 = 0; j0 < N - 1; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N - 1; ++j) Ac[j] = Ac[j + 1] - l * prow[j + 1];
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      Ac[N - 1] = 0.0;
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j) An[j] -= l * prow[N + j];
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      rhs -= l * prow[2 * N];
+      used = used || (r == p);
+      // U row c: left block with the diagonal as its reciprocal, right block, rhs.
+      for (int t = r; t < N; t += 64) {
+        pp[c * N + t] = (t < c) ? 0.0 : ((t == c) ? inv : prow[t - c]);
+        pp[N * N + c * N + t] = prow[N + t];
+      }
+      if (r == 0) pp[2 * N * N + c] = prow[2 * N];
+    }
+  };
+
+  // Interior panels: continuity at the interface below layer ip
+  // (:851-887, :1094-1118).
+  for (int ip = 0; ip < M - 1; ++ip) {
+    const int64_t tt = t0 + ip;
+    layer_exp(tt, ecur);
+    layer_exp(tt + 1, enxt);
+    wave_lds_order();
+    const int q = used_rank(used);
+    const bool fill = row && used;
+    if (fill) {
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j) {
+          Ac[j] = g_entry<NN>(a, tt, q, j) * ebot_of(ecur, j);
+          An[j] = -g_entry<NN>(a, tt + 1, q, j) * etop_of(enxt, j);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      rhs = a.eig[eo.at(tt + 1, eo.ztop() + q)] - a.eig[eo.at(tt, eo.zbot() + q)];
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        Ac[j] = An[j];
+        An[j] = 0.0;
+      }
+    }
+    used = !row;
+    eliminate(a.panel + tt * PS);
+  }
+
+  // Last panel: bottom boundary, Lambertian reflection of the downward field
+  // (:909-959, :992-1091).
+  {
+    const int64_t tt = t0 + M - 1;
+    const bool refl = !(lyrcut || diffusion);
+    layer_exp(tt, ecur);
+    if (r < N) {
+      double s = 0.0;
+      for (int k = 0; k < NN; ++k) s += tb[to.cwt() + k] * tb[to.cmu() + k] * alb * g_entry<NN>(a, tt, NN - 1 - k, r);
+      rsb[r] = refl ? s : 0.0;
+    }
+    wave_lds_order();
+    double zrefl = 0.0;
+    for (int k = 0; k < NN; ++k) zrefl += tb[to.cwt() + k] * tb[to.cmu() + k] * alb * a.eig[eo.at(tt, eo.zbot() + NN - 1 - k)];
+    const int q = used_rank(used);
+    const bool fill = row && used;
+    const bool bot = fill && q < NN;
+    const int tq = bot ? q : 0;
+    if (bot) {
+#pragma unroll
+      for (int j0 = 0; j0 < N; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8 && j < N; ++j)
+          Ac[j] = (g_entry<NN>(a, tt, NN + tq, j) - 2.0 * rsb[j]) * ebot_of(ecur, j);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    } else if (fill) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) Ac[j] = 0.0;
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) Ac[j] = An[j];
+    }
+#pragma unroll
+    for (int j = 0; j < N; ++j) An[j] = 0.0;
+    {
+      const double btn = a.inst_d[i * kInstD + ID_BTRANS_NCUT];
+      const double plb = a.inst_d[i * kInstD + ID_PL_BOT];
+      const double plbd = a.inst_d[i * kInstD + ID_PL_BOT_DERIV];
+      double extra = sc[DISORT_S_ISO_BOT];
+      if (refl) extra += 2.0 * zrefl;
+      if (F0 > 0.0 && !lyrcut) extra += alb * mu0 * F0 * btn / kPi;
+      if (thermal) extra += (1.0 - alb) * plb;
+      const double zb = -a.eig[eo.at(tt, eo.zbot() + NN + tq)];
+      rhs = bot ? zb + (diffusion ? plb + tb[to.cmu() + tq] * plbd : extra) : (fill ? 0.0 : rhs);
+    }
+    used = !row || (fill && !bot);
+    eliminate(a.panel + tt * PS);
+  }
+
+  // Back substitution, lane k = U row k of each panel.
+  const int k = r;
+  for (int ip = M - 1; ip >= 0; --ip) {
+    const int64_t tt = t0 + ip;
+    const double* pp = a.panel + tt * PS;
+    double vk = 0.0;
+    if (k < N) {
+      vk = pp[2 * N * N + k];
+      if (ip < M - 1)
+        for (int c = 0; c < N; ++c) vk -= pp[N * N + k * N + c] * xn[c];
+    }
+    for (int c = N - 1; c >= 0; --c) {
+      if (k == c) xs[c] = vk * pp[c * N + c];
+      wave_lds_order();
+      if (k < c) vk -= pp[k * N + c] * xs[c];
+    }
+    wave_lds_order();
+    if (k < N) {
+      a.cvec[tt * N + k] = xs[k];
+      xn[k] = xs[k];
+    }
+    wave_lds_order();
+  }
+}
+
 template <int NN>
 void launch_wide(const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
   using C = WideCfg<NN>;
@@ -766,7 +989,8 @@ void launch_wide(const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
   if (ev) (void)hipEventRecord(ev[1], s);
   hipLaunchKernelGGL(k_layer_wg<NN>, dim3((unsigned)a.n_tasks), dim3(C::LT), 0, s, a);
   if (ev) (void)hipEventRecord(ev[2], s);
-  hipLaunchKernelGGL(k_boundary_wg<NN>, dim3((unsigned)a.nb), dim3(C::BT), 0, s, a);
+  if constexpr (NN <= 8) hipLaunchKernelGGL(k_boundary_wg1<NN>, dim3((unsigned)a.nb), dim3(64), 0, s, a);  // faster there; not at N = 32
+  else hipLaunchKernelGGL(k_boundary_wg<NN>, dim3((unsigned)a.nb), dim3(C::BT), 0, s, a);
   if (ev) (void)hipEventRecord(ev[3], s);
   launch_flux_levels<NN>(a, s);
   if (ev) (void)hipEventRecord(ev[4], s);

@@ -3,23 +3,26 @@
 // alternative schedule for N = 4 ... 16.
 //
 // The register kernels of flux_kernels.hip keep a whole NN x NN eigenproblem
-// in one lane and a 3NN-row panel in 16 lanes; at N = 32 and 64 that no
-// longer fits.  Here
-//   k_layer_wg     one workgroup per (instance, layer): the matrices of the
-//                  reduced eigenproblem (src/FluxSolver.cpp:459-588) live in
-//                  LDS; Cholesky factors are built column by column, the
-//                  symmetric eigenproblem is solved by parallel cyclic Jacobi
-//                  (round-robin pairing, NN/2 disjoint rotations per round),
-//                  and the particular solutions (:595-643, :745-831) use the
-//                  same eigen-decomposition / Cholesky formulation as the
-//                  register kernel, with one thread per vector entry.
+// in one lane and a 3NN-row panel in 16 or 32 lanes; at N = 32 and 64 that
+// no longer fits.  Here
+//   k_layer_wg     one workgroup per (instance, layer): one wavefront up to
+//                  N = 32, four at N = 64.  The matrices of the reduced
+//                  eigenproblem (src/FluxSolver.cpp:459-588) live in three
+//                  LDS slots; Cholesky factors are built column by column,
+//                  the symmetric eigenproblem is solved by parallel cyclic
+//                  Jacobi (round-robin pairing, NN/2 disjoint rotations per
+//                  round), and the particular solutions (:595-643, :745-831)
+//                  use the same eigen-decomposition / Cholesky formulation as
+//                  the register kernel, with one thread per vector entry.
 //   k_boundary_wg  one workgroup per instance: the panel LU with partial
 //                  pivoting of the register kernel, with one thread per
 //                  (panel row, block): rows stay in registers, the pivot row
 //                  and column go through LDS, the elimination steps are a
-//                  run-time loop.  Pivot choices and operation order are those
-//                  of the register kernel, so for N <= 16 both produce the
-//                  same integration constants from the same eigen-system.
+//                  run-time loop.  k_boundary_wg1 is its one-wavefront form
+//                  (lane = panel row holding both blocks), used up to N = 16.
+//                  Pivot choices and operation order are those of the
+//                  register kernel, so for N <= 16 both produce the same
+//                  integration constants from the same eigen-system.
 // k_column and k_flux (flux_common.hpp) are shared with the register path.
 #include "wg_common.hpp"
 

@@ -9,12 +9,13 @@
 //   k_glevels       per column: layer and scaled depth of each user level
 //   k_gsurf_d/_b    BRDF Fourier coefficients per (RPV parameter set, mode)
 //                   and the beam column per (column, mode)
-//   k_glayer        per (column, mode, layer) workgroup: phase matrix at mode
-//                   m, reduced eigenproblem (parallel Jacobi in LDS), beam and
-//                   thermal particular solutions, eigenvectors and sources
-//                   interpolated to the user angles
-//   k_gboundary     per (column, mode) workgroup: panel LU of the boundary
-//                   problem with the mode's surface coupling
+//   k_glayer        per (column, mode, layer) wavefront: phase matrix at
+//                   mode m, reduced eigenproblem (parallel Jacobi in LDS),
+//                   beam and thermal particular solutions, eigenvectors and
+//                   sources interpolated to the user angles
+//   k_gboundary1    per (column, mode) wavefront (3NN <= 64 panel rows) or
+//   k_gboundary     workgroup: panel LU of the boundary problem with the
+//                   mode's surface coupling
 //   k_gflux         per (column, level): fluxes at the user levels (m = 0)
 //   k_guser         per (column, mode) workgroup: intensity components at the
 //                   user angles and levels

@@ -154,10 +154,12 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
         rq[k] = q;
         if (q < NN) {
           const double apq = sS[q * LD + p], app = sS[p * LD + p], aqq = sS[q * LD + q];
-          const double theta = (aqq - app) / (2.0 * apq);
-          const double at = fabs(theta);
-          double tt = (at > 1e150) ? 0.5 / at : 1.0 / (at + sqrt(theta * theta + 1.0));
-          tt = (theta < 0.0) ? -tt : tt;
+          // t = tan of the rotation angle, the root of t^2 + 2 theta t = 1
+          // (theta = d / h) of smaller magnitude, sign(theta) |h| /
+          // (|d| + sqrt(d^2 + h^2)): one division instead of three.
+          const double d = aqq - app, h = 2.0 * apq;
+          const bool neg = (d != 0.0) && ((d < 0.0) != (h < 0.0));
+          double tt = (neg ? -fabs(h) : fabs(h)) / (fabs(d) + sqrt(d * d + h * h));
           tt = (apq == 0.0) ? 0.0 : tt;
           const double cs = 1.0 / sqrt(tt * tt + 1.0);
           rc[k] = cs;

@@ -61,6 +61,9 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
   if (t >= a.n_tasks) return;
   const int i = a.t_inst[t];
   const int lc = a.t_lc[t];
+  // No output depends on a layer below the absorption cut-off (ncut is nlyr
+  // when the cut-off is not in effect).
+  if (lc >= a.inst_i[(int64_t)i * kInstI + II_NCUT]) return;
   const int64_t g = a.lay_off[i] + lc;
   const int64_t v = a.lay_off[i] + i + lc;
   const int P = a.pstride[i];

@@ -210,6 +210,11 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   const int64_t cm = t / L;
   const int m = (int)(cm % a.nm);
   const int64_t i = a.c0 + cm / a.nm;
+  // A layer below the absorption cut-off enters no output: the boundary
+  // problem spans the ncut layers above it, and fluxes and intensities at
+  // levels below it are zero.  Its task ends here (ncut is nlyr when the
+  // cut-off is not in effect).
+  if (lc >= a.inst_i[i * kInstI + II_NCUT]) return;
   const int64_t g = i * L + lc, v = i * (L + 1) + lc;
   const int P = a.pstride;
   const double* pm = a.pmom + g * P;
@@ -1392,6 +1397,9 @@ __global__ __launch_bounds__(64) void k_gucoef(GenArgs a, double* ucoef) {
   const int64_t lo = i * L, vo = i * (L + 1), TL = a.n * L, TV = a.n * (L + 1);
   const int ncut = a.inst_i[i * kInstI + II_NCUT];
   const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
+  // No record below the cut-off (k_glayer skipped the layer); k_guser reads
+  // none of those tasks' coefficients.
+  if (lc >= ncut) return;
   const double* sc = a.scal + i * DISORT_NSCAL;
   const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
   const double* E = a.eig + t * ge.size();
@@ -1409,8 +1417,7 @@ __global__ __launch_bounds__(64) void k_gucoef(GenArgs a, double* ucoef) {
   __syncthreads();
   for (int iu = threadIdx.x; iu < numu; iu += blockDim.x) {
     double* g = gu + (int64_t)iu * N;
-    if (lc < ncut)
-      for (int q = 0; q < N; ++q) g[q] *= LL[q];
+    for (int q = 0; q < N; ++q) g[q] *= LL[q];
     const double mu = a.mu_user[iu];
     const double sgn = mu < 0.0 ? -1.0 : 1.0;
     double c1 = 0.0, c2 = 0.0, c0 = 0.0;
@@ -1571,7 +1578,9 @@ __global__ __launch_bounds__(256) void k_guser(GenArgs a, const double* ucoef) {
       double bnd = 0.0;
       if (neg) {
         if (m == 0) bnd = (sc[DISORT_S_ISO_TOP] + pl_top) * exp(ut / mu);
-      } else {
+      } else if (!lyrcut) {
+        // With the cut-off in effect the surface term is zero (and the last
+        // layer's task did not run).
         const double* ucL = ucoef + (t0 + L - 1) * (int64_t)kUCoef * numu;
         bnd = ucL[UC_BND * numu + iu] * exp((ut - taus[L]) / mu);
       }

@@ -64,23 +64,25 @@ def general_mode_flops(nstr: int, nlyr: int, m: int, numu: int, ntau: int, ncut:
                        thermal: bool = False, beam: bool = True) -> dict:
     """Work of one azimuthal mode m of one column of the general solver
     (DisortSolver::solve, src/DisortSolver.cpp:218-350), by GPU kernel.
-    Legendre sums run over l = m..N-1; the boundary problem over the ncut
-    layers above the absorption cut-off; user intensities integrate on
-    average half of the layers per (level, angle)."""
+    Legendre sums run over l = m..N-1; the per-layer work (eigen-system,
+    particular solutions, interpolation to the user angles) and the boundary
+    problem over the ncut layers above the absorption cut-off (a layer below
+    it enters no output); user intensities integrate on average half of the
+    layers per (level, angle)."""
     N, L = nstr, nlyr
     NN = N // 2
     A = N - m  # active Legendre orders
     ncut = L if ncut is None else ncut
     n = N * ncut
     w = 3 * NN - 1
-    layer = L * (2 * NN * NN * A + 3 * NN * N            # A_s, -B_s from the scaled tables
+    layer = ncut * (2 * NN * NN * A + 3 * NN * N         # A_s, -B_s from the scaled tables
                  + (50.0 / 3.0) * NN**3 + 12 * NN * NN   # reduced eigenproblem
                  + ((2.0 / 3.0) * N**3 + 5 * N * N + N if beam else 0.0)
                  + ((2.0 / 3.0) * N**3 + 5 * N * N + 3 * N if thermal and m == 0 else 0.0)
                  + 2 * A * N * NN + 2 * numu * N * A      # eigenvectors at the user angles
                  + 2 * 3 * A * NN + 2 * 3 * numu * A)     # sources at the user angles
     boundary = ncut * 2 * N * N + 2 * n * w * w + 6 * n * w
-    user = L * numu * (N * 10.0) + ntau * numu * ((L / 2.0) * 8.0 + N * 12.0)
+    user = ncut * numu * (N * 10.0) + ntau * numu * ((L / 2.0) * 8.0 + N * 12.0)
     return {"k_glayer": float(layer), "k_gboundary": float(boundary), "k_guser": float(user)}
 
 

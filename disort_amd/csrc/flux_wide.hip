@@ -511,7 +511,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT, 2) void k_boundary_wg(FluxArgs a) 
   __shared__ double red_v[NWH];
   __shared__ int red_i[NWH];
   __shared__ int uflag[RP];
-  __shared__ double ecur[NN], enxt[NN], rsb[N], xs[N], xn[N];
+  __shared__ double ecur[NN], enxt[NN], rsb[N];
 
   const int tid = threadIdx.x;
   const int h = tid / RP, r = tid % RP;
@@ -741,28 +741,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT, 2) void k_boundary_wg(FluxArgs a) 
 
   // Back substitution, lane k = U row k of each panel, first wavefront only.
   if (tid >= 64) return;
-  const int k = tid;
-  for (int ip = M - 1; ip >= 0; --ip) {
-    const int64_t tt = t0 + ip;
-    const double* pp = a.panel + tt * PS;
-    double vk = 0.0;
-    if (k < N) {
-      vk = pp[2 * N * N + k];
-      if (ip < M - 1)
-        for (int c = 0; c < N; ++c) vk -= pp[N * N + k * N + c] * xn[c];
-    }
-    for (int c = N - 1; c >= 0; --c) {
-      if (k == c) xs[c] = vk * pp[c * N + c];
-      wave_lds_order();
-      if (k < c) vk -= pp[k * N + c] * xs[c];
-    }
-    wave_lds_order();
-    if (k < N) {
-      a.cvec[tt * N + k] = xs[k];
-      xn[k] = xs[k];
-    }
-    wave_lds_order();
-  }
+  panel_back_substitute<N>(a.panel, t0, M, a.cvec);
 }
 
 // One-wavefront form of k_boundary_wg (launched for N <= 16; it works up to
@@ -779,7 +758,7 @@ __global__ __launch_bounds__(64) void k_boundary_wg1(FluxArgs a) {
   constexpr TabOff to(N);
   constexpr int PS = panel_stride(N);
   __shared__ double prow[2 * N + 2];  // pivot row: Ac (shifted), An, rhs
-  __shared__ double ecur[NN], enxt[NN], rsb[N], xs[N], xn[N];
+  __shared__ double ecur[NN], enxt[NN], rsb[N];
 
   const int r = threadIdx.x;
   const bool row = r < R;
@@ -964,28 +943,7 @@ __global__ __launch_bounds__(64) void k_boundary_wg1(FluxArgs a) {
   }
 
   // Back substitution, lane k = U row k of each panel.
-  const int k = r;
-  for (int ip = M - 1; ip >= 0; --ip) {
-    const int64_t tt = t0 + ip;
-    const double* pp = a.panel + tt * PS;
-    double vk = 0.0;
-    if (k < N) {
-      vk = pp[2 * N * N + k];
-      if (ip < M - 1)
-        for (int c = 0; c < N; ++c) vk -= pp[N * N + k * N + c] * xn[c];
-    }
-    for (int c = N - 1; c >= 0; --c) {
-      if (k == c) xs[c] = vk * pp[c * N + c];
-      wave_lds_order();
-      if (k < c) vk -= pp[k * N + c] * xs[c];
-    }
-    wave_lds_order();
-    if (k < N) {
-      a.cvec[tt * N + k] = xs[k];
-      xn[k] = xs[k];
-    }
-    wave_lds_order();
-  }
+  panel_back_substitute<N>(a.panel, t0, M, a.cvec);
 }
 
 template <int NN>

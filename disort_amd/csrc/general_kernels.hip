@@ -752,7 +752,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
   __shared__ double red_v[NWH];
   __shared__ int red_i[NWH];
   __shared__ int uflag[RP];
-  __shared__ double ecur[NN], enxt[NN], xs[N], xn[N];
+  __shared__ double ecur[NN], enxt[NN];
 
   const int tid = threadIdx.x;
   const int h = tid / RP, r = tid % RP;
@@ -1007,28 +1007,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
 
   // Back substitution, lane k = U row k of each panel, first wavefront only.
   if (tid >= 64) return;
-  const int k = tid;
-  for (int ip = M - 1; ip >= 0; --ip) {
-    const int64_t tt = t0 + ip;
-    const double* pp = a.panel + tt * PS;
-    double vk = 0.0;
-    if (k < N) {
-      vk = pp[2 * N * N + k];
-      if (ip < M - 1)
-        for (int c = 0; c < N; ++c) vk -= pp[N * N + k * N + c] * xn[c];
-    }
-    for (int c = N - 1; c >= 0; --c) {
-      if (k == c) xs[c] = vk * pp[c * N + c];
-      wave_lds_order();
-      if (k < c) vk -= pp[k * N + c] * xs[c];
-    }
-    wave_lds_order();
-    if (k < N) {
-      a.cvec[tt * N + k] = xs[k];
-      xn[k] = xs[k];
-    }
-    wave_lds_order();
-  }
+  panel_back_substitute<N>(a.panel, t0, M, a.cvec);
 }
 
 // One-wavefront form of k_gboundary for 3NN <= 64 panel rows: lane r holds
@@ -1044,7 +1023,7 @@ __global__ __launch_bounds__(64) void k_gboundary1(GenArgs a) {
   constexpr GE ge(N);
   constexpr int PS = panel_stride(N);
   __shared__ double prow[2 * N + 2];  // pivot row: Ac (shifted), An, rhs
-  __shared__ double ecur[NN], enxt[NN], xs[N], xn[N];
+  __shared__ double ecur[NN], enxt[NN];
 
   const int r = threadIdx.x;
   const bool row = r < R;
@@ -1260,28 +1239,7 @@ __global__ __launch_bounds__(64) void k_gboundary1(GenArgs a) {
   for (int e = r; e < (L - M) * N; e += 64) a.cvec[(t0 + M) * N + e] = 0.0;
 
   // Back substitution, lane k = U row k of each panel.
-  const int k = r;
-  for (int ip = M - 1; ip >= 0; --ip) {
-    const int64_t tt = t0 + ip;
-    const double* pp = a.panel + tt * PS;
-    double vk = 0.0;
-    if (k < N) {
-      vk = pp[2 * N * N + k];
-      if (ip < M - 1)
-        for (int c = 0; c < N; ++c) vk -= pp[N * N + k * N + c] * xn[c];
-    }
-    for (int c = N - 1; c >= 0; --c) {
-      if (k == c) xs[c] = vk * pp[c * N + c];
-      wave_lds_order();
-      if (k < c) vk -= pp[k * N + c] * xs[c];
-    }
-    wave_lds_order();
-    if (k < N) {
-      a.cvec[tt * N + k] = xs[k];
-      xn[k] = xs[k];
-    }
-    wave_lds_order();
-  }
+  panel_back_substitute<N>(a.panel, t0, M, a.cvec);
 }
 
 // ===========================================================================

@@ -19,6 +19,51 @@ struct WideCfg {
   static constexpr int LD = NN + 1;               // LDS row pitch of NN x NN matrices
 };
 
+// Value of x in lane `lane` (wave-uniform) of the calling wavefront.
+__device__ __forceinline__ double lane_bcast(double x, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), lane),
+                          __builtin_amdgcn_readlane(__double2loint(x), lane));
+}
+
+// Back substitution over the M panels of one boundary problem, the whole
+// wavefront, lane k = U row k of each panel (N <= 64; other lanes repeat row
+// 0 and store nothing):
+//   x_k = (r_k - sum_c V(k,c) xnext_c - sum_{c>k} U(k,c) x_c) / U(k,k),
+// with the panel stored as [U with 1/U(k,k) on its diagonal | V | r].  The
+// x_c of other lanes come by readlane at a compile-time lane index, so the
+// column chain waits on no memory access and every load of the panel can be
+// issued up front.
+template <int N>
+__device__ __forceinline__ void panel_back_substitute(const double* panel, int64_t t0, int M, double* cvec) {
+  static_assert(N <= 64, "one U row per lane");
+  constexpr int PS = panel_stride(N);
+  const int lane = threadIdx.x & 63;
+  const int k = lane < N ? lane : 0;
+  double xn = 0.0;  // this lane's entry of the layer below
+  for (int ip = M - 1; ip >= 0; --ip) {
+    const int64_t tt = t0 + ip;
+    const double* pp = panel + tt * PS;
+    double v = pp[2 * N * N + k];
+    if (ip < M - 1) {
+      static_for<0, N>([&](auto cj) {
+        constexpr int c = decltype(cj)::value;
+        v -= pp[N * N + k * N + c] * lane_bcast(xn, c);
+      });
+    }
+    const double rdg = pp[k * N + k];
+    double xk = 0.0;
+    static_for<0, N>([&](auto cj) {
+      constexpr int c = N - 1 - decltype(cj)::value;
+      const double xc = lane_bcast(v * rdg, c);
+      const double u = pp[k * N + c];
+      xk = (k == c) ? xc : xk;
+      v = (k < c) ? v - u * xc : v;
+    });
+    xn = xk;
+    if (lane < N) cvec[tt * N + k] = xk;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Block-cooperative vector kernels on LDS (all threads call them).
 // ---------------------------------------------------------------------------

@@ -1013,16 +1013,18 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
 // One-wavefront form of k_gboundary for 3NN <= 64 panel rows: lane r holds
 // both blocks of panel row r (Ac, the block of the pivot columns, shifted
 // left by one column per step; An, the next layer's columns) and the
-// right-hand side, so every synchronisation is wave-local.  Same pivots,
-// operations and stored factors as k_gboundary.
+// right-hand side, so every synchronisation is wave-local.  Same pivots and
+// operations as k_gboundary; the pivot row reaches the other lanes by
+// readlane (no LDS traffic) and is stored from its lane with U in shifted
+// form.  Three wavefronts per SIMD: the register budget (168) spills only
+// outside the elimination loop.
 template <int NN>
-__global__ __launch_bounds__(64) void k_gboundary1(GenArgs a) {
+__global__ __launch_bounds__(64, 3) void k_gboundary1(GenArgs a) {
   constexpr int N = 2 * NN, R = 3 * NN;
   static_assert(R <= 64, "one panel row per lane");
   constexpr TabOff to(N);
   constexpr GE ge(N);
   constexpr int PS = panel_stride(N);
-  __shared__ double prow[2 * N + 2];  // pivot row: Ac (shifted), An, rhs
   __shared__ double ecur[NN], enxt[NN];
 
   const int r = threadIdx.x;
@@ -1092,46 +1094,40 @@ __global__ __launch_bounds__(64) void k_gboundary1(GenArgs a) {
     for (int c = 0; c < N; ++c) {
       const double key = (row && !used) ? fabs(Ac[0]) : -1.0;
       double mx = row_pair_max(row_max(key));
-      const double m0 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(mx), 0),
-                                         __builtin_amdgcn_readlane(__double2loint(mx), 0));
-      const double m1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(mx), 32),
-                                         __builtin_amdgcn_readlane(__double2loint(mx), 32));
-      mx = fmax(m0, m1);
+      mx = fmax(lane_bcast(mx, 0), lane_bcast(mx, 32));
       const int p = __ffsll((long long)__ballot(key == mx)) - 1;
-      wave_lds_order();
-      if (row && r == p) {
-#pragma unroll
-        for (int j = 0; j < N; ++j) {
-          prow[j] = Ac[j];
-          prow[N + j] = An[j];
-        }
-        prow[2 * N] = rhs;
-      }
-      wave_lds_order();
-      const double piv = prow[0];
+      // The pivot row stays in lane p: the other lanes read its entries by
+      // readlane (wave-uniform values, no LDS traffic), and lane p stores it
+      // as U row c in shifted form, [1/U(c,c), U(c,c+1), ..., U(c,N-1), 0...]
+      // | V(c,0..N-1) | r_c.
+      const double piv = lane_bcast(Ac[0], p);
       const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
+      if (r == p) {
+        double2* u = reinterpret_cast<double2*>(pp + c * N);
+        double2* w = reinterpret_cast<double2*>(pp + N * N + c * N);
+        u[0] = make_double2(inv, Ac[1]);
+#pragma unroll
+        for (int j = 2; j < N; j += 2) u[j / 2] = make_double2(Ac[j], Ac[j + 1]);
+#pragma unroll
+        for (int j = 0; j < N; j += 2) w[j / 2] = make_double2(An[j], An[j + 1]);
+        pp[2 * N * N + c] = rhs;
+      }
       const double l = (row && !used && r != p) ? Ac[0] * inv : 0.0;
 #pragma unroll
       for (int j0 = 0; j0 < N - 1; j0 += 8) {
 #pragma unroll
-        for (int j = j0; j < j0 + 8 && j < N - 1; ++j) Ac[j] = Ac[j + 1] - l * prow[j + 1];
+        for (int j = j0; j < j0 + 8 && j < N - 1; ++j) Ac[j] = Ac[j + 1] - l * lane_bcast(Ac[j + 1], p);
         __builtin_amdgcn_sched_barrier(0);
       }
       Ac[N - 1] = 0.0;
 #pragma unroll
       for (int j0 = 0; j0 < N; j0 += 8) {
 #pragma unroll
-        for (int j = j0; j < j0 + 8 && j < N; ++j) An[j] -= l * prow[N + j];
+        for (int j = j0; j < j0 + 8 && j < N; ++j) An[j] -= l * lane_bcast(An[j], p);
         __builtin_amdgcn_sched_barrier(0);
       }
-      rhs -= l * prow[2 * N];
+      rhs -= l * lane_bcast(rhs, p);
       used = used || (r == p);
-      // U row c: left block with the diagonal as its reciprocal, right block, rhs.
-      for (int t = r; t < N; t += 64) {
-        pp[c * N + t] = (t < c) ? 0.0 : ((t == c) ? inv : prow[t - c]);
-        pp[N * N + c * N + t] = prow[N + t];
-      }
-      if (r == 0) pp[2 * N * N + c] = prow[2 * N];
     }
   };
 
@@ -1239,7 +1235,7 @@ __global__ __launch_bounds__(64) void k_gboundary1(GenArgs a) {
   for (int e = r; e < (L - M) * N; e += 64) a.cvec[(t0 + M) * N + e] = 0.0;
 
   // Back substitution, lane k = U row k of each panel.
-  panel_back_substitute<N>(a.panel, t0, M, a.cvec);
+  panel_back_substitute<N, true>(a.panel, t0, M, a.cvec);
 }
 
 // ===========================================================================

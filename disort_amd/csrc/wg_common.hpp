@@ -29,17 +29,19 @@ __device__ __forceinline__ double lane_bcast(double x, int lane) {
 // wavefront, lane k = U row k of each panel (N <= 64; other lanes repeat row
 // 0 and store nothing):
 //   x_k = (r_k - sum_c V(k,c) xnext_c - sum_{c>k} U(k,c) x_c) / U(k,k),
-// with the panel stored as [U with 1/U(k,k) on its diagonal | V | r].  The
-// x_c of other lanes come by readlane at a compile-time lane index, so the
-// column chain waits on no memory access and every load of the panel can be
-// issued up front.
-template <int N>
+// with the panel stored as [U with 1/U(k,k) on its diagonal | V | r], U row
+// k either in place or, with Shifted, starting at its diagonal (entry (k, c)
+// at column c - k).  The x_c of other lanes come by readlane at a
+// compile-time lane index, so the column chain waits on no memory access and
+// every load of the panel can be issued up front.
+template <int N, bool Shifted = false>
 __device__ __forceinline__ void panel_back_substitute(const double* panel, int64_t t0, int M, double* cvec) {
   static_assert(N <= 64, "one U row per lane");
   constexpr int PS = panel_stride(N);
   const int lane = threadIdx.x & 63;
   const int k = lane < N ? lane : 0;
-  double xn = 0.0;  // this lane's entry of the layer below
+  wave_lds_order();  // the panel rows were stored by other lanes of this wavefront
+  double xn = 0.0;   // this lane's entry of the layer below
   for (int ip = M - 1; ip >= 0; --ip) {
     const int64_t tt = t0 + ip;
     const double* pp = panel + tt * PS;
@@ -50,12 +52,12 @@ __device__ __forceinline__ void panel_back_substitute(const double* panel, int64
         v -= pp[N * N + k * N + c] * lane_bcast(xn, c);
       });
     }
-    const double rdg = pp[k * N + k];
+    const double rdg = Shifted ? pp[k * N] : pp[k * N + k];
     double xk = 0.0;
     static_for<0, N>([&](auto cj) {
       constexpr int c = N - 1 - decltype(cj)::value;
       const double xc = lane_bcast(v * rdg, c);
-      const double u = pp[k * N + c];
+      const double u = Shifted ? pp[k * N + (c > k ? c - k : 0)] : pp[k * N + c];
       xk = (k == c) ? xc : xk;
       v = (k < c) ? v - u * xc : v;
     });

@@ -179,14 +179,13 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   constexpr TabOff to(N);
   constexpr GE ge(N);
   // LDS holds the dense NN x NN work matrices and short vectors; the Y_l^m
-  // table (shared by every task of the mode) and G+/- (just written to the
-  // task record) are read through the caches, to keep the footprint small
-  // enough for more wavefronts per CU.
+  // table (shared by every task of the mode) is read through the caches, to
+  // keep the footprint small enough for more wavefronts per CU.
   // Three NN x NN slots: A_s, then S (formed once T = -A_s L is held in the
   // third slot), then U = L^-T V, then the Cholesky factor of -A_s (rebuilt
   // when a thermal or spherical source needs it) | -B_s, then L | T, then
-  // V.  The interpolation matrix W takes two of them, half its columns at a
-  // time.
+  // V.  For the interpolation to the user angles: W, a block of its columns
+  // at a time | G+ | G-.
   __shared__ double sbuf[3 * NN * LD];
   double* const sA = sbuf;
   double* const sS = sbuf;
@@ -685,35 +684,50 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
     psi[e] = 0.5 * chi[l] * s;
   }
   // W(l, col) = chi_l / 2 sum_jq w_jq Y_l^m(mu_jq) G(row(jq), col), then
-  // G_u = Y_u W, half of the N columns at a time (W[l][col - cb] in sbuf).
-  double* W = sbuf;
+  // G_u = Y_u W, CW columns at a time (W[l][col - cb] in the first LDS
+  // slot); G+ and G- of the task record staged in the other two (column j'
+  // contiguous), so the inner sums read only LDS and the cached Y tables.
+  constexpr int CW = (NN + 1) / 2;  // N * CW <= NN * LD
+  double* const W = sbuf;
+  double* const sGp = sbuf + NN * LD;
+  double* const sGm = sbuf + 2 * NN * LD;
+  for (int e = tid; e < NN * NN; e += blockDim.x) {
+    const int j = e / NN, r = e % NN;
+    sGp[j * LD + r] = E[ge.gp() + e];
+    sGm[j * LD + r] = E[ge.gm() + e];
+  }
+  __syncthreads();
   double* gu = a.gu + t * (int64_t)numu * N;
 #pragma unroll 1
-  for (int cb = 0; cb < N; cb += NN) {
+  for (int cb = 0; cb < N; cb += CW) {
 #pragma unroll 1
-    for (int e = tid; e < N * NN; e += blockDim.x) {
-      const int l = e / NN, col = cb + e % NN;
+    for (int e = tid; e < N * CW; e += blockDim.x) {
+      const int l = e / CW, col = cb + e % CW;
       double s = 0.0;
-      if (l >= m) {
+      if (l >= m && col < N) {
         const double sgn = ((l - m) & 1) ? -1.0 : 1.0;
         const bool up = col >= NN;
         const int jp = up ? col - NN : NN - 1 - col;
-#pragma unroll 2
+        const double* gpc = sGp + jp * LD;
+        const double* gmc = sGm + jp * LD;
+#pragma unroll 4
         for (int r = 0; r < NN; ++r) {
           const double w = cw[r] * Pm[l * NN + r];
-          const double gp = E[ge.gp() + r + jp * NN], gmv = E[ge.gm() + r + jp * NN];
+          const double gp = gpc[r], gmv = gmc[r];
           s += w * (up ? gp + sgn * gmv : -(gmv + sgn * gp));
         }
       }
       W[e] = s * (0.5 * chi[l]);
     }
     __syncthreads();
-    for (int e = tid; e < numu * NN; e += blockDim.x) {
-      const int iu = e / NN, cc = e % NN;
-      double s = 0.0;
+    for (int e = tid; e < numu * CW; e += blockDim.x) {
+      const int iu = e / CW, cc = e % CW;
+      if (cb + cc < N) {
+        double s = 0.0;
 #pragma unroll 8
-      for (int l = m; l < N; ++l) s += Yu[l * numu + iu] * W[l * NN + cc];
-      gu[iu * N + cb + cc] = s;
+        for (int l = m; l < N; ++l) s += Yu[l * numu + iu] * W[l * CW + cc];
+        gu[iu * N + cb + cc] = s;
+      }
     }
     __syncthreads();
   }

@@ -148,11 +148,15 @@ def test_accuracy_cut_and_degenerate_angles(ctx):
     solve_both(ctx, gb, "accuracy cut, overhead sun, no beam")
 
 
-def test_layer_cut_off(ctx):
-    gb = random_general(2, 8, 16, seed=80)
+@pytest.mark.parametrize("nstr,extra", [(16, {}), (32, {"rpv": True, "spherical": True, "corr": True}),
+                                         (64, {})])
+def test_layer_cut_off(ctx, nstr, extra):
+    # Layers 3.. absorb (abstau >= 10 below layer 4): the tasks of the layers
+    # below the cut-off do not run, their levels read zero.
+    gb = random_general(2, 8, nstr, seed=80 + nstr, **extra)
     gb.dtau[:, 3:] = 30.0
     gb.ssa[:, 3:] = 0.05
-    solve_both(ctx, gb, "absorption cut-off")
+    solve_both(ctx, gb, f"absorption cut-off nstr={nstr}")
 
 
 def test_delta_m_plus(ctx):

@@ -35,6 +35,18 @@ def test_wide_delta_m_plus_and_diffusion(ctx, nstr):
     solve_and_check(ctx, b, label=f"diffusion nstr={nstr}")
 
 
+@pytest.mark.parametrize("nstr", [32, 64])
+def test_wide_layer_cutoff(ctx, nstr):
+    # Beam only, thick absorbing layers: the absorption cut-off fires after
+    # layer 3 and the layer tasks below it do not run.
+    L = 8
+    b = W.random_flux_batch(4, L, nstr, seed=600 + nstr, thermal=False)
+    b.dtau[:] = 4.0
+    b.ssa[:] = 0.2
+    ref = solve_and_check(ctx, b, label=f"lyrcut nstr={nstr}")
+    assert (ref[2].reshape(4, L + 1)[:, -1] == 0).all()  # levels below the cut are zero
+
+
 @pytest.mark.parametrize("nstr", [8, 16])
 def test_workgroup_variant_at_small_n(ctx, nstr):
     from disort_amd._capi import V_LANE, V_WORKGROUP, DeviceFluxBatch

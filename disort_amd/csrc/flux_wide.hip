@@ -40,6 +40,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
   constexpr int N = C::N, LD = C::LD, NNE = NN + (NN & 1), NP = NNE / 2;
   constexpr TabOff to(N);
   constexpr EigOff eo(N);
+  constexpr bool OW = C::LT == 64;  // one wavefront per block
   // Three NN x NN slots: A_s, then S = L^T (-A_s) L (diagonalised in
   // place; formed once T = -A_s L is held in the third slot), then U, then
   // the Cholesky factor of -A_s (rebuilt when a thermal or spherical source
@@ -111,7 +112,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
     sL[r0 * LD + c0] = dg - so;
   }
   __syncthreads();
-  if (!blk_cholesky<NN>(sL, LD, &flag) && tid == 0) atomicOr(&a.inst_i[i * kInstI + II_STATUS], DISORT_ST_EIGEN);
+  if (!blk_cholesky<NN, OW>(sL, LD, &flag) && tid == 0) atomicOr(&a.inst_i[i * kInstI + II_STATUS], DISORT_ST_EIGEN);
 
   // S = L^T (-A_s) L: T = -A_s L into sV, then S (lower, mirrored).
   for (int e = tid; e < NN * NN; e += blockDim.x) {
@@ -241,13 +242,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
   // G+/- (:560-587): u = S_d^-1 L^-T w, (alpha-beta) u / k = -S_d^-1 L w / k.
   // U = L^-T V column by column into sS (S is no longer needed), then every
   // entry (r, j) combines U with (L V)(r, j).
-  for (int j = tid; j < NN; j += blockDim.x) {
-    for (int r = NN - 1; r >= 0; --r) {
-      double s = sV[r * LD + j];
-      for (int m = r + 1; m < NN; ++m) s -= sL[m * LD + r] * sS[m * LD + j];
-      sS[r * LD + j] = s / sL[r * LD + r];
-    }
-  }
+  blk_ltsolve_cols<NN>(sL, LD, sV, sS);
   __syncthreads();
   for (int e = tid; e < NN * NN; e += blockDim.x) {
     const int r = e % NN, j = e / NN;
@@ -284,7 +279,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
       sA[r0 * LD + c0] = -(se - dg);
     }
     __syncthreads();
-    blk_cholesky<NN>(sA, LD, &flag);
+    blk_cholesky<NN, OW>(sA, LD, &flag);
   }
 
   // Solves (I + xba diag(mu) - C) z = b for z_up (+mu_r) and z_dn (-mu_r),
@@ -308,10 +303,10 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
         v1[tid] = tb[to.sd() + tid] * pr;
       }
       __syncthreads();
-      blk_lsolve<NN>(sL, LD, v0);
-      blk_ltsolve<NN>(sL, LD, v0);
-      blk_lsolve<NN>(sA, LD, v1);
-      blk_ltsolve<NN>(sA, LD, v1);
+      blk_lsolve<NN, OW>(sL, LD, v0);
+      blk_ltsolve<NN, OW>(sL, LD, v0);
+      blk_lsolve<NN, OW>(sA, LD, v1);
+      blk_ltsolve<NN, OW>(sA, LD, v1);
       if (tid < NN) {
         dr = -tb[to.sdi() + tid] * v0[tid];
         sr = -tb[to.sdi() + tid] * v1[tid];
@@ -320,9 +315,9 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
       if (tid < NN) v0[tid] = tb[to.sd() + tid] * qr;  // w
       __syncthreads();
       // (alpha+beta) q = -S_d^-1 L^-T V Lambda V^T L^-1 S_d q
-      blk_lsolve<NN>(sL, LD, v0);
-      blk_vdv<NN>(sV, LD, lam, v0, v1);
-      blk_ltsolve<NN>(sL, LD, v0);
+      blk_lsolve<NN, OW>(sL, LD, v0);
+      blk_vdv<NN, OW>(sV, LD, lam, v0, v1);
+      blk_ltsolve<NN, OW>(sL, LD, v0);
       if (tid < NN) {
         const double apbq = -tb[to.sdi() + tid] * v0[tid];
         const double rr = (pr + mub * apbq) / mub;
@@ -331,17 +326,17 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
       }
       __syncthreads();
       // d = S_d^-1 L^-T V (Lambda - xba^2)^-1 V^T L^T S_d r
-      blk_lmul<NN, true>(sL, LD, v2, v1);
-      blk_vdv<NN>(sV, LD, dv, v2, v1);
-      blk_ltsolve<NN>(sL, LD, v2);
+      blk_lmul<NN, true, OW>(sL, LD, v2, v1);
+      blk_vdv<NN, OW>(sV, LD, dv, v2, v1);
+      blk_ltsolve<NN, OW>(sL, LD, v2);
       if (tid < NN) {
         dr = tb[to.sdi() + tid] * v2[tid];
         v3[tid] = tb[to.sd() + tid] * dr;
       }
       __syncthreads();
       // (alpha-beta) d = -S_d^-1 L L^T S_d d
-      blk_lmul<NN, true>(sL, LD, v3, v1);
-      blk_lmul<NN, false>(sL, LD, v3, v1);
+      blk_lmul<NN, true, OW>(sL, LD, v3, v1);
+      blk_lmul<NN, false, OW>(sL, LD, v3, v1);
       if (tid < NN) {
         const double ambd = -tb[to.sdi() + tid] * v3[tid];
         sr = mub * (ambd - qr);
@@ -443,8 +438,8 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
     xr0 = pt - xr1 * taut;
     if (tid < NN) v0[tid] = tb[to.sd() + tid] * (2.0 / tb[to.cmu() + tid]);
     __syncthreads();
-    blk_lsolve<NN>(sA, LD, v0);
-    blk_ltsolve<NN>(sA, LD, v0);
+    blk_lsolve<NN, OW>(sA, LD, v0);
+    blk_ltsolve<NN, OW>(sA, LD, v0);
     const double c1 = (1.0 - ws) * xr1, c0 = (1.0 - ws) * xr0;
     double base = 0.0;
     if (tid < NN) {
@@ -453,8 +448,8 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
       v1[tid] = tb[to.sd() + tid] * z1h;
     }
     __syncthreads();
-    blk_lsolve<NN>(sL, LD, v1);
-    blk_ltsolve<NN>(sL, LD, v1);
+    blk_lsolve<NN, OW>(sL, LD, v1);
+    blk_ltsolve<NN, OW>(sL, LD, v1);
     if (tid < NN) {
       const double d0 = 2.0 * tb[to.sdi() + tid] * v1[tid];
       const double s0 = c0 * base;

@@ -178,6 +178,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   constexpr int N = C::N, LD = C::LD, NNE = NN + (NN & 1), NP = NNE / 2;
   constexpr TabOff to(N);
   constexpr GE ge(N);
+  constexpr bool OW = kGLT == 64;  // one wavefront per block
   // LDS holds the dense NN x NN work matrices and short vectors; the Y_l^m
   // table (shared by every task of the mode) is read through the caches, to
   // keep the footprint small enough for more wavefronts per CU.
@@ -320,7 +321,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
       xba *= 1.00000005;
     }
   }
-  if (!blk_cholesky<NN>(sL, LD, &flag) && tid == 0) atomicOr(&a.status[i], DISORT_ST_EIGEN);
+  if (!blk_cholesky<NN, OW>(sL, LD, &flag) && tid == 0) atomicOr(&a.status[i], DISORT_ST_EIGEN);
 
   for (int e = tid; e < NN * NN; e += blockDim.x) {
     const int r = e / NN, c = e % NN;
@@ -444,15 +445,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   __syncthreads();
 
   // G+/- (src/DisortSolver.cpp:1110-1138).
-  for (int j = tid; j < NN; j += blockDim.x) {
-#pragma unroll 1
-    for (int r = NN - 1; r >= 0; --r) {
-      double s = sV[r * LD + j];
-#pragma unroll 1
-      for (int mm = r + 1; mm < NN; ++mm) s -= sL[mm * LD + r] * sS[mm * LD + j];
-      sS[r * LD + j] = s / sL[r * LD + r];
-    }
-  }
+  blk_ltsolve_cols<NN>(sL, LD, sV, sS);  // U = L^-T V
   __syncthreads();
 #pragma unroll 1
   for (int e = tid; e < NN * NN; e += blockDim.x) {
@@ -486,7 +479,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
       sA[r0 * LD + c0] = -(qn[r] * qn[c] * se - dg);
     }
     __syncthreads();
-    blk_cholesky<NN>(sA, LD, &flag);
+    blk_cholesky<NN, OW>(sA, LD, &flag);
   }
 
   // (I + xba diag(mu) - C) z = b through the eigen-decomposition, as in the
@@ -505,10 +498,10 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
         v1[tid] = tb[to.sd() + tid] * pr;
       }
       __syncthreads();
-      blk_lsolve<NN>(sL, LD, v0);
-      blk_ltsolve<NN>(sL, LD, v0);
-      blk_lsolve<NN>(sA, LD, v1);
-      blk_ltsolve<NN>(sA, LD, v1);
+      blk_lsolve<NN, OW>(sL, LD, v0);
+      blk_ltsolve<NN, OW>(sL, LD, v0);
+      blk_lsolve<NN, OW>(sA, LD, v1);
+      blk_ltsolve<NN, OW>(sA, LD, v1);
       if (tid < NN) {
         dr = -tb[to.sdi() + tid] * v0[tid];
         sr = -tb[to.sdi() + tid] * v1[tid];
@@ -516,9 +509,9 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
     } else {
       if (tid < NN) v0[tid] = tb[to.sd() + tid] * qr;
       __syncthreads();
-      blk_lsolve<NN>(sL, LD, v0);
-      blk_vdv<NN>(sV, LD, lam, v0, v1);
-      blk_ltsolve<NN>(sL, LD, v0);
+      blk_lsolve<NN, OW>(sL, LD, v0);
+      blk_vdv<NN, OW>(sV, LD, lam, v0, v1);
+      blk_ltsolve<NN, OW>(sL, LD, v0);
       if (tid < NN) {
         const double apbq = -tb[to.sdi() + tid] * v0[tid];
         const double rr = (pr + mub * apbq) / mub;
@@ -526,16 +519,16 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
         dv[tid] = 1.0 / (lam[tid] - 1.0 / (mub * mub));
       }
       __syncthreads();
-      blk_lmul<NN, true>(sL, LD, v2, v1);
-      blk_vdv<NN>(sV, LD, dv, v2, v1);
-      blk_ltsolve<NN>(sL, LD, v2);
+      blk_lmul<NN, true, OW>(sL, LD, v2, v1);
+      blk_vdv<NN, OW>(sV, LD, dv, v2, v1);
+      blk_ltsolve<NN, OW>(sL, LD, v2);
       if (tid < NN) {
         dr = tb[to.sdi() + tid] * v2[tid];
         v3[tid] = tb[to.sd() + tid] * dr;
       }
       __syncthreads();
-      blk_lmul<NN, true>(sL, LD, v3, v1);
-      blk_lmul<NN, false>(sL, LD, v3, v1);
+      blk_lmul<NN, true, OW>(sL, LD, v3, v1);
+      blk_lmul<NN, false, OW>(sL, LD, v3, v1);
       if (tid < NN) {
         const double ambd = -tb[to.sdi() + tid] * v3[tid];
         sr = mub * (ambd - qr);
@@ -596,8 +589,8 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
     xr0 = pt - xr1 * taut;
     if (tid < NN) v0[tid] = tb[to.sd() + tid] * (2.0 / tb[to.cmu() + tid]);
     __syncthreads();
-    blk_lsolve<NN>(sA, LD, v0);
-    blk_ltsolve<NN>(sA, LD, v0);
+    blk_lsolve<NN, OW>(sA, LD, v0);
+    blk_ltsolve<NN, OW>(sA, LD, v0);
     const double c1 = (1.0 - ws) * xr1, c0 = (1.0 - ws) * xr0;
     double base = 0.0;
     if (tid < NN) {
@@ -606,8 +599,8 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
       v1[tid] = tb[to.sd() + tid] * z1h;
     }
     __syncthreads();
-    blk_lsolve<NN>(sL, LD, v1);
-    blk_ltsolve<NN>(sL, LD, v1);
+    blk_lsolve<NN, OW>(sL, LD, v1);
+    blk_ltsolve<NN, OW>(sL, LD, v1);
     if (tid < NN) {
       const double d0 = 2.0 * tb[to.sdi() + tid] * v1[tid];
       const double s0 = c0 * base;

@@ -69,42 +69,105 @@ __device__ __forceinline__ void panel_back_substitute(const double* panel, int64
 // ---------------------------------------------------------------------------
 // Block-cooperative vector kernels on LDS (all threads call them).
 // ---------------------------------------------------------------------------
+// With OW (a one-wavefront block) the serial chains run in registers: the
+// solved entry reaches the other lanes by readlane and the barriers become
+// wave-local ordering points; the operations and their order are those of
+// the block form.
+
 // x <- M^{-1} x, M lower triangular (pitch ld).  Entry r accumulates its
 // subtractions in ascending column order, then divides (as a row-oriented
 // forward substitution would).
-template <int NN>
+template <int NN, bool OW = false>
 __device__ void blk_lsolve(const double* M, int ld, double* x) {
   const int tid = threadIdx.x;
   double xr = (tid < NN) ? x[tid] : 0.0;
-  for (int r = 0; r < NN; ++r) {
-    if (tid == r) {
-      xr = xr / M[r * ld + r];
-      x[r] = xr;
+  if constexpr (OW) {
+    const int tr = tid < NN ? tid : 0;
+    static_for<0, NN>([&](auto ri) {
+      constexpr int r = decltype(ri)::value;
+      const double xs = lane_bcast(xr, r) / M[r * ld + r];
+      const double mr = M[tr * ld + r];
+      if (tid == r) xr = xs;
+      if (tid > r && tid < NN) xr -= mr * xs;
+    });
+    wave_lds_order();
+    if (tid < NN) x[tid] = xr;
+    wave_lds_order();
+  } else {
+    for (int r = 0; r < NN; ++r) {
+      if (tid == r) {
+        xr = xr / M[r * ld + r];
+        x[r] = xr;
+      }
+      __syncthreads();
+      if (tid > r && tid < NN) xr -= M[tid * ld + r] * x[r];
     }
     __syncthreads();
-    if (tid > r && tid < NN) xr -= M[tid * ld + r] * x[r];
   }
-  __syncthreads();
 }
 
 // x <- M^{-T} x, M lower triangular.
-template <int NN>
+template <int NN, bool OW = false>
 __device__ void blk_ltsolve(const double* M, int ld, double* x) {
   const int tid = threadIdx.x;
   double xr = (tid < NN) ? x[tid] : 0.0;
-  for (int r = NN - 1; r >= 0; --r) {
-    if (tid == r) {
-      xr = xr / M[r * ld + r];
-      x[r] = xr;
+  if constexpr (OW) {
+    const int tr = tid < NN ? tid : 0;
+    static_for<0, NN>([&](auto ri) {
+      constexpr int r = NN - 1 - decltype(ri)::value;
+      const double xs = lane_bcast(xr, r) / M[r * ld + r];
+      const double mr = M[r * ld + tr];
+      if (tid == r) xr = xs;
+      if (tid < r) xr -= mr * xs;
+    });
+    wave_lds_order();
+    if (tid < NN) x[tid] = xr;
+    wave_lds_order();
+  } else {
+    for (int r = NN - 1; r >= 0; --r) {
+      if (tid == r) {
+        xr = xr / M[r * ld + r];
+        x[r] = xr;
+      }
+      __syncthreads();
+      if (tid < r) xr -= M[r * ld + tid] * x[r];
     }
     __syncthreads();
-    if (tid < r) xr -= M[r * ld + tid] * x[r];
   }
-  __syncthreads();
+}
+
+// U = M^{-T} V column by column, M lower triangular (pitch ld), thread j < NN
+// holding column j in registers: U(r,j) = (V(r,j) - sum_{m>r} M(m,r) U(m,j))
+// / M(r,r), the subtractions in ascending m.
+template <int NN>
+__device__ void blk_ltsolve_cols(const double* M, int ld, const double* V, double* U) {
+  const int j = threadIdx.x;
+  if (j < NN) {
+    double u[NN];
+    static_for<0, NN>([&](auto ri) {
+      constexpr int r = NN - 1 - decltype(ri)::value;
+      double s = V[r * ld + j];
+      static_for<r + 1, NN>([&](auto mi) {
+        constexpr int m = decltype(mi)::value;
+        s -= M[m * ld + r] * u[m];
+      });
+      u[r] = s / M[r * ld + r];
+    });
+#pragma unroll
+    for (int r = 0; r < NN; ++r) U[r * ld + j] = u[r];
+  }
+}
+
+// Ordering point of the block: a barrier, or with one wavefront only the
+// compiler's (its LDS accesses execute in program order).
+template <bool OW>
+__device__ __forceinline__ void blk_sync() {
+  if constexpr (OW) wave_lds_order();
+  else __syncthreads();
 }
 
 // x <- M x (lower) or M^T x, through scratch y.
-template <int NN, bool TRANS>
+template <int NN, bool TRANS, bool OW = false>
 __device__ void blk_lmul(const double* M, int ld, double* x, double* y) {
   const int tid = threadIdx.x;
   if (tid < NN) {
@@ -115,13 +178,13 @@ __device__ void blk_lmul(const double* M, int ld, double* x, double* y) {
       for (int m = tid; m < NN; ++m) s += M[m * ld + tid] * x[m];
     y[tid] = s;
   }
-  __syncthreads();
+  blk_sync<OW>();
   if (tid < NN) x[tid] = y[tid];
-  __syncthreads();
+  blk_sync<OW>();
 }
 
 // x <- V diag(d) V^T x (V column eigenvectors, pitch ld), scratch y.
-template <int NN>
+template <int NN, bool OW = false>
 __device__ void blk_vdv(const double* V, int ld, const double* d, double* x, double* y) {
   const int tid = threadIdx.x;
   if (tid < NN) {
@@ -129,40 +192,59 @@ __device__ void blk_vdv(const double* V, int ld, const double* d, double* x, dou
     for (int r = 0; r < NN; ++r) s += V[r * ld + tid] * x[r];
     y[tid] = s * d[tid];
   }
-  __syncthreads();
+  blk_sync<OW>();
   if (tid < NN) {
     double s = 0.0;
     for (int j = 0; j < NN; ++j) s += V[tid * ld + j] * y[j];
     x[tid] = s;
   }
-  __syncthreads();
+  blk_sync<OW>();
 }
 
 // In-place Cholesky of the lower triangle of M (left-looking, column by
 // column, the operation order of the register kernel).  Returns 0 when a
-// pivot is not positive (the factor then uses 1 in its place).
-template <int NN>
+// pivot is not positive (the factor then uses 1 in its place).  With one
+// wavefront the lane of row c forms the diagonal in the same pass as the
+// rows below it.
+template <int NN, bool OW = false>
 __device__ int blk_cholesky(double* M, int ld, int* flag) {
   const int tid = threadIdx.x;
-  if (tid == 0) *flag = 1;
-  __syncthreads();
-  for (int c = 0; c < NN; ++c) {
-    if (tid == 0) {
-      double x = M[c * ld + c];
-      for (int k = 0; k < c; ++k) x -= M[c * ld + k] * M[c * ld + k];
-      if (!(x > 0.0)) *flag = 0;
-      M[c * ld + c] = sqrt(x > 0.0 ? x : 1.0);
-    }
-    __syncthreads();
-    const double lcc = M[c * ld + c];
-    for (int r = c + 1 + tid; r < NN; r += blockDim.x) {
+  if constexpr (OW) {
+    bool ok = true;
+    for (int c = 0; c < NN; ++c) {
+      const int r = tid >= c && tid < NN ? tid : c;
       double s = M[r * ld + c];
       for (int k = 0; k < c; ++k) s -= M[r * ld + k] * M[c * ld + k];
-      M[r * ld + c] = s / lcc;
+      const double x = lane_bcast(s, c);
+      ok = ok && (x > 0.0);
+      const double lcc = sqrt(x > 0.0 ? x : 1.0);
+      wave_lds_order();
+      if (tid == c) M[c * ld + c] = lcc;
+      else if (tid > c && tid < NN) M[r * ld + c] = s / lcc;
+      wave_lds_order();
     }
+    return ok ? 1 : 0;
+  } else {
+    if (tid == 0) *flag = 1;
     __syncthreads();
+    for (int c = 0; c < NN; ++c) {
+      if (tid == 0) {
+        double x = M[c * ld + c];
+        for (int k = 0; k < c; ++k) x -= M[c * ld + k] * M[c * ld + k];
+        if (!(x > 0.0)) *flag = 0;
+        M[c * ld + c] = sqrt(x > 0.0 ? x : 1.0);
+      }
+      __syncthreads();
+      const double lcc = M[c * ld + c];
+      for (int r = c + 1 + tid; r < NN; r += blockDim.x) {
+        double s = M[r * ld + c];
+        for (int k = 0; k < c; ++k) s -= M[r * ld + k] * M[c * ld + k];
+        M[r * ld + c] = s / lcc;
+      }
+      __syncthreads();
+    }
+    return *flag;
   }
-  return *flag;
 }
 
 // Block-wide sum of one double per thread.

@@ -1342,41 +1342,56 @@ __global__ __launch_bounds__(64) void k_gflux(GenArgs a) {
 // ===========================================================================
 enum { UC_E1 = 0, UC_E2 = 1, UC_E0 = 2, UC_BND = 3, kUCoef = 4 };
 
+// Tasks per 64-thread block: lanes (task, user angle), so that short user
+// angle lists still fill the wavefront.
+constexpr int kUcMaxT = 8;
+inline int ucoef_tasks_per_block(int numu) {
+  const int t = 64 / (numu > 0 ? numu : 1);
+  return t < 1 ? 1 : (t > kUcMaxT ? kUcMaxT : t);
+}
+
 template <int NN>
-__global__ __launch_bounds__(64) void k_gucoef(GenArgs a, double* ucoef) {
+__global__ __launch_bounds__(64) void k_gucoef(GenArgs a, double* ucoef, int tpb) {
   constexpr int N = 2 * NN;
   constexpr TabOff to(N);
   constexpr GE ge(N);
-  const int64_t t = blockIdx.x;
-  if (t >= a.n_tasks) return;
+  __shared__ double wk_s[kUcMaxT * NN];
   const int L = a.nlyr, numu = a.numu;
+  // Lane (sub-task st, first user angle iu0); with numu > 64 one task per
+  // block and each lane steps through the angles by 64.
+  const int span = tpb > 1 ? numu : 64;
+  const int st = (int)threadIdx.x / span, iu0 = (int)threadIdx.x % span;
+  const int64_t t = (int64_t)blockIdx.x * tpb + (st < tpb ? st : 0);
+  const bool live = st < tpb && t < a.n_tasks;
   const int lc = (int)(t % L);
   const int64_t cm = t / L;
   const int m = (int)(cm % a.nm);
   const int64_t cl = cm / a.nm;
   const int64_t i = a.c0 + cl;
   const int64_t lo = i * L, vo = i * (L + 1), TL = a.n * L, TV = a.n * (L + 1);
-  const int ncut = a.inst_i[i * kInstI + II_NCUT];
-  const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
+  const int ncut = live ? a.inst_i[i * kInstI + II_NCUT] : 0;
+  const bool lyrcut = live && a.inst_i[i * kInstI + II_LYRCUT] != 0;
   // No record below the cut-off (k_glayer skipped the layer); k_guser reads
   // none of those tasks' coefficients.
-  if (lc >= ncut) return;
+  const bool run = live && lc < ncut;
   const double* sc = a.scal + i * DISORT_NSCAL;
-  const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
   const double* E = a.eig + t * ge.size();
+  double* const wk = wk_s + (st < kUcMaxT ? st : 0) * NN;
+  const double dt = run ? a.lay_d[LD_DTAU_S * TL + lo + lc] : 0.0;
+  for (int q = iu0; q < NN; q += span)
+    if (run) wk[q] = exp(-E[ge.k() + NN - 1 - q] * dt);  // exp(kk_q dt), kk_q < 0
+  __syncthreads();
+  if (!run) return;
+  const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
   const double* LL = a.cvec + t * N;
   double* gu = a.gu + t * (int64_t)numu * N;
   const double* us = a.usrc + t * (int64_t)kUSrc * numu;
   double* uc = ucoef + t * (int64_t)kUCoef * numu;
   const bool thermal = (a.flags & DISORT_F_THERMAL) != 0 && m == 0;
   const double d0 = (m == 0) ? 1.0 : 0.0;
-  const double dt = a.lay_d[LD_DTAU_S * TL + lo + lc];
   const double ttop = a.lev_d[VD_TAU_S * TV + vo + lc], tbot = a.lev_d[VD_TAU_S * TV + vo + lc + 1];
   const double btt = a.lay_d[LD_BT_TOP * TL + lo + lc], btb = a.lay_d[LD_BT_BOT * TL + lo + lc];
-  __shared__ double wk[NN];
-  for (int q = threadIdx.x; q < NN; q += blockDim.x) wk[q] = exp(-E[ge.k() + NN - 1 - q] * dt);  // exp(kk_q dt), kk_q < 0
-  __syncthreads();
-  for (int iu = threadIdx.x; iu < numu; iu += blockDim.x) {
+  for (int iu = iu0; iu < numu; iu += span) {
     double* g = gu + (int64_t)iu * N;
     for (int q = 0; q < N; ++q) g[q] *= LL[q];
     const double mu = a.mu_user[iu];
@@ -1826,7 +1841,8 @@ void general_chunk_n(const GenArgs& a, const GenLaunch& g, hipStream_t s) {
   hipLaunchKernelGGL(k_gflux<NN>, dim3((unsigned)((a.nc * a.ntau + 63) / 64)), dim3(64), 0, s, a);
   rec(4);
   if (g.do_user) {
-    hipLaunchKernelGGL(k_gucoef<NN>, dim3((unsigned)a.n_tasks), dim3(64), 0, s, a, g.ucoef);
+    const int tpb = ucoef_tasks_per_block(a.numu);
+    hipLaunchKernelGGL(k_gucoef<NN>, dim3((unsigned)((a.n_tasks + tpb - 1) / tpb)), dim3(64), 0, s, a, g.ucoef, tpb);
     rec(5);
     hipLaunchKernelGGL(k_guser<NN>, dim3(ncm), dim3(256), 0, s, a, (const double*)g.ucoef);
     rec(6);

@@ -159,6 +159,14 @@ def test_layer_cut_off(ctx, nstr, extra):
     solve_both(ctx, gb, f"absorption cut-off nstr={nstr}")
 
 
+@pytest.mark.parametrize("numu", [3, 20, 40, 70])
+def test_user_angle_counts(ctx, numu):
+    # Up to eight (column, mode, layer) tasks share a wavefront of the
+    # user-angle coefficient kernel, or one task steps through 64-angle strides.
+    gb = random_general(2, 5, 8, seed=95 + numu, numu=numu, nphi=3)
+    solve_both(ctx, gb, f"numu={gb.numu}")
+
+
 def test_delta_m_plus(ctx):
     gb = random_general(3, 6, 16, seed=90, flags=4, P=19, corr=True)
     solve_both(ctx, gb, "Delta-M+")

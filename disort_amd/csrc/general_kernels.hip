@@ -1571,8 +1571,11 @@ __global__ __launch_bounds__(256) void k_guser(GenArgs a, const double* ucoef) {
 // accuracy_fourier_series early exit (src/DisortSolver.cpp:336-346,
 // :3228-3266), and the azimuthal average; block per chunk column.
 // ===========================================================================
+constexpr int kAccPhi = 64;
+
 __global__ __launch_bounds__(256) void k_gaccum(GenArgs a, int32_t* modes_used) {
   __shared__ double red[4];
+  __shared__ double cph[kAccPhi];
   const int64_t cl = blockIdx.x;
   if (cl >= a.nc) return;
   const int64_t i = a.c0 + cl;
@@ -1594,20 +1597,35 @@ __global__ __launch_bounds__(256) void k_gaccum(GenArgs a, int32_t* modes_used) 
   for (int m = 1; m <= naz; ++m) {
     used = m + 1;
     const double* um = u + m * per;
-    double mx = 0.0;
-    for (int64_t e = threadIdx.x; e < per * nphi; e += blockDim.x) {
-      const int64_t lu_iu = e / nphi;
-      const int j = (int)(e % nphi);
+    // cos(m (phi_j - phi0)) once per mode and azimuth (the first kAccPhi in
+    // LDS; beyond them evaluated in place).
+    for (int j = threadIdx.x; j < nphi && j < kAccPhi; j += blockDim.x) {
       const double phirad = (a.phi_user[j] - phi0) * kPi / 180.0;
-      const double az = um[lu_iu] * cos((double)m * phirad);
-      const double v = I[e] + az;
-      I[e] = v;
-      const double ua = fabs(v);
-      double r;
-      if (ua == 0.0) r = 1.0 + fabs(az);
-      else if (fabs(az) == 0.0) r = 0.0;
-      else r = fabs(az) / ua;
-      mx = r > mx ? r : mx;
+      cph[j] = cos((double)m * phirad);
+    }
+    __syncthreads();
+    double mx = 0.0;
+    for (int64_t q = threadIdx.x; q < per; q += blockDim.x) {
+      const double uq = um[q];
+      double* Iq = I + q * nphi;
+      for (int j = 0; j < nphi; ++j) {
+        double cj;
+        if (j < kAccPhi) {
+          cj = cph[j];
+        } else {
+          const double phirad = (a.phi_user[j] - phi0) * kPi / 180.0;
+          cj = cos((double)m * phirad);
+        }
+        const double az = uq * cj;
+        const double v = Iq[j] + az;
+        Iq[j] = v;
+        const double ua = fabs(v);
+        double r;
+        if (ua == 0.0) r = 1.0 + fabs(az);
+        else if (fabs(az) == 0.0) r = 0.0;
+        else r = fabs(az) / ua;
+        mx = r > mx ? r : mx;
+      }
     }
     if (a.accuracy > 0.0) {
       // Block maximum (values are >= 0).

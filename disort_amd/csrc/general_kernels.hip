@@ -145,23 +145,50 @@ __global__ __launch_bounds__(256) void k_gsurf_d(GenArgs a) {
   }
 }
 
-// Beam column of the BRDF coefficients, block per (chunk column, mode).
+// Beam column of the BRDF coefficients of every mode, block per chunk
+// column: each reflectivity value rho(mu, mu0, phi_k) is evaluated once and
+// enters all the modes' quadrature sums (in the order of rpv_fourier, with
+// the same cos(m pi x_k) table), rather than once per mode.
 template <int NN>
 __global__ __launch_bounds__(64) void k_gsurf_b(GenArgs a) {
-  const int64_t cl = blockIdx.x / a.nm;
-  const int m = blockIdx.x % a.nm;
+  constexpr int N = 2 * NN, K = kNMUG / 2;
+  __shared__ double cmk[N * K];  // cos(m pi x_k), m < N (1 at m = 0)
+  const int64_t cl = blockIdx.x;
   if (cl >= a.nc) return;
+  const int nm = a.nm;
   const int64_t i = a.c0 + cl;
   const double mu0 = a.scal[i * DISORT_NSCAL + DISORT_S_BEAM_MU];
   const bool beam = a.scal[i * DISORT_NSCAL + DISORT_S_BEAM_FLUX] > 0.0;
   const double* p = a.set_rpv + (int64_t)a.rpv_set[i] * DISORT_NRPV;
   const bool do_user = (a.flags & DISORT_G_USER_MU) && !(a.flags & DISORT_G_ONLY_FLUXES);
-  double* b = a.srf_b + (cl * a.nm + m) * surf_b_size(NN, a.numu);
+  const double* gmu = a.gaz;
+  const double* gwt = a.gaz + kNMUG;
+  for (int e = threadIdx.x; e < nm * K; e += blockDim.x) {
+    const int m = e / K, k = e % K;
+    cmk[e] = (m == 0) ? 1.0 : cos((double)m * kPi * gmu[k]);
+  }
+  __syncthreads();
+  const int64_t bs = surf_b_size(NN, a.numu);
+  double* b0 = a.srf_b + cl * nm * bs;
   for (int e = threadIdx.x; e < NN + a.numu; e += blockDim.x) {
     double mu = 0.0;
     if (e < NN) mu = a.tab[e];
     else if (do_user) mu = a.mu_user[e - NN];
-    b[e] = (beam && mu > 0.0) ? rpv_fourier(a, p, m, mu, mu0) : 0.0;
+    const bool act = beam && mu > 0.0;
+    double sm[N];
+#pragma unroll
+    for (int m = 0; m < N; ++m) sm[m] = 0.0;
+    if (act) {
+      for (int k = 0; k < K; ++k) {
+        const double x = gwt[k] * rpv_rho(p, mu, mu0, kPi * gmu[k]);
+#pragma unroll
+        for (int m = 0; m < N; ++m)
+          if (m < nm) sm[m] += x * cmk[m * K + k];
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < N; ++m)
+      if (m < nm) b0[m * bs + e] = act ? (m == 0 ? 1.0 : 2.0) * sm[m] : 0.0;
   }
 }
 
@@ -1849,7 +1876,7 @@ void general_chunk_n(const GenArgs& a, const GenLaunch& g, hipStream_t s) {
     if (g.ev) (void)hipEventRecord(g.ev[k], s);
   };
   rec(0);
-  if (g.brdf) hipLaunchKernelGGL(k_gsurf_b<NN>, dim3(ncm), dim3(64), 0, s, a);
+  if (g.brdf) hipLaunchKernelGGL(k_gsurf_b<NN>, dim3((unsigned)a.nc), dim3(64), 0, s, a);
   rec(1);
   hipLaunchKernelGGL(k_glayer<NN>, dim3((unsigned)a.n_tasks), dim3(kGLT), 0, s, a);
   rec(2);

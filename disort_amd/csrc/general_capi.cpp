@@ -57,8 +57,8 @@ void legendre_modes(const double* mu, int nmu, int N, std::vector<double>& out) 
   }
 }
 
-// [ylm | ylq] for N streams: Y_l^m at the positive nodes and scaled by
-// sqrt(w/mu), cached per stream count.
+// [ylm | ylq | ylw] for N streams: Y_l^m at the positive nodes, scaled by
+// sqrt(w/mu), and times the quadrature weight w, cached per stream count.
 int mode_tables(disort_ctx* ctx, int N, double** out) {
   auto it = ctx->mode_tables.find(N);
   if (it != ctx->mode_tables.end()) {
@@ -70,11 +70,12 @@ int mode_tables(disort_ctx* ctx, int N, double** out) {
   if (!gauss_nodes(NN, cmu, cwt)) return fail(ctx, DISORT_E_NUMERIC, "gaussianQuadrature: Failed to converge");
   std::vector<double> y;
   legendre_modes(cmu.data(), NN, N, y);
-  std::vector<double> t(2 * y.size());
+  std::vector<double> t(3 * y.size());
   for (size_t e = 0; e < y.size(); ++e) {
     const int r = (int)(e % NN);
     t[e] = y[e];
     t[y.size() + e] = y[e] * std::sqrt(cwt[r] / cmu[r]);
+    t[2 * y.size() + e] = cwt[r] * y[e];
   }
   double* d = nullptr;
   HIPCHK(ctx, hipMalloc(&d, t.size() * sizeof(double)));
@@ -360,6 +361,7 @@ extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch*
   a.tab = tab;
   a.ylm = mt;
   a.ylq = mt + (size_t)N * N * NN;
+  a.ylw = mt + 2 * (size_t)N * N * NN;
   std::vector<double> ylu_h;
   if (do_user) legendre_modes(mu_h.data(), numu, N, ylu_h);
   std::vector<double> gaz_h(2 * 200, 0.0);

@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   double* const sL = sbuf + NN * LD;
   double* const sV = sbuf + 2 * NN * LD;
   __shared__ double chi[N], p0[N];
-  __shared__ double qn[NN], cw[NN];  // sqrt(w_r / mu_r), w_r
+  __shared__ double qn[NN];  // sqrt(w_r / mu_r)
   __shared__ double v0[NN], v1[NN], v2[NN], v3[NN], lam[NN], dv[NN];
   __shared__ double zr_u[NN], zr_d[NN], z0_u[NN], z0_d[NN], z1_h[NN];
   __shared__ double psi[3 * N];
@@ -268,7 +268,6 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   }
   for (int r = tid; r < NN; r += blockDim.x) {
     qn[r] = tb[to.sd() + r] / tb[to.cmu() + r];
-    cw[r] = tb[to.cwt() + r];
   }
   // Y_l^m(-mu0) by the recurrence of src/Legendre.cpp:67-120 (the diagonal
   // term as the product the reference builds over successive modes).
@@ -687,6 +686,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   __syncthreads();
   const int numu = a.numu;
   const double* Yu = a.ylu + (int64_t)m * N * numu;  // [l][iu]
+  const double* Pw = a.ylw + (int64_t)m * N * NN;    // [l][r]
   // psi for the beam and thermal sources.
   for (int e = tid; e < 3 * N; e += blockDim.x) {
     const int k = e / N, l = e % N;
@@ -695,7 +695,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
       const double sgn = ((l - m) & 1) ? -1.0 : 1.0;
 #pragma unroll 2
       for (int r = 0; r < NN; ++r) {
-        const double w = cw[r] * Pm[l * NN + r];
+        const double w = Pw[l * NN + r];  // w_r Y_l^m(mu_r)
         const double xu = (k == 0) ? zr_u[r] : (k == 1 ? z0_u[r] : z1_h[r]);
         const double xd = (k == 0) ? zr_d[r] : (k == 1 ? z0_d[r] : z1_h[r]);
         s += w * (xu + sgn * xd);
@@ -705,16 +705,20 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   }
   // W(l, col) = chi_l / 2 sum_jq w_jq Y_l^m(mu_jq) G(row(jq), col), then
   // G_u = Y_u W, CW columns at a time (W[l][col - cb] in the first LDS
-  // slot); G+ and G- of the task record staged in the other two (column j'
-  // contiguous), so the inner sums read only LDS and the cached Y tables.
+  // slot).  With j' the column's eigenvector and the parity of l - m, the
+  // entry G(row, col) is +-(G+ + G-) or +-(G+ - G-) (exactly: a sign flip
+  // and a commuted sum); both combinations are staged in the other two
+  // slots (column j' contiguous), so the inner sums read LDS and the cached
+  // weighted Y table only.
   constexpr int CW = (NN + 1) / 2;  // N * CW <= NN * LD
   double* const W = sbuf;
-  double* const sGp = sbuf + NN * LD;
-  double* const sGm = sbuf + 2 * NN * LD;
+  double* const sGs = sbuf + NN * LD;
+  double* const sGd = sbuf + 2 * NN * LD;
   for (int e = tid; e < NN * NN; e += blockDim.x) {
     const int j = e / NN, r = e % NN;
-    sGp[j * LD + r] = E[ge.gp() + e];
-    sGm[j * LD + r] = E[ge.gm() + e];
+    const double gp = E[ge.gp() + e], gmv = E[ge.gm() + e];
+    sGs[j * LD + r] = gp + gmv;
+    sGd[j * LD + r] = gp - gmv;
   }
   __syncthreads();
   double* gu = a.gu + t * (int64_t)numu * N;
@@ -725,17 +729,15 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
       const int l = e / CW, col = cb + e % CW;
       double s = 0.0;
       if (l >= m && col < N) {
-        const double sgn = ((l - m) & 1) ? -1.0 : 1.0;
+        // up: G+ + sgn G-; down: -(G- + sgn G+), sgn = (-1)^(l-m).
+        const bool even = ((l - m) & 1) == 0;
         const bool up = col >= NN;
         const int jp = up ? col - NN : NN - 1 - col;
-        const double* gpc = sGp + jp * LD;
-        const double* gmc = sGm + jp * LD;
+        const double* gc = (even ? sGs : sGd) + jp * LD;
+        const double sg = (!up && even) ? -1.0 : 1.0;
+        const double* pw = Pw + l * NN;
 #pragma unroll 4
-        for (int r = 0; r < NN; ++r) {
-          const double w = cw[r] * Pm[l * NN + r];
-          const double gp = gpc[r], gmv = gmc[r];
-          s += w * (up ? gp + sgn * gmv : -(gmv + sgn * gp));
-        }
+        for (int r = 0; r < NN; ++r) s += pw[r] * (sg * gc[r]);
       }
       W[e] = s * (0.5 * chi[l]);
     }

@@ -76,6 +76,7 @@ struct GenArgs {
   const double* tab;  // TabOff(N)
   const double* ylm;  // [N][N][NN]: Y_l^m at the positive nodes, [m][l][r]
   const double* ylq;  // [N][N][NN]: Y_l^m(mu_r) sqrt(w_r / mu_r)
+  const double* ylw;  // [N][N][NN]: w_r Y_l^m(mu_r)
   const double* ylu;  // [N][N][numu]: Y_l^m at the user angles
   const double* gaz;  // [2][NMUG]: BRDF azimuth nodes and weights
   // Surface (BRDF) tables.

@@ -192,6 +192,14 @@ __global__ __launch_bounds__(64) void k_gsurf_b(GenArgs a) {
   }
 }
 
+// Row and column (r >= c) of entry e of a packed lower triangle.
+__device__ __forceinline__ void tri_rc(int e, int& r, int& c) {
+  r = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
+  if ((r + 1) * (r + 2) / 2 <= e) ++r;
+  if (r * (r + 1) / 2 > e) --r;
+  c = e - r * (r + 1) / 2;
+}
+
 // ===========================================================================
 // k_glayer: one workgroup per (column, mode, layer) task.
 // ===========================================================================
@@ -317,10 +325,12 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
   // q_r = sqrt(w_r/mu_r), A_s = q_r q_c e - 1/mu_r delta, -B_s = 1/mu_r delta
   // - q_r q_c o.  The phase matrix C has the blocks w_c (e +/- o) / 2, from
   // which the Frobenius norm of I + xba diag(mu) - C is accumulated.
+  // One lane per entry r >= c of the lower triangle, stored at (r, c) and
+  // (c, r); the norm takes both entries (column weights w_c and w_r).
   double fro = 0.0;
-  for (int e = tid; e < NN * NN; e += blockDim.x) {
-    const int r0 = e / NN, c0 = e % NN;
-    const int r = r0 >= c0 ? r0 : c0, c = r0 >= c0 ? c0 : r0;
+  for (int e = tid; e < NN * (NN + 1) / 2; e += blockDim.x) {
+    int r, c;
+    tri_rc(e, r, c);
     double se = 0.0, so = 0.0;
     for (int l = m; l < N; ++l) {
       const double q = chi[l] * Pm[l * NN + r] * Pm[l * NN + c];
@@ -329,14 +339,20 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
     }
     const double qq = qn[r] * qn[c];
     const double dg = (r == c) ? 1.0 / tb[to.cmu() + r] : 0.0;
-    sA[r0 * LD + c0] = qq * se - dg;
-    sL[r0 * LD + c0] = dg - qq * so;
+    const double av = qq * se - dg, bv = dg - qq * so;
+    sA[r * LD + c] = av;
+    sA[c * LD + r] = av;
+    sL[r * LD + c] = bv;
+    sL[c * LD + r] = bv;
     if (spherical) {
-      const double hw = 0.5 * tb[to.cwt() + c0];
-      const double tl = hw * (se + so), tr = hw * (se - so);
-      const double du = (r0 == c0) ? 1.0 + xba * tb[to.cmu() + r0] : 0.0;
-      const double dd = (r0 == c0) ? 1.0 - xba * tb[to.cmu() + r0] : 0.0;
-      fro += (du - tl) * (du - tl) + (dd - tl) * (dd - tl) + 2.0 * tr * tr;
+      for (int side = 0; side < (r == c ? 1 : 2); ++side) {
+        const int c0 = side ? r : c;
+        const double hw = 0.5 * tb[to.cwt() + c0];
+        const double tl = hw * (se + so), tr = hw * (se - so);
+        const double du = (r == c) ? 1.0 + xba * tb[to.cmu() + r] : 0.0;
+        const double dd = (r == c) ? 1.0 - xba * tb[to.cmu() + r] : 0.0;
+        fro += (du - tl) * (du - tl) + (dd - tl) * (dd - tl) + 2.0 * tr * tr;
+      }
     }
   }
   __syncthreads();
@@ -493,16 +509,15 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
     // -A_s again (the slot held S and U meanwhile), entry by entry as built
     // above, then its Cholesky factor.
     __syncthreads();
-    for (int e = tid; e < NN * NN; e += blockDim.x) {
-      const int r0 = e / NN, c0 = e % NN;
-      const int r = r0 >= c0 ? r0 : c0, c = r0 >= c0 ? c0 : r0;
+    for (int e = tid; e < NN * (NN + 1) / 2; e += blockDim.x) {
+      int r, c;
+      tri_rc(e, r, c);
       double se = 0.0;
-      for (int l = m; l < N; ++l) {
-        const double q = chi[l] * Pm[l * NN + r] * Pm[l * NN + c];
-        if (((l - m) & 1) == 0) se += q;
-      }
+      for (int l = m; l < N; l += 2) se += chi[l] * Pm[l * NN + r] * Pm[l * NN + c];
       const double dg = (r == c) ? 1.0 / tb[to.cmu() + r] : 0.0;
-      sA[r0 * LD + c0] = -(qn[r] * qn[c] * se - dg);
+      const double v = -(qn[r] * qn[c] * se - dg);
+      sA[r * LD + c] = v;
+      sA[c * LD + r] = v;
     }
     __syncthreads();
     blk_cholesky<NN, OW>(sA, LD, &flag);

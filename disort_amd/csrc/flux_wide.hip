@@ -145,8 +145,8 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
       if (r > c) off += x * x;
       if (r == c) dia += x * x;
     }
-    off = blk_sum(off, red);
-    dia = blk_sum(dia, red);
+    off = blk_sum<OW>(off, red);
+    dia = blk_sum<OW>(dia, red);
     if (off == 0.0 || off <= 1e-36 * dia) break;
     for (int round = 0; round < NNE - 1; ++round) {
       if (tid < NP) {
@@ -178,7 +178,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
           rt[k] = 0.0;
         }
       }
-      __syncthreads();
+      blk_sync<OW>();
       // S <- J^T S J by 2x2 blocks (pair k rows, pair k2 columns), k >= k2;
       // V <- V J by column pairs.  Indices >= NN (the dummy) are absent.
       for (int e = tid; e < NP * NP; e += blockDim.x) {
@@ -230,7 +230,7 @@ __global__ __launch_bounds__(WideCfg<NN>::LT, WideCfg<NN>::LW) void k_layer_wg(F
         sV[m * LD + p] = rc[k] * vp - rs[k] * vq;
         sV[m * LD + q] = rs[k] * vp + rc[k] * vq;
       }
-      __syncthreads();
+      blk_sync<OW>();
     }
   }
   if (tid < NN) {

@@ -394,8 +394,8 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
       if (r > c) off += x * x;
       if (r == c) dia += x * x;
     }
-    off = blk_sum(off, red);
-    dia = blk_sum(dia, red);
+    off = blk_sum<OW>(off, red);
+    dia = blk_sum<OW>(dia, red);
     if (off == 0.0 || off <= 1e-36 * dia) break;
     for (int round = 0; round < NNE - 1; ++round) {
       if (tid < NP) {
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
           rt[k] = 0.0;
         }
       }
-      __syncthreads();
+      blk_sync<OW>();
       for (int e = tid; e < NP * NP; e += blockDim.x) {
         const int k = e / NP, k2 = e % NP;
         if (k2 > k) continue;
@@ -477,7 +477,7 @@ __global__ __launch_bounds__(kGLT) void k_glayer(GenArgs a) {
         sV[mm * LD + p] = rc[k] * vp - rs[k] * vq;
         sV[mm * LD + q] = rs[k] * vp + rc[k] * vq;
       }
-      __syncthreads();
+      blk_sync<OW>();
     }
   }
   if (tid < NN) {

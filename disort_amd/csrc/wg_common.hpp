@@ -247,9 +247,12 @@ __device__ int blk_cholesky(double* M, int ld, int* flag) {
   }
 }
 
-// Block-wide sum of one double per thread.
+// Block-wide sum of one double per thread (every lane of a wavefront ends
+// with the same butterfly sum; with one wavefront that is the result).
+template <bool OW = false>
 __device__ double blk_sum(double v, double* red) {
   for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  if constexpr (OW) return v;
   const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[w] = v;

@@ -1,7 +1,7 @@
 #!/usr/bin/env python3
 """Headline benchmark: batched FP64 flux-only DISORT on one MI355X.
 
-Workload (BASELINE config 2): 10,000 wavenumber bands of 1 cm^-1 over
+Workload: 10,000 wavenumber bands of 1 cm^-1 over
 100-10,100 cm^-1 x one column, nlyr=60, nstr=16, thermal emission, seeded
 synthetic optical properties (disort_amd.workloads.c2).  A step is one solve
 of all 10,000 instances with inputs already resident in HBM
@@ -401,7 +401,7 @@ def main() -> int:
     if args.gpus != 1:
         print(
             "bench.py measures one GPU: this build's instances are independent and shard "
-            "across GPUs without communication (see DESIGN.md); run with --gpus 1.",
+            "across GPUs without communication; run with --gpus 1.",
             file=sys.stderr,
         )
         return 2

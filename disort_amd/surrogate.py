@@ -16,7 +16,7 @@ batched FP64 solver generates, and served behind a gate:
     the columns the gate accepts, at most `alpha` exceed the net-flux error
     bound; at serving time the sampled columns measure the error online.
 
-Plain DISORT stays the fallback and the judge: every error quoted is against
+Plain DISORT stays the fallback and the reference: every error quoted is against
 the FP64 GPU solver, which is itself checked against the CPU oracle.
 
 Workload (C5): 60 layers x (log10 delta_tau, single_scat_albedo, g,

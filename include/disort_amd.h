@@ -306,7 +306,7 @@ int disort_general_solve(disort_ctx* ctx, const disort_general_batch* batch, int
  * perceptron with BF16 weights and activations and FP32 accumulation on the
  * gfx950 matrix cores, trained by the library's own forward, backward and
  * Adam kernels on columns the batched solver generates; plain DISORT stays
- * the fallback and the judge of its error (disort_amd/surrogate.py holds
+ * the fallback and the measure of its error (disort_amd/surrogate.py holds
  * the training loop and the gate).  All array pointers are device pointers.
  * ========================================================================= */
 typedef struct disort_mlp disort_mlp;

@@ -400,7 +400,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            "bench.py measures one GPU: this build's instances are independent and shard "
+            "bench.py measures one GPU: DISORT instances are independent and shard "
             "across GPUs without communication; run with --gpus 1.",
             file=sys.stderr,
         )

@@ -1,7 +1,7 @@
 """ctypes access to the CPU oracle (oracle/liboracle.so) and, where it was
 built, to the reference's own Eigen-free numerics (oracle/_ref).
 
-Test infrastructure: the oracle is the checker here, never the thing tested.
+Test infrastructure: the oracle checks the GPU results and is never the thing tested.
 """
 from __future__ import annotations
 

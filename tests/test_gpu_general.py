@@ -187,6 +187,15 @@ def test_wide_64(ctx):
     solve_both(ctx, random_general(1, 4, 64, seed=99, rpv=True, corr=True, numu=4, nphi=2), "nstr=64")
 
 
+@pytest.mark.parametrize("nstr", [32, 64])
+def test_spherical_rpv_user_tau_wide(ctx, nstr):
+    # One-wavefront boundary kernel at 32 streams, the two-block form
+    # (3NN > 64 rows) at 64; spherical beam, RPV, corrections, user depths.
+    gb = random_general(2, 5, nstr, seed=120 + nstr, rpv=True, spherical=True, corr=True, user_tau=True,
+                        numu=4, nphi=3)
+    solve_both(ctx, gb, f"spherical RPV user-tau nstr={nstr}")
+
+
 def test_validation_messages(ctx):
     from disort_amd._capi import DisortInvalid
 

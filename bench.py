@@ -38,35 +38,117 @@ def cpu_threads() -> int:
         return os.cpu_count() or 1
 
 
-def cpu_baseline(batch, reps: int = 3) -> dict:
-    """Oracle (CPU FP64 restatement) on all host threads: warm-up, then the
-    median of `reps` timed passes over the instances of `batch`."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_lib
-
-    threads = cpu_threads()
-    warm = batch.__class__  # keep a reference to the type only
-    del warm
-    oracle_lib.flux_batch(batch, nthreads=threads)  # warm-up
-    times = sorted(oracle_lib.flux_batch(batch, nthreads=threads) for _ in range(reps))
-    sec = times[len(times) // 2]
+def cpu_model() -> str:
     model = platform.processor() or platform.machine()
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
                 if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
+                    return line.split(":", 1)[1].strip()
     except OSError:
         pass
-    return {
-        "value": batch.n / sec,
+    return model
+
+
+def general_from_flux(batch):
+    """The same C2 instances as a comp_only_fluxes general batch (the input of
+    solve_spectral_bands): one column per band."""
+    import numpy as np
+
+    from disort_amd import general as G
+
+    n, L, P = batch.n, int(batch.nlyr[0]), int(batch.pstride[0])
+    return G.GeneralBatch(flags=int(batch.flags) | G.G_ONLY_FLUXES, nstr=int(batch.nstr[0]),
+                          dtau=batch.dtau.reshape(n, L), ssa=batch.ssa.reshape(n, L),
+                          pmom=batch.pmom.reshape(n, L, P), scal=batch.scal.reshape(n, -1),
+                          temper=None if batch.temper is None else batch.temper.reshape(n, L + 1),
+                          phi_user=np.zeros(0))
+
+
+def cpu_baseline(batch, reps: int = 5, general: bool = True) -> tuple[dict, object]:
+    """CPU restatement of DisORT++'s two spectral drivers on all host threads,
+    on the same inputs as the GPU: the flux driver (solve_flux_spectral_bands:
+    DisortFluxSolver<16> per band) and the general driver (solve_spectral_bands
+    with comp_only_fluxes: DisortSolver per band), each an OpenMP static band
+    loop (src/python_bindings.cpp:93-104, :177-188).  One warm-up, then the
+    median of `reps` passes, timed inside C++ (the loop) and around the call.
+    Returns (baseline dict, flux-driver output [8][levels])."""
+    import ctypes as C
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
+    threads = cpu_threads()
+    flux_loop, flux_call = [], []
+    for k in range(reps + 1):
+        t0 = time.perf_counter()
+        sec = oracle_lib.flux_batch(batch, nthreads=threads)
+        if k:
+            flux_loop.append(sec)
+            flux_call.append(time.perf_counter() - t0)
+    out = batch.out.copy()
+    med = lambda v: float(np.median(v))  # noqa: E731
+    model = cpu_model()
+    base = {
+        "value": batch.n / med(flux_loop),
         "unit": "instances/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"all {batch.n} C2 bands, median of {reps} passes after one warm-up, "
-        f"OpenMP static band loop, {threads} threads on {model}",
+        "sample": f"all {batch.n} instances; CPU restatement of DisORT++'s solve_flux_spectral_bands "
+                  f"(flux driver), OpenMP static loop, {threads} threads on {model}; median of {reps} "
+                  f"passes after one warm-up, C++ loop time",
+        "solve_flux_spectral_bands": {"instances_per_s": batch.n / med(flux_loop),
+                                      "instances_per_s_python_call": batch.n / med(flux_call),
+                                      "loop_s": flux_loop},
+        "nproc": os.cpu_count(),
+        "cpu_model": model,
     }
+    if not general:
+        return base, out
+    gb = general_from_flux(batch)
+    lib = oracle_lib.lib()
+    lib.oracle_general_batch.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    lib.oracle_general_batch.restype = C.c_int64
+    st = gb.c_struct()
+    gen_loop, gen_call = [], []
+    for k in range(reps + 1):
+        secs = C.c_double()
+        t0 = time.perf_counter()
+        failed = lib.oracle_general_batch(C.byref(st), 0, gb.n, threads, C.byref(secs))
+        if failed:
+            raise RuntimeError(f"{failed} general oracle columns failed")
+        if k:
+            gen_loop.append(secs.value)
+            gen_call.append(time.perf_counter() - t0)
+    # The two drivers solve the same problems: their fluxes agree.
+    gflux = np.concatenate([gb.flux[:, i, :] for i in range(gb.n)], axis=1)
+    drivers_diff = float(np.max(np.abs(gflux - out) / (np.abs(out) + 1e-3)))
+    base["solve_spectral_bands"] = {"instances_per_s": gb.n / med(gen_loop),
+                                    "instances_per_s_python_call": gb.n / med(gen_call),
+                                    "loop_s": gen_loop, "comp_only_fluxes": True}
+    base["drivers_max_rel_diff"] = drivers_diff
+    return base, out
+
+
+def output_errors(gpu, ref, batch) -> dict:
+    """Per output field: max |gpu - cpu| / (|cpu| + 1e-3) over every band and
+    level, i.e. <= 1e-9 exactly when |gpu - cpu| <= 1e-9 |cpu| + 1e-12, with
+    the band and level where it occurs."""
+    import numpy as np
+
+    from disort_amd.batch import OUT_FIELDS
+
+    L1 = int(batch.nlyr[0]) + 1
+    res = {}
+    for f, name in enumerate(OUT_FIELDS):
+        e = np.abs(gpu[f] - ref[f]) / (np.abs(ref[f]) + 1e-3)
+        j = int(np.argmax(e))
+        res[name] = {"max": float(e[j]), "band": j // L1, "level": j % L1,
+                     "gpu": float(gpu[f, j]), "cpu": float(ref[f, j]),
+                     "n_over_1e-9": int((e > 1e-9).sum())}
+    return res
 
 
 def bench_c4(args) -> dict:
@@ -130,9 +212,9 @@ def bench_c4(args) -> dict:
     }
     if not args.no_cpu_baseline:
         sample = workloads.c4(args.cpu_sample, first=10 ** 6)
-        cb = cpu_baseline(sample, reps=1)
+        cb, _ = cpu_baseline(sample, reps=1, general=False)
         cb["sample"] = (f"{args.cpu_sample} C4-distributed instances (a disjoint seeded sample), "
-                        + cb["sample"].split(", ", 1)[1])
+                        + cb["sample"].split("; ", 1)[1])
         line["cpu_baseline"] = cb
     dev.close()
     return line
@@ -333,7 +415,7 @@ def bench_c3(args) -> dict:
                    "columns": n, "nlyr": gb.nlyr, "nstr": gb.nstr, "numu": gb.numu, "nphi": gb.nphi,
                    "ntau": gb.ntau, "mean_ncut": float(ncut.mean()),
                    "path": "disort_general_solve on device arrays (DISORT_MEM_DEVICE)"},
-        "roofline": {"bound": "mfma", "kernel": dom, "achieved": fp[dom], "peak": PEAK_FP64_TFLOPS,
+        "roofline": {"bound": "fp64", "kernel": dom, "achieved": fp[dom], "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": fp[dom] / PEAK_FP64_TFLOPS, "traffic": None},
         "kernel_fp64_tflops": fp,
         "stage_ms": stage_ms,
@@ -482,7 +564,7 @@ def main() -> int:
             "path": "plain GPU DISORT (disort_flux_execute, inputs resident in HBM)",
         },
         "roofline": {
-            "bound": "mfma",
+            "bound": "fp64",
             "kernel": dom_name,
             "achieved": achieved,
             "peak": PEAK_FP64_TFLOPS,
@@ -497,7 +579,9 @@ def main() -> int:
         / (sec / args.steps) / 1e9,
     }
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(batch)
+        gpu_out = out.out.copy()
+        line["cpu_baseline"], ref = cpu_baseline(batch)
+        line["err_vs_cpu_oracle"] = output_errors(gpu_out, ref, batch)
     dev.close()
     print(json.dumps(line))
     return 0

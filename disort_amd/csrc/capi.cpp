@@ -397,9 +397,13 @@ std::vector<const void*> scratch_key(disort_ctx* ctx, const disort_flux_plan* pl
   return k;
 }
 
+// Key of a captured graph: every field of the batch descriptor (the graph
+// holds the device addresses the kernels read and write).
 bool same_batch(const disort_flux_batch& x, const disort_flux_batch& y) {
-  return x.n == y.n && x.flags == y.flags && x.dtau == y.dtau && x.ssa == y.ssa && x.pmom == y.pmom &&
-         x.temper == y.temper && x.scal == y.scal && x.out == y.out;
+  return x.n == y.n && x.flags == y.flags && x.nstr == y.nstr && x.nlyr == y.nlyr && x.pstride == y.pstride &&
+         x.lay_off == y.lay_off && x.mom_off == y.mom_off && x.dtau == y.dtau && x.ssa == y.ssa &&
+         x.pmom == y.pmom && x.temper == y.temper && x.alt == y.alt && x.scal == y.scal && x.out == y.out &&
+         x.total_levels == y.total_levels && x.status == y.status;
 }
 
 }  // namespace

@@ -7,8 +7,8 @@
 //   k_column        per column: Delta-M scaling, cut-off, beam transmission,
 //                   Chapman factors, Planck (shared with the flux path)
 //   k_glevels       per column: layer and scaled depth of each user level
-//   k_gsurf_d/_b    BRDF Fourier coefficients per (RPV parameter set, mode)
-//                   and the beam column per (column, mode)
+//   k_gsurf_c/_e/_b BRDF Fourier coefficients of all modes per (RPV parameter
+//                   set, entry), emissivities, the beam column per column
 //   k_glayer        per (column, mode, layer) wavefront: phase matrix at
 //                   mode m, reduced eigenproblem (parallel Jacobi in LDS),
 //                   beam and thermal particular solutions, eigenvectors and
@@ -106,43 +106,97 @@ __global__ __launch_bounds__(64) void k_glevels(GenArgs a) {
 }
 
 // ===========================================================================
-// k_gsurf_d: diffuse BRDF coefficients and emissivities, block per
-// (parameter set, mode) (src/DisortSolver.cpp:2604-2707).
+// Diffuse BRDF Fourier coefficients and directional emissivities of every
+// RPV parameter set (src/DisortSolver.cpp:2604-2707).
+//
+// k_gsurf_c: one thread per (parameter set, coefficient entry); the entry's
+// reflectivity rho(mu, mu', pi x_k) is evaluated once per azimuth node and
+// enters the quadrature sums of all modes (each sum in the order of the
+// reference's per-mode loop), so a set costs (NN + numu) NN x 100
+// reflectivities instead of that times the mode count.
+// k_gsurf_e: one block per (parameter set, angle with mu > 0), thread per
+// outer azimuth node, the 200 x 100 emissivity double sum.  The emissivity
+// enters only through the bottom thermal emission (multiplied by the
+// bottom Planck function, zero without thermal emission), so it is computed
+// only with DISORT_F_THERMAL; otherwise the slots hold 0.
 // ===========================================================================
 template <int NN>
-__global__ __launch_bounds__(256) void k_gsurf_d(GenArgs a) {
-  __shared__ double red[4];
-  const int s = blockIdx.x / a.nm, m = blockIdx.x % a.nm;
-  if (s >= a.nsets) return;
-  const double* p = a.set_rpv + (int64_t)s * DISORT_NRPV;
-  const double* cmu = a.tab;  // TabOff::cmu() == 0
+__global__ __launch_bounds__(256) void k_gsurf_c(GenArgs a) {
+  constexpr int N = 2 * NN, K = kNMUG / 2;
+  __shared__ double cmk[N * K];  // cos(m pi x_k), m < N (1 at m = 0)
+  const int nm = a.nm;
   const int numu = a.numu;
+  const int ne = (NN + numu) * NN;  // coefficient entries per set and mode
+  const int64_t sz = surf_d_size(NN, numu);
+  const double* gmu = a.gaz;
+  const double* gwt = a.gaz + kNMUG;
+  for (int e = threadIdx.x; e < nm * K; e += blockDim.x) {
+    const int m = e / K, k = e % K;
+    cmk[e] = (m == 0) ? 1.0 : cos((double)m * kPi * gmu[k]);
+  }
+  __syncthreads();
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)a.nsets * ne) return;
+  const int64_t s = t / ne;
+  const int e = (int)(t % ne);
+  const double* p = a.set_rpv + s * DISORT_NRPV;
+  const double* cmu = a.tab;  // TabOff::cmu() == 0
   const bool do_user = (a.flags & DISORT_G_USER_MU) && !(a.flags & DISORT_G_ONLY_FLUXES);
-  double* d = a.srf_d + ((int64_t)s * a.nm + m) * surf_d_size(NN, numu);
-  for (int e = threadIdx.x; e < NN * NN; e += blockDim.x) d[e] = rpv_fourier(a, p, m, cmu[e / NN], cmu[e % NN]);
-  for (int e = threadIdx.x; e < numu * NN; e += blockDim.x) {
-    const double u = do_user ? a.mu_user[e / NN] : 0.0;
-    d[NN * NN + e] = (u > 0.0) ? rpv_fourier(a, p, m, u, cmu[e % NN]) : 0.0;
+  const double mup = cmu[e % NN];
+  double mu;
+  if (e < NN * NN) mu = cmu[e / NN];
+  else mu = do_user ? a.mu_user[(e - NN * NN) / NN] : 0.0;
+  const bool act = mu > 0.0;
+  double sm[N];
+#pragma unroll
+  for (int m = 0; m < N; ++m) sm[m] = 0.0;
+  if (act)
+    for (int k = 0; k < K; ++k) {
+      const double x = gwt[k] * rpv_rho(p, mu, mup, kPi * gmu[k]);
+#pragma unroll
+      for (int m = 0; m < N; ++m)
+        if (m < nm) sm[m] += x * cmk[m * K + k];
+    }
+  double* d = a.srf_d + s * nm * sz + e;
+#pragma unroll
+  for (int m = 0; m < N; ++m)
+    if (m < nm) d[m * sz] = act ? (m == 0 ? 1.0 : 2.0) * sm[m] : 0.0;
+  // Emissivity slots (written here for sets without thermal emission, and
+  // for modes m > 0 and angles mu <= 0 in any case).
+  const int ee = e;
+  if (ee < NN + numu) {
+    double mue = 0.0;
+    if (ee < NN) mue = cmu[ee];
+    else if (do_user) mue = a.mu_user[ee - NN];
+    const bool computed = (a.flags & DISORT_F_THERMAL) && mue > 0.0;
+    double* eq = a.srf_d + s * nm * sz + NN * NN + numu * NN + ee;
+    for (int m = 0; m < nm; ++m)
+      if (m > 0 || !computed) eq[m * sz] = 0.0;
   }
-  // Directional emissivity 1 - int int rho mu' (m = 0 only).
-  double* eq = d + NN * NN + numu * NN;
-  for (int e = 0; e < NN + numu; ++e) {
-    double mu = 0.0;
-    if (e < NN) mu = cmu[e];
-    else if (do_user) mu = a.mu_user[e - NN];
-    const bool want = m == 0 && mu > 0.0;
-    double part = 0.0;
-    if (want)
-      for (int jg = threadIdx.x; jg < kNMUG; jg += blockDim.x) {
-        double sm = 0.0;
-        for (int k = 0; k < kNMUG / 2; ++k)
-          sm += a.gaz[kNMUG + k] * a.gaz[k] * rpv_rho(p, mu, a.gaz[k], kPi * a.gaz[jg]);
-        part += a.gaz[kNMUG + jg] * sm;
-      }
-    const double tot = blk_sum(part, red);
-    if (threadIdx.x == 0) eq[e] = want ? 1.0 - tot : 0.0;
-    __syncthreads();
+}
+
+template <int NN>
+__global__ __launch_bounds__(256) void k_gsurf_e(GenArgs a) {
+  __shared__ double red[4];
+  const int numu = a.numu;
+  const int64_t s = blockIdx.x / (NN + numu);
+  const int e = (int)(blockIdx.x % (NN + numu));
+  const bool do_user = (a.flags & DISORT_G_USER_MU) && !(a.flags & DISORT_G_ONLY_FLUXES);
+  double mu = 0.0;
+  if (e < NN) mu = a.tab[e];
+  else if (do_user) mu = a.mu_user[e - NN];
+  if (!(mu > 0.0)) return;  // block-uniform
+  const double* p = a.set_rpv + s * DISORT_NRPV;
+  const double* gmu = a.gaz;
+  const double* gwt = a.gaz + kNMUG;
+  double part = 0.0;
+  for (int jg = threadIdx.x; jg < kNMUG; jg += blockDim.x) {
+    double sm = 0.0;
+    for (int k = 0; k < kNMUG / 2; ++k) sm += gwt[k] * gmu[k] * rpv_rho(p, mu, gmu[k], kPi * gmu[jg]);
+    part += gwt[jg] * sm;
   }
+  const double tot = blk_sum(part, red);
+  if (threadIdx.x == 0) a.srf_d[s * a.nm * surf_d_size(NN, numu) + NN * NN + numu * NN + e] = 1.0 - tot;
 }
 
 // Beam column of the BRDF coefficients of every mode, block per chunk
@@ -1882,7 +1936,12 @@ void general_setup_n(const FluxArgs& fa, const GenArgs& a, hipStream_t s, bool b
   constexpr int N = 2 * NN;
   hipLaunchKernelGGL(k_column<N>, dim3((unsigned)fa.nb), dim3(64), 0, s, fa);
   hipLaunchKernelGGL(k_glevels, dim3((unsigned)((a.n + 63) / 64)), dim3(64), 0, s, a);
-  if (brdf) hipLaunchKernelGGL(k_gsurf_d<NN>, dim3((unsigned)(a.nsets * a.nm)), dim3(256), 0, s, a);
+  if (brdf) {
+    const int64_t ne = (int64_t)a.nsets * (NN + a.numu) * NN;
+    hipLaunchKernelGGL(k_gsurf_c<NN>, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, s, a);
+    if (a.flags & DISORT_F_THERMAL)
+      hipLaunchKernelGGL(k_gsurf_e<NN>, dim3((unsigned)(a.nsets * (NN + a.numu))), dim3(256), 0, s, a);
+  }
 }
 
 template <int NN>

@@ -493,7 +493,8 @@ class GatedSurrogate:
         od = ood.get(np.int32, n).astype(bool)
         r = np.random.Generator(np.random.Philox(seed))
         sampled = r.random(n) < sample_frac
-        fail = od | (sp > self.tau)
+        # A NaN spread (non-finite network output) fails the gate.
+        fail = od | ~(sp <= self.tau)
         route = np.nonzero(fail | sampled)[0].astype(np.int32)
         m = int(route.size)
         if m:
@@ -502,6 +503,10 @@ class GatedSurrogate:
             _ck(ctx, l.disort_surrogate_gather(ctx.ptr, m, didx.ptr, NLYR, NSTR + 1, C.byref(dc.cstruct),
                                                C.byref(sub.cstruct)))
             sub.solve()
+            st = sub.status()
+            if st.any():
+                raise RuntimeError(f"plain DISORT failed on {int((st != 0).sum())} routed columns "
+                                   f"(status codes {sorted(set(st[st != 0].tolist()))})")
             _ck(ctx, l.disort_surrogate_scatter(ctx.ptr, m, didx.ptr, NLEV, sub.cstruct.out, sub.cstruct.total_levels,
                                                 flux.ptr))
             ctx.sync()
@@ -509,7 +514,7 @@ class GatedSurrogate:
             didx.free()
         spread.free()
         ood.free()
-        return flux, {"n": n, "n_ood": int(od.sum()), "n_spread_fail": int((sp > self.tau).sum()),
+        return flux, {"n": n, "n_ood": int(od.sum()), "n_spread_fail": int((~(sp <= self.tau)).sum()),
                       "n_sampled": int(sampled.sum()), "n_plain": m, "sampled_idx": np.nonzero(sampled)[0],
                       "accepted_idx": np.nonzero(~(fail | sampled))[0]}
 
@@ -540,6 +545,11 @@ class _SubBatch:
 
     def solve(self):
         _ck(self.ctx, _capi.lib().disort_flux_execute(self.ctx.ptr, self.plan, C.byref(self.cstruct)))
+
+    def status(self) -> np.ndarray:
+        st = np.zeros(self.m, np.int32)
+        _ck(self.ctx, _capi.lib().disort_flux_status(self.ctx.ptr, self.plan, st.ctypes.data))
+        return st
 
     def close(self):
         if self.plan:

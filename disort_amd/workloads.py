@@ -14,10 +14,11 @@ c2  spectral bands: 10,000 bands [100+b, 101+b] cm^-1 x one column, nlyr=60,
     emissivity_top 1 (choices not fixed by the workload statement).
 c3  full solver: 4,096 columns, nlyr=80, nstr=32, nmom=33; per layer
     delta_tau = 10^U(-4,1), single_scat_albedo ~ U(0,0.999), phase moments
-    w Rayleigh + (1-w) HG with w ~ U(0,1), g ~ U(0,0.9); per column
-    mu0 ~ U(0.1,1), beam flux pi, beam azimuth 0; pseudo-spherical beam
-    (radius 6371 km, level altitudes 80 -> 0 km in 1 km steps); RPV surface
-    rho0=0.027, k=0.647, theta=-0.169, sigma=t1=t2=0, scale=1; intensities at
+    w Rayleigh + (1-w) HG with w ~ U(0,1), g ~ U(0,0.95); per column
+    mu0 ~ U(0.05,1), beam flux pi, beam azimuth 0; pseudo-spherical beam
+    (radius 6371 km, level altitudes 80 -> 0 km in 1 km steps); an RPV
+    surface per column, rho0 ~ U(0.01,0.3), k ~ U(0.5,1.5),
+    theta ~ U(-0.3,0), sigma=t1=t2=0, scale=1; intensities at
     16 polar cosines +-{0.1,0.2,0.35,0.5,0.65,0.8,0.9,1} and 8 azimuths
     0..315 deg at the 81 layer boundaries; Delta-M+ requested;
     intensity_corr_buras and intensity_corr_nakajima on; every Fourier mode
@@ -159,12 +160,17 @@ def c4(n: int = 50000, seed: int = C4_SEED, first: int = 0, streams=C4_STREAMS, 
 C3_SEED = 20261205
 C3_MU = np.array([-1.0, -0.9, -0.8, -0.65, -0.5, -0.35, -0.2, -0.1, 0.1, 0.2, 0.35, 0.5, 0.65, 0.8, 0.9, 1.0])
 C3_PHI = np.arange(8) * 45.0
-C3_RPV = (0.027, 0.647, -0.169, 0.0, 0.0, 0.0, 1.0)
 
 
 def c3(n: int = 4096, nlyr: int = 80, nstr: int = 32, seed: int = C3_SEED, first: int = 0):
     """The general-solver workload (see the module docstring); column k draws
-    from its own Philox stream, so any subset reproduces the same columns."""
+    from its own Philox stream, so any subset reproduces the same columns.
+
+    Per column: dtau log-uniform[1e-4, 10], single-scattering albedo
+    U[0, 0.999], phase function w * Rayleigh + (1 - w) * HG with w ~ U[0, 1],
+    g ~ U[0, 0.95]; beam mu0 ~ U[0.05, 1]; its own RPV surface with
+    rho0 ~ U[0.01, 0.3], k ~ U[0.5, 1.5], theta ~ U[-0.3, 0] (sigma = t1 =
+    t2 = 0, scale 1)."""
     from . import general as G
 
     P = nstr + 2  # moments 0..nmom, nmom = nstr + 1
@@ -172,18 +178,23 @@ def c3(n: int = 4096, nlyr: int = 80, nstr: int = 32, seed: int = C3_SEED, first
     ssa = np.empty((n, nlyr))
     pmom = np.empty((n, nlyr, P))
     mu0 = np.empty(n)
+    rpv = np.zeros((n, 7))
+    rpv[:, 6] = 1.0
     ray = np.zeros(P)
     ray[0], ray[2] = 1.0, 0.1
     for k in range(n):
         r = rng(seed + 1 + first + k)
         dtau[k] = 10.0 ** r.uniform(-4.0, 1.0, nlyr)
         ssa[k] = r.uniform(0.0, 0.999, nlyr)
-        g = r.uniform(0.0, 0.9, nlyr)
+        g = r.uniform(0.0, 0.95, nlyr)
         w = r.uniform(0.0, 1.0, nlyr)
         pmom[k] = w[:, None] * ray + (1.0 - w[:, None]) * hg_moments(g, P)
-        mu0[k] = r.uniform(0.1, 1.0)
+        mu0[k] = r.uniform(0.05, 1.0)
+        rpv[k, 0] = r.uniform(0.01, 0.3)
+        rpv[k, 1] = r.uniform(0.5, 1.5)
+        rpv[k, 2] = r.uniform(-0.3, 0.0)
     sc = scalars(n, beam_flux=np.pi, beam_mu=mu0, radius=6371.0)
     alt = np.repeat(np.linspace(float(nlyr), 0.0, nlyr + 1)[None], n, axis=0)
     flags = (2 | 4 | G.G_USER_MU | G.G_BRDF_RPV | G.G_CORR_BURAS | G.G_CORR_NAKAJIMA)
     return G.GeneralBatch(flags=flags, nstr=nstr, dtau=dtau, ssa=ssa, pmom=pmom, scal=sc, alt=alt,
-                          rpv=np.tile(np.array(C3_RPV), (n, 1)), mu_user=C3_MU, phi_user=C3_PHI, accuracy=0.0)
+                          rpv=rpv, mu_user=C3_MU, phi_user=C3_PHI, accuracy=0.0)

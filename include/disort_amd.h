@@ -165,7 +165,7 @@ int disort_flux_status(disort_ctx* ctx, disort_flux_plan* plan, int32_t* status_
  * workgroup per layer and per instance, every N; the only variant for
  * N = 32, 64 and for the spherical beam.  Both give
  * results within the parity tolerance of each other; the default is LANE
- * where it exists.  The autotuner (disort_amd/schedule.py) picks by timing.
+ * where it exists.  The autotuner (disort_flux_autotune) picks by timing.
  */
 enum { DISORT_V_DEFAULT = 0, DISORT_V_LANE = 1, DISORT_V_WORKGROUP = 2 };
 int disort_flux_plan_set_variant(disort_ctx* ctx, disort_flux_plan* plan, int nstr, int variant);

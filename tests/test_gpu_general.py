@@ -196,6 +196,31 @@ def test_spherical_rpv_user_tau_wide(ctx, nstr):
     solve_both(ctx, gb, f"spherical RPV user-tau nstr={nstr}")
 
 
+@pytest.mark.parametrize("first", [0, 1000])
+def test_c3_workload(ctx, first):
+    # The C3 shape itself: 80 layers, 32 streams, Delta-M+ requested,
+    # pseudo-spherical beam with mu0 down to 0.05, a distinct RPV surface per
+    # column, 16 user polar angles x 8 azimuths at all 81 levels,
+    # Nakajima-Tanaka correction, every Fourier mode.
+    from disort_amd import workloads
+
+    gb = workloads.c3(8, first=first)
+    assert len({tuple(r) for r in gb.rpv}) == gb.n
+    solve_both(ctx, gb, f"C3 columns {first}..{first + gb.n - 1}")
+
+
+@pytest.mark.parametrize("nstr", [8, 16, 64])
+def test_rpv_thermal_distinct_surfaces(ctx, nstr):
+    # Thermal emission with an RPV surface: the directional emissivities
+    # (200 x 100 azimuth double sums) of a distinct surface per column.
+    gb = random_general(4, 5, nstr, seed=140 + nstr, rpv=True, thermal=True, numu=6, nphi=3)
+    r = np.random.default_rng(nstr)
+    gb.rpv[:, 0] = r.uniform(0.01, 0.3, gb.n)
+    gb.rpv[:, 1] = r.uniform(0.5, 1.5, gb.n)
+    gb.rpv[:, 2] = r.uniform(-0.3, 0.0, gb.n)
+    solve_both(ctx, gb, f"RPV + thermal, distinct surfaces nstr={nstr}")
+
+
 def test_validation_messages(ctx):
     from disort_amd._capi import DisortInvalid
 

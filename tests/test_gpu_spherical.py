@@ -78,3 +78,36 @@ def test_spherical_needs_workgroup_variant(ctx):
     with pytest.raises(DisortError):
         dev.set_variant(8, V_LANE)
     dev.close()
+
+
+def test_graph_recaptures_on_new_altitudes(ctx):
+    # The captured graph holds the device address of every batch array: a
+    # replay with a new altitude buffer (all other pointers unchanged) must
+    # solve with the new altitudes, as the eager execute does.
+    from disort_amd._capi import DeviceFluxBatch
+
+    b = spherical_batch(4, 6, 8, seed=811)
+    dev = DeviceFluxBatch(ctx, b)
+    first_alt = dev.cstruct.alt
+    dev.execute_graph()
+    g1 = dev.download().out.copy()
+    dev.execute()
+    e1 = dev.download().out.copy()
+    np.testing.assert_array_equal(g1, e1)
+    alt2 = np.ascontiguousarray(b.alt * 0.5)
+    p2 = ctx.alloc(alt2.nbytes)
+    dev._bufs.append(p2)
+    ctx.h2d(p2, alt2)
+    dev.cstruct.alt = p2
+    dev.execute_graph()
+    g2 = dev.download().out.copy()
+    dev.execute()
+    e2 = dev.download().out.copy()
+    np.testing.assert_array_equal(g2, e2)
+    assert not np.array_equal(g2, g1)
+    b2 = spherical_batch(4, 6, 8, seed=811)
+    b2.alt[:] = alt2
+    solve_and_check(ctx, b2, label="spherical, halved altitudes")
+    np.testing.assert_allclose(e2, b2.out, rtol=1e-12, atol=1e-15)
+    dev.cstruct.alt = first_alt
+    dev.close()

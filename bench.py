@@ -140,6 +140,9 @@ def output_errors(gpu, ref, batch) -> dict:
 
     from disort_amd.batch import OUT_FIELDS
 
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
     L1 = int(batch.nlyr[0]) + 1
     res = {}
     for f, name in enumerate(OUT_FIELDS):
@@ -148,6 +151,13 @@ def output_errors(gpu, ref, batch) -> dict:
         res[name] = {"max": float(e[j]), "band": j // L1, "level": j % L1,
                      "gpu": float(gpu[f, j]), "cpu": float(ref[f, j]),
                      "n_over_1e-9": int((e > 1e-9).sum())}
+        if e[j] > 1e-9:
+            # Which side is off: both against the same solve in x87 extended
+            # precision (64-bit significand) of that band.
+            x = float(oracle_lib.flux_one_extended(batch, j // L1)[f, j % L1])
+            res[name]["extended_precision"] = x
+            res[name]["gpu_vs_extended"] = abs(gpu[f, j] - x) / (abs(x) + 1e-3)
+            res[name]["cpu_vs_extended"] = abs(ref[f, j] - x) / (abs(x) + 1e-3)
     return res
 
 

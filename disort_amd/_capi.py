@@ -75,6 +75,10 @@ def lib():
         l.disort_flux_plan_report.argtypes = [vp]
         l.disort_flux_plan_report.restype = C.c_char_p
         l.disort_general_solve.argtypes = [vp, vp, C.c_int]
+        l.disort_flux_execute_static.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_int32]
+        l.disort_flux_static_bytes.argtypes = [C.c_int64, C.c_int32, C.c_int32]
+        l.disort_flux_static_bytes.restype = C.c_size_t
+        l.disort_static_scratch_bytes.restype = C.c_size_t
         _lib = l
     return _lib
 

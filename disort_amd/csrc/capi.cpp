@@ -92,11 +92,22 @@ int table_for(disort_ctx* ctx, int nstr, double** out) {
     *out = it->second;
     return DISORT_OK;
   }
+  std::vector<double> tab;
+  if (!build_table(nstr, tab)) return fail(ctx, DISORT_E_NUMERIC, "gaussianQuadrature: Failed to converge");
+  double* d = nullptr;
+  HIPCHK(ctx, hipMalloc(&d, tab.size() * sizeof(double)));
+  HIPCHK(ctx, hipMemcpy(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  ctx->tables[nstr] = d;
+  *out = d;
+  return DISORT_OK;
+}
+
+bool build_table(int nstr, std::vector<double>& tab) {
   const int nn = nstr / 2;
   std::vector<double> cmu, cwt;
-  if (!gauss_nodes(nn, cmu, cwt)) return fail(ctx, DISORT_E_NUMERIC, "gaussianQuadrature: Failed to converge");
+  if (!gauss_nodes(nn, cmu, cwt)) return false;
   dsa::TabOff to(nstr);
-  std::vector<double> tab(to.size(), 0.0);
+  tab.assign(to.size(), 0.0);
   for (int k = 0; k < nstr; ++k) {
     tab[to.cmu() + k] = cmu[k];
     tab[to.cwt() + k] = cwt[k];
@@ -116,12 +127,7 @@ int table_for(disort_ctx* ctx, int nstr, double** out) {
       tab[to.Q() + l * nn + r] = p[l] * sc;
     }
   }
-  double* d = nullptr;
-  HIPCHK(ctx, hipMalloc(&d, tab.size() * sizeof(double)));
-  HIPCHK(ctx, hipMemcpy(d, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
-  ctx->tables[nstr] = d;
-  *out = d;
-  return DISORT_OK;
+  return true;
 }
 
 bool gpu_supported_nstr(int n) { return (n >= 4 && n <= 16 && n % 2 == 0) || n == 32 || n == 64; }

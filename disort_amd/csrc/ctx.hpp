@@ -47,5 +47,8 @@ int fail(disort_ctx* ctx, int code, const std::string& msg);
 int get_buf(disort_ctx* ctx, const char* name, size_t bytes, void** out);
 bool gauss_nodes(int n, std::vector<double>& cmu, std::vector<double>& cwt);
 int table_for(disort_ctx* ctx, int nstr, double** out);
+// Constant table of stream count nstr (TabOff layout: nodes, weights,
+// symmetrisation factors, Legendre values at the nodes).
+bool build_table(int nstr, std::vector<double>& tab);
 bool gpu_supported_nstr(int n);
 }  // namespace dsa_host

@@ -198,6 +198,32 @@ int disort_flux_execute_graph(disort_ctx* ctx, disort_flux_plan* plan, const dis
 int disort_flux_autotune(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* device_batch, int reps);
 const char* disort_flux_plan_report(disort_flux_plan* plan);
 
+/*
+ * Device-resident flux solve that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same solve and kernels as disort_flux_execute for a uniform batch --
+ * every instance has nstr streams, nlyr layers and pstride moments per layer
+ * row; the batch's nstr/nlyr/pstride/lay_off/mom_off arrays are ignored and
+ * instance i owns layers [i*nlyr, (i+1)*nlyr) -- with every intermediate in
+ * one static device array of the library (DISORT_STATIC_SCRATCH_BYTES per
+ * device, held until the process ends).  Device pointers only; inputs are not
+ * validated; outputs and, if batch->status is not NULL, the per-instance
+ * status (device int32 array [n]) are written asynchronously on the context
+ * stream.  A batch whose intermediates exceed the static array is rejected
+ * with DISORT_E_INVALID (disort_flux_static_bytes gives the requirement; at
+ * nstr=16, nlyr=60 about 350 KB per instance).  All calls on one device share
+ * that array: calls that overlap in time on one device (other streams or
+ * contexts) race.  The first call for a stream count on a device uploads its
+ * constant table into the array (one synchronous host-to-device copy).
+ */
+#define DISORT_STATIC_SCRATCH_BYTES ((size_t)256 << 20)
+int disort_flux_execute_static(disort_ctx* ctx, const disort_flux_batch* device_batch, int32_t nstr, int32_t nlyr,
+                               int32_t pstride);
+/* Bytes of static scratch a uniform batch needs, and the bytes available. */
+size_t disort_flux_static_bytes(int64_t n, int32_t nstr, int32_t nlyr);
+size_t disort_static_scratch_bytes(void);
+
 /* Device memory helpers for callers without their own allocator (ctypes). */
 int disort_device_alloc(disort_ctx* ctx, size_t bytes, void** ptr);
 int disort_device_free(disort_ctx* ctx, void* ptr);

@@ -251,6 +251,22 @@ double oracle_rpv(const double* rpv, double mu1, double mu2, double phi) {
   return oracle::rpv_reflectivity(rpv, mu1, mu2, phi);
 }
 
+}  // extern "C"
+
+namespace oracle_ld {
+void set_perturbation(int stage, double eps);
+}
+
+extern "C" {
+
+/* Mutation hook of the pin tests (see oracle::set_perturbation): selects
+ * one stage of both the FP64 and the extended-precision oracle; stage 0
+ * restores the unperturbed oracle. */
+void oracle_set_perturbation(int stage, double eps) {
+  oracle::set_perturbation(stage, eps);
+  oracle_ld::set_perturbation(stage, eps);
+}
+
 int oracle_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

@@ -62,6 +62,11 @@ void validate_flux_config(const FluxConfig& c) {
 
 namespace {
 
+// Mutation-hook stage numbers of the flux-only solver: its own intermediates
+// are selected apart from the general solver's (stage + kFluxStages), the
+// shared numerics (Planck, Chapman) by their common stage number.
+constexpr int kFluxStages = 32;
+
 struct FluxSolver {
   const FluxConfig& c;
   int N, NN, L, ncut = 0;
@@ -564,6 +569,7 @@ struct FluxSolver {
     band_solve(band.data(), lda, nc, ncd, ncd, ipvt.data(), b.data());
     LL.assign(nn, 0.0);
     for (int i = 0; i < nc; ++i) LL[i] = b[i];  // the reference's reordering is the identity
+    perturb(kFluxStages + PS_CONSTANTS, LL);
   }
 
   // src/FluxSolver.cpp:1138-1287
@@ -655,6 +661,9 @@ struct FluxSolver {
       if (deltam_plus) deltam = false;
     }
     set_up();
+    perturb(kFluxStages + PS_SCALED_COEF, gl);
+    perturb(kFluxStages + PS_SCALED_SSA, ssas);
+    perturb(kFluxStages + PS_QUAD, cwt);
     pl_top = pl_bot = pl_bot_deriv = 0.0;
     if (c.thermal) {
       pl_top = planck_band(c.wn_lo, c.wn_hi, c.temp_top) * c.emis_top;
@@ -666,10 +675,13 @@ struct FluxSolver {
     }
     ylmc.assign((size_t)(N + 1) * N, 0.0);
     legendre_flat(N, 0, N - 1, N - 1, cmu.data(), ylmc.data());
+    perturb(kFluxStages + PS_LEGENDRE, ylmc);
     kk.assign((size_t)L * N, 0.0);
     gc.assign((size_t)L * N * N, 0.0);
     cc.assign((size_t)L * N * N, 0.0);
     for (int lc = 0; lc < L; ++lc) solve_eigen(lc);
+    perturb(kFluxStages + PS_EIGVAL, kk);
+    perturb(kFluxStages + PS_EIGVEC, gc);
     zb.assign((size_t)L * N, 0.0);
     zbs.assign((size_t)L * N, 0.0);
     xba.assign(L, 0.0);
@@ -681,10 +693,14 @@ struct FluxSolver {
       if (c.spherical) beam_source_spherical();
       else beam_source();
     }
+    perturb(kFluxStages + PS_BEAM, zb);
     if (c.thermal) thermal_source();
+    perturb(kFluxStages + PS_THERMAL, z0);
+    perturb(kFluxStages + PS_THERMAL, z1);
     boundary_solve();
     FluxOutput out;
     fluxes(out);
+    for (auto& f : out.f) perturb(kFluxStages + PS_FLUX, f);
     return out;
   }
 };

@@ -24,6 +24,10 @@ namespace ORACLE_NS {
 // src/DisortSolver.cpp:2734-2780 with badmu = 0 (the reference's dispatcher,
 // :2912-2919, always passes 0).
 Real rpv_reflectivity(const Real* p, Real mu1, Real mu2, Real phi) {
+  return perturb1(PS_BRDF, rpv_reflectivity_exact(p, mu1, mu2, phi));
+}
+
+Real rpv_reflectivity_exact(const Real* p, Real mu1, Real mu2, Real phi) {
   const Real rho0 = p[0], k = p[1], theta = p[2], sigma = p[3], t1 = p[4], t2 = p[5], scale = p[6];
   phi = M_PI - phi;  // RPV measures azimuth from the backward direction
   const Real hspot = rho0 * (std::pow(2.0 * mu1 * mu1 * mu1, k - 1.0) * (1.0 - theta) /
@@ -768,6 +772,7 @@ struct GeneralSolver {
     band_solve(band.data(), lda, nc, ncd, ncd, ipvt.data(), b.data());
     LL.assign(nn, 0.0);
     for (int i = 0; i < nc; ++i) LL[i] = b[i];
+    perturb(PS_CONSTANTS, LL);
   }
 
   // src/DisortSolver.cpp:1980-2124 (m = 0), at the user levels.
@@ -1164,6 +1169,9 @@ struct GeneralSolver {
     nphi = c.only_fluxes ? 0 : (int)c.phi_user.size();
     do_user = !c.only_fluxes && c.use_user_mu;
     set_up();
+    perturb(PS_SCALED_COEF, gl);
+    perturb(PS_SCALED_SSA, ssas);
+    perturb(PS_QUAD, cwt);
 
     GeneralOutput out;
     out.ntau = ntau;
@@ -1221,15 +1229,23 @@ struct GeneralSolver {
       out.modes = m + 1;
       legendre_flat(N, m, N - 1, N - 1, cmu.data(), ylmc.data());
       if (do_user) legendre_flat(numu, m, numu - 1, N - 1, umu.data(), ylmu.data());
+      perturb(PS_LEGENDRE, ylmc);
+      perturb(PS_LEGENDRE, ylmu);
       for (int lc = 0; lc < L; ++lc) {
         solve_eigen(lc, m);
         if (do_user) interp_eigvec(lc, m);
       }
+      perturb(PS_EIGVAL, kk);
+      perturb(PS_EIGVEC, gc);
+      perturb(PS_USER_EIGVEC, gu);
       if (c.beam_flux > 0.0) {
         if (c.spherical) beam_source_spherical(m);
         else beam_source(m);
       }
+      perturb(PS_BEAM, zb);
       if (c.thermal && m == 0) thermal_source();
+      perturb(PS_THERMAL, z0);
+      perturb(PS_THERMAL, z1);
       if (do_user)
         for (int lc = 0; lc < L; ++lc) interp_source(lc, m);
       surface_bidir(m);
@@ -1249,9 +1265,19 @@ struct GeneralSolver {
     // Nakajima-Tanaka unless a tabulated phase function selects Buras-Emde
     // (src/DisortSolver.cpp:369-377); this interface carries none
     // (num_phase_func_angles = 0), so corr_nakajima does not change the result.
-    if (corint) nt_correction(out);
+    if (corint) {
+      std::vector<Real> before = out.inten;
+      nt_correction(out);
+      // Mutation hook: scale the correction itself.
+      std::vector<Real> d(out.inten.size());
+      for (size_t q = 0; q < d.size(); ++q) d[q] = out.inten[q] - before[q];
+      perturb(PS_NT, d);
+      for (size_t q = 0; q < d.size(); ++q) out.inten[q] = before[q] + d[q];
+    }
     if (c.corr_buras && !deltam_plus && !c.lambertian && !c.only_fluxes && c.use_user_mu && c.beam_flux > 0.0)
       brdf_correction(out);
+    for (auto& f : out.f) perturb(PS_FLUX, f);
+    perturb(PS_FLUX, out.inten);
     return out;
   }
 };

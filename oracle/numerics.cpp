@@ -87,7 +87,29 @@ Real planck_band_strict(Real lo, Real hi, Real t) {
 }
 }  // namespace
 
-Real planck_band(Real lo, Real hi, Real t) {
+namespace {
+int g_pstage = PS_NONE;
+double g_peps = 0.0;
+}  // namespace
+
+void set_perturbation(int stage, double eps) {
+  g_pstage = stage;
+  g_peps = eps;
+}
+
+void perturb(int stage, std::vector<Real>& v) {
+  if (stage != g_pstage || g_pstage == PS_NONE) return;
+  static const double w[4] = {-1.0, -1.0 / 3.0, 1.0 / 3.0, 1.0};
+  for (size_t i = 0; i < v.size(); ++i) v[i] *= 1.0 + g_peps * w[i % 4];
+}
+
+Real perturb1(int stage, Real x) {
+  return (stage == g_pstage && g_pstage != PS_NONE) ? x * (1.0 + g_peps) : x;
+}
+
+Real planck_band(Real lo, Real hi, Real t) { return perturb1(PS_PLANCK, planck_band_exact(lo, hi, t)); }
+
+Real planck_band_exact(Real lo, Real hi, Real t) {
   // src/Planck.cpp:146-169
   if (t < 0.0 || hi < lo || lo < 0.0)
     throw std::invalid_argument("planckFunction2: Invalid arguments (temp=" + std::to_string(t) +
@@ -189,6 +211,10 @@ void gauss_quadrature(int n, Real* cmu, Real* cwt) {
 // Restates src/FluxSolver.cpp:1503-1566 (1-based layer index lc).
 // ---------------------------------------------------------------------------
 Real chapman(int lc, Real tau_frac, const Real* zd, const Real* dtau, int nlyr, Real r, Real mu0) {
+  return perturb1(PS_CHAPMAN, chapman_exact(lc, tau_frac, zd, dtau, nlyr, r, mu0));
+}
+
+Real chapman_exact(int lc, Real tau_frac, const Real* zd, const Real* dtau, int nlyr, Real r, Real mu0) {
   const Real deg = M_PI / 180.0;
   const Real zenang = std::acos(mu0) * 180.0 / M_PI;
   const Real zenrad = zenang * deg;

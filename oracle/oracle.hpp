@@ -32,6 +32,10 @@
 //        m=0 term, the accuracy_fourier_series truncation, index_from_bottom.
 //   rpv_reflectivity: reciprocity rho(mu1,mu2,phi) = rho(mu2,mu1,phi), and
 //        the transparent-medium closed form (mu0 F0/pi rho at the top).
+//   Strength of these pins: tests/test_oracle_pins.py plants a 1e-6 relative
+//   error in each stage of flux_solve and general_solve (set_perturbation)
+//   and checks that a fixture, asserted at the precision its literals carry,
+//   fails.
 #pragma once
 #include <cstdint>
 
@@ -50,12 +54,33 @@
 namespace ORACLE_NS {
 using Real = ORACLE_REAL;
 
+// ---- mutation hooks (tests of the pins only) ------------------------------
+// With a stage selected by set_perturbation, the oracle multiplies that
+// stage's intermediate values by 1 + eps * w_i, w_i in {-1, -1/3, 1/3, 1}
+// cycling with the element index (non-uniform, so that no gauge freedom such
+// as the scale of an eigenvector absorbs it).  Stage 0 (the default) leaves
+// every value untouched.  The flux-only solver's own stages are numbered
+// 32 + stage (the shared numerics -- Planck, Chapman, RPV -- keep their
+// number).  The pin tests show that each stage, perturbed by 1e-6, fails at
+// least one fixture.
+enum PerturbStage {
+  PS_NONE = 0, PS_SCALED_COEF = 1, PS_SCALED_SSA = 2, PS_EIGVAL = 3, PS_EIGVEC = 4, PS_BEAM = 5,
+  PS_THERMAL = 6, PS_PLANCK = 7, PS_CONSTANTS = 8, PS_QUAD = 9, PS_LEGENDRE = 10, PS_USER_EIGVEC = 11,
+  PS_NT = 12, PS_BRDF = 13, PS_CHAPMAN = 14, PS_FLUX = 15, PS_BURAS = 16, PS_COUNT = 17
+};
+void set_perturbation(int stage, double eps);
+void perturb(int stage, std::vector<Real>& v);
+Real perturb1(int stage, Real x);
+
 // ---- numerics (restating src/Planck.cpp, src/Legendre.cpp, ...) ----------
 Real planck_band(Real wnumlo, Real wnumhi, Real temp);        // planckFunction2
+Real planck_band_exact(Real wnumlo, Real wnumhi, Real temp);  // the same, never perturbed
 Real planck_band_deriv(Real wnumlo, Real wnumhi, Real temp);  // planckFunctionDeriv2
 void legendre_flat(int nmu, int m, int maxmu, int twonm1, const Real* mu, Real* ylm);
 void gauss_quadrature(int nn, Real* cmu /*[2nn]*/, Real* cwt /*[2nn]*/);
 Real chapman(int lc, Real tau_frac, const Real* level_altitudes,
+               const Real* dtau_lyr, int num_layers, Real r, Real mu0);
+Real chapman_exact(int lc, Real tau_frac, const Real* level_altitudes,
                const Real* dtau_lyr, int num_layers, Real r, Real mu0);
 
 // LINPACK-semantics band LU (column-major, lda rows, 1-based ipvt).
@@ -132,5 +157,6 @@ struct GeneralOutput {
 void validate_general_config(const GeneralConfig& c);
 GeneralOutput general_solve(const GeneralConfig& c);
 Real rpv_reflectivity(const Real* rpv, Real mu1, Real mu2, Real phi);
+Real rpv_reflectivity_exact(const Real* rpv, Real mu1, Real mu2, Real phi);
 
 }  // namespace ORACLE_NS

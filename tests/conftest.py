@@ -16,6 +16,17 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
 
 
+def printed_ulp(v: float) -> float:
+    """One unit in the last printed digit of a fixture literal (the shortest
+    repr of a float reproduces the decimal literal it was typed as)."""
+    import math
+
+    r = repr(abs(float(v)))
+    mant, _, _ = r.partition("e")
+    digits = len(mant.replace(".", "").lstrip("0"))
+    return 10.0 ** (math.floor(math.log10(abs(v))) - (digits - 1))
+
+
 def load_golden(name):
     with open(os.path.join(HERE, "golden", name)) as f:
         return json.load(f)

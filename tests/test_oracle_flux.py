@@ -3,7 +3,7 @@
 invariants of the radiative-transfer solution."""
 import numpy as np
 import pytest
-from conftest import case_batch, max_rel_err
+from conftest import case_batch, max_rel_err, printed_ulp
 
 import oracle_lib as ol
 from disort_amd import workloads as W
@@ -12,16 +12,32 @@ from disort_amd.batch import F_THERMAL, OUT_FIELDS, scalars, uniform_batch
 FIELD = {name: k for k, name in enumerate(OUT_FIELDS)}
 
 
+def reference_case_failures(case, out):
+    """Checks of one fixture case at what its literals pin: the 13-digit
+    DisORT++ self-regression values at the case's rtol (1e-11; 1e-8 for the
+    conservative cloud case, the reference test's own bound), every shorter
+    value to half a unit in its last printed digit; exact zeros to 1e-12
+    (the reference asserts them at 1e-6)."""
+    bad = []
+    for field, levels in case["expect"].items():
+        for lev, expect in levels.items():
+            got = out[FIELD[field], int(lev)]
+            if expect == 0.0:
+                tol = 1e-12
+            elif printed_ulp(expect) < 1e-11 * abs(expect):  # 12 or more printed digits
+                tol = case["rtol"] * abs(expect)
+            else:
+                tol = min(case["rtol"] * abs(expect), 0.5 * printed_ulp(expect))
+            if not abs(got - expect) <= tol:
+                bad.append((case["name"], field, lev, got, expect, tol))
+    return bad
+
+
 def test_reference_cases(reference_cases):
+    bad = []
     for case in reference_cases:
-        out = ol.flux_one(case_batch(case), 0)
-        for field, levels in case["expect"].items():
-            for lev, expect in levels.items():
-                got = out[FIELD[field], int(lev)]
-                if expect == 0.0:
-                    assert abs(got) < 1e-6, (case["name"], field, lev, got)
-                else:
-                    assert got == pytest.approx(expect, rel=case["rtol"]), (case["name"], field, lev, got, expect)
+        bad += reference_case_failures(case, ol.flux_one(case_batch(case), 0))
+    assert not bad, bad[:5]
 
 
 def test_beer_law_and_no_diffuse():

@@ -11,6 +11,7 @@ import pytest
 import oracle_lib as ol
 from disort_amd import general as G
 from disort_amd.batch import NSCAL, OUT_FIELDS, scalars, uniform_batch
+from conftest import printed_ulp
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CASES = json.load(open(os.path.join(HERE, "golden", "intensity_cases.json")))
@@ -33,7 +34,19 @@ def case_batch(c, n=1):
         tau_user=rep(c["tau_user"]), mu_user=c["mu_user"], phi_user=c["phi_user"], accuracy=c["accuracy"])
 
 
-def check_expect(c, flux, inten, label=""):
+def tight_tol(e):
+    """What a fixture value pins: half a unit in its last printed digit, or
+    1e-8 relative where that is larger (the C-DISORT tables of the reference's
+    tests differ from DisORT++'s own FP64 result by up to 8e-9 relative in
+    their ninth digit); the fixture's absolute tolerance for exact zeros.  The
+    reference's tests assert these tables at 0.05-1% (the fixture's rtol)."""
+    if e["atol"] is not None:
+        return e["atol"]
+    return max(0.5 * printed_ulp(e["v"]), 1e-8 * abs(e["v"]))
+
+
+def expect_failures(c, flux, inten, label=""):
+    bad = []
     for e in c["expect"]:
         if e["q"] == "intensity":
             got = inten[e["lu"], e["iu"], e["j"]]
@@ -41,8 +54,15 @@ def check_expect(c, flux, inten, label=""):
         else:
             got = flux[OUT_FIELDS.index(e["q"]), e["lu"]]
             where = f"{e['q']}[{e['lu']}]"
-        tol = e["atol"] if e["atol"] is not None else e["rtol"] * abs(e["v"])
-        assert abs(got - e["v"]) <= tol, f"{label}{c['name']} {where}: {got!r} vs {e['v']!r} (tol {tol:g})"
+        tol = tight_tol(e)
+        if not abs(got - e["v"]) <= tol:
+            bad.append(f"{label}{c['name']} {where}: {got!r} vs {e['v']!r} (tol {tol:g})")
+    return bad
+
+
+def check_expect(c, flux, inten, label=""):
+    bad = expect_failures(c, flux, inten, label)
+    assert not bad, bad[:5]
 
 
 @pytest.mark.parametrize("c", CASES, ids=[c["name"] for c in CASES])

@@ -13,6 +13,7 @@
 #include <mutex>
 
 #include "../../include/disort_amd.h"
+#include "garcia_siewert.hpp"
 
 namespace disortpp {
 
@@ -25,6 +26,16 @@ template <class F>
 void for_layers(int nlyr, int lc, F&& f) {
   const int lo = lc < 0 ? 0 : lc, hi = lc < 0 ? nlyr - 1 : lc;
   for (int l = lo; l <= hi; ++l) f(l);
+}
+
+// Moments chi_0 = 1, chi_k = beta_k / (2k+1) from a table of expansion
+// coefficients beta_1..beta_count, zero beyond it.
+void fill_tabulated(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, int lc, const double* beta,
+                    int count) {
+  for_layers(nlyr, lc, [&](int l) {
+    m[l][0] = 1.0;
+    for (int k = 1; k <= nmom_nstr; ++k) m[l][k] = k <= count ? beta[k - 1] / (2 * k + 1) : 0.0;
+  });
 }
 }  // namespace
 
@@ -47,6 +58,14 @@ void fillRayleigh(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, 
   });
 }
 
+void fillHazeGarciaSiewert(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, int lc) {
+  fill_tabulated(m, nmom_nstr, nlyr, lc, garcia_siewert::kHaze, garcia_siewert::kHazeCount);
+}
+
+void fillCloudGarciaSiewert(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, int lc) {
+  fill_tabulated(m, nmom_nstr, nlyr, lc, garcia_siewert::kCloud, garcia_siewert::kCloudUsed);
+}
+
 void fillPhaseFunction(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, PhaseFunction type, double g,
                        int lc) {
   switch (type) {
@@ -54,10 +73,9 @@ void fillPhaseFunction(std::vector<std::vector<double>>& m, int nmom_nstr, int n
     case PhaseFunction::Rayleigh: return fillRayleigh(m, nmom_nstr, nlyr, lc);
     case PhaseFunction::HenyeyGreenstein: return fillHenyeyGreenstein(m, nmom_nstr, nlyr, g, lc);
     case PhaseFunction::HazeGarciaSiewert:
+      return fill_tabulated(m, nmom_nstr, nlyr, lc, garcia_siewert::kHaze, garcia_siewert::kHazeCount);
     case PhaseFunction::CloudGarciaSiewert:
-      throw std::invalid_argument(
-          "Garcia-Siewert tabulated phase functions are not bundled with this library; "
-          "set phase_function_moments directly");
+      return fill_tabulated(m, nmom_nstr, nlyr, lc, garcia_siewert::kCloud, garcia_siewert::kCloudUsed);
   }
   throw std::invalid_argument("unknown phase function type");
 }

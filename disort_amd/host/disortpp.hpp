@@ -35,6 +35,8 @@ namespace phase_function {
 void fillHenyeyGreenstein(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, double g, int lc);
 void fillIsotropic(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, int lc);
 void fillRayleigh(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, int lc);
+void fillHazeGarciaSiewert(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, int lc);
+void fillCloudGarciaSiewert(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, int lc);
 void fillPhaseFunction(std::vector<std::vector<double>>& m, int nmom_nstr, int nlyr, PhaseFunction type, double g,
                        int lc);
 }  // namespace phase_function

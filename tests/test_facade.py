@@ -141,7 +141,10 @@ def test_general_config_validation_messages():
         cfg.validate()
 
 
-def test_garcia_siewert_tables_are_not_bundled():
+def test_flux_config_garcia_siewert_fill():
     c = c1_flux_config()
-    with pytest.raises(ValueError, match="Garcia-Siewert"):
-        c.set_haze_garcia_siewert()
+    c.set_haze_garcia_siewert()
+    assert c.phase_function_moments[3][:3] == (1.0, 2.41260 / 3, 3.23047 / 5)
+    c.set_cloud_garcia_siewert(lc=0)
+    assert c.phase_function_moments[0][16] == 13.058 / 33
+

@@ -221,6 +221,14 @@ def test_rpv_thermal_distinct_surfaces(ctx, nstr):
     solve_both(ctx, gb, f"RPV + thermal, distinct surfaces nstr={nstr}")
 
 
+@pytest.mark.parametrize("name", ["disotest_4a", "disotest_4b", "disotest_4c"])
+def test_garcia_siewert_haze(ctx, name):
+    # Haze-L moments from the facade's fillHazeGarciaSiewert, 32 streams.
+    from test_garcia_siewert import CASES, case_batch as gs_batch
+
+    solve_both(ctx, gs_batch(next(c for c in CASES if c[0] == name), n=2), name)
+
+
 def test_validation_messages(ctx):
     from disort_amd._capi import DisortInvalid
 

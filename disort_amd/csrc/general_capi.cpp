@@ -137,6 +137,15 @@ std::string validate_column(const disort_general_batch* b, int64_t i, int* code)
     for (int j = 0; j < b->nphi; ++j)
       if (b->phi_user[j] < 0.0 || b->phi_user[j] > 360.0) return "phi_user[" + to_string(j) + "] must be in [0, 360]";
   }
+  if ((f & DISORT_G_CORR_BURAS) && !(f & DISORT_G_CORR_NAKAJIMA) && b->nphase > 0) {
+    if (b->nphase < 2 || b->nphase > dsa::kMaxPhaseAngles)
+      return "num_phase_func_angles must be in [2, " + to_string(dsa::kMaxPhaseAngles) + "]";
+    if (!b->mu_phase || !b->phase) return "mu_phase_function and phase_function must be set when num_phase_func_angles > 0";
+    for (int it = 0; it < b->nphase; ++it) {
+      if (std::abs(b->mu_phase[it]) > 1.0) return "mu_phase_function[" + to_string(it) + "] out of range [-1, 1]";
+      if (it > 0 && b->mu_phase[it] <= b->mu_phase[it - 1]) return "mu_phase_function must be in increasing order";
+    }
+  }
   if ((f & DISORT_F_DELTA_M_PLUS) && P - 1 < N + 1)
     return "use_delta_m_plus=true requires num_phase_func_moments >= num_streams + 1 (need moments M and M+1)";
   if (b->accuracy < 0.0) return "accuracy_fourier_series must be non-negative (0 = disabled)";
@@ -252,6 +261,18 @@ extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch*
   if (user_tau) ups.push_back({"g_utau", b->tau_user, (size_t)n * ntau * 8, &a.tau_user});
   if (do_user) ups.push_back({"g_umu", b->mu_user, (size_t)numu * 8, &a.mu_user});
   if (nphi) ups.push_back({"g_phi", b->phi_user, (size_t)nphi * 8, &a.phi_user});
+  // Buras-Emde correction: a tabulated phase function with
+  // intensity_corr_buras and without intensity_corr_nakajima
+  // (src/DisortSolver.cpp:369-377).
+  const bool buras = do_user && (f & DISORT_G_CORR_BURAS) && !(f & DISORT_G_CORR_NAKAJIMA) && b->nphase > 0;
+  const int np = buras ? b->nphase : 0;
+  if (buras) {
+    if (np < 2 || np > dsa::kMaxPhaseAngles || !b->mu_phase || !b->phase)
+      return fail(ctx, DISORT_E_INVALID, "tabulated phase function: need 2..2048 angles with mu_phase and phase");
+    ups.push_back({"g_muph", b->mu_phase, (size_t)np * 8, &a.mu_phase});
+    ups.push_back({"g_phase", b->phase, (size_t)n * L * np * 8, &a.phase});
+  }
+  a.nphase = np;
   for (auto& u : ups) {
     if (host) {
       void* p;
@@ -393,7 +414,8 @@ extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch*
   size_t free_b = 0, total_b = 0;
   HIPCHK(ctx, hipMemGetInfo(&free_b, &total_b));
   const size_t budget = std::max<size_t>(std::min<size_t>(free_b / 2, (size_t)24 << 30), (size_t)256 << 20);
-  int64_t nc = (int64_t)std::max<size_t>(1, budget / (per_cm * nm));
+  const size_t be_per_col = buras ? (size_t)ntau * ((size_t)np * 8 + (size_t)dsa::kBeNF * 12 + 8) : 0;
+  int64_t nc = (int64_t)std::max<size_t>(1, budget / (per_cm * nm + be_per_col));
   nc = std::min<int64_t>(nc, n);
   const int64_t nt_max = nc * nm * L;
   void *d_eig, *d_gu, *d_us, *d_pan, *d_cv, *d_uum, *d_uc, *d_srfb, *d_modes;
@@ -413,6 +435,17 @@ extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch*
   a.cvec = (double*)d_cv;
   a.uum = (double*)d_uum;
   a.srf_b = (double*)d_srfb;
+  if (buras) {
+    void *d_ph2, *d_grid, *d_neg, *d_norm;
+    if ((rc = get_buf(ctx, "g_be_ph2", (size_t)nc * ntau * np * 8, &d_ph2))) return rc;
+    if ((rc = get_buf(ctx, "g_be_grid", (size_t)nc * ntau * dsa::kBeNF * 8, &d_grid))) return rc;
+    if ((rc = get_buf(ctx, "g_be_neg", (size_t)nc * ntau * dsa::kBeNF * 4, &d_neg))) return rc;
+    if ((rc = get_buf(ctx, "g_be_norm", (size_t)nc * ntau * 8, &d_norm))) return rc;
+    a.be_ph2 = (double*)d_ph2;
+    a.be_grid = (double*)d_grid;
+    a.be_neg = (int32_t*)d_neg;
+    a.be_norm = (double*)d_norm;
+  }
 
   const bool timed = ctx->timing;
   for (double& v : ctx->gstage_ms) v = 0.0;
@@ -432,10 +465,12 @@ extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch*
   g.modes_used = (int32_t*)d_modes;
   g.brdf = brdf;
   g.do_user = do_user;
-  // Nakajima-Tanaka whenever intensity_corr_buras is set: the tabulated
-  // phase function that selects Buras-Emde instead (src/DisortSolver.cpp:
-  // 369-377, num_phase_func_angles > 0) is not part of this interface.
+  // The intensity correction runs whenever intensity_corr_buras is set (the
+  // kernels apply the rest of the gate per column): Buras-Emde with a
+  // tabulated phase function and without intensity_corr_nakajima,
+  // Nakajima-Tanaka otherwise (src/DisortSolver.cpp:364-377).
   g.nt = do_user && (f & DISORT_G_CORR_BURAS);
+  g.buras = buras;
   g.brdf_corr = do_user && brdf && (f & DISORT_G_CORR_BURAS);
   g.ev = timed ? ctx->gev + 1 : nullptr;
   for (int64_t c0 = 0; c0 < n; c0 += nc) {

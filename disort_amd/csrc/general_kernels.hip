@@ -21,7 +21,9 @@
 //                   user angles and levels
 //   k_gaccum        per column: Fourier sum over the modes (with the
 //                   accuracy_fourier_series cut), azimuthal average
-//   k_gnt, k_gbrdf  Nakajima-Tanaka and BRDF beam-reflection corrections
+//   k_gnt, k_gbe_prep/k_gbe, k_gbrdf
+//                   Nakajima-Tanaka or Buras-Emde, and BRDF beam-reflection
+//                   corrections
 #include "general_layout.hpp"
 #include "wg_common.hpp"
 
@@ -1744,6 +1746,51 @@ __global__ __launch_bounds__(256) void k_gaccum(GenArgs a, int32_t* modes_used) 
   if (threadIdx.x == 0) modes_used[cl] = used;
 }
 
+// Single-scattered beam intensity at scaled depth tu in 1-based layer layru
+// with phase function ph[1..ncut] and albedos om(l) (singleScat,
+// src/DisortSolver.cpp:3287-3328); taus: scaled cumulative depths.
+template <class Om>
+__device__ double single_scat(double dither, int layru, int ncut, const double* ph, Om om, const double* taus,
+                              double um, double mu0, double tu, double F0) {
+  double ans = 0.0, e0 = exp(-tu / mu0);
+  if (fabs(um + mu0) <= dither) {
+    for (int l = 1; l <= layru - 1; ++l) ans += om(l) * ph[l] * (taus[l] - taus[l - 1]);
+    return F0 / (4.0 * kPi * mu0) * e0 * (ans + om(layru) * ph[layru] * (tu - taus[layru - 1]));
+  }
+  if (um > 0.0) {
+    for (int l = layru; l <= ncut; ++l) {
+      const double e1 = exp(-((taus[l] - tu) / um + taus[l] / mu0));
+      ans += om(l) * ph[l] * (e0 - e1);
+      e0 = e1;
+    }
+  } else {
+    for (int l = layru; l >= 1; --l) {
+      const double e1 = exp(-((taus[l - 1] - tu) / um + taus[l - 1] / mu0));
+      ans += om(l) * ph[l] * (e0 - e1);
+      e0 = e1;
+    }
+  }
+  return ans * F0 / (4.0 * kPi * (1.0 + um / mu0));
+}
+
+// xi function (src/DisortSolver.cpp:3274-3285).
+__device__ double xi_func(double u1, double u2, double tau) {
+  const double x1 = (u2 - u1) / (u2 * u1);
+  const double ex1 = exp(-tau / u1);
+  return (x1 != 0.0) ? ((tau * x1 - 1.0) * exp(-tau / u2) + ex1) / (x1 * x1 * u1 * u2)
+                     : tau * tau * ex1 / (2.0 * u1 * u2);
+}
+
+// Gate of the intensity corrections (src/DisortSolver.cpp:364-368): Delta-M
+// (not Delta-M+) applied, beam present, scattering present.
+__device__ bool correction_gate(const GenArgs& a, int64_t i) {
+  const int scal_kind = a.inst_i[i * kInstI + II_SCALING];
+  if (!(scal_kind & 1) || !(a.scal[i * DISORT_NSCAL + DISORT_S_BEAM_FLUX] > 0.0)) return false;
+  bool scat = false;
+  for (int lc = 0; lc < a.nlyr; ++lc) scat = scat || a.ssa[i * a.nlyr + lc] > 0.0;
+  return scat;
+}
+
 // ===========================================================================
 // k_gnt: Nakajima-Tanaka correction (src/DisortSolver.cpp:3274-3524), block
 // per (chunk column, user angle, azimuth); threads over layers, then levels.
@@ -1763,12 +1810,7 @@ __global__ __launch_bounds__(64) void k_gnt(GenArgs a) {
   const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
   const int ncut = a.inst_i[i * kInstI + II_NCUT];
   const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
-  const int scal_kind = a.inst_i[i * kInstI + II_SCALING];
-  bool scat = false;
-  for (int lc = 0; lc < L; ++lc) scat = scat || a.ssa[lo + lc] > 0.0;
-  // Gate (src/DisortSolver.cpp:364-377): Delta-M (not Delta-M+) applied,
-  // beam present, scattering present.
-  if (!((scal_kind & 1) && F0 > 0.0 && scat)) return;
+  if (!correction_gate(a, i)) return;
   double* phast = dyn;
   double* phasm = dyn + (L + 1);
   double* pl = dyn + 2 * (L + 1);
@@ -1814,35 +1856,9 @@ __global__ __launch_bounds__(64) void k_gnt(GenArgs a) {
     if (lyrcut && lyr >= ncut) continue;
     const double tu = a.u_tau[i * a.ntau + lu];
     const int layru = lyr + 1;
-    // single_scat with the two phase functions; om indexing 1-based.
-    double s[2];
-    for (int w = 0; w < 2; ++w) {
-      const double* ph = w == 0 ? phast : phasm;
-      const double dither = 100.0 * kEps;
-      double ans = 0.0, e0 = exp(-tu / mu0);
-      if (fabs(um + mu0) <= dither) {
-        for (int l = 1; l <= layru - 1; ++l) ans += (w == 0 ? om_unscaled(l) : om_scaled(l)) * ph[l] * (taus[l] - taus[l - 1]);
-        ans = F0 / (4.0 * kPi * mu0) * e0 *
-              (ans + (w == 0 ? om_unscaled(layru) : om_scaled(layru)) * ph[layru] * (tu - taus[layru - 1]));
-      } else {
-        if (um > 0.0) {
-          for (int l = layru; l <= ncut; ++l) {
-            const double e1 = exp(-((taus[l] - tu) / um + taus[l] / mu0));
-            ans += (w == 0 ? om_unscaled(l) : om_scaled(l)) * ph[l] * (e0 - e1);
-            e0 = e1;
-          }
-        } else {
-          for (int l = layru; l >= 1; --l) {
-            const double e1 = exp(-((taus[l - 1] - tu) / um + taus[l - 1] / mu0));
-            ans += (w == 0 ? om_unscaled(l) : om_scaled(l)) * ph[l] * (e0 - e1);
-            e0 = e1;
-          }
-        }
-        ans *= F0 / (4.0 * kPi * (1.0 + um / mu0));
-      }
-      s[w] = ans;
-    }
-    double corr = s[0] - s[1];
+    const double dither = 100.0 * kEps;
+    double corr = single_scat(dither, layru, ncut, phast, om_unscaled, taus, um, mu0, tu, F0) -
+                  single_scat(dither, layru, ncut, phasm, om_scaled, taus, um, mu0, tu, F0);
     if (ims && lu >= ltau) {
       // secondary_scat (src/DisortSolver.cpp:3330-3401) with unscaled depths.
       const double tiny = 1.0e-4;
@@ -1877,15 +1893,252 @@ __global__ __launch_bounds__(64) void k_gnt(GenArgs a) {
           pspike += gbar * (2.0 - gbar) * (double)(2 * k + 1) * p;
         }
         const double umu0p = mu0 / (1.0 - fbar * wbar);
-        // xi function (src/DisortSolver.cpp:3274-3285) at (-um, umu0p, ut).
-        const double u1 = -um, u2 = umu0p;
-        const double x1 = (u2 - u1) / (u2 * u1);
-        const double ex1 = exp(-ut / u1);
-        const double xi = (x1 != 0.0) ? ((ut * x1 - 1.0) * exp(-ut / u2) + ex1) / (x1 * x1 * u1 * u2)
-                                      : ut * ut * ex1 / (2.0 * u1 * u2);
-        ims_v = F0 / (4.0 * kPi) * (fbar * wbar) * (fbar * wbar) / (1.0 - fbar * wbar) * pspike * xi;
+        ims_v = F0 / (4.0 * kPi) * (fbar * wbar) * (fbar * wbar) / (1.0 - fbar * wbar) * pspike *
+                xi_func(-um, umu0p, ut);
       }
       corr -= ims_v;
+    }
+    a.inten[((i * a.ntau + lu) * numu + iu) * nphi + jp] += corr;
+  }
+}
+
+// ===========================================================================
+// Buras-Emde correction (src/DisortSolver.cpp:3532-3745, :3812-4019): exact
+// single scattering from the tabulated phase function replaces the Delta-M
+// one (TMS), and near the beam direction a secondary-scattering term from
+// the residual phase function phas2 = (P - P_trunc) / (f omega) is
+// subtracted (IMS).
+// ===========================================================================
+
+// Table interval j of x in the increasing table xx[0..n) -- the bisection of
+// locate (src/Utils.hpp:27-64), clamped to an interval of the table (at x
+// beyond the last entry the reference reads one past the table).
+__device__ int phase_interval(const double* xx, int n, double x) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (x >= xx[mid]) lo = mid;
+    else hi = mid;
+  }
+  if (x == xx[0]) lo = 0;
+  else if (x == xx[n - 1]) lo = n - 2;
+  return lo < 0 ? 0 : (lo > n - 2 ? n - 2 : lo);
+}
+
+// Some downward user angle within 10 degrees of the beam direction.
+__device__ bool aureole_angles(const GenArgs& a, double mu0) {
+  const double deg = kPi / 180.0, theta0 = acos(-mu0) / deg;
+  for (int iu = 0; iu < a.numu; ++iu) {
+    const double um = a.mu_user[iu];
+    if (um < 0.0 && fabs(theta0 - acos(um) / deg) <= 10.0) return true;
+  }
+  return false;
+}
+
+// k_gbe_prep: block per (chunk column, user level).  Threads over the table
+// angles form phas2 before normalisation (sum over the layers above the
+// level of (P - P_trunc) omega dtau, P_trunc the N-term Delta-M truncation
+// not divided by 1 - f); lane 0 then normalises it (1 / fbar, then to unit
+// half-integral 2) and builds the grid equidistant in the cumulative
+// integral of |phas2| (prepDoubleScatIntegr).  Dynamic LDS: nphase doubles.
+__global__ __launch_bounds__(64) void k_gbe_prep(GenArgs a) {
+  extern __shared__ double cum[];
+  const int np = a.nphase, L = a.nlyr, N = a.nstr, P = a.pstride;
+  const int64_t cl = blockIdx.x / a.ntau;
+  const int lu = (int)(blockIdx.x % a.ntau);
+  if (cl >= a.nc) return;
+  const int64_t i = a.c0 + cl;
+  const double F0 = a.scal[i * DISORT_NSCAL + DISORT_S_BEAM_FLUX], mu0 = a.scal[i * DISORT_NSCAL + DISORT_S_BEAM_MU];
+  if (!correction_gate(a, i) || !aureole_angles(a, mu0)) return;  // block-uniform
+  const int64_t lo = i * L, vo = i * (L + 1), TL = a.n * L, TV = a.n * (L + 1);
+  const double* tauc = a.lev_d + VD_TAUC * TV + vo;
+  const int ll = a.u_lyr[i * a.ntau + lu];
+  const double ut = (a.flags & DISORT_G_USER_TAU) ? a.tau_user[i * a.ntau + lu] : tauc[lu];
+  const double* mp = a.mu_phase;
+  const double* tab = a.phase + lo * np;  // [L][np] of this column
+  double* ph = a.be_ph2 + (cl * a.ntau + lu) * np;
+  for (int it = threadIdx.x; it < np; it += blockDim.x) {
+    const double x = mp[it];
+    double acc = 0.0;
+    for (int lc = 0; lc <= ll; ++lc) {
+      const double fl = a.lay_d[LD_F * TL + lo + lc];
+      const double* pm = a.pmom + (lo + lc) * P;
+      double tr = 1.0 - fl, p1 = 1.0, p2 = 0.0;
+      for (int k = 1; k <= N - 1; ++k) {
+        const double p = ((2 * k - 1) * x * p1 - (k - 1) * p2) / k;
+        p2 = p1;
+        p1 = p;
+        tr += (2 * k + 1) * p * (pm[k] - fl);
+      }
+      const double w = (lc < ll) ? a.dtau[lo + lc] : ut - tauc[ll];
+      acc += (tab[(int64_t)lc * np + it] - tr) * a.ssa[lo + lc] * w;
+    }
+    ph[it] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double tiny = 1.0e-4;
+  double fbar = a.lay_d[LD_F * TL + lo + ll] * a.ssa[lo + ll] * (ut - tauc[ll]);
+  for (int lc = 0; lc < ll; ++lc) fbar += a.ssa[lo + lc] * a.dtau[lo + lc] * a.lay_d[LD_F * TL + lo + lc];
+  const double sc = (fbar <= tiny || F0 <= tiny) ? 0.0 : 1.0 / fbar;
+  for (int it = 0; it < np; ++it) ph[it] *= sc;
+  double area = 0.0;
+  for (int it = 1; it < np; ++it) area += (mp[it] - mp[it - 1]) * 0.5 * (ph[it] + ph[it - 1]);
+  if (area != 0.0) {
+    const double nrm = 2.0 / area;
+    for (int it = 0; it < np; ++it) ph[it] *= nrm;
+  }
+  cum[0] = 0.0;
+  for (int it = 1; it < np; ++it) cum[it] = cum[it - 1] + (mp[it] - mp[it - 1]) * 0.5 * (fabs(ph[it]) + fabs(ph[it - 1]));
+  double* grid = a.be_grid + (cl * a.ntau + lu) * kBeNF;
+  int32_t* neg = a.be_neg + (cl * a.ntau + lu) * kBeNF;
+  const double df = cum[np - 1] / (kBeNF - 1);
+  grid[0] = -1.0;
+  neg[0] = ph[0] > 0.0 ? 0 : 1;
+  double target = 0.0;
+  int it = 0;
+  for (int q = 1; q < kBeNF - 1; ++q) {
+    target += df;
+    while (it < np - 2 && cum[it + 1] < target) ++it;
+    const double w = (target - cum[it]) / (cum[it + 1] - cum[it]);
+    grid[q] = mp[it] + w * (mp[it + 1] - mp[it]);
+    if (ph[it] > 0.0 && ph[it + 1] > 0.0) neg[q] = 0;
+    else if (ph[it] < 0.0 && ph[it + 1] < 0.0) neg[q] = 1;
+    else neg[q] = (ph[it] + w * (ph[it + 1] - ph[it])) > 0.0 ? 0 : 1;
+  }
+  grid[kBeNF - 1] = 1.0;
+  neg[kBeNF - 1] = ph[np - 1] > 0.0 ? 0 : 1;
+  a.be_norm[cl * a.ntau + lu] = cum[np - 1] / ((kBeNF - 1) * kPi);
+}
+
+// p" * p" at scattering-angle cosine ct: azimuthal integral of the
+// piecewise-linear phas2 over the equal-area grid (calcPhaseSquared).
+__device__ double phase_squared(const double* mp, int np, const double* ph, const double* grid, const int32_t* neg,
+                                double norm, double ct) {
+  const double st = sqrt(1.0 - ct * ct);
+  double sum = 0.0;
+  for (int j = 0; j < kBeNF; ++j) {
+    const double u = grid[j];
+    double phint = 0.0;
+    if (ct == 1.0 || u == 1.0) {
+      const int k = phase_interval(mp, np, u * ct);
+      phint = kPi * (ph[k] + (u * ct - mp[k]) / (mp[k + 1] - mp[k]) * (ph[k + 1] - ph[k]));
+      if (ct == 1.0) phint /= 2.0;
+    } else {
+      const double su = sqrt(1.0 - u * u);
+      const double lo = ct * u - st * su;
+      double hi = ct * u + st * su;
+      const bool cut = u < hi;
+      if (cut) hi = u;
+      if (lo < hi) {
+        const int kmin = phase_interval(mp, np, lo), kmax = phase_interval(mp, np, hi);
+        double D = (ph[kmin + 1] - ph[kmin]) / (mp[kmin + 1] - mp[kmin]);
+        double C = ph[kmin] - mp[kmin] * D;
+        phint += (D * ct * u + C) * kPi / 2.0;
+        for (int k = kmin + 1; k <= kmax; ++k) {
+          const double D2 = (ph[k + 1] - ph[k]) / (mp[k + 1] - mp[k]);
+          const double C2 = ph[k] - mp[k] * D2;
+          phint += (D2 - D) * sqrt(1.0 - ct * ct - u * u + 2.0 * ct * u * mp[k] - mp[k] * mp[k]) +
+                   ((D2 - D) * ct * u + C2 - C) * asin((ct * u - mp[k]) / (su * st));
+          D = D2;
+          C = C2;
+        }
+        if (cut)
+          phint += -D * sqrt(1.0 - ct * ct + 2.0 * u * u * (ct - 1.0)) - (D * ct * u + C) * asin((ct - 1.0) * u / (su * st));
+        else
+          phint += (D * ct * u + C) * kPi / 2.0;
+      }
+    }
+    const double w = (j == 0 || j == kBeNF - 1) ? 0.5 : 1.0;
+    sum += (neg[j] ? -w : w) * phint;
+  }
+  return sum * norm;
+}
+
+// k_gbe: block per (chunk column, user angle, azimuth), threads over layers
+// (the two phase functions at the scattering angle), then over levels.
+// Dynamic LDS: phast[L+1], phasm[L+1], pl[N].
+__global__ __launch_bounds__(64) void k_gbe(GenArgs a) {
+  extern __shared__ double dyn[];
+  const int L = a.nlyr, numu = a.numu, nphi = a.nphi, P = a.pstride, N = a.nstr, np = a.nphase;
+  const int64_t bk = blockIdx.x;
+  const int64_t cl = bk / ((int64_t)numu * nphi);
+  const int iu = (int)(bk / nphi % numu), jp = (int)(bk % nphi);
+  if (cl >= a.nc) return;
+  const int64_t i = a.c0 + cl;
+  if (!correction_gate(a, i)) return;
+  const int64_t lo = i * L, vo = i * (L + 1), TL = a.n * L, TV = a.n * (L + 1);
+  const double* sc = a.scal + i * DISORT_NSCAL;
+  const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
+  const int ncut = a.inst_i[i * kInstI + II_NCUT];
+  const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
+  double* phast = dyn;
+  double* phasm = dyn + (L + 1);
+  double* pl = dyn + 2 * (L + 1);
+  const double deg = kPi / 180.0;
+  const double um = a.mu_user[iu];
+  const double phirad = (a.phi_user[jp] - sc[DISORT_S_BEAM_PHI]) * deg;
+  const double ct = -mu0 * um + sqrt((1.0 - mu0 * mu0) * (1.0 - um * um)) * cos(phirad);
+  const double* mp = a.mu_phase;
+  const int kt = phase_interval(mp, np, ct);
+  if (threadIdx.x == 0) {
+    double p1 = 1.0, p2 = 0.0;
+    pl[0] = 1.0;
+    for (int k = 1; k <= N - 1; ++k) {
+      const double p = ((2 * k - 1) * ct * p1 - (k - 1) * p2) / k;
+      p2 = p1;
+      p1 = p;
+      pl[k] = p;
+    }
+  }
+  __syncthreads();
+  for (int lc = 1 + threadIdx.x; lc <= ncut; lc += blockDim.x) {
+    const double* tab = a.phase + (lo + lc - 1) * np;
+    const double fl = a.lay_d[LD_F * TL + lo + lc - 1];
+    const double pa = tab[kt] + (ct - mp[kt]) / (mp[kt + 1] - mp[kt]) * (tab[kt + 1] - tab[kt]);
+    phast[lc] = pa / (1.0 - fl * a.ssa[lo + lc - 1]);
+    const double* pm = a.pmom + (lo + lc - 1) * P;
+    double pmm = 1.0;
+    for (int k = 1; k <= N - 1; ++k) pmm += (2.0 * k + 1) * pl[k] * (pm[k] - fl) / (1.0 - fl);
+    phasm[lc] = pmm;
+  }
+  __syncthreads();
+  const double* taus = a.lev_d + VD_TAU_S * TV + vo;
+  const double* tauc = a.lev_d + VD_TAUC * TV + vo;
+  auto om_unscaled = [&](int l) { return a.ssa[lo + l - 1]; };
+  auto om_scaled = [&](int l) { return a.lay_d[LD_SSA_S * TL + lo + l - 1]; };
+  const bool ims = um < 0.0 && fabs(acos(-mu0) / deg - acos(um) / deg) <= 10.0;
+  const double u0 = (a.flags & DISORT_G_USER_TAU) ? a.tau_user[i * a.ntau] : tauc[0];
+  const int ltau = (u0 <= 100.0 * kEps) ? 1 : 0;
+  const double dither = 100.0 * kEps;
+  for (int lu = threadIdx.x; lu < a.ntau; lu += blockDim.x) {
+    const int lyr = a.u_lyr[i * a.ntau + lu];
+    if (lyrcut && lyr >= ncut) continue;
+    const double tu = a.u_tau[i * a.ntau + lu];
+    double corr = single_scat(dither, lyr + 1, ncut, phast, om_unscaled, taus, um, mu0, tu, F0) -
+                  single_scat(dither, lyr + 1, ncut, phasm, om_scaled, taus, um, mu0, tu, F0);
+    if (ims && lu >= ltau) {
+      // newSecondaryScat (src/DisortSolver.cpp:3905-3945), unscaled depths.
+      const double tiny = 1.0e-4;
+      const double ut = (a.flags & DISORT_G_USER_TAU) ? a.tau_user[i * a.ntau + lu] : tauc[lu];
+      const double dtl = ut - tauc[lyr];
+      double wbar = a.ssa[lo + lyr] * dtl, fbar = a.lay_d[LD_F * TL + lo + lyr] * wbar, stau = dtl;
+      for (int l = 0; l < lyr; ++l) {
+        wbar += a.dtau[lo + l] * a.ssa[lo + l];
+        fbar += a.dtau[lo + l] * a.ssa[lo + l] * a.lay_d[LD_F * TL + lo + l];
+        stau += a.dtau[lo + l];
+      }
+      if (!(wbar <= tiny || fbar <= tiny || stau <= tiny || F0 <= tiny)) {
+        fbar /= wbar;
+        wbar /= stau;
+        const int64_t r = cl * a.ntau + lu;
+        const double* ph = a.be_ph2 + r * np;
+        const double p1 = ph[kt] + (ct - mp[kt]) / (mp[kt + 1] - mp[kt]) * (ph[kt + 1] - ph[kt]);
+        const double pspike =
+            2.0 * p1 - phase_squared(mp, np, ph, a.be_grid + r * kBeNF, a.be_neg + r * kBeNF, a.be_norm[r], ct);
+        const double fw = fbar * wbar;
+        corr -= F0 / (4.0 * kPi) * fw * fw / (1.0 - fw) * pspike * xi_func(-um, mu0 / (1.0 - fw), ut);
+      }
     }
     a.inten[((i * a.ntau + lu) * numu + iu) * nphi + jp] += corr;
   }
@@ -1969,7 +2222,12 @@ void general_chunk_n(const GenArgs& a, const GenLaunch& g, hipStream_t s) {
     rec(6);
     hipLaunchKernelGGL(k_gaccum, dim3((unsigned)a.nc), dim3(256), 0, s, a, g.modes_used);
     rec(7);
-    if (g.nt) {
+    if (g.nt && g.buras) {
+      hipLaunchKernelGGL(k_gbe_prep, dim3((unsigned)(a.nc * a.ntau)), dim3(64), sizeof(double) * (size_t)a.nphase,
+                         s, a);
+      const size_t lds = sizeof(double) * (2 * (size_t)(a.nlyr + 1) + (size_t)a.nstr);
+      hipLaunchKernelGGL(k_gbe, dim3((unsigned)(a.nc * a.numu * a.nphi)), dim3(64), lds, s, a);
+    } else if (g.nt) {
       const size_t lds = sizeof(double) * (2 * (size_t)(a.nlyr + 1) + (size_t)a.pstride);
       hipLaunchKernelGGL(k_gnt, dim3((unsigned)(a.nc * a.numu * a.nphi)), dim3(64), lds, s, a);
     }

@@ -93,13 +93,25 @@ struct GenArgs {
   double* cvec;   // [n_tasks][N] integration constants
   double* uum;    // [nc][nm][ntau][numu] intensity Fourier components
   int32_t* status;
+  // Buras-Emde correction: the tabulated phase function and, per (chunk
+  // column, user level), the residual phase function and its equal-area
+  // integration grid (kBeNF points).
+  const double* mu_phase;  // [nphase] increasing
+  const double* phase;     // [n][nlyr][nphase]
+  int32_t nphase, pad1;
+  double* be_ph2;          // [nc][ntau][nphase]
+  double* be_grid;         // [nc][ntau][kBeNF]
+  int32_t* be_neg;         // [nc][ntau][kBeNF]
+  double* be_norm;         // [nc][ntau]
 };
+constexpr int kBeNF = 100;
+constexpr int kMaxPhaseAngles = 2048;
 
 // Per-chunk launch options of the general solver.
 struct GenLaunch {
   double* ucoef;        // [n_tasks][4][numu] layer coefficients of the user intensities
   int32_t* modes_used;  // [nc] Fourier modes summed per column
-  bool brdf, do_user, nt, brdf_corr;
+  bool brdf, do_user, nt, buras, brdf_corr;  // nt: intensity correction on; buras: Buras-Emde form
   hipEvent_t* ev;       // 9 events for stage timing, or null
 };
 

@@ -53,6 +53,10 @@ class CGeneralBatch(C.Structure):
         ("inten", C.c_void_p),
         ("inten_avg", C.c_void_p),
         ("status", C.c_void_p),
+        ("nphase", C.c_int32),
+        ("reserved2", C.c_int32),
+        ("mu_phase", C.c_void_p),
+        ("phase", C.c_void_p),
     ]
 
 
@@ -89,6 +93,8 @@ class GeneralBatch:
     mu_user: np.ndarray | None = None
     phi_user: np.ndarray | None = None
     accuracy: float = 0.0
+    mu_phase: np.ndarray | None = None  # [nphase] tabulated phase function angles (Buras-Emde)
+    phase: np.ndarray | None = None     # [n, L, nphase]
     ntau: int = field(init=False)
     numu: int = field(init=False)
     nphi: int = field(init=False)
@@ -108,6 +114,9 @@ class GeneralBatch:
         self.rpv = _f64(self.rpv, None if self.rpv is None else (n, NRPV))
         self.mu_user = _f64(self.mu_user)
         self.phi_user = _f64(self.phi_user)
+        self.mu_phase = _f64(self.mu_phase)
+        self.nphase = 0 if self.mu_phase is None else int(self.mu_phase.size)
+        self.phase = _f64(self.phase, None if self.phase is None else (n, L, self.nphase))
         if self.flags & G_USER_TAU:
             self.tau_user = _f64(self.tau_user)
             if self.tau_user.ndim == 1:
@@ -157,8 +166,9 @@ class GeneralBatch:
         s.reserved = 0
         s.accuracy = float(self.accuracy)
         for name in ("dtau", "ssa", "pmom", "temper", "alt", "scal", "rpv", "tau_user", "mu_user", "phi_user",
-                     "flux", "inten", "inten_avg", "status"):
+                     "flux", "inten", "inten_avg", "status", "mu_phase", "phase"):
             setattr(s, name, _ptr(getattr(self, name)))
+        s.nphase = self.nphase
         return s
 
     def fluxes(self, i: int) -> dict:
@@ -184,7 +194,8 @@ class DeviceGeneralBatch:
             ctx.h2d(p, np.ascontiguousarray(a))
             return p
 
-        for name in ("dtau", "ssa", "pmom", "temper", "alt", "scal", "rpv", "tau_user", "mu_user", "phi_user"):
+        for name in ("dtau", "ssa", "pmom", "temper", "alt", "scal", "rpv", "tau_user", "mu_user", "phi_user",
+                     "mu_phase", "phase"):
             setattr(st, name, up(getattr(gb, name)))
         for name in ("flux", "inten", "inten_avg", "status"):
             a = getattr(gb, name)

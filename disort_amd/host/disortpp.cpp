@@ -495,7 +495,7 @@ void DisortConfig::allocate() {
     if (bottom_radius <= 0.0) throw std::invalid_argument("bottom_radius must be positive when use_spherical_beam=true");
     level_altitudes.resize(num_layers + 1, 0.0);
   }
-  if (flags.intensity_corr_buras && !flags.intensity_corr_nakajima && num_phase_func_angles > 0) {
+  if (num_phase_func_angles > 0) {
     mu_phase_function.resize(num_phase_func_angles, 0.0);
     phase_function.assign(num_layers, std::vector<double>(num_phase_func_angles, 0.0));
   }
@@ -596,6 +596,7 @@ void reverse_general_inputs(DisortConfig& c) {
   std::reverse(c.phase_function_moments.begin(), c.phase_function_moments.end());
   std::reverse(c.temperature.begin(), c.temperature.end());
   std::reverse(c.level_altitudes.begin(), c.level_altitudes.end());
+  if (!c.phase_function.empty()) std::reverse(c.phase_function.begin(), c.phase_function.end());
   if (c.flags.use_user_tau && !c.tau_user.empty()) {
     double total = 0.0;
     for (double d : c.delta_tau) total += d;
@@ -619,9 +620,6 @@ void require_gpu_subset(const DisortConfig& c) {
     if (c.flags.brdf_type != BrdfType::RPV) no("a BRDF other than RPV");
     if (!c.brdf.rpv) throw std::runtime_error("bidirReflectivity: RPV BRDF selected but rpv spec is null");
   }
-  if (c.flags.intensity_corr_buras && !c.flags.intensity_corr_nakajima && c.num_phase_func_angles > 0 &&
-      !c.flags.comp_only_fluxes)
-    no("the Buras-Emde intensity correction (tabulated phase functions)");
   if (c.flags.output_fourier_expansion) no("output_fourier_expansion");
 }
 
@@ -705,8 +703,15 @@ struct GeneralPack {
   int N = 0, L = 0, P = 0, ntau = 0, numu = 0, nphi = 0;
   double accuracy = 0.0;
   std::vector<double> dtau, ssa, pmom, temper, alt, scal, rpv, tau_user, mu_user, phi_user, flux, inten, avg;
+  std::vector<double> mu_phase, phase;  // tabulated phase function (Buras-Emde correction only)
   std::vector<int32_t> status;
   int64_t n = 0;
+
+  // The tabulated phase function enters only the Buras-Emde correction.
+  static bool uses_phase_table(const DisortConfig& c) {
+    return c.flags.intensity_corr_buras && !c.flags.intensity_corr_nakajima && c.num_phase_func_angles > 0 &&
+           !c.flags.comp_only_fluxes && c.flags.use_user_mu;
+  }
 
   // Whether config c can join this batch.
   bool fits(const DisortConfig& c) const {
@@ -715,7 +720,11 @@ struct GeneralPack {
            c.num_user_tau == ntau && c.num_user_mu == numu && (c.flags.comp_only_fluxes ? 0 : c.num_phi) == nphi &&
            c.accuracy_fourier_series == accuracy &&
            std::equal(mu_user.begin(), mu_user.end(), c.mu_user.begin()) &&
-           std::equal(phi_user.begin(), phi_user.end(), c.phi_user.begin());
+           std::equal(phi_user.begin(), phi_user.end(), c.phi_user.begin()) &&
+           (uses_phase_table(c) ? c.num_phase_func_angles == (int)mu_phase.size() &&
+                                      c.mu_phase_function.size() >= mu_phase.size() &&
+                                      std::equal(mu_phase.begin(), mu_phase.end(), c.mu_phase_function.begin())
+                                : mu_phase.empty());
   }
 
   void add(const DisortConfig& cin) {
@@ -732,6 +741,21 @@ struct GeneralPack {
       accuracy = c.accuracy_fourier_series;
       mu_user.assign(c.mu_user.begin(), c.mu_user.begin() + numu);
       phi_user.assign(c.phi_user.begin(), c.phi_user.begin() + nphi);
+      if (uses_phase_table(c)) {
+        if ((int)c.mu_phase_function.size() < c.num_phase_func_angles)
+          throw std::invalid_argument("mu_phase_function not allocated for num_phase_func_angles");
+        mu_phase.assign(c.mu_phase_function.begin(), c.mu_phase_function.begin() + c.num_phase_func_angles);
+      }
+    }
+    if (!mu_phase.empty()) {
+      const size_t np = mu_phase.size();
+      if (c.mu_phase_function.size() < np || c.phase_function.size() < (size_t)L)
+        throw std::invalid_argument("mu_phase_function / phase_function not allocated for num_phase_func_angles");
+      for (int l = 0; l < L; ++l) {
+        if (c.phase_function[l].size() < np)
+          throw std::invalid_argument("phase_function[" + std::to_string(l) + "] has fewer than num_phase_func_angles values");
+        phase.insert(phase.end(), c.phase_function[l].begin(), c.phase_function[l].begin() + np);
+      }
     }
     dtau.insert(dtau.end(), c.delta_tau.begin(), c.delta_tau.begin() + L);
     ssa.insert(ssa.end(), c.single_scat_albedo.begin(), c.single_scat_albedo.begin() + L);
@@ -785,6 +809,9 @@ struct GeneralPack {
     b.inten = inten.empty() ? nullptr : inten.data();
     b.inten_avg = avg.data();
     b.status = status.data();
+    b.nphase = (int32_t)mu_phase.size();
+    b.mu_phase = mu_phase.empty() ? nullptr : mu_phase.data();
+    b.phase = phase.empty() ? nullptr : phase.data();
     Gpu& g = gpu();
     std::lock_guard<std::mutex> lock(g.mu);
     if (!g.ctx) {

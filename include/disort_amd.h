@@ -299,6 +299,14 @@ typedef struct disort_general_batch {
   double* inten;        /* out [n][ntau][numu][nphi] (DisortResult::intensity)   */
   double* inten_avg;    /* out [n][ntau][numu] (intensity_azimuth_avg)           */
   int32_t* status;      /* out [n], may be NULL                                  */
+  /* Tabulated phase function of the Buras-Emde correction
+   * (DisortConfig::num_phase_func_angles, mu_phase_function, phase_function):
+   * used when DISORT_G_CORR_BURAS is set, DISORT_G_CORR_NAKAJIMA is not and
+   * nphase > 0 (src/DisortSolver.cpp:364-377); otherwise ignored. */
+  int32_t nphase;       /* num_phase_func_angles (0: none)                       */
+  int32_t reserved2;
+  const double* mu_phase; /* [nphase] scattering-angle cosines, increasing, shared */
+  const double* phase;    /* [n][nlyr][nphase] phase function values              */
 } disort_general_batch;
 
 /*
@@ -315,9 +323,10 @@ typedef struct disort_general_batch {
  * plane-parallel or pseudo-spherical beam; thermal emission; Delta-M /
  * Delta-M+; layer cut-off; diffusion lower boundary; accuracy_fourier_series;
  * with DISORT_G_CORR_BURAS the Nakajima-Tanaka correction
- * (src/DisortSolver.cpp:3403-3524) and the BRDF beam-reflection correction
- * (:3751-3808).  Not here: SPECIAL_BC, the Hapke / Cox-Munk / Ross-Li BRDFs,
- * the Buras-Emde correction (it needs tabulated phase functions), temperature
+ * (src/DisortSolver.cpp:3403-3524) or, with a tabulated phase function and
+ * without DISORT_G_CORR_NAKAJIMA, the Buras-Emde correction (:3532-3745,
+ * :3812-4019), and the BRDF beam-reflection correction (:3751-3808).  Not
+ * here: SPECIAL_BC, the Hapke / Cox-Munk / Ross-Li BRDFs, temperature
  * Jacobians.  All modes of every column are solved in parallel; the
  * accuracy_fourier_series cut is applied when they are summed, which gives
  * the reference's result.  Stage times (disort_stage_times, 8 stages):

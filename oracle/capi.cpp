@@ -202,6 +202,11 @@ int oracle_general_one(const disort_general_batch* b, int64_t i, int index_from_
     if (c.use_user_tau && b->tau_user) c.tau_user.assign(b->tau_user + i * b->ntau, b->tau_user + (i + 1) * b->ntau);
     if (c.use_user_mu && b->mu_user) c.mu_user.assign(b->mu_user, b->mu_user + b->numu);
     if (b->phi_user) c.phi_user.assign(b->phi_user, b->phi_user + b->nphi);
+    if (b->nphase > 0 && b->mu_phase && b->phase) {
+      c.nphase = b->nphase;
+      c.mu_phase.assign(b->mu_phase, b->mu_phase + b->nphase);
+      c.phase.assign(b->phase + (size_t)i * L * b->nphase, b->phase + (size_t)(i + 1) * L * b->nphase);
+    }
     oracle::GeneralOutput o = oracle::general_solve(c);
     for (int f = 0; f < 8; ++f) std::memcpy(flux + (size_t)f * o.ntau, o.f[f].data(), sizeof(double) * o.ntau);
     if (inten) std::memcpy(inten, o.inten.data(), sizeof(double) * o.inten.size());

@@ -1116,6 +1116,221 @@ struct GeneralSolver {
     }
   }
 
+  // ---- Buras-Emde correction (src/DisortSolver.cpp:3532-3745, :3812-4019) ----
+  // Tabulated phase function of layer lc at table angle it (0-based).
+  Real tab(int lc, int it) const { return c.phase[(size_t)lc * c.nphase + it]; }
+  // Table interval of x, clamped to an interval of the table (the
+  // reference's locate; at x beyond the last angle the reference reads one
+  // past the table, which the clamp replaces by the last interval).
+  int interval(Real x) const {
+    int j = locate_sorted(c.mu_phase, x);
+    if (j < 0) j = 0;
+    if (j > c.nphase - 2) j = c.nphase - 2;
+    return j;
+  }
+
+  // Residual phase function phas2 [ntau][nphase] normalised to integrate to
+  // 2, and its equal-|area| integration grid (prepDoubleScatIntegr).
+  struct Residual {
+    std::vector<Real> phas2, mu_eq, norm;
+    std::vector<int> neg;
+  };
+  static constexpr int kNF = 100;
+
+  Residual residual_phase() const {
+    const int np = c.nphase;
+    const Real tiny = 1.0e-4;
+    Residual r;
+    r.phas2.assign((size_t)ntau * np, 0.0);
+    std::vector<Real> trunc(L);
+    for (int it = 0; it < np; ++it) {
+      // Delta-M truncated phase function (not divided by 1 - f), N terms.
+      const Real x = c.mu_phase[it];
+      for (int lc = 0; lc < L; ++lc) trunc[lc] = 1.0 - fwd[lc];
+      Real p1 = 1.0, p2 = 0.0;
+      for (int k = 1; k <= N - 1; ++k) {
+        const Real p = ((2 * k - 1) * x * p1 - (k - 1) * p2) / k;
+        p2 = p1;
+        p1 = p;
+        for (int lc = 0; lc < L; ++lc) trunc[lc] += (2 * k + 1) * p * (c.pm(k, lc) - fwd[lc]);
+      }
+      for (int lu = 0; lu < ntau; ++lu) {
+        const int ll = lyr_of[lu];
+        Real acc = 0.0;
+        for (int lc = 0; lc < ll; ++lc) acc += (tab(lc, it) - trunc[lc]) * c.ssa[lc] * c.dtau[lc];
+        acc += (tab(ll, it) - trunc[ll]) * c.ssa[ll] * (utau[lu] - tauc[ll]);
+        r.phas2[(size_t)lu * np + it] = acc;
+      }
+    }
+    r.mu_eq.assign((size_t)ntau * kNF, 0.0);
+    r.neg.assign((size_t)ntau * kNF, 0);
+    r.norm.assign(ntau, 0.0);
+    std::vector<Real> cum(np);
+    for (int lu = 0; lu < ntau; ++lu) {
+      Real* ph = &r.phas2[(size_t)lu * np];
+      const int ll = lyr_of[lu];
+      Real fbar = fwd[ll] * c.ssa[ll] * (utau[lu] - tauc[ll]);
+      for (int lc = 0; lc < ll; ++lc) fbar += c.ssa[lc] * c.dtau[lc] * fwd[lc];
+      const Real scale = (fbar <= tiny || c.beam_flux <= tiny) ? 0.0 : 1.0 / fbar;
+      for (int it = 0; it < np; ++it) ph[it] *= scale;
+      Real area = 0.0;
+      for (int it = 1; it < np; ++it) area += (c.mu_phase[it] - c.mu_phase[it - 1]) * 0.5 * (ph[it] + ph[it - 1]);
+      if (area != 0.0)
+        for (int it = 0; it < np; ++it) ph[it] *= 2.0 / area;
+      // Grid equidistant in the cumulative integral of |phas2|.
+      cum[0] = 0.0;
+      for (int it = 1; it < np; ++it)
+        cum[it] = cum[it - 1] + (c.mu_phase[it] - c.mu_phase[it - 1]) * 0.5 * (std::abs(ph[it]) + std::abs(ph[it - 1]));
+      Real* me = &r.mu_eq[(size_t)lu * kNF];
+      int* ng = &r.neg[(size_t)lu * kNF];
+      const Real df = cum[np - 1] / (kNF - 1);
+      me[0] = -1.0;
+      ng[0] = ph[0] > 0.0 ? 0 : 1;
+      Real target = 0.0;
+      int it = 0;
+      for (int i = 1; i < kNF - 1; ++i) {
+        target += df;
+        while (it < np - 2 && cum[it + 1] < target) ++it;
+        const Real w = (target - cum[it]) / (cum[it + 1] - cum[it]);
+        me[i] = c.mu_phase[it] + w * (c.mu_phase[it + 1] - c.mu_phase[it]);
+        if (ph[it] > 0.0 && ph[it + 1] > 0.0) ng[i] = 0;
+        else if (ph[it] < 0.0 && ph[it + 1] < 0.0) ng[i] = 1;
+        else ng[i] = (ph[it] + w * (ph[it + 1] - ph[it])) > 0.0 ? 0 : 1;
+      }
+      me[kNF - 1] = 1.0;
+      ng[kNF - 1] = ph[np - 1] > 0.0 ? 0 : 1;
+      r.norm[lu] = cum[np - 1] / ((kNF - 1) * M_PI);
+    }
+    return r;
+  }
+
+  // Azimuthal integral of phas2(mu_1) phas2(theta_12) over the equal-area
+  // grid (calcPhaseSquared).
+  Real phase_squared(const Residual& r, int lu, Real ct) const {
+    const int np = c.nphase;
+    const Real* ph = &r.phas2[(size_t)lu * np];
+    const Real* me = &r.mu_eq[(size_t)lu * kNF];
+    const int* ng = &r.neg[(size_t)lu * kNF];
+    const Real st = std::sqrt(1.0 - ct * ct);
+    const Real* mp = c.mu_phase.data();
+    Real sum = 0.0;
+    for (int j = 0; j < kNF; ++j) {
+      const Real u = me[j];
+      Real phint = 0.0;
+      if (ct == 1.0 || u == 1.0) {
+        const int k = interval(u * ct);
+        phint = M_PI * (ph[k] + (u * ct - mp[k]) / (mp[k + 1] - mp[k]) * (ph[k + 1] - ph[k]));
+        if (ct == 1.0) phint /= 2.0;
+      } else {
+        const Real su = std::sqrt(1.0 - u * u);
+        const Real lo = ct * u - st * su;
+        Real hi = ct * u + st * su;
+        const bool cut = u < hi;
+        if (cut) hi = u;
+        if (lo < hi) {
+          const int kmin = interval(lo), kmax = interval(hi);
+          // Piecewise-linear phas2 = D mu + C on each table interval.
+          Real D = (ph[kmin + 1] - ph[kmin]) / (mp[kmin + 1] - mp[kmin]);
+          Real C = ph[kmin] - mp[kmin] * D;
+          phint += (D * ct * u + C) * M_PI / 2.0;
+          for (int k = kmin + 1; k <= kmax; ++k) {
+            const Real D2 = (ph[k + 1] - ph[k]) / (mp[k + 1] - mp[k]);
+            const Real C2 = ph[k] - mp[k] * D2;
+            phint += (D2 - D) * std::sqrt(1.0 - ct * ct - u * u + 2.0 * ct * u * mp[k] - mp[k] * mp[k]) +
+                     ((D2 - D) * ct * u + C2 - C) * std::asin((ct * u - mp[k]) / (su * st));
+            D = D2;
+            C = C2;
+          }
+          if (cut)
+            phint += -D * std::sqrt(1.0 - ct * ct + 2.0 * u * u * (ct - 1.0)) -
+                     (D * ct * u + C) * std::asin((ct - 1.0) * u / (su * st));
+          else
+            phint += (D * ct * u + C) * M_PI / 2.0;
+        }
+      }
+      const Real w = (j == 0 || j == kNF - 1) ? 0.5 : 1.0;
+      sum += (ng[j] ? -w : w) * phint;
+    }
+    return sum * r.norm[lu];
+  }
+
+  // IMS term of the Buras-Emde correction (newSecondaryScat), lu and iu 0-based.
+  Real buras_secondary(const Residual& r, int iu, int lu, int k, Real ct) const {
+    const Real tiny = 1.0e-4;
+    const int ll = lyr_of[lu];
+    const Real dt = utau[lu] - tauc[ll];
+    Real wbar = c.ssa[ll] * dt, fbar = fwd[ll] * wbar, stau = dt;
+    for (int lc = 0; lc < ll; ++lc) {
+      wbar += c.dtau[lc] * c.ssa[lc];
+      fbar += c.dtau[lc] * c.ssa[lc] * fwd[lc];
+      stau += c.dtau[lc];
+    }
+    if (wbar <= tiny || fbar <= tiny || stau <= tiny || c.beam_flux <= tiny) return 0.0;
+    fbar /= wbar;
+    wbar /= stau;
+    const Real* ph = &r.phas2[(size_t)lu * c.nphase];
+    const Real* mp = c.mu_phase.data();
+    const Real p1 = ph[k] + (ct - mp[k]) / (mp[k + 1] - mp[k]) * (ph[k + 1] - ph[k]);
+    const Real pspike = 2.0 * p1 - phase_squared(r, lu, ct);
+    const Real fw = fbar * wbar;
+    const Real umu0p = c.beam_mu / (1.0 - fw);
+    return c.beam_flux / (4.0 * M_PI) * fw * fw / (1.0 - fw) * pspike * xi_func(-umu[iu], umu0p, utau[lu]);
+  }
+
+  void buras_correction(GeneralOutput& out) {
+    const Real dither = 100.0 * std::numeric_limits<double>::epsilon();
+    const Real dtheta = 10.0, deg = M_PI / 180.0;
+    const Real theta0 = std::acos(-c.beam_mu) / deg;
+    bool aureole = false;
+    for (int iu = 0; iu < numu; ++iu)
+      if (umu[iu] < 0.0 && std::abs(theta0 - std::acos(umu[iu]) / deg) <= dtheta) aureole = true;
+    Residual res;
+    if (aureole) res = residual_phase();
+    std::vector<Real> phasm(ncut + 1), phast(ncut + 1), om_a(ncut + 1), om_m(ncut + 1);
+    for (int lc = 1; lc <= ncut; ++lc) {
+      om_a[lc] = c.ssa[lc - 1];
+      om_m[lc] = ssas[lc - 1];
+    }
+    for (int iu = 0; iu < numu; ++iu) {
+      const Real um = umu[iu];
+      const bool ims = um < 0.0 && std::abs(theta0 - std::acos(um) / deg) <= dtheta;
+      for (int jp = 0; jp < nphi; ++jp) {
+        const Real phirad = (c.phi_user[jp] - c.beam_phi) * deg;
+        const Real ct = -c.beam_mu * um + std::sqrt((1.0 - c.beam_mu * c.beam_mu) * (1.0 - um * um)) * std::cos(phirad);
+        // Exact phase function interpolated from the table, and the Delta-M
+        // phase function from N moments (1-based layer arrays).
+        const int k = interval(ct);
+        const Real* mp = c.mu_phase.data();
+        for (int lc = 1; lc <= ncut; ++lc) {
+          const Real pa = tab(lc - 1, k) + (ct - mp[k]) / (mp[k + 1] - mp[k]) * (tab(lc - 1, k + 1) - tab(lc - 1, k));
+          phast[lc] = pa / (1.0 - fwd[lc - 1] * c.ssa[lc - 1]);
+          phasm[lc] = 1.0;
+        }
+        Real p1 = 1.0, p2 = 0.0;
+        for (int l = 1; l <= N - 1; ++l) {
+          const Real p = ((2 * l - 1) * ct * p1 - (l - 1) * p2) / l;
+          p2 = p1;
+          p1 = p;
+          for (int lc = 1; lc <= ncut; ++lc)
+            phasm[lc] += (2.0 * l + 1) * p * (c.pm(l, lc - 1) - fwd[lc - 1]) / (1.0 - fwd[lc - 1]);
+        }
+        for (int lu = 0; lu < ntau; ++lu) {
+          if (lyrcut && lyr_of[lu] >= ncut) continue;
+          const Real a = single_scat(dither, lyr_of[lu] + 1, ncut, phast, om_a, um, sutau[lu]);
+          const Real b = single_scat(dither, lyr_of[lu] + 1, ncut, phasm, om_m, um, sutau[lu]);
+          out.inten[((size_t)lu * numu + iu) * nphi + jp] += a - b;
+        }
+        if (ims) {
+          const int l0 = (utau[0] <= dither) ? 1 : 0;
+          for (int lu = l0; lu < ntau; ++lu) {
+            if (lyrcut && lyr_of[lu] >= ncut) continue;
+            out.inten[((size_t)lu * numu + iu) * nphi + jp] -= buras_secondary(res, iu, lu, k, ct);
+          }
+        }
+      }
+    }
+  }
+
   // src/DisortSolver.cpp:3751-3808
   void brdf_correction(GeneralOutput& out) {
     const Real deg = M_PI / 180.0;
@@ -1262,16 +1477,17 @@ struct GeneralSolver {
       }
     }
     const bool corint = c.corr_buras && deltam && c.beam_flux > 0.0 && scat_yes && !c.only_fluxes;
-    // Nakajima-Tanaka unless a tabulated phase function selects Buras-Emde
-    // (src/DisortSolver.cpp:369-377); this interface carries none
-    // (num_phase_func_angles = 0), so corr_nakajima does not change the result.
     if (corint) {
+      // Nakajima-Tanaka unless a tabulated phase function and
+      // intensity_corr_nakajima = false select Buras-Emde (:369-377).
+      const bool buras = !c.corr_nakajima && c.nphase > 0;
       std::vector<Real> before = out.inten;
-      nt_correction(out);
+      if (buras) buras_correction(out);
+      else nt_correction(out);
       // Mutation hook: scale the correction itself.
       std::vector<Real> d(out.inten.size());
       for (size_t q = 0; q < d.size(); ++q) d[q] = out.inten[q] - before[q];
-      perturb(PS_NT, d);
+      perturb(buras ? PS_BURAS : PS_NT, d);
       for (size_t q = 0; q < d.size(); ++q) out.inten[q] = before[q] + d[q];
     }
     if (c.corr_buras && !deltam_plus && !c.lambertian && !c.only_fluxes && c.use_user_mu && c.beam_flux > 0.0)

@@ -210,6 +210,23 @@ void gauss_quadrature(int n, Real* cmu, Real* cwt) {
 // Chapman slant-path factor for pseudo-spherical geometry.
 // Restates src/FluxSolver.cpp:1503-1566 (1-based layer index lc).
 // ---------------------------------------------------------------------------
+int locate_sorted(const std::vector<Real>& xx, Real x) {
+  // src/Utils.hpp:27-64: bisection on an increasing or decreasing table.
+  const int n = (int)xx.size();
+  if (n == 0) return -1;
+  if (n == 1) return 0;
+  const bool up = xx[1] > xx[0];
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (up ? (x >= xx[mid]) : (x <= xx[mid])) lo = mid;
+    else hi = mid;
+  }
+  if (x == xx[0]) return 0;
+  if (x == xx[n - 1]) return n - 2;
+  return lo;
+}
+
 Real chapman(int lc, Real tau_frac, const Real* zd, const Real* dtau, int nlyr, Real r, Real mu0) {
   return perturb1(PS_CHAPMAN, chapman_exact(lc, tau_frac, zd, dtau, nlyr, r, mu0));
 }

@@ -142,9 +142,18 @@ struct GeneralConfig {
   Real wn_lo = 0, wn_hi = 0, radius = 0, accur = 0;
   std::vector<Real> dtau, ssa, temper, alt, pmom;  // pmom [nlyr][pstride]
   std::vector<Real> tau_user, mu_user, phi_user;
+  // Tabulated phase function (num_phase_func_angles, mu_phase_function,
+  // phase_function[layer][angle]) of the Buras-Emde correction.
+  int nphase = 0;
+  std::vector<Real> mu_phase, phase;  // [nphase], [nlyr][nphase]
   int pstride() const { return (nmom > nstr ? nmom : nstr) + 1; }
   Real pm(int k, int lc) const { return pmom[(size_t)lc * pstride() + k]; }
 };
+
+// Index j with xx[j] <= x < xx[j+1] in a sorted table (locate,
+// src/Utils.hpp:27-64): 0 at or below an increasing table's first entry,
+// n-2 at its last entry, n-1 beyond it.
+int locate_sorted(const std::vector<Real>& xx, Real x);
 
 struct GeneralOutput {
   int ntau = 0, numu = 0, nphi = 0, modes = 0;  // modes = Fourier modes computed

@@ -220,13 +220,16 @@ def intensity_cases():
     cases = []
 
     def add(name, src, nlyr, nstr, nmom, dtau, ssa, pmom, flags, scal, expect, accuracy=0.0, tau_user=None,
-            mu_user=None, phi_user=None, temper=None, alt=None, rpv=None):
+            mu_user=None, phi_user=None, temper=None, alt=None, rpv=None, mu_phase=None, phase=None):
         P = max(nmom, nstr) + 1
         assert all(len(r) == P for r in pmom), name
-        cases.append(dict(name=name, source=src, nlyr=nlyr, nstr=nstr, nmom=nmom, pstride=P, dtau=list(dtau),
-                          ssa=list(ssa), pmom=[list(r) for r in pmom], flags=flags, scal=scal,
-                          accuracy=accuracy, tau_user=tau_user, mu_user=mu_user, phi_user=phi_user,
-                          temper=temper, alt=alt, rpv=rpv, expect=expect))
+        case = dict(name=name, source=src, nlyr=nlyr, nstr=nstr, nmom=nmom, pstride=P, dtau=list(dtau),
+                    ssa=list(ssa), pmom=[list(r) for r in pmom], flags=flags, scal=scal,
+                    accuracy=accuracy, tau_user=tau_user, mu_user=mu_user, phi_user=phi_user,
+                    temper=temper, alt=alt, rpv=rpv, expect=expect)
+        if mu_phase is not None:
+            case.update(mu_phase=list(mu_phase), phase=[list(r) for r in phase])
+        cases.append(case)
 
     def I(iu, lu, j, v, rtol=None, atol=None):
         return dict(q="intensity", iu=iu, lu=lu, j=j, v=v, rtol=rtol, atol=atol)
@@ -310,6 +313,22 @@ def intensity_cases():
         add(name, src, 1, 16, 16, [tau], [1.0], [_hg_rows(0.75, 17)],
             ["user_mu", "user_tau", "corr_buras", "corr_nakajima"], dict(beam_flux=pi, beam_mu=1.0),
             six(top_up, bot_dn, 5e-4, False), tau_user=[0.0, tau], mu_user=umu6, phi_user=[0.0])
+
+    # --- Buras-Emde corrected tables (NewICHG3a/3b): the same HG g = 0.75
+    # layer with its phase function tabulated at 100 equidistant cosines
+    # (tests/test_solver.cpp:3940-3988) and intensity_corr_nakajima off ---
+    ct = [-1.0 + 2.0 * it / 99 for it in range(100)]
+    hg_tab = [(1.0 - 0.75 ** 2) / (1.0 + 0.75 ** 2 - 1.5 * x) ** 1.5 for x in ct]
+    for name, src, tau, top_up, bot_dn in [
+        ("ic_buras_hg3a", "tests/test_solver.cpp:3989-4008", 1.0,
+         [1.51254859e-01, 1.01163783e-01, 3.92360939e-02], [3.06133539e+00, 2.66600162e-01, 2.13783659e-01]),
+        ("ic_buras_hg3b", "tests/test_solver.cpp:4009-4026", 8.0,
+         [3.79836194e-01, 5.19661316e-01, 4.92942191e-01], [6.69642948e-01, 4.22349774e-01, 2.36362060e-01]),
+    ]:
+        add(name, src, 1, 16, 16, [tau], [1.0], [_hg_rows(0.75, 17)],
+            ["user_mu", "user_tau", "corr_buras"], dict(beam_flux=pi, beam_mu=1.0),
+            six(top_up, bot_dn, 1e-3, False), tau_user=[0.0, tau], mu_user=umu6, phi_user=[0.0],
+            mu_phase=ct, phase=[hg_tab])
 
     # --- Thermal intensities (disotest 7b, 7c, 9c, 10a) ---
     add("disotest_7b", "tests/test_solver.cpp:3415-3472", 1, 16, 0, [100.0], [0.95], [_hg_rows(0.75, 17)],

@@ -248,3 +248,34 @@ def test_spectral_bands_with_intensities(d):
         ref = d.DisortSolver().solve(one)
         np.testing.assert_allclose(np.array(out[k].intensity), np.array(ref.intensity), rtol=1e-12, atol=1e-15)
         np.testing.assert_allclose(out[k].flux_up, ref.flux_up, rtol=1e-12, atol=1e-15)
+
+
+def test_general_solver_buras_emde_table(d):
+    """NewICHG3a (tests/test_solver.cpp:3940-4008): HG g = 0.75 tabulated at
+    100 cosines, intensity_corr_buras without intensity_corr_nakajima; the
+    C-DISORT values the reference quotes (at 1e-3; they hold to 1e-8)."""
+    c = d.DisortConfig(1, 16, 16)
+    c.flags.use_user_tau = True
+    c.flags.use_user_mu = True
+    c.flags.intensity_corr_buras = True
+    c.flags.intensity_corr_nakajima = False
+    c.num_phase_func_angles = 100
+    c.num_user_tau = 2
+    c.num_user_mu = 6
+    c.num_phi = 1
+    c.allocate()
+    c.set_henyey_greenstein(0.75, 0)
+    ct = [-1.0 + 2.0 * it / 99 for it in range(100)]
+    c.mu_phase_function = ct
+    c.phase_function = [[(1 - 0.75 ** 2) / (1 + 0.75 ** 2 - 1.5 * x) ** 1.5 for x in ct]]
+    c.single_scat_albedo = [1.0]
+    c.delta_tau = [1.0]
+    c.tau_user = [0.0, 1.0]
+    c.mu_user = [-1.0, -0.5, -0.1, 0.1, 0.5, 1.0]
+    c.phi_user = [0.0]
+    c.bc.direct_beam_flux = math.pi
+    c.bc.direct_beam_mu = 1.0
+    r = d.DisortSolver().solve(c)
+    for (iu, lu), v in {(3, 0): 1.51254859e-01, (4, 0): 1.01163783e-01, (5, 0): 3.92360939e-02,
+                        (0, 1): 3.06133539e+00, (1, 1): 2.66600162e-01, (2, 1): 2.13783659e-01}.items():
+        assert r.intensities(iu, lu, 0) == pytest.approx(v, rel=1e-8)

@@ -229,6 +229,56 @@ def test_garcia_siewert_haze(ctx, name):
     solve_both(ctx, gs_batch(next(c for c in CASES if c[0] == name), n=2), name)
 
 
+def tabulated_hg(gb, g, nphase, seed):
+    """Buras-Emde inputs: each layer's phase function tabulated at nphase
+    increasing cosines (denser near forward scattering), HG of the layer's
+    asymmetry g with a Rayleigh admixture, plus the matching moments."""
+    r = np.random.default_rng(seed)
+    x = np.sort(np.concatenate([np.linspace(-1.0, 0.89, nphase // 2), 1.0 - np.geomspace(1e-4, 0.1, nphase - nphase // 2)]))
+    assert (np.diff(x) > 0).all()
+    x[-1] = 1.0
+    w = r.uniform(0.0, 0.3, g.shape)
+    hg = (1 - g[..., None] ** 2) / (1 + g[..., None] ** 2 - 2 * g[..., None] * x) ** 1.5
+    ray = 0.75 * (1 + x ** 2)
+    phase = w[..., None] * ray + (1 - w[..., None]) * hg
+    k = np.arange(gb.pstride)
+    mom = (1 - w[..., None]) * g[..., None] ** k
+    mom[..., 0] += w
+    mom[..., 2] += 0.1 * w
+    gb.pmom[:] = mom
+    gb.mu_phase = x
+    gb.nphase = x.size
+    gb.phase = np.ascontiguousarray(phase)
+    return gb
+
+
+@pytest.mark.parametrize("nstr,extra", [(8, {}), (16, {"user_tau": True}), (32, {"spherical": True}),
+                                         (16, {"thermal": True, "rpv": True})])
+def test_buras_emde_random(ctx, nstr, extra):
+    # Buras-Emde correction: tabulated phase function, intensity_corr_buras
+    # without intensity_corr_nakajima; user angles in the 10-degree aureole of
+    # the beam (secondary-scattering term) and away from it.
+    n, L = 3, 5
+    gb = random_general(n, L, nstr, seed=160 + nstr, corr=True, numu=8, nphi=3, mu0=np.array([0.8, 0.55, 1.0]),
+                        P=nstr + 8, **extra)
+    gb.flags &= ~G.G_CORR_NAKAJIMA
+    gb.mu_user[:] = np.array([-1.0, -0.81, -0.56, -0.3, 0.2, 0.5, 0.8, 1.0])
+    gb.phi_user[:] = np.array([0.0, 5.0, 120.0])
+    g = np.random.default_rng(nstr).uniform(0.6, 0.9, (n, L))
+    tabulated_hg(gb, g, 181, seed=nstr)
+    solve_both(ctx, gb, f"Buras-Emde nstr={nstr} {sorted(extra)}")
+
+
+def test_buras_emde_layer_cut(ctx):
+    gb = random_general(2, 8, 16, seed=171, corr=True, numu=6, nphi=2, mu0=np.array([0.9, 0.7]), P=24)
+    gb.flags &= ~G.G_CORR_NAKAJIMA
+    gb.dtau[:, 3:] = 30.0
+    gb.ssa[:, 3:] = 0.05
+    gb.mu_user[:] = np.array([-0.95, -0.9, -0.7, 0.3, 0.6, 1.0])
+    tabulated_hg(gb, np.full((2, 8), 0.8), 100, seed=3)
+    solve_both(ctx, gb, "Buras-Emde with the absorption cut-off")
+
+
 def test_validation_messages(ctx):
     from disort_amd._capi import DisortInvalid
 

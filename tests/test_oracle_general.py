@@ -31,7 +31,8 @@ def case_batch(c, n=1):
     return G.GeneralBatch(
         flags=flags, nstr=c["nstr"], dtau=rep(c["dtau"]), ssa=rep(c["ssa"]), pmom=rep(c["pmom"]),
         scal=np.repeat(sc, n, axis=0), temper=rep(c["temper"]), alt=rep(c["alt"]), rpv=rep(c["rpv"]),
-        tau_user=rep(c["tau_user"]), mu_user=c["mu_user"], phi_user=c["phi_user"], accuracy=c["accuracy"])
+        tau_user=rep(c["tau_user"]), mu_user=c["mu_user"], phi_user=c["phi_user"], accuracy=c["accuracy"],
+        mu_phase=c.get("mu_phase"), phase=rep(c.get("phase")))
 
 
 def tight_tol(e):

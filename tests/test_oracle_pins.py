@@ -9,8 +9,8 @@ Stages covered by the flux-only oracle: Delta-M scaled phase coefficients
 and albedos, quadrature weights, Legendre values, eigenvalues, eigenvectors,
 beam and thermal particular solutions, Planck function, integration
 constants (band LU solution), output assembly.  The general oracle adds the
-user-angle eigenvectors, the Nakajima-Tanaka correction, the RPV
-reflectivity and the Chapman factors."""
+user-angle eigenvectors, the Nakajima-Tanaka and Buras-Emde corrections,
+the RPV reflectivity and the Chapman factors."""
 import ctypes as C
 
 import pytest
@@ -22,7 +22,7 @@ from test_oracle_general import CASES, case_batch as general_case_batch, expect_
 
 STAGES = {"scaled_coef": 1, "scaled_ssa": 2, "eigval": 3, "eigvec": 4, "beam": 5, "thermal": 6,
           "planck": 7, "constants": 8, "quadrature": 9, "legendre": 10, "flux_assembly": 15}
-GENERAL_ONLY = {"user_eigvec": 11, "nakajima_tanaka": 12, "rpv": 13, "chapman": 14}
+GENERAL_ONLY = {"user_eigvec": 11, "nakajima_tanaka": 12, "rpv": 13, "chapman": 14, "buras_emde": 16}
 SHARED = {"planck", "chapman", "rpv"}  # numerics both solvers call
 FLUX_OFFSET = 32  # the flux-only solver's own stages
 

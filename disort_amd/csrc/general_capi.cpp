@@ -94,7 +94,7 @@ std::string validate_column(const disort_general_batch* b, int64_t i, int* code)
   const int L = b->nlyr, P = b->pstride, N = b->nstr;
   const uint32_t f = (uint32_t)b->flags;
   const bool thermal = f & DISORT_F_THERMAL, diffusion = f & DISORT_F_DIFFUSION_BC;
-  const bool brdf = f & DISORT_G_BRDF_RPV, user_mu = f & DISORT_G_USER_MU, only = f & DISORT_G_ONLY_FLUXES;
+  const bool brdf = f & dsa::kBrdfFlags, user_mu = f & DISORT_G_USER_MU, only = f & DISORT_G_ONLY_FLUXES;
   const double* s = b->scal + i * DISORT_NSCAL;
   if (L <= 0) return "num_layers must be positive";
   if (N < 4) return "num_streams must be >= 4";
@@ -181,7 +181,13 @@ extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch*
   const bool host = memspace == DISORT_MEM_HOST;
   const bool user_tau = f & DISORT_G_USER_TAU, only = f & DISORT_G_ONLY_FLUXES;
   const bool do_user = (f & DISORT_G_USER_MU) && !only;
-  const bool brdf = f & DISORT_G_BRDF_RPV;
+  const bool brdf = f & dsa::kBrdfFlags;
+  const int32_t bflags = f & dsa::kBrdfFlags;
+  if (bflags & (bflags - 1)) return fail(ctx, DISORT_E_INVALID, "more than one BRDF type selected");
+  const int brdf_kind = (bflags == DISORT_G_BRDF_HAPKE)     ? dsa::kBrdfHapke
+                        : (bflags == DISORT_G_BRDF_COXMUNK) ? dsa::kBrdfCoxMunk
+                        : (bflags == DISORT_G_BRDF_ROSSLI)  ? dsa::kBrdfRossLi
+                                                            : dsa::kBrdfRpv;
   const int64_t n = b->n;
   const int ntau = user_tau ? b->ntau : L + 1;
   const int numu = do_user ? b->numu : N;
@@ -374,6 +380,7 @@ extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch*
   a.rpv_set = (const int32_t*)d_set;
   a.set_rpv = (const double*)d_sets;
   a.nsets = nsets;
+  a.brdf_kind = brdf_kind;
 
   // Constant tables.
   double *tab, *mt;

@@ -70,17 +70,87 @@ __device__ double rpv_rho(const double* p, double mu1, double mu2, double phi) {
   return ans < 0.0 ? 0.0 : ans;
 }
 
-// Fourier coefficient (2 - delta_m0) (1/pi) int_0^pi rho cos(m phi) dphi by
-// the 100-point half of the azimuth quadrature (src/DisortSolver.cpp:2614-2635).
-__device__ double rpv_fourier(const GenArgs& a, const double* p, int m, double mu, double mup) {
-  const double* gmu = a.gaz;
-  const double* gwt = a.gaz + kNMUG;
-  double s = 0.0;
-  for (int k = 0; k < kNMUG / 2; ++k) {
-    const double cm = (m == 0) ? 1.0 : cos((double)m * kPi * gmu[k]);
-    s += gwt[k] * rpv_rho(p, mu, mup, kPi * gmu[k]) * cm;
+// Hapke (src/DisortSolver.cpp:2710-2731): p = {b0, hh, w}.
+__device__ double hapke_rho(const double* p, double mu, double mup, double dphi) {
+  const double b0 = p[0], hh = p[1], w = p[2];
+  const double ct = mu * mup + sqrt((1.0 - mu * mu) * (1.0 - mup * mup)) * cos(dphi);
+  const double b = b0 * hh / (hh + tan(0.5 * acos(ct)));
+  const double gam = sqrt(1.0 - w);
+  const double h0 = (1.0 + 2.0 * mup) / (1.0 + 2.0 * gam * mup);
+  const double h = (1.0 + 2.0 * mu) / (1.0 + 2.0 * gam * mu);
+  return 0.25 * w * ((1.0 + b) * (1.0 + 0.5 * ct) + h0 * h - 1.0) / (mu + mup);
+}
+
+// Ross-thick + Li-sparse-reciprocal kernels (src/DisortSolver.cpp:2785-2840):
+// p = {iso, vol, geo}; crown height/base 2, branch/radius 1, hot spot 1.5 deg.
+__device__ double rossli_rho(const double* p, double mu, double mup, double dphi) {
+  const double a0 = 1.5 * kPi / 180.0;
+  const double si = sqrt(1.0 - mup * mup), sr = sqrt(1.0 - mu * mu);
+  const double ti = mup > 0.0 ? si / mup : 0.0, tr = mu > 0.0 ? sr / mu : 0.0;
+  const double cd = cos(dphi);
+  double ca = mup * mu + si * sr * cd;
+  ca = ca < -1.0 ? -1.0 : (ca > 1.0 ? 1.0 : ca);
+  const double al = acos(ca);
+  const double kvol =
+      4.0 / (3.0 * kPi) * (1.0 / (mup + mu)) * ((kPi / 2.0 - al) * ca + sin(al)) * (1.0 + 1.0 / (1.0 + al / a0)) -
+      1.0 / 3.0;
+  // Branch-to-radius ratio 1: the scaled tangents are the tangents.
+  const double ci1 = 1.0 / sqrt(1.0 + ti * ti), cr1 = 1.0 / sqrt(1.0 + tr * tr);
+  const double si1 = ti * ci1, sr1 = tr * cr1;
+  const double ca1 = ci1 * cr1 + si1 * sr1 * cd;
+  double g2 = ti * ti + tr * tr - 2.0 * ti * tr * cd;
+  if (g2 < 0.0) g2 = 0.0;
+  const double x = ti * tr * sin(dphi);
+  const double cost = 2.0 * (ci1 * cr1) / (ci1 + cr1) * sqrt(g2 + x * x);
+  const double t = (cost >= -1.0 && cost <= 1.0) ? acos(cost) : 0.0;
+  const double kgeo =
+      (ci1 + cr1) / (kPi * ci1 * cr1) * (t - sin(t) * cos(t) - kPi) + (1.0 + ca1) / (2.0 * ci1 * cr1);
+  const double r = p[0] + p[1] * kvol + p[2] * kgeo;
+  return r > 0.0 ? r : 0.0;
+}
+
+// Tsang shadowing term (src/DisortSolver.cpp:2842-2856).
+__device__ double shadow_eta(double ct, double s2) {
+  const double st = sqrt(1.0 - ct * ct);
+  if (st < 1e-15) return 0.0;
+  const double cot = ct / st;
+  return 0.5 * (sqrt(s2 / kPi) / cot * exp(-cot * cot / s2) - erfc(cot / sqrt(s2)));
+}
+
+// Cox-Munk ocean surface (src/DisortSolver.cpp:2858-2900): p = {u10,
+// refractive index, shadowing 1 / 0}.
+__device__ double coxmunk_rho(const double* p, double mu, double mup, double dphi) {
+  const double si = sqrt(1.0 - mup * mup), sr = sqrt(1.0 - mu * mu);
+  const double ct = -mup * mu + si * sr * cos(dphi);
+  const double omc = 1.0 - ct;
+  if (omc < 1e-15) return 0.0;
+  const double mn2 = (mup + mu) * (mup + mu) / (2.0 * omc);
+  if (mn2 < 1e-15) return 0.0;
+  const double s2 = 0.003 + 0.00512 * p[0];
+  const double pslope = 1.0 / (kPi * s2) * exp(-(1.0 - mn2) / (s2 * mn2));
+  const double nt = p[1];
+  const double cli = sqrt(0.5 * (1.0 + ct));
+  const double q = 1.0 - cli * cli;
+  const double sli = sqrt(q > 0.0 ? q : 0.0);
+  const double slt = sli / nt;
+  if (slt >= 1.0) return 0.0;
+  const double clt = sqrt(1.0 - slt * slt);
+  const double rs = (cli - nt * clt) / (cli + nt * clt);
+  const double rp = (nt * cli - clt) / (clt + nt * cli);
+  double r = (pslope * 0.5 * (rs * rs + rp * rp)) / (4.0 * mup * mu * mn2 * mn2);
+  if (p[2] != 0.0) r *= 1.0 / (shadow_eta(mup, s2) + shadow_eta(mu, s2) + 1.0);
+  return r > 0.0 ? r : 0.0;
+}
+
+// Bidirectional reflectivity of the batch's BRDF kind (bidirReflectivity,
+// src/DisortSolver.cpp:2902-2936) with parameter block p.
+__device__ double brdf_rho(int kind, const double* p, double mu, double mup, double dphi) {
+  switch (kind) {
+    case kBrdfHapke: return hapke_rho(p, mu, mup, dphi);
+    case kBrdfCoxMunk: return coxmunk_rho(p, mu, mup, dphi);
+    case kBrdfRossLi: return rossli_rho(p, mu, mup, dphi);
+    default: return rpv_rho(p, mu, mup, dphi);
   }
-  return (m == 0 ? 1.0 : 2.0) * s;
 }
 
 // ===========================================================================
@@ -154,7 +224,7 @@ __global__ __launch_bounds__(256) void k_gsurf_c(GenArgs a) {
   for (int m = 0; m < N; ++m) sm[m] = 0.0;
   if (act)
     for (int k = 0; k < K; ++k) {
-      const double x = gwt[k] * rpv_rho(p, mu, mup, kPi * gmu[k]);
+      const double x = gwt[k] * brdf_rho(a.brdf_kind, p, mu, mup, kPi * gmu[k]);
 #pragma unroll
       for (int m = 0; m < N; ++m)
         if (m < nm) sm[m] += x * cmk[m * K + k];
@@ -194,7 +264,7 @@ __global__ __launch_bounds__(256) void k_gsurf_e(GenArgs a) {
   double part = 0.0;
   for (int jg = threadIdx.x; jg < kNMUG; jg += blockDim.x) {
     double sm = 0.0;
-    for (int k = 0; k < kNMUG / 2; ++k) sm += gwt[k] * gmu[k] * rpv_rho(p, mu, gmu[k], kPi * gmu[jg]);
+    for (int k = 0; k < kNMUG / 2; ++k) sm += gwt[k] * gmu[k] * brdf_rho(a.brdf_kind, p, mu, gmu[k], kPi * gmu[jg]);
     part += gwt[jg] * sm;
   }
   const double tot = blk_sum(part, red);
@@ -203,7 +273,7 @@ __global__ __launch_bounds__(256) void k_gsurf_e(GenArgs a) {
 
 // Beam column of the BRDF coefficients of every mode, block per chunk
 // column: each reflectivity value rho(mu, mu0, phi_k) is evaluated once and
-// enters all the modes' quadrature sums (in the order of rpv_fourier, with
+// enters all the modes' quadrature sums (in the order of the reference's per-mode loop, with
 // the same cos(m pi x_k) table), rather than once per mode.
 template <int NN>
 __global__ __launch_bounds__(64) void k_gsurf_b(GenArgs a) {
@@ -236,7 +306,7 @@ __global__ __launch_bounds__(64) void k_gsurf_b(GenArgs a) {
     for (int m = 0; m < N; ++m) sm[m] = 0.0;
     if (act) {
       for (int k = 0; k < K; ++k) {
-        const double x = gwt[k] * rpv_rho(p, mu, mu0, kPi * gmu[k]);
+        const double x = gwt[k] * brdf_rho(a.brdf_kind, p, mu, mu0, kPi * gmu[k]);
 #pragma unroll
         for (int m = 0; m < N; ++m)
           if (m < nm) sm[m] += x * cmk[m * K + k];
@@ -878,7 +948,7 @@ __global__ __launch_bounds__(WideCfg<NN>::BT) void k_gboundary(GenArgs a) {
   const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
   const bool diffusion = (a.flags & DISORT_F_DIFFUSION_BC) != 0;
   const bool thermal = (a.flags & DISORT_F_THERMAL) != 0;
-  const bool lamb = (a.flags & DISORT_G_BRDF_RPV) == 0;
+  const bool lamb = (a.flags & kBrdfFlags) == 0;
   const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
   const double alb = sc[DISORT_S_ALBEDO];
   const double d0 = (m == 0) ? 1.0 : 0.0;
@@ -1150,7 +1220,7 @@ __global__ __launch_bounds__(64, 3) void k_gboundary1(GenArgs a) {
   const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
   const bool diffusion = (a.flags & DISORT_F_DIFFUSION_BC) != 0;
   const bool thermal = (a.flags & DISORT_F_THERMAL) != 0;
-  const bool lamb = (a.flags & DISORT_G_BRDF_RPV) == 0;
+  const bool lamb = (a.flags & kBrdfFlags) == 0;
   const double F0 = sc[DISORT_S_BEAM_FLUX], mu0 = sc[DISORT_S_BEAM_MU];
   const double alb = sc[DISORT_S_ALBEDO];
   const double d0 = (m == 0) ? 1.0 : 0.0;
@@ -1539,7 +1609,7 @@ __global__ __launch_bounds__(64) void k_gucoef(GenArgs a, double* ucoef, int tpb
     double bnd = 0.0;
     if (lc == L - 1 && mu > 0.0) {
       const bool diffusion = (a.flags & DISORT_F_DIFFUSION_BC) != 0;
-      const bool lamb = (a.flags & DISORT_G_BRDF_RPV) == 0;
+      const bool lamb = (a.flags & kBrdfFlags) == 0;
       if (diffusion) {
         if (!lyrcut && m == 0)
           bnd = a.inst_d[i * kInstD + ID_PL_BOT] + mu * a.inst_d[i * kInstD + ID_PL_BOT_DERIV];
@@ -2163,7 +2233,7 @@ __global__ __launch_bounds__(64) void k_gbrdf(GenArgs a, const int32_t* modes_us
   if (a.inst_i[i * kInstI + II_SCALING] & 2) return;  // Delta-M+ active
   const double* p = a.set_rpv + (int64_t)a.rpv_set[i] * DISORT_NRPV;
   const double dphi = (a.phi_user[j] - sc[DISORT_S_BEAM_PHI]) * (kPi / 180.0);
-  const double exact = rpv_rho(p, mu, mu0, dphi);
+  const double exact = brdf_rho(a.brdf_kind, p, mu, mu0, dphi);
   if (exact < 0.0) return;
   double approx = 0.0;
   const int mu_ = modes_used[cl];

@@ -8,6 +8,8 @@
 
 #include <cstdint>
 
+#include "../../include/disort_amd.h"
+
 namespace dsa {
 
 // Per-task record (doubles), AoS: eig[t * GE(N).size() + field].  G is the
@@ -81,8 +83,8 @@ struct GenArgs {
   const double* gaz;  // [2][NMUG]: BRDF azimuth nodes and weights
   // Surface (BRDF) tables.
   const int32_t* rpv_set;  // [n] parameter-set index of each column
-  const double* set_rpv;   // [nsets][7]
-  int32_t nsets, pad0;
+  const double* set_rpv;   // [nsets][7] BRDF parameter blocks
+  int32_t nsets, brdf_kind;  // brdf_kind: kBrdf* of the batch
   double* srf_d;  // [nsets][nm][surf_d_size]
   double* srf_b;  // [nc][nm][surf_b_size] (chunk-local)
   // Chunk scratch.
@@ -105,6 +107,10 @@ struct GenArgs {
   double* be_norm;         // [nc][ntau]
 };
 constexpr int kBeNF = 100;
+enum { kBrdfRpv = 0, kBrdfHapke = 1, kBrdfCoxMunk = 2, kBrdfRossLi = 3 };
+enum : int32_t {
+  kBrdfFlags = DISORT_G_BRDF_RPV | DISORT_G_BRDF_HAPKE | DISORT_G_BRDF_COXMUNK | DISORT_G_BRDF_ROSSLI
+};
 constexpr int kMaxPhaseAngles = 2048;
 
 // Per-chunk launch options of the general solver.

@@ -21,6 +21,10 @@ G_ONLY_FLUXES = 64
 G_BRDF_RPV = 128
 G_CORR_BURAS = 256
 G_CORR_NAKAJIMA = 512
+G_BRDF_HAPKE = 1024
+G_BRDF_COXMUNK = 2048
+G_BRDF_ROSSLI = 4096
+G_BRDF_ANY = G_BRDF_RPV | G_BRDF_HAPKE | G_BRDF_COXMUNK | G_BRDF_ROSSLI
 S_BEAM_PHI = 11
 NRPV = 7
 

@@ -617,8 +617,23 @@ void require_gpu_subset(const DisortConfig& c) {
   if (c.flags.ibcnd != BoundaryConditionType::General) no("SPECIAL_BC");
   if (c.flags.compute_temperature_jacobian) no("compute_temperature_jacobian");
   if (!c.flags.use_lambertian_surface) {
-    if (c.flags.brdf_type != BrdfType::RPV) no("a BRDF other than RPV");
-    if (!c.brdf.rpv) throw std::runtime_error("bidirReflectivity: RPV BRDF selected but rpv spec is null");
+    // Missing specifications fail as bidirReflectivity does
+    // (src/DisortSolver.cpp:2902-2936); Hapke falls back to its defaults.
+    switch (c.flags.brdf_type) {
+      case BrdfType::RPV:
+        if (!c.brdf.rpv) throw std::runtime_error("bidirReflectivity: RPV BRDF selected but rpv spec is null");
+        break;
+      case BrdfType::CoxMunk:
+        if (!c.brdf.cox_munk)
+          throw std::runtime_error("bidirReflectivity: Cox-Munk BRDF selected but cox_munk spec is null");
+        break;
+      case BrdfType::Ambrals:
+        if (!c.brdf.ambrals)
+          throw std::runtime_error("bidirReflectivity: Ross-Li/Ambrals BRDF selected but ambrals spec is null");
+        break;
+      case BrdfType::Hapke: break;
+      default: throw std::runtime_error("bidirReflectivity: unknown BRDF type");
+    }
   }
   if (c.flags.output_fourier_expansion) no("output_fourier_expansion");
 }
@@ -692,7 +707,14 @@ int32_t full_flags(const DisortConfig& c) {
   if (c.flags.use_user_tau) f |= DISORT_G_USER_TAU;
   if (c.flags.use_user_mu) f |= DISORT_G_USER_MU;
   if (c.flags.comp_only_fluxes) f |= DISORT_G_ONLY_FLUXES;
-  if (!c.flags.use_lambertian_surface) f |= DISORT_G_BRDF_RPV;
+  if (!c.flags.use_lambertian_surface) {
+    switch (c.flags.brdf_type) {
+      case BrdfType::Hapke: f |= DISORT_G_BRDF_HAPKE; break;
+      case BrdfType::CoxMunk: f |= DISORT_G_BRDF_COXMUNK; break;
+      case BrdfType::Ambrals: f |= DISORT_G_BRDF_ROSSLI; break;
+      default: f |= DISORT_G_BRDF_RPV; break;
+    }
+  }
   if (c.flags.intensity_corr_buras) f |= DISORT_G_CORR_BURAS;
   if (c.flags.intensity_corr_nakajima) f |= DISORT_G_CORR_NAKAJIMA;
   return f;
@@ -771,9 +793,31 @@ struct GeneralPack {
               c.wavenumber_low, c.wavenumber_high, c.bottom_radius);
     s[DISORT_S_BEAM_PHI] = c.bc.direct_beam_phi;
     scal.insert(scal.end(), s, s + DISORT_NSCAL);
-    if (flags & DISORT_G_BRDF_RPV) {
-      const RpvBrdfSpec& r = *c.brdf.rpv;
-      const double v[DISORT_NRPV] = {r.rho0, r.k, r.theta, r.sigma, r.t1, r.t2, r.scale};
+    if (!c.flags.use_lambertian_surface) {
+      // The BRDF's parameter block (include/disort_amd.h, DISORT_NRPV slots).
+      double v[DISORT_NRPV] = {0, 0, 0, 0, 0, 0, 0};
+      switch (c.flags.brdf_type) {
+        case BrdfType::Hapke: {
+          const HapkeBrdfSpec h = c.brdf.hapke ? *c.brdf.hapke : HapkeBrdfSpec{};
+          v[0] = h.b0, v[1] = h.hh, v[2] = h.w;
+          break;
+        }
+        case BrdfType::CoxMunk: {
+          const CoxMunkBrdfSpec& m = *c.brdf.cox_munk;
+          v[0] = m.u10, v[1] = m.refractive_index, v[2] = m.do_shadow ? 1.0 : 0.0;
+          break;
+        }
+        case BrdfType::Ambrals: {
+          const AmbralsBrdfSpec& r = *c.brdf.ambrals;
+          v[0] = r.iso, v[1] = r.vol, v[2] = r.geo;
+          break;
+        }
+        default: {
+          const RpvBrdfSpec& r = *c.brdf.rpv;
+          const double w[DISORT_NRPV] = {r.rho0, r.k, r.theta, r.sigma, r.t1, r.t2, r.scale};
+          std::copy(w, w + DISORT_NRPV, v);
+        }
+      }
       rpv.insert(rpv.end(), v, v + DISORT_NRPV);
     }
     ++n;

@@ -266,9 +266,16 @@ enum {
   DISORT_G_ONLY_FLUXES = 64,    /* flags.comp_only_fluxes                      */
   DISORT_G_BRDF_RPV = 128,      /* use_lambertian_surface = false, BrdfType::RPV */
   DISORT_G_CORR_BURAS = 256,    /* flags.intensity_corr_buras                  */
-  DISORT_G_CORR_NAKAJIMA = 512  /* flags.intensity_corr_nakajima               */
+  DISORT_G_CORR_NAKAJIMA = 512, /* flags.intensity_corr_nakajima               */
+  DISORT_G_BRDF_HAPKE = 1024,   /* use_lambertian_surface = false, BrdfType::Hapke   */
+  DISORT_G_BRDF_COXMUNK = 2048, /* use_lambertian_surface = false, BrdfType::CoxMunk */
+  DISORT_G_BRDF_ROSSLI = 4096   /* use_lambertian_surface = false, BrdfType::Ambrals */
 };
-/* RPV parameter block rpv[i*DISORT_NRPV + slot] (src/DisortConfig.hpp RpvBrdfSpec). */
+/* BRDF parameter block rpv[i*DISORT_NRPV + slot], one of the BRDF flags set
+ * (src/DisortConfig.hpp:52-96): RPV (RpvBrdfSpec) in the slots below; Hapke
+ * (HapkeBrdfSpec) {b0, hh, w}; Cox-Munk (CoxMunkBrdfSpec) {u10,
+ * refractive_index, do_shadow as 1.0 / 0.0}; Ross-Li (AmbralsBrdfSpec)
+ * {iso, vol, geo}; unused slots 0. */
 enum {
   DISORT_RPV_RHO0 = 0, DISORT_RPV_K = 1, DISORT_RPV_THETA = 2, DISORT_RPV_SIGMA = 3,
   DISORT_RPV_T1 = 4, DISORT_RPV_T2 = 5, DISORT_RPV_SCALE = 6, DISORT_NRPV = 7
@@ -291,7 +298,7 @@ typedef struct disort_general_batch {
   const double* temper; /* [n][nlyr+1] with DISORT_F_THERMAL, else NULL          */
   const double* alt;    /* [n][nlyr+1] with DISORT_F_SPHERICAL, else NULL        */
   const double* scal;   /* [n][DISORT_NSCAL]                                     */
-  const double* rpv;    /* [n][DISORT_NRPV] with DISORT_G_BRDF_RPV, else NULL    */
+  const double* rpv;    /* [n][DISORT_NRPV] BRDF parameters with a BRDF flag, else NULL */
   const double* tau_user; /* [n][ntau] with DISORT_G_USER_TAU, else NULL         */
   const double* mu_user;  /* [numu], increasing, with DISORT_G_USER_MU           */
   const double* phi_user; /* [nphi] degrees                                      */

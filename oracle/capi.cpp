@@ -173,8 +173,13 @@ int oracle_general_one(const disort_general_batch* b, int64_t i, int index_from_
     c.use_user_tau = (b->flags & DISORT_G_USER_TAU) != 0;
     c.use_user_mu = (b->flags & DISORT_G_USER_MU) != 0;
     c.only_fluxes = (b->flags & DISORT_G_ONLY_FLUXES) != 0;
-    c.lambertian = (b->flags & DISORT_G_BRDF_RPV) == 0;
-    c.brdf = c.lambertian ? oracle::BRDF_NONE : oracle::BRDF_RPV;
+    const int bf = b->flags;
+    c.brdf = (bf & DISORT_G_BRDF_RPV)       ? oracle::BRDF_RPV
+             : (bf & DISORT_G_BRDF_HAPKE)   ? oracle::BRDF_HAPKE
+             : (bf & DISORT_G_BRDF_COXMUNK) ? oracle::BRDF_COXMUNK
+             : (bf & DISORT_G_BRDF_ROSSLI)  ? oracle::BRDF_ROSSLI
+                                            : oracle::BRDF_NONE;
+    c.lambertian = c.brdf == oracle::BRDF_NONE;
     c.corr_buras = (b->flags & DISORT_G_CORR_BURAS) != 0;
     c.corr_nakajima = (b->flags & DISORT_G_CORR_NAKAJIMA) != 0;
     c.index_from_bottom = index_from_bottom != 0;

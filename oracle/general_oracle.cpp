@@ -27,6 +27,88 @@ Real rpv_reflectivity(const Real* p, Real mu1, Real mu2, Real phi) {
   return perturb1(PS_BRDF, rpv_reflectivity_exact(p, mu1, mu2, phi));
 }
 
+// Hapke (src/DisortSolver.cpp:2710-2731): p = {b0, hh, w}.
+Real hapke_reflectivity(const Real* p, Real mu, Real mup, Real dphi) {
+  const Real b0 = p[0], hh = p[1], w = p[2];
+  const Real ct = mu * mup + std::sqrt((1.0 - mu * mu) * (1.0 - mup * mup)) * std::cos(dphi);
+  const Real half = 0.5 * std::acos(ct);
+  const Real b = b0 * hh / (hh + std::tan(half));
+  const Real gam = std::sqrt(1.0 - w);
+  const Real h0 = (1.0 + 2.0 * mup) / (1.0 + 2.0 * gam * mup);
+  const Real h = (1.0 + 2.0 * mu) / (1.0 + 2.0 * gam * mu);
+  return 0.25 * w * ((1.0 + b) * (1.0 + 0.5 * ct) + h0 * h - 1.0) / (mu + mup);
+}
+
+// Ross-thick volume + Li-sparse-reciprocal geometric kernels
+// (src/DisortSolver.cpp:2785-2840): p = {iso, vol, geo}; crown
+// height/base 2, branch/radius 1, hot-spot width 1.5 degrees.
+Real rossli_reflectivity(const Real* p, Real mu, Real mup, Real dphi) {
+  const Real hb = 2.0, br = 1.0, a0 = 1.5 * M_PI / 180.0;
+  const Real si = std::sqrt(1.0 - mup * mup), sr = std::sqrt(1.0 - mu * mu);
+  const Real ti = mup > 0.0 ? si / mup : 0.0, tr = mu > 0.0 ? sr / mu : 0.0;
+  Real ca = mup * mu + si * sr * std::cos(dphi);
+  ca = ca < -1.0 ? -1.0 : (ca > 1.0 ? 1.0 : ca);
+  const Real al = std::acos(ca);
+  const Real kvol = 4.0 / (3.0 * M_PI) * (1.0 / (mup + mu)) * ((M_PI / 2.0 - al) * ca + std::sin(al)) *
+                        (1.0 + 1.0 / (1.0 + al / a0)) -
+                    1.0 / 3.0;
+  const Real ti1 = br * ti, tr1 = br * tr;
+  const Real ci1 = 1.0 / std::sqrt(1.0 + ti1 * ti1), cr1 = 1.0 / std::sqrt(1.0 + tr1 * tr1);
+  const Real si1 = ti1 * ci1, sr1 = tr1 * cr1;
+  const Real ca1 = ci1 * cr1 + si1 * sr1 * std::cos(dphi);
+  Real g2 = ti1 * ti1 + tr1 * tr1 - 2.0 * ti1 * tr1 * std::cos(dphi);
+  if (g2 < 0.0) g2 = 0.0;
+  const Real x = ti1 * tr1 * std::sin(dphi);
+  const Real cost = hb * (ci1 * cr1) / (ci1 + cr1) * std::sqrt(g2 + x * x);
+  const Real t = (cost >= -1.0 && cost <= 1.0) ? std::acos(cost) : 0.0;
+  const Real kgeo = (ci1 + cr1) / (M_PI * ci1 * cr1) * (t - std::sin(t) * std::cos(t) - M_PI) +
+                    (1.0 + ca1) / (2.0 * ci1 * cr1);
+  const Real r = p[0] + p[1] * kvol + p[2] * kgeo;
+  return r > 0.0 ? r : 0.0;
+}
+
+// Tsang shadowing term of the Cox-Munk surface (src/DisortSolver.cpp:2842-2856).
+static Real shadow_eta(Real ct, Real s2) {
+  const Real st = std::sqrt(1.0 - ct * ct);
+  if (st < 1e-15) return 0.0;
+  const Real cot = ct / st;
+  return 0.5 * (std::sqrt(s2 / M_PI) / cot * std::exp(-cot * cot / s2) - std::erfc(cot / std::sqrt(s2)));
+}
+
+// Cox-Munk ocean surface (src/DisortSolver.cpp:2858-2900): p = {u10,
+// refractive index, shadowing on (1) / off (0)}.
+Real coxmunk_reflectivity(const Real* p, Real mu, Real mup, Real dphi) {
+  const Real si = std::sqrt(1.0 - mup * mup), sr = std::sqrt(1.0 - mu * mu);
+  const Real ct = -mup * mu + si * sr * std::cos(dphi);
+  const Real omc = 1.0 - ct;
+  if (omc < 1e-15) return 0.0;
+  const Real mn2 = (mup + mu) * (mup + mu) / (2.0 * omc);
+  if (mn2 < 1e-15) return 0.0;
+  const Real s2 = 0.003 + 0.00512 * p[0];
+  const Real pslope = 1.0 / (M_PI * s2) * std::exp(-(1.0 - mn2) / (s2 * mn2));
+  const Real nt = p[1];
+  const Real cli = std::sqrt(0.5 * (1.0 + ct));
+  const Real q = 1.0 - cli * cli;
+  const Real sli = std::sqrt(q > 0.0 ? q : 0.0);
+  const Real slt = sli / nt;
+  if (slt >= 1.0) return 0.0;
+  const Real clt = std::sqrt(1.0 - slt * slt);
+  const Real rs = (cli - nt * clt) / (cli + nt * clt);
+  const Real rp = (nt * cli - clt) / (clt + nt * cli);
+  Real r = (pslope * 0.5 * (rs * rs + rp * rp)) / (4.0 * mup * mu * mn2 * mn2);
+  if (p[2] != 0.0) r *= 1.0 / (shadow_eta(mup, s2) + shadow_eta(mu, s2) + 1.0);
+  return r > 0.0 ? r : 0.0;
+}
+
+Real brdf_reflectivity(int kind, const Real* p, Real mu, Real mup, Real dphi) {
+  switch (kind) {
+    case BRDF_HAPKE: return perturb1(PS_BRDF, hapke_reflectivity(p, mu, mup, dphi));
+    case BRDF_COXMUNK: return perturb1(PS_BRDF, coxmunk_reflectivity(p, mu, mup, dphi));
+    case BRDF_ROSSLI: return perturb1(PS_BRDF, rossli_reflectivity(p, mu, mup, dphi));
+    default: return rpv_reflectivity(p, mu, mup, dphi);
+  }
+}
+
 Real rpv_reflectivity_exact(const Real* p, Real mu1, Real mu2, Real phi) {
   const Real rho0 = p[0], k = p[1], theta = p[2], sigma = p[3], t1 = p[4], t2 = p[5], scale = p[6];
   phi = M_PI - phi;  // RPV measures azimuth from the backward direction
@@ -109,7 +191,7 @@ void validate_general_config(const GeneralConfig& c) {
       bad("use_thermal_emission=true requires non-negative temperature_top and temperature_bottom");
     if (c.emis_top < 0.0 || c.emis_top > 1.0) bad("use_thermal_emission=true requires 0 <= emissivity_top <= 1");
   }
-  if (!c.lambertian && c.brdf != BRDF_RPV) bad("oracle: only the RPV BRDF is restated");
+  if (!c.lambertian && (c.brdf < BRDF_RPV || c.brdf > BRDF_ROSSLI)) bad("bidirReflectivity: unknown BRDF type");
   if (c.use_user_tau && c.tau_user.empty()) bad("oracle: use_user_tau needs tau_user values");
 }
 
@@ -148,7 +230,7 @@ struct GeneralSolver {
   Real& RQ(int iq, int col) { return rq[(size_t)iq * (NN + 1) + col]; }
   Real& RU(int iu, int col) { return ru[(size_t)iu * (NN + 1) + col]; }
   Real K(int lc, int q) const { return kk[(size_t)lc * N + q]; }
-  Real rho(Real mu, Real mup, Real dphi) const { return rpv_reflectivity(c.rpv, mu, mup, dphi); }
+  Real rho(Real mu, Real mup, Real dphi) const { return brdf_reflectivity(c.brdf, c.rpv, mu, mup, dphi); }
 
   // src/DisortSolver.cpp:641-820 (disortSet); identical to the flux solver's
   // set-up plus the user-level mapping.

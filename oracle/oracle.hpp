@@ -129,7 +129,10 @@ FluxOutput flux_solve(const FluxConfig& c);
 // ---- general solver (restating src/DisortSolver.cpp DisortSolver) ---------
 // RPV parameters in the order rho0, k, theta, sigma, t1, t2, scale
 // (src/DisortConfig.hpp RpvBrdfSpec).
-enum { BRDF_NONE = 0, BRDF_RPV = 1 };
+// BRDF kinds; the parameter block rpv[7] holds the kind's parameters:
+// RPV {rho0, k, theta, sigma, t1, t2, scale}, Hapke {b0, hh, w},
+// Cox-Munk {u10, refractive index, shadowing 0/1}, Ross-Li {iso, vol, geo}.
+enum { BRDF_NONE = 0, BRDF_RPV = 1, BRDF_HAPKE = 2, BRDF_COXMUNK = 3, BRDF_ROSSLI = 4 };
 struct GeneralConfig {
   int nlyr = 0, nstr = 0, nmom = 0;
   bool thermal = false, spherical = false, deltam_plus = false, diffusion_bc = false;
@@ -167,5 +170,9 @@ void validate_general_config(const GeneralConfig& c);
 GeneralOutput general_solve(const GeneralConfig& c);
 Real rpv_reflectivity(const Real* rpv, Real mu1, Real mu2, Real phi);
 Real rpv_reflectivity_exact(const Real* rpv, Real mu1, Real mu2, Real phi);
+Real hapke_reflectivity(const Real* p, Real mu, Real mup, Real dphi);
+Real coxmunk_reflectivity(const Real* p, Real mu, Real mup, Real dphi);
+Real rossli_reflectivity(const Real* p, Real mu, Real mup, Real dphi);
+Real brdf_reflectivity(int kind, const Real* p, Real mu, Real mup, Real dphi);
 
 }  // namespace ORACLE_NS

@@ -414,6 +414,76 @@ def intensity_cases():
         [_ray_rows(17), _hg_rows(0.7, 17)], ["user_tau", "user_mu", "rpv"], dict(beam_flux=1.0, beam_mu=mu30), ml,
         tau_user=[0.0, 0.32, 0.64], mu_user=[0.1, 0.2, 0.5, 1.0], phi_user=[0.0, 90.0, 180.0], rpv=rpv_veg)
 
+    # --- Hapke, Cox-Munk and Ross-Li (Ambrals) surfaces: parameter blocks
+    # [b0, hh, w], [u10, refractive index, do_shadow], [iso, vol, geo] ---
+    hapke = [1.0, 0.06, 0.6, 0.0, 0.0, 0.0, 0.0]
+    add("hapke_1", "tests/test_solver.cpp:4083-4102", 1, 16, 16, [1.0], [1.0], [_hg_rows(0.75, 17)],
+        ["user_tau", "user_mu", "hapke"], dict(beam_flux=pi, beam_mu=0.5),
+        six([1.03364126e+00, 4.09048726e-01, 1.29319940e-01], [1.42300629e-01, 2.55523611e+00, 1.00072067e+00],
+            1e-3, False),
+        tau_user=[0.0, 1.0], mu_user=umu6, phi_user=[0.0], rpv=hapke)
+    add("hapke_2", "tests/test_solver.cpp:4104-4122", 1, 16, 16, [0.5], [0.5], [_iso_rows(17)],
+        ["user_tau", "user_mu", "hapke"], dict(beam_flux=1.0, beam_mu=0.8),
+        six([4.86338877e-02, 4.47684548e-02, 3.66203890e-02], [1.72356855e-02, 2.73309928e-02, 3.98292120e-02],
+            1e-3, False),
+        tau_user=[0.0, 0.5], mu_user=umu6, phi_user=[0.0], rpv=hapke)
+
+    def transparent(name, src, kind, params, phis, flux_up, vals, zero_atol=None):
+        e = [F("flux_direct_beam", 0, 8.66025404e-01, rtol=1e-3), F("flux_up", 0, flux_up, rtol=1e-3)]
+        for j, row in enumerate(vals):
+            for iu, v in enumerate(row):
+                e.append(I(iu, 0, j, v, atol=zero_atol) if v == 0.0 else I(iu, 0, j, v, rtol=1e-3))
+        add(name, src, 1, 16, 16, [1e-10], [0.0], [_iso_rows(17)], ["user_tau", "user_mu", kind],
+            dict(beam_flux=1.0, beam_mu=mu30), e, tau_user=[0.0], mu_user=[0.1, 0.2, 0.5, 1.0], phi_user=phis,
+            rpv=params)
+
+    transparent("hapke_transparent", "tests/test_solver.cpp:4827-4893", "hapke", hapke, [0.0, 90.0, 180.0],
+                1.80952123e-01, [[7.77476519e-02, 7.54425629e-02, 6.93954190e-02, 5.36729764e-02],
+                                 [6.40460953e-02, 6.26773809e-02, 5.81122830e-02, 5.36729764e-02],
+                                 [5.19251669e-02, 5.17173008e-02, 5.00653878e-02, 5.36729764e-02]])
+    transparent("coxmunk_transparent", "tests/test_solver.cpp:4827-4862,4895-4926", "coxmunk",
+                [12.0, 1.34, 0.0, 0.0, 0.0, 0.0, 0.0], [0.0, 45.0, 90.0], 2.26858686e-02,
+                [[2.42487295e-03, 2.88842623e-03, 8.53078961e-03, 3.17964917e-02],
+                 [3.12678310e-05, 9.02852476e-05, 1.49471537e-03, 3.17964917e-02],
+                 [0.0, 0.0, 2.39371326e-05, 3.17964917e-02]], zero_atol=1e-5)
+    transparent("rossli_transparent", "tests/test_solver.cpp:4827-4862,4960-4990", "rossli",
+                [0.091, 0.02, 0.01, 0.0, 0.0, 0.0, 0.0], [0.0, 90.0, 180.0], 6.80276718e-02,
+                [[2.06350335e-02, 2.22951259e-02, 2.37114733e-02, 2.31711490e-02],
+                 [1.22654153e-02, 1.78864384e-02, 2.10323702e-02, 2.31711490e-02],
+                 [4.53790437e-03, 1.40564310e-02, 1.94761137e-02, 2.31711490e-02]])
+
+    def multilayer(name, src, kind, params, phis, fluxes, table):
+        e = [F("flux_direct_beam", 0, 8.66025404e-01, rtol=1e-3)]
+        e += [F(q, lu, v, rtol=1e-3) for (q, lu), v in fluxes.items()]
+        for (lu, j), vals in table.items():
+            e += [I(iu, lu, j, v, rtol=1e-3) for iu, v in enumerate(vals)]
+        add(name, src, 2, 16, 16, [0.32, 0.32], [1.0, 1.0], [_ray_rows(17), _hg_rows(0.7, 17)],
+            ["user_tau", "user_mu", kind], dict(beam_flux=1.0, beam_mu=mu30), e, tau_user=[0.0, 0.32, 0.64],
+            mu_user=[0.1, 0.2, 0.5, 1.0], phi_user=phis, rpv=params)
+
+    multilayer("hapke_multilayer", "tests/test_solver.cpp:5000-5088", "hapke", hapke, [0.0, 90.0, 180.0],
+               {("flux_up", 0): 2.83184981e-01, ("flux_up", 1): 1.91350794e-01, ("flux_down", 1): 1.75701065e-01,
+                ("flux_up", 2): 1.63014593e-01, ("flux_down", 2): 3.32252296e-01},
+               {(0, 0): [1.25984467e-01, 1.19170604e-01, 9.42674171e-02, 7.79502273e-02],
+                (1, 0): [1.23054866e-01, 1.05759903e-01, 7.46212352e-02, 5.09419742e-02],
+                (2, 0): [7.41448782e-02, 6.96104265e-02, 6.06974324e-02, 4.66773354e-02],
+                (0, 2): [1.33177919e-01, 1.26289727e-01, 1.02989027e-01, 7.79502273e-02]})
+    multilayer("coxmunk_multilayer", "tests/test_solver.cpp:5000-5049,5090-5127", "coxmunk",
+               [12.0, 1.34, 1.0, 0.0, 0.0, 0.0, 0.0], [0.0, 45.0, 90.0],
+               {("flux_up", 0): 1.78716405e-01, ("flux_up", 1): 5.75091570e-02, ("flux_down", 1): 1.46328004e-01,
+                ("flux_up", 2): 1.75230219e-02, ("flux_down", 2): 2.91229302e-01},
+               {(0, 0): [1.06393677e-01, 9.40595383e-02, 5.80077466e-02, 5.36951653e-02],
+                (2, 0): [1.85109472e-03, 2.29113269e-03, 5.36341643e-03, 2.53947801e-02],
+                (0, 2): [9.60808047e-02, 8.62969047e-02, 5.75538280e-02, 5.36951653e-02]})
+    multilayer("rossli_multilayer", "tests/test_solver.cpp:5000-5049,5164-5200", "rossli",
+               [0.091, 0.02, 0.01, 0.0, 0.0, 0.0, 0.0], [0.0, 90.0, 180.0],
+               {("flux_up", 0): 2.06450422e-01, ("flux_up", 1): 9.33373536e-02, ("flux_down", 1): 1.54422183e-01,
+                ("flux_up", 2): 5.65249803e-02, ("flux_down", 2): 3.02497242e-01},
+               {(0, 0): [1.11775573e-01, 1.00828251e-01, 6.73995934e-02, 5.31758411e-02],
+                (2, 0): [1.94024875e-02, 1.93417692e-02, 1.93954026e-02, 1.86845785e-02],
+                (0, 2): [1.19144501e-01, 1.09042063e-01, 8.01501157e-02, 5.31758411e-02],
+                (1, 1): [6.47230847e-02, 5.12384525e-02, 3.05390598e-02, 2.26323978e-02]})
+
     # --- Pseudo-spherical beam through the general solver (SphericalGeometry) ---
     add("spherical_general", "tests/test_solver.cpp:4459-4538", 2, 16, 0, [0.5, 1.0], [0.9, 0.8],
         [_hg_rows(0.8, 17)] * 2, ["user_mu", "spherical"], dict(beam_flux=1.0, beam_mu=0.5, albedo=0.1, radius=6371.0),

@@ -95,8 +95,8 @@ def test_index_from_bottom_validates_in_top_down_order():
 
 
 def test_general_solver_rejects_what_the_gpu_path_does_not_cover():
-    """Checked before any GPU work: SPECIAL_BC, temperature Jacobians, BRDFs
-    other than RPV, an RPV surface without parameters."""
+    """Checked before any GPU work: SPECIAL_BC, a BRDF selected without its
+    parameters (the reference's bidirReflectivity messages)."""
     cfg = d.DisortConfig(4, 16, 0)
     cfg.allocate()
     cfg.delta_tau = [0.1, 0.5, 1.0, 2.0]
@@ -105,8 +105,11 @@ def test_general_solver_rejects_what_the_gpu_path_does_not_cover():
     cfg.num_phi = 1
     cfg.phi_user = [0.0]
     cfg.flags.use_lambertian_surface = False
-    cfg.flags.brdf_type = d.BrdfType.Hapke
-    with pytest.raises(RuntimeError, match="a BRDF other than RPV"):
+    cfg.flags.brdf_type = d.BrdfType.CoxMunk
+    with pytest.raises(RuntimeError, match="Cox-Munk BRDF selected but cox_munk spec is null"):
+        d.DisortSolver().solve(cfg)
+    cfg.flags.brdf_type = d.BrdfType.Ambrals
+    with pytest.raises(RuntimeError, match="Ross-Li/Ambrals BRDF selected but ambrals spec is null"):
         d.DisortSolver().solve(cfg)
     cfg.flags.brdf_type = d.BrdfType.RPV
     with pytest.raises(RuntimeError, match="rpv spec is null"):

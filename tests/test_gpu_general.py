@@ -279,6 +279,29 @@ def test_buras_emde_layer_cut(ctx):
     solve_both(ctx, gb, "Buras-Emde with the absorption cut-off")
 
 
+BRDF_KINDS = {"hapke": (G.G_BRDF_HAPKE, lambda r: [r.uniform(0.5, 1.5), r.uniform(0.02, 0.1), r.uniform(0.3, 0.9)]),
+              "coxmunk": (G.G_BRDF_COXMUNK, lambda r: [r.uniform(2.0, 15.0), r.uniform(1.3, 1.4), float(r.integers(2))]),
+              "rossli": (G.G_BRDF_ROSSLI, lambda r: [r.uniform(0.05, 0.2), r.uniform(0.0, 0.05), r.uniform(0.0, 0.03)])}
+
+
+@pytest.mark.parametrize("kind,nstr,extra", [("hapke", 8, {"thermal": True}), ("coxmunk", 16, {"user_tau": True}),
+                                              ("rossli", 32, {"spherical": True}), ("hapke", 64, {}),
+                                              ("coxmunk", 8, {"thermal": True}), ("rossli", 16, {"thermal": True})])
+def test_random_brdf_kinds(ctx, kind, nstr, extra):
+    # Hapke, Cox-Munk and Ross-Li surfaces with their own parameters per
+    # column: BRDF Fourier coefficients, emissivities (thermal), the BRDF
+    # beam-reflection and Nakajima-Tanaka corrections.
+    flag, draw = BRDF_KINDS[kind]
+    gb = random_general(3, 4, nstr, seed=200 + nstr, rpv=True, corr=True, numu=6, nphi=3, **extra)
+    gb.flags = (gb.flags & ~G.G_BRDF_RPV) | flag
+    r = np.random.default_rng(nstr)
+    gb.rpv[:] = 0.0
+    for i in range(gb.n):
+        p = draw(r)
+        gb.rpv[i, :len(p)] = p
+    solve_both(ctx, gb, f"{kind} surface nstr={nstr} {sorted(extra)}")
+
+
 def test_validation_messages(ctx):
     from disort_amd._capi import DisortInvalid
 

@@ -16,7 +16,8 @@ from conftest import printed_ulp
 HERE = os.path.dirname(os.path.abspath(__file__))
 CASES = json.load(open(os.path.join(HERE, "golden", "intensity_cases.json")))
 FLAG = {"thermal": 1, "spherical": 2, "delta_m_plus": 4, "user_tau": G.G_USER_TAU, "user_mu": G.G_USER_MU,
-        "only_fluxes": G.G_ONLY_FLUXES, "rpv": G.G_BRDF_RPV, "corr_buras": G.G_CORR_BURAS,
+        "only_fluxes": G.G_ONLY_FLUXES, "rpv": G.G_BRDF_RPV, "hapke": G.G_BRDF_HAPKE,
+        "coxmunk": G.G_BRDF_COXMUNK, "rossli": G.G_BRDF_ROSSLI, "corr_buras": G.G_CORR_BURAS,
         "corr_nakajima": G.G_CORR_NAKAJIMA}
 
 

@@ -323,7 +323,7 @@ __global__ __launch_bounds__(64) void k_column(FluxArgs a) {
       a.lay_d[LD_BT_BOT * TL + g] = bt;
       bt_prev = bt;
     }
-    const bool lyrcut = (abst >= 10.0 && !thermal && L > 1);
+    const bool lyrcut = (abst >= 10.0 && !thermal && L > 1 && !(a.flags & kFluxNoCutoff));
     const int ncut_f = lyrcut ? ncut : L;
     a.inst_i[i * kInstI + II_NCUT] = ncut_f;
     a.inst_i[i * kInstI + II_LYRCUT] = lyrcut ? 1 : 0;

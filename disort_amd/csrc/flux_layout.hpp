@@ -21,6 +21,10 @@ struct TabOff {
   __host__ __device__ constexpr int size() const { return Q() + n * nn; }
 };
 
+// FluxArgs::flags bit (beside the DISORT_F_* bits) of batches solved
+// without absorption cut-off (general solver, DISORT_G_NO_CUTOFF).
+constexpr int32_t kFluxNoCutoff = 1 << 20;
+
 // Per-instance record (doubles), index inst*kInstD + field.
 enum InstD {
   ID_PL_TOP = 0,      // emissivity_top * B(T_top)

@@ -352,6 +352,7 @@ extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch*
   if ((rc = get_buf(ctx, "g_ulyr", (size_t)n * ntau * 4, &d_ulyr))) return rc;
   if ((rc = get_buf(ctx, "g_utaus", (size_t)n * ntau * 8, &d_utau))) return rc;
   fa.flags = b->flags & (DISORT_F_THERMAL | DISORT_F_SPHERICAL | DISORT_F_DELTA_M_PLUS | DISORT_F_DIFFUSION_BC);
+  if (b->flags & DISORT_G_NO_CUTOFF) fa.flags |= kFluxNoCutoff;
   fa.nstr = (const int32_t*)d_vn;
   fa.nlyr = (const int32_t*)d_vl;
   fa.pstride = (const int32_t*)d_vp;

@@ -25,6 +25,7 @@ G_BRDF_HAPKE = 1024
 G_BRDF_COXMUNK = 2048
 G_BRDF_ROSSLI = 4096
 G_BRDF_ANY = G_BRDF_RPV | G_BRDF_HAPKE | G_BRDF_COXMUNK | G_BRDF_ROSSLI
+G_NO_CUTOFF = 8192
 S_BEAM_PHI = 11
 NRPV = 7
 

@@ -614,7 +614,6 @@ void require_gpu_subset(const DisortConfig& c) {
   auto no = [](const std::string& what) {
     throw std::runtime_error("disortpp: " + what + " is not available on the GPU path");
   };
-  if (c.flags.ibcnd != BoundaryConditionType::General) no("SPECIAL_BC");
   if (c.flags.compute_temperature_jacobian) no("compute_temperature_jacobian");
   if (!c.flags.use_lambertian_surface) {
     // Missing specifications fail as bidirReflectivity does
@@ -722,6 +721,7 @@ int32_t full_flags(const DisortConfig& c) {
 
 struct GeneralPack {
   int32_t flags = 0;
+  int32_t force_flags = 0;  // ABI flags added to every column's (DISORT_G_NO_CUTOFF)
   int N = 0, L = 0, P = 0, ntau = 0, numu = 0, nphi = 0;
   double accuracy = 0.0;
   std::vector<double> dtau, ssa, pmom, temper, alt, scal, rpv, tau_user, mu_user, phi_user, flux, inten, avg;
@@ -738,7 +738,7 @@ struct GeneralPack {
   // Whether config c can join this batch.
   bool fits(const DisortConfig& c) const {
     if (n == 0) return true;
-    return full_flags(c) == flags && c.num_streams == N && c.num_layers == L && c.nmomNstr() + 1 == P &&
+    return (full_flags(c) | force_flags) == flags && c.num_streams == N && c.num_layers == L && c.nmomNstr() + 1 == P &&
            c.num_user_tau == ntau && c.num_user_mu == numu && (c.flags.comp_only_fluxes ? 0 : c.num_phi) == nphi &&
            c.accuracy_fourier_series == accuracy &&
            std::equal(mu_user.begin(), mu_user.end(), c.mu_user.begin()) &&
@@ -753,7 +753,7 @@ struct GeneralPack {
     DisortConfig c = cin;
     if (c.flags.index_from_bottom) reverse_general_inputs(c);
     if (n == 0) {
-      flags = full_flags(c);
+      flags = full_flags(c) | force_flags;
       N = c.num_streams;
       L = c.num_layers;
       P = c.nmomNstr() + 1;
@@ -904,10 +904,173 @@ struct GeneralPack {
   }
 };
 
+// SPECIAL_BC (src/DisortSolver.cpp:4025-4163): albedo and transmissivity
+// of the medium at the positive user polar angles (the quadrature nodes
+// without user angles) from its response to unit isotropic illumination at
+// the top and, with several layers, at the bottom, over a black surface,
+// without beam, thermal sources or absorption cut-off; with surface_albedo > 0
+// the spherical-albedo correction (:4126-4153).  Each illumination is one
+// GPU general solve of the same medium at the doubled angles -mu .. +mu: its
+// azimuthally averaged intensity at the top (bottom) is the reference's
+// source integral plus the transmitted unit boundary intensity exp(-tau*/mu)
+// that the reference adds, and its fluxes are pi times the spherical albedo
+// and transmissivity sums.  The config is left as the reference leaves it.
+std::vector<DisortResult> special_bc_solve(std::vector<DisortConfig*>& cs) {
+  const size_t n = cs.size();
+  std::vector<DisortConfig> base(n);
+  std::vector<int> half(n);
+  std::vector<std::vector<double>> ref_mu(n);
+  std::vector<std::vector<int>> rank(n);
+  for (size_t i = 0; i < n; ++i) {
+    DisortConfig b = *cs[i];
+    if (b.flags.index_from_bottom) reverse_general_inputs(b);
+    std::vector<double> pos;
+    if (b.flags.use_user_mu) {
+      pos.assign(b.mu_user.begin(), b.mu_user.begin() + b.num_user_mu);
+    } else {
+      std::vector<double> cmu(b.num_streams), cwt(b.num_streams);
+      if (disort_quadrature(b.num_streams, cmu.data(), cwt.data()) != DISORT_OK)
+        throw std::runtime_error("gaussianQuadrature: Failed to converge");
+      pos.assign(cmu.begin(), cmu.begin() + b.num_streams / 2);
+    }
+    const int h = static_cast<int>(pos.size());
+    half[i] = h;
+    // The reference's angle order [-mu_h .. -mu_1, mu_1 .. mu_h] (the
+    // quadrature nodes come in decreasing order); the base solves take the
+    // same cosines sorted increasing, and rank[k] places mu_k among them.
+    ref_mu[i].assign(2 * h, 0.0);
+    for (int k = 0; k < h; ++k) {
+      ref_mu[i][h + k] = pos[k];
+      ref_mu[i][k] = -pos[h - 1 - k];
+    }
+    std::vector<double> sorted = pos;
+    std::sort(sorted.begin(), sorted.end());
+    rank[i].resize(h);
+    for (int k = 0; k < h; ++k)
+      rank[i][k] = static_cast<int>(std::lower_bound(sorted.begin(), sorted.end(), pos[k]) - sorted.begin());
+    b.flags = DisortFlags{};
+    b.flags.use_user_mu = true;
+    b.flags.use_delta_m_plus = cs[i]->flags.use_delta_m_plus;
+    b.num_user_mu = 2 * h;
+    b.mu_user.assign(2 * h, 0.0);
+    for (int k = 0; k < h; ++k) {
+      b.mu_user[h + k] = sorted[k];
+      b.mu_user[k] = -sorted[h - 1 - k];
+    }
+    b.num_user_tau = b.num_layers + 1;
+    b.tau_user.clear();
+    b.num_phi = 1;
+    b.phi_user.assign(1, 0.0);
+    b.accuracy_fourier_series = 0.0;
+    b.num_phase_func_angles = 0;
+    b.bc = BoundaryConditions{};
+    b.bc.isotropic_flux_top = 1.0;
+    b.bc.emissivity_top = cs[i]->bc.emissivity_top;
+    std::vector<double> cmu;
+    prepare_general(b, cmu);
+    base[i] = std::move(b);
+  }
+  // One GPU general batch per illumination and shape.
+  auto run = [&](std::vector<DisortConfig>& cfgs, const std::vector<size_t>& which) {
+    std::vector<DisortResult> out(cfgs.size());
+    std::vector<bool> done(cfgs.size(), true);
+    for (size_t i : which) done[i] = false;
+    for (size_t i0 = 0; i0 < cfgs.size(); ++i0) {
+      if (done[i0]) continue;
+      GeneralPack g;
+      g.force_flags = DISORT_G_NO_CUTOFF;
+      std::vector<size_t> idx;
+      for (size_t i = i0; i < cfgs.size(); ++i)
+        if (!done[i] && g.fits(cfgs[i])) {
+          g.add(cfgs[i]);
+          idx.push_back(i);
+          done[i] = true;
+        }
+      g.solve();
+      for (size_t j = 0; j < idx.size(); ++j) out[idx[j]] = g.unpack(static_cast<int64_t>(j), cfgs[idx[j]]);
+    }
+    return out;
+  };
+  std::vector<size_t> all(n), multi;
+  for (size_t i = 0; i < n; ++i) {
+    all[i] = i;
+    if (base[i].num_layers > 1) multi.push_back(i);
+  }
+  const std::vector<DisortResult> top = run(base, all);
+  std::vector<DisortConfig> bottom = base;
+  for (size_t i : multi) {
+    bottom[i].bc.isotropic_flux_top = 0.0;
+    bottom[i].bc.isotropic_flux_bottom = 1.0;
+  }
+  const std::vector<DisortResult> bot = run(bottom, multi);
+
+  std::vector<DisortResult> res(n);
+  for (size_t i = 0; i < n; ++i) {
+    DisortConfig& c = *cs[i];
+    const int h = half[i], L = c.num_layers;
+    DisortResult& r = res[i];
+    const int ntau = c.flags.use_user_tau ? c.num_user_tau : L + 1;
+    for (auto* f : {&r.flux_direct_beam, &r.flux_down, &r.flux_up, &r.flux_tau_divergence, &r.mean_intensity,
+                    &r.mean_intensity_down, &r.mean_intensity_up, &r.mean_intensity_direct_beam})
+      f->assign(ntau, 0.0);
+    r.mu_angles = ref_mu[i];
+    if (!c.flags.comp_only_fluxes) {
+      r.intensity.assign(ntau, std::vector<std::vector<double>>(2 * h, std::vector<double>(c.num_phi, 0.0)));
+      r.intensity_azimuth_avg.assign(ntau, std::vector<double>(2 * h, 0.0));
+    }
+    r.albedo_medium.assign(2 * h, 0.0);
+    r.transmissivity_medium.assign(2 * h, 0.0);
+    const DisortResult& a = top[i];
+    const std::vector<int>& rk = rank[i];
+    for (int k = 0; k < h; ++k) r.albedo_medium[k] = a.intensity_azimuth_avg[0][h + rk[k]];
+    const DisortResult* sph = &a;
+    if (L == 1) {
+      for (int k = 0; k < h; ++k) r.transmissivity_medium[k] = a.intensity_azimuth_avg[L][h - 1 - rk[k]];
+    } else {
+      for (int k = 0; k < h; ++k) r.transmissivity_medium[k] = bot[i].intensity_azimuth_avg[0][h + rk[k]];
+      sph = &bot[i];
+    }
+    const double alb = c.bc.surface_albedo;
+    if (alb > 0.0) {
+      const double sflup = sph->flux_up[0] / M_PI, sfldn = sph->flux_down[L] / M_PI;
+      const double sphalb = (L == 1) ? sflup : sfldn, sphtrn = (L == 1) ? sfldn : sflup;
+      const double corr = alb / (1.0 - alb * sphalb);
+      for (int k = 0; k < h; ++k) {
+        const double t = r.transmissivity_medium[k];
+        r.albedo_medium[k] += corr * sphtrn * t;
+        r.transmissivity_medium[k] += corr * sphalb * t;
+      }
+      r.spherical_albedo = sphalb;
+      r.spherical_transmissivity = sphtrn;
+    }
+    // What the reference leaves in the config (:48-66, :842-865,
+    // :4155-4162): user levels at the layer boundaries, num_user_mu = h,
+    // and mu_user the doubled angles (2h + 1 entries with user angles, the
+    // 2h quadrature angles without) whose entries [1..h] are then taken
+    // from [1+h..2h] (a 1-based copy loop over a 0-based array).
+    if (!c.flags.use_user_tau) {
+      c.num_user_tau = L + 1;
+      c.tau_user.resize(L + 1);
+    }
+    std::vector<double> m = ref_mu[i];
+    if (c.flags.use_user_mu) m.resize(2 * h + 1, 0.0);
+    for (int iu = 1; iu <= h && iu + h < static_cast<int>(m.size()); ++iu) m[iu] = m[iu + h];
+    c.mu_user = m;
+    c.num_user_mu = h;
+  }
+  return res;
+}
+
 }  // namespace
 
 DisortResult DisortSolver::solve(DisortConfig& config) {
   std::vector<double> cmu;
+  if (config.flags.ibcnd == BoundaryConditionType::Special) {
+    if (config.delta_tau.empty()) config.allocate();
+    config.validate();
+    std::vector<DisortConfig*> one{&config};
+    return special_bc_solve(one)[0];
+  }
   prepare_general(config, cmu);
   if (flux_route(config)) {
     Packed p;
@@ -938,9 +1101,26 @@ std::vector<DisortResult> solveSpectralBands(DisortConfig config,
       if (!ov.isotropic_flux_top.empty()) x.bc.isotropic_flux_top = ov.isotropic_flux_top.at(i);
       if (!ov.isotropic_flux_bottom.empty()) x.bc.isotropic_flux_bottom = ov.isotropic_flux_bottom.at(i);
     });
-    prepare_general(c, cmu);
+    if (c.flags.ibcnd == BoundaryConditionType::Special) {
+      if (c.delta_tau.empty()) c.allocate();
+      c.validate();
+    } else {
+      prepare_general(c, cmu);
+    }
   }
   std::vector<DisortResult> results(bands.size());
+  // SPECIAL_BC bands: albedo and transmissivity of the medium.
+  std::vector<DisortConfig*> special;
+  std::vector<size_t> special_idx;
+  for (size_t i = 0; i < cfgs.size(); ++i)
+    if (cfgs[i].flags.ibcnd == BoundaryConditionType::Special) {
+      special.push_back(&cfgs[i]);
+      special_idx.push_back(i);
+    }
+  if (!special.empty()) {
+    std::vector<DisortResult> r = special_bc_solve(special);
+    for (size_t j = 0; j < special_idx.size(); ++j) results[special_idx[j]] = std::move(r[j]);
+  }
   // Flux-only bands: one flux batch per batch-uniform flag set.
   std::vector<int32_t> seen;
   for (const auto& c : cfgs)
@@ -960,7 +1140,8 @@ std::vector<DisortResult> solveSpectralBands(DisortConfig config,
   }
   // The rest: full-solver batches of equal shape.
   std::vector<bool> done(cfgs.size(), false);
-  for (size_t i = 0; i < cfgs.size(); ++i) done[i] = flux_route(cfgs[i]);
+  for (size_t i = 0; i < cfgs.size(); ++i)
+    done[i] = flux_route(cfgs[i]) || cfgs[i].flags.ibcnd == BoundaryConditionType::Special;
   for (size_t i0 = 0; i0 < cfgs.size(); ++i0) {
     if (done[i0]) continue;
     GeneralPack g;

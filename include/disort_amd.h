@@ -269,7 +269,9 @@ enum {
   DISORT_G_CORR_NAKAJIMA = 512, /* flags.intensity_corr_nakajima               */
   DISORT_G_BRDF_HAPKE = 1024,   /* use_lambertian_surface = false, BrdfType::Hapke   */
   DISORT_G_BRDF_COXMUNK = 2048, /* use_lambertian_surface = false, BrdfType::CoxMunk */
-  DISORT_G_BRDF_ROSSLI = 4096   /* use_lambertian_surface = false, BrdfType::Ambrals */
+  DISORT_G_BRDF_ROSSLI = 4096,  /* use_lambertian_surface = false, BrdfType::Ambrals */
+  DISORT_G_NO_CUTOFF = 8192     /* no absorption cut-off (ncut = nlyr, lyrcut = false), as the
+                                   SPECIAL_BC base solves of src/DisortSolver.cpp:4037-4039 */
 };
 /* BRDF parameter block rpv[i*DISORT_NRPV + slot], one of the BRDF flags set
  * (src/DisortConfig.hpp:52-96): RPV (RpvBrdfSpec) in the slots below; Hapke

@@ -154,6 +154,63 @@ int64_t oracle_flux_batch(const disort_flux_batch* b, int nthreads, double* seco
   return failed;
 }
 
+// Column i of a general batch as an oracle configuration.
+static oracle::GeneralConfig general_config(const disort_general_batch* b, int64_t i, int index_from_bottom) {
+  oracle::GeneralConfig c;
+  const int L = b->nlyr, P = b->pstride;
+  c.nlyr = L;
+  c.nstr = b->nstr;
+  c.nmom = P - 1;
+  c.thermal = (b->flags & DISORT_F_THERMAL) != 0;
+  c.spherical = (b->flags & DISORT_F_SPHERICAL) != 0;
+  c.deltam_plus = (b->flags & DISORT_F_DELTA_M_PLUS) != 0;
+  c.diffusion_bc = (b->flags & DISORT_F_DIFFUSION_BC) != 0;
+  c.use_user_tau = (b->flags & DISORT_G_USER_TAU) != 0;
+  c.use_user_mu = (b->flags & DISORT_G_USER_MU) != 0;
+  c.only_fluxes = (b->flags & DISORT_G_ONLY_FLUXES) != 0;
+  const int bf = b->flags;
+  c.brdf = (bf & DISORT_G_BRDF_RPV)       ? oracle::BRDF_RPV
+           : (bf & DISORT_G_BRDF_HAPKE)   ? oracle::BRDF_HAPKE
+           : (bf & DISORT_G_BRDF_COXMUNK) ? oracle::BRDF_COXMUNK
+           : (bf & DISORT_G_BRDF_ROSSLI)  ? oracle::BRDF_ROSSLI
+                                          : oracle::BRDF_NONE;
+  c.lambertian = c.brdf == oracle::BRDF_NONE;
+  c.corr_buras = (b->flags & DISORT_G_CORR_BURAS) != 0;
+  c.corr_nakajima = (b->flags & DISORT_G_CORR_NAKAJIMA) != 0;
+  c.index_from_bottom = index_from_bottom != 0;
+  c.no_cutoff = (b->flags & DISORT_G_NO_CUTOFF) != 0;
+  c.accur = b->accuracy;
+  const double* s = b->scal + i * DISORT_NSCAL;
+  c.beam_flux = s[DISORT_S_BEAM_FLUX];
+  c.beam_mu = s[DISORT_S_BEAM_MU];
+  c.beam_phi = s[DISORT_S_BEAM_PHI];
+  c.iso_top = s[DISORT_S_ISO_TOP];
+  c.iso_bot = s[DISORT_S_ISO_BOT];
+  c.temp_top = s[DISORT_S_TEMP_TOP];
+  c.temp_bot = s[DISORT_S_TEMP_BOT];
+  c.emis_top = s[DISORT_S_EMIS_TOP];
+  c.albedo = s[DISORT_S_ALBEDO];
+  c.wn_lo = s[DISORT_S_WN_LO];
+  c.wn_hi = s[DISORT_S_WN_HI];
+  c.radius = s[DISORT_S_RADIUS];
+  if (!c.lambertian && b->rpv)
+    for (int k = 0; k < DISORT_NRPV; ++k) c.rpv[k] = b->rpv[i * DISORT_NRPV + k];
+  c.dtau.assign(b->dtau + i * L, b->dtau + (i + 1) * L);
+  c.ssa.assign(b->ssa + i * L, b->ssa + (i + 1) * L);
+  c.pmom.assign(b->pmom + i * L * P, b->pmom + (i + 1) * L * P);
+  if (c.thermal && b->temper) c.temper.assign(b->temper + i * (L + 1), b->temper + (i + 1) * (L + 1));
+  if (c.spherical && b->alt) c.alt.assign(b->alt + i * (L + 1), b->alt + (i + 1) * (L + 1));
+  if (c.use_user_tau && b->tau_user) c.tau_user.assign(b->tau_user + i * b->ntau, b->tau_user + (i + 1) * b->ntau);
+  if (c.use_user_mu && b->mu_user) c.mu_user.assign(b->mu_user, b->mu_user + b->numu);
+  if (b->phi_user) c.phi_user.assign(b->phi_user, b->phi_user + b->nphi);
+  if (b->nphase > 0 && b->mu_phase && b->phase) {
+    c.nphase = b->nphase;
+    c.mu_phase.assign(b->mu_phase, b->mu_phase + b->nphase);
+    c.phase.assign(b->phase + (size_t)i * L * b->nphase, b->phase + (size_t)(i + 1) * L * b->nphase);
+  }
+  return c;
+}
+
 /* One column of a general batch (DisortSolver::solve).  flux is
  * [8][ntau], inten [ntau][numu][nphi], inten_avg [ntau][numu]; *modes gets
  * the number of Fourier modes computed.  Returns 0, 1 (invalid argument) or
@@ -161,62 +218,39 @@ int64_t oracle_flux_batch(const disort_flux_batch* b, int nthreads, double* seco
 int oracle_general_one(const disort_general_batch* b, int64_t i, int index_from_bottom, double* flux,
                        double* inten, double* inten_avg, int* modes, char* err, int errlen) {
   try {
-    oracle::GeneralConfig c;
-    const int L = b->nlyr, P = b->pstride;
-    c.nlyr = L;
-    c.nstr = b->nstr;
-    c.nmom = P - 1;
-    c.thermal = (b->flags & DISORT_F_THERMAL) != 0;
-    c.spherical = (b->flags & DISORT_F_SPHERICAL) != 0;
-    c.deltam_plus = (b->flags & DISORT_F_DELTA_M_PLUS) != 0;
-    c.diffusion_bc = (b->flags & DISORT_F_DIFFUSION_BC) != 0;
-    c.use_user_tau = (b->flags & DISORT_G_USER_TAU) != 0;
-    c.use_user_mu = (b->flags & DISORT_G_USER_MU) != 0;
-    c.only_fluxes = (b->flags & DISORT_G_ONLY_FLUXES) != 0;
-    const int bf = b->flags;
-    c.brdf = (bf & DISORT_G_BRDF_RPV)       ? oracle::BRDF_RPV
-             : (bf & DISORT_G_BRDF_HAPKE)   ? oracle::BRDF_HAPKE
-             : (bf & DISORT_G_BRDF_COXMUNK) ? oracle::BRDF_COXMUNK
-             : (bf & DISORT_G_BRDF_ROSSLI)  ? oracle::BRDF_ROSSLI
-                                            : oracle::BRDF_NONE;
-    c.lambertian = c.brdf == oracle::BRDF_NONE;
-    c.corr_buras = (b->flags & DISORT_G_CORR_BURAS) != 0;
-    c.corr_nakajima = (b->flags & DISORT_G_CORR_NAKAJIMA) != 0;
-    c.index_from_bottom = index_from_bottom != 0;
-    c.accur = b->accuracy;
-    const double* s = b->scal + i * DISORT_NSCAL;
-    c.beam_flux = s[DISORT_S_BEAM_FLUX];
-    c.beam_mu = s[DISORT_S_BEAM_MU];
-    c.beam_phi = s[DISORT_S_BEAM_PHI];
-    c.iso_top = s[DISORT_S_ISO_TOP];
-    c.iso_bot = s[DISORT_S_ISO_BOT];
-    c.temp_top = s[DISORT_S_TEMP_TOP];
-    c.temp_bot = s[DISORT_S_TEMP_BOT];
-    c.emis_top = s[DISORT_S_EMIS_TOP];
-    c.albedo = s[DISORT_S_ALBEDO];
-    c.wn_lo = s[DISORT_S_WN_LO];
-    c.wn_hi = s[DISORT_S_WN_HI];
-    c.radius = s[DISORT_S_RADIUS];
-    if (!c.lambertian && b->rpv)
-      for (int k = 0; k < DISORT_NRPV; ++k) c.rpv[k] = b->rpv[i * DISORT_NRPV + k];
-    c.dtau.assign(b->dtau + i * L, b->dtau + (i + 1) * L);
-    c.ssa.assign(b->ssa + i * L, b->ssa + (i + 1) * L);
-    c.pmom.assign(b->pmom + i * L * P, b->pmom + (i + 1) * L * P);
-    if (c.thermal && b->temper) c.temper.assign(b->temper + i * (L + 1), b->temper + (i + 1) * (L + 1));
-    if (c.spherical && b->alt) c.alt.assign(b->alt + i * (L + 1), b->alt + (i + 1) * (L + 1));
-    if (c.use_user_tau && b->tau_user) c.tau_user.assign(b->tau_user + i * b->ntau, b->tau_user + (i + 1) * b->ntau);
-    if (c.use_user_mu && b->mu_user) c.mu_user.assign(b->mu_user, b->mu_user + b->numu);
-    if (b->phi_user) c.phi_user.assign(b->phi_user, b->phi_user + b->nphi);
-    if (b->nphase > 0 && b->mu_phase && b->phase) {
-      c.nphase = b->nphase;
-      c.mu_phase.assign(b->mu_phase, b->mu_phase + b->nphase);
-      c.phase.assign(b->phase + (size_t)i * L * b->nphase, b->phase + (size_t)(i + 1) * L * b->nphase);
-    }
+    const oracle::GeneralConfig c = general_config(b, i, index_from_bottom);
     oracle::GeneralOutput o = oracle::general_solve(c);
+    if (o.ntau != b->ntau || o.numu != b->numu || o.nphi != (b->flags & DISORT_G_ONLY_FLUXES ? 0 : b->nphi))
+      throw std::invalid_argument("oracle: output shape differs from the batch's ntau/numu/nphi");
     for (int f = 0; f < 8; ++f) std::memcpy(flux + (size_t)f * o.ntau, o.f[f].data(), sizeof(double) * o.ntau);
     if (inten) std::memcpy(inten, o.inten.data(), sizeof(double) * o.inten.size());
     if (inten_avg) std::memcpy(inten_avg, o.inten_avg.data(), sizeof(double) * o.inten_avg.size());
     if (modes) *modes = o.modes;
+    return 0;
+  } catch (const std::invalid_argument& e) {
+    put_err(err, errlen, e.what());
+    return 1;
+  } catch (const std::exception& e) {
+    put_err(err, errlen, e.what());
+    return 2;
+  }
+}
+
+/* SPECIAL_BC of column i of a general batch (DisortSolver::solve with
+ * ibcnd = SPECIAL_BC): albedo and transmissivity of the medium [2 h] (h =
+ * positive user polar angles, or nstr/2 without DISORT_G_USER_MU; the first
+ * h filled), spherical albedo and transmissivity sph[2].  Returns 0, 1 or 2
+ * as oracle_general_one. */
+int oracle_general_albtrans(const disort_general_batch* b, int64_t i, double* albedo, double* trans, double* sph,
+                            char* err, int errlen) {
+  try {
+    oracle::GeneralConfig c = general_config(b, i, 0);
+    c.special_bc = true;
+    const oracle::GeneralOutput o = oracle::general_solve(c);
+    std::memcpy(albedo, o.albedo_medium.data(), sizeof(double) * o.albedo_medium.size());
+    std::memcpy(trans, o.transmissivity_medium.data(), sizeof(double) * o.transmissivity_medium.size());
+    sph[0] = o.spherical_albedo;
+    sph[1] = o.spherical_transmissivity;
     return 0;
   } catch (const std::invalid_argument& e) {
     put_err(err, errlen, e.what());

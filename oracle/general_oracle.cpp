@@ -176,7 +176,7 @@ void validate_general_config(const GeneralConfig& c) {
     if ((int)c.alt.size() < c.nlyr + 1)
       bad("level_altitudes array is too small (need num_layers+1 when use_spherical_beam=true)");
   }
-  if (!c.only_fluxes) {
+  if (!c.only_fluxes && !c.special_bc) {
     if (c.phi_user.empty()) bad("num_phi must be positive when comp_only_fluxes=false");
     for (size_t j = 0; j < c.phi_user.size(); ++j)
       if (c.phi_user[j] < 0.0 || c.phi_user[j] > 360.0) bad("phi_user[" + to_string(j) + "] must be in [0, 360]");
@@ -308,7 +308,7 @@ struct GeneralSolver {
         }
       }
     }
-    lyrcut = (abstau >= abscut && !c.thermal && L > 1);
+    lyrcut = (abstau >= abscut && !c.thermal && L > 1 && !c.no_cutoff);
     if (!lyrcut) ncut = L;
     // User levels (src/DisortSolver.cpp:803-820).
     lyr_of.assign(ntau, 0);
@@ -1582,7 +1582,110 @@ struct GeneralSolver {
 
 }  // namespace
 
+// SPECIAL_BC (src/DisortSolver.cpp:4025-4163): the albedo and the
+// transmissivity of the medium from its response to unit isotropic
+// illumination at the top (and, with several layers, at the bottom) over a
+// black surface, without sources or absorption cut-off, at the user polar
+// angles doubled to -mu .. +mu (:54-58, :854-865); with surface_albedo > 0
+// the spherical-albedo correction (:4126-4153).  Each illumination is one
+// general solve of the same medium: the azimuthally averaged intensity it
+// gives at the top (bottom) is the reference's source integral plus the
+// directly transmitted unit boundary intensity exp(-tau*/mu), and its
+// fluxes are pi times the spherical albedo / transmissivity sums.
+GeneralOutput special_bc_solve(const GeneralConfig& c) {
+  GeneralConfig b = c;
+  b.special_bc = false;
+  b.no_cutoff = true;
+  b.index_from_bottom = false;
+  b.beam_flux = 0.0;
+  b.iso_top = 1.0;
+  b.iso_bot = 0.0;
+  b.albedo = 0.0;
+  b.lambertian = true;
+  b.brdf = BRDF_NONE;
+  b.thermal = false;
+  b.spherical = false;
+  b.diffusion_bc = false;
+  b.only_fluxes = false;
+  b.corr_buras = b.corr_nakajima = false;
+  b.use_user_tau = false;
+  b.tau_user.clear();
+  b.accur = 0.0;
+  b.phi_user.assign(1, 0.0);
+  b.nphase = 0;
+  // Positive user cosines mu_1 < ... < mu_h, or the positive quadrature
+  // nodes without user angles; doubled to [-mu_h .. -mu_1, mu_1 .. mu_h].
+  std::vector<Real> pos;
+  if (c.use_user_mu) {
+    pos = c.mu_user;
+  } else {
+    std::vector<Real> cm(c.nstr), cw(c.nstr);
+    gauss_quadrature(c.nstr / 2, cm.data(), cw.data());
+    pos.assign(cm.begin(), cm.begin() + c.nstr / 2);
+  }
+  const int h = (int)pos.size();
+  // The base solves take the cosines sorted increasing (the quadrature
+  // nodes come in decreasing order); rank[k] places mu_k among them.
+  std::vector<Real> sorted = pos;
+  std::sort(sorted.begin(), sorted.end());
+  std::vector<int> rank(h);
+  for (int k = 0; k < h; ++k) rank[k] = (int)(std::lower_bound(sorted.begin(), sorted.end(), pos[k]) - sorted.begin());
+  b.use_user_mu = true;
+  b.mu_user.assign(2 * h, 0.0);
+  for (int k = 0; k < h; ++k) {
+    b.mu_user[h + k] = sorted[k];
+    b.mu_user[k] = -sorted[h - 1 - k];
+  }
+  const int L = c.nlyr;
+  GeneralOutput top = general_solve(b);
+  GeneralOutput out;
+  out.ntau = L + 1;
+  out.numu = 2 * h;
+  out.nphi = (int)c.phi_user.size();
+  for (auto& f : out.f) f.assign(out.ntau, 0.0);
+  out.inten.assign((size_t)out.ntau * out.numu * out.nphi, 0.0);
+  out.inten_avg.assign((size_t)out.ntau * out.numu, 0.0);
+  out.mu_angles.assign(2 * h, 0.0);
+  for (int k = 0; k < h; ++k) {
+    out.mu_angles[h + k] = pos[k];
+    out.mu_angles[k] = -pos[h - 1 - k];
+  }
+  out.modes = top.modes;
+  out.albedo_medium.assign(2 * h, 0.0);
+  out.transmissivity_medium.assign(2 * h, 0.0);
+  auto avg = [&](const GeneralOutput& o, int lu, int iu) { return o.inten_avg[(size_t)lu * o.numu + iu]; };
+  for (int k = 0; k < h; ++k) out.albedo_medium[k] = avg(top, 0, h + rank[k]);
+  GeneralOutput bot;
+  const GeneralOutput* sph = &top;
+  if (L == 1) {
+    for (int k = 0; k < h; ++k) out.transmissivity_medium[k] = avg(top, L, h - 1 - rank[k]);
+  } else {
+    b.iso_top = 0.0;
+    b.iso_bot = 1.0;
+    bot = general_solve(b);
+    for (int k = 0; k < h; ++k) out.transmissivity_medium[k] = avg(bot, 0, h + rank[k]);
+    sph = &bot;
+  }
+  if (c.albedo > 0.0) {
+    const Real sflup = sph->f[2][0] / M_PI, sfldn = sph->f[1][L] / M_PI;
+    const Real sphalb = (L == 1) ? sflup : sfldn, sphtrn = (L == 1) ? sfldn : sflup;
+    const Real corr = c.albedo / (1.0 - c.albedo * sphalb);
+    for (int k = 0; k < h; ++k) {
+      const Real t = out.transmissivity_medium[k];
+      out.albedo_medium[k] += corr * sphtrn * t;
+      out.transmissivity_medium[k] += corr * sphalb * t;
+    }
+    out.spherical_albedo = sphalb;
+    out.spherical_transmissivity = sphtrn;
+  }
+  return out;
+}
+
 GeneralOutput general_solve(const GeneralConfig& cin) {
+  if (cin.special_bc) {
+    validate_general_config(cin);
+    return special_bc_solve(cin);
+  }
   if (!cin.index_from_bottom) {
     validate_general_config(cin);
     GeneralSolver s(cin);

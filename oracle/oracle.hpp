@@ -138,6 +138,9 @@ struct GeneralConfig {
   bool thermal = false, spherical = false, deltam_plus = false, diffusion_bc = false;
   bool use_user_tau = false, use_user_mu = false, only_fluxes = false;
   bool lambertian = true, corr_buras = false, corr_nakajima = false, index_from_bottom = false;
+  // special_bc: ibcnd = SPECIAL_BC (albedo and transmissivity of the medium);
+  // no_cutoff: no absorption cut-off (the base solves of SPECIAL_BC).
+  bool special_bc = false, no_cutoff = false;
   int brdf = BRDF_NONE;
   Real rpv[7] = {0, 0, 0, 0, 0, 0, 1};
   Real beam_flux = 0, beam_mu = 0.5, beam_phi = 0, iso_top = 0, iso_bot = 0;
@@ -164,6 +167,11 @@ struct GeneralOutput {
   std::vector<Real> inten;                     // [ntau][numu][nphi]
   std::vector<Real> inten_avg;                 // [ntau][numu]
   std::vector<Real> mu_angles;                 // [numu]
+  // SPECIAL_BC: albedo and transmissivity of the medium at the user polar
+  // angles ([numu], the first numu/2 filled), spherical albedo and
+  // transmissivity.
+  std::vector<Real> albedo_medium, transmissivity_medium;
+  Real spherical_albedo = 0, spherical_transmissivity = 0;
 };
 
 void validate_general_config(const GeneralConfig& c);

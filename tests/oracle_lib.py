@@ -47,6 +47,8 @@ def lib():
         l.oracle_general_one.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                          PI, C.c_char_p, C.c_int]
         l.oracle_rpv.argtypes = [C.c_void_p, D, D, D]
+        l.oracle_general_albtrans.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                              C.c_char_p, C.c_int]
         l.oracle_rpv.restype = D
         _lib = l
     return _lib
@@ -166,6 +168,20 @@ def general_all(gb):
         gb.inten[i] = u
         gb.inten_avg[i] = a
     return gb
+
+
+def general_albtrans(gb, i):
+    """SPECIAL_BC of column i (albedo and transmissivity of the medium):
+    returns (albedo [2h], transmissivity [2h], (spherical albedo, spherical
+    transmissivity)) with h positive polar angles, the first h entries set."""
+    h = gb.mu_user.size if (gb.flags & 32) else gb.nstr // 2
+    alb, trn, sph = np.zeros(2 * h), np.zeros(2 * h), np.zeros(2)
+    err = C.create_string_buffer(512)
+    st = gb.c_struct()
+    rc = lib().oracle_general_albtrans(C.byref(st), i, alb.ctypes.data, trn.ctypes.data, sph.ctypes.data, err, 512)
+    if rc:
+        raise OracleError(rc, err.value.decode())
+    return alb, trn, (float(sph[0]), float(sph[1]))
 
 
 def rpv(params, mu1, mu2, phi):

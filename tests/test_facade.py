@@ -95,8 +95,8 @@ def test_index_from_bottom_validates_in_top_down_order():
 
 
 def test_general_solver_rejects_what_the_gpu_path_does_not_cover():
-    """Checked before any GPU work: SPECIAL_BC, a BRDF selected without its
-    parameters (the reference's bidirReflectivity messages)."""
+    """Checked before any GPU work: a BRDF selected without its parameters
+    (the reference's bidirReflectivity messages)."""
     cfg = d.DisortConfig(4, 16, 0)
     cfg.allocate()
     cfg.delta_tau = [0.1, 0.5, 1.0, 2.0]
@@ -113,10 +113,6 @@ def test_general_solver_rejects_what_the_gpu_path_does_not_cover():
         d.DisortSolver().solve(cfg)
     cfg.flags.brdf_type = d.BrdfType.RPV
     with pytest.raises(RuntimeError, match="rpv spec is null"):
-        d.DisortSolver().solve(cfg)
-    cfg.flags.use_lambertian_surface = True
-    cfg.flags.ibcnd = d.BoundaryConditionType.Special
-    with pytest.raises(RuntimeError, match="SPECIAL_BC"):
         d.DisortSolver().solve(cfg)
 
 

@@ -279,3 +279,64 @@ def test_general_solver_buras_emde_table(d):
     for (iu, lu), v in {(3, 0): 1.51254859e-01, (4, 0): 1.01163783e-01, (5, 0): 3.92360939e-02,
                         (0, 1): 3.06133539e+00, (1, 1): 2.66600162e-01, (2, 1): 2.13783659e-01}.items():
         assert r.intensities(iu, lu, 0) == pytest.approx(v, rel=1e-8)
+
+
+def special_config(d, dtau, ssa, mu, albedo, g=0.8, nstr=16, user_mu=True):
+    c = d.DisortConfig(len(dtau), nstr, 0)
+    c.flags.ibcnd = d.BoundaryConditionType.Special
+    c.flags.use_user_mu = user_mu
+    c.num_user_mu = len(mu) if user_mu else 0
+    c.allocate()
+    c.delta_tau = list(dtau)
+    c.single_scat_albedo = list(ssa)
+    for lc in range(len(dtau)):
+        c.set_henyey_greenstein(g, lc)
+    if user_mu:
+        c.mu_user = list(mu)
+    c.bc.surface_albedo = albedo
+    return c
+
+
+def test_special_bc_tables(d):
+    """SpecialBC13a / 13c (tests/test_solver.cpp:4206-4318): the C-DISORT
+    values to 11 digits (asserted at 1e-10)."""
+    for dtau, ssa, alb, trn in (([1.0], [0.99], 5.4525838793e-01, 8.4499871160e-01),
+                                ([0.5, 0.5], [0.99, 0.50], 2.7620066856e-01, 5.0331887919e-01)):
+        c = special_config(d, dtau, ssa, [0.5], 0.5)
+        r = d.DisortSolver().solve(c)
+        assert r.albedo_medium[0] == pytest.approx(alb, rel=1e-10)
+        assert r.transmissivity_medium[0] == pytest.approx(trn, rel=1e-10)
+        assert list(r.mu_angles) == [-0.5, 0.5] and c.num_user_mu == 1
+
+
+@pytest.mark.parametrize("nstr,L,user_mu,albedo", [(8, 1, True, 0.0), (16, 3, True, 0.3), (32, 4, False, 0.2),
+                                                   (64, 2, True, 0.0)])
+def test_special_bc_against_oracle(d, nstr, L, user_mu, albedo):
+    from disort_amd import general as G
+    from disort_amd.batch import scalars
+
+    r = np.random.default_rng(nstr + L)
+    dtau = r.uniform(0.05, 2.0, L)
+    ssa = r.uniform(0.3, 0.999, L)
+    mu = [0.15, 0.4, 0.75, 1.0]
+    c = special_config(d, dtau, ssa, mu, albedo, g=0.7, nstr=nstr, user_mu=user_mu)
+    res = d.DisortSolver().solve(c)
+    pm = np.repeat((0.7 ** np.arange(nstr + 1))[None, None, :], L, axis=1)
+    gb = G.GeneralBatch(flags=G.G_USER_MU if user_mu else 0, nstr=nstr, dtau=dtau[None], ssa=ssa[None], pmom=pm,
+                        scal=scalars(1, albedo=albedo), mu_user=mu if user_mu else None, phi_user=[0.0])
+    alb, trn, sph = ol.general_albtrans(gb, 0)
+    check(np.array(res.albedo_medium), alb, "albedo_medium")
+    check(np.array(res.transmissivity_medium), trn, "transmissivity_medium")
+    assert res.spherical_albedo == pytest.approx(sph[0], rel=1e-9, abs=1e-12)
+    assert res.spherical_transmissivity == pytest.approx(sph[1], rel=1e-9, abs=1e-12)
+
+
+def test_special_bc_spectral_bands(d):
+    c = special_config(d, [0.4, 0.9], [0.95, 0.6], [0.3, 0.8], 0.1)
+    bands = [(500.0 + k, 501.0 + k) for k in range(3)]
+    res = d.solve_spectral_bands(c, bands, single_scat_albedo=[[0.95, 0.6], [0.9, 0.5], [0.99, 0.2]])
+    one = [d.DisortSolver().solve(special_config(d, [0.4, 0.9], w, [0.3, 0.8], 0.1))
+           for w in ([0.95, 0.6], [0.9, 0.5], [0.99, 0.2])]
+    for a, b in zip(res, one):
+        np.testing.assert_allclose(a.albedo_medium, b.albedo_medium, rtol=1e-12)
+        np.testing.assert_allclose(a.transmissivity_medium, b.transmissivity_medium, rtol=1e-12)

@@ -167,3 +167,49 @@ def test_validation_messages():
         ol.general_one(case_batch(dict(c, mu_user=[0.5, 0.1, 0.2, 0.3, 0.4, 0.6])), 0)
     with pytest.raises(ol.OracleError, match=r"phi_user\[0\] must be in \[0, 360\]"):
         ol.general_one(case_batch(dict(c, phi_user=[400.0])), 0)
+
+
+SPECIAL_BC_TABLES = [
+    # SpecialBC13a / 13c (tests/test_solver.cpp:4206-4318): HG g = 0.8, 16
+    # streams, mu = 0.5, surface albedo 0.5; C-DISORT values to 11 digits.
+    ("13a", [1.0], [0.99], 5.4525838793e-01, 8.4499871160e-01),
+    ("13c", [0.5, 0.5], [0.99, 0.50], 2.7620066856e-01, 5.0331887919e-01),
+]
+
+
+def special_batch(dtau, ssa, mu, albedo, nstr=16, g=0.8, n=1):
+    L = len(dtau)
+    P = nstr + 1
+    pm = np.repeat((g ** np.arange(P))[None, None, :], L, axis=1)
+    return G.GeneralBatch(flags=G.G_USER_MU, nstr=nstr, dtau=np.repeat([dtau], n, axis=0),
+                          ssa=np.repeat([ssa], n, axis=0), pmom=np.repeat(pm, n, axis=0),
+                          scal=scalars(n, albedo=albedo), mu_user=mu, phi_user=[0.0])
+
+
+@pytest.mark.parametrize("name,dtau,ssa,alb,trn", SPECIAL_BC_TABLES, ids=[t[0] for t in SPECIAL_BC_TABLES])
+def test_special_bc_tables(name, dtau, ssa, alb, trn):
+    """SPECIAL_BC (albedo and transmissivity of the medium) against the
+    reference's C-DISORT values, at half a unit in their last printed digit."""
+    a, t, sph = ol.general_albtrans(special_batch(dtau, ssa, [0.5], 0.5), 0)
+    assert abs(a[0] - alb) <= 0.5 * printed_ulp(alb), (a[0], alb)
+    assert abs(t[0] - trn) <= 0.5 * printed_ulp(trn), (t[0], trn)
+    assert a[1] == 0.0 and t[1] == 0.0
+    assert 0.0 < sph[0] < 1.0 and 0.0 < sph[1] < 1.0
+
+
+def test_special_bc_reciprocity_with_beam_solve():
+    """Without a reflecting surface the medium's albedo at mu equals the
+    reflected flux of a beam solve at mu0 = mu divided by mu0 F0, and its
+    transmissivity the total transmitted flux (the reference's own check of
+    13a/13c against 13b/13d, tests/test_solver.cpp:4319-4440)."""
+    for dtau, ssa in ([1.0], [0.99]), ([0.5, 0.7, 0.2], [0.99, 0.5, 0.8]):
+        mus = [0.3, 0.5, 0.9]
+        a, t, _ = ol.general_albtrans(special_batch(dtau, ssa, mus, 0.0), 0)
+        for k, mu0 in enumerate(mus):
+            sb = special_batch(dtau, ssa, [mu0], 0.0)
+            gb = G.GeneralBatch(flags=G.G_ONLY_FLUXES, nstr=sb.nstr, dtau=sb.dtau, ssa=sb.ssa, pmom=sb.pmom,
+                                scal=scalars(1, beam_flux=1.0, beam_mu=mu0))
+            flux, _, _, _ = ol.general_one(gb, 0)
+            L = len(dtau)
+            assert a[k] == pytest.approx(flux[OUT_FIELDS.index("flux_up"), 0] / mu0, rel=1e-9)
+            assert t[k] == pytest.approx((flux[0, L] + flux[1, L]) / mu0, rel=1e-9)

@@ -55,6 +55,7 @@ def lib():
         l.disort_ctx_stream.restype = vp
         l.disort_sync.argtypes = [vp]
         l.disort_flux_solve.argtypes = [vp, vp, C.c_int]
+        l.disort_flux_jacobian.argtypes = [vp, vp, vp]
         l.disort_flux_plan_create.argtypes = [vp, C.c_int64, vp, vp, vp, C.c_int32, C.POINTER(vp)]
         l.disort_flux_plan_destroy.argtypes = [vp, vp]
         l.disort_flux_execute.argtypes = [vp, vp, vp]
@@ -75,6 +76,7 @@ def lib():
         l.disort_flux_plan_report.argtypes = [vp]
         l.disort_flux_plan_report.restype = C.c_char_p
         l.disort_general_solve.argtypes = [vp, vp, C.c_int]
+        l.disort_general_jacobian.argtypes = [vp, vp, vp]
         l.disort_flux_execute_static.argtypes = [vp, vp, C.c_int32, C.c_int32, C.c_int32]
         l.disort_flux_static_bytes.argtypes = [C.c_int64, C.c_int32, C.c_int32]
         l.disort_flux_static_bytes.restype = C.c_size_t
@@ -133,6 +135,21 @@ class Context:
         _check(self._ptr, rc)
         return batch
 
+    def flux_jacobian(self, batch: FluxBatch) -> list:
+        """Temperature Jacobians of a thermal host batch: one array
+        [6, L+1, L+2] per instance (fields flux_up, flux_down,
+        mean_intensity, mean_intensity_down, mean_intensity_up,
+        flux_tau_divergence; dof L+1 is the surface temperature)."""
+        per = (batch.nlyr.astype(np.int64) + 1) * (batch.nlyr + 2)
+        jac = np.zeros(int(6 * per.sum()), np.float64)
+        st = batch.c_struct()
+        _check(self._ptr, lib().disort_flux_jacobian(self._ptr, C.byref(st), jac.ctypes.data))
+        res, off = [], 0
+        for i, L in enumerate(batch.nlyr.tolist()):
+            res.append(jac[off:off + 6 * int(per[i])].reshape(6, L + 1, L + 2))
+            off += 6 * int(per[i])
+        return res
+
     def general_solve(self, gb):
         """Solve a host disort_amd.general.GeneralBatch in place (gb.flux,
         gb.inten, gb.inten_avg, gb.status)."""
@@ -140,6 +157,14 @@ class Context:
         rc = lib().disort_general_solve(self._ptr, C.byref(st), MEM_HOST)
         _check(self._ptr, rc)
         return gb
+
+    def general_jacobian(self, gb) -> np.ndarray:
+        """Temperature Jacobians of a thermal GeneralBatch: [n, 6, ntau,
+        nlyr+2] (fields as flux_jacobian; rows the output levels)."""
+        jac = np.zeros((gb.n, 6, gb.ntau, gb.nlyr + 2), np.float64)
+        st = gb.c_struct()
+        _check(self._ptr, lib().disort_general_jacobian(self._ptr, C.byref(st), jac.ctypes.data))
+        return jac
 
     # -- device helpers -----------------------------------------------------
     def alloc(self, nbytes: int) -> int:

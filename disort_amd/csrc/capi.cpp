@@ -20,6 +20,7 @@
 #include "flux_args.hpp"
 #include "flux_layout.hpp"
 #include "ctx.hpp"
+#include "planck.hpp"
 
 namespace dsa {
 // Register kernels (flux_kernels.hip), N in {4,...,16}.
@@ -805,6 +806,145 @@ int disort_flux_solve(disort_ctx* ctx, const disort_flux_batch* b, int memspace)
     bad += st[i] != 0;
   }
   if (bad) return fail(ctx, DISORT_E_NUMERIC, std::to_string(bad) + " instance(s) failed (see status)");
+  return DISORT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Temperature Jacobians (DisortFluxSolver::computeThermalJacobian,
+// src/FluxSolver.cpp:1294-1497).  With the optical properties fixed, every
+// output is affine in the level Planck values B_k and in B(T_bottom): its
+// derivative with respect to B_k is the solve of the same medium with the
+// unit source B = e_k and no other source (no beam, no isotropic or top
+// emission), and with respect to B(T_bottom) the solve with only a unit
+// surface emission.  The reference obtains the same columns from the same
+// linear system by extra right-hand sides on its band LU.  Here each
+// (instance, degree of freedom) is one instance of an expanded GPU batch
+// whose Planck values are given (FluxArgs::planck_ovr); the columns are then
+// scaled by dB/dT (planckFunctionDeriv2, src/Planck.cpp:171-214).
+// ---------------------------------------------------------------------------
+int disort_flux_jacobian(disort_ctx* ctx, const disort_flux_batch* b, double* jac) {
+  if (!ctx || !b || !jac) return fail(ctx, DISORT_E_INVALID, "null argument");
+  if (!(b->flags & DISORT_F_THERMAL))
+    return fail(ctx, DISORT_E_INVALID, "temperature Jacobians need use_thermal_emission = true");
+  if (b->n <= 0) return DISORT_OK;
+  for (int64_t i = 0; i < b->n; ++i) {
+    const std::string m = validate_instance(b, i);
+    if (!m.empty()) return fail(ctx, DISORT_E_INVALID, m);
+  }
+  const int32_t flags = b->flags & (DISORT_F_THERMAL | DISORT_F_DELTA_M_PLUS | DISORT_F_DIFFUSION_BC);
+  // Chunks of instances whose degrees of freedom (nlyr + 2 each) fit an
+  // expanded batch of about 16k instances.
+  const int64_t kMaxExpanded = 16384;
+  int64_t jac_off = 0;
+  for (int64_t c0 = 0; c0 < b->n;) {
+    int64_t c1 = c0, ne = 0;
+    while (c1 < b->n && (c1 == c0 || ne + b->nlyr[c1] + 2 <= kMaxExpanded)) ne += b->nlyr[c1++] + 2;
+    std::vector<int32_t> vn, vl, vp;
+    std::vector<int64_t> src;  // expanded instance -> batch instance
+    std::vector<int> dof;      // expanded instance -> degree of freedom
+    vn.reserve(ne), vl.reserve(ne), vp.reserve(ne), src.reserve(ne), dof.reserve(ne);
+    for (int64_t i = c0; i < c1; ++i)
+      for (int k = 0; k <= b->nlyr[i] + 1; ++k) {
+        vn.push_back(b->nstr[i]);
+        vl.push_back(b->nlyr[i]);
+        vp.push_back(b->pstride[i]);
+        src.push_back(i);
+        dof.push_back(k);
+      }
+    std::vector<double> dtau, ssa, pmom, temper, scal, ovr;
+    int64_t TV = 0;
+    for (int64_t e = 0; e < ne; ++e) TV += vl[e] + 1;
+    ovr.assign(TV + 2 * ne, 0.0);
+    int64_t v = 0;
+    for (int64_t e = 0; e < ne; ++e) {
+      const int64_t i = src[e];
+      const int L = b->nlyr[i];
+      const int64_t lo = b->lay_off[i];
+      dtau.insert(dtau.end(), b->dtau + lo, b->dtau + lo + L);
+      ssa.insert(ssa.end(), b->ssa + lo, b->ssa + lo + L);
+      pmom.insert(pmom.end(), b->pmom + b->mom_off[i], b->pmom + b->mom_off[i] + (int64_t)L * b->pstride[i]);
+      temper.insert(temper.end(), b->temper + lo + i, b->temper + lo + i + L + 1);
+      double sc[DISORT_NSCAL];
+      std::memcpy(sc, b->scal + i * DISORT_NSCAL, sizeof sc);
+      sc[DISORT_S_BEAM_FLUX] = 0.0;
+      sc[DISORT_S_ISO_TOP] = 0.0;
+      sc[DISORT_S_ISO_BOT] = 0.0;
+      scal.insert(scal.end(), sc, sc + DISORT_NSCAL);
+      if (dof[e] <= L) ovr[v + dof[e]] = 1.0;
+      else ovr[TV + 2 * e + 1] = 1.0;
+      v += L + 1;
+    }
+    disort_flux_plan* plan = nullptr;
+    int rc = disort_flux_plan_create(ctx, ne, vn.data(), vl.data(), vp.data(), flags, &plan);
+    if (rc) return rc;
+    std::unique_ptr<disort_flux_plan, int (*)(disort_flux_plan*)> guard(plan, [](disort_flux_plan* p) {
+      for (void* q : p->owned) (void)hipFree(q);
+      delete p;
+      return 0;
+    });
+    struct Up {
+      const char* name;
+      const void* src;
+      size_t bytes;
+      void** dst;
+    };
+    void *p_dtau, *p_ssa, *p_pmom, *p_temp, *p_scal, *p_ovr, *p_out;
+    const std::vector<Up> ups = {{"jac_dtau", dtau.data(), dtau.size() * 8, &p_dtau},
+                                 {"jac_ssa", ssa.data(), ssa.size() * 8, &p_ssa},
+                                 {"jac_pmom", pmom.data(), pmom.size() * 8, &p_pmom},
+                                 {"jac_temp", temper.data(), temper.size() * 8, &p_temp},
+                                 {"jac_scal", scal.data(), scal.size() * 8, &p_scal},
+                                 {"jac_ovr", ovr.data(), ovr.size() * 8, &p_ovr}};
+    for (const Up& u : ups) {
+      if ((rc = get_buf(ctx, u.name, u.bytes, u.dst))) return rc;
+      HIPCHK(ctx, hipMemcpyAsync(*u.dst, u.src, u.bytes, hipMemcpyHostToDevice, ctx->stream));
+    }
+    if ((rc = get_buf(ctx, "jac_out", (size_t)DISORT_NOUT * TV * 8, &p_out))) return rc;
+    disort_flux_batch d{};
+    d.n = ne;
+    d.flags = flags;
+    d.dtau = (const double*)p_dtau;
+    d.ssa = (const double*)p_ssa;
+    d.pmom = (const double*)p_pmom;
+    d.temper = (const double*)p_temp;
+    d.scal = (const double*)p_scal;
+    d.out = (double*)p_out;
+    d.total_levels = TV;
+    PlanArgs pa;
+    if ((rc = prepare(ctx, plan, &d, pa))) return rc;
+    pa.base.planck_ovr = (const double*)p_ovr;
+    if ((rc = launch_plan(ctx, plan, pa, false))) return rc;
+    std::vector<double> out((size_t)DISORT_NOUT * TV);
+    HIPCHK(ctx, hipMemcpyAsync(out.data(), p_out, out.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<int32_t> st(ne);
+    if ((rc = disort_flux_status(ctx, plan, st.data()))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    for (int64_t e = 0; e < ne; ++e)
+      if (st[e]) return fail(ctx, DISORT_E_NUMERIC, "instance " + std::to_string(src[e]) + ": Jacobian solve failed");
+    // jac[i][f][lu][k] = d out_f(lu) / d T_k; f in the order flux_up,
+    // flux_down, mean_intensity, mean_intensity_down, mean_intensity_up,
+    // flux_tau_divergence (the *_temperature_jac members of FluxResult).
+    static const int field[6] = {DISORT_O_FLUX_UP, DISORT_O_FLUX_DOWN, DISORT_O_MEAN_INTENSITY,
+                                 DISORT_O_MEAN_INTENSITY_DOWN, DISORT_O_MEAN_INTENSITY_UP,
+                                 DISORT_O_FLUX_TAU_DIVERGENCE};
+    v = 0;
+    int64_t base = jac_off;
+    for (int64_t e = 0; e < ne; ++e) {
+      const int64_t i = src[e];
+      const int L = b->nlyr[i], k = dof[e];
+      const double* s2 = b->scal + i * DISORT_NSCAL;
+      const double t = (k <= L) ? b->temper[b->lay_off[i] + i + k] : s2[DISORT_S_TEMP_BOT];
+      const double dbdt = dsa::planck_band_deriv(s2[DISORT_S_WN_LO], s2[DISORT_S_WN_HI], t);
+      const int64_t per = (int64_t)(L + 1) * (L + 2);
+      for (int f = 0; f < 6; ++f)
+        for (int lu = 0; lu <= L; ++lu)
+          jac[base + f * per + (int64_t)lu * (L + 2) + k] = dbdt * out[(size_t)field[f] * TV + v + lu];
+      v += L + 1;
+      if (k == L + 1) base += 6 * per;
+    }
+    jac_off = base;
+    c0 = c1;
+  }
   return DISORT_OK;
 }
 

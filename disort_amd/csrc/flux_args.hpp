@@ -42,6 +42,10 @@ struct FluxArgs {
   double* panel;             // [n_tasks][panel_stride]
   double* cvec;              // [n_tasks][N]
   const double* tab;         // TabOff layout for this N
+  // Planck values given instead of computed from the temperatures (the
+  // temperature-Jacobian solves): [total_levels] at the levels, then [n][2]
+  // the top emission (emissivity_top B(T_top)) and B(T_bottom); null = off.
+  const double* planck_ovr;
 };
 
 }  // namespace dsa

@@ -13,6 +13,7 @@
 #include "../../include/disort_amd.h"
 #include "flux_args.hpp"
 #include "flux_layout.hpp"
+#include "planck.hpp"
 
 namespace dsa {
 namespace {
@@ -20,70 +21,6 @@ namespace {
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kEps = 2.220446049250313e-16;     // DBL_EPSILON
 constexpr double kLogDblMax = 709.782712893384;    // log(DBL_MAX)
-
-// ---------------------------------------------------------------------------
-// Band-integrated Planck function (W m^-2 sr^-1 over [lo, hi] cm^-1), the
-// algorithm of src/Planck.cpp:35-169 (Simpson refinement for narrow bands,
-// power series below v = 1.5, exponential series above), with the constants
-// of src/Planck.hpp:10-22.  Arguments are validated on the host.
-// ---------------------------------------------------------------------------
-__device__ inline double planck_shape(double v) { return v * v * v / (exp(v) - 1.0); }
-
-__device__ double planck_band(double lo, double hi, double t) {
-  const double c2 = 1.438786, sigma = 5.67032e-8;
-  if (t < 1.e-4) return 0.0;
-  if (hi == lo) {
-    const double arg = exp(-c2 * hi / t);
-    return 1.1911e-8 * hi * hi * hi * arg / (1.0 - arg);
-  }
-  const double sigdpi = sigma / kPi;
-  const double conc = 15.0 / (kPi * kPi * kPi * kPi);
-  const double t4 = t * t * t * t;
-  const double v0 = c2 * lo / t, v1 = c2 * hi / t;
-  if (v0 > kEps && v1 < kLogDblMax && (hi - lo) / hi < 1.e-2) {
-    const double span = v1 - v0;
-    const double ends = planck_shape(v0) + planck_shape(v1);
-    double prev = 0.0;
-    for (int n = 1; n <= 10; ++n) {
-      const double h = span / (2.0 * n);
-      double acc = ends;
-      for (int k = 1; k <= 2 * n - 1; ++k) acc += 2.0 * (1 + k % 2) * planck_shape(v0 + k * h);
-      acc *= h * (1.0 / 3.0);
-      if (fabs((acc - prev) / acc) <= 1.e-6) return sigdpi * t4 * conc * acc;
-      prev = acc;
-    }
-    return sigdpi * t4 * conc * prev;
-  }
-  const double a1 = 1.0 / 3.0, a2 = -1.0 / 8.0, a3 = 1.0 / 60.0;
-  const double a4 = -1.0 / 5040.0, a5 = 1.0 / 272160.0, a6 = -1.0 / 13305600.0;
-  const double vcp[7] = {10.25, 5.7, 3.9, 2.9, 2.3, 1.9, 0.0};
-  double p[2] = {0.0, 0.0}, d[2] = {0.0, 0.0};
-  const double v[2] = {v0, v1};
-  int nsmall = 0;
-  for (int i = 0; i < 2; ++i) {
-    if (v[i] < 1.5) {
-      ++nsmall;
-      const double vs = v[i] * v[i];
-      p[i] = conc * vs * v[i] * (a1 + v[i] * (a2 + v[i] * (a3 + vs * (a4 + vs * (a5 + vs * a6)))));
-    } else {
-      int terms = 1;
-      while (v[i] < vcp[terms - 1]) ++terms;
-      const double ex = exp(-v[i]);
-      double exm = 1.0, s = 0.0;
-      for (int m = 1; m <= terms; ++m) {
-        const double mv = m * v[i];
-        exm *= ex;
-        s += exm * (6.0 + mv * (6.0 + mv * (3.0 + mv))) / (double)(m * m * m * m);
-      }
-      d[i] = s * conc;
-    }
-  }
-  double ans;
-  if (nsmall == 2) ans = p[1] - p[0];
-  else if (nsmall == 1) ans = 1.0 - p[0] - d[1];
-  else ans = d[0] - d[1];
-  return ans * (sigdpi * t4);
-}
 
 // Broadcast lane `src` of each row of 16 lanes to the whole row (DPP row_newbcast).
 template <int SRC>
@@ -332,7 +269,21 @@ __global__ __launch_bounds__(64) void k_column(FluxArgs a) {
     a.inst_d[i * kInstD + ID_BTRANS_NCUT] = (ncut_f == 0) ? bt_prev : a.lay_d[LD_BT_BOT * TL + lo + ncut_f - 1];
   }
   // Planck function per level (src/FluxSolver.cpp:104-122, :748-752).
-  if (thermal) {
+  if (thermal && a.planck_ovr) {
+    // Given Planck values (temperature-Jacobian solves).
+    for (int l = lane; l <= L; l += 64) a.lev_d[VD_PLANCK * TV + vo + l] = a.planck_ovr[vo + l];
+    if (lane == 0) {
+      a.inst_d[i * kInstD + ID_PL_TOP] = a.planck_ovr[TV + 2 * i];
+      const double pb = a.planck_ovr[TV + 2 * i + 1];
+      a.inst_d[i * kInstD + ID_PL_BOT] = pb;
+      double pbd = 0.0;
+      if (a.flags & DISORT_F_DIFFUSION_BC) {
+        const double dl = a.lay_d[LD_DTAU_S * TL + lo + L - 1];
+        if (dl > 0.0) pbd = (pb - a.planck_ovr[vo + L - 1]) / dl;
+      }
+      a.inst_d[i * kInstD + ID_PL_BOT_DERIV] = pbd;
+    }
+  } else if (thermal) {
     const double wlo = sc[DISORT_S_WN_LO], whi = sc[DISORT_S_WN_HI];
     for (int l = lane; l <= L; l += 64) a.lev_d[VD_PLANCK * TV + vo + l] = planck_band(wlo, whi, a.temper[vo + l]);
     if (lane == 0) {

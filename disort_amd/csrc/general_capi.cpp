@@ -18,6 +18,7 @@
 #include "flux_args.hpp"
 #include "flux_layout.hpp"
 #include "general_layout.hpp"
+#include "planck.hpp"
 
 using namespace dsa_host;
 
@@ -171,7 +172,11 @@ struct Dev {
 
 }  // namespace
 
-extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch* b, int memspace) {
+namespace {
+// The general solve; with ovr (host, [n (nlyr+1)] level Planck values then
+// [n][2] top emission and B(T_bottom)) the Planck values are given instead
+// of computed from the temperatures (the temperature-Jacobian solves).
+int general_solve(disort_ctx* ctx, const disort_general_batch* b, int memspace, const double* ovr) {
   using namespace dsa;
   if (!ctx || !b) return fail(ctx, DISORT_E_INVALID, "null argument");
   if (b->n <= 0) return DISORT_OK;
@@ -353,6 +358,13 @@ extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch*
   if ((rc = get_buf(ctx, "g_utaus", (size_t)n * ntau * 8, &d_utau))) return rc;
   fa.flags = b->flags & (DISORT_F_THERMAL | DISORT_F_SPHERICAL | DISORT_F_DELTA_M_PLUS | DISORT_F_DIFFUSION_BC);
   if (b->flags & DISORT_G_NO_CUTOFF) fa.flags |= kFluxNoCutoff;
+  if (ovr) {
+    void* d_ovr;
+    const size_t ob = (size_t)(TV + 2 * n) * 8;
+    if ((rc = get_buf(ctx, "g_ovr", ob, &d_ovr))) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(d_ovr, ovr, ob, hipMemcpyHostToDevice, ctx->stream));
+    fa.planck_ovr = (const double*)d_ovr;
+  }
   fa.nstr = (const int32_t*)d_vn;
   fa.nlyr = (const int32_t*)d_vl;
   fa.pstride = (const int32_t*)d_vp;
@@ -510,5 +522,109 @@ extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch*
     bad += st[i] != 0;
   }
   if (bad) return fail(ctx, DISORT_E_NUMERIC, std::to_string(bad) + " column(s) failed (see status)");
+  return DISORT_OK;
+}
+}  // namespace
+
+extern "C" int disort_general_solve(disort_ctx* ctx, const disort_general_batch* b, int memspace) {
+  return general_solve(ctx, b, memspace, nullptr);
+}
+
+// ---------------------------------------------------------------------------
+// Temperature Jacobians of the general solver's fluxes
+// (DisortSolver::computeThermalJacobian, src/DisortSolver.cpp:2130-2377).
+// As for the flux path: the outputs are affine in the level Planck values
+// and in B(T_bottom), so each degree of freedom is one column of an
+// expanded flux-only batch with a unit Planck source there and no other
+// source (no beam, no isotropic or top emission), scaled by dB/dT.  The
+// surface emissivity (Lambertian or BRDF) and the diffusion lower boundary
+// enter exactly as in the forward solve.
+// ---------------------------------------------------------------------------
+extern "C" int disort_general_jacobian(disort_ctx* ctx, const disort_general_batch* b, double* jac) {
+  using namespace dsa;
+  if (!ctx || !b || !jac) return fail(ctx, DISORT_E_INVALID, "null argument");
+  if (!(b->flags & DISORT_F_THERMAL))
+    return fail(ctx, DISORT_E_INVALID, "temperature Jacobians need use_thermal_emission = true");
+  if (b->n <= 0) return DISORT_OK;
+  const int L = b->nlyr, P = b->pstride, nd = L + 2;
+  const bool user_tau = b->flags & DISORT_G_USER_TAU;
+  const int ntau = user_tau ? b->ntau : L + 1;
+  if (ntau != b->ntau) return fail(ctx, DISORT_E_INVALID, "ntau must be nlyr+1 without DISORT_G_USER_TAU");
+  if (user_tau && (ntau <= 0 || !b->tau_user)) return fail(ctx, DISORT_E_INVALID, "use_user_tau needs tau_user");
+  const bool brdf = b->flags & kBrdfFlags;
+  if (brdf && !b->rpv) return fail(ctx, DISORT_E_INVALID, "bidirReflectivity: RPV BRDF selected but rpv spec is null");
+  for (int64_t i = 0; i < b->n; ++i) {
+    int code;
+    const std::string m = validate_column(b, i, &code);
+    if (!m.empty()) return fail(ctx, code, "column " + std::to_string(i) + ": " + m);
+  }
+  const int32_t flags = (b->flags & (DISORT_F_THERMAL | DISORT_F_DELTA_M_PLUS | DISORT_F_DIFFUSION_BC |
+                                     DISORT_G_USER_TAU | DISORT_G_NO_CUTOFF | kBrdfFlags)) |
+                        DISORT_G_ONLY_FLUXES;
+  static const int field[6] = {DISORT_O_FLUX_UP, DISORT_O_FLUX_DOWN, DISORT_O_MEAN_INTENSITY,
+                               DISORT_O_MEAN_INTENSITY_DOWN, DISORT_O_MEAN_INTENSITY_UP,
+                               DISORT_O_FLUX_TAU_DIVERGENCE};
+  const int64_t per = (int64_t)6 * ntau * nd;
+  const int64_t cols = std::max<int64_t>(1, 16384 / nd);  // columns per expanded batch
+  for (int64_t c0 = 0; c0 < b->n; c0 += cols) {
+    const int64_t nc = std::min<int64_t>(cols, b->n - c0), ne = nc * nd;
+    std::vector<double> dtau((size_t)ne * L), ssa((size_t)ne * L), pmom((size_t)ne * L * P),
+        temper((size_t)ne * (L + 1)), scal((size_t)ne * DISORT_NSCAL), ovr((size_t)ne * (L + 1) + 2 * ne, 0.0),
+        rpv(brdf ? (size_t)ne * DISORT_NRPV : 0), tau(user_tau ? (size_t)ne * ntau : 0);
+    for (int64_t e = 0; e < ne; ++e) {
+      const int64_t i = c0 + e / nd;
+      const int k = (int)(e % nd);
+      std::memcpy(&dtau[e * L], b->dtau + i * L, (size_t)L * 8);
+      std::memcpy(&ssa[e * L], b->ssa + i * L, (size_t)L * 8);
+      std::memcpy(&pmom[e * L * P], b->pmom + i * L * P, (size_t)L * P * 8);
+      std::memcpy(&temper[e * (L + 1)], b->temper + i * (L + 1), (size_t)(L + 1) * 8);
+      double* sc = &scal[e * DISORT_NSCAL];
+      std::memcpy(sc, b->scal + i * DISORT_NSCAL, DISORT_NSCAL * 8);
+      sc[DISORT_S_BEAM_FLUX] = 0.0;
+      sc[DISORT_S_ISO_TOP] = 0.0;
+      sc[DISORT_S_ISO_BOT] = 0.0;
+      if (brdf) std::memcpy(&rpv[e * DISORT_NRPV], b->rpv + i * DISORT_NRPV, DISORT_NRPV * 8);
+      if (user_tau) std::memcpy(&tau[e * ntau], b->tau_user + i * ntau, (size_t)ntau * 8);
+      if (k <= L) ovr[e * (L + 1) + k] = 1.0;
+      else ovr[ne * (L + 1) + 2 * e + 1] = 1.0;
+    }
+    std::vector<double> flux((size_t)DISORT_NOUT * ne * ntau);
+    std::vector<int32_t> st(ne);
+    disort_general_batch x{};
+    x.n = ne;
+    x.flags = flags;
+    x.nstr = b->nstr;
+    x.nlyr = L;
+    x.pstride = P;
+    x.ntau = ntau;
+    x.numu = b->nstr;
+    x.nphi = 0;
+    x.accuracy = b->accuracy;
+    x.dtau = dtau.data();
+    x.ssa = ssa.data();
+    x.pmom = pmom.data();
+    x.temper = temper.data();
+    x.scal = scal.data();
+    x.rpv = brdf ? rpv.data() : nullptr;
+    x.tau_user = user_tau ? tau.data() : nullptr;
+    x.flux = flux.data();
+    x.status = st.data();
+    int rc = general_solve(ctx, &x, DISORT_MEM_HOST, ovr.data());
+    if (rc) {
+      for (int64_t e = 0; e < ne; ++e)
+        if (st[e]) return fail(ctx, DISORT_E_NUMERIC, "column " + std::to_string(c0 + e / nd) + ": Jacobian solve failed");
+      return rc;
+    }
+    for (int64_t e = 0; e < ne; ++e) {
+      const int64_t i = c0 + e / nd;
+      const int k = (int)(e % nd);
+      const double* s2 = b->scal + i * DISORT_NSCAL;
+      const double t = (k <= L) ? b->temper[i * (L + 1) + k] : s2[DISORT_S_TEMP_BOT];
+      const double dbdt = planck_band_deriv(s2[DISORT_S_WN_LO], s2[DISORT_S_WN_HI], t);
+      for (int f = 0; f < 6; ++f)
+        for (int lu = 0; lu < ntau; ++lu)
+          jac[i * per + ((int64_t)f * ntau + lu) * nd + k] = dbdt * flux[((size_t)field[f] * ne + e) * ntau + lu];
+    }
+  }
   return DISORT_OK;
 }

@@ -166,8 +166,51 @@ struct Packed {
     if (n == 0) return;
     out.assign(static_cast<size_t>(DISORT_NOUT * levels), 0.0);
     status.assign(static_cast<size_t>(n), 0);
+    disort_flux_batch b = view();
+    b.out = out.data();
+    b.status = status.data();
+
+    run([&](disort_ctx* ctx) { return disort_flux_solve(ctx, &b, DISORT_MEM_HOST); });
+  }
+
+  // Temperature Jacobians of every instance (a thermal batch) into jac,
+  // [instance][6][L+1][L+2] (disort_flux_jacobian's layout).
+  void jacobian() {
+    const int64_t n = static_cast<int64_t>(nstr.size());
+    jac_off.assign(static_cast<size_t>(n) + 1, 0);
+    for (int64_t i = 0; i < n; ++i) jac_off[i + 1] = jac_off[i] + 6LL * (nlyr[i] + 1) * (nlyr[i] + 2);
+    jac.assign(static_cast<size_t>(jac_off[n]), 0.0);
+    if (n == 0) return;
+    disort_flux_batch b = view();
+    run([&](disort_ctx* ctx) { return disort_flux_jacobian(ctx, &b, jac.data()); });
+  }
+
+  // Instance i's Jacobians into the six *_temperature_jac members, [level
+  // lu][dof]; indexed from the bottom, rows and level columns are reversed
+  // and the surface column stays last (src/IndexConversion.hpp:57-68).
+  template <class R>
+  void unpack_jacobian(int64_t i, bool reverse, R& r) const {
+    const int L = nlyr[i], nd = L + 2;
+    std::vector<std::vector<double>>* f[6] = {&r.flux_up_temperature_jac, &r.flux_down_temperature_jac,
+                                              &r.mean_intensity_temperature_jac,
+                                              &r.mean_intensity_down_temperature_jac,
+                                              &r.mean_intensity_up_temperature_jac, &r.flux_divergence_temperature_jac};
+    for (int q = 0; q < 6; ++q) {
+      f[q]->assign(L + 1, std::vector<double>(nd, 0.0));
+      const double* src = jac.data() + jac_off[i] + static_cast<int64_t>(q) * (L + 1) * nd;
+      for (int lu = 0; lu <= L; ++lu)
+        for (int k = 0; k < nd; ++k)
+          (*f[q])[reverse ? L - lu : lu][reverse && k <= L ? L - k : k] = src[static_cast<int64_t>(lu) * nd + k];
+    }
+  }
+
+  std::vector<double> jac;
+  std::vector<int64_t> jac_off;
+
+ private:
+  disort_flux_batch view() {
     disort_flux_batch b{};
-    b.n = n;
+    b.n = static_cast<int64_t>(nstr.size());
     b.flags = flags;
     b.nstr = nstr.data();
     b.nlyr = nlyr.data();
@@ -180,10 +223,14 @@ struct Packed {
     b.temper = (flags & DISORT_F_THERMAL) ? temper.data() : nullptr;
     b.alt = (flags & DISORT_F_SPHERICAL) ? alt.data() : nullptr;
     b.scal = scal.data();
-    b.out = out.data();
     b.total_levels = levels;
-    b.status = status.data();
+    return b;
+  }
 
+  // Runs one C-ABI call on the process's context (created on first use);
+  // throws the reference's exception types.
+  template <class F>
+  void run(F call) {
     Gpu& g = gpu();
     std::lock_guard<std::mutex> lock(g.mu);
     if (!g.ctx) {
@@ -194,13 +241,15 @@ struct Packed {
       }
       g.device = dev;
     }
-    const int rc = disort_flux_solve(g.ctx, &b, DISORT_MEM_HOST);
+    const int rc = call(g.ctx);
     if (rc == DISORT_OK) return;
     std::string msg = disort_last_error(g.ctx);
-    if (n == 1 && msg.rfind("instance 0: ", 0) == 0) msg = msg.substr(12);
+    if (nstr.size() == 1 && msg.rfind("instance 0: ", 0) == 0) msg = msg.substr(12);
     if (rc == DISORT_E_INVALID) throw std::invalid_argument(msg);
     throw std::runtime_error("disortpp: " + msg);
   }
+
+ public:
 
   // Copies instance i's outputs into the eight result vectors (reversed
   // when the instance was indexed from the bottom).
@@ -258,6 +307,9 @@ Instance flux_instance(const DisortFluxConfig& c) {
   return in;
 }
 
+// src/FluxSolver.cpp:1299-1307: Jacobians only with thermal emission on.
+bool wants_jacobian(const DisortFluxConfig& c) { return c.compute_temperature_jacobian && c.use_thermal_emission; }
+
 int32_t flux_flags(const DisortFluxConfig& c) {
   return batch_flags(c.use_thermal_emission, c.use_spherical_beam, c.use_delta_m_plus, c.use_diffusion_lower_bc);
 }
@@ -276,8 +328,6 @@ void validate_flux(const DisortFluxConfig& c) {
   } else {
     c.validate();
   }
-  if (c.compute_temperature_jacobian)
-    throw std::runtime_error("disortpp: compute_temperature_jacobian is not available on the GPU path");
 }
 
 }  // namespace
@@ -400,6 +450,19 @@ std::vector<FluxResult> solveFluxBatch(const std::vector<DisortFluxConfig>& conf
       }
     p.solve();
     for (size_t j = 0; j < idx.size(); ++j) p.unpack(static_cast<int64_t>(j), configs[idx[j]].index_from_bottom, results[idx[j]]);
+    // Temperature Jacobians for the instances that ask for them.
+    Packed pj;
+    pj.flags = fl;
+    std::vector<size_t> jdx;
+    for (size_t i : idx)
+      if (wants_jacobian(configs[i])) {
+        pj.add(flux_instance(configs[i]));
+        jdx.push_back(i);
+      }
+    if (jdx.empty()) continue;
+    pj.jacobian();
+    for (size_t j = 0; j < jdx.size(); ++j)
+      pj.unpack_jacobian(static_cast<int64_t>(j), configs[jdx[j]].index_from_bottom, results[jdx[j]]);
   }
   return results;
 }
@@ -416,6 +479,10 @@ FluxResult DisortFluxSolver<NStr>::solve(const DisortFluxConfig& config) {
   p.solve();
   FluxResult r;
   p.unpack(0, config.index_from_bottom, r);
+  if (wants_jacobian(config)) {
+    p.jacobian();
+    p.unpack_jacobian(0, config.index_from_bottom, r);
+  }
   return r;
 }
 
@@ -607,14 +674,19 @@ void reverse_general_inputs(DisortConfig& c) {
 
 bool flux_route(const DisortConfig& c) {
   return c.flags.comp_only_fluxes && c.flags.use_lambertian_surface && c.flags.ibcnd == BoundaryConditionType::General &&
-         !c.flags.use_user_tau && !c.flags.compute_temperature_jacobian;
+         !c.flags.use_user_tau;
+}
+
+// src/DisortSolver.cpp:2143-2151: Jacobians only with thermal emission on.
+bool wants_general_jacobian(const DisortConfig& c) {
+  return c.flags.compute_temperature_jacobian && c.flags.use_thermal_emission &&
+         c.flags.ibcnd == BoundaryConditionType::General;
 }
 
 void require_gpu_subset(const DisortConfig& c) {
   auto no = [](const std::string& what) {
     throw std::runtime_error("disortpp: " + what + " is not available on the GPU path");
   };
-  if (c.flags.compute_temperature_jacobian) no("compute_temperature_jacobian");
   if (!c.flags.use_lambertian_surface) {
     // Missing specifications fail as bidirReflectivity does
     // (src/DisortSolver.cpp:2902-2936); Hapke falls back to its defaults.
@@ -829,6 +901,45 @@ struct GeneralPack {
     inten.assign(static_cast<size_t>(n) * ntau * numu * std::max(nphi, 0), 0.0);
     avg.assign(static_cast<size_t>(n) * ntau * numu, 0.0);
     status.assign(static_cast<size_t>(n), 0);
+    disort_general_batch b = view();
+    b.flux = flux.data();
+    b.inten = inten.empty() ? nullptr : inten.data();
+    b.inten_avg = avg.data();
+    b.status = status.data();
+    run([&](disort_ctx* ctx) { return disort_general_solve(ctx, &b, DISORT_MEM_HOST); });
+  }
+
+  // Temperature Jacobians of every column into jac, [column][6][ntau][L+2].
+  void jacobian() {
+    jac.assign(static_cast<size_t>(n) * 6 * ntau * (L + 2), 0.0);
+    if (n == 0) return;
+    disort_general_batch b = view();
+    run([&](disort_ctx* ctx) { return disort_general_jacobian(ctx, &b, jac.data()); });
+  }
+
+  // Column i's Jacobians into the six *_temperature_jac members of r, with
+  // index_from_bottom's reversal of rows and level columns
+  // (src/IndexConversion.hpp:57-68).
+  void unpack_jacobian(int64_t i, const DisortConfig& c, DisortResult& r) const {
+    const bool rev = c.flags.index_from_bottom;
+    const int nd = L + 2;
+    std::vector<std::vector<double>>* f[6] = {&r.flux_up_temperature_jac, &r.flux_down_temperature_jac,
+                                              &r.mean_intensity_temperature_jac,
+                                              &r.mean_intensity_down_temperature_jac,
+                                              &r.mean_intensity_up_temperature_jac, &r.flux_divergence_temperature_jac};
+    for (int q = 0; q < 6; ++q) {
+      f[q]->assign(ntau, std::vector<double>(nd, 0.0));
+      const double* src = jac.data() + (static_cast<size_t>(i) * 6 + q) * ntau * nd;
+      for (int lu = 0; lu < ntau; ++lu)
+        for (int k = 0; k < nd; ++k)
+          (*f[q])[rev ? ntau - 1 - lu : lu][rev && k <= L ? L - k : k] = src[static_cast<size_t>(lu) * nd + k];
+    }
+  }
+
+  std::vector<double> jac;
+
+ private:
+  disort_general_batch view() {
     disort_general_batch b{};
     b.n = n;
     b.flags = flags;
@@ -849,13 +960,14 @@ struct GeneralPack {
     b.tau_user = tau_user.empty() ? nullptr : tau_user.data();
     b.mu_user = mu_user.data();
     b.phi_user = phi_user.empty() ? nullptr : phi_user.data();
-    b.flux = flux.data();
-    b.inten = inten.empty() ? nullptr : inten.data();
-    b.inten_avg = avg.data();
-    b.status = status.data();
     b.nphase = (int32_t)mu_phase.size();
     b.mu_phase = mu_phase.empty() ? nullptr : mu_phase.data();
     b.phase = phase.empty() ? nullptr : phase.data();
+    return b;
+  }
+
+  template <class F>
+  void run(F call) {
     Gpu& g = gpu();
     std::lock_guard<std::mutex> lock(g.mu);
     if (!g.ctx) {
@@ -866,13 +978,15 @@ struct GeneralPack {
       }
       g.device = dev;
     }
-    const int rc = disort_general_solve(g.ctx, &b, DISORT_MEM_HOST);
+    const int rc = call(g.ctx);
     if (rc == DISORT_OK) return;
     std::string msg = disort_last_error(g.ctx);
     if (n == 1 && msg.rfind("column 0: ", 0) == 0) msg = msg.substr(10);
     if (rc == DISORT_E_INVALID) throw std::invalid_argument(msg);
     throw std::runtime_error("disortpp: " + msg);
   }
+
+ public:
 
   // Column i as a DisortResult (levels reversed for index_from_bottom,
   // src/IndexConversion.hpp:70-97).
@@ -903,6 +1017,26 @@ struct GeneralPack {
     return r;
   }
 };
+
+// Temperature Jacobians (src/DisortSolver.cpp:2130-2377) of the configs
+// that ask for them, in full-solver batches of equal shape.
+void general_jacobians(const std::vector<const DisortConfig*>& cs, const std::vector<DisortResult*>& rs) {
+  std::vector<bool> done(cs.size());
+  for (size_t i = 0; i < cs.size(); ++i) done[i] = !wants_general_jacobian(*cs[i]);
+  for (size_t i0 = 0; i0 < cs.size(); ++i0) {
+    if (done[i0]) continue;
+    GeneralPack g;
+    std::vector<size_t> idx;
+    for (size_t i = i0; i < cs.size(); ++i)
+      if (!done[i] && g.fits(*cs[i])) {
+        g.add(*cs[i]);
+        idx.push_back(i);
+        done[i] = true;
+      }
+    g.jacobian();
+    for (size_t j = 0; j < idx.size(); ++j) g.unpack_jacobian(static_cast<int64_t>(j), *cs[idx[j]], *rs[idx[j]]);
+  }
+}
 
 // SPECIAL_BC (src/DisortSolver.cpp:4025-4163): albedo and transmissivity
 // of the medium at the positive user polar angles (the quadrature nodes
@@ -1072,17 +1206,21 @@ DisortResult DisortSolver::solve(DisortConfig& config) {
     return special_bc_solve(one)[0];
   }
   prepare_general(config, cmu);
+  DisortResult r;
   if (flux_route(config)) {
     Packed p;
     p.flags = general_flags(config);
     p.add(general_instance(config));
     p.solve();
-    return to_result(p, 0, config);
+    r = to_result(p, 0, config);
+  } else {
+    GeneralPack g;
+    g.add(config);
+    g.solve();
+    r = g.unpack(0, config);
   }
-  GeneralPack g;
-  g.add(config);
-  g.solve();
-  return g.unpack(0, config);
+  general_jacobians({&config}, {&r});
+  return r;
 }
 
 std::vector<DisortResult> solveSpectralBands(DisortConfig config,
@@ -1155,6 +1293,13 @@ std::vector<DisortResult> solveSpectralBands(DisortConfig config,
     g.solve();
     for (size_t j = 0; j < idx.size(); ++j) results[idx[j]] = g.unpack(static_cast<int64_t>(j), cfgs[idx[j]]);
   }
+  std::vector<const DisortConfig*> cp;
+  std::vector<DisortResult*> rp;
+  for (size_t i = 0; i < cfgs.size(); ++i) {
+    cp.push_back(&cfgs[i]);
+    rp.push_back(&results[i]);
+  }
+  general_jacobians(cp, rp);
   return results;
 }
 

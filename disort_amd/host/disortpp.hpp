@@ -101,8 +101,9 @@ struct FluxResult {
   std::vector<double> mean_intensity_down;
   std::vector<double> mean_intensity_up;
   std::vector<double> mean_intensity_direct_beam;
-  // Temperature Jacobians: empty (the GPU path does not compute them yet;
-  // compute_temperature_jacobian=true is rejected).
+  // Temperature Jacobians [level][dof], dof 0..num_layers the level
+  // temperatures and num_layers+1 temperature_bottom; filled only when
+  // compute_temperature_jacobian and use_thermal_emission are set.
   std::vector<std::vector<double>> flux_up_temperature_jac;
   std::vector<std::vector<double>> flux_down_temperature_jac;
   std::vector<std::vector<double>> mean_intensity_temperature_jac;

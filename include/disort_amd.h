@@ -140,6 +140,20 @@ int disort_sync(disort_ctx* ctx);
 int disort_flux_solve(disort_ctx* ctx, const disort_flux_batch* batch, int memspace);
 
 /*
+ * Temperature Jacobians of a thermal batch on host arrays (the
+ * compute_temperature_jacobian option of DisortFluxSolver<N>::solve,
+ * src/FluxSolver.cpp:1294-1497, FluxResult::*_temperature_jac).  For
+ * instance i with L = nlyr[i], jac holds 6 (L+1) (L+2) values at offset
+ * 6 sum_{j<i} (L_j+1)(L_j+2), as [field][level lu][dof k]: fields flux_up,
+ * flux_down, mean_intensity, mean_intensity_down, mean_intensity_up,
+ * flux_tau_divergence; dof k = 0..L the level temperature temper[k], k = L+1
+ * temperature_bottom.  The derivatives are the GPU solves of the same media
+ * with a unit Planck source at one degree of freedom and no other source,
+ * times dB/dT.  Validated as disort_flux_solve; needs DISORT_F_THERMAL.
+ */
+int disort_flux_jacobian(disort_ctx* ctx, const disort_flux_batch* batch, double* jac);
+
+/*
  * Device-resident path.  A plan holds the bucket schedule (instances grouped
  * by stream count, layer and level task lists) computed once from host-side
  * shapes; disort_flux_execute then only launches kernels, asynchronously on
@@ -335,14 +349,30 @@ typedef struct disort_general_batch {
  * (src/DisortSolver.cpp:3403-3524) or, with a tabulated phase function and
  * without DISORT_G_CORR_NAKAJIMA, the Buras-Emde correction (:3532-3745,
  * :3812-4019), and the BRDF beam-reflection correction (:3751-3808).  Not
- * here: SPECIAL_BC, the Hapke / Cox-Munk / Ross-Li BRDFs, temperature
- * Jacobians.  All modes of every column are solved in parallel; the
+ * here: SPECIAL_BC (the C++ facade builds it from general solves with
+ * DISORT_G_NO_CUTOFF) and temperature Jacobians (disort_general_jacobian).
+ * All modes of every column are solved in parallel; the
  * accuracy_fourier_series cut is applied when they are summed, which gives
  * the reference's result.  Stage times (disort_stage_times, 8 stages):
  * set-up, BRDF beam terms, layers, boundary, fluxes, user coefficients, user
  * intensities, mode sum and corrections.
  */
 int disort_general_solve(disort_ctx* ctx, const disort_general_batch* batch, int memspace);
+
+/*
+ * Temperature Jacobians of a thermal general batch on host arrays (the
+ * compute_temperature_jacobian option of DisortSolver::solve,
+ * src/DisortSolver.cpp:2130-2377, DisortResult::*_temperature_jac).  jac
+ * holds, per column, 6 ntau (nlyr+2) values as [field][output level
+ * lu][dof k]: fields flux_up, flux_down, mean_intensity, mean_intensity_down,
+ * mean_intensity_up, flux_tau_divergence; rows the output levels (user
+ * optical depths with DISORT_G_USER_TAU); dof k = 0..nlyr the level
+ * temperature, k = nlyr+1 temperature_bottom.  Lambertian or BRDF surface,
+ * diffusion lower boundary.  Validated as disort_general_solve; needs
+ * DISORT_F_THERMAL.  User angles, azimuths and intensity corrections do not
+ * enter (the Jacobians are of fluxes and mean intensities).
+ */
+int disort_general_jacobian(disort_ctx* ctx, const disort_general_batch* batch, double* jac);
 
 
 /* =========================================================================

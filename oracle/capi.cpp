@@ -125,6 +125,34 @@ int oracle_flux_one(const disort_flux_batch* b, int64_t i, int index_from_bottom
   }
 }
 
+/* One instance with temperature Jacobians: out as oracle_flux_one, jac
+ * [6][nlyr+1][nlyr+2] in the field order flux_up, flux_down,
+ * mean_intensity, mean_intensity_down, mean_intensity_up,
+ * flux_tau_divergence. */
+int oracle_flux_jacobian_one(const disort_flux_batch* b, int64_t i, int index_from_bottom, double* out, double* jac,
+                             char* err, int errlen) {
+  try {
+    oracle::FluxConfig c = from_batch(b, i);
+    c.index_from_bottom = index_from_bottom != 0;
+    c.jacobian = true;
+    oracle::FluxOutput o = oracle::flux_solve(c);
+    const int nl = c.nlyr + 1;
+    for (int f = 0; f < 8; ++f) std::memcpy(out + (size_t)f * nl, o.f[f].data(), sizeof(double) * nl);
+    const size_t per = (size_t)nl * (nl + 1);
+    for (int f = 0; f < 6; ++f) {
+      if (o.jac[f].size() != per) throw std::runtime_error("oracle: no Jacobian (thermal emission off)");
+      for (size_t q = 0; q < per; ++q) jac[f * per + q] = (double)o.jac[f][q];
+    }
+    return 0;
+  } catch (const std::invalid_argument& e) {
+    put_err(err, errlen, e.what());
+    return 1;
+  } catch (const std::exception& e) {
+    put_err(err, errlen, e.what());
+    return 2;
+  }
+}
+
 /* Whole batch into b->out ([8][total_levels]) with an OpenMP static loop over
  * instances -- the structure of solveFluxSpectralBandsImpl<N>
  * (src/python_bindings.cpp:170-190).  Returns the number of failed instances;
@@ -226,6 +254,32 @@ int oracle_general_one(const disort_general_batch* b, int64_t i, int index_from_
     if (inten) std::memcpy(inten, o.inten.data(), sizeof(double) * o.inten.size());
     if (inten_avg) std::memcpy(inten_avg, o.inten_avg.data(), sizeof(double) * o.inten_avg.size());
     if (modes) *modes = o.modes;
+    return 0;
+  } catch (const std::invalid_argument& e) {
+    put_err(err, errlen, e.what());
+    return 1;
+  } catch (const std::exception& e) {
+    put_err(err, errlen, e.what());
+    return 2;
+  }
+}
+
+/* Column i of a general batch with temperature Jacobians: flux as
+ * oracle_general_one, jac [6][ntau][nlyr+2] (fields flux_up, flux_down,
+ * mean_intensity, mean_intensity_down, mean_intensity_up,
+ * flux_tau_divergence).  Returns 0, 1 or 2 as oracle_general_one. */
+int oracle_general_jacobian_one(const disort_general_batch* b, int64_t i, int index_from_bottom, double* flux,
+                                double* jac, char* err, int errlen) {
+  try {
+    oracle::GeneralConfig c = general_config(b, i, index_from_bottom);
+    c.jacobian = true;
+    const oracle::GeneralOutput o = oracle::general_solve(c);
+    for (int f = 0; f < 8; ++f) std::memcpy(flux + (size_t)f * o.ntau, o.f[f].data(), sizeof(double) * o.ntau);
+    const size_t per = (size_t)o.ntau * (c.nlyr + 2);
+    for (int f = 0; f < 6; ++f) {
+      if (o.jac[f].size() != per) throw std::runtime_error("oracle: no Jacobian (thermal emission off)");
+      std::memcpy(jac + f * per, o.jac[f].data(), sizeof(double) * per);
+    }
     return 0;
   } catch (const std::invalid_argument& e) {
     put_err(err, errlen, e.what());

@@ -8,9 +8,11 @@
 #include "../include/disort_amd.h"
 #include "oracle.hpp"
 
-extern "C" int oracle_ld_flux_one(const disort_flux_batch* b, int64_t i, double* out, char* err, int errlen) {
+namespace {
+int solve_ld(const disort_flux_batch* b, int64_t i, double* out, double* jac, char* err, int errlen) {
   try {
     oracle_ld::FluxConfig c;
+    c.jacobian = jac != nullptr;
     c.nstr = b->nstr[i];
     c.nlyr = b->nlyr[i];
     c.nmom = b->pstride[i] - 1;
@@ -40,6 +42,13 @@ extern "C" int oracle_ld_flux_one(const disort_flux_batch* b, int64_t i, double*
     const int nl = c.nlyr + 1;
     for (int f = 0; f < 8; ++f)
       for (int l = 0; l < nl; ++l) out[(size_t)f * nl + l] = (double)o.f[f][l];
+    if (jac) {
+      const size_t per = (size_t)nl * (nl + 1);
+      for (int f = 0; f < 6; ++f) {
+        if (o.jac[f].size() != per) throw std::runtime_error("oracle: no Jacobian (thermal emission off)");
+        for (size_t q = 0; q < per; ++q) jac[f * per + q] = (double)o.jac[f][q];
+      }
+    }
     return 0;
   } catch (const std::exception& e) {
     if (err && errlen > 0) {
@@ -48,4 +57,14 @@ extern "C" int oracle_ld_flux_one(const disort_flux_batch* b, int64_t i, double*
     }
     return 1;
   }
+}
+}  // namespace
+
+extern "C" int oracle_ld_flux_one(const disort_flux_batch* b, int64_t i, double* out, char* err, int errlen) {
+  return solve_ld(b, i, out, nullptr, err, errlen);
+}
+
+extern "C" int oracle_ld_flux_jacobian_one(const disort_flux_batch* b, int64_t i, double* out, double* jac, char* err,
+                                           int errlen) {
+  return solve_ld(b, i, out, jac, err, errlen);
 }

@@ -79,6 +79,10 @@ struct FluxSolver {
   std::vector<Real> zb, zbs, xba, z0, z1, xr0, xr1; // particular solutions
   Real pl_top = 0, pl_bot = 0, pl_bot_deriv = 0;
   std::vector<Real> LL;                             // integration constants
+  // Temperature-Jacobian state: derivatives of z0/z1 ([layer][pt|pb][N]),
+  // of the Planck slope and intercept ([layer][pt|pb]), and the band LU.
+  std::vector<Real> dz0, dz1, dslope, dicpt, band_lu;
+  std::vector<int> band_piv;
 
   explicit FluxSolver(const FluxConfig& cfg) : c(cfg), N(cfg.nstr), NN(cfg.nstr / 2), L(cfg.nlyr) {}
 
@@ -380,7 +384,9 @@ struct FluxSolver {
     }
   }
 
-  // src/FluxSolver.cpp:745-831 (Jacobian derivatives not restated)
+  // src/FluxSolver.cpp:745-831; with the Jacobian requested also the
+  // derivatives of the particular solution and of the Planck intercept/slope
+  // with respect to the layer's top (pt) and bottom (pb) level Planck values.
   void thermal_source() {
     std::vector<Real> pk(L + 1);
     for (int l = 0; l <= L; ++l) pk[l] = planck_band(c.wn_lo, c.wn_hi, c.temper[l]);
@@ -406,6 +412,9 @@ struct FluxSolver {
         z1[(size_t)lc * N + NN + iq] = b1[iq];
         z1[(size_t)lc * N + NN - iq - 1] = b1[iq + NN];
       }
+      if (c.jacobian)
+        thermal_source_derivs(lu, N, 1.0 - ssas[lc], dtaus[lc], taus[lc], cmu.data(), &dz0[(size_t)2 * lc * N],
+                              &dz1[(size_t)2 * lc * N], &dslope[2 * lc], &dicpt[2 * lc]);
     }
   }
 
@@ -570,6 +579,48 @@ struct FluxSolver {
     LL.assign(nn, 0.0);
     for (int i = 0; i < nc; ++i) LL[i] = b[i];  // the reference's reordering is the identity
     perturb(kFluxStages + PS_CONSTANTS, LL);
+    if (c.jacobian) {
+      band_lu.swap(band);
+      band_piv.swap(ipvt);
+    }
+  }
+
+  // src/FluxSolver.cpp:1294-1497: temperature Jacobians at the layer
+  // boundaries, Lambertian surface.
+  void jacobian(FluxOutput& out) {
+    std::vector<int> rows(L + 1);
+    for (int lu = 0; lu <= L; ++lu) rows[lu] = std::min(lu == 0 ? 0 : lu - 1, ncut - 1);
+    std::vector<Real> refl((size_t)NN * NN, c.albedo), emis(NN, 1.0 - c.albedo);
+    ThermalJacobianInputs in;
+    in.N = N;
+    in.L = L;
+    in.ncut = ncut;
+    in.lyrcut = lyrcut;
+    in.diffusion_bc = c.diffusion_bc;
+    in.cmu = cmu.data();
+    in.cwt = cwt.data();
+    in.taus = taus.data();
+    in.dtaus = dtaus.data();
+    in.kk = kk.data();
+    in.gc = gc.data();
+    in.band = band_lu.data();
+    in.ipvt = band_piv.data();
+    in.dz0 = dz0.data();
+    in.dz1 = dz1.data();
+    in.dslope = dslope.data();
+    in.dicpt = dicpt.data();
+    in.ssa = c.ssa.data();
+    in.refl = refl.data();
+    in.emis = emis.data();
+    in.nrows = L + 1;
+    in.row_layer = rows.data();
+    in.row_tau = taus.data();
+    in.wn_lo = c.wn_lo;
+    in.wn_hi = c.wn_hi;
+    in.temp_bot = c.temp_bot;
+    in.temper = c.temper.data();
+    in.perturb_stage = kFluxStages + PS_JACOBIAN;
+    thermal_jacobian(in, out.jac);
   }
 
   // src/FluxSolver.cpp:1138-1287
@@ -694,6 +745,12 @@ struct FluxSolver {
       else beam_source();
     }
     perturb(kFluxStages + PS_BEAM, zb);
+    if (c.jacobian) {
+      dz0.assign((size_t)2 * L * N, 0.0);
+      dz1.assign((size_t)2 * L * N, 0.0);
+      dslope.assign((size_t)2 * L, 0.0);
+      dicpt.assign((size_t)2 * L, 0.0);
+    }
     if (c.thermal) thermal_source();
     perturb(kFluxStages + PS_THERMAL, z0);
     perturb(kFluxStages + PS_THERMAL, z1);
@@ -701,6 +758,7 @@ struct FluxSolver {
     FluxOutput out;
     fluxes(out);
     for (auto& f : out.f) perturb(kFluxStages + PS_FLUX, f);
+    if (c.jacobian && c.thermal) jacobian(out);
     return out;
   }
 };
@@ -708,7 +766,6 @@ struct FluxSolver {
 }  // namespace
 
 FluxOutput flux_solve(const FluxConfig& cin) {
-  if (cin.jacobian) throw std::invalid_argument("oracle: temperature Jacobian is not restated");
   if (!cin.index_from_bottom) {
     validate_flux_config(cin);
     FluxSolver s(cin);
@@ -728,6 +785,7 @@ FluxOutput flux_solve(const FluxConfig& cin) {
   FluxSolver s(c);
   FluxOutput out = s.run();
   for (auto& v : out.f) std::reverse(v.begin(), v.end());
+  reverse_jacobian(out.jac, L + 1, L);
   return out;
 }
 

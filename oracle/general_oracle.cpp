@@ -218,6 +218,9 @@ struct GeneralSolver {
   std::vector<Real> LL;
   std::vector<Real> utau, sutau, umu;               // user levels (unscaled, scaled), user angles
   std::vector<int> lyr_of;                          // layer holding each user level
+  // Temperature-Jacobian state (see ThermalJacobianInputs).
+  std::vector<Real> dz0, dz1, dslope, dicpt, band_lu;
+  std::vector<int> band_piv;
 
   explicit GeneralSolver(const GeneralConfig& cfg) : c(cfg), N(cfg.nstr), NN(cfg.nstr / 2), L(cfg.nlyr) {}
 
@@ -576,7 +579,49 @@ struct GeneralSolver {
         z1[(size_t)lc * N + NN + iq] = b1[iq];
         z1[(size_t)lc * N + NN - iq - 1] = b1[iq + NN];
       }
+      if (c.jacobian)
+        thermal_source_derivs(lu, N, 1.0 - ssas[lc], dtaus[lc], taus[lc], cmu.data(), &dz0[(size_t)2 * lc * N],
+                              &dz1[(size_t)2 * lc * N], &dslope[2 * lc], &dicpt[2 * lc]);
     }
+  }
+
+  // src/DisortSolver.cpp:2130-2377: temperature Jacobians of the m = 0
+  // fluxes at the user levels, with the surface's m = 0 reflectivity and
+  // emissivity at the quadrature angles.
+  void jacobian(GeneralOutput& out) {
+    std::vector<Real> refl((size_t)NN * NN);
+    for (int iq = 0; iq < NN; ++iq)
+      for (int jq = 0; jq < NN; ++jq) refl[(size_t)iq * NN + jq] = RQ(iq, jq + 1);
+    ThermalJacobianInputs in;
+    in.N = N;
+    in.L = L;
+    in.ncut = ncut;
+    in.lyrcut = lyrcut;
+    in.diffusion_bc = c.diffusion_bc;
+    in.cmu = cmu.data();
+    in.cwt = cwt.data();
+    in.taus = taus.data();
+    in.dtaus = dtaus.data();
+    in.kk = kk.data();
+    in.gc = gc.data();
+    in.band = band_lu.data();
+    in.ipvt = band_piv.data();
+    in.dz0 = dz0.data();
+    in.dz1 = dz1.data();
+    in.dslope = dslope.data();
+    in.dicpt = dicpt.data();
+    in.ssa = c.ssa.data();
+    in.refl = refl.data();
+    in.emis = eq.data();
+    in.nrows = ntau;
+    in.row_layer = lyr_of.data();
+    in.row_tau = sutau.data();
+    in.wn_lo = c.wn_lo;
+    in.wn_hi = c.wn_hi;
+    in.temp_bot = c.temp_bot;
+    in.temper = c.temper.data();
+    in.perturb_stage = PS_JACOBIAN;
+    thermal_jacobian(in, out.jac);
   }
 
   // src/DisortSolver.cpp:2506-2580
@@ -855,6 +900,10 @@ struct GeneralSolver {
     LL.assign(nn, 0.0);
     for (int i = 0; i < nc; ++i) LL[i] = b[i];
     perturb(PS_CONSTANTS, LL);
+    if (m == 0 && c.jacobian) {
+      band_lu.swap(band);
+      band_piv.swap(ipvt);
+    }
   }
 
   // src/DisortSolver.cpp:1980-2124 (m = 0), at the user levels.
@@ -1520,6 +1569,12 @@ struct GeneralSolver {
     gmu.assign(NMUG, 0.0);
     gwt.assign(NMUG, 0.0);
     gauss_quadrature(NMUG / 2, gmu.data(), gwt.data());
+    if (c.jacobian) {
+      dz0.assign((size_t)2 * L * N, 0.0);
+      dz1.assign((size_t)2 * L * N, 0.0);
+      dslope.assign((size_t)2 * L, 0.0);
+      dicpt.assign((size_t)2 * L, 0.0);
+    }
 
     int kconv = 0;
     for (int m = 0; m <= naz; ++m) {
@@ -1550,6 +1605,7 @@ struct GeneralSolver {
         for (int iu = 0; iu < numu; ++iu) rho_beam_user[(size_t)iu * N + m] = RU(iu, 0);
       boundary_solve(m);
       if (m == 0) fluxes(out);
+      if (m == 0 && c.jacobian && c.thermal) jacobian(out);
       if (do_user) {
         const Real r = user_intensities(m, out);
         if (m > 0 && c.accur > 0.0) {
@@ -1711,6 +1767,7 @@ GeneralOutput general_solve(const GeneralConfig& cin) {
   GeneralSolver s(c);
   GeneralOutput o = s.run();
   for (auto& v : o.f) std::reverse(v.begin(), v.end());
+  reverse_jacobian(o.jac, o.ntau, L);
   const size_t lvl = (size_t)o.numu * o.nphi, lva = (size_t)o.numu;
   for (int a = 0, b = o.ntau - 1; a < b; ++a, --b) {
     std::swap_ranges(o.inten.begin() + a * lvl, o.inten.begin() + (a + 1) * lvl, o.inten.begin() + b * lvl);

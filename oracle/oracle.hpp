@@ -66,7 +66,8 @@ using Real = ORACLE_REAL;
 enum PerturbStage {
   PS_NONE = 0, PS_SCALED_COEF = 1, PS_SCALED_SSA = 2, PS_EIGVAL = 3, PS_EIGVEC = 4, PS_BEAM = 5,
   PS_THERMAL = 6, PS_PLANCK = 7, PS_CONSTANTS = 8, PS_QUAD = 9, PS_LEGENDRE = 10, PS_USER_EIGVEC = 11,
-  PS_NT = 12, PS_BRDF = 13, PS_CHAPMAN = 14, PS_FLUX = 15, PS_BURAS = 16, PS_COUNT = 17
+  PS_NT = 12, PS_BRDF = 13, PS_CHAPMAN = 14, PS_FLUX = 15, PS_BURAS = 16, PS_JACOBIAN = 17,
+  PS_COUNT = 18
 };
 void set_perturbation(int stage, double eps);
 void perturb(int stage, std::vector<Real>& v);
@@ -99,6 +100,38 @@ struct DenseLU {
   void solve(Real* b) const;
   Real frob_norm_of_input = 0.0;
 };
+
+// ---- temperature Jacobians (src/FluxSolver.cpp:786-831, :1294-1497;
+// src/DisortSolver.cpp:2130-2377), shared by both solvers -----------------
+// Derivatives of one layer's thermal particular solution (reordered like
+// z0/z1: dz0/dz1 [2][N], e = 0 top level, 1 bottom level) and of its Planck
+// slope/intercept ([2]) from the LU of [I - C].
+void thermal_source_derivs(const DenseLU& lu, int N, Real omm, Real dtau_s, Real tau_top, const Real* cmu, Real* dz0,
+                           Real* dz1, Real* dslope, Real* dicpt);
+struct ThermalJacobianInputs {
+  int N = 0, L = 0, ncut = 0;
+  bool lyrcut = false, diffusion_bc = false;
+  const Real *cmu = nullptr, *cwt = nullptr;      // quadrature [N], [NN]
+  const Real *taus = nullptr, *dtaus = nullptr;   // scaled [L+1], [L]
+  const Real *kk = nullptr, *gc = nullptr;        // eigenvalues [L][N], eigenvectors [L][N x N] column-major
+  const Real* band = nullptr;                     // the forward solve's band LU and pivots
+  const int* ipvt = nullptr;
+  const Real *dz0 = nullptr, *dz1 = nullptr;      // [L][2][N]
+  const Real *dslope = nullptr, *dicpt = nullptr; // [L][2]
+  const Real* ssa = nullptr;                      // unscaled [L]
+  const Real* refl = nullptr;                     // [NN][NN] m = 0 surface reflectivity at quadrature angles
+  const Real* emis = nullptr;                     // [NN] surface emissivity
+  int nrows = 0;                                  // output levels
+  const int* row_layer = nullptr;
+  const Real* row_tau = nullptr;                  // scaled optical depth of each output level
+  Real wn_lo = 0, wn_hi = 0, temp_bot = 0;
+  const Real* temper = nullptr;                   // [L+1]
+  int perturb_stage = 0;
+};
+// jac[f] = [nrows][L+2], f: flux_up, flux_down, mean_intensity,
+// mean_intensity_down, mean_intensity_up, flux_tau_divergence.
+void thermal_jacobian(const ThermalJacobianInputs& in, std::vector<Real> jac[6]);
+void reverse_jacobian(std::vector<Real> jac[6], int nrows, int L);
 
 // ---- flux solver (restating src/FluxSolver.cpp DisortFluxSolver<N>) -------
 struct FluxConfig {
@@ -140,7 +173,7 @@ struct GeneralConfig {
   bool lambertian = true, corr_buras = false, corr_nakajima = false, index_from_bottom = false;
   // special_bc: ibcnd = SPECIAL_BC (albedo and transmissivity of the medium);
   // no_cutoff: no absorption cut-off (the base solves of SPECIAL_BC).
-  bool special_bc = false, no_cutoff = false;
+  bool special_bc = false, no_cutoff = false, jacobian = false;
   int brdf = BRDF_NONE;
   Real rpv[7] = {0, 0, 0, 0, 0, 0, 1};
   Real beam_flux = 0, beam_mu = 0.5, beam_phi = 0, iso_top = 0, iso_bot = 0;
@@ -172,6 +205,8 @@ struct GeneralOutput {
   // transmissivity.
   std::vector<Real> albedo_medium, transmissivity_medium;
   Real spherical_albedo = 0, spherical_transmissivity = 0;
+  // Temperature Jacobians [6][ntau][nlyr+2] when requested (thermal only).
+  std::vector<Real> jac[6];
 };
 
 void validate_general_config(const GeneralConfig& c);

@@ -41,11 +41,16 @@ def lib():
         l.oracle_chapman.argtypes = [C.c_int, D, C.c_void_p, C.c_void_p, C.c_int, D, D]
         l.oracle_chapman.restype = D
         l.oracle_flux_one.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_char_p, C.c_int]
+        l.oracle_flux_jacobian_one.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p,
+                                               C.c_char_p, C.c_int]
+        l.oracle_ld_flux_jacobian_one.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_char_p, C.c_int]
         l.oracle_ld_flux_one.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_char_p, C.c_int]
         l.oracle_flux_batch.argtypes = [C.c_void_p, C.c_int, PD]
         l.oracle_flux_batch.restype = C.c_int64
         l.oracle_general_one.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                          PI, C.c_char_p, C.c_int]
+        l.oracle_general_jacobian_one.argtypes = [C.c_void_p, C.c_int64, C.c_int, C.c_void_p, C.c_void_p,
+                                                  C.c_char_p, C.c_int]
         l.oracle_rpv.argtypes = [C.c_void_p, D, D, D]
         l.oracle_general_albtrans.argtypes = [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
                                               C.c_char_p, C.c_int]
@@ -120,6 +125,36 @@ def flux_one(batch: FluxBatch, i: int, index_from_bottom=False):
     return out
 
 
+def flux_jacobian_one(batch: FluxBatch, i: int, index_from_bottom=False):
+    """Oracle solve of instance i with temperature Jacobians: returns
+    (out [8, L+1], jac [6, L+1, L+2]); jac fields flux_up, flux_down,
+    mean_intensity, mean_intensity_down, mean_intensity_up,
+    flux_tau_divergence; dof L+1 is the surface temperature."""
+    L = int(batch.nlyr[i])
+    out = np.zeros((NOUT, L + 1))
+    jac = np.zeros((6, L + 1, L + 2))
+    err = C.create_string_buffer(512)
+    st = batch.c_struct()
+    rc = lib().oracle_flux_jacobian_one(C.byref(st), i, int(index_from_bottom), out.ctypes.data, jac.ctypes.data,
+                                        err, 512)
+    if rc:
+        raise OracleError(rc, err.value.decode())
+    return out, jac
+
+
+def flux_jacobian_one_extended(batch: FluxBatch, i: int):
+    """flux_jacobian_one in x87 extended precision, rounded to FP64."""
+    L = int(batch.nlyr[i])
+    out = np.zeros((NOUT, L + 1))
+    jac = np.zeros((6, L + 1, L + 2))
+    err = C.create_string_buffer(512)
+    st = batch.c_struct()
+    rc = lib().oracle_ld_flux_jacobian_one(C.byref(st), i, out.ctypes.data, jac.ctypes.data, err, 512)
+    if rc:
+        raise OracleError(rc, err.value.decode())
+    return out, jac
+
+
 def flux_one_extended(batch: FluxBatch, i: int):
     """The same solve in x87 extended precision (64-bit significand),
     rounded to FP64: a higher-precision truth for rounding comparisons."""
@@ -158,6 +193,20 @@ def general_one(gb, i, index_from_bottom=False):
     if rc:
         raise OracleError(rc, err.value.decode())
     return flux, inten, avg, modes.value
+
+
+def general_jacobian_one(gb, i, index_from_bottom=False):
+    """Column i of a GeneralBatch with temperature Jacobians: returns
+    (fluxes [8][ntau], jac [6][ntau][nlyr+2])."""
+    flux = np.zeros((NOUT, gb.ntau))
+    jac = np.zeros((6, gb.ntau, gb.nlyr + 2))
+    err = C.create_string_buffer(512)
+    st = gb.c_struct()
+    rc = lib().oracle_general_jacobian_one(C.byref(st), i, int(index_from_bottom), flux.ctypes.data,
+                                           jac.ctypes.data, err, 512)
+    if rc:
+        raise OracleError(rc, err.value.decode())
+    return flux, jac
 
 
 def general_all(gb):

@@ -6,6 +6,7 @@
 // whole (cfg.delta_tau = [...]), as in the reference module: element
 // assignment on a read-back copy would silently do nothing, so the tuple
 // makes it an error instead.
+#include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
@@ -84,15 +85,62 @@ using Mat = std::vector<std::vector<double>>;
 using Cube = std::vector<std::vector<std::vector<double>>>;
 using Vec = std::vector<double>;
 
-SpectralOverrides overrides(Mat dtau, Mat ssa, Cube pmom, Vec alb, Vec emis, Vec beam, Vec itop, Vec ibot) {
-  return SpectralOverrides{std::move(dtau), std::move(ssa), std::move(pmom), std::move(alb),
-                           std::move(emis), std::move(beam), std::move(itop), std::move(ibot)};
-}
+#define DISORTPP_OVERRIDE_ARGS                                                                                \
+  py::arg("delta_tau") = py::none(), py::arg("single_scat_albedo") = py::none(),                                \
+      py::arg("phase_function_moments") = py::none(), py::arg("surface_albedo") = py::none(),                   \
+      py::arg("emissivity_top") = py::none(), py::arg("direct_beam_flux") = py::none(),                         \
+      py::arg("isotropic_flux_top") = py::none(), py::arg("isotropic_flux_bottom") = py::none()
 
-#define DISORTPP_OVERRIDE_ARGS                                                                              \
-  py::arg("delta_tau") = Mat{}, py::arg("single_scat_albedo") = Mat{}, py::arg("phase_function_moments") = Cube{}, \
-      py::arg("surface_albedo") = Vec{}, py::arg("emissivity_top") = Vec{}, py::arg("direct_beam_flux") = Vec{},   \
-      py::arg("isotropic_flux_top") = Vec{}, py::arg("isotropic_flux_bottom") = Vec{}
+using F64 = py::array_t<double, py::array::c_style | py::array::forcecast>;
+
+// The eight per-band override arguments, in either form.  None or an empty
+// sequence means "not given" (the reference's default, an empty list).  When
+// every given argument converts to a float64 C-contiguous array of its rank
+// (2, 2, 3, then 1 for the scalars) the arrays are used in place (NumPy
+// arrays of that dtype and layout are not copied); ragged nested lists take
+// the nested-list form.
+struct Overrides {
+  std::vector<F64> keep;
+  SpectralArrays arrays;
+  bool as_arrays = true;
+  SpectralOverrides lists;
+
+  Overrides(py::object dtau, py::object ssa, py::object pmom, py::object alb, py::object emis, py::object beam,
+            py::object itop, py::object ibot) {
+    py::object src[8] = {dtau, ssa, pmom, alb, emis, beam, itop, ibot};
+    const int rank[8] = {2, 2, 3, 1, 1, 1, 1, 1};
+    ArrayView* dst[8] = {&arrays.delta_tau,      &arrays.single_scat_albedo, &arrays.phase_function_moments,
+                         &arrays.surface_albedo, &arrays.emissivity_top,     &arrays.direct_beam_flux,
+                         &arrays.isotropic_flux_top, &arrays.isotropic_flux_bottom};
+    bool given[8];
+    for (int k = 0; k < 8; ++k) given[k] = !src[k].is_none() && py::len(src[k]) > 0;
+    for (int k = 0; k < 8; ++k) {
+      if (!given[k]) continue;
+      F64 a = F64::ensure(src[k]);
+      if (!a || a.ndim() != rank[k]) {
+        as_arrays = false;
+        break;
+      }
+      keep.push_back(a);
+      ArrayView& v = *dst[k];
+      v.data = a.data();
+      v.n = a.shape(0);
+      if (rank[k] >= 2) v.cols = a.shape(rank[k] - 1);
+      if (rank[k] == 3) v.rows = a.shape(1);
+    }
+    if (as_arrays) return;
+    keep.clear();
+    arrays = SpectralArrays{};
+    if (given[0]) lists.delta_tau = src[0].cast<Mat>();
+    if (given[1]) lists.single_scat_albedo = src[1].cast<Mat>();
+    if (given[2]) lists.phase_function_moments = src[2].cast<Cube>();
+    if (given[3]) lists.surface_albedo = src[3].cast<Vec>();
+    if (given[4]) lists.emissivity_top = src[4].cast<Vec>();
+    if (given[5]) lists.direct_beam_flux = src[5].cast<Vec>();
+    if (given[6]) lists.isotropic_flux_top = src[6].cast<Vec>();
+    if (given[7]) lists.isotropic_flux_bottom = src[7].cast<Vec>();
+  }
+};
 
 }  // namespace
 
@@ -313,51 +361,57 @@ PYBIND11_MODULE(disortpp, m) {
 
   m.def(
       "solve_spectral",
-      [](DisortConfig config, const Vec& wavenumbers, Mat dtau, Mat ssa, Cube pmom, Vec alb, Vec emis, Vec beam,
-         Vec itop, Vec ibot) {
-        auto ov = overrides(std::move(dtau), std::move(ssa), std::move(pmom), std::move(alb), std::move(emis),
-                            std::move(beam), std::move(itop), std::move(ibot));
+      [](DisortConfig config, const Vec& wavenumbers, py::object dtau, py::object ssa, py::object pmom, py::object alb,
+         py::object emis, py::object beam, py::object itop, py::object ibot) {
+        Overrides ov(dtau, ssa, pmom, alb, emis, beam, itop, ibot);
         py::gil_scoped_release release;
-        return solveSpectral(std::move(config), wavenumbers, ov);
+        return ov.as_arrays ? solveSpectral(std::move(config), wavenumbers, ov.arrays)
+                            : solveSpectral(std::move(config), wavenumbers, ov.lists);
       },
       py::arg("config"), py::arg("wavenumbers"), DISORTPP_OVERRIDE_ARGS,
-      "Solve at single wavenumbers (wavenumber_low = wavenumber_high = wn), one GPU batch; returns DisortResult list.");
+      "Solve at single wavenumbers (wavenumber_low = wavenumber_high = wn), one GPU batch; returns DisortResult "
+      "list. Per-band overrides are nested lists or NumPy arrays ([bands][layers], [bands][layers][moments], "
+      "[bands]).");
 
   m.def(
       "solve_spectral_bands",
-      [](DisortConfig config, const std::vector<std::pair<double, double>>& bands, Mat dtau, Mat ssa, Cube pmom,
-         Vec alb, Vec emis, Vec beam, Vec itop, Vec ibot) {
-        auto ov = overrides(std::move(dtau), std::move(ssa), std::move(pmom), std::move(alb), std::move(emis),
-                            std::move(beam), std::move(itop), std::move(ibot));
+      [](DisortConfig config, const std::vector<std::pair<double, double>>& bands, py::object dtau, py::object ssa,
+         py::object pmom, py::object alb, py::object emis, py::object beam, py::object itop, py::object ibot) {
+        Overrides ov(dtau, ssa, pmom, alb, emis, beam, itop, ibot);
         py::gil_scoped_release release;
-        return solveSpectralBands(std::move(config), bands, ov);
+        return ov.as_arrays ? solveSpectralBands(std::move(config), bands, ov.arrays)
+                            : solveSpectralBands(std::move(config), bands, ov.lists);
       },
       py::arg("config"), py::arg("wavenumber_bands"), DISORTPP_OVERRIDE_ARGS,
-      "Solve for (low, high) wavenumber bands as one GPU batch; returns DisortResult list.");
+      "Solve for (low, high) wavenumber bands as one GPU batch; returns DisortResult list. Per-band overrides "
+      "are nested lists or NumPy arrays.");
 
   m.def(
       "solve_flux_spectral",
-      [](DisortFluxConfig& config, const Vec& wavenumbers, Mat dtau, Mat ssa, Cube pmom, Vec alb, Vec emis, Vec beam,
-         Vec itop, Vec ibot) {
-        auto ov = overrides(std::move(dtau), std::move(ssa), std::move(pmom), std::move(alb), std::move(emis),
-                            std::move(beam), std::move(itop), std::move(ibot));
+      [](DisortFluxConfig& config, const Vec& wavenumbers, py::object dtau, py::object ssa, py::object pmom,
+         py::object alb, py::object emis, py::object beam, py::object itop, py::object ibot) {
+        Overrides ov(dtau, ssa, pmom, alb, emis, beam, itop, ibot);
         py::gil_scoped_release release;
-        return solveFluxSpectral(config, wavenumbers, ov);
+        return ov.as_arrays ? solveFluxSpectral(config, wavenumbers, ov.arrays)
+                            : solveFluxSpectral(config, wavenumbers, ov.lists);
       },
       py::arg("config"), py::arg("wavenumbers"), DISORTPP_OVERRIDE_ARGS,
-      "Flux-only solve at single wavenumbers as one GPU batch; returns FluxResult list.");
+      "Flux-only solve at single wavenumbers as one GPU batch; returns FluxResult list. NumPy override arrays "
+      "are packed into the GPU batch directly.");
 
   m.def(
       "solve_flux_spectral_bands",
-      [](DisortFluxConfig& config, const std::vector<std::pair<double, double>>& bands, Mat dtau, Mat ssa, Cube pmom,
-         Vec alb, Vec emis, Vec beam, Vec itop, Vec ibot) {
-        auto ov = overrides(std::move(dtau), std::move(ssa), std::move(pmom), std::move(alb), std::move(emis),
-                            std::move(beam), std::move(itop), std::move(ibot));
+      [](DisortFluxConfig& config, const std::vector<std::pair<double, double>>& bands, py::object dtau,
+         py::object ssa, py::object pmom, py::object alb, py::object emis, py::object beam, py::object itop,
+         py::object ibot) {
+        Overrides ov(dtau, ssa, pmom, alb, emis, beam, itop, ibot);
         py::gil_scoped_release release;
-        return solveFluxSpectralBands(config, bands, ov);
+        return ov.as_arrays ? solveFluxSpectralBands(config, bands, ov.arrays)
+                            : solveFluxSpectralBands(config, bands, ov.lists);
       },
       py::arg("config"), py::arg("wavenumber_bands"), DISORTPP_OVERRIDE_ARGS,
-      "Flux-only solve for (low, high) wavenumber bands as one GPU batch; returns FluxResult list.");
+      "Flux-only solve for (low, high) wavenumber bands as one GPU batch; returns FluxResult list. NumPy "
+      "override arrays are packed into the GPU batch directly.");
 
   m.def(
       "solve_flux_batch",

@@ -133,6 +133,9 @@ struct Packed {
   std::vector<int64_t> lay_off, mom_off;
   std::vector<double> dtau, ssa, pmom, temper, alt, scal, out;
   int64_t layers = 0, levels = 0, moments = 0;
+  // Caller-owned arrays used in place of dtau / ssa / pmom when set (the
+  // array-form spectral drivers with inputs already in the batch layout).
+  const double *ext_dtau = nullptr, *ext_ssa = nullptr, *ext_pmom = nullptr;
 
   void add(const Instance& in) {
     const int L = in.nlyr, P = in.pstride;
@@ -217,9 +220,9 @@ struct Packed {
     b.pstride = pstride.data();
     b.lay_off = lay_off.data();
     b.mom_off = mom_off.data();
-    b.dtau = dtau.data();
-    b.ssa = ssa.data();
-    b.pmom = pmom.data();
+    b.dtau = ext_dtau ? ext_dtau : dtau.data();
+    b.ssa = ext_ssa ? ext_ssa : ssa.data();
+    b.pmom = ext_pmom ? ext_pmom : pmom.data();
     b.temper = (flags & DISORT_F_THERMAL) ? temper.data() : nullptr;
     b.alt = (flags & DISORT_F_SPHERICAL) ? alt.data() : nullptr;
     b.scal = scal.data();
@@ -227,6 +230,7 @@ struct Packed {
     return b;
   }
 
+ public:
   // Runs one C-ABI call on the process's context (created on first use);
   // throws the reference's exception types.
   template <class F>
@@ -534,6 +538,190 @@ std::vector<FluxResult> solveFluxSpectralBands(DisortFluxConfig& config,
 
 std::vector<FluxResult> solveFluxSpectral(DisortFluxConfig& config, const std::vector<double>& wavenumbers,
                                           const SpectralOverrides& ov) {
+  std::vector<std::pair<double, double>> bands;
+  bands.reserve(wavenumbers.size());
+  for (double w : wavenumbers) bands.emplace_back(w, w);
+  return solveFluxSpectralBands(config, bands, ov);
+}
+
+// ---------------------------------------------------------------------------
+// Array-form spectral drivers
+// ---------------------------------------------------------------------------
+namespace {
+void need_rows(const ArrayView& v, size_t bands, const char* name) {
+  if (v.given() && v.n < static_cast<int64_t>(bands))
+    throw std::out_of_range(std::string(name) + " has " + std::to_string(v.n) + " rows for " + std::to_string(bands) +
+                            " bands");
+}
+
+void need_rows(const SpectralArrays& ov, size_t bands) {
+  need_rows(ov.delta_tau, bands, "delta_tau");
+  need_rows(ov.single_scat_albedo, bands, "single_scat_albedo");
+  need_rows(ov.phase_function_moments, bands, "phase_function_moments");
+  need_rows(ov.surface_albedo, bands, "surface_albedo");
+  need_rows(ov.emissivity_top, bands, "emissivity_top");
+  need_rows(ov.direct_beam_flux, bands, "direct_beam_flux");
+  need_rows(ov.isotropic_flux_top, bands, "isotropic_flux_top");
+  need_rows(ov.isotropic_flux_bottom, bands, "isotropic_flux_bottom");
+}
+
+double scalar_or(const ArrayView& v, size_t i, double dflt) { return v.given() ? v.data[i] : dflt; }
+
+// Band i's configuration as the nested-list driver builds it.
+DisortFluxConfig flux_band_config(const DisortFluxConfig& config, const std::pair<double, double>& band,
+                                  const SpectralArrays& ov, size_t i) {
+  DisortFluxConfig c = config;
+  c.wavenumber_low = band.first;
+  c.wavenumber_high = band.second;
+  if (ov.delta_tau.given())
+    c.delta_tau.assign(ov.delta_tau.data + i * ov.delta_tau.cols, ov.delta_tau.data + (i + 1) * ov.delta_tau.cols);
+  if (ov.single_scat_albedo.given())
+    c.single_scat_albedo.assign(ov.single_scat_albedo.data + i * ov.single_scat_albedo.cols,
+                                ov.single_scat_albedo.data + (i + 1) * ov.single_scat_albedo.cols);
+  if (ov.phase_function_moments.given()) {
+    const ArrayView& v = ov.phase_function_moments;
+    c.phase_function_moments.assign(v.rows, {});
+    for (int64_t r = 0; r < v.rows; ++r)
+      c.phase_function_moments[r].assign(v.data + (i * v.rows + r) * v.cols, v.data + (i * v.rows + r + 1) * v.cols);
+  }
+  c.surface_albedo = scalar_or(ov.surface_albedo, i, c.surface_albedo);
+  c.emissivity_top = scalar_or(ov.emissivity_top, i, c.emissivity_top);
+  c.direct_beam_flux = scalar_or(ov.direct_beam_flux, i, c.direct_beam_flux);
+  c.isotropic_flux_top = scalar_or(ov.isotropic_flux_top, i, c.isotropic_flux_top);
+  c.isotropic_flux_bottom = scalar_or(ov.isotropic_flux_bottom, i, c.isotropic_flux_bottom);
+  return c;
+}
+}  // namespace
+
+SpectralOverrides SpectralArrays::to_overrides(size_t bands) const {
+  need_rows(*this, bands);
+  SpectralOverrides o;
+  auto mat = [](const ArrayView& v, std::vector<std::vector<double>>& dst) {
+    if (!v.given()) return;
+    dst.resize(v.n);
+    for (int64_t i = 0; i < v.n; ++i) dst[i].assign(v.data + i * v.cols, v.data + (i + 1) * v.cols);
+  };
+  auto vec = [](const ArrayView& v, std::vector<double>& dst) {
+    if (v.given()) dst.assign(v.data, v.data + v.n);
+  };
+  mat(delta_tau, o.delta_tau);
+  mat(single_scat_albedo, o.single_scat_albedo);
+  if (phase_function_moments.given()) {
+    const ArrayView& v = phase_function_moments;
+    o.phase_function_moments.resize(v.n);
+    for (int64_t i = 0; i < v.n; ++i) {
+      o.phase_function_moments[i].resize(v.rows);
+      for (int64_t r = 0; r < v.rows; ++r)
+        o.phase_function_moments[i][r].assign(v.data + (i * v.rows + r) * v.cols, v.data + (i * v.rows + r + 1) * v.cols);
+    }
+  }
+  vec(surface_albedo, o.surface_albedo);
+  vec(emissivity_top, o.emissivity_top);
+  vec(direct_beam_flux, o.direct_beam_flux);
+  vec(isotropic_flux_top, o.isotropic_flux_top);
+  vec(isotropic_flux_bottom, o.isotropic_flux_bottom);
+  return o;
+}
+
+// The bands share every shape (layers, streams, moment rows, flags), so the
+// batch is packed from the arrays directly: one pass over the inputs, no
+// per-band configuration, and rows already in the batch layout (top-down,
+// num_layers entries, nmomNstr()+1 moments) are passed to the GPU copy in
+// place.  Band 0 is validated as a whole configuration (the structure every
+// band shares); the per-band values are validated by disort_flux_solve with
+// the reference's rules, and a failing band is re-validated as a
+// configuration so that the message is the reference's.
+std::vector<FluxResult> solveFluxSpectralBands(DisortFluxConfig& config,
+                                               const std::vector<std::pair<double, double>>& bands,
+                                               const SpectralArrays& ov) {
+  if (!isSupportedStreamCount(config.num_streams)) unsupported_streams(config.num_streams);
+  need_rows(ov, bands.size());
+  const int64_t nb = static_cast<int64_t>(bands.size());
+  if (nb == 0) return {};
+  validate_flux(flux_band_config(config, bands[0], ov, 0));
+  const int L = config.num_layers, N = config.num_streams, P = config.nmomNstr() + 1;
+  const bool rev = config.index_from_bottom;
+  Packed p;
+  p.flags = flux_flags(config);
+  p.nstr.assign(nb, N);
+  p.nlyr.assign(nb, L);
+  p.pstride.assign(nb, P);
+  p.lay_off.resize(nb);
+  p.mom_off.resize(nb);
+  for (int64_t i = 0; i < nb; ++i) {
+    p.lay_off[i] = i * L;
+    p.mom_off[i] = i * L * P;
+  }
+  p.layers = nb * L;
+  p.levels = nb * (L + 1);
+  p.moments = nb * L * P;
+  // Layer rows: from the override array (row i) or the shared config.
+  auto layer_rows = [&](const ArrayView& v, const std::vector<double>& dflt, std::vector<double>& dst,
+                        const double** ext) {
+    if (v.given() && !rev && v.cols == L) {
+      *ext = v.data;
+      return;
+    }
+    dst.resize(static_cast<size_t>(nb) * L);
+    for (int64_t i = 0; i < nb; ++i) {
+      const double* src = v.given() ? v.data + i * v.cols : dflt.data();
+      const int64_t sz = v.given() ? v.cols : static_cast<int64_t>(dflt.size());
+      for (int l = 0; l < L; ++l) dst[i * L + l] = src[rev ? sz - 1 - l : l];
+    }
+  };
+  layer_rows(ov.delta_tau, config.delta_tau, p.dtau, &p.ext_dtau);
+  layer_rows(ov.single_scat_albedo, config.single_scat_albedo, p.ssa, &p.ext_ssa);
+  const ArrayView& pm = ov.phase_function_moments;
+  if (pm.given() && !rev && pm.rows == L && pm.cols == P) {
+    p.ext_pmom = pm.data;
+  } else {
+    p.pmom.resize(static_cast<size_t>(p.moments));
+    const int64_t rows = pm.given() ? pm.rows : static_cast<int64_t>(config.phase_function_moments.size());
+    for (int64_t i = 0; i < nb; ++i)
+      for (int l = 0; l < L; ++l) {
+        const int64_t r = rev ? rows - 1 - l : l;
+        const double* src = pm.given() ? pm.data + (i * pm.rows + r) * pm.cols : config.phase_function_moments[r].data();
+        std::copy(src, src + P, p.pmom.begin() + (i * L + l) * P);
+      }
+  }
+  auto level_rows = [&](const std::vector<double>& v, std::vector<double>& dst) {
+    dst.resize(static_cast<size_t>(p.levels));
+    const int64_t sz = static_cast<int64_t>(v.size());
+    for (int64_t i = 0; i < nb; ++i)
+      for (int l = 0; l <= L; ++l) dst[i * (L + 1) + l] = v[rev ? sz - 1 - l : l];
+  };
+  if (p.flags & DISORT_F_THERMAL) level_rows(config.temperature, p.temper);
+  if (p.flags & DISORT_F_SPHERICAL) level_rows(config.level_altitudes, p.alt);
+  p.scal.resize(static_cast<size_t>(nb) * DISORT_NSCAL);
+  for (int64_t i = 0; i < nb; ++i)
+    fill_scal(p.scal.data() + i * DISORT_NSCAL, scalar_or(ov.direct_beam_flux, i, config.direct_beam_flux),
+              config.direct_beam_mu, scalar_or(ov.isotropic_flux_top, i, config.isotropic_flux_top),
+              scalar_or(ov.isotropic_flux_bottom, i, config.isotropic_flux_bottom), config.temperature_top,
+              config.temperature_bottom, scalar_or(ov.emissivity_top, i, config.emissivity_top),
+              scalar_or(ov.surface_albedo, i, config.surface_albedo), bands[i].first, bands[i].second,
+              config.bottom_radius);
+  try {
+    p.solve();
+    if (wants_jacobian(config)) p.jacobian();
+  } catch (const std::invalid_argument& e) {
+    // "instance i: <message>": re-validate band i as a configuration.
+    const std::string m = e.what();
+    if (m.rfind("instance ", 0) == 0) {
+      const size_t i = std::stoul(m.substr(9));
+      if (i < bands.size()) validate_flux(flux_band_config(config, bands[i], ov, i));
+    }
+    throw;
+  }
+  std::vector<FluxResult> results(nb);
+  for (int64_t i = 0; i < nb; ++i) {
+    p.unpack(i, rev, results[i]);
+    if (wants_jacobian(config)) p.unpack_jacobian(i, rev, results[i]);
+  }
+  return results;
+}
+
+std::vector<FluxResult> solveFluxSpectral(DisortFluxConfig& config, const std::vector<double>& wavenumbers,
+                                          const SpectralArrays& ov) {
   std::vector<std::pair<double, double>> bands;
   bands.reserve(wavenumbers.size());
   for (double w : wavenumbers) bands.emplace_back(w, w);
@@ -1309,6 +1497,19 @@ std::vector<DisortResult> solveSpectral(DisortConfig config, const std::vector<d
   bands.reserve(wavenumbers.size());
   for (double w : wavenumbers) bands.emplace_back(w, w);
   return solveSpectralBands(std::move(config), bands, ov);
+}
+
+// The general drivers build a configuration per band anyway (user grids,
+// intensities, mutation of the config), so the arrays become the nested
+// form in C++.
+std::vector<DisortResult> solveSpectralBands(DisortConfig config, const std::vector<std::pair<double, double>>& bands,
+                                             const SpectralArrays& ov) {
+  return solveSpectralBands(std::move(config), bands, ov.to_overrides(bands.size()));
+}
+
+std::vector<DisortResult> solveSpectral(DisortConfig config, const std::vector<double>& wavenumbers,
+                                        const SpectralArrays& ov) {
+  return solveSpectral(std::move(config), wavenumbers, ov.to_overrides(wavenumbers.size()));
 }
 
 }  // namespace disortpp

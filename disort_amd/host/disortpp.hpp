@@ -270,6 +270,23 @@ struct SpectralOverrides {
   std::vector<double> isotropic_flux_bottom;
 };
 
+// The same overrides as borrowed C-contiguous arrays (the NumPy form; no
+// counterpart in the reference, whose bindings take nested lists).  A null
+// pointer means "not given".  delta_tau and single_scat_albedo are
+// [n][cols], phase_function_moments [n][rows][cols], the scalars [n]; row i
+// overrides band i exactly as the nested-list entry i does, and n smaller
+// than the band count raises std::out_of_range like the lists' .at(i).
+struct ArrayView {
+  const double* data = nullptr;
+  int64_t n = 0, rows = 1, cols = 1;
+  bool given() const { return data != nullptr; }
+};
+struct SpectralArrays {
+  ArrayView delta_tau, single_scat_albedo, phase_function_moments;
+  ArrayView surface_albedo, emissivity_top, direct_beam_flux, isotropic_flux_top, isotropic_flux_bottom;
+  SpectralOverrides to_overrides(size_t bands) const;  // the nested-list form (copies)
+};
+
 // Solve many flux configurations as ONE GPU batch (ragged in layers and
 // streams).  Configurations must share the batch-uniform flags
 // (use_thermal_emission, use_spherical_beam, use_delta_m_plus,
@@ -318,6 +335,20 @@ std::vector<DisortResult> solveSpectralBands(DisortConfig config,
                                              const SpectralOverrides& ov);
 std::vector<DisortResult> solveSpectral(DisortConfig config, const std::vector<double>& wavenumbers,
                                         const SpectralOverrides& ov);
+// Array forms: the flux drivers pack the per-band arrays straight into the
+// GPU batch (no per-band configuration is built; with top-down layers and
+// rows of exactly num_layers (and nmomNstr()+1) entries the arrays are handed
+// to the GPU copy as they are).
+std::vector<FluxResult> solveFluxSpectralBands(DisortFluxConfig& config,
+                                               const std::vector<std::pair<double, double>>& wavenumber_bands,
+                                               const SpectralArrays& ov);
+std::vector<FluxResult> solveFluxSpectral(DisortFluxConfig& config, const std::vector<double>& wavenumbers,
+                                          const SpectralArrays& ov);
+std::vector<DisortResult> solveSpectralBands(DisortConfig config,
+                                             const std::vector<std::pair<double, double>>& wavenumber_bands,
+                                             const SpectralArrays& ov);
+std::vector<DisortResult> solveSpectral(DisortConfig config, const std::vector<double>& wavenumbers,
+                                        const SpectralArrays& ov);
 
 // GPU device used by the facade (default 0, or DISORT_AMD_DEVICE).
 void setDevice(int device);

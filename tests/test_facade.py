@@ -147,3 +147,21 @@ def test_flux_config_garcia_siewert_fill():
     c.set_cloud_garcia_siewert(lc=0)
     assert c.phase_function_moments[0][16] == 13.058 / 33
 
+
+
+def test_spectral_override_rows_checked_before_the_gpu():
+    """Fewer override rows than bands: IndexError in the nested-list form
+    (.at(i)) and in the NumPy-array form alike; band 0's configuration is
+    validated with the reference's messages before any GPU work."""
+    import numpy as np
+
+    c = c1_flux_config()
+    bands = [(100.0, 101.0)] * 3
+    with pytest.raises(IndexError):
+        d.solve_flux_spectral_bands(c, bands, delta_tau=np.ones((2, 4)))
+    with pytest.raises(IndexError):
+        d.solve_flux_spectral_bands(c, bands, surface_albedo=[0.1, 0.2])
+    with pytest.raises(ValueError, match=r"delta_tau\[2\] must be >= 0"):
+        d.solve_flux_spectral_bands(c, bands, delta_tau=np.array([[0.1, 0.2, -1.0, 0.3]] * 3))
+    with pytest.raises(ValueError, match=r"delta_tau size \(3\) < num_layers \(4\)"):
+        d.solve_flux_spectral_bands(c, bands, delta_tau=np.ones((3, 3)))

@@ -22,7 +22,7 @@ DISORT_E_UNSUPPORTED = 2
 DISORT_E_NUMERIC = 3
 MEM_HOST = 0
 MEM_DEVICE = 1
-V_DEFAULT, V_LANE, V_WORKGROUP = 0, 1, 2
+V_DEFAULT, V_LANE, V_WORKGROUP, V_LANE_PAIR = 0, 1, 2, 3
 
 _lib = None
 
@@ -234,7 +234,7 @@ class DeviceFluxBatch:
         self.cstruct = s
 
     def set_variant(self, nstr: int, variant: int):
-        """Kernel variant of the bucket with `nstr` streams (V_LANE / V_WORKGROUP)."""
+        """Kernel variant of the bucket with `nstr` streams (V_LANE / V_LANE_PAIR / V_WORKGROUP)."""
         _check(self.ctx.ptr, lib().disort_flux_plan_set_variant(self.ctx.ptr, self.plan, nstr, variant))
 
     def get_variant(self, nstr: int) -> int:

@@ -24,7 +24,7 @@
 
 namespace dsa {
 // Register kernels (flux_kernels.hip), N in {4,...,16}.
-bool launch_flux_bucket(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev);
+bool launch_flux_bucket(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev, bool pair);
 // Workgroup kernels (flux_wide.hip), N in {8, 16, 32, 64}.
 bool launch_flux_bucket_wg(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev);
 }
@@ -141,8 +141,19 @@ namespace {
 
 bool lane_kernels(int n) { return n >= 4 && n <= 16 && n % 2 == 0; }
 bool wg_kernels(int n) { return (n >= 4 && n <= 16 && n % 2 == 0) || n == 32 || n == 64; }
-int default_variant(int n) { return lane_kernels(n) ? DISORT_V_LANE : DISORT_V_WORKGROUP; }
-bool variant_ok(int n, int v) { return (v == DISORT_V_LANE && lane_kernels(n)) || (v == DISORT_V_WORKGROUP && wg_kernels(n)); }
+// Register kernels where they exist; the two-lanes-per-layer eigen kernel
+// where it measured faster on the 10,000-band workload (6 to 12 streams:
+// no spill at two or three wavefronts per SIMD).
+int default_variant(int n) {
+  if (!lane_kernels(n)) return DISORT_V_WORKGROUP;
+  return (n >= 6 && n <= 12) ? DISORT_V_LANE_PAIR : DISORT_V_LANE;
+}
+bool variant_ok(int n, int v) {
+  return ((v == DISORT_V_LANE || v == DISORT_V_LANE_PAIR) && lane_kernels(n)) || (v == DISORT_V_WORKGROUP && wg_kernels(n));
+}
+const char* variant_name(int v) {
+  return v == DISORT_V_LANE ? "lane" : v == DISORT_V_LANE_PAIR ? "lane_pair" : "workgroup";
+}
 
 // Host-side validation of one instance, with the reference's messages
 // (src/DisortFluxConfig.cpp:43-143).
@@ -192,7 +203,7 @@ std::string validate_instance(const disort_flux_batch* b, int64_t i) {
 struct Bucket {
   int nstr = 0;
   int variant = 0;
-  double tuned_ms[3] = {-1.0, -1.0, -1.0};  // per variant, from the autotuner
+  double tuned_ms[4] = {-1.0, -1.0, -1.0, -1.0};  // per variant, from the autotuner
   int64_t nb = 0, n_tasks = 0, n_lv = 0;
   int32_t* d_inst = nullptr;
   int64_t* d_task0 = nullptr;
@@ -345,7 +356,8 @@ FluxArgs bucket_args(disort_ctx* ctx, const PlanArgs& pa, const Bucket& bk, size
 }
 
 int launch_bucket(disort_ctx* ctx, const FluxArgs& a, const Bucket& bk, hipStream_t st, hipEvent_t* ev) {
-  const bool ok = bk.variant == DISORT_V_LANE ? dsa::launch_flux_bucket(bk.nstr, a, st, ev)
+  const bool ok = bk.variant != DISORT_V_WORKGROUP
+                      ? dsa::launch_flux_bucket(bk.nstr, a, st, ev, bk.variant == DISORT_V_LANE_PAIR)
                                               : dsa::launch_flux_bucket_wg(bk.nstr, a, st, ev);
   if (!ok) return fail(ctx, DISORT_E_UNSUPPORTED, "no kernels for num_streams=" + std::to_string(bk.nstr));
   HIPCHK(ctx, hipGetLastError());
@@ -699,7 +711,7 @@ int disort_flux_autotune(disort_ctx* ctx, disort_flux_plan* plan, const disort_f
     const FluxArgs a = bucket_args(ctx, pa, bk, k);
     int best = bk.variant;
     double best_ms = 1e300;
-    for (int v : {DISORT_V_LANE, DISORT_V_WORKGROUP}) {
+    for (int v : {DISORT_V_LANE, DISORT_V_LANE_PAIR, DISORT_V_WORKGROUP}) {
       if (!variant_ok(bk.nstr, v)) continue;
       if ((plan->flags & DISORT_F_SPHERICAL) && v != DISORT_V_WORKGROUP) continue;
       bk.variant = v;
@@ -712,10 +724,12 @@ int disort_flux_autotune(disort_ctx* ctx, disort_flux_plan* plan, const disort_f
       }
     }
     bk.variant = best;
-    char buf[256];
-    std::snprintf(buf, sizeof buf, "%s{\"nstr\": %d, \"instances\": %lld, \"layers\": %lld, \"lane_ms\": %.6g, \"workgroup_ms\": %.6g, \"chosen\": \"%s\"}",
+    char buf[320];
+    std::snprintf(buf, sizeof buf,
+                  "%s{\"nstr\": %d, \"instances\": %lld, \"layers\": %lld, \"lane_ms\": %.6g, \"lane_pair_ms\": %.6g, "
+                  "\"workgroup_ms\": %.6g, \"chosen\": \"%s\"}",
                   k ? ", " : "", bk.nstr, (long long)bk.nb, (long long)bk.n_tasks, bk.tuned_ms[DISORT_V_LANE],
-                  bk.tuned_ms[DISORT_V_WORKGROUP], best == DISORT_V_LANE ? "lane" : "workgroup");
+                  bk.tuned_ms[DISORT_V_LANE_PAIR], bk.tuned_ms[DISORT_V_WORKGROUP], variant_name(best));
     js += buf;
   }
   double serial_ms = 0.0, conc_ms = 0.0;

@@ -119,7 +119,10 @@ __device__ void jacobi_eigen(double (&S)[NN * (NN + 1) / 2], double (&V)[NN][NN]
         const double app = S[tri(p, p)], aqq = S[tri(q, q)];
         const double theta = (aqq - app) / (2.0 * apq);
         const double at = fabs(theta);
-        double t = (at > 1e150) ? 0.5 / at : 1.0 / (at + sqrt(theta * theta + 1.0));
+        // No overflow guard (for |theta| > 1e154, theta^2 = inf gives t = 0, a
+        // rotation below rounding): a select between two divisions becomes a
+        // branch inside the unrolled sweep and multiplies the register demand.
+        double t = 1.0 / (at + sqrt(theta * theta + 1.0));
         t = (theta < 0.0) ? -t : t;
         t = (apq == 0.0) ? 0.0 : t;
         const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
@@ -468,6 +471,519 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
 }
 
 // ===========================================================================
+// k_layer2: the same eigen-system and particular solutions with two lanes
+// per layer (a lane pair, lanes 2p and 2p+1 of the wavefront).  Both lanes
+// hold the symmetric matrices (S, the Cholesky factors) and perform the same
+// rotations on them, so they agree bit for bit; the eigenvector matrix V is
+// split by rows (lane h owns rows h*RH .. h*RH+RH-1), which halves the
+// register demand of the Jacobi sweep (S + half of V) so that two or three
+// wavefronts fit per SIMD without spilling.  Products with V combine the
+// two halves by a DPP exchange within the pair (quad_perm 1,0,3,2); IEEE
+// addition is commutative, so both lanes form identical sums.  The
+// Cholesky factor of -B_s is parked in LDS (one copy per pair) during the
+// sweep.  Same formulas and operation order per element as k_layer except
+// the V^T x products, which are summed in two halves.
+// ===========================================================================
+__device__ __forceinline__ double pair_swap(double x) { return dpp_f64<0xB1>(x); }  // quad_perm(1,0,3,2)
+
+template <int NN>
+__global__ __launch_bounds__(64, (NN <= 4 ? 3 : 2)) void k_layer2(FluxArgs a) {
+  constexpr int N = 2 * NN;
+  constexpr int NT = NN * (NN + 1) / 2;
+  constexpr int LS = NT | 1;        // LDS pitch of a parked factor (odd: fewer bank conflicts)
+  constexpr int RH = (NN + 1) / 2;  // V rows per lane (lane 1 pads one row when NN is odd)
+  __shared__ double Lsh[32 * LS];   // Cholesky factor of -B_s, one per pair, for the whole kernel
+  const int h = threadIdx.x & 1;
+  const int64_t t = ((int64_t)blockIdx.x * 64 + threadIdx.x) >> 1;
+  if (t >= a.n_tasks) return;  // pair-uniform: both lanes of a pair leave together
+  double* Lp = Lsh + (threadIdx.x >> 1) * LS;
+  const int i = a.t_inst[t];
+  const int lc = a.t_lc[t];
+  const int64_t g = a.lay_off[i] + lc;
+  const int64_t v = a.lay_off[i] + i + lc;
+  const int P = a.pstride[i];
+  const double* pm = a.pmom + a.mom_off[i] + (int64_t)lc * P;
+  const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
+  const int64_t TL = a.total_layers, TV = a.total_levels;
+  const double* tb = a.tab;
+  constexpr TabOff to(N);
+  constexpr EigOff eo(N);
+
+  const double ws = a.lay_d[LD_SSA_S * TL + g];
+  const double f = a.lay_d[LD_F * TL + g];
+  const double sig2 = a.lay_d[LD_SIG2 * TL + g];
+  auto scaled_coeffs = [&](double (&chi)[N]) {
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      const double pl_ = pm[l];
+      double c;
+      if (sig2 < 0.0) c = (2.0 * l + 1.0) * ws * pl_;
+      else if (sig2 == 0.0) c = (2.0 * l + 1.0) * ws * (pl_ - f) / (1.0 - f);
+      else c = (2.0 * l + 1.0) * ws * (pl_ - f * exp(-(double)(l * l) / (2.0 * sig2))) / (1.0 - f);
+      chi[l] = c;
+    }
+  };
+  // One scaled coefficient chi'_l (the same expression as scaled_coeffs).
+  auto chi_of = [&](int l) -> double {
+    const double pl_ = pm[l];
+    if (sig2 < 0.0) return (2.0 * l + 1.0) * ws * pl_;
+    if (sig2 == 0.0) return (2.0 * l + 1.0) * ws * (pl_ - f) / (1.0 - f);
+    return (2.0 * l + 1.0) * ws * (pl_ - f * exp(-(double)(l * l) / (2.0 * sig2))) / (1.0 - f);
+  };
+  const double F0 = sc[DISORT_S_BEAM_FLUX];
+  const double mu0 = sc[DISORT_S_BEAM_MU];
+  const bool beam = F0 > 0.0;
+
+  // Full vector x from the two halves (own rows in `mine`).
+  auto join = [&](const double (&mine)[RH], double (&x)[NN]) {
+#pragma unroll
+    for (int q = 0; q < RH; ++q) {
+      const double other = pair_swap(mine[q]);
+      x[q] = h ? other : mine[q];
+      if (RH + q < NN) x[RH + q] = h ? mine[q] : other;
+    }
+  };
+  // x of the rows this lane owns.
+  auto own = [&](const double (&x)[NN], int q) -> double {
+    return (RH + q < NN) ? (h ? x[RH + q] : x[q]) : (h ? 0.0 : x[q]);
+  };
+  // Triangular products and solves with the parked factor L (LayerMath's
+  // operations, L read from LDS).  Each starts with a compiler memory
+  // barrier so that the factor is re-read from LDS rather than kept live in
+  // registers between uses (the LDS reads are cheap, the registers are not).
+  auto reload = []() { asm volatile("" ::: "memory"); };
+  auto Lx = [&](int r, int m) -> double { return Lp[tri(r, m)]; };
+  auto lsolve = [&](double (&x)[NN]) {
+    reload();
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      double s2 = x[r];
+#pragma unroll
+      for (int m = 0; m < r; ++m) s2 -= Lx(r, m) * x[m];
+      x[r] = s2 / Lx(r, r);
+    }
+  };
+  auto ltsolve = [&](double (&x)[NN]) {
+    reload();
+#pragma unroll
+    for (int r = NN - 1; r >= 0; --r) {
+      double s2 = x[r];
+#pragma unroll
+      for (int m = r + 1; m < NN; ++m) s2 -= Lx(m, r) * x[m];
+      x[r] = s2 / Lx(r, r);
+    }
+  };
+  auto lmul = [&](double (&x)[NN]) {
+    reload();
+#pragma unroll
+    for (int r = NN - 1; r >= 0; --r) {
+      double s2 = 0.0;
+#pragma unroll
+      for (int m = 0; m <= r; ++m) s2 += Lx(r, m) * x[m];
+      x[r] = s2;
+    }
+  };
+  auto ltmul = [&](double (&x)[NN]) {
+    reload();
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      double s2 = 0.0;
+#pragma unroll
+      for (int m = r; m < NN; ++m) s2 += Lx(m, r) * x[m];
+      x[r] = s2;
+    }
+  };
+
+  // -B_s, then its Cholesky factor in place; parked in LDS (one copy per
+  // pair: lane h stores the entries q = h mod 2).
+  {
+    double nB[NT];
+    {
+      double chi[N];
+      scaled_coeffs(chi);
+      // Odd-l sums accumulated one l at a time (the per-element order of
+      // the additions is unchanged), so only one table row is live.
+#pragma unroll
+      for (int q = 0; q < NT; ++q) nB[q] = 0.0;
+#pragma unroll
+      for (int l = 1; l < N; l += 2) {
+        double ql[NN];
+#pragma unroll
+        for (int r = 0; r < NN; ++r) ql[r] = tb[to.Q() + l * NN + r];
+#pragma unroll
+        for (int r = 0; r < NN; ++r)
+#pragma unroll
+          for (int c = 0; c <= r; ++c) nB[tri(r, c)] += chi[l] * ql[r] * ql[c];
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int r = 0; r < NN; ++r)
+#pragma unroll
+        for (int c = 0; c <= r; ++c) {
+          const double dg = (r == c) ? 1.0 / tb[to.cmu() + r] : 0.0;
+          nB[tri(r, c)] = dg - nB[tri(r, c)];
+        }
+    }
+    int ok = 1;
+#pragma unroll
+    for (int c = 0; c < NN; ++c) {
+      double x = nB[tri(c, c)];
+#pragma unroll
+      for (int k = 0; k < c; ++k) x -= nB[tri(c, k)] * nB[tri(c, k)];
+      ok &= (x > 0.0);
+      const double lcc = sqrt(x > 0.0 ? x : 1.0);
+      nB[tri(c, c)] = lcc;
+#pragma unroll
+      for (int r = c + 1; r < NN; ++r) {
+        double s2 = nB[tri(r, c)];
+#pragma unroll
+        for (int k = 0; k < c; ++k) s2 -= nB[tri(r, k)] * nB[tri(c, k)];
+        nB[tri(r, c)] = s2 / lcc;
+      }
+    }
+    if (!ok && h == 0) atomicOr(&a.inst_i[i * kInstI + II_STATUS], DISORT_ST_EIGEN);
+#pragma unroll
+    for (int q = 0; q < NT; ++q)
+      if ((q & 1) == h) Lp[q] = nB[q];
+  }
+  wave_lds_order();
+
+  // S = L^T (-A_s) L, column by column.
+  double S[NT];
+  {
+    double As[NT];
+    {
+      double chi[N];
+      scaled_coeffs(chi);
+#pragma unroll
+      for (int q = 0; q < NT; ++q) As[q] = 0.0;
+#pragma unroll
+      for (int l = 0; l < N; l += 2) {
+        double ql[NN];
+#pragma unroll
+        for (int r = 0; r < NN; ++r) ql[r] = tb[to.Q() + l * NN + r];
+#pragma unroll
+        for (int r = 0; r < NN; ++r)
+#pragma unroll
+          for (int c = 0; c <= r; ++c) As[tri(r, c)] += chi[l] * ql[r] * ql[c];
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int r = 0; r < NN; ++r)
+#pragma unroll
+        for (int c = 0; c <= r; ++c) {
+          const double dg = (r == c) ? 1.0 / tb[to.cmu() + r] : 0.0;
+          As[tri(r, c)] = As[tri(r, c)] - dg;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NN; ++c) {
+      double tc[NN];
+#pragma unroll
+      for (int r = 0; r < NN; ++r) {
+        double s2 = 0.0;
+#pragma unroll
+        for (int k = c; k < NN; ++k) s2 -= ((r >= k) ? As[tri(r, k)] : As[tri(k, r)]) * Lx(k, c);
+        tc[r] = s2;
+      }
+#pragma unroll
+      for (int r = 0; r <= c; ++r) {
+        double s2 = 0.0;
+#pragma unroll
+        for (int k = r; k < NN; ++k) s2 += Lx(k, r) * tc[k];
+        S[tri(c, r)] = s2;
+      }
+    }
+  }
+
+  // Cyclic Jacobi: S in both lanes, V rows split.
+  double Vh[RH][NN];
+#pragma unroll
+  for (int q = 0; q < RH; ++q)
+#pragma unroll
+    for (int c = 0; c < NN; ++c) Vh[q][c] = (h * RH + q == c) ? 1.0 : 0.0;
+  for (int sweep = 0; sweep < 16; ++sweep) {
+    double off = 0.0, dia = 0.0;
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      dia += S[tri(r, r)] * S[tri(r, r)];
+#pragma unroll
+      for (int c = 0; c < r; ++c) off += S[tri(r, c)] * S[tri(r, c)];
+    }
+    const int done = (off == 0.0) || (off <= 1e-36 * dia);
+    if (__all(done)) break;
+#pragma unroll
+    for (int p = 0; p < NN - 1; ++p) {
+#pragma unroll
+      for (int q = p + 1; q < NN; ++q) {
+        const double apq = S[tri(q, p)];
+        const double app = S[tri(p, p)], aqq = S[tri(q, q)];
+        const double theta = (aqq - app) / (2.0 * apq);
+        const double at = fabs(theta);
+        double tt = 1.0 / (at + sqrt(theta * theta + 1.0));  // as in jacobi_eigen
+        tt = (theta < 0.0) ? -tt : tt;
+        tt = (apq == 0.0) ? 0.0 : tt;
+        const double c = 1.0 / sqrt(tt * tt + 1.0), sn = tt * c;
+#pragma unroll
+        for (int k = 0; k < NN; ++k) {
+          if (k == p || k == q) continue;
+          const int ip = (k > p) ? tri(k, p) : tri(p, k);
+          const int iq = (k > q) ? tri(k, q) : tri(q, k);
+          const double akp = S[ip], akq = S[iq];
+          S[ip] = c * akp - sn * akq;
+          S[iq] = sn * akp + c * akq;
+        }
+        S[tri(p, p)] = app - tt * apq;
+        S[tri(q, q)] = aqq + tt * apq;
+        S[tri(q, p)] = 0.0;
+#pragma unroll
+        for (int k = 0; k < RH; ++k) {
+          const double vkp = Vh[k][p], vkq = Vh[k][q];
+          Vh[k][p] = c * vkp - sn * vkq;
+          Vh[k][q] = sn * vkp + c * vkq;
+        }
+      }
+    }
+  }
+  double lam[NN];
+#pragma unroll
+  for (int r = 0; r < NN; ++r) lam[r] = S[tri(r, r)];
+
+  // x <- V diag(d) V^T x with V split by rows.
+  auto vdv = [&](double (&x)[NN], const double (&d)[NN]) {
+    double y[NN];
+#pragma unroll
+    for (int j = 0; j < NN; ++j) {
+      double s2 = 0.0;
+#pragma unroll
+      for (int q = 0; q < RH; ++q) s2 += Vh[q][j] * own(x, q);
+      y[j] = (s2 + pair_swap(s2)) * d[j];
+    }
+    double mine[RH];
+#pragma unroll
+    for (int q = 0; q < RH; ++q) {
+      double s2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < NN; ++j) s2 += Vh[q][j] * y[j];
+      mine[q] = s2;
+    }
+    join(mine, x);
+  };
+
+  // k and G+/- : lane h takes columns j = 2 jj + h.
+  double kv[NN];
+#pragma unroll
+  for (int j = 0; j < NN; ++j) kv[j] = sqrt(fabs(lam[j]));
+#pragma unroll
+  for (int jj = 0; jj < (NN + 1) / 2; ++jj) {
+    const int j0 = 2 * jj, j1 = (2 * jj + 1 < NN) ? 2 * jj + 1 : 2 * jj;
+    // Full column j: our rows of it, and the partner's rows, received in
+    // exchange for our rows of the partner's column.
+    double col[NN];
+#pragma unroll
+    for (int q = 0; q < RH; ++q) {
+      const double keep = h ? Vh[q][j1] : Vh[q][j0];
+      const double other = pair_swap(h ? Vh[q][j0] : Vh[q][j1]);
+      col[q] = h ? other : keep;
+      if (RH + q < NN) col[RH + q] = h ? keep : other;
+    }
+    const int j = h ? j1 : j0;
+    const bool write = (2 * jj + h) < NN;
+    const double kj = h ? kv[j1] : kv[j0];
+    double u[NN], y[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) u[r] = y[r] = col[r];
+    ltsolve(u);
+    lmul(y);
+    if (write) {
+      a.eig[eo.at(t, eo.k() + j)] = kj;
+#pragma unroll
+      for (int r = 0; r < NN; ++r) {
+        const double sdi = tb[to.sdi() + r];
+        const double uu = sdi * u[r];
+        const double tt = -sdi * y[r] / kj;
+        a.eig[eo.at(t, eo.gp() + r + j * NN)] = 0.5 * (tt + uu);
+        a.eig[eo.at(t, eo.gm() + r + j * NN)] = 0.5 * (tt - uu);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+
+  double zb[N], z0[N], z1[N];
+#pragma unroll
+  for (int r = 0; r < N; ++r) zb[r] = z0[r] = z1[r] = 0.0;
+  double bs[NN], bd[NN];
+  {
+    // Sums over l accumulated one l at a time (same per-element order),
+    // with P_l(-mu0) by its recurrence.
+    const double x = -mu0;
+    double pm2 = 0.0, pm1 = 0.0;
+#pragma unroll
+    for (int r = 0; r < NN; ++r) bs[r] = bd[r] = 0.0;
+#pragma unroll
+    for (int l = 0; l < N; ++l) {
+      const double p0l = (l == 0) ? 1.0 : (l == 1) ? x : ((2.0 * l - 1.0) * x * pm1 - (l - 1.0) * pm2) / l;
+      pm2 = pm1;
+      pm1 = p0l;
+      const double cl = chi_of(l);
+#pragma unroll
+      for (int r = 0; r < NN; ++r) {
+        const double term = cl * tb[to.P() + l * NN + r] * p0l;
+        if (l % 2 == 0) bs[r] += term;
+        else bd[r] += term;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      bs[r] = F0 * bs[r] / (2.0 * kPi);
+      bd[r] = F0 * bd[r] / (2.0 * kPi);
+    }
+  }
+  if (beam) {
+    double mub = mu0;
+    const double mu0sq = mu0 * mu0;
+#pragma unroll
+    for (int j = 0; j < NN; ++j)
+      if (fabs(1.0 - mu0sq * kv[j] * kv[j]) < 1000.0 * kEps) mub = 0.999 * mu0;
+    double p[NN], q[NN], w[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      const double cm = tb[to.cmu() + r];
+      p[r] = -bs[r] / cm;
+      q[r] = -bd[r] / cm;
+      w[r] = tb[to.sd() + r] * q[r];
+    }
+    lsolve(w);
+    vdv(w, lam);
+    ltsolve(w);
+    double rr[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      const double apbq = -tb[to.sdi() + r] * w[r];
+      rr[r] = (p[r] + mub * apbq) / mub;
+      rr[r] *= tb[to.sd() + r];
+    }
+    double dinv[NN];
+    const double im2 = 1.0 / (mub * mub);
+#pragma unroll
+    for (int j = 0; j < NN; ++j) dinv[j] = 1.0 / (lam[j] - im2);
+    ltmul(rr);
+    vdv(rr, dinv);
+    ltsolve(rr);
+    double d[NN], e[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      d[r] = tb[to.sdi() + r] * rr[r];
+      e[r] = tb[to.sd() + r] * d[r];
+    }
+    ltmul(e);
+    lmul(e);
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      const double ambd = -tb[to.sdi() + r] * e[r];
+      const double s2 = mub * (ambd - q[r]);
+      zb[NN + r] = 0.5 * (s2 + d[r]);
+      zb[NN - 1 - r] = 0.5 * (s2 - d[r]);
+    }
+  }
+
+  const double taut = a.lev_d[VD_TAU_S * TV + v];
+  const double taub = a.lev_d[VD_TAU_S * TV + v + 1];
+  double xr0 = 0.0, xr1 = 0.0;
+  if (a.flags & DISORT_F_THERMAL) {
+    const double pt = a.lev_d[VD_PLANCK * TV + v], pb = a.lev_d[VD_PLANCK * TV + v + 1];
+    const double dts = a.lay_d[LD_DTAU_S * TL + g];
+    if (dts > 1e-4) xr1 = (pb - pt) / dts;
+    xr0 = pt - xr1 * taut;
+    double base[NN];
+    {
+      double R[NT];
+      {
+#pragma unroll
+        for (int q = 0; q < NT; ++q) R[q] = 0.0;
+#pragma unroll
+        for (int l = 0; l < N; l += 2) {
+          const double cl = chi_of(l);
+          double ql[NN];
+#pragma unroll
+          for (int r = 0; r < NN; ++r) ql[r] = tb[to.Q() + l * NN + r];
+#pragma unroll
+          for (int r = 0; r < NN; ++r)
+#pragma unroll
+            for (int c = 0; c <= r; ++c) R[tri(r, c)] += cl * ql[r] * ql[c];
+          __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll
+        for (int r = 0; r < NN; ++r)
+#pragma unroll
+          for (int c = 0; c <= r; ++c) R[tri(r, c)] = ((r == c) ? 1.0 / tb[to.cmu() + r] : 0.0) - R[tri(r, c)];
+      }
+#pragma unroll
+      for (int c = 0; c < NN; ++c) {
+        double x = R[tri(c, c)];
+#pragma unroll
+        for (int k = 0; k < c; ++k) x -= R[tri(c, k)] * R[tri(c, k)];
+        const double rcc = sqrt(x > 0.0 ? x : 1.0);
+        R[tri(c, c)] = rcc;
+#pragma unroll
+        for (int r = c + 1; r < NN; ++r) {
+          double s2 = R[tri(r, c)];
+#pragma unroll
+          for (int k = 0; k < c; ++k) s2 -= R[tri(r, k)] * R[tri(c, k)];
+          R[tri(r, c)] = s2 / rcc;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < NN; ++r) {
+        double s2 = tb[to.sd() + r] * (2.0 / tb[to.cmu() + r]);
+#pragma unroll
+        for (int k = 0; k < r; ++k) s2 -= R[tri(r, k)] * base[k];
+        base[r] = s2 / R[tri(r, r)];
+      }
+#pragma unroll
+      for (int r = NN - 1; r >= 0; --r) {
+        double s2 = base[r];
+#pragma unroll
+        for (int k = r + 1; k < NN; ++k) s2 -= R[tri(k, r)] * base[k];
+        base[r] = s2 / R[tri(r, r)];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NN; ++r) base[r] *= tb[to.sdi() + r];
+    const double c1 = (1.0 - ws) * xr1, c0 = (1.0 - ws) * xr0;
+    double hh[NN], dd[NN];
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      hh[r] = 0.5 * c1 * base[r];
+      dd[r] = tb[to.sd() + r] * hh[r];
+    }
+    lsolve(dd);
+    ltsolve(dd);
+#pragma unroll
+    for (int r = 0; r < NN; ++r) {
+      const double d0 = 2.0 * tb[to.sdi() + r] * dd[r];
+      const double s0 = c0 * base[r];
+      z1[NN + r] = hh[r];
+      z1[NN - 1 - r] = hh[r];
+      z0[NN + r] = 0.5 * (s0 + d0);
+      z0[NN - 1 - r] = 0.5 * (s0 - d0);
+    }
+  }
+
+  const double btt = a.lay_d[LD_BT_TOP * TL + g], btb = a.lay_d[LD_BT_BOT * TL + g];
+  const double fact = beam ? exp(-taub / mu0) : 0.0;
+#pragma unroll
+  for (int r = 0; r < N; ++r) {
+    if ((r & 1) != h) continue;
+    a.eig[eo.at(t, eo.ztop() + r)] = zb[r] * btt + z0[r] + z1[r] * taut;
+    a.eig[eo.at(t, eo.zbot() + r)] = zb[r] * btb + z0[r] + z1[r] * taub;
+    a.eig[eo.at(t, eo.zfl() + r)] = zb[r] * fact + z0[r] + z1[r] * taub;
+  }
+  if (h == 0) a.eig[eo.at(t, eo.pls())] = xr0 + xr1 * taub;
+}
+
+// ===========================================================================
 // k_boundary: elimination of the boundary-value problem by LU with partial
 // pivoting, one layer panel at a time.
 //
@@ -755,12 +1271,15 @@ __global__ __launch_bounds__(64, 3) void k_boundary(FluxArgs a) {
 
 // Host-callable launchers, one set per stream count.
 template <int NN>
-static void launch_all(const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
+static void launch_all(const FluxArgs& a, hipStream_t s, hipEvent_t* ev, bool pair) {
   constexpr int N = 2 * NN;
   if (ev) (void)hipEventRecord(ev[0], s);
   hipLaunchKernelGGL(k_column<N>, dim3(a.nb), dim3(64), 0, s, a);
   if (ev) (void)hipEventRecord(ev[1], s);
-  hipLaunchKernelGGL(k_layer<NN>, dim3((unsigned)((a.n_tasks + 63) / 64)), dim3(64), 0, s, a);
+  if (pair)
+    hipLaunchKernelGGL(k_layer2<NN>, dim3((unsigned)((a.n_tasks + 31) / 32)), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_layer<NN>, dim3((unsigned)((a.n_tasks + 63) / 64)), dim3(64), 0, s, a);
   if (ev) (void)hipEventRecord(ev[2], s);
   hipLaunchKernelGGL(k_boundary<NN>, dim3((unsigned)((a.nb + RowCfg<NN>::G - 1) / RowCfg<NN>::G)), dim3(64), 0, s, a);
   if (ev) (void)hipEventRecord(ev[3], s);
@@ -768,15 +1287,15 @@ static void launch_all(const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
   if (ev) (void)hipEventRecord(ev[4], s);
 }
 
-bool launch_flux_bucket(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev) {
+bool launch_flux_bucket(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev, bool pair) {
   switch (nstr) {
-    case 4: launch_all<2>(a, s, ev); return true;
-    case 6: launch_all<3>(a, s, ev); return true;
-    case 8: launch_all<4>(a, s, ev); return true;
-    case 10: launch_all<5>(a, s, ev); return true;
-    case 12: launch_all<6>(a, s, ev); return true;
-    case 14: launch_all<7>(a, s, ev); return true;
-    case 16: launch_all<8>(a, s, ev); return true;
+    case 4: launch_all<2>(a, s, ev, pair); return true;
+    case 6: launch_all<3>(a, s, ev, pair); return true;
+    case 8: launch_all<4>(a, s, ev, pair); return true;
+    case 10: launch_all<5>(a, s, ev, pair); return true;
+    case 12: launch_all<6>(a, s, ev, pair); return true;
+    case 14: launch_all<7>(a, s, ev, pair); return true;
+    case 16: launch_all<8>(a, s, ev, pair); return true;
     default: return false;
   }
 }

@@ -26,7 +26,7 @@
 using dsa_host::fail;
 
 namespace dsa {
-bool launch_flux_bucket(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev);
+bool launch_flux_bucket(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev, bool pair);
 bool launch_flux_bucket_wg(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev);
 }  // namespace dsa
 
@@ -239,7 +239,11 @@ int disort_flux_execute_static(disort_ctx* ctx, const disort_flux_batch* b, int3
   // Kernel variant as a plan's default: register kernels up to 16 streams,
   // workgroup kernels at 32 and 64 streams and for the spherical beam.
   const bool lane = nstr <= 16 && !(b->flags & DISORT_F_SPHERICAL);
-  const bool ok = lane ? dsa::launch_flux_bucket(nstr, a, ctx->stream, nullptr)
+  // The plan's default kernels for this stream count (two lanes per layer
+  // in the eigen kernel at 6 to 12 streams), so that results equal
+  // disort_flux_execute's bit for bit.
+  const bool pair = nstr >= 6 && nstr <= 12;
+  const bool ok = lane ? dsa::launch_flux_bucket(nstr, a, ctx->stream, nullptr, pair)
                        : dsa::launch_flux_bucket_wg(nstr, a, ctx->stream, nullptr);
   if (!ok) return fail(ctx, DISORT_E_UNSUPPORTED, "no kernels for num_streams=" + std::to_string(nstr));
   HIPCHK(ctx, hipGetLastError());

@@ -173,15 +173,17 @@ int disort_flux_status(disort_ctx* ctx, disort_flux_plan* plan, int32_t* status_
 
 /*
  * Kernel variant of one stream-count bucket of a plan (no counterpart in the
- * reference: it selects between two gfx950 implementations of the same
+ * reference: it selects between gfx950 implementations of the same
  * solve).  DISORT_V_LANE: register kernels, one lane per layer and 16 lanes
- * per instance, N in {4,...,16}.  DISORT_V_WORKGROUP: LDS kernels, one
- * workgroup per layer and per instance, every N; the only variant for
- * N = 32, 64 and for the spherical beam.  Both give
- * results within the parity tolerance of each other; the default is LANE
- * where it exists.  The autotuner (disort_flux_autotune) picks by timing.
+ * per instance, N in {4,...,16}.  DISORT_V_LANE_PAIR: the same with two
+ * lanes per layer in the eigen-system kernel (half the registers, more
+ * wavefronts per SIMD).  DISORT_V_WORKGROUP: LDS kernels, one workgroup per
+ * layer and per instance, every N; the only variant for N = 32, 64 and for
+ * the spherical beam.  All give results within the parity tolerance of each
+ * other; the default is the register form where it exists.  The autotuner
+ * (disort_flux_autotune) picks by timing.
  */
-enum { DISORT_V_DEFAULT = 0, DISORT_V_LANE = 1, DISORT_V_WORKGROUP = 2 };
+enum { DISORT_V_DEFAULT = 0, DISORT_V_LANE = 1, DISORT_V_WORKGROUP = 2, DISORT_V_LANE_PAIR = 3 };
 int disort_flux_plan_set_variant(disort_ctx* ctx, disort_flux_plan* plan, int nstr, int variant);
 int disort_flux_plan_get_variant(disort_ctx* ctx, disort_flux_plan* plan, int nstr, int* variant);
 

@@ -54,17 +54,18 @@ def test_serial_concurrent_graph_agree(ctx, ragged):
 
 
 def test_autotune_keeps_results(ctx, ragged):
-    from disort_amd._capi import V_LANE, V_WORKGROUP, DeviceFluxBatch
+    from disort_amd._capi import V_LANE, V_LANE_PAIR, V_WORKGROUP, DeviceFluxBatch
 
     ref = oracle_out(ragged)
     dev = DeviceFluxBatch(ctx, ragged)
     rep = dev.autotune(reps=2)
     got = {b["nstr"]: b for b in rep["buckets"]}
     assert sorted(got) == [4, 8, 16, 32, 64]
-    assert got[4]["chosen"] == "lane" and got[64]["chosen"] == "workgroup"
+    assert got[4]["chosen"] in ("lane", "lane_pair", "workgroup") and got[64]["chosen"] == "workgroup"
+    codes = {"lane": V_LANE, "lane_pair": V_LANE_PAIR, "workgroup": V_WORKGROUP}
     for n in (8, 16):
-        assert got[n]["lane_ms"] > 0 and got[n]["workgroup_ms"] > 0
-        assert dev.get_variant(n) == (V_LANE if got[n]["chosen"] == "lane" else V_WORKGROUP)
+        assert got[n]["lane_ms"] > 0 and got[n]["lane_pair_ms"] > 0 and got[n]["workgroup_ms"] > 0
+        assert dev.get_variant(n) == codes[got[n]["chosen"]]
     assert rep["schedule"] in ("serial", "concurrent")
     out = run(dev, "execute_graph")
     assert_parity(out, ref, label="autotuned graph")

@@ -49,12 +49,12 @@ def test_wide_layer_cutoff(ctx, nstr):
 
 @pytest.mark.parametrize("nstr", [8, 16])
 def test_workgroup_variant_at_small_n(ctx, nstr):
-    from disort_amd._capi import V_LANE, V_WORKGROUP, DeviceFluxBatch
+    from disort_amd._capi import V_LANE, V_LANE_PAIR, V_WORKGROUP, DeviceFluxBatch
 
     b = W.random_flux_batch(10, 12, nstr, seed=600 + nstr)
     ref = oracle_out(b)
     dev = DeviceFluxBatch(ctx, b)
-    assert dev.get_variant(nstr) == V_LANE
+    assert dev.get_variant(nstr) == (V_LANE_PAIR if nstr <= 12 else V_LANE)
     dev.set_variant(nstr, V_WORKGROUP)
     assert dev.get_variant(nstr) == V_WORKGROUP
     dev.execute()
@@ -83,3 +83,25 @@ def test_c4_ragged_subset(ctx):
     b = W.c4(40, first=1000)
     assert set(int(x) for x in b.nstr) == {4, 8, 16, 32, 64}
     solve_and_check(ctx, b, label="c4 subset")
+
+
+@pytest.mark.parametrize("nstr", [4, 6, 8, 10, 12, 14, 16])
+@pytest.mark.parametrize("flags", [0, F_DELTA_M_PLUS, F_DIFFUSION_BC])
+def test_register_variants_agree_with_oracle(ctx, nstr, flags):
+    """Both register eigen kernels (one lane per layer, two lanes per layer)
+    against the oracle, with beam and thermal sources."""
+    from disort_amd._capi import V_LANE, V_LANE_PAIR, DeviceFluxBatch
+
+    P = nstr + 2 if flags & F_DELTA_M_PLUS else None
+    b = W.random_flux_batch(9, 11, nstr, seed=700 + nstr + flags, flags_extra=flags, pstride=P)
+    if flags & F_DELTA_M_PLUS:
+        b.pmom.reshape(-1, nstr + 2)[:, -1] *= 0.9
+    ref = oracle_out(b)
+    for v in (V_LANE, V_LANE_PAIR):
+        dev = DeviceFluxBatch(ctx, b)
+        dev.set_variant(nstr, v)
+        dev.execute()
+        out = dev.download()
+        assert (out.status == 0).all()
+        assert_parity(out.out, ref, label=f"variant {v} nstr={nstr} flags={flags}")
+        dev.close()

@@ -71,6 +71,8 @@ def lib():
         l.disort_flux_plan_get_variant.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
         l.disort_quadrature.argtypes = [C.c_int, vp, vp]
         l.disort_flux_plan_set_concurrent.argtypes = [vp, vp, C.c_int]
+        l.disort_flux_plan_set_streams.argtypes = [vp, vp, C.c_int]
+        l.disort_flux_plan_set_chunks.argtypes = [vp, vp, C.c_int, C.c_int]
         l.disort_flux_execute_graph.argtypes = [vp, vp, vp]
         l.disort_flux_autotune.argtypes = [vp, vp, vp, C.c_int]
         l.disort_flux_plan_report.argtypes = [vp]
@@ -245,6 +247,14 @@ class DeviceFluxBatch:
     def set_concurrent(self, on: bool):
         """Run stream-count buckets concurrently on side streams (or serially)."""
         _check(self.ctx.ptr, lib().disort_flux_plan_set_concurrent(self.ctx.ptr, self.plan, int(on)))
+
+    def set_streams(self, streams: int):
+        """1: buckets serially on the context stream; 2..4: concurrently on that many side streams."""
+        _check(self.ctx.ptr, lib().disort_flux_plan_set_streams(self.ctx.ptr, self.plan, int(streams)))
+
+    def set_chunks(self, nstr: int, chunks: int):
+        """Split the bucket of stream count `nstr` into `chunks` buckets (1..8)."""
+        _check(self.ctx.ptr, lib().disort_flux_plan_set_chunks(self.ctx.ptr, self.plan, int(nstr), int(chunks)))
 
     def autotune(self, reps: int = 3) -> dict:
         """Time every kernel variant per bucket and both schedules; keep the fastest."""

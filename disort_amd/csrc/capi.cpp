@@ -139,6 +139,12 @@ using namespace dsa_host;
 
 namespace {
 
+// Side streams of the concurrent schedule: a process gets four hardware
+// queues, and more concurrent streams than three measured slower.
+constexpr int kMaxSideStreams = 4;
+constexpr int kDefaultSideStreams = 3;
+constexpr int kMaxChunks = 8;
+
 bool lane_kernels(int n) { return n >= 4 && n <= 16 && n % 2 == 0; }
 bool wg_kernels(int n) { return (n >= 4 && n <= 16 && n % 2 == 0) || n == 32 || n == 64; }
 // Register kernels where they exist; the two-lanes-per-layer eigen kernel
@@ -200,17 +206,22 @@ std::string validate_instance(const disort_flux_batch* b, int64_t i) {
 // ---------------------------------------------------------------------------
 // Plans: bucket schedule computed from host-side shapes.
 // ---------------------------------------------------------------------------
+// A bucket: the instances of one stream count, or one chunk of them (a
+// contiguous share of the stream count's instance list).
 struct Bucket {
   int nstr = 0;
   int variant = 0;
+  int part = 0, parts = 1;  // chunk index and chunk count of this stream count
   double tuned_ms[4] = {-1.0, -1.0, -1.0, -1.0};  // per variant, from the autotuner
   int64_t nb = 0, n_tasks = 0, n_lv = 0;
+  double cost = 0.0;  // relative cost used to place it on a stream (model, or the tuned time)
   int32_t* d_inst = nullptr;
   int64_t* d_task0 = nullptr;
   int32_t* d_t_inst = nullptr;
   int32_t* d_t_lc = nullptr;
   int32_t* d_v_bk = nullptr;
   int32_t* d_v_lu = nullptr;
+  std::vector<void*> dev;  // device arrays above
 };
 
 struct disort_flux_plan {
@@ -225,9 +236,17 @@ struct disort_flux_plan {
   std::vector<Bucket> buckets;
   int64_t max_tasks = 0, max_eig = 0, max_panel = 0, max_cvec = 0;
   std::vector<void*> owned;
-  // Schedule: buckets one after another on the context stream (default), or
-  // concurrently on side streams with bucket-private scratch.
-  bool concurrent = false;
+  // Shapes kept to rebuild the buckets when the chunking changes.
+  std::map<int, std::vector<int32_t>> groups;  // stream count -> instances
+  std::vector<int32_t> nlyr_h;
+  std::map<int, int> chunks;      // stream count -> chunks (absent: 1)
+  std::map<int, int> variant_of;  // stream count -> kernel variant
+  std::map<int, double> tuned_of;  // stream count -> tuned time of the chosen variant (ms)
+  // Schedule: buckets one after another on the context stream (streams = 1),
+  // or concurrently on a pool of `streams` side streams with bucket-private
+  // scratch, each bucket on the least loaded stream, largest first.
+  int streams = 1;
+  bool concurrent = false;  // streams > 1
   std::vector<hipStream_t> side;
   std::vector<hipEvent_t> join;
   hipEvent_t fork = nullptr;
@@ -237,7 +256,7 @@ struct disort_flux_plan {
   hipGraphExec_t gexec = nullptr;
   disort_flux_batch gkey{};
   std::vector<const void*> gbufs;  // scratch and tables the graph was captured with
-  bool gconcurrent = false;
+  int gstreams = 1;
   std::vector<int> gvariant;
   std::string report;  // autotuner log (JSON)
 };
@@ -252,6 +271,18 @@ void release_graph(disort_flux_plan* plan) {
   if (plan->graph) (void)hipGraphDestroy(plan->graph);
   plan->gexec = nullptr;
   plan->graph = nullptr;
+}
+
+// Every resource a plan holds (the caller has synchronised its stream).
+void free_plan(disort_flux_plan* plan) {
+  release_graph(plan);
+  for (hipStream_t st : plan->side) (void)hipStreamDestroy(st);
+  for (hipEvent_t e : plan->join) (void)hipEventDestroy(e);
+  if (plan->fork) (void)hipEventDestroy(plan->fork);
+  for (Bucket& bk : plan->buckets)
+    for (void* q : bk.dev) (void)hipFree(q);
+  for (void* p : plan->owned) (void)hipFree(p);
+  delete plan;
 }
 
 // Batch-level arguments and every scratch buffer the plan needs, allocated
@@ -301,7 +332,7 @@ int prepare(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b,
   pa.eig.assign(nb, nullptr);
   pa.panel.assign(nb, nullptr);
   pa.cvec.assign(nb, nullptr);
-  if (!plan->concurrent) {
+  if (plan->streams <= 1) {
     if ((rc = get_buf(ctx, "eig", plan->max_eig * sizeof(double), &p))) return rc;
     std::fill(pa.eig.begin(), pa.eig.end(), (double*)p);
     if ((rc = get_buf(ctx, "panel", plan->max_panel * sizeof(double), &p))) return rc;
@@ -309,10 +340,10 @@ int prepare(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b,
     if ((rc = get_buf(ctx, "cvec", plan->max_cvec * sizeof(double), &p))) return rc;
     std::fill(pa.cvec.begin(), pa.cvec.end(), (double*)p);
   } else {
-    // Bucket-private scratch: buffers of stream count N are named by N.
+    // Bucket-private scratch, named by bucket index.
     for (size_t k = 0; k < nb; ++k) {
       const Bucket& bk = plan->buckets[k];
-      const std::string tag = "#" + std::to_string(bk.nstr);
+      const std::string tag = "#" + std::to_string(k);
       if ((rc = get_buf(ctx, ("eig" + tag).c_str(), (size_t)dsa::EigOff(bk.nstr).stride() * bk.n_tasks * 8, &p))) return rc;
       pa.eig[k] = (double*)p;
       if ((rc = get_buf(ctx, ("panel" + tag).c_str(), (size_t)dsa::panel_stride(bk.nstr) * bk.n_tasks * 8, &p))) return rc;
@@ -320,7 +351,7 @@ int prepare(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b,
       if ((rc = get_buf(ctx, ("cvec" + tag).c_str(), (size_t)bk.nstr * bk.n_tasks * 8, &p))) return rc;
       pa.cvec[k] = (double*)p;
     }
-    while (plan->side.size() < nb) {
+    while (plan->side.size() < (size_t)plan->streams) {
       hipStream_t st;
       HIPCHK(ctx, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
       plan->side.push_back(st);
@@ -371,7 +402,7 @@ int launch_plan(disort_ctx* ctx, disort_flux_plan* plan, const PlanArgs& pa, boo
   int rc;
   for (int k = 0; k < 4; ++k) ctx->stage_ms[k] = 0.0;
   ctx->n_stages = 4;
-  if (!plan->concurrent || timed) {
+  if (plan->streams <= 1 || timed) {
     for (size_t k = 0; k < plan->buckets.size(); ++k) {
       const Bucket& bk = plan->buckets[k];
       if ((rc = launch_bucket(ctx, bucket_args(ctx, pa, bk, k), bk, ctx->stream, timed ? ctx->ev : nullptr))) return rc;
@@ -387,19 +418,26 @@ int launch_plan(disort_ctx* ctx, disort_flux_plan* plan, const PlanArgs& pa, boo
     return DISORT_OK;
   }
   HIPCHK(ctx, hipEventRecord(plan->fork, ctx->stream));
-  // Largest buckets first, so that the long ones start earliest.
+  // Largest buckets first, each on the stream with the least work so far.
   std::vector<size_t> order(plan->buckets.size());
   for (size_t k = 0; k < order.size(); ++k) order[k] = k;
-  std::sort(order.begin(), order.end(), [&](size_t x, size_t y) {
-    const Bucket &bx = plan->buckets[x], &by = plan->buckets[y];
-    return (double)bx.n_tasks * bx.nstr * bx.nstr > (double)by.n_tasks * by.nstr * by.nstr;
-  });
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t x, size_t y) { return plan->buckets[x].cost > plan->buckets[y].cost; });
+  const int S = std::max(1, std::min<int>(plan->streams, (int)plan->side.size()));
+  std::vector<double> load(S, 0.0);
+  std::vector<bool> used(S, false);
   for (size_t k : order) {
-    hipStream_t st = plan->side[k];
-    HIPCHK(ctx, hipStreamWaitEvent(st, plan->fork, 0));
+    const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    hipStream_t st = plan->side[q];
+    if (!used[q]) HIPCHK(ctx, hipStreamWaitEvent(st, plan->fork, 0));
+    used[q] = true;
+    load[q] += plan->buckets[k].cost;
     if ((rc = launch_bucket(ctx, bucket_args(ctx, pa, plan->buckets[k], k), plan->buckets[k], st, nullptr))) return rc;
-    HIPCHK(ctx, hipEventRecord(plan->join[k], st));
-    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, plan->join[k], 0));
+  }
+  for (int q = 0; q < S; ++q) {
+    if (!used[q]) continue;
+    HIPCHK(ctx, hipEventRecord(plan->join[q], plan->side[q]));
+    HIPCHK(ctx, hipStreamWaitEvent(ctx->stream, plan->join[q], 0));
   }
   return DISORT_OK;
 }
@@ -423,6 +461,78 @@ bool same_batch(const disort_flux_batch& x, const disort_flux_batch& y) {
          x.lay_off == y.lay_off && x.mom_off == y.mom_off && x.dtau == y.dtau && x.ssa == y.ssa &&
          x.pmom == y.pmom && x.temper == y.temper && x.alt == y.alt && x.scal == y.scal && x.out == y.out &&
          x.total_levels == y.total_levels && x.status == y.status;
+}
+
+// (Re)build the buckets from the stream-count groups: one bucket per
+// stream count, or `chunks` contiguous chunks of its instance list.  Device
+// arrays of the previous buckets are freed; variants and tuned times are
+// kept per stream count.
+int build_buckets(disort_ctx* ctx, disort_flux_plan* plan) {
+  if (!plan->buckets.empty()) HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  release_graph(plan);
+  for (Bucket& bk : plan->buckets)
+    for (void* q : bk.dev) (void)hipFree(q);
+  plan->buckets.clear();
+  plan->max_tasks = plan->max_eig = plan->max_panel = plan->max_cvec = 0;
+  for (auto& kv : plan->groups) {
+    const int nstr = kv.first;
+    const std::vector<int32_t>& all = kv.second;
+    const auto ch = plan->chunks.find(nstr);
+    const int parts = (int)std::max<int64_t>(1, std::min<int64_t>(ch == plan->chunks.end() ? 1 : ch->second, (int64_t)all.size()));
+    int64_t total_tasks = 0;
+    for (int32_t i : all) total_tasks += plan->nlyr_h[i];
+    for (int part = 0; part < parts; ++part) {
+      Bucket bk;
+      bk.nstr = nstr;
+      bk.part = part;
+      bk.parts = parts;
+      bk.variant = plan->variant_of[nstr];
+      const size_t lo = all.size() * part / parts, hi = all.size() * (part + 1) / parts;
+      const std::vector<int32_t> inst(all.begin() + lo, all.begin() + hi);
+      bk.nb = (int64_t)inst.size();
+      std::vector<int64_t> task0(bk.nb);
+      std::vector<int32_t> t_inst, t_lc, v_bk, v_lu;
+      for (int64_t k = 0; k < bk.nb; ++k) {
+        const int32_t i = inst[k];
+        task0[k] = (int64_t)t_inst.size();
+        for (int lc = 0; lc < plan->nlyr_h[i]; ++lc) {
+          t_inst.push_back(i);
+          t_lc.push_back(lc);
+        }
+        for (int lu = 0; lu <= plan->nlyr_h[i]; ++lu) {
+          v_bk.push_back((int32_t)k);
+          v_lu.push_back(lu);
+        }
+      }
+      bk.n_tasks = (int64_t)t_inst.size();
+      bk.n_lv = (int64_t)v_bk.size();
+      const auto tu = plan->tuned_of.find(nstr);
+      bk.cost = (tu != plan->tuned_of.end() ? tu->second : 1e-9 * total_tasks * nstr * nstr * nstr) *
+                ((double)bk.n_tasks / (double)std::max<int64_t>(total_tasks, 1));
+      auto upload = [&](const void* src, size_t bytes, void** dst) -> int {
+        HIPCHK(ctx, hipMalloc(dst, std::max<size_t>(bytes, 8)));
+        bk.dev.push_back(*dst);
+        HIPCHK(ctx, hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return DISORT_OK;
+      };
+      int rc;
+      if ((rc = upload(inst.data(), inst.size() * 4, (void**)&bk.d_inst)) ||
+          (rc = upload(task0.data(), task0.size() * 8, (void**)&bk.d_task0)) ||
+          (rc = upload(t_inst.data(), t_inst.size() * 4, (void**)&bk.d_t_inst)) ||
+          (rc = upload(t_lc.data(), t_lc.size() * 4, (void**)&bk.d_t_lc)) ||
+          (rc = upload(v_bk.data(), v_bk.size() * 4, (void**)&bk.d_v_bk)) ||
+          (rc = upload(v_lu.data(), v_lu.size() * 4, (void**)&bk.d_v_lu))) {
+        for (void* q : bk.dev) (void)hipFree(q);
+        return rc;
+      }
+      plan->max_tasks = std::max(plan->max_tasks, bk.n_tasks);
+      plan->max_eig = std::max<int64_t>(plan->max_eig, (int64_t)dsa::EigOff(bk.nstr).stride() * bk.n_tasks);
+      plan->max_panel = std::max<int64_t>(plan->max_panel, (int64_t)dsa::panel_stride(bk.nstr) * bk.n_tasks);
+      plan->max_cvec = std::max<int64_t>(plan->max_cvec, (int64_t)bk.nstr * bk.n_tasks);
+      plan->buckets.push_back(std::move(bk));
+    }
+  }
+  return DISORT_OK;
 }
 
 }  // namespace
@@ -517,12 +627,7 @@ int disort_memcpy_d2h(disort_ctx* ctx, void* dst, const void* src, size_t bytes)
 int disort_flux_plan_destroy(disort_ctx* ctx, disort_flux_plan* plan) {
   if (!plan) return DISORT_OK;
   if (ctx) (void)hipStreamSynchronize(ctx->stream);
-  release_graph(plan);
-  for (hipStream_t st : plan->side) (void)hipStreamDestroy(st);
-  for (hipEvent_t e : plan->join) (void)hipEventDestroy(e);
-  if (plan->fork) (void)hipEventDestroy(plan->fork);
-  for (void* p : plan->owned) (void)hipFree(p);
-  delete plan;
+  free_plan(plan);
   return DISORT_OK;
 }
 
@@ -536,7 +641,9 @@ int disort_flux_plan_set_variant(disort_ctx* ctx, disort_flux_plan* plan, int ns
       if (!variant_ok(nstr, variant))
         return fail(ctx, DISORT_E_UNSUPPORTED,
                     "kernel variant " + std::to_string(variant) + " has no kernels for num_streams=" + std::to_string(nstr));
-      bk.variant = variant;
+      for (Bucket& b2 : plan->buckets)
+        if (b2.nstr == nstr) b2.variant = variant;
+      plan->variant_of[nstr] = variant;
       return DISORT_OK;
     }
   return fail(ctx, DISORT_E_INVALID, "the plan has no bucket with num_streams=" + std::to_string(nstr));
@@ -593,40 +700,14 @@ int disort_flux_plan_create(disort_ctx* ctx, int64_t n, const int32_t* nstr, con
   if ((rc = upload(nstr, n * sizeof(int32_t), (void**)&plan->d_nstr))) return rc;
   if ((rc = upload(nlyr, n * sizeof(int32_t), (void**)&plan->d_nlyr))) return rc;
   if ((rc = upload(pstride, n * sizeof(int32_t), (void**)&plan->d_pstride))) return rc;
-  for (auto& kv : groups) {
-    Bucket bk;
-    bk.nstr = kv.first;
+  plan->nlyr_h.assign(nlyr, nlyr + n);
+  for (auto& kv : groups)
     // The pseudo-spherical beam source is built by the workgroup kernels only.
-    bk.variant = (flags & DISORT_F_SPHERICAL) ? DISORT_V_WORKGROUP : default_variant(bk.nstr);
-    const auto& inst = kv.second;
-    bk.nb = (int64_t)inst.size();
-    std::vector<int64_t> task0(bk.nb);
-    std::vector<int32_t> t_inst, t_lc, v_bk, v_lu;
-    for (int64_t k = 0; k < bk.nb; ++k) {
-      const int32_t i = inst[k];
-      task0[k] = (int64_t)t_inst.size();
-      for (int lc = 0; lc < nlyr[i]; ++lc) {
-        t_inst.push_back(i);
-        t_lc.push_back(lc);
-      }
-      for (int lu = 0; lu <= nlyr[i]; ++lu) {
-        v_bk.push_back((int32_t)k);
-        v_lu.push_back(lu);
-      }
-    }
-    bk.n_tasks = (int64_t)t_inst.size();
-    bk.n_lv = (int64_t)v_bk.size();
-    if ((rc = upload(inst.data(), inst.size() * 4, (void**)&bk.d_inst))) return rc;
-    if ((rc = upload(task0.data(), task0.size() * 8, (void**)&bk.d_task0))) return rc;
-    if ((rc = upload(t_inst.data(), t_inst.size() * 4, (void**)&bk.d_t_inst))) return rc;
-    if ((rc = upload(t_lc.data(), t_lc.size() * 4, (void**)&bk.d_t_lc))) return rc;
-    if ((rc = upload(v_bk.data(), v_bk.size() * 4, (void**)&bk.d_v_bk))) return rc;
-    if ((rc = upload(v_lu.data(), v_lu.size() * 4, (void**)&bk.d_v_lu))) return rc;
-    plan->max_tasks = std::max(plan->max_tasks, bk.n_tasks);
-    plan->max_eig = std::max<int64_t>(plan->max_eig, (int64_t)dsa::EigOff(bk.nstr).stride() * bk.n_tasks);
-    plan->max_panel = std::max<int64_t>(plan->max_panel, (int64_t)dsa::panel_stride(bk.nstr) * bk.n_tasks);
-    plan->max_cvec = std::max<int64_t>(plan->max_cvec, (int64_t)bk.nstr * bk.n_tasks);
-    plan->buckets.push_back(bk);
+    plan->variant_of[kv.first] = (flags & DISORT_F_SPHERICAL) ? DISORT_V_WORKGROUP : default_variant(kv.first);
+  plan->groups = std::move(groups);
+  if ((rc = build_buckets(ctx, plan.get()))) {
+    free_plan(plan.release());
+    return rc;
   }
   *out = plan.release();
   return DISORT_OK;
@@ -641,10 +722,26 @@ int disort_flux_execute(disort_ctx* ctx, disort_flux_plan* plan, const disort_fl
   return launch_plan(ctx, plan, pa, ctx->timing);
 }
 
-int disort_flux_plan_set_concurrent(disort_ctx* ctx, disort_flux_plan* plan, int on) {
+int disort_flux_plan_set_streams(disort_ctx* ctx, disort_flux_plan* plan, int streams) {
   if (!plan) return fail(ctx, DISORT_E_INVALID, "null plan");
-  plan->concurrent = on != 0;
+  if (streams < 1 || streams > kMaxSideStreams)
+    return fail(ctx, DISORT_E_INVALID, "streams must be in 1.." + std::to_string(kMaxSideStreams));
+  plan->streams = streams;
+  plan->concurrent = streams > 1;
   return DISORT_OK;
+}
+
+int disort_flux_plan_set_concurrent(disort_ctx* ctx, disort_flux_plan* plan, int on) {
+  return disort_flux_plan_set_streams(ctx, plan, on ? kDefaultSideStreams : 1);
+}
+
+int disort_flux_plan_set_chunks(disort_ctx* ctx, disort_flux_plan* plan, int nstr, int chunks) {
+  if (!plan) return fail(ctx, DISORT_E_INVALID, "null plan");
+  if (chunks < 1 || chunks > kMaxChunks) return fail(ctx, DISORT_E_INVALID, "chunks must be in 1.." + std::to_string(kMaxChunks));
+  if (!plan->groups.count(nstr))
+    return fail(ctx, DISORT_E_INVALID, "the plan has no bucket with num_streams=" + std::to_string(nstr));
+  plan->chunks[nstr] = chunks;
+  return build_buckets(ctx, plan);
 }
 
 int disort_flux_execute_graph(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b) {
@@ -654,7 +751,7 @@ int disort_flux_execute_graph(disort_ctx* ctx, disort_flux_plan* plan, const dis
   std::vector<int> variants;
   for (const Bucket& bk : plan->buckets) variants.push_back(bk.variant);
   const std::vector<const void*> bufs = scratch_key(ctx, plan, pa);
-  if (!plan->gexec || !same_batch(plan->gkey, *b) || plan->gconcurrent != plan->concurrent || plan->gvariant != variants ||
+  if (!plan->gexec || !same_batch(plan->gkey, *b) || plan->gstreams != plan->streams || plan->gvariant != variants ||
       plan->gbufs != bufs) {
     release_graph(plan);
     HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -670,24 +767,30 @@ int disort_flux_execute_graph(disort_ctx* ctx, disort_flux_plan* plan, const dis
     HIPCHK(ctx, hipGraphInstantiate(&plan->gexec, g, nullptr, nullptr, 0));
     plan->gkey = *b;
     plan->gbufs = bufs;
-    plan->gconcurrent = plan->concurrent;
+    plan->gstreams = plan->streams;
     plan->gvariant = variants;
   }
   HIPCHK(ctx, hipGraphLaunch(plan->gexec, ctx->stream));
   return DISORT_OK;
 }
 
-// Autotuner: for every bucket, time each kernel variant that exists for its
-// stream count (median of `reps` runs after one warm-up, hipEvents on the
-// context stream), keep the fastest; then time the whole plan serial versus
-// concurrent and keep the faster schedule.  Runs the real solve on the
+// Autotuner.  (1) For every stream count, time each kernel variant that
+// exists for it on the whole bucket (median of `reps` runs after one
+// warm-up, hipEvents on the context stream) and keep the fastest.  (2) Time
+// the bucket schedule: serial on the context stream, or concurrent on a pool
+// of 2 or 3 side streams with the buckets larger than half a stream's share
+// split into 2 or 3 chunks (each chunk's own kernels; a chunk's tail overlaps
+// the other streams' work), and keep the fastest.  Runs the real solve on the
 // given device batch (its outputs are overwritten).
 int disort_flux_autotune(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* b, int reps) {
+  if (!ctx || !plan || !b) return fail(ctx, DISORT_E_INVALID, "null argument");
   if (reps < 1) reps = 3;
-  PlanArgs pa;
   int rc;
-  const bool was_concurrent = plan ? plan->concurrent : false;
-  if (plan) plan->concurrent = false;
+  plan->chunks.clear();
+  plan->streams = 1;
+  plan->concurrent = false;
+  if ((rc = build_buckets(ctx, plan))) return rc;
+  PlanArgs pa;
   if ((rc = prepare(ctx, plan, b, pa))) return rc;
   auto time_it = [&](auto&& enqueue, double* ms_out) -> int {
     std::vector<float> t;
@@ -706,6 +809,7 @@ int disort_flux_autotune(disort_ctx* ctx, disort_flux_plan* plan, const disort_f
     return DISORT_OK;
   };
   std::string js = "{\"buckets\": [";
+  double total_ms = 0.0;
   for (size_t k = 0; k < plan->buckets.size(); ++k) {
     Bucket& bk = plan->buckets[k];
     const FluxArgs a = bucket_args(ctx, pa, bk, k);
@@ -724,6 +828,9 @@ int disort_flux_autotune(disort_ctx* ctx, disort_flux_plan* plan, const disort_f
       }
     }
     bk.variant = best;
+    plan->variant_of[bk.nstr] = best;
+    plan->tuned_of[bk.nstr] = best_ms;
+    total_ms += best_ms;
     char buf[320];
     std::snprintf(buf, sizeof buf,
                   "%s{\"nstr\": %d, \"instances\": %lld, \"layers\": %lld, \"lane_ms\": %.6g, \"lane_pair_ms\": %.6g, "
@@ -732,17 +839,39 @@ int disort_flux_autotune(disort_ctx* ctx, disort_flux_plan* plan, const disort_f
                   bk.tuned_ms[DISORT_V_LANE_PAIR], bk.tuned_ms[DISORT_V_WORKGROUP], variant_name(best));
     js += buf;
   }
-  double serial_ms = 0.0, conc_ms = 0.0;
-  if ((rc = time_it([&]() { return launch_plan(ctx, plan, pa, false); }, &serial_ms))) return rc;
-  plan->concurrent = true;
-  PlanArgs pc;
-  if ((rc = prepare(ctx, plan, b, pc))) return rc;
-  if ((rc = time_it([&]() { return launch_plan(ctx, plan, pc, false); }, &conc_ms))) return rc;
-  plan->concurrent = plan->buckets.size() > 1 && conc_ms < serial_ms;
-  (void)was_concurrent;
-  char buf[256];
-  std::snprintf(buf, sizeof buf, "], \"serial_ms\": %.6g, \"concurrent_ms\": %.6g, \"schedule\": \"%s\"}", serial_ms,
-                conc_ms, plan->concurrent ? "concurrent" : "serial");
+  js += "], \"schedules\": [";
+  struct Cand {
+    int streams, chunks;
+    double ms;
+  };
+  std::vector<Cand> cands = {{1, 1, 0}, {2, 1, 0}, {2, 2, 0}, {3, 1, 0}, {3, 2, 0}, {3, 3, 0}};
+  auto apply = [&](int S, int c) -> int {
+    plan->streams = S;
+    plan->concurrent = S > 1;
+    plan->chunks.clear();
+    for (auto& kv : plan->tuned_of)
+      if (S > 1 && kv.second >= 0.5 * total_ms / S) plan->chunks[kv.first] = c;
+    return build_buckets(ctx, plan);
+  };
+  size_t best = 0;
+  for (size_t q = 0; q < cands.size(); ++q) {
+    Cand& cd = cands[q];
+    if ((rc = apply(cd.streams, cd.chunks))) return rc;
+    PlanArgs pc;
+    if ((rc = prepare(ctx, plan, b, pc))) return rc;
+    if ((rc = time_it([&]() { return launch_plan(ctx, plan, pc, false); }, &cd.ms))) return rc;
+    if (cd.ms < cands[best].ms) best = q;
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%s{\"streams\": %d, \"chunks\": %d, \"buckets\": %zu, \"ms\": %.6g}", q ? ", " : "",
+                  cd.streams, cd.chunks, plan->buckets.size(), cd.ms);
+    js += buf;
+  }
+  if ((rc = apply(cands[best].streams, cands[best].chunks))) return rc;
+  char buf[320];
+  std::snprintf(buf, sizeof buf,
+                "], \"serial_ms\": %.6g, \"best_ms\": %.6g, \"streams\": %d, \"chunks\": %d, \"schedule\": \"%s\"}",
+                cands[0].ms, cands[best].ms, cands[best].streams, cands[best].chunks,
+                plan->streams > 1 ? "concurrent" : "serial");
   js += buf;
   plan->report = js;
   return DISORT_OK;
@@ -774,10 +903,7 @@ int disort_flux_solve(disort_ctx* ctx, const disort_flux_batch* b, int memspace)
   disort_flux_plan* plan = nullptr;
   int rc = disort_flux_plan_create(ctx, b->n, b->nstr, b->nlyr, b->pstride, b->flags, &plan);
   if (rc) return rc;
-  std::unique_ptr<disort_flux_plan, void (*)(disort_flux_plan*)> guard(plan, [](disort_flux_plan* p) {
-    for (void* q : p->owned) (void)hipFree(q);
-    delete p;
-  });
+  std::unique_ptr<disort_flux_plan, void (*)(disort_flux_plan*)> guard(plan, free_plan);
   const int64_t TL = plan->total_layers, TV = plan->total_levels;
   int64_t mom_total = 0;
   for (int64_t i = 0; i < b->n; ++i) mom_total += (int64_t)b->nlyr[i] * b->pstride[i];
@@ -891,11 +1017,7 @@ int disort_flux_jacobian(disort_ctx* ctx, const disort_flux_batch* b, double* ja
     disort_flux_plan* plan = nullptr;
     int rc = disort_flux_plan_create(ctx, ne, vn.data(), vl.data(), vp.data(), flags, &plan);
     if (rc) return rc;
-    std::unique_ptr<disort_flux_plan, int (*)(disort_flux_plan*)> guard(plan, [](disort_flux_plan* p) {
-      for (void* q : p->owned) (void)hipFree(q);
-      delete p;
-      return 0;
-    });
+    std::unique_ptr<disort_flux_plan, void (*)(disort_flux_plan*)> guard(plan, free_plan);
     struct Up {
       const char* name;
       const void* src;

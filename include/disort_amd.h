@@ -189,12 +189,20 @@ int disort_flux_plan_get_variant(disort_ctx* ctx, disort_flux_plan* plan, int ns
 
 /*
  * Bucket schedule of a plan (no counterpart in the reference, whose band loop
- * is one OpenMP loop per stream count, src/python_bindings.cpp:170-190):
- * on = 0 runs the buckets one after another on the context stream; on = 1
- * forks them onto side streams (bucket-private scratch) and joins them back,
- * so that small buckets overlap large ones.
+ * is one OpenMP loop per stream count, src/python_bindings.cpp:170-190).
+ * set_streams: 1 runs the buckets one after another on the context stream;
+ * 2..4 forks them onto that many side streams (bucket-private scratch), each
+ * bucket on the stream with the least work so far, largest first, and joins
+ * them back, so that buckets overlap.  set_concurrent(on) is set_streams(3)
+ * or set_streams(1).  set_chunks splits the bucket of one stream count into
+ * 1..8 chunks of its instance list, each launched as its own bucket, so that
+ * a large bucket spreads over the side streams and the tail of one chunk's
+ * boundary sweep overlaps the next chunk's kernels.  Results do not depend
+ * on the schedule (every instance runs the same kernels).
  */
+int disort_flux_plan_set_streams(disort_ctx* ctx, disort_flux_plan* plan, int streams);
 int disort_flux_plan_set_concurrent(disort_ctx* ctx, disort_flux_plan* plan, int on);
+int disort_flux_plan_set_chunks(disort_ctx* ctx, disort_flux_plan* plan, int nstr, int chunks);
 
 /*
  * disort_flux_execute captured once into a hipGraph (all buckets, all four
@@ -206,8 +214,9 @@ int disort_flux_plan_set_concurrent(disort_ctx* ctx, disort_flux_plan* plan, int
 int disort_flux_execute_graph(disort_ctx* ctx, disort_flux_plan* plan, const disort_flux_batch* device_batch);
 
 /*
- * Autotuner: times every kernel variant of every bucket and the serial and
- * concurrent schedules on the given device batch (median of `reps` after a
+ * Autotuner: times every kernel variant of every stream count, then the
+ * bucket schedules (serial; 2 or 3 side streams with the large buckets in
+ * 1, 2 or 3 chunks) on the given device batch (median of `reps` after a
  * warm-up; the batch's outputs are overwritten), and keeps the fastest in
  * the plan.  disort_flux_plan_report returns the measurements as JSON.
  */

@@ -141,6 +141,7 @@ def _mlp_lib():
         for f in ("disort_mlp_set_params", "disort_mlp_get_params", "disort_mlp_get_grads"):
             getattr(l, f).argtypes = [vp, vp, vp]
         l.disort_mlp_forward.argtypes = [vp, vp, vp, C.c_int, vp]
+        l.disort_mlp_infer.argtypes = [vp, vp, vp, C.c_int, vp]
         l.disort_mlp_backward.argtypes = [vp, vp, vp]
         l.disort_mlp_adam.argtypes = [vp, vp, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int]
         l.disort_mlp_train_step.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
@@ -215,6 +216,10 @@ class MLP:
 
     def forward(self, x_ptr: int, batch: int, y_ptr: int):
         _ck(self.ctx, _mlp_lib().disort_mlp_forward(self.ctx.ptr, self.h, x_ptr, batch, y_ptr), self.h)
+
+    def infer(self, x_ptr: int, batch: int, y_ptr: int):
+        """Forward pass for serving (keeps nothing for a backward pass)."""
+        _ck(self.ctx, _mlp_lib().disort_mlp_infer(self.ctx.ptr, self.h, x_ptr, batch, y_ptr), self.h)
 
     def backward(self, dy_ptr: int):
         _ck(self.ctx, _mlp_lib().disort_mlp_backward(self.ctx.ptr, self.h, dy_ptr), self.h)
@@ -446,7 +451,7 @@ class GatedSurrogate:
         for m, y in zip(self.members, ys):
             for b0 in range(0, n_pad, m.max_batch):
                 bb = min(m.max_batch, n_pad - b0)
-                m.forward(x.ptr + b0 * WIDTHS[0] * 2, bb, y.ptr + b0 * WIDTHS[-1] * 4)
+                m.infer(x.ptr + b0 * WIDTHS[0] * 2, bb, y.ptr + b0 * WIDTHS[-1] * 4)
         cl, cf = METRICS[self.metric]
         _ck(ctx, l.disort_surrogate_decode(ctx.ptr, n, NLYR, WIDTHS[-1], ys[0].ptr, ys[1].ptr if len(ys) > 1 else None,
                                            self.dn["t_shift"].ptr, self.dn["t_scale"].ptr, C.byref(dc.cstruct),

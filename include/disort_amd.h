@@ -407,8 +407,11 @@ int64_t disort_mlp_param_count(const disort_mlp* mlp);
 int disort_mlp_set_params(disort_ctx* ctx, disort_mlp* mlp, const float* host_params); /* also resets Adam state */
 int disort_mlp_get_params(disort_ctx* ctx, disort_mlp* mlp, float* host_params);
 int disort_mlp_get_grads(disort_ctx* ctx, disort_mlp* mlp, float* host_grads);
-/* y [B x widths[L]] FP32 = network(x [B x widths[0]] BF16 bits); B a multiple of 64. */
+/* y [B x widths[L]] FP32 = network(x [B x widths[0]] BF16 bits); B a multiple of 64.
+ * disort_mlp_forward keeps what disort_mlp_backward needs; disort_mlp_infer
+ * (serving) keeps nothing and writes the same y. */
 int disort_mlp_forward(disort_ctx* ctx, disort_mlp* mlp, const uint16_t* x, int batch, float* y);
+int disort_mlp_infer(disort_ctx* ctx, disort_mlp* mlp, const uint16_t* x, int batch, float* y);
 /* Parameter gradients for output gradient dy [B x widths[L]] (BF16 bits) of the last forward. */
 int disort_mlp_backward(disort_ctx* ctx, disort_mlp* mlp, const uint16_t* dy);
 int disort_mlp_adam(disort_ctx* ctx, disort_mlp* mlp, float lr, float beta1, float beta2, float eps, int step);

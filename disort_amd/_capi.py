@@ -70,6 +70,7 @@ def lib():
         l.disort_flux_plan_set_variant.argtypes = [vp, vp, C.c_int, C.c_int]
         l.disort_flux_plan_get_variant.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
         l.disort_quadrature.argtypes = [C.c_int, vp, vp]
+        l.disort_planck_band.argtypes = [C.c_double, C.c_double, C.c_int64, vp, vp]
         l.disort_flux_plan_set_concurrent.argtypes = [vp, vp, C.c_int]
         l.disort_flux_plan_set_streams.argtypes = [vp, vp, C.c_int]
         l.disort_flux_plan_set_chunks.argtypes = [vp, vp, C.c_int, C.c_int]
@@ -300,3 +301,13 @@ class DeviceFluxBatch:
 
 def version() -> str:
     return lib().disort_version().decode()
+
+
+def planck_band(lo: float, hi: float, t) -> np.ndarray:
+    """Band-integrated Planck function at host temperatures t (the kernels' arithmetic)."""
+    t = np.ascontiguousarray(t, dtype=np.float64).reshape(-1)
+    b = np.empty_like(t)
+    rc = lib().disort_planck_band(float(lo), float(hi), t.size, t.ctypes.data, b.ctypes.data)
+    if rc:
+        raise DisortInvalid(rc, "disort_planck_band: invalid band or temperature")
+    return b

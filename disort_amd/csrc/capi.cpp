@@ -550,6 +550,15 @@ int disort_quadrature(int nstr, double* cmu, double* cwt) {
   return DISORT_OK;
 }
 
+int disort_planck_band(double lo, double hi, int64_t n, const double* t, double* b) {
+  if (!(lo >= 0.0) || !(hi >= lo) || n < 0 || (n > 0 && (!t || !b))) return DISORT_E_INVALID;
+  for (int64_t k = 0; k < n; ++k) {
+    if (!(t[k] >= 0.0)) return DISORT_E_INVALID;
+    b[k] = dsa::planck_band(lo, hi, t[k]);
+  }
+  return DISORT_OK;
+}
+
 int disort_ctx_create(int device, disort_ctx** out) {
   if (!out) return DISORT_E_INVALID;
   auto ctx = std::make_unique<disort_ctx>();

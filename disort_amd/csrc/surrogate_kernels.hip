@@ -6,12 +6,19 @@
 //
 // Columns are uniform: n columns of nlyr layers, moment stride pstride,
 // arrays laid out as disort_flux_batch with lay_off[i] = i*nlyr.
-// Features of column i (4*nlyr + 2 of them, then zero padding to w0):
+// Features of column i (4*nlyr + 2 of them, with w0 >= 7*nlyr + 5 another
+// 3*(nlyr+1) level features, then zero padding to w0):
 //   [l]          log10(delta_tau_l)
 //   [nlyr + l]   single_scat_albedo_l
 //   [2nlyr + l]  g_l = pmom_l[1]
 //   [3nlyr + l]  layer-mean temperature (T_l + T_{l+1}) / 2
 //   [4nlyr]      mu0,  [4nlyr + 1] surface albedo
+//   [4nlyr + 2 + v]              log10(tau_v + 1e-3), tau_v the optical depth of level v
+//   [4nlyr + 2 + (nlyr+1) + v]   exp(-tau_v / mu0), the direct-beam transmission
+//   [4nlyr + 2 + 2(nlyr+1) + v]  log10(sum_{l<v} (1 - omega_l) dtau_l + 1e-3)
+// and, with w0 >= 8*nlyr + 7, the thermal source in units of the beam:
+//   [4nlyr + 2 + 3(nlyr+1) + v]  B(T_v) / (mu0 F0), B the band Planck function
+//   [4nlyr + 2 + 4(nlyr+1)]      (1 - albedo) B(T_bottom) / (mu0 F0)
 // Targets (3*nlev, then zero padding to wL), in units of the incident beam
 // mu0*F0 at the nlev = nlyr+1 levels:
 //   [v]          (flux_down - flux_up) / (mu0 F0)   diffuse net flux
@@ -24,31 +31,87 @@
 #include <cstdint>
 
 #include "../../include/disort_amd.h"
+#include "planck.hpp"
 
 namespace {
 
 unsigned blocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
-__global__ void k_features(int64_t n, int64_t n_pad, int L, int P, const double* dtau, const double* ssa,
-                           const double* pmom, const double* temper, const double* scal, const float* shift,
-                           const float* scale, const float* lo, const float* hi, int w0, __bf16* x, int32_t* ood) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ double beam_unit(const double* scal, int64_t i) {
+  const double u = scal[i * DISORT_NSCAL + DISORT_S_BEAM_MU] * scal[i * DISORT_NSCAL + DISORT_S_BEAM_FLUX];
+  return u > 1e-300 ? u : 1.0;
+}
+
+// One wavefront per column (four per workgroup): lane v first forms the
+// values of level v -- optical depth and absorption optical depth summed
+// from the top in layer order, beam transmission, band Planck value -- into
+// LDS, then the lanes write the column's features, 64 at a time.
+__global__ __launch_bounds__(256) void k_features(int64_t n, int64_t n_pad, int L, int P, const double* dtau,
+                                                  const double* ssa, const double* pmom, const double* temper,
+                                                  const double* scal, const float* shift, const float* scale,
+                                                  const float* lo, const float* hi, int w0, __bf16* x, int32_t* ood) {
+  __shared__ double lev[4][4][64];  // [column][tau, absorption tau, transmission, B/(mu0 F0)][level]
+  __shared__ double lay[4][2][64];  // [column][dtau, (1 - omega) dtau][layer]
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + wv;
   if (i >= n_pad) return;
   __bf16* xr = x + i * w0;
   if (i >= n) {
-    for (int k = 0; k < w0; ++k) xr[k] = (__bf16)0.0f;
+    for (int k = lane; k < w0; k += 64) xr[k] = (__bf16)0.0f;
     return;
   }
-  const int nf = 4 * L + 2;
+  const int base = 4 * L + 2, nlev = L + 1;
+  const int ext = (w0 >= base + 4 * nlev + 1) ? 4 * nlev + 1 : (w0 >= base + 3 * nlev) ? 3 * nlev : 0;
+  const int nf = base + ext;
+  const double* sc = scal + i * DISORT_NSCAL;
+  const double mu0 = sc[DISORT_S_BEAM_MU];
+  const double u = beam_unit(scal, i);
+  double (*lv)[64] = lev[wv];
+  double (*ly)[64] = lay[wv];
+  if (lane < L) {
+    const double dt = dtau[i * L + lane];
+    ly[0][lane] = dt;
+    ly[1][lane] = (1.0 - ssa[i * L + lane]) * dt;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (int v = lane; v <= nlev && ext > 0; v += 64) {  // v == nlev: the surface (Planck term only)
+    double tau = 0.0, tab = 0.0;
+    for (int l = 0; l < v && l < L; ++l) {
+      tau += ly[0][l];
+      tab += ly[1][l];
+    }
+    if (v < nlev) {
+      lv[0][v] = log10(tau + 1e-3);
+      lv[1][v] = log10(tab + 1e-3);
+      lv[2][v] = exp(-tau / mu0);
+    }
+    if (ext > 3 * nlev) {
+      const double t = (v < nlev) ? temper[i * nlev + v] : sc[DISORT_S_TEMP_BOT];
+      const double bb = dsa::planck_band(sc[DISORT_S_WN_LO], sc[DISORT_S_WN_HI], t);
+      lv[3][v] = (v < nlev ? bb : (1.0 - sc[DISORT_S_ALBEDO]) * bb) / u;
+    }
+  }
+  // Only this wavefront reads its LDS rows: order the LDS writes before the
+  // reads within the wavefront.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   int bad = 0;
-  for (int k = 0; k < w0; ++k) {
+  for (int k = lane; k < w0; k += 64) {
     double f = 0.0;
     if (k < L) f = log10(dtau[i * L + k]);
     else if (k < 2 * L) f = ssa[i * L + (k - L)];
     else if (k < 3 * L) f = pmom[(i * L + (k - 2 * L)) * P + 1];
-    else if (k < 4 * L) f = 0.5 * (temper[i * (L + 1) + (k - 3 * L)] + temper[i * (L + 1) + (k - 3 * L) + 1]);
-    else if (k == 4 * L) f = scal[i * DISORT_NSCAL + DISORT_S_BEAM_MU];
-    else if (k == 4 * L + 1) f = scal[i * DISORT_NSCAL + DISORT_S_ALBEDO];
+    else if (k < 4 * L) f = 0.5 * (temper[i * nlev + (k - 3 * L)] + temper[i * nlev + (k - 3 * L) + 1]);
+    else if (k == 4 * L) f = mu0;
+    else if (k == 4 * L + 1) f = sc[DISORT_S_ALBEDO];
+    else if (k < nf) {
+      const int blk = (k - base) / nlev, v = (k - base) % nlev;
+      // blocks: optical depth, transmission, absorption depth, Planck (the last has nlev + 1 entries)
+      f = (blk == 0) ? lv[0][v] : (blk == 1) ? lv[2][v] : (blk == 2) ? lv[1][v] : lv[3][k - base - 3 * nlev];
+    }
     if (k < nf) {
       const float ff = (float)f;
       bad |= !(ff >= lo[k] && ff <= hi[k]);
@@ -57,12 +120,8 @@ __global__ void k_features(int64_t n, int64_t n_pad, int L, int P, const double*
       xr[k] = (__bf16)0.0f;
     }
   }
-  if (ood) ood[i] = bad;
-}
-
-__device__ __forceinline__ double beam_unit(const double* scal, int64_t i) {
-  const double u = scal[i * DISORT_NSCAL + DISORT_S_BEAM_MU] * scal[i * DISORT_NSCAL + DISORT_S_BEAM_FLUX];
-  return u > 1e-300 ? u : 1.0;
+  const unsigned long long any_bad = __ballot(bad);
+  if (ood && lane == 0) ood[i] = any_bad ? 1 : 0;
 }
 
 __global__ void k_targets(int64_t n, int64_t n_pad, int nlev, const double* out, int64_t tv, const double* scal,
@@ -91,18 +150,29 @@ __global__ void k_targets(int64_t n, int64_t n_pad, int nlev, const double* out,
 // spread = max over levels of |Fnet_1 - Fnet_2| / max(c_local |Fnet|, c_floor mu0 F0)
 // (c_local = 1, c_floor = 1e-3: relative to the local net flux with a floor;
 // c_local = 0, c_floor = 1: relative to the incident beam).
-__global__ void k_decode(int64_t n, int L, int wL, const float* y1, const float* y2, const float* shift,
-                         const float* scale, const double* dtau, const double* scal, double c_local, double c_floor,
-                         double* flux, float* spread) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_decode(int64_t n, int L, int wL, const float* y1, const float* y2,
+                                                const float* shift, const float* scale, const double* dtau,
+                                                const double* scal, double c_local, double c_floor, double* flux,
+                                                float* spread) {
+  // One wavefront per column, lane = level (nlev <= 64).
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= n) return;
   const int nlev = L + 1;
   const double mu0 = scal[i * DISORT_NSCAL + DISORT_S_BEAM_MU], F0 = scal[i * DISORT_NSCAL + DISORT_S_BEAM_FLUX];
   const double u = beam_unit(scal, i);
   const double floor = c_floor * u;
-  double tau = 0.0, worst = 0.0;
-  for (int lv = 0; lv < nlev; ++lv) {
-    if (lv > 0) tau += dtau[i * L + lv - 1];
+  double worst = 0.0;
+  __shared__ double lay[4][64];
+  double* ly = lay[threadIdx.x >> 6];
+  if (lane < L) ly[lane] = dtau[i * L + lane];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int lv = lane;
+  if (lv < nlev) {
+    double tau = 0.0;
+    for (int l = 0; l < lv; ++l) tau += ly[l];  // the solver's summation order
     const double dir = (F0 > 0.0) ? mu0 * F0 * exp(-tau / mu0) : 0.0;
     const int ka = lv, ks = nlev + lv;
     const double a1 = ((double)y1[i * wL + ka] / scale[ka] + shift[ka]) * u;  // diffuse net
@@ -115,32 +185,16 @@ __global__ void k_decode(int64_t n, int L, int wL, const float* y1, const float*
       sm = 0.5 * (s1 + s2);
       const double net = c_local * fabs(dir + a);
       const double den = net > floor ? net : floor;
-      const double sp = fabs(a1 - a2) / den;
-      worst = sp > worst ? sp : worst;
+      worst = fabs(a1 - a2) / den;
+      if (!(worst == worst)) worst = __builtin_inf();  // a non-finite output fails the gate
     }
     double* fr = flux + i * 3 * nlev;
     fr[lv] = 0.5 * (sm - a);
     fr[nlev + lv] = 0.5 * (sm + a);
     fr[2 * nlev + lv] = dir;
   }
-  spread[i] = (float)worst;
-}
-
-// dst column k <- src column idx[k] (uniform layout).
-__global__ void k_gather(int64_t m, const int32_t* idx, int L, int P, const double* dtau, const double* ssa,
-                         const double* pmom, const double* temper, const double* scal, double* odtau, double* ossa,
-                         double* opmom, double* otemper, double* oscal) {
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= m) return;
-  const int64_t i = idx[k];
-  for (int l = 0; l < L; ++l) {
-    odtau[k * L + l] = dtau[i * L + l];
-    ossa[k * L + l] = ssa[i * L + l];
-  }
-  for (int q = 0; q < L * P; ++q) opmom[k * L * P + q] = pmom[i * L * P + q];
-  if (temper)
-    for (int l = 0; l <= L; ++l) otemper[k * (L + 1) + l] = temper[i * (L + 1) + l];
-  for (int q = 0; q < DISORT_NSCAL; ++q) oscal[k * DISORT_NSCAL + q] = scal[i * DISORT_NSCAL + q];
+  for (int off = 32; off >= 1; off >>= 1) worst = fmax(worst, __shfl_xor(worst, off));
+  if (lane == 0) spread[i] = (float)worst;
 }
 
 // flux rows idx[k] <- plain-solver outputs of sub-batch column k.
@@ -155,14 +209,93 @@ __global__ void k_scatter(int64_t m, const int32_t* idx, int nlev, const double*
   fr[2 * nlev + lv] = out[DISORT_O_FLUX_DIRECT_BEAM * tv + k * nlev + lv];
 }
 
-// Rows of a row-major matrix (elem_bytes 2 or 4): dst[k] <- src[idx[k]].
+// Rows of a row-major matrix (elem_bytes 2, 4 or 8): dst[k] <- src[idx[k]];
+// consecutive threads copy consecutive elements of a row (coalesced).
 __global__ void k_rows(int64_t m, const int32_t* idx, int64_t row_elems, int elem_bytes, const void* src, void* dst) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m * row_elems) return;
   const int64_t k = e / row_elems, c = e % row_elems;
   const int64_t s = (int64_t)idx[k] * row_elems + c;
   if (elem_bytes == 2) ((uint16_t*)dst)[e] = ((const uint16_t*)src)[s];
-  else ((uint32_t*)dst)[e] = ((const uint32_t*)src)[s];
+  else if (elem_bytes == 4) ((uint32_t*)dst)[e] = ((const uint32_t*)src)[s];
+  else ((uint64_t*)dst)[e] = ((const uint64_t*)src)[s];
+}
+
+// Counter-based uniform draw in [0, 1) for column i of a serve call
+// (SplitMix64 finaliser of seed + i): the sampled re-solve needs no host RNG.
+__device__ __forceinline__ double unit_draw(uint64_t seed, int64_t i) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ull * (uint64_t)(i + 1);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void k_zero_i32(int32_t* p, int n) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// Gate decision per column: flags bit 0 routed to plain DISORT (out of
+// distribution, spread above tau or not a number, sampled, or route_all), bit 1
+// sampled, bit 2 passes the gate.  Routed indices are appended to idx (one
+// atomic per wavefront).
+__global__ void k_route(int64_t n, const float* spread, const int32_t* ood, float tau, double frac, uint64_t seed,
+                        int route_all, int32_t* flags, int32_t* idx, int32_t* count) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool in = i < n;
+  bool routed = false;
+  if (in) {
+    const bool sampled = unit_draw(seed, i) < frac;
+    const bool pass = !(ood && ood[i] != 0) && spread[i] <= tau;
+    routed = route_all || sampled || !pass;
+    flags[i] = (routed ? 1 : 0) | (sampled ? 2 : 0) | (pass ? 4 : 0);
+  }
+  const unsigned long long bal = __ballot(routed);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0 && bal) base = atomicAdd(count, __popcll(bal));
+  base = __shfl(base, 0);
+  if (routed) idx[base + __popcll(bal & ((1ull << lane) - 1ull))] = (int32_t)i;
+}
+
+// idx[m .. mpad) = idx[0]: padding of a routed list to a cached sub-batch
+// size (the copies solve the same column again and scatter the same values).
+__global__ void k_pad_idx(int64_t m, int64_t mpad, int32_t* idx) {
+  const int64_t k = m + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < mpad) idx[k] = idx[0];
+}
+
+// Online error monitor: for every routed column that was sampled although
+// it passes the gate, the surrogate's net-flux error against the plain solve (max over levels of
+// |dF_net| / max(c_local |F_net|, c_floor mu0 F0)); counters[0] += sampled
+// columns, counters[1] += those above tol.
+__global__ void k_monitor(int64_t m, const int32_t* idx, const int32_t* flags, int nlev, const double* out, int64_t tv,
+                          const double* pred, const double* scal, double c_local, double c_floor, double tol,
+                          int32_t* counters) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool smp = false, bad = false;
+  if (k < m) {
+    const int64_t i = idx[k];
+    smp = (flags[i] & 6) == 6;
+    if (smp) {
+      const double u = beam_unit(scal, i), floor = c_floor * u;
+      const double* fp = pred + i * 3 * nlev;
+      double worst = 0.0;
+      for (int lv = 0; lv < nlev; ++lv) {
+        const double rnet = out[DISORT_O_FLUX_DIRECT_BEAM * tv + k * nlev + lv] +
+                            out[DISORT_O_FLUX_DOWN * tv + k * nlev + lv] - out[DISORT_O_FLUX_UP * tv + k * nlev + lv];
+        const double pnet = fp[2 * nlev + lv] + fp[nlev + lv] - fp[lv];
+        const double den = fmax(c_local * fabs(rnet), floor);
+        worst = fmax(worst, fabs(pnet - rnet) / den);
+      }
+      bad = !(worst <= tol);
+    }
+  }
+  const unsigned long long bs = __ballot(smp), bb = __ballot(bad);
+  if ((threadIdx.x & 63) == 0) {
+    if (bs) atomicAdd(&counters[0], __popcll(bs));
+    if (bb) atomicAdd(&counters[1], __popcll(bb));
+  }
 }
 
 }  // namespace
@@ -179,7 +312,8 @@ int disort_surrogate_features(disort_ctx* ctx, int64_t n, int64_t n_pad, int nly
                               const float* hi, int w0, uint16_t* x, int32_t* ood) {
   if (!ctx || !b || n < 0 || n_pad < n || nlyr <= 0 || pstride < 2 || w0 < 4 * nlyr + 2) return DISORT_E_INVALID;
   if (!b->temper) return DISORT_E_INVALID;
-  hipLaunchKernelGGL(k_features, dim3(blocks(n_pad)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n, n_pad, nlyr,
+  if (nlyr + 2 > 64) return DISORT_E_INVALID;  // one lane per level
+  hipLaunchKernelGGL(k_features, dim3(blocks(n_pad, 4)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n, n_pad, nlyr,
                      pstride, b->dtau, b->ssa, b->pmom, b->temper, b->scal, shift, scale, lo, hi, w0, (__bf16*)x, ood);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;
@@ -198,7 +332,8 @@ int disort_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const 
                             const float* shift, const float* scale, const disort_flux_batch* b, double c_local,
                             double c_floor, double* flux, float* spread) {
   if (!ctx || !b || !y1 || !flux || !spread || wL < 3 * (nlyr + 1) || !(c_floor > 0.0)) return DISORT_E_INVALID;
-  hipLaunchKernelGGL(k_decode, dim3(blocks(n)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n, nlyr, wL, y1, y2,
+  if (nlyr + 1 > 64) return DISORT_E_INVALID;  // one lane per level
+  hipLaunchKernelGGL(k_decode, dim3(blocks(n, 4)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n, nlyr, wL, y1, y2,
                      shift, scale, b->dtau, b->scal, c_local, c_floor, flux, spread);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;
@@ -208,9 +343,15 @@ int disort_surrogate_gather(disort_ctx* ctx, int64_t m, const int32_t* idx, int 
                             const disort_flux_batch* src, disort_flux_batch* dst) {
   if (!ctx || !idx || !src || !dst || m < 0) return DISORT_E_INVALID;
   if (m == 0) return DISORT_OK;
-  hipLaunchKernelGGL(k_gather, dim3(blocks(m, 64)), dim3(64), 0, (hipStream_t)disort_ctx_stream(ctx), m, idx, nlyr,
-                     pstride, src->dtau, src->ssa, src->pmom, src->temper, src->scal, (double*)dst->dtau,
-                     (double*)dst->ssa, (double*)dst->pmom, (double*)dst->temper, (double*)dst->scal);
+  hipStream_t st = (hipStream_t)disort_ctx_stream(ctx);
+  auto rows = [&](int64_t elems, const void* s, void* d) {
+    hipLaunchKernelGGL(k_rows, dim3(blocks(m * elems)), dim3(256), 0, st, m, idx, elems, 8, s, d);
+  };
+  rows(nlyr, src->dtau, (void*)dst->dtau);
+  rows(nlyr, src->ssa, (void*)dst->ssa);
+  rows((int64_t)nlyr * pstride, src->pmom, (void*)dst->pmom);
+  if (src->temper) rows(nlyr + 1, src->temper, (void*)dst->temper);
+  rows(DISORT_NSCAL, src->scal, (void*)dst->scal);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;
 }
@@ -225,9 +366,44 @@ int disort_surrogate_scatter(disort_ctx* ctx, int64_t m, const int32_t* idx, int
   return DISORT_OK;
 }
 
+int disort_surrogate_route(disort_ctx* ctx, int64_t n, const float* spread, const int32_t* ood, float tau,
+                           double sample_frac, uint64_t seed, int route_all, int32_t* flags, int32_t* idx,
+                           int32_t* count) {
+  if (!ctx || !spread || !flags || !idx || !count || n < 0) return DISORT_E_INVALID;
+  hipStream_t st = (hipStream_t)disort_ctx_stream(ctx);
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, count, 1);
+  if (n > 0)
+    hipLaunchKernelGGL(k_route, dim3(blocks(n)), dim3(256), 0, st, n, spread, ood, tau, sample_frac, seed, route_all,
+                       flags, idx, count);
+  SURR_LAUNCH_CHECK();
+  return DISORT_OK;
+}
+
+int disort_surrogate_pad_index(disort_ctx* ctx, int64_t m, int64_t mpad, int32_t* idx) {
+  if (!ctx || !idx || m <= 0 || mpad < m) return DISORT_E_INVALID;
+  if (mpad > m)
+    hipLaunchKernelGGL(k_pad_idx, dim3(blocks(mpad - m)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), m, mpad, idx);
+  SURR_LAUNCH_CHECK();
+  return DISORT_OK;
+}
+
+int disort_surrogate_monitor(disort_ctx* ctx, int64_t m, const int32_t* idx, const int32_t* flags, int nlev,
+                             const double* out, int64_t total_levels, const double* pred, const disort_flux_batch* b,
+                             double c_local, double c_floor, double tol, int32_t* counters) {
+  if (!ctx || !idx || !flags || !out || !pred || !b || !counters || m < 0 || !(c_floor > 0.0)) return DISORT_E_INVALID;
+  hipStream_t st = (hipStream_t)disort_ctx_stream(ctx);
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, counters, 2);
+  if (m > 0)
+    hipLaunchKernelGGL(k_monitor, dim3(blocks(m)), dim3(256), 0, st, m, idx, flags, nlev, out, total_levels, pred, b->scal,
+                       c_local, c_floor, tol, counters);
+  SURR_LAUNCH_CHECK();
+  return DISORT_OK;
+}
+
 int disort_rows_gather(disort_ctx* ctx, int64_t m, const int32_t* idx, int64_t row_elems, int elem_bytes,
                        const void* src, void* dst) {
-  if (!ctx || !idx || !src || !dst || m < 0 || (elem_bytes != 2 && elem_bytes != 4)) return DISORT_E_INVALID;
+  if (!ctx || !idx || !src || !dst || m < 0 || (elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8))
+    return DISORT_E_INVALID;
   if (m == 0) return DISORT_OK;
   hipLaunchKernelGGL(k_rows, dim3(blocks(m * row_elems)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), m, idx,
                      row_elems, elem_bytes, src, dst);

@@ -40,13 +40,17 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _capi
-from .batch import F_THERMAL, FluxBatch, NSCAL, S_ALBEDO, S_BEAM_FLUX, S_BEAM_MU, scalars
+from .batch import (F_THERMAL, NSCAL, S_ALBEDO, S_BEAM_FLUX, S_BEAM_MU, S_TEMP_BOT, S_WN_HI, S_WN_LO, FluxBatch,
+                    scalars)
 
 NLYR, NSTR = 60, 16
 NLEV = NLYR + 1
-N_FEAT = 4 * NLYR + 2  # 242
-N_OUT = 3 * NLEV       # 183
-WIDTHS = (256, 512, 512, 512, 512, 192)
+N_BASE = 4 * NLYR + 2               # 242 per-layer features + mu0 + albedo
+N_LEVEL = N_BASE + 3 * NLEV          # + level optical depth, beam transmission, absorption depth: 425
+N_FEAT = N_LEVEL + NLEV + 1          # + level Planck values and surface emission in beam units: 487
+N_OUT = 3 * NLEV                     # 183
+N_LOSS = 2 * NLEV                    # the direct-beam outputs are replaced by the closed form
+WIDTHS = (512, 512, 512, 512, 512, 192)
 TOL = 1e-3
 C5_SEED = 20261205
 
@@ -77,19 +81,40 @@ def c5_columns(n: int, seed: int = C5_SEED) -> FluxBatch:
                      pstride=np.full(n, NSTR + 1, np.int32), dtau=dtau, ssa=ssa, pmom=pmom, scal=sc, temper=temper)
 
 
-def raw_features(b: FluxBatch) -> np.ndarray:
-    """Host copy of the feature map of the device kernel (statistics only)."""
+def raw_features(b: FluxBatch, n_feat: int = N_FEAT) -> np.ndarray:
+    """Host copy of the feature map of the device kernel (statistics only):
+    the first N_BASE features, the level features when n_feat >= N_LEVEL and
+    the thermal-source features when n_feat = N_FEAT."""
     n, L = b.n, NLYR
     dtau = b.dtau.reshape(n, L)
-    f = np.empty((n, N_FEAT))
+    ssa = b.ssa.reshape(n, L)
+    f = np.empty((n, n_feat))
     f[:, :L] = np.log10(dtau)
-    f[:, L:2 * L] = b.ssa.reshape(n, L)
+    f[:, L:2 * L] = ssa
     f[:, 2 * L:3 * L] = b.pmom.reshape(n, L, -1)[:, :, 1]
     t = b.temper.reshape(n, L + 1)
     f[:, 3 * L:4 * L] = 0.5 * (t[:, :-1] + t[:, 1:])
     s = b.scal.reshape(n, NSCAL)
     f[:, 4 * L] = s[:, S_BEAM_MU]
     f[:, 4 * L + 1] = s[:, S_ALBEDO]
+    if n_feat >= N_LEVEL:
+        z = np.zeros((n, 1))
+        tau = np.concatenate([z, np.cumsum(dtau, axis=1)], axis=1)
+        tabs = np.concatenate([z, np.cumsum((1.0 - ssa) * dtau, axis=1)], axis=1)
+        f[:, N_BASE:N_BASE + NLEV] = np.log10(tau + 1e-3)
+        f[:, N_BASE + NLEV:N_BASE + 2 * NLEV] = np.exp(-tau / s[:, S_BEAM_MU][:, None])
+        f[:, N_BASE + 2 * NLEV:N_BASE + 3 * NLEV] = np.log10(tabs + 1e-3)
+    if n_feat >= N_FEAT:
+        u = s[:, S_BEAM_MU] * s[:, S_BEAM_FLUX]
+        u = np.where(u > 1e-300, u, 1.0)
+        tb = np.concatenate([t, s[:, S_TEMP_BOT][:, None]], axis=1)
+        pb = np.empty_like(tb)
+        bands = np.stack([s[:, S_WN_LO], s[:, S_WN_HI]], axis=1)
+        for lo, hi in np.unique(bands, axis=0):
+            sel = (bands[:, 0] == lo) & (bands[:, 1] == hi)
+            pb[sel] = _capi.planck_band(lo, hi, tb[sel]).reshape(-1, NLEV + 1)
+        f[:, N_LEVEL:N_LEVEL + NLEV] = pb[:, :NLEV] / u[:, None]
+        f[:, N_FEAT - 1] = (1.0 - s[:, S_ALBEDO]) * pb[:, NLEV] / u
     return f
 
 
@@ -154,6 +179,11 @@ def _mlp_lib():
         l.disort_surrogate_gather.argtypes = [vp, C.c_int64, vp, C.c_int, C.c_int, vp, vp]
         l.disort_surrogate_scatter.argtypes = [vp, C.c_int64, vp, C.c_int, vp, C.c_int64, vp]
         l.disort_rows_gather.argtypes = [vp, C.c_int64, vp, C.c_int64, C.c_int, vp, vp]
+        l.disort_surrogate_route.argtypes = [vp, C.c_int64, vp, vp, C.c_float, C.c_double, C.c_uint64, C.c_int, vp,
+                                             vp, vp]
+        l.disort_surrogate_pad_index.argtypes = [vp, C.c_int64, C.c_int64, vp]
+        l.disort_surrogate_monitor.argtypes = [vp, C.c_int64, vp, vp, C.c_int, vp, C.c_int64, vp, vp, C.c_double,
+                                               C.c_double, C.c_double, vp]
         l._mlp_bound = True
     return l
 
@@ -256,18 +286,18 @@ class Normalizer:
     t_scale: np.ndarray
 
     @classmethod
-    def fit(cls, feats: np.ndarray, targets: np.ndarray, margin: float = 0.0):
-        w0, wl = WIDTHS[0], WIDTHS[-1]
+    def fit(cls, feats: np.ndarray, targets: np.ndarray, margin: float = 0.0, w0: int = WIDTHS[0]):
+        wl, nf = WIDTHS[-1], feats.shape[1]
         fs = np.zeros(w0, np.float32)
         fc = np.ones(w0, np.float32)
         lo = np.full(w0, -np.inf, np.float32)
         hi = np.full(w0, np.inf, np.float32)
         m, s = feats.mean(0), feats.std(0)
-        fs[:N_FEAT] = m
-        fc[:N_FEAT] = 1.0 / np.where(s > 0, s, 1.0)
+        fs[:nf] = m
+        fc[:nf] = 1.0 / np.where(s > 0, s, 1.0)
         span = feats.max(0) - feats.min(0)
-        lo[:N_FEAT] = feats.min(0) - margin * span
-        hi[:N_FEAT] = feats.max(0) + margin * span
+        lo[:nf] = feats.min(0) - margin * span
+        hi[:nf] = feats.max(0) + margin * span
         ts = np.zeros(wl, np.float32)
         tc = np.ones(wl, np.float32)
         tm, tsd = targets.mean(0), targets.std(0)
@@ -314,12 +344,14 @@ class DeviceColumns:
         self.dev.close()
 
 
-def build_dataset(ctx, n: int, seed: int, norm: Normalizer | None = None, chunk: int = 100_000):
+def build_dataset(ctx, n: int, seed: int, norm: Normalizer | None = None, chunk: int = 100_000, w0: int = WIDTHS[0]):
     """Solve n fresh columns on the GPU and keep standardised BF16 features and
-    FP32 targets on the device.  Returns (x, t, n, norm)."""
+    FP32 targets on the device.  Returns (x, t, n, norm).  w0: the network's
+    input width (the level features need w0 >= N_FEAT)."""
     l = _mlp_lib()
     n_pad = _pad64(n)
-    x = DevArray(ctx, n_pad * WIDTHS[0] * 2)
+    n_feat = N_FEAT if w0 >= N_FEAT else N_LEVEL if w0 >= N_LEVEL else N_BASE
+    x = DevArray(ctx, n_pad * w0 * 2)
     t = DevArray(ctx, n_pad * WIDTHS[-1] * 4)
     dn = None
     done = 0
@@ -331,13 +363,13 @@ def build_dataset(ctx, n: int, seed: int, norm: Normalizer | None = None, chunk:
         dc.solve()
         if norm is None:
             out = dc.dev.download()
-            norm = Normalizer.fit(raw_features(hb), host_targets(out), margin=0.0)
+            norm = Normalizer.fit(raw_features(hb, n_feat), host_targets(out), margin=0.0, w0=w0)
         if dn is None:
             dn = norm.device(ctx)
         m_pad = m if done + m < n else n_pad - done
         _ck(ctx, l.disort_surrogate_features(ctx.ptr, m, m_pad, NLYR, NSTR + 1, C.byref(dc.cstruct),
                                              dn["f_shift"].ptr, dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr,
-                                             WIDTHS[0], x.ptr + done * WIDTHS[0] * 2, None))
+                                             w0, x.ptr + done * w0 * 2, None))
         _ck(ctx, l.disort_surrogate_targets(ctx.ptr, m, m_pad, NLEV, C.byref(dc.cstruct), dn["t_shift"].ptr,
                                             dn["t_scale"].ptr, WIDTHS[-1], t.ptr + done * WIDTHS[-1] * 4))
         ctx.sync()
@@ -348,24 +380,25 @@ def build_dataset(ctx, n: int, seed: int, norm: Normalizer | None = None, chunk:
 
 
 def train(ctx, mlp: MLP, x: DevArray, t: DevArray, n: int, epochs: int, batch: int = 2048, lr: float = 1e-3,
-          lr_min: float = 1e-5, seed: int = 0, n_out: int | None = None, log=None):
+          lr_min: float = 1e-5, seed: int = 0, n_out: int | None = None, log=None, weights=None):
     """Adam with a cosine learning-rate schedule over shuffled minibatches.
     x: [n x w_0] BF16 rows, t: [n x w_L] FP32 rows (the network's widths);
     the first n_out outputs enter the loss (default: all of them, or the
-    183 flux outputs for the surrogate shape)."""
+    diffuse net and sum outputs for the surrogate shape), each with weight
+    weights[k] (default 1)."""
     l = _mlp_lib()
     w_in, w_out = mlp.widths[0], mlp.widths[-1]
     if x.nbytes < n * w_in * 2 or t.nbytes < n * w_out * 4:
         raise ValueError("training arrays are smaller than n rows of the network's widths")
     if n_out is None:
-        n_out = N_OUT if mlp.widths == WIDTHS else w_out
+        n_out = N_LOSS if mlp.widths[-1] == WIDTHS[-1] else w_out
     B = _pad64(batch)
     if B > mlp.max_batch:
         raise ValueError("batch exceeds the network's max_batch")
     steps_per_epoch = n // B
     total = epochs * steps_per_epoch
     mask = np.zeros(w_out, np.float32)
-    mask[:n_out] = 1.0
+    mask[:n_out] = 1.0 if weights is None else np.asarray(weights, np.float32)[:n_out]
     dmask = DevArray.of(ctx, mask)
     xb = DevArray(ctx, B * w_in * 2)
     tb = DevArray(ctx, B * w_out * 4)
@@ -422,36 +455,75 @@ def column_error(flux, ref, mu0f0, metric: str):
     return net_flux_error(flux, ref, mu0f0) if metric == "local" else net_flux_error_beam(flux, ref, mu0f0)
 
 
+class GateMonitor:
+    """Online error monitor of the gate.  The sampled re-solve gives the
+    surrogate's true error on columns the gate passes; over a window of recent
+    serve calls, when the fraction of those above the bound exceeds alpha by
+    more than three binomial standard deviations (and at least `min_samples`
+    were seen) the monitor trips and every column goes to plain DISORT; the
+    sampled columns keep being compared, and the gate reopens once the window's
+    violation rate is back at or below alpha."""
+
+    def __init__(self, alpha: float, window: int = 8, min_samples: int = 200):
+        self.alpha, self.window, self.min_samples = alpha, window, min_samples
+        self.hist = []  # (sampled passing columns, violations) per serve call
+        self.tripped = False
+
+    def update(self, sampled: int, bad: int):
+        self.hist = (self.hist + [(sampled, bad)])[-self.window:]
+        ns = sum(h[0] for h in self.hist)
+        nb = sum(h[1] for h in self.hist)
+        if ns < self.min_samples:
+            return
+        rate = nb / ns
+        if not self.tripped and rate > self.alpha + 3.0 * math.sqrt(self.alpha * (1.0 - self.alpha) / ns):
+            self.tripped = True
+        elif self.tripped and rate <= self.alpha:
+            self.tripped = False
+
+    def stats(self):
+        ns = sum(h[0] for h in self.hist)
+        nb = sum(h[1] for h in self.hist)
+        return {"sampled_passing": ns, "violations": nb, "rate": nb / ns if ns else None, "tripped": self.tripped}
+
+
 class GatedSurrogate:
     """Ensemble of trained members behind the OOD + spread + sampled re-solve gate.
 
     metric "local": |dF_net| / max(|F_net|, 1e-3 mu0 F0) (relative to the
     local net flux); "beam": |dF_net| / (mu0 F0) (relative to the incident
     beam).  tol is the bound the gate enforces, alpha the tolerated fraction
-    of accepted columns above it."""
+    of accepted columns above it.  Routing, the sampled draw and the error
+    monitor run on the device; routed columns are solved in a sub-batch whose
+    plan is cached per padded size."""
+
+    SUB_GRANULE = 4096
 
     def __init__(self, ctx, members, norm: Normalizer, tau: float = 0.0, alpha: float = 0.01, tol: float = TOL,
                  metric: str = "local"):
         self.ctx, self.members, self.norm = ctx, list(members), norm
         self.dn = norm.device(ctx)
         self.tau, self.alpha, self.tol, self.metric = tau, alpha, tol, metric
+        self.monitor = GateMonitor(alpha)
+        self._subs = {}
 
     def predict(self, dc: DeviceColumns):
         """Device fluxes [n, 3, nlev] (FP64), spread [n] and OOD flags [n] for a resident batch."""
         ctx, l, n = self.ctx, _mlp_lib(), dc.n
         n_pad = _pad64(n)
-        x = DevArray(ctx, n_pad * WIDTHS[0] * 2)
+        w0 = self.members[0].widths[0]
+        x = DevArray(ctx, n_pad * w0 * 2)
         ood = DevArray(ctx, n * 4)
         ys = [DevArray(ctx, n_pad * WIDTHS[-1] * 4) for _ in self.members]
         flux = DevArray(ctx, n * 3 * NLEV * 8)
         spread = DevArray(ctx, n * 4)
         _ck(ctx, l.disort_surrogate_features(ctx.ptr, n, n_pad, NLYR, NSTR + 1, C.byref(dc.cstruct),
                                              self.dn["f_shift"].ptr, self.dn["f_scale"].ptr, self.dn["f_lo"].ptr,
-                                             self.dn["f_hi"].ptr, WIDTHS[0], x.ptr, ood.ptr))
+                                             self.dn["f_hi"].ptr, w0, x.ptr, ood.ptr))
         for m, y in zip(self.members, ys):
             for b0 in range(0, n_pad, m.max_batch):
                 bb = min(m.max_batch, n_pad - b0)
-                m.infer(x.ptr + b0 * WIDTHS[0] * 2, bb, y.ptr + b0 * WIDTHS[-1] * 4)
+                m.infer(x.ptr + b0 * w0 * 2, bb, y.ptr + b0 * WIDTHS[-1] * 4)
         cl, cf = METRICS[self.metric]
         _ck(ctx, l.disort_surrogate_decode(ctx.ptr, n, NLYR, WIDTHS[-1], ys[0].ptr, ys[1].ptr if len(ys) > 1 else None,
                                            self.dn["t_shift"].ptr, self.dn["t_scale"].ptr, C.byref(dc.cstruct),
@@ -488,40 +560,88 @@ class GatedSurrogate:
                 "viol_frac_accepted": float((err[acc] > self.tol).mean()) if acc.any() else 0.0,
                 "viol_frac_all": float((err > self.tol).mean())}
 
+    def sub_size(self, m: int, cap: int) -> int:
+        """Padded sub-batch size for m routed columns: a multiple of the
+        granule on a grid of eight sizes per octave, so that the cached plans
+        are reused across calls (at most 1/8 of the sub-batch is padding)."""
+        g = self.SUB_GRANULE
+        q = -(-m // g)
+        if q > 8:
+            step = 1 << (q.bit_length() - 4)
+            q = -(-q // step) * step
+        return min(q * g, cap)
+
+    def _sub(self, size: int) -> "_SubBatch":
+        sub = self._subs.get(size)
+        if sub is None:
+            sub = self._subs[size] = _SubBatch(self.ctx, size)
+        return sub
+
     def serve(self, dc: DeviceColumns, sample_frac: float, seed: int = 0):
         """Gated fluxes for a resident batch: surrogate where the gate accepts,
-        plain DISORT for OOD / high-spread columns and for a random sample.
-        Returns (device flux [n,3,nlev], info)."""
+        plain DISORT for OOD / high-spread columns and for a random sample, or
+        for every column while the error monitor is tripped.  Returns (device
+        flux [n,3,nlev], info); info["flags"] is the device array of per-column
+        flags (bit 0 routed, bit 1 sampled, bit 2 passes the gate)."""
         ctx, l, n = self.ctx, _mlp_lib(), dc.n
         flux, spread, ood = self.predict(dc)
-        sp = spread.get(np.float32, n)
-        od = ood.get(np.int32, n).astype(bool)
-        r = np.random.Generator(np.random.Philox(seed))
-        sampled = r.random(n) < sample_frac
-        # A NaN spread (non-finite network output) fails the gate.
-        fail = od | ~(sp <= self.tau)
-        route = np.nonzero(fail | sampled)[0].astype(np.int32)
-        m = int(route.size)
+        g = self.SUB_GRANULE
+        cap = -(-max(n, 1) // g) * g  # routed lists are padded to a multiple of the granule
+        flags, idx, cnt = DevArray(ctx, n * 4), DevArray(ctx, cap * 4), DevArray(ctx, 8)
+        tripped = self.monitor.tripped
+        _ck(ctx, l.disort_surrogate_route(ctx.ptr, n, spread.ptr, ood.ptr, self.tau, sample_frac, seed, int(tripped),
+                                          flags.ptr, idx.ptr, cnt.ptr))
+        m = int(cnt.get(np.int32, 1)[0])
+        sampled = bad = 0
         if m:
-            didx = DevArray.of(ctx, route)
-            sub = _SubBatch(ctx, m)
-            _ck(ctx, l.disort_surrogate_gather(ctx.ptr, m, didx.ptr, NLYR, NSTR + 1, C.byref(dc.cstruct),
-                                               C.byref(sub.cstruct)))
-            sub.solve()
-            st = sub.status()
-            if st.any():
-                raise RuntimeError(f"plain DISORT failed on {int((st != 0).sum())} routed columns "
-                                   f"(status codes {sorted(set(st[st != 0].tolist()))})")
-            _ck(ctx, l.disort_surrogate_scatter(ctx.ptr, m, didx.ptr, NLEV, sub.cstruct.out, sub.cstruct.total_levels,
-                                                flux.ptr))
-            ctx.sync()
-            sub.close()
-            didx.free()
+            if m == n:
+                # Every column routed: solve the batch in place (no gather).
+                idx.free()
+                idx = DevArray.of(ctx, np.arange(n, dtype=np.int32))
+                dc.solve()
+                out, tv, status = dc.cstruct.out, dc.cstruct.total_levels, dc.dev
+            else:
+                mpad = self.sub_size(m, cap)
+                _ck(ctx, l.disort_surrogate_pad_index(ctx.ptr, m, mpad, idx.ptr))
+                sub = self._sub(mpad)
+                _ck(ctx, l.disort_surrogate_gather(ctx.ptr, mpad, idx.ptr, NLYR, NSTR + 1, C.byref(dc.cstruct),
+                                                   C.byref(sub.cstruct)))
+                sub.solve()
+                out, tv, status = sub.cstruct.out, sub.cstruct.total_levels, sub
+            cl, cf = METRICS[self.metric]
+            _ck(ctx, l.disort_surrogate_monitor(ctx.ptr, m, idx.ptr, flags.ptr, NLEV, out, tv, flux.ptr,
+                                                C.byref(dc.cstruct), cl, cf, self.tol, cnt.ptr))
+            _ck(ctx, l.disort_surrogate_scatter(ctx.ptr, m, idx.ptr, NLEV, out, tv, flux.ptr))
+            st = status.status() if status is not dc.dev else _plan_status(dc.dev)
+            if st[:m].any():
+                raise RuntimeError(f"plain DISORT failed on {int((st[:m] != 0).sum())} routed columns "
+                                   f"(status codes {sorted(set(st[:m][st[:m] != 0].tolist()))})")
+            sampled, bad = (int(v) for v in cnt.get(np.int32, 2))
+        self.monitor.update(sampled, bad)
+        idx.free()
+        cnt.free()
         spread.free()
         ood.free()
-        return flux, {"n": n, "n_ood": int(od.sum()), "n_spread_fail": int((~(sp <= self.tau)).sum()),
-                      "n_sampled": int(sampled.sum()), "n_plain": m, "sampled_idx": np.nonzero(sampled)[0],
-                      "accepted_idx": np.nonzero(~(fail | sampled))[0]}
+        return flux, {"n": n, "n_plain": m, "tripped": tripped, "monitor_sampled": sampled, "monitor_violations": bad,
+                      "flags": flags}
+
+    def close(self):
+        for sub in self._subs.values():
+            sub.close()
+        self._subs = {}
+
+
+def _plan_status(dev) -> np.ndarray:
+    st = np.zeros(dev.host.n, np.int32)
+    _ck(dev.ctx, _capi.lib().disort_flux_status(dev.ctx.ptr, dev.plan, st.ctypes.data))
+    return st
+
+
+def flags_summary(flags: np.ndarray) -> dict:
+    """Host statistics of a serve call's flags (routed, sampled, passes the gate)."""
+    routed, sampled, passes = (flags & 1) != 0, (flags & 2) != 0, (flags & 4) != 0
+    return {"accepted_idx": np.nonzero(~routed)[0], "sampled_idx": np.nonzero(sampled)[0],
+            "n_sampled": int(sampled.sum()), "n_gate_fail": int((~passes).sum())}
 
 
 class _SubBatch:

@@ -274,6 +274,10 @@ const char* disort_version(void);
  * mirrored.  Host-only; needs no context or GPU.
  */
 int disort_quadrature(int nstr, double* cmu, double* cwt);
+/* Band-integrated Planck function B(lo, hi, t[k]) into b[k] for n host
+ * temperatures: src/Planck.cpp:35-169 (planckFunction2), the same arithmetic
+ * the kernels use.  Returns DISORT_E_INVALID for lo < 0, hi < lo or t < 0. */
+int disort_planck_band(double lo, double hi, int64_t n, const double* t, double* b);
 
 /* =========================================================================
  * General solver: DisortSolver::solve (src/DisortSolver.cpp:21-399) for a
@@ -446,7 +450,29 @@ int disort_surrogate_gather(disort_ctx* ctx, int64_t m, const int32_t* idx, int 
                             const disort_flux_batch* src, disort_flux_batch* dst);
 int disort_surrogate_scatter(disort_ctx* ctx, int64_t m, const int32_t* idx, int nlev, const double* out,
                              int64_t total_levels, double* flux);
-/* dst row k <- src row idx[k] of a row-major matrix of 2- or 4-byte elements. */
+/* Gate routing on the device: column i goes to plain DISORT when it is out
+ * of distribution (ood[i] != 0, ood may be null), its ensemble spread is not
+ * <= tau (NaN included), or it is sampled (a counter-based draw from seed
+ * and i below sample_frac); with route_all every column is routed.
+ * flags[i]: bit 0 routed, bit 1 sampled, bit 2
+ * passes the gate (in distribution, spread <= tau).  The
+ * routed indices are written to idx (in no particular order) and their
+ * number to *count (device). */
+int disort_surrogate_route(disort_ctx* ctx, int64_t n, const float* spread, const int32_t* ood, float tau,
+                           double sample_frac, uint64_t seed, int route_all, int32_t* flags, int32_t* idx,
+                           int32_t* count);
+/* idx[m .. mpad) = idx[0] (a routed list padded to a reusable sub-batch size). */
+int disort_surrogate_pad_index(disort_ctx* ctx, int64_t m, int64_t mpad, int32_t* idx);
+/* Online error monitor: among the m routed columns idx[k] (plain outputs in
+ * out, sub-batch column k), the sampled ones that pass the gate (flags
+ * bits 1 and 2) are compared
+ * with the surrogate fluxes pred [n][3][nlev] of batch b:
+ * counters[0] = sampled columns, counters[1] = those whose max over levels of
+ * |dF_net| / max(c_local |F_net|, c_floor mu0 F0) is not <= tol. */
+int disort_surrogate_monitor(disort_ctx* ctx, int64_t m, const int32_t* idx, const int32_t* flags, int nlev,
+                             const double* out, int64_t total_levels, const double* pred, const disort_flux_batch* b,
+                             double c_local, double c_floor, double tol, int32_t* counters);
+/* dst row k <- src row idx[k] of a row-major matrix of 2-, 4- or 8-byte elements. */
 int disort_rows_gather(disort_ctx* ctx, int64_t m, const int32_t* idx, int64_t row_elems, int elem_bytes,
                        const void* src, void* dst);
 

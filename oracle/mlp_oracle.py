@@ -12,7 +12,10 @@ accumulation differs from it only by accumulation rounding.
 
 Pins (tests/test_mlp_oracle.py): the float64 backward pass against central
 finite differences of the float64 forward pass; GELU and its derivative
-against closed-form values; Adam against its textbook recurrence on scalars.
+against closed-form values; Adam against its textbook recurrence on scalars;
+the column features against hand-computed layouts and the level features
+against closed forms (optical depth sums, Beer's law, half the optical depth
+at single-scattering albedo 0.5).
 """
 from __future__ import annotations
 
@@ -134,3 +137,29 @@ def features(dtau, ssa, pmom, temper, mu0, albedo):
     f[:, 4 * L] = mu0
     f[:, 4 * L + 1] = albedo
     return f
+
+
+def level_features(dtau, ssa, mu0):
+    """Level features of the surrogate [n, 3(L+1)]: log10(tau_v + 1e-3),
+    exp(-tau_v / mu0) and log10(sum_{l<v} (1 - omega_l) dtau_l + 1e-3) at the
+    L+1 levels, tau_v the optical depth from the top to level v."""
+    n, L = dtau.shape
+    f = np.empty((n, 3 * (L + 1)))
+    tau = np.zeros(n)
+    tab = np.zeros(n)
+    for v in range(L + 1):
+        if v > 0:
+            tau = tau + dtau[:, v - 1]
+            tab = tab + (1.0 - ssa[:, v - 1]) * dtau[:, v - 1]
+        f[:, v] = np.log10(tau + 1e-3)
+        f[:, L + 1 + v] = np.exp(-tau / mu0)
+        f[:, 2 * (L + 1) + v] = np.log10(tab + 1e-3)
+    return f
+
+
+def thermal_features(b_levels, b_bottom, albedo, mu0f0):
+    """Thermal-source features of the surrogate [n, L+2]: the level Planck
+    values and the surface emission (1 - albedo) B(T_bottom), in units of the
+    beam mu0 F0 (b_levels [n, L+1], b_bottom [n]: band Planck values)."""
+    u = np.where(mu0f0 > 1e-300, mu0f0, 1.0)
+    return np.concatenate([b_levels / u[:, None], ((1.0 - albedo) * b_bottom / u)[:, None]], axis=1)

@@ -17,6 +17,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import mlp_oracle as mo  # noqa: E402
 
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+import oracle_lib as ol  # noqa: E402
+
 pytestmark = pytest.mark.gpu
 
 
@@ -128,12 +131,32 @@ def test_column_kernels(ctx):
     assert l.disort_surrogate_features(ctx.ptr, n, n_pad, 60, 17, C.byref(dc.cstruct), dn["f_shift"].ptr,
                                        dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, 256, x.ptr, ood.ptr) == 0
     xs = x.get(np.uint16, (n_pad, 256))
-    want = mo.bf16_bits(((feats.astype(np.float32) - norm.f_shift[:242]) * norm.f_scale[:242]).astype(np.float32))
+    want = mo.bf16_bits(((feats[:, :242].astype(np.float32) - norm.f_shift[:242]) * norm.f_scale[:242]).astype(np.float32))
     # Same float32 arithmetic up to the log10 / average in double: at most 1 BF16 ulp apart.
     diff = np.abs(xs[:n, :242].astype(np.int32) - want.astype(np.int32))
     assert diff.max() <= 1
     assert (xs[n:] == 0).all() and (xs[:, 242:] == 0).all()
     assert (ood.get(np.int32, n) == 0).all()  # the columns define the range
+    # With the network's input width the 3 x 61 level features follow (oracle restatement).
+    w0, nf = S.WIDTHS[0], S.N_FEAT
+    dtau, ssa = hb.dtau.reshape(n, 60), hb.ssa.reshape(n, 60)
+    mu0 = hb.scal.reshape(n, 12)[:, 1]
+    sc12 = hb.scal.reshape(n, 12)
+    temper = hb.temper.reshape(n, 61)
+    b_lev = np.array([[ol.planck(sc12[i, 8], sc12[i, 9], tv) for tv in temper[i]] for i in range(n)])
+    b_bot = np.array([ol.planck(sc12[i, 8], sc12[i, 9], sc12[i, 5]) for i in range(n)])
+    full = np.concatenate([mo.features(dtau, ssa, hb.pmom.reshape(n, 60, -1), temper, mu0, sc12[:, 7]),
+                           mo.level_features(dtau, ssa, mu0),
+                           mo.thermal_features(b_lev, b_bot, sc12[:, 7], mu0 * sc12[:, 0])], axis=1)
+    np.testing.assert_allclose(S.raw_features(hb), full, rtol=1e-12, atol=1e-15)
+    x2 = S.DevArray(ctx, n_pad * w0 * 2)
+    assert l.disort_surrogate_features(ctx.ptr, n, n_pad, 60, 17, C.byref(dc.cstruct), dn["f_shift"].ptr,
+                                       dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, w0, x2.ptr, ood.ptr) == 0
+    xs2 = x2.get(np.uint16, (n_pad, w0))
+    want2 = mo.bf16_bits(((full.astype(np.float32) - norm.f_shift[:nf]) * norm.f_scale[:nf]).astype(np.float32))
+    assert np.abs(xs2[:n, :nf].astype(np.int32) - want2.astype(np.int32)).max() <= 1
+    assert (xs2[n:] == 0).all() and (xs2[:, nf:] == 0).all()
+    assert (ood.get(np.int32, n) == 0).all()
     t = S.DevArray(ctx, n_pad * 192 * 4)
     assert l.disort_surrogate_targets(ctx.ptr, n, n_pad, 61, C.byref(dc.cstruct), dn["t_shift"].ptr,
                                       dn["t_scale"].ptr, 192, t.ptr) == 0
@@ -194,13 +217,36 @@ def test_end_to_end_gate_small(ctx):
     ref = S.host_fluxes(dc.dev.download())
     flux, info = gs.serve(dc, sample_frac=0.02, seed=3)
     f = flux.get(np.float64, (dc.n, 3, 61))
-    routed = np.setdiff1d(np.arange(dc.n), info["accepted_idx"])
+    fl = info["flags"].get(np.int32, dc.n)
+    summ = S.flags_summary(fl)
+    routed = np.setdiff1d(np.arange(dc.n), summ["accepted_idx"])
     assert routed.size == info["n_plain"]
+    # flags: routed = sampled or failing the gate; about 2% sampled
+    assert ((fl & 1) != 0).tolist() == (((fl & 2) != 0) | ((fl & 4) == 0)).tolist()
+    assert 0.01 < summ["n_sampled"] / dc.n < 0.03
     np.testing.assert_allclose(f[routed], ref[routed], rtol=1e-9, atol=1e-12)
     sc = fresh.scal.reshape(-1, 12)
     err = S.column_error(f, ref, sc[:, 1] * sc[:, 0], gs.metric)
     assert (err[routed] <= 1e-8).all()
-    if info["accepted_idx"].size > 100:
-        assert (err[info["accepted_idx"]] > gs.tol).mean() < 0.15
+    if summ["accepted_idx"].size > 100:
+        assert (err[summ["accepted_idx"]] > gs.tol).mean() < 0.15
+    # The monitor's count: sampled columns that pass the gate.
+    assert info["monitor_sampled"] == int(((fl & 6) == 6).sum())
+
+    # Online monitor: a gate that passes everything with a bound no column
+    # meets trips after the first call; the next call routes every column to
+    # plain DISORT, which still measures the surrogate on the sampled ones.
+    strict = S.GatedSurrogate(ctx, members, norm, tau=np.inf, alpha=0.01, tol=1e-9, metric="beam")
+    strict.monitor.min_samples = 50
+    f1, i1 = strict.serve(dc, sample_frac=0.05, seed=4)
+    assert i1["monitor_sampled"] >= 50 and i1["monitor_violations"] == i1["monitor_sampled"]
+    assert not i1["tripped"] and strict.monitor.tripped
+    f2, i2 = strict.serve(dc, sample_frac=0.05, seed=5)
+    assert i2["tripped"] and i2["n_plain"] == dc.n
+    np.testing.assert_allclose(f2.get(np.float64, (dc.n, 3, 61)), ref, rtol=1e-9, atol=1e-12)
+    for a in (f1, f2, i1["flags"], i2["flags"], flux, info["flags"]):
+        a.free()
+    strict.close()
+    gs.close()
     for m in members + [mean_net]:
         m.close()

@@ -51,6 +51,25 @@ def test_quadrature_matches_gauss_legendre():
     assert lib.disort_quadrature(5, mu.ctypes.data, wt.ctypes.data) == 1
 
 
+def test_planck_band_matches_oracle():
+    """The library's host Planck entry point (the kernels' arithmetic) is bit
+    for bit the oracle's band integral, itself pinned to the reference's
+    src/Planck.cpp; invalid arguments are rejected."""
+    import sys
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from disort_amd import _capi
+
+    t = np.array([0.0, 1e-5, 150.0, 200.0, 255.5, 300.0, 1000.0])
+    for lo, hi in ((1000.0, 1001.0), (100.0, 101.0), (10.0, 2500.0), (0.0, 50.0)):
+        b = _capi.planck_band(lo, hi, t)
+        assert all(b[k] == ol.planck(lo, hi, tk) for k, tk in enumerate(t)), (lo, hi)
+    for bad in ((-1.0, 2.0, [200.0]), (5.0, 2.0, [200.0]), (1.0, 2.0, [-3.0])):
+        with pytest.raises(_capi.DisortInvalid):
+            _capi.planck_band(*bad)
+
+
 def test_facade_module_loads():
     import disort_amd.disortpp as d
 

@@ -79,3 +79,19 @@ def test_features_layout():
     np.testing.assert_allclose(f[0, :3], [-4.0, 0.0, 1.0])
     np.testing.assert_allclose(f[0, 9:12], [205.0, 220.0, 245.0])
     assert f[1, 12] == 0.4 and f[1, 13] == 0.2
+
+
+def test_level_features_closed_forms():
+    dtau = np.array([[1e-4, 1.0, 10.0], [0.1, 0.2, 0.3]])
+    ssa = np.array([[0.5, 0.5, 0.5], [0.0, 1.0, 0.25]])
+    mu0 = np.array([0.5, 1.0])
+    f = mo.level_features(dtau, ssa, mu0)
+    assert f.shape == (2, 12)
+    # optical depth of the four levels
+    np.testing.assert_allclose(10.0 ** f[0, :4] - 1e-3, [0.0, 1e-4, 1.0001, 11.0001], rtol=1e-12, atol=1e-15)
+    # Beer's law of the direct beam at mu0 = 0.5 and 1
+    np.testing.assert_allclose(f[0, 4:8], np.exp(-2.0 * np.array([0.0, 1e-4, 1.0001, 11.0001])), rtol=1e-12)
+    np.testing.assert_allclose(f[1, 4:8], np.exp(-np.array([0.0, 0.1, 0.3, 0.6])), rtol=1e-12)
+    # absorption depth: half the optical depth at omega = 0.5; 0.1, 0.1, 0.325 in the second column
+    np.testing.assert_allclose(10.0 ** f[0, 8:12] - 1e-3, [0.0, 5e-5, 0.50005, 5.50005], rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(10.0 ** f[1, 8:12] - 1e-3, [0.0, 0.1, 0.1, 0.325], rtol=1e-12, atol=1e-15)

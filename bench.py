@@ -411,6 +411,43 @@ def bench_c5(args) -> dict:
     return line
 
 
+def bench_mlp(args) -> dict:
+    """The surrogate network's inference GEMMs alone (disort_mlp_infer: five
+    k_gemm launches per call, widths 512 x 512 x 512 x 512 x 512 x 192) on
+    resident 65,536-row BF16 batches: the measurement behind the C5 line's
+    roofline entry.  Random weights; the arithmetic does not depend on them."""
+    import numpy as np
+
+    from disort_amd import surrogate as S
+    from disort_amd._capi import Context
+
+    ctx = Context(0)
+    rows = 65536
+    m = S.MLP(ctx, S.WIDTHS, max_batch=rows)
+    m.set_params(m.init_params(1, S.N_FEAT, S.N_OUT))
+    x = S.DevArray(ctx, rows * S.WIDTHS[0] * 2)
+    y = S.DevArray(ctx, rows * S.WIDTHS[-1] * 4)
+    r = np.random.Generator(np.random.Philox(3)).normal(0, 1, rows * S.WIDTHS[0]).astype(np.float32)
+    ctx.h2d(x.ptr, (r.view(np.uint32) >> 16).astype(np.uint16))
+    for _ in range(args.warmup):
+        m.infer(x.ptr, rows, y.ptr)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m.infer(x.ptr, rows, y.ptr)
+    ctx.sync()
+    sec = (time.perf_counter() - t0) / args.steps
+    flops = 2.0 * rows * sum(a * b for a, b in zip(S.WIDTHS[:-1], S.WIDTHS[1:]))
+    line = {"metric": "surrogate MLP inference rows/sec (BF16 MFMA, FP32 accumulate)", "value": rows / sec,
+            "unit": "rows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3,
+            "higher_is_better": True, "dtype": "bf16", "data": "synthetic (random weights and inputs)",
+            "config": {"workload": "disort_mlp_infer, 65,536 rows", "widths": list(S.WIDTHS)},
+            "roofline": {"bound": "mfma", "kernel": "k_gemm (5 launches per row batch)", "achieved": flops / sec / 1e12,
+                         "peak": 2500.0, "unit": "TFLOP/s", "frac": flops / sec / 1e12 / 2500.0, "traffic": None}}
+    m.close()
+    return line
+
+
 def bench_c3(args) -> dict:
     """C3: the general solver (azimuthal modes, user intensities, RPV, pseudo-
     spherical beam, Nakajima-Tanaka) on 4,096 columns resident in HBM."""
@@ -518,9 +555,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--bands", type=int, default=10000)
-    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "mlp"), default="c2",
                     help="c2: the headline spectral flux solve (default); c3: the general solver with "
-                         "intensities; c4: ragged autotuned batch; c5: learned surrogate behind the error gate")
+                         "intensities; c4: ragged autotuned batch; c5: learned surrogate behind the error gate; "
+                         "mlp: the surrogate network's inference GEMMs alone")
     ap.add_argument("--columns", type=int, default=4096, help="C3 columns")
     ap.add_argument("--cpu-columns", type=int, default=64, help="C3 columns timed on the CPU")
     ap.add_argument("--train-columns", type=int, default=8388608, help="C5 training columns")
@@ -549,6 +587,9 @@ def main() -> int:
     if args.workload == "c4":
         print(json.dumps(bench_c4(args)))
         return 0
+    if args.workload == "mlp":
+        print(json.dumps(bench_mlp(args)))
+        return 0
     if args.workload == "c5":
         print(json.dumps(bench_c5(args)))
         return 0
@@ -575,7 +616,11 @@ def main() -> int:
     sec = t1 - t0
     value = args.bands * args.steps / sec
 
-    # Per-kernel device time with hipEvents on the context stream (separate passes).
+    # Per-kernel device time with hipEvents on the context stream (separate
+    # passes), each kernel as one launch over the whole batch: the plan is
+    # set back to one bucket on the context stream for these passes.
+    dev.set_streams(1)
+    dev.set_chunks(nstr, 1)
     ctx.set_timing(True)
     reps = 5
     stage = [0.0, 0.0, 0.0, 0.0]

@@ -71,6 +71,7 @@ def lib():
         l.disort_flux_plan_get_variant.argtypes = [vp, vp, C.c_int, C.POINTER(C.c_int)]
         l.disort_quadrature.argtypes = [C.c_int, vp, vp]
         l.disort_planck_band.argtypes = [C.c_double, C.c_double, C.c_int64, vp, vp]
+        l.disort_planck_band_deriv.argtypes = [C.c_double, C.c_double, C.c_int64, vp, vp]
         l.disort_flux_plan_set_concurrent.argtypes = [vp, vp, C.c_int]
         l.disort_flux_plan_set_streams.argtypes = [vp, vp, C.c_int]
         l.disort_flux_plan_set_chunks.argtypes = [vp, vp, C.c_int, C.c_int]
@@ -84,6 +85,15 @@ def lib():
         l.disort_flux_static_bytes.argtypes = [C.c_int64, C.c_int32, C.c_int32]
         l.disort_flux_static_bytes.restype = C.c_size_t
         l.disort_static_scratch_bytes.restype = C.c_size_t
+        l.disort_jac_plan_create.argtypes = [vp, C.c_int64, C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int64,
+                                             C.POINTER(vp)]
+        l.disort_jac_plan_destroy.argtypes = [vp, vp]
+        l.disort_jac_plan_chunk.argtypes = [vp]
+        l.disort_jac_plan_chunk.restype = C.c_int64
+        l.disort_jac_plan_flux.argtypes = [vp]
+        l.disort_jac_plan_flux.restype = vp
+        l.disort_jac_execute.argtypes = [vp, vp, vp, vp]
+        l.disort_jac_status.argtypes = [vp, vp, vp]
         _lib = l
     return _lib
 
@@ -299,6 +309,52 @@ class DeviceFluxBatch:
             pass
 
 
+class JacPlan:
+    """Device-resident temperature Jacobians of n uniform columns
+    (disort_jac_plan_create / disort_jac_execute): execute(batch, jac_ptr)
+    writes jac [n, 6, nlyr+1, nlyr+2] (FP64, device) for a device batch
+    descriptor, asynchronously on the context stream."""
+
+    def __init__(self, ctx: Context, n: int, nstr: int, nlyr: int, pstride: int, flags: int, max_expanded: int = 0):
+        self.ctx, self.n, self.nstr, self.nlyr = ctx, int(n), int(nstr), int(nlyr)
+        self.plan = C.c_void_p()
+        _check(ctx.ptr, lib().disort_jac_plan_create(ctx.ptr, self.n, self.nstr, self.nlyr, int(pstride), int(flags),
+                                                      int(max_expanded), C.byref(self.plan)))
+        self.chunk = int(lib().disort_jac_plan_chunk(self.plan))
+
+    @property
+    def jac_bytes(self) -> int:
+        return self.n * 6 * (self.nlyr + 1) * (self.nlyr + 2) * 8
+
+    def set_streams(self, streams: int):
+        """Schedule of the inner flux plan (see DeviceFluxBatch.set_streams)."""
+        _check(self.ctx.ptr, lib().disort_flux_plan_set_streams(self.ctx.ptr, lib().disort_jac_plan_flux(self.plan),
+                                                                int(streams)))
+
+    def set_chunks(self, chunks: int):
+        _check(self.ctx.ptr, lib().disort_flux_plan_set_chunks(self.ctx.ptr, lib().disort_jac_plan_flux(self.plan),
+                                                               self.nstr, int(chunks)))
+
+    def execute(self, cstruct, jac_ptr: int):
+        _check(self.ctx.ptr, lib().disort_jac_execute(self.ctx.ptr, self.plan, C.byref(cstruct), C.c_void_p(jac_ptr)))
+
+    def status(self) -> np.ndarray:
+        st = np.zeros(self.n, np.int32)
+        _check(self.ctx.ptr, lib().disort_jac_status(self.ctx.ptr, self.plan, st.ctypes.data))
+        return st
+
+    def close(self):
+        if self.plan:
+            lib().disort_jac_plan_destroy(self.ctx.ptr, self.plan)
+            self.plan = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def version() -> str:
     return lib().disort_version().decode()
 
@@ -311,3 +367,13 @@ def planck_band(lo: float, hi: float, t) -> np.ndarray:
     if rc:
         raise DisortInvalid(rc, "disort_planck_band: invalid band or temperature")
     return b
+
+
+def planck_band_deriv(lo: float, hi: float, t) -> np.ndarray:
+    """dB/dT of the band-integrated Planck function at host temperatures t."""
+    t = np.ascontiguousarray(t, dtype=np.float64).reshape(-1)
+    d = np.empty_like(t)
+    rc = lib().disort_planck_band_deriv(float(lo), float(hi), t.size, t.ctypes.data, d.ctypes.data)
+    if rc:
+        raise DisortInvalid(rc, "disort_planck_band_deriv: invalid band or temperature")
+    return d

@@ -27,6 +27,12 @@ namespace dsa {
 bool launch_flux_bucket(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev, bool pair);
 // Workgroup kernels (flux_wide.hip), N in {8, 16, 32, 64}.
 bool launch_flux_bucket_wg(int nstr, const FluxArgs& a, hipStream_t s, hipEvent_t* ev);
+// Jacobian expansion and scaling (jacobian_kernels.hip).
+bool launch_jac_expand(int64_t n, int64_t c0, int64_t ne, int L, int P, const disort_flux_batch* b, double* xdtau,
+                       double* xssa, double* xpmom, double* xtemper, double* xscal, double* ovr, double* dbdt,
+                       hipStream_t st);
+bool launch_jac_scale(int64_t cols, int64_t c0, int L, const double* out, int64_t TV, const double* dbdt,
+                      double* jac, const int32_t* inst_i, int32_t* status, hipStream_t st);
 }
 
 using dsa::FluxArgs;
@@ -555,6 +561,15 @@ int disort_planck_band(double lo, double hi, int64_t n, const double* t, double*
   for (int64_t k = 0; k < n; ++k) {
     if (!(t[k] >= 0.0)) return DISORT_E_INVALID;
     b[k] = dsa::planck_band(lo, hi, t[k]);
+  }
+  return DISORT_OK;
+}
+
+int disort_planck_band_deriv(double lo, double hi, int64_t n, const double* t, double* d) {
+  if (!(lo >= 0.0) || !(hi >= lo) || n < 0 || (n > 0 && (!t || !d))) return DISORT_E_INVALID;
+  for (int64_t k = 0; k < n; ++k) {
+    if (!(t[k] >= 0.0)) return DISORT_E_INVALID;
+    d[k] = dsa::planck_band_deriv(lo, hi, t[k]);
   }
   return DISORT_OK;
 }
@@ -1090,6 +1105,126 @@ int disort_flux_jacobian(disort_ctx* ctx, const disort_flux_batch* b, double* ja
     jac_off = base;
     c0 = c1;
   }
+  return DISORT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Device-resident temperature Jacobians of uniform columns: the expansion of
+// disort_flux_jacobian done on the device, chunk by chunk, into buffers the
+// plan owns (jacobian_kernels.hip), solved by an inner flux plan of
+// chunk * (nlyr + 2) instances.
+// ---------------------------------------------------------------------------
+struct disort_jac_plan {
+  int64_t n = 0, chunk = 0, ne = 0;
+  int32_t nstr = 0, nlyr = 0, pstride = 0, flags = 0;
+  disort_flux_plan* flux = nullptr;
+  double *dtau = nullptr, *ssa = nullptr, *pmom = nullptr, *temper = nullptr, *scal = nullptr;
+  double *ovr = nullptr, *out = nullptr, *dbdt = nullptr;
+  int32_t* status = nullptr;  // per column
+  std::vector<void*> owned;
+};
+
+namespace {
+void free_jac_plan(disort_jac_plan* jp) {
+  if (jp->flux) free_plan(jp->flux);
+  for (void* p : jp->owned) (void)hipFree(p);
+  delete jp;
+}
+}  // namespace
+
+extern "C" {
+
+int disort_jac_plan_create(disort_ctx* ctx, int64_t n, int32_t nstr, int32_t nlyr, int32_t pstride, int32_t flags,
+                           int64_t max_expanded, disort_jac_plan** out) {
+  if (!ctx || !out) return fail(ctx, DISORT_E_INVALID, "null argument");
+  *out = nullptr;
+  if (n <= 0 || nlyr <= 0) return fail(ctx, DISORT_E_INVALID, "need n > 0 columns of nlyr > 0 layers");
+  if (!(flags & DISORT_F_THERMAL))
+    return fail(ctx, DISORT_E_INVALID, "temperature Jacobians need use_thermal_emission = true");
+  if (flags & ~(DISORT_F_THERMAL | DISORT_F_DELTA_M_PLUS | DISORT_F_DIFFUSION_BC))
+    return fail(ctx, DISORT_E_INVALID, "device Jacobians support the thermal, Delta-M+ and diffusion-BC flags only");
+  if (!gpu_supported_nstr(nstr)) return fail(ctx, DISORT_E_UNSUPPORTED, "no kernels for num_streams=" + std::to_string(nstr));
+  if (pstride < nstr + 1 + ((flags & DISORT_F_DELTA_M_PLUS) ? 1 : 0))
+    return fail(ctx, DISORT_E_INVALID, "pstride below the moments the solver reads");
+  const int D = nlyr + 2;
+  if (max_expanded <= 0) max_expanded = 131072;
+  auto* jp = new disort_jac_plan;
+  std::unique_ptr<disort_jac_plan, void (*)(disort_jac_plan*)> guard(jp, free_jac_plan);
+  jp->n = n;
+  jp->nstr = nstr, jp->nlyr = nlyr, jp->pstride = pstride, jp->flags = flags;
+  jp->chunk = std::max<int64_t>(1, std::min<int64_t>(n, max_expanded / D));
+  jp->ne = jp->chunk * D;
+  const int64_t ne = jp->ne, L = nlyr;
+  HIPCHK(ctx, hipSetDevice(ctx->device));
+  auto alloc = [&](size_t bytes, void** p) -> int {
+    HIPCHK(ctx, hipMalloc(p, std::max<size_t>(bytes, 256)));
+    jp->owned.push_back(*p);
+    return DISORT_OK;
+  };
+  int rc;
+  if ((rc = alloc(ne * L * 8, (void**)&jp->dtau)) || (rc = alloc(ne * L * 8, (void**)&jp->ssa)) ||
+      (rc = alloc(ne * L * pstride * 8, (void**)&jp->pmom)) || (rc = alloc(ne * (L + 1) * 8, (void**)&jp->temper)) ||
+      (rc = alloc(ne * DISORT_NSCAL * 8, (void**)&jp->scal)) ||
+      (rc = alloc((ne * (L + 1) + 2 * ne) * 8, (void**)&jp->ovr)) ||
+      (rc = alloc((size_t)DISORT_NOUT * ne * (L + 1) * 8, (void**)&jp->out)) || (rc = alloc(ne * 8, (void**)&jp->dbdt)) ||
+      (rc = alloc(n * 4, (void**)&jp->status)))
+    return rc;
+  std::vector<int32_t> vn(ne, nstr), vl(ne, nlyr), vp(ne, pstride);
+  if ((rc = disort_flux_plan_create(ctx, ne, vn.data(), vl.data(), vp.data(), flags, &jp->flux))) return rc;
+  *out = guard.release();
+  return DISORT_OK;
+}
+
+int disort_jac_plan_destroy(disort_ctx* ctx, disort_jac_plan* jp) {
+  if (!jp) return DISORT_OK;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  free_jac_plan(jp);
+  return DISORT_OK;
+}
+
+int64_t disort_jac_plan_chunk(const disort_jac_plan* jp) { return jp ? jp->chunk : 0; }
+
+disort_flux_plan* disort_jac_plan_flux(disort_jac_plan* jp) { return jp ? jp->flux : nullptr; }
+
+int disort_jac_execute(disort_ctx* ctx, disort_jac_plan* jp, const disort_flux_batch* b, double* jac) {
+  if (!ctx || !jp || !b || !jac) return fail(ctx, DISORT_E_INVALID, "null argument");
+  if (b->n != jp->n) return fail(ctx, DISORT_E_INVALID, "batch size differs from the plan");
+  if (!b->dtau || !b->ssa || !b->pmom || !b->temper || !b->scal)
+    return fail(ctx, DISORT_E_INVALID, "the batch needs dtau, ssa, pmom, temper and scal");
+  const int L = jp->nlyr;
+  const int64_t ne = jp->ne, TV = ne * (L + 1);
+  for (int64_t c0 = 0; c0 < jp->n; c0 += jp->chunk) {
+    const int64_t cols = std::min(jp->chunk, jp->n - c0);
+    if (!dsa::launch_jac_expand(jp->n, c0, ne, L, jp->pstride, b, jp->dtau, jp->ssa, jp->pmom, jp->temper, jp->scal,
+                                jp->ovr, jp->dbdt, ctx->stream))
+      return fail(ctx, DISORT_E_HIP, "Jacobian expansion launch failed");
+    disort_flux_batch d{};
+    d.n = ne;
+    d.flags = jp->flags;
+    d.dtau = jp->dtau;
+    d.ssa = jp->ssa;
+    d.pmom = jp->pmom;
+    d.temper = jp->temper;
+    d.scal = jp->scal;
+    d.out = jp->out;
+    d.total_levels = TV;
+    PlanArgs pa;
+    int rc;
+    if ((rc = prepare(ctx, jp->flux, &d, pa))) return rc;
+    pa.base.planck_ovr = jp->ovr;
+    if ((rc = launch_plan(ctx, jp->flux, pa, false))) return rc;
+    if (!dsa::launch_jac_scale(cols, c0, L, jp->out, TV, jp->dbdt, jac, pa.base.inst_i, jp->status, ctx->stream))
+      return fail(ctx, DISORT_E_HIP, "Jacobian scaling launch failed");
+  }
+  return DISORT_OK;
+}
+
+int disort_jac_status(disort_ctx* ctx, disort_jac_plan* jp, int32_t* status_host) {
+  if (!ctx || !jp || !status_host) return fail(ctx, DISORT_E_INVALID, "null argument");
+  HIPCHK(ctx, hipMemcpyAsync(status_host, jp->status, jp->n * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
   return DISORT_OK;
 }
 

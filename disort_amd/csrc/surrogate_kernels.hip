@@ -298,6 +298,192 @@ __global__ void k_monitor(int64_t m, const int32_t* idx, const int32_t* flags, i
   }
 }
 
+// ---------------------------------------------------------------------------
+// Jacobian surrogate: the temperature Jacobians of the top-of-atmosphere
+// upward flux and the surface downward flux, dF/dT_k = R_k dB/dT(T_k) for the
+// D = nlyr + 2 degrees of freedom k (level temperatures, then the surface).
+// The responses R_k = dF/dB_k depend on the optical properties only; the
+// network learns R / pi and dB/dT is evaluated exactly.
+// Emission features of column i (3 nlyr + 1 + 6 (nlyr+1), zero padding to w0),
+// with mubar = 0.6 the diffusivity cosine:
+//   [l] log10(delta_tau_l), [nlyr + l] single_scat_albedo_l, [2nlyr + l] g_l,
+//   [3nlyr] surface albedo, then six blocks over the levels v = 0..nlyr:
+//   log10(tau_v + 1e-3), log10(a_v + 1e-3), exp(-tau_v / mubar),
+//   exp(-a_v / mubar), exp(-(tau_L - tau_v) / mubar), exp(-(a_L - a_v) / mubar)
+// with tau_v the optical depth and a_v = sum_{l<v} (1 - omega_l) dtau_l the
+// absorption optical depth of level v (L = nlyr: the surface level).
+// Targets (2D, then zero padding to wL):
+//   [k]      R_k / pi of flux_up at the top level
+//   [D + k]  R_k / pi of flux_down at the surface level
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kMuBar = 0.6;
+
+__device__ __forceinline__ double dbdt_dof(const double* temper, const double* scal, int64_t i, int L, int k) {
+  const double* sc = scal + i * DISORT_NSCAL;
+  const double t = (k <= L) ? temper[i * (L + 1) + k] : sc[DISORT_S_TEMP_BOT];
+  return dsa::planck_band_deriv(sc[DISORT_S_WN_LO], sc[DISORT_S_WN_HI], t);
+}
+
+// Offset of the Jacobian row of target row r (0: flux_up at level 0, 1:
+// flux_down at level L) within column i's [6][L+1][D] block.
+__device__ __forceinline__ int64_t jac_row(int64_t i, int L, int r) {
+  const int D = L + 2;
+  const int64_t per = (int64_t)(L + 1) * D;
+  return i * 6 * per + (r == 0 ? 0 : per + (int64_t)L * D);
+}
+
+// One wavefront per column (four per workgroup), as k_features.
+__global__ __launch_bounds__(256) void k_features_emission(int64_t n, int64_t n_pad, int L, int P,
+                                                           const double* dtau, const double* ssa,
+                                                           const double* pmom, const double* scal,
+                                                           const float* shift, const float* scale, const float* lo,
+                                                           const float* hi, int w0, __bf16* x, int32_t* ood) {
+  __shared__ double lev[4][2][64];  // [column][tau, absorption tau][level]
+  __shared__ double lay[4][2][64];  // [column][dtau, (1 - omega) dtau][layer]
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + wv;
+  if (i >= n_pad) return;
+  __bf16* xr = x + i * w0;
+  if (i >= n) {
+    for (int k = lane; k < w0; k += 64) xr[k] = (__bf16)0.0f;
+    return;
+  }
+  const int nlev = L + 1, base = 3 * L + 1, nf = base + 6 * nlev;
+  double (*lv)[64] = lev[wv];
+  double (*ly)[64] = lay[wv];
+  if (lane < L) {
+    const double dt = dtau[i * L + lane];
+    ly[0][lane] = dt;
+    ly[1][lane] = (1.0 - ssa[i * L + lane]) * dt;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane < nlev) {
+    double tau = 0.0, tab = 0.0;
+    for (int l = 0; l < lane; ++l) {
+      tau += ly[0][l];
+      tab += ly[1][l];
+    }
+    lv[0][lane] = tau;
+    lv[1][lane] = tab;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const double tauL = lv[0][L], tabL = lv[1][L];
+  int bad = 0;
+  for (int k = lane; k < w0; k += 64) {
+    double f = 0.0;
+    if (k < L) f = log10(dtau[i * L + k]);
+    else if (k < 2 * L) f = ssa[i * L + (k - L)];
+    else if (k < 3 * L) f = pmom[(i * L + (k - 2 * L)) * P + 1];
+    else if (k == 3 * L) f = scal[i * DISORT_NSCAL + DISORT_S_ALBEDO];
+    else if (k < nf) {
+      const int blk = (k - base) / nlev, v = (k - base) % nlev;
+      const double tau = lv[0][v], tab = lv[1][v];
+      f = (blk == 0)   ? log10(tau + 1e-3)
+          : (blk == 1) ? log10(tab + 1e-3)
+          : (blk == 2) ? exp(-tau / kMuBar)
+          : (blk == 3) ? exp(-tab / kMuBar)
+          : (blk == 4) ? exp(-(tauL - tau) / kMuBar)
+                       : exp(-(tabL - tab) / kMuBar);
+    }
+    if (k < nf) {
+      const float ff = (float)f;
+      bad |= !(ff >= lo[k] && ff <= hi[k]);
+      xr[k] = (__bf16)((ff - shift[k]) * scale[k]);
+    } else {
+      xr[k] = (__bf16)0.0f;
+    }
+  }
+  const unsigned long long any_bad = __ballot(bad);
+  if (ood && lane == 0) ood[i] = any_bad ? 1 : 0;
+}
+
+__global__ void k_jac_targets(int64_t n, int64_t n_pad, int L, const double* jac, const double* temper,
+                              const double* scal, const float* shift, const float* scale, int wL, float* t) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_pad * wL) return;
+  const int64_t i = e / wL;
+  const int k = (int)(e % wL), D = L + 2;
+  float v = 0.f;
+  if (i < n && k < 2 * D) {
+    const int r = k / D, d = k % D;
+    const double rr = jac[jac_row(i, L, r) + d] / (kPi * dbdt_dof(temper, scal, i, L, d));
+    v = ((float)rr - shift[k]) * scale[k];
+  }
+  t[e] = v;
+}
+
+// Network outputs -> Jacobian rows jrows [n][2][D] (mean of the members) and
+// the members' spread max_k |R_1 - R_2| / pi per column (lane = degree of freedom).
+__global__ __launch_bounds__(256) void k_jac_decode(int64_t n, int L, int wL, const float* y1, const float* y2,
+                                                    const float* shift, const float* scale, const double* temper,
+                                                    const double* scal, double* jrows, float* spread) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const int D = L + 2;
+  double worst = 0.0;
+  if (lane < D) {
+    const double db = kPi * dbdt_dof(temper, scal, i, L, lane);
+    for (int r = 0; r < 2; ++r) {
+      const int k = r * D + lane;
+      const double a1 = (double)y1[i * wL + k] / scale[k] + shift[k];
+      double a = a1;
+      if (y2) {
+        const double a2 = (double)y2[i * wL + k] / scale[k] + shift[k];
+        a = 0.5 * (a1 + a2);
+        worst = fmax(worst, fabs(a1 - a2));
+        if (!(a1 - a2 == a1 - a2)) worst = __builtin_inf();
+      }
+      jrows[(i * 2 + r) * D + lane] = a * db;
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) worst = fmax(worst, __shfl_xor(worst, off));
+  if (lane == 0) spread[i] = (float)worst;
+}
+
+// Online error monitor of the Jacobian gate: sampled columns that pass the
+// gate, error max over both rows and every k of |dF/dT_k - ref| / (pi dB/dT(T_k)).
+__global__ void k_jac_monitor(int64_t m, const int32_t* idx, const int32_t* flags, int L, const double* jac_sub,
+                              const double* jrows, const double* temper, const double* scal, double tol,
+                              int32_t* counters) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool smp = false, bad = false;
+  if (k < m) {
+    const int64_t i = idx[k];
+    smp = (flags[i] & 6) == 6;
+    if (smp) {
+      const int D = L + 2;
+      double worst = 0.0;
+      for (int d = 0; d < D; ++d) {
+        const double den = kPi * dbdt_dof(temper, scal, i, L, d);
+        for (int r = 0; r < 2; ++r)
+          worst = fmax(worst, fabs(jrows[(i * 2 + r) * D + d] - jac_sub[jac_row(k, L, r) + d]) / den);
+      }
+      bad = !(worst <= tol);
+    }
+  }
+  const unsigned long long bs = __ballot(smp), bb = __ballot(bad);
+  if ((threadIdx.x & 63) == 0) {
+    if (bs) atomicAdd(&counters[0], __popcll(bs));
+    if (bb) atomicAdd(&counters[1], __popcll(bb));
+  }
+}
+
+// jrows of column idx[k] <- the two rows of sub-batch column k's Jacobian.
+__global__ void k_jac_scatter(int64_t m, const int32_t* idx, int L, const double* jac_sub, double* jrows) {
+  const int D = L + 2;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m * 2 * D) return;
+  const int64_t k = e / (2 * D);
+  const int r = (int)((e / D) % 2), d = (int)(e % D);
+  jrows[((int64_t)idx[k] * 2 + r) * D + d] = jac_sub[jac_row(k, L, r) + d];
+}
+
 }  // namespace
 
 extern "C" {
@@ -396,6 +582,62 @@ int disort_surrogate_monitor(disort_ctx* ctx, int64_t m, const int32_t* idx, con
   if (m > 0)
     hipLaunchKernelGGL(k_monitor, dim3(blocks(m)), dim3(256), 0, st, m, idx, flags, nlev, out, total_levels, pred, b->scal,
                        c_local, c_floor, tol, counters);
+  SURR_LAUNCH_CHECK();
+  return DISORT_OK;
+}
+
+int disort_jac_surrogate_features(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlyr, int pstride,
+                                  const disort_flux_batch* b, const float* shift, const float* scale, const float* lo,
+                                  const float* hi, int w0, uint16_t* x, int32_t* ood) {
+  if (!ctx || !b || n < 0 || n_pad < n || nlyr <= 0 || pstride < 2 || w0 < 3 * nlyr + 1 + 6 * (nlyr + 1))
+    return DISORT_E_INVALID;
+  if (nlyr + 1 > 64) return DISORT_E_INVALID;  // one lane per level
+  hipLaunchKernelGGL(k_features_emission, dim3(blocks(n_pad, 4)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n,
+                     n_pad, nlyr, pstride, b->dtau, b->ssa, b->pmom, b->scal, shift, scale, lo, hi, w0, (__bf16*)x, ood);
+  SURR_LAUNCH_CHECK();
+  return DISORT_OK;
+}
+
+int disort_jac_surrogate_targets(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlyr, const double* jac,
+                                 const disort_flux_batch* b, const float* shift, const float* scale, int wL,
+                                 float* t) {
+  if (!ctx || !b || !jac || !b->temper || n < 0 || n_pad < n || wL < 2 * (nlyr + 2)) return DISORT_E_INVALID;
+  hipLaunchKernelGGL(k_jac_targets, dim3(blocks(n_pad * wL)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n,
+                     n_pad, nlyr, jac, b->temper, b->scal, shift, scale, wL, t);
+  SURR_LAUNCH_CHECK();
+  return DISORT_OK;
+}
+
+int disort_jac_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const float* y1, const float* y2,
+                                const float* shift, const float* scale, const disort_flux_batch* b, double* jrows,
+                                float* spread) {
+  if (!ctx || !b || !b->temper || !y1 || !jrows || !spread || wL < 2 * (nlyr + 2)) return DISORT_E_INVALID;
+  if (nlyr + 2 > 64) return DISORT_E_INVALID;  // one lane per degree of freedom
+  hipLaunchKernelGGL(k_jac_decode, dim3(blocks(n, 4)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n, nlyr, wL,
+                     y1, y2, shift, scale, b->temper, b->scal, jrows, spread);
+  SURR_LAUNCH_CHECK();
+  return DISORT_OK;
+}
+
+int disort_jac_surrogate_monitor(disort_ctx* ctx, int64_t m, const int32_t* idx, const int32_t* flags, int nlyr,
+                                 const double* jac_sub, const double* jrows, const disort_flux_batch* b, double tol,
+                                 int32_t* counters) {
+  if (!ctx || !idx || !flags || !jac_sub || !jrows || !b || !b->temper || !counters || m < 0) return DISORT_E_INVALID;
+  hipStream_t st = (hipStream_t)disort_ctx_stream(ctx);
+  hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(64), 0, st, counters, 2);
+  if (m > 0)
+    hipLaunchKernelGGL(k_jac_monitor, dim3(blocks(m)), dim3(256), 0, st, m, idx, flags, nlyr, jac_sub, jrows, b->temper,
+                       b->scal, tol, counters);
+  SURR_LAUNCH_CHECK();
+  return DISORT_OK;
+}
+
+int disort_jac_surrogate_scatter(disort_ctx* ctx, int64_t m, const int32_t* idx, int nlyr, const double* jac_sub,
+                                 double* jrows) {
+  if (!ctx || !idx || !jac_sub || !jrows || m < 0) return DISORT_E_INVALID;
+  if (m == 0) return DISORT_OK;
+  hipLaunchKernelGGL(k_jac_scatter, dim3(blocks(m * 2 * (nlyr + 2))), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx),
+                     m, idx, nlyr, jac_sub, jrows);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;
 }

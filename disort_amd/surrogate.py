@@ -286,8 +286,9 @@ class Normalizer:
     t_scale: np.ndarray
 
     @classmethod
-    def fit(cls, feats: np.ndarray, targets: np.ndarray, margin: float = 0.0, w0: int = WIDTHS[0]):
-        wl, nf = WIDTHS[-1], feats.shape[1]
+    def fit(cls, feats: np.ndarray, targets: np.ndarray, margin: float = 0.0, w0: int = WIDTHS[0],
+            wl: int = WIDTHS[-1]):
+        nf, nt = feats.shape[1], targets.shape[1]
         fs = np.zeros(w0, np.float32)
         fc = np.ones(w0, np.float32)
         lo = np.full(w0, -np.inf, np.float32)
@@ -301,8 +302,8 @@ class Normalizer:
         ts = np.zeros(wl, np.float32)
         tc = np.ones(wl, np.float32)
         tm, tsd = targets.mean(0), targets.std(0)
-        ts[:N_OUT] = tm
-        tc[:N_OUT] = 1.0 / np.where(tsd > 1e-30, tsd, 1.0)
+        ts[:nt] = tm
+        tc[:nt] = 1.0 / np.where(tsd > 1e-30, tsd, 1.0)
         return cls(fs, fc, lo, hi, ts, tc)
 
     def device(self, ctx):

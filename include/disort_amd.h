@@ -172,6 +172,31 @@ int disort_flux_execute(disort_ctx* ctx, disort_flux_plan* plan, const disort_fl
 int disort_flux_status(disort_ctx* ctx, disort_flux_plan* plan, int32_t* status_host);
 
 /*
+ * Device-resident temperature Jacobians of n uniform columns (nstr, nlyr,
+ * pstride shared; lay_off[i] = i*nlyr): disort_flux_jacobian
+ * (src/FluxSolver.cpp:1294-1497, FluxResult::*_temperature_jac) on device
+ * arrays.  disort_jac_execute reads the batch's dtau, ssa, pmom, temper and
+ * scal (device pointers; values are not validated) and writes the device
+ * array jac [n][6][nlyr+1][nlyr+2] in the layout of disort_flux_jacobian,
+ * asynchronously on the context stream.  The plan expands up to
+ * max_expanded (0: 131072) column x degree-of-freedom instances at a time
+ * into buffers it owns and solves them with an inner flux plan
+ * (disort_jac_plan_flux: its schedule setters apply); disort_jac_plan_chunk
+ * gives the columns per expansion.  flags: DISORT_F_THERMAL (required),
+ * DISORT_F_DELTA_M_PLUS, DISORT_F_DIFFUSION_BC.  disort_jac_status copies
+ * the per-column DISORT_ST_* codes of the last execute (the first nonzero
+ * code among the column's solves) to a host array [n].
+ */
+typedef struct disort_jac_plan disort_jac_plan;
+int disort_jac_plan_create(disort_ctx* ctx, int64_t n, int32_t nstr, int32_t nlyr, int32_t pstride, int32_t flags,
+                           int64_t max_expanded, disort_jac_plan** plan);
+int disort_jac_plan_destroy(disort_ctx* ctx, disort_jac_plan* plan);
+int64_t disort_jac_plan_chunk(const disort_jac_plan* plan);
+disort_flux_plan* disort_jac_plan_flux(disort_jac_plan* plan);
+int disort_jac_execute(disort_ctx* ctx, disort_jac_plan* plan, const disort_flux_batch* device_batch, double* jac);
+int disort_jac_status(disort_ctx* ctx, disort_jac_plan* plan, int32_t* status_host);
+
+/*
  * Kernel variant of one stream-count bucket of a plan (no counterpart in the
  * reference: it selects between gfx950 implementations of the same
  * solve).  DISORT_V_LANE: register kernels, one lane per layer and 16 lanes
@@ -278,6 +303,9 @@ int disort_quadrature(int nstr, double* cmu, double* cwt);
  * temperatures: src/Planck.cpp:35-169 (planckFunction2), the same arithmetic
  * the kernels use.  Returns DISORT_E_INVALID for lo < 0, hi < lo or t < 0. */
 int disort_planck_band(double lo, double hi, int64_t n, const double* t, double* b);
+/* Its temperature derivative dB/dT (planckFunctionDeriv2, src/Planck.cpp:171-214)
+ * into d[k], the factor of the temperature Jacobians; same argument rules. */
+int disort_planck_band_deriv(double lo, double hi, int64_t n, const double* t, double* d);
 
 /* =========================================================================
  * General solver: DisortSolver::solve (src/DisortSolver.cpp:21-399) for a
@@ -472,6 +500,40 @@ int disort_surrogate_pad_index(disort_ctx* ctx, int64_t m, int64_t mpad, int32_t
 int disort_surrogate_monitor(disort_ctx* ctx, int64_t m, const int32_t* idx, const int32_t* flags, int nlev,
                              const double* out, int64_t total_levels, const double* pred, const disort_flux_batch* b,
                              double c_local, double c_floor, double tol, int32_t* counters);
+/* Jacobian surrogate (no counterpart in the reference): the temperature
+ * Jacobians of the top-of-atmosphere flux_up and the surface flux_down
+ * (rows (0, level 0) and (1, level nlyr) of disort_flux_jacobian's layout)
+ * for uniform columns, D = nlyr + 2 degrees of freedom each.  The network
+ * learns the responses R_k = dF/dB_k / pi, which depend on the optical
+ * properties only; dF/dT_k = pi R_k dB/dT(T_k) with dB/dT evaluated
+ * exactly.  Features: 3*nlyr + 1 + 6*(nlyr+1) per column (log10 dtau,
+ * albedo, g per layer; surface albedo; per level log10 of the optical and
+ * absorption depths, their transmissions exp(-x/0.6) from the top and to
+ * the surface), standardised into BF16 rows of width w0 as
+ * disort_surrogate_features.  Targets: [k] R_k of the top flux_up, [D+k]
+ * R_k of the surface flux_down, from a device Jacobian jac
+ * [n][6][nlyr+1][nlyr+2] (disort_jac_execute).  Decode writes jrows
+ * [n][2][D] (dF/dT of the two fluxes) and the members' spread max_k
+ * |R_1 - R_2|.  The monitor compares the sampled columns that pass the gate
+ * (flags bits 1 and 2) among the m routed idx[k] with the sub-batch
+ * Jacobian jac_sub [m][6][nlyr+1][nlyr+2]: counters[0] = sampled columns,
+ * counters[1] = those whose max over both rows and all k of
+ * |dF/dT_k - ref| / (pi dB/dT(T_k)) is not <= tol.  Scatter copies the two
+ * rows of jac_sub column k into jrows row idx[k]. */
+int disort_jac_surrogate_features(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlyr, int pstride,
+                                  const disort_flux_batch* batch, const float* shift, const float* scale,
+                                  const float* lo, const float* hi, int w0, uint16_t* x, int32_t* ood);
+int disort_jac_surrogate_targets(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlyr, const double* jac,
+                                 const disort_flux_batch* batch, const float* shift, const float* scale, int wL,
+                                 float* targets);
+int disort_jac_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const float* y1, const float* y2,
+                                const float* shift, const float* scale, const disort_flux_batch* batch,
+                                double* jrows, float* spread);
+int disort_jac_surrogate_monitor(disort_ctx* ctx, int64_t m, const int32_t* idx, const int32_t* flags, int nlyr,
+                                 const double* jac_sub, const double* jrows, const disort_flux_batch* batch,
+                                 double tol, int32_t* counters);
+int disort_jac_surrogate_scatter(disort_ctx* ctx, int64_t m, const int32_t* idx, int nlyr, const double* jac_sub,
+                                 double* jrows);
 /* dst row k <- src row idx[k] of a row-major matrix of 2-, 4- or 8-byte elements. */
 int disort_rows_gather(disort_ctx* ctx, int64_t m, const int32_t* idx, int64_t row_elems, int elem_bytes,
                        const void* src, void* dst);

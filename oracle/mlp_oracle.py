@@ -15,7 +15,9 @@ finite differences of the float64 forward pass; GELU and its derivative
 against closed-form values; Adam against its textbook recurrence on scalars;
 the column features against hand-computed layouts and the level features
 against closed forms (optical depth sums, Beer's law, half the optical depth
-at single-scattering albedo 0.5).
+at single-scattering albedo 0.5); the emission features of the Jacobian
+surrogate against the same sums and the invariant that the transmissions
+from the top and to the surface multiply to the whole column's.
 """
 from __future__ import annotations
 
@@ -163,3 +165,20 @@ def thermal_features(b_levels, b_bottom, albedo, mu0f0):
     beam mu0 F0 (b_levels [n, L+1], b_bottom [n]: band Planck values)."""
     u = np.where(mu0f0 > 1e-300, mu0f0, 1.0)
     return np.concatenate([b_levels / u[:, None], ((1.0 - albedo) * b_bottom / u)[:, None]], axis=1)
+
+
+def emission_features(dtau, ssa, g, albedo, mubar=0.6):
+    """Emission features of the Jacobian surrogate [n, 3L + 1 + 6(L+1)]:
+    log10(dtau), omega, g per layer, the surface albedo, then per level v
+    log10(tau_v + 1e-3), log10(a_v + 1e-3), exp(-tau_v / mubar),
+    exp(-a_v / mubar), exp(-(tau_L - tau_v) / mubar), exp(-(a_L - a_v) / mubar),
+    a_v = sum_{l<v} (1 - omega_l) dtau_l the absorption depth."""
+    n, L = dtau.shape
+    tau = np.zeros((n, L + 1))
+    ab = np.zeros((n, L + 1))
+    for v in range(1, L + 1):
+        tau[:, v] = tau[:, v - 1] + dtau[:, v - 1]
+        ab[:, v] = ab[:, v - 1] + (1.0 - ssa[:, v - 1]) * dtau[:, v - 1]
+    lev = [np.log10(tau + 1e-3), np.log10(ab + 1e-3), np.exp(-tau / mubar), np.exp(-ab / mubar),
+           np.exp(-(tau[:, L:] - tau) / mubar), np.exp(-(ab[:, L:] - ab) / mubar)]
+    return np.concatenate([np.log10(dtau), ssa, g, np.asarray(albedo).reshape(n, 1)] + lev, axis=1)

@@ -1,0 +1,176 @@
+"""Jacobian surrogate (disort_amd/jac_surrogate.py) on the GPU: the emission
+features against the oracle's restatement, the response targets and the
+decode against the plain GPU Jacobian (itself checked against the oracle in
+test_gpu_jacobian_device.py) and the host dB/dT, and a small end-to-end
+train / calibrate / serve run of the gate with its online monitor.
+
+Tolerances: features at most 1 BF16 ulp from the oracle's (the same float32
+standardisation of double features); targets to float32 rounding (rtol
+1e-6); decoded rows of the targets to float32 rounding of R / pi; routed
+columns bit for bit the plain Jacobian."""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from disort_amd._capi import Context
+
+    return Context(0)
+
+
+def test_jacobian_column_kernels(ctx):
+    from disort_amd import jac_surrogate as J
+    from disort_amd import surrogate as S
+
+    l = J._jlib()
+    n, L = 100, S.NLYR
+    hb = S.c5_columns(n, seed=21)
+    print("[jac surrogate] column kernels, 100 columns", flush=True)
+    cols = J.JacColumns(ctx, hb)
+    cols.plain()
+    ref_rows = cols.plain_rows_host()
+    assert not cols.plan.status().any()
+    feats = J.emission_features(hb)
+    want_f = mo.emission_features(hb.dtau.reshape(n, L), hb.ssa.reshape(n, L), hb.pmom.reshape(n, L, -1)[:, :, 1],
+                                  hb.scal.reshape(n, 12)[:, 7])
+    np.testing.assert_allclose(feats, want_f, rtol=1e-12, atol=1e-15)
+    raw = J.raw_targets(ctx, cols)
+    dbdt = J.dbdt_dofs(hb)
+    want_t = (ref_rows / (np.pi * dbdt[:, None, :])).reshape(n, -1)
+    np.testing.assert_allclose(raw, want_t, rtol=1e-6, atol=1e-7 * np.abs(want_t).max())
+    norm = S.Normalizer.fit(feats, raw, w0=J.JWIDTHS[0], wl=J.JWIDTHS[-1])
+    dn = norm.device(ctx)
+    n_pad, w0, wl = 128, J.JWIDTHS[0], J.JWIDTHS[-1]
+    x = S.DevArray(ctx, n_pad * w0 * 2)
+    ood = S.DevArray(ctx, n * 4)
+    assert l.disort_jac_surrogate_features(ctx.ptr, n, n_pad, L, 17, C.byref(cols.cstruct), dn["f_shift"].ptr,
+                                           dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, w0, x.ptr, ood.ptr) == 0
+    xs = x.get(np.uint16, (n_pad, w0))
+    nf = J.N_EFEAT
+    want = mo.bf16_bits(((want_f.astype(np.float32) - norm.f_shift[:nf]) * norm.f_scale[:nf]).astype(np.float32))
+    assert np.abs(xs[:n, :nf].astype(np.int32) - want.astype(np.int32)).max() <= 1
+    assert (xs[n:] == 0).all() and (xs[:, nf:] == 0).all()
+    assert (ood.get(np.int32, n) == 0).all()  # the columns define the range
+    # An out-of-range column is flagged.
+    lo = norm.f_lo.copy()
+    lo[5] = np.float32(feats[7, 5] + 1e-3)
+    dlo = S.DevArray.of(ctx, lo)
+    assert l.disort_jac_surrogate_features(ctx.ptr, n, n_pad, L, 17, C.byref(cols.cstruct), dn["f_shift"].ptr,
+                                           dn["f_scale"].ptr, dlo.ptr, dn["f_hi"].ptr, w0, x.ptr, ood.ptr) == 0
+    od = ood.get(np.int32, n)
+    assert od[7] == 1 and od.sum() == (feats[:, 5].astype(np.float32) < lo[5]).sum()
+    # Standardised targets, decoded (one member and two equal members), give the plain rows.
+    t = S.DevArray(ctx, n_pad * wl * 4)
+    assert l.disort_jac_surrogate_targets(ctx.ptr, n, n_pad, L, cols.jac.ptr, C.byref(cols.cstruct),
+                                          dn["t_shift"].ptr, dn["t_scale"].ptr, wl, t.ptr) == 0
+    ts = t.get(np.float32, (n_pad, wl))
+    np.testing.assert_allclose(ts[:n, :J.N_JOUT], (raw - norm.t_shift[:J.N_JOUT]) * norm.t_scale[:J.N_JOUT],
+                               rtol=1e-5, atol=1e-5)
+    assert (ts[n:] == 0).all() and (ts[:, J.N_JOUT:] == 0).all()
+    rows, spread = S.DevArray(ctx, n * 2 * J.D * 8), S.DevArray(ctx, n * 4)
+    for y2 in (None, t.ptr):
+        assert l.disort_jac_surrogate_decode(ctx.ptr, n, L, wl, t.ptr, y2, dn["t_shift"].ptr, dn["t_scale"].ptr,
+                                             C.byref(cols.cstruct), rows.ptr, spread.ptr) == 0
+        got = rows.get(np.float64, (n, 2, J.D))
+        assert J.jacobian_error(got, ref_rows, dbdt).max() < 1e-5
+        assert (spread.get(np.float32, n) == 0).all()
+    # Two members differing by a known amount in one output: spread = that amount in R / pi.
+    t2 = ts.copy()
+    t2[:n, 3] += np.float32(0.5)
+    dt2 = S.DevArray.of(ctx, t2)
+    assert l.disort_jac_surrogate_decode(ctx.ptr, n, L, wl, t.ptr, dt2.ptr, dn["t_shift"].ptr, dn["t_scale"].ptr,
+                                         C.byref(cols.cstruct), rows.ptr, spread.ptr) == 0
+    np.testing.assert_allclose(spread.get(np.float32, n), 0.5 / norm.t_scale[3], rtol=1e-5)
+    # Gather + plain Jacobian + scatter of a subset reproduces those columns' rows.
+    idx = np.array([3, 17, 42, 99], np.int32)
+    sub = J._JacSub(ctx, idx.size)
+    didx = S.DevArray.of(ctx, idx)
+    assert l.disort_surrogate_gather(ctx.ptr, idx.size, didx.ptr, L, 17, C.byref(cols.cstruct),
+                                     C.byref(sub.cstruct)) == 0
+    sub.solve()
+    r2 = S.DevArray.of(ctx, np.zeros((n, 2, J.D)))
+    assert l.disort_jac_surrogate_scatter(ctx.ptr, idx.size, didx.ptr, L, sub.jac.ptr, r2.ptr) == 0
+    g = r2.get(np.float64, (n, 2, J.D))
+    np.testing.assert_array_equal(g[idx], ref_rows[idx])
+    assert (g[np.setdiff1d(np.arange(n), idx)] == 0).all()
+    for a in (x, ood, dlo, t, rows, spread, dt2, didx, r2):
+        a.free()
+    sub.close()
+    cols.close()
+
+
+def test_jacobian_gate_end_to_end(ctx):
+    """Train two members on 16k columns' plain Jacobians, calibrate, serve 3k
+    fresh columns: routed columns carry the plain Jacobian rows bit for bit,
+    the trained network beats the training mean by a clear margin, the
+    monitor counts the sampled passing columns, and a gate with a bound no
+    column meets trips and then routes every column."""
+    from disort_amd import jac_surrogate as J
+    from disort_amd import surrogate as S
+
+    print("[jac surrogate] dataset 16384 columns", flush=True)
+    x, t, n, norm = J.build_jac_dataset(ctx, 16384, seed=5, chunk=8192)
+    cal = J.JacColumns(ctx, S.c5_columns(1024, seed=91))
+    cal.plain()
+    cal_ref = cal.plain_rows_host()
+    mean_net = S.MLP(ctx, J.JWIDTHS, max_batch=4096)
+    mean_net.set_params(np.zeros(mean_net.n_param, np.float32))
+    base_cal, base_err = J.JacGatedSurrogate(ctx, [mean_net, mean_net], norm).calibrate(cal, cal_ref)
+    print("[jac surrogate] training 2 members", flush=True)
+    members = [J.train_member(ctx, x, t, n, epochs=12, seed=200 + k, batch=512, max_batch=4096)[0] for k in range(2)]
+    gs = J.JacGatedSurrogate(ctx, members, norm, alpha=0.05, tol=5e-2)
+    after, err_cal = gs.calibrate(cal, cal_ref)
+    assert after["err_median_all"] < 0.5 * base_cal["err_median_all"], (base_cal, after)
+    fresh = S.c5_columns(3000, seed=92)
+    dc = J.JacColumns(ctx, fresh)
+    dc.plain()
+    ref = dc.plain_rows_host()
+    print("[jac surrogate] serve 3000 columns", flush=True)
+    rows, info = gs.serve(dc, sample_frac=0.03, seed=7)
+    got = rows.get(np.float64, (dc.n, 2, J.D))
+    fl = info["flags"].get(np.int32, dc.n)
+    routed = np.nonzero(fl & 1)[0]
+    assert routed.size == info["n_plain"]
+    assert ((fl & 1) != 0).tolist() == (((fl & 2) != 0) | ((fl & 4) == 0)).tolist()
+    assert 0.015 < ((fl & 2) != 0).mean() < 0.045
+    np.testing.assert_array_equal(got[routed], ref[routed])
+    err = J.jacobian_error(got, ref, J.dbdt_dofs(fresh))
+    smp = (fl & 6) == 6
+    assert info["monitor_sampled"] == int(smp.sum())
+    # The monitor scores the surrogate's prediction on the sampled passing
+    # columns: recompute those errors from the prediction alone.
+    pred, _, _ = gs.predict(dc)
+    perr = J.jacobian_error(pred.get(np.float64, (dc.n, 2, J.D)), ref, J.dbdt_dofs(fresh))
+    assert info["monitor_violations"] == int((perr[smp] > gs.tol).sum())
+    acc = np.nonzero((fl & 1) == 0)[0]
+    np.testing.assert_array_equal(err[acc], perr[acc])
+    if acc.size > 100:
+        assert (err[acc] > gs.tol).mean() < 0.15
+
+    strict = J.JacGatedSurrogate(ctx, members, norm, tau=np.inf, alpha=0.01, tol=1e-9)
+    strict.monitor.min_samples = 50
+    r1, i1 = strict.serve(dc, sample_frac=0.05, seed=8)
+    assert i1["monitor_sampled"] >= 50 and i1["monitor_violations"] == i1["monitor_sampled"]
+    assert not i1["tripped"] and strict.monitor.tripped
+    r2, i2 = strict.serve(dc, sample_frac=0.05, seed=9)
+    assert i2["tripped"] and i2["n_plain"] == dc.n
+    np.testing.assert_array_equal(r2.get(np.float64, (dc.n, 2, J.D)), ref)
+    for a in (rows, info["flags"], pred, r1, r2, i1["flags"], i2["flags"], x, t):
+        a.free()
+    strict.close()
+    gs.close()
+    for m in members + [mean_net]:
+        m.close()
+    cal.close()
+    dc.close()

@@ -70,6 +70,28 @@ def test_planck_band_matches_oracle():
             _capi.planck_band(*bad)
 
 
+def test_planck_band_deriv_matches_oracle():
+    """The host dB/dT entry point agrees with the oracle's derivative (pinned
+    to the reference's planckFunctionDeriv2) to rounding, and with a central
+    difference of the band integral."""
+    import sys
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    from disort_amd import _capi
+
+    t = np.array([150.0, 200.0, 255.5, 300.0, 1000.0])
+    for lo, hi in ((1000.0, 1001.0), (100.0, 101.0), (10.0, 2500.0)):
+        d = _capi.planck_band_deriv(lo, hi, t)
+        ref = np.array([ol.planck_deriv(lo, hi, tk) for tk in t])
+        np.testing.assert_allclose(d, ref, rtol=1e-13)
+        h = 1e-3
+        fd = (_capi.planck_band(lo, hi, t + h) - _capi.planck_band(lo, hi, t - h)) / (2 * h)
+        np.testing.assert_allclose(d, fd, rtol=1e-5)
+    with pytest.raises(_capi.DisortInvalid):
+        _capi.planck_band_deriv(1.0, 2.0, [-3.0])
+
+
 def test_facade_module_loads():
     import disort_amd.disortpp as d
 

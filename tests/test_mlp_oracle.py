@@ -95,3 +95,27 @@ def test_level_features_closed_forms():
     # absorption depth: half the optical depth at omega = 0.5; 0.1, 0.1, 0.325 in the second column
     np.testing.assert_allclose(10.0 ** f[0, 8:12] - 1e-3, [0.0, 5e-5, 0.50005, 5.50005], rtol=1e-12, atol=1e-15)
     np.testing.assert_allclose(10.0 ** f[1, 8:12] - 1e-3, [0.0, 0.1, 0.1, 0.325], rtol=1e-12, atol=1e-15)
+
+
+def test_emission_features_closed_forms():
+    dtau = np.array([[1e-4, 1.0, 10.0], [0.1, 0.2, 0.3]])
+    ssa = np.array([[0.5, 0.5, 0.5], [0.0, 1.0, 0.25]])
+    g = np.full((2, 3), 0.3)
+    f = mo.emission_features(dtau, ssa, g, np.array([0.1, 0.2]))
+    assert f.shape == (2, 3 * 3 + 1 + 6 * 4)
+    np.testing.assert_allclose(f[0, :3], [-4.0, 0.0, 1.0])
+    assert f[1, 9] == 0.2 and (f[:, 6:9] == 0.3).all()
+    b = 10
+    tau0 = np.array([0.0, 1e-4, 1.0001, 11.0001])
+    np.testing.assert_allclose(10.0 ** f[0, b:b + 4] - 1e-3, tau0, rtol=1e-12, atol=1e-15)
+    # absorption depth: half the optical depth at omega = 0.5; 0.1, 0.1, 0.325 in the second column
+    np.testing.assert_allclose(10.0 ** f[0, b + 4:b + 8] - 1e-3, tau0 / 2, rtol=1e-12, atol=1e-15)
+    np.testing.assert_allclose(10.0 ** f[1, b + 4:b + 8] - 1e-3, [0.0, 0.1, 0.1, 0.325], rtol=1e-12, atol=1e-15)
+    # diffuse transmissions: exp(-x / 0.6) from the top, to the surface
+    np.testing.assert_allclose(f[0, b + 8:b + 12], np.exp(-tau0 / 0.6), rtol=1e-12)
+    np.testing.assert_allclose(f[1, b + 12:b + 16], np.exp(-np.array([0.0, 0.1, 0.1, 0.325]) / 0.6), rtol=1e-12)
+    # top and surface transmissions multiply to the whole column's, at every level
+    for q in (0, 1):
+        top, bot = f[:, b + (8 + 4 * q):b + (12 + 4 * q)], f[:, b + (16 + 4 * q):b + (20 + 4 * q)]
+        np.testing.assert_allclose(top * bot, top[:, -1:] * np.ones((1, 4)), rtol=1e-12)
+    assert (f[:, b + 19] == 1.0).all() and (f[:, b + 23] == 1.0).all() and (f[:, b + 8] == 1.0).all()

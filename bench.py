@@ -411,6 +411,236 @@ def bench_c5(args) -> dict:
     return line
 
 
+def bench_jac(args) -> dict:
+    """Learned temperature Jacobians (the surrogate extension): two members
+    (547 emission features -> 512 x 4 -> 124, BF16 MFMA) learn the responses
+    of the top-of-atmosphere upward flux and the surface downward flux to
+    every level's Planck value from solver-generated Jacobians; the gate
+    (OOD + spread + sampled re-solve, online monitor) routes failing columns
+    to the plain GPU Jacobian (disort_jac_execute).  Prints the gated
+    throughput at the 1e-2 bound (error relative to pi dB/dT at each
+    temperature) against the plain GPU Jacobian on the same columns, a
+    tolerance sweep, and the CPU oracle's Jacobian on a bounded sample."""
+    import numpy as np
+
+    from disort_amd import jac_surrogate as J
+    from disort_amd import surrogate as S
+    from disort_amd._capi import Context
+
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    x, t, n, norm = J.build_jac_dataset(ctx, args.jac_train_columns, seed=1, chunk=16384)
+    ctx.sync()
+    gen_s = time.perf_counter() - t0
+    print(f"dataset {n} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
+    members, train_s, hist = [], [], []
+    for k in range(2):
+        t0 = time.perf_counter()
+        m, h = J.train_member(ctx, x, t, n, args.jac_epochs, seed=300 + k, batch=2048, lr=1e-3, max_batch=65536)
+        ctx.sync()
+        train_s.append(time.perf_counter() - t0)
+        hist.append(h[-1])
+        members.append(m)
+        print(f"member {k}: {train_s[-1]:.1f} s, final loss {h[-1]:.3e}", file=sys.stderr, flush=True)
+    x.free()
+    t.free()
+    W = J.JWIDTHS
+    mlp_flops = 2.0 * (J.N_EFEAT * 512 + 3 * 512 * 512 + 512 * J.N_JOUT)
+    train_tflops = 3.0 * mlp_flops * n * args.jac_epochs / (sum(train_s) / 2) / 1e12
+
+    cal = J.JacColumns(ctx, S.c5_columns(16384, seed=501))
+    cal.plain()
+    cal_ref = cal.plain_rows_host()
+    gate = J.JacGatedSurrogate(ctx, members, norm, alpha=0.01, tol=J.JTOL)
+    cal_info, cal_err = gate.calibrate(cal, cal_ref)
+    sweep = []
+    for tol in (3e-2, 1e-2, 5e-3, 1e-3):
+        g = J.JacGatedSurrogate(ctx, members, norm, alpha=0.01, tol=tol)
+        sweep.append((tol, g, g.calibrate(cal, cal_ref)[0]))
+    cal.close()
+
+    chunk = 65536
+    wdc = J.JacColumns(ctx, S.c5_columns(chunk, seed=4999))
+    for g_ in [gate] + [g for _, g, _ in sweep]:
+        r_, i_ = g_.serve(wdc, sample_frac=args.sample_frac, seed=4999)
+        r_.free()
+        i_["flags"].free()
+    wdc.close()
+    n_serve = args.jac_serve_columns
+    plain_s = gated_s = mlp_s = 0.0
+    errs, acc_all, n_plain, n_sampled = [], 0, 0, 0
+    sweep_s = [0.0] * len(sweep)
+    sweep_acc = [0] * len(sweep)
+    sweep_viol = [0] * len(sweep)
+    done = k = 0
+    while done < n_serve:
+        m_ = min(chunk, n_serve - done)
+        hb = S.c5_columns(m_, seed=5000 + k)
+        dc = J.JacColumns(ctx, hb)
+        dc.plain()
+        ctx.sync()
+        _, dt = S.timed(ctx, dc.plain)
+        plain_s += dt
+        ref = dc.plain_rows_host()
+        db = J.dbdt_dofs(hb)
+        (r_, sp_, od_), dt = S.timed(ctx, lambda: gate.predict(dc))
+        mlp_s += dt
+        for a in (r_, sp_, od_):
+            a.free()
+        (rows, info), dt = S.timed(ctx, lambda: gate.serve(dc, sample_frac=args.sample_frac, seed=k))
+        gated_s += dt
+        got = rows.get(np.float64, (m_, 2, J.D))
+        fl = info["flags"].get(np.int32, m_)
+        rows.free()
+        info["flags"].free()
+        e = J.jacobian_error(got, ref, db)
+        acc = np.nonzero((fl & 1) == 0)[0]
+        errs.append(e[acc])
+        acc_all += acc.size
+        n_plain += info["n_plain"]
+        n_sampled += int(((fl & 2) != 0).sum())
+        for j, (tol, g, _) in enumerate(sweep):
+            (r2, i2), dt = S.timed(ctx, lambda: g.serve(dc, sample_frac=args.sample_frac, seed=k))
+            sweep_s[j] += dt
+            got2 = r2.get(np.float64, (m_, 2, J.D))
+            acc2 = np.nonzero((i2["flags"].get(np.int32, m_) & 1) == 0)[0]
+            r2.free()
+            i2["flags"].free()
+            sweep_acc[j] += acc2.size
+            sweep_viol[j] += int((J.jacobian_error(got2, ref, db)[acc2] > tol).sum())
+        dc.close()
+        done += m_
+        k += 1
+        print(f"served {done}/{n_serve}", file=sys.stderr, flush=True)
+    e_all = np.concatenate(errs) if errs else np.zeros(0)
+    plain_rate, gated_rate, mlp_rate = n_serve / plain_s, n_serve / gated_s, n_serve / mlp_s
+
+    rows_ = 65536
+    xb = S.DevArray(ctx, rows_ * W[0] * 2)
+    yb = S.DevArray(ctx, rows_ * W[-1] * 4)
+    ctx.h2d(xb.ptr, np.zeros(rows_ * W[0], np.uint16))
+    for m in members:
+        m.infer(xb.ptr, rows_, yb.ptr)
+    ctx.sync()
+    t0 = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        for m in members:
+            m.infer(xb.ptr, rows_, yb.ptr)
+    ctx.sync()
+    gemm_s = (time.perf_counter() - t0) / (reps * len(members))
+    pad_flops = 2.0 * rows_ * sum(a * b for a, b in zip(W[:-1], W[1:]))
+    xb.free()
+    yb.free()
+
+    line = {
+        "metric": "gated Jacobian surrogate vs plain GPU temperature Jacobian, columns/sec at <=1e-2 error "
+                  "(nlyr=60, nstr=16)",
+        "value": gated_rate,
+        "unit": "columns/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "bf16 (MLP, fp32 accumulate) + f64 (plain fallback)",
+        "data": "synthetic (seeded Philox draws; C5 column distributions)",
+        "config": {"workload": f"Jacobian surrogate: train {n} columns x {args.jac_epochs} epochs x 2 members, "
+                               f"serve {n_serve} fresh columns",
+                   "outputs": "dF/dT_k of flux_up at the top and flux_down at the surface, k = 61 levels + surface",
+                   "mlp": f"{J.N_EFEAT} emission features -> 512x4 (GELU) -> {J.N_JOUT}",
+                   "gate": "OOD range + 2-member spread + sampled re-solve (online error monitor), routed on the device",
+                   "error_metric": "max_k |dF/dT_k - ref| / (pi dB/dT(T_k)) over both rows", "tol": J.JTOL,
+                   "sample_frac": args.sample_frac},
+        "plain_gpu_columns_per_s": plain_rate,
+        "plain_gpu_solves_per_s": plain_rate * J.D,
+        "gated_vs_plain": gated_rate / plain_rate,
+        "accepted_frac": acc_all / n_serve,
+        "plain_routed_frac": n_plain / n_serve,
+        "sampled_frac": n_sampled / n_serve,
+        "accepted_err_quantiles": ({p: float(np.quantile(e_all, p / 100)) for p in (50, 90, 99)}
+                                   if e_all.size else None),
+        "accepted_violation_frac": float((e_all > J.JTOL).mean()) if e_all.size else None,
+        "calibration": cal_info,
+        "calibration_err_quantiles_all": {p: float(np.quantile(cal_err, p / 100)) for p in (50, 90, 99, 99.9)},
+        "monitor": gate.monitor.stats(),
+        "tolerance_sweep": [
+            {"tol": tol, "calibration_accept_frac": inf["accept_frac"], "served_accept_frac": sweep_acc[j] / n_serve,
+             "accepted_violation_frac": sweep_viol[j] / max(sweep_acc[j], 1),
+             "columns_per_s": n_serve / sweep_s[j], "vs_plain": (n_serve / sweep_s[j]) / plain_rate}
+            for j, (tol, _, inf) in enumerate(sweep)],
+        "mlp_inference_columns_per_s": mlp_rate,
+        "roofline": {"bound": "mfma", "kernel": "k_gemm (disort_mlp_infer, 65,536 rows, padded widths "
+                     + "x".join(str(w) for w in W) + ")",
+                     "achieved": pad_flops / gemm_s / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": pad_flops / gemm_s / 1e12 / 2500.0, "traffic": None},
+        "training": {"columns": n, "epochs": args.jac_epochs, "seconds_per_member": train_s, "final_loss": hist,
+                     "tflops": train_tflops, "dataset_seconds": gen_s},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = jac_cpu_baseline(ctx, args.jac_cpu_sample)
+    gate.close()
+    for _, g, _ in sweep:
+        g.close()
+    for m in members:
+        m.close()
+    return line
+
+
+def jac_cpu_baseline(ctx, sample: int) -> dict:
+    """The oracle's temperature Jacobian (the analytic linearisation of
+    DisortFluxSolver::computeThermalJacobian, src/FluxSolver.cpp:1294-1497)
+    on `sample` fresh columns, one column per task on all host threads, and
+    the plain GPU Jacobian of the same columns against it."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    from disort_amd import jac_surrogate as J
+    from disort_amd import surrogate as S
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+
+    hb = S.c5_columns(sample, seed=9001)
+    threads = cpu_threads()
+    oracle_lib.flux_jacobian_one(hb, 0)
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        ref = list(ex.map(lambda i: oracle_lib.flux_jacobian_one(hb, i)[1], range(sample)))
+    sec = time.perf_counter() - t0
+    ref = np.stack(ref)
+    cols = J.JacColumns(ctx, hb)
+    cols.plain()
+    ctx.sync()
+    got = cols.jac.get(np.float64, (sample, 6, S.NLEV, J.D))
+    cols.close()
+    # The parity bar of the Jacobian tests: 1e-9 relative with an absolute
+    # floor of 1e-12 of the column's largest derivative (ratio <= 1 passes).
+    bar = 1e-9 * np.abs(ref) + 1e-12 * np.abs(ref).max(axis=(1, 2, 3), keepdims=True)
+    ratio = np.abs(got - ref) / bar
+    # Entries outside it are arbitrated as in the tests: against the same
+    # linearisation in x87 extended precision x, within the bar of x or no
+    # further from x than 4 times the FP64 oracle's largest error on that
+    # degree of freedom (thin deep layers make the algorithm ill-conditioned).
+    bad_cols = np.nonzero((ratio > 1.0).any(axis=(1, 2, 3)))[0]
+    still = 0
+    for i in bad_cols:
+        x = oracle_lib.flux_jacobian_one_extended(hb, int(i))[1]
+        col_err = np.abs(ref[i] - x).max(axis=(0, 1), keepdims=True)
+        fl = 1e-12 * np.abs(ref[i]).max()
+        ok = np.abs(got[i] - x) <= np.maximum(1e-9 * np.abs(x) + fl, 4.0 * col_err)
+        still += int(((ratio[i] > 1.0) & ~ok).sum())
+    big = np.abs(ref) > 1e-6 * np.abs(ref).max(axis=(1, 2, 3), keepdims=True)
+    return {"value": sample / sec, "unit": "columns/s", "cores": threads, "kind": "port",
+            "sample": f"{sample} fresh columns; the oracle's Jacobian (CPU restatement of "
+                      f"computeThermalJacobian), one column per task on {threads} threads on {cpu_model()}",
+            "plain_gpu_vs_oracle": {"bar": "|gpu - oracle| <= 1e-9 |oracle| + 1e-12 max|oracle of the column|",
+                                    "frac_entries_within": float((ratio <= 1.0).mean()),
+                                    "max_ratio_to_bar": float(ratio.max()),
+                                    "columns_arbitrated_extended": int(bad_cols.size),
+                                    "entries_failing_after_arbitration": still,
+                                    "max_rel_of_entries_above_1e-6_of_column_max": float(
+                                        (np.abs(got - ref)[big] / np.abs(ref)[big]).max())}}
+
+
 def bench_mlp(args) -> dict:
     """The surrogate network's inference GEMMs alone (disort_mlp_infer: five
     k_gemm launches per call, widths 512 x 512 x 512 x 512 x 512 x 192) on
@@ -555,9 +785,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--bands", type=int, default=10000)
-    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "mlp"), default="c2",
+    ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5", "jac", "mlp"), default="c2",
                     help="c2: the headline spectral flux solve (default); c3: the general solver with "
                          "intensities; c4: ragged autotuned batch; c5: learned surrogate behind the error gate; "
+                         "jac: learned temperature Jacobians behind the same gate; "
                          "mlp: the surrogate network's inference GEMMs alone")
     ap.add_argument("--columns", type=int, default=4096, help="C3 columns")
     ap.add_argument("--cpu-columns", type=int, default=64, help="C3 columns timed on the CPU")
@@ -565,6 +796,10 @@ def main() -> int:
     ap.add_argument("--epochs", type=int, default=100, help="C5 training epochs per member")
     ap.add_argument("--serve-columns", type=int, default=1048576, help="C5 fresh columns served")
     ap.add_argument("--sample-frac", type=float, default=0.01, help="C5 sampled re-solve fraction")
+    ap.add_argument("--jac-train-columns", type=int, default=2097152, help="Jacobian surrogate training columns")
+    ap.add_argument("--jac-epochs", type=int, default=150, help="Jacobian surrogate epochs per member")
+    ap.add_argument("--jac-serve-columns", type=int, default=262144, help="Jacobian surrogate columns served")
+    ap.add_argument("--jac-cpu-sample", type=int, default=2048, help="columns of the oracle Jacobian timed")
     ap.add_argument("--instances", type=int, default=50000, help="C4 batch size")
     ap.add_argument("--cpu-sample", type=int, default=2000, help="C4 instances timed on the CPU")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -592,6 +827,9 @@ def main() -> int:
         return 0
     if args.workload == "c5":
         print(json.dumps(bench_c5(args)))
+        return 0
+    if args.workload == "jac":
+        print(json.dumps(bench_jac(args)))
         return 0
 
     from disort_amd import flops, workloads

@@ -40,6 +40,7 @@ N_JOUT = 2 * D                         # 124 outputs: top flux_up and surface fl
 JWIDTHS = (576, 512, 512, 512, 512, 128)
 MU_BAR = 0.6
 JTOL = 1e-2
+T_FLOOR = 0.1  # target scale floor (R / pi units): the loss weighs absolute errors alike, as the gate's metric does
 
 
 def _jlib():
@@ -175,14 +176,22 @@ def build_jac_dataset(ctx, n: int, seed: int, norm: Normalizer | None = None, ch
     x = DevArray(ctx, n_pad * w0 * 2)
     t = DevArray(ctx, n_pad * wl * 4)
     dn = None
+    fit = norm is None
+    lo = hi = None
     done = k = 0
     while done < n:
         m = min(chunk, n - done)
         hb = c5_columns(m, seed=seed * 1000 + k)
         cols = JacColumns(ctx, hb)
         cols.plain()
-        if norm is None:
-            norm = Normalizer.fit(emission_features(hb), raw_targets(ctx, cols), margin=0.0, w0=w0, wl=wl)
+        if fit:
+            # Standardisation from the first chunk; the feature range (the
+            # gate's out-of-distribution box) from every training column.
+            f = emission_features(hb)
+            if norm is None:
+                norm = Normalizer.fit(f, raw_targets(ctx, cols), margin=0.0, w0=w0, wl=wl, t_floor=T_FLOOR)
+            lo = f.min(0) if lo is None else np.minimum(lo, f.min(0))
+            hi = f.max(0) if hi is None else np.maximum(hi, f.max(0))
         if dn is None:
             dn = norm.device(ctx)
         m_pad = m if done + m < n else n_pad - done
@@ -198,6 +207,9 @@ def build_jac_dataset(ctx, n: int, seed: int, norm: Normalizer | None = None, ch
         cols.close()
         done += m
         k += 1
+    if fit:
+        norm.f_lo[:N_EFEAT] = lo
+        norm.f_hi[:N_EFEAT] = hi
     return x, t, n, norm
 
 

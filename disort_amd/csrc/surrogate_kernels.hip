@@ -37,6 +37,10 @@ namespace {
 
 unsigned blocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 
+// Row length of the two-stream baseline array ts: U and D at the nlyr+1
+// levels, the level and surface Planck values, the layer asymmetry factors.
+__host__ __device__ constexpr int ts_row(int L) { return 3 * (L + 1) + 1 + L; }
+
 __device__ __forceinline__ double beam_unit(const double* scal, int64_t i) {
   const double u = scal[i * DISORT_NSCAL + DISORT_S_BEAM_MU] * scal[i * DISORT_NSCAL + DISORT_S_BEAM_FLUX];
   return u > 1e-300 ? u : 1.0;
@@ -49,9 +53,9 @@ __device__ __forceinline__ double beam_unit(const double* scal, int64_t i) {
 __global__ __launch_bounds__(256) void k_features(int64_t n, int64_t n_pad, int L, int P, const double* dtau,
                                                   const double* ssa, const double* pmom, const double* temper,
                                                   const double* scal, const float* shift, const float* scale,
-                                                  const float* lo, const float* hi, int w0, __bf16* x, int32_t* ood) {
+                                                  const float* lo, const float* hi, int w0, __bf16* x, int32_t* ood,
+                                                  const double* ts) {
   __shared__ double lev[4][4][64];  // [column][tau, absorption tau, transmission, B/(mu0 F0)][level]
-  __shared__ double lay[4][2][64];  // [column][dtau, (1 - omega) dtau][layer]
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * 4 + wv;
   if (i >= n_pad) return;
@@ -62,34 +66,47 @@ __global__ __launch_bounds__(256) void k_features(int64_t n, int64_t n_pad, int 
   }
   const int base = 4 * L + 2, nlev = L + 1;
   const int ext = (w0 >= base + 4 * nlev + 1) ? 4 * nlev + 1 : (w0 >= base + 3 * nlev) ? 3 * nlev : 0;
-  const int nf = base + ext;
+  // Two-stream baseline block (net and sum of its diffuse fluxes in beam units).
+  const int tsb = base + 4 * nlev + 1;
+  const bool with_ts = ts && ext == 4 * nlev + 1 && w0 >= tsb + 2 * nlev;
+  const int nf = base + ext + (with_ts ? 2 * nlev : 0);
+  const double* tsr = with_ts ? ts + i * ts_row(L) : nullptr;
   const double* sc = scal + i * DISORT_NSCAL;
   const double mu0 = sc[DISORT_S_BEAM_MU];
   const double u = beam_unit(scal, i);
   double (*lv)[64] = lev[wv];
-  double (*ly)[64] = lay[wv];
+  // Optical and absorption depth of level v = lane: exclusive prefix sums
+  // over the layers by a wavefront scan (L + 1 < 64).
+  double dsum = 0.0, asum = 0.0;
   if (lane < L) {
-    const double dt = dtau[i * L + lane];
-    ly[0][lane] = dt;
-    ly[1][lane] = (1.0 - ssa[i * L + lane]) * dt;
+    dsum = dtau[i * L + lane];
+    asum = (1.0 - ssa[i * L + lane]) * dsum;
   }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int v = lane; v <= nlev && ext > 0; v += 64) {  // v == nlev: the surface (Planck term only)
-    double tau = 0.0, tab = 0.0;
-    for (int l = 0; l < v && l < L; ++l) {
-      tau += ly[0][l];
-      tab += ly[1][l];
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const double td = __shfl_up(dsum, off), ta = __shfl_up(asum, off);
+    if (lane >= off) {
+      dsum += td;
+      asum += ta;
     }
-    if (v < nlev) {
+  }
+  const double tau_ex = __shfl_up(dsum, 1), tab_ex = __shfl_up(asum, 1);
+  {
+    const int v = lane;  // v == nlev: the surface (Planck term only)
+    const double tau = (lane > 0) ? tau_ex : 0.0, tab = (lane > 0) ? tab_ex : 0.0;
+    if (ext > 0 && v < nlev) {
       lv[0][v] = log10(tau + 1e-3);
       lv[1][v] = log10(tab + 1e-3);
       lv[2][v] = exp(-tau / mu0);
     }
-    if (ext > 3 * nlev) {
-      const double t = (v < nlev) ? temper[i * nlev + v] : sc[DISORT_S_TEMP_BOT];
-      const double bb = dsa::planck_band(sc[DISORT_S_WN_LO], sc[DISORT_S_WN_HI], t);
+    if (ext > 3 * nlev && v <= nlev) {
+      double bb;
+      if (with_ts) {
+        bb = tsr[2 * nlev + v];  // the baseline's level Planck values
+      } else {
+        const double t = (v < nlev) ? temper[i * nlev + v] : sc[DISORT_S_TEMP_BOT];
+        bb = dsa::planck_band(sc[DISORT_S_WN_LO], sc[DISORT_S_WN_HI], t);
+      }
       lv[3][v] = (v < nlev ? bb : (1.0 - sc[DISORT_S_ALBEDO]) * bb) / u;
     }
   }
@@ -103,11 +120,15 @@ __global__ __launch_bounds__(256) void k_features(int64_t n, int64_t n_pad, int 
     double f = 0.0;
     if (k < L) f = log10(dtau[i * L + k]);
     else if (k < 2 * L) f = ssa[i * L + (k - L)];
-    else if (k < 3 * L) f = pmom[(i * L + (k - 2 * L)) * P + 1];
+    else if (k < 3 * L) f = with_ts ? tsr[3 * nlev + 1 + (k - 2 * L)] : pmom[(i * L + (k - 2 * L)) * P + 1];
     else if (k < 4 * L) f = 0.5 * (temper[i * nlev + (k - 3 * L)] + temper[i * nlev + (k - 3 * L) + 1]);
     else if (k == 4 * L) f = mu0;
     else if (k == 4 * L + 1) f = sc[DISORT_S_ALBEDO];
-    else if (k < nf) {
+    else if (k >= tsb && k < nf) {
+      const int v = (k - tsb) % nlev;
+      const double up = tsr[v], dn = tsr[nlev + v];
+      f = ((k - tsb) < nlev ? dn - up : dn + up) / u;
+    } else if (k < nf) {
       const int blk = (k - base) / nlev, v = (k - base) % nlev;
       // blocks: optical depth, transmission, absorption depth, Planck (the last has nlev + 1 entries)
       f = (blk == 0) ? lv[0][v] : (blk == 1) ? lv[2][v] : (blk == 2) ? lv[1][v] : lv[3][k - base - 3 * nlev];
@@ -125,7 +146,7 @@ __global__ __launch_bounds__(256) void k_features(int64_t n, int64_t n_pad, int 
 }
 
 __global__ void k_targets(int64_t n, int64_t n_pad, int nlev, const double* out, int64_t tv, const double* scal,
-                          const float* shift, const float* scale, int wL, float* t) {
+                          const float* shift, const float* scale, int wL, float* t, const double* ts) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_pad * wL) return;
   const int64_t i = e / wL;
@@ -137,7 +158,11 @@ __global__ void k_targets(int64_t n, int64_t n_pad, int nlev, const double* out,
     const double dn = out[DISORT_O_FLUX_DOWN * tv + i * nlev + lv];
     const double dir = out[DISORT_O_FLUX_DIRECT_BEAM * tv + i * nlev + lv];
     const double u = beam_unit(scal, i);
-    const double x = (q == 0) ? (dn - up) / u : (q == 1 ? (dn + up) / u : dir / u);
+    double x = (q == 0) ? (dn - up) / u : (q == 1 ? (dn + up) / u : dir / u);
+    if (ts && q < 2) {
+      const double bu = ts[i * ts_row(nlev - 1) + lv], bd = ts[i * ts_row(nlev - 1) + nlev + lv];
+      x -= (q == 0 ? bd - bu : bd + bu) / u;  // residual over the two-stream baseline
+    }
     v = ((float)x - shift[k]) * scale[k];
   }
   t[e] = v;
@@ -153,7 +178,7 @@ __global__ void k_targets(int64_t n, int64_t n_pad, int nlev, const double* out,
 __global__ __launch_bounds__(256) void k_decode(int64_t n, int L, int wL, const float* y1, const float* y2,
                                                 const float* shift, const float* scale, const double* dtau,
                                                 const double* scal, double c_local, double c_floor, double* flux,
-                                                float* spread) {
+                                                float* spread, const double* ts) {
   // One wavefront per column, lane = level (nlev <= 64).
   const int lane = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -175,12 +200,16 @@ __global__ __launch_bounds__(256) void k_decode(int64_t n, int L, int wL, const 
     for (int l = 0; l < lv; ++l) tau += ly[l];  // the solver's summation order
     const double dir = (F0 > 0.0) ? mu0 * F0 * exp(-tau / mu0) : 0.0;
     const int ka = lv, ks = nlev + lv;
-    const double a1 = ((double)y1[i * wL + ka] / scale[ka] + shift[ka]) * u;  // diffuse net
-    const double s1 = ((double)y1[i * wL + ks] / scale[ks] + shift[ks]) * u;  // diffuse sum
+    // Two-stream baseline (zero without ts): the network predicts the residual.
+    const double* tr = ts ? ts + i * ts_row(nlev - 1) : nullptr;
+    const double bn = ts ? tr[nlev + lv] - tr[lv] : 0.0;
+    const double bs = ts ? tr[nlev + lv] + tr[lv] : 0.0;
+    const double a1 = ((double)y1[i * wL + ka] / scale[ka] + shift[ka]) * u + bn;  // diffuse net
+    const double s1 = ((double)y1[i * wL + ks] / scale[ks] + shift[ks]) * u + bs;  // diffuse sum
     double a = a1, sm = s1;
     if (y2) {
-      const double a2 = ((double)y2[i * wL + ka] / scale[ka] + shift[ka]) * u;
-      const double s2 = ((double)y2[i * wL + ks] / scale[ks] + shift[ks]) * u;
+      const double a2 = ((double)y2[i * wL + ka] / scale[ka] + shift[ka]) * u + bn;
+      const double s2 = ((double)y2[i * wL + ks] / scale[ks] + shift[ks]) * u + bs;
       a = 0.5 * (a1 + a2);
       sm = 0.5 * (s1 + s2);
       const double net = c_local * fabs(dir + a);
@@ -299,6 +328,247 @@ __global__ void k_monitor(int64_t m, const int32_t* idx, const int32_t* flags, i
 }
 
 // ---------------------------------------------------------------------------
+// Two-stream baseline of the flux surrogate: per column (one thread each),
+// delta-Eddington scaling (f = g^2), quadrature coefficients
+//   gamma1 = sqrt3 (2 - w(1+g))/2, gamma2 = sqrt3 w (1-g)/2,
+//   gamma3 = (1 - sqrt3 g mu0)/2, gamma4 = 1 - gamma3,
+// and per layer the exact solution of
+//   dU/dt = gamma1 U - gamma2 D - w F0 gamma3 e^{-tau/mu0} - (gamma1-gamma2) pi B(t)
+//   dD/dt = gamma2 U - gamma1 D + w F0 gamma4 e^{-tau/mu0} + (gamma1-gamma2) pi B(t)
+// (t the scaled depth in the layer, B linear in t): the homogeneous modes
+// A e^{lambda (t - dtau)} (1, Gamma) + B e^{-lambda t} (Gamma, 1), the beam
+// particular solution (mu0 moved by 0.1% within 1e-4 of resonance) and the
+// thermal one pi B(t) +- pi B' / (gamma1 + gamma2).  No diffuse flux enters
+// at the top; the surface is Lambertian with emission (1 - a) pi B_sfc.
+// The 2L constants follow from the interface continuity by elimination
+// with partial pivoting on each layer's 3 x 2 panel (the carried equation
+// and the two continuity equations), keeping x_l = M_l x_{l+1} + m_l for the
+// back substitution.  Output row i of ts (3 (L+1) + 1 doubles): U at the
+// levels, D at the levels (without the unscaled direct beam, as the solver
+// reports it), then the band Planck values B(T_v) of the levels and
+// B(T_sfc) (computed first by k_planck_levels, one thread per value, and
+// shared with the feature kernel).
+// work: 12 doubles per (layer, column), layout [k][layer][column] so that
+// the per-column threads of a wavefront touch consecutive addresses: M_l,
+// m_l, Gamma_l, E_l, the particular solutions at the layer top and the
+// scaled and unscaled depths of the layer top; the back substitution
+// overwrites slots 8-9 of layer l with U, D of level l (slots 0-1 of the
+// last layer with those of the surface level), and k_ts_pack moves them
+// into the rows of ts through LDS.
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kSqrt3 = 1.7320508075688772;
+
+struct TsLayer {
+  double gam, E, up0, up1, dn0, dn1, g1, g2, T0, T1;
+};
+
+__device__ __forceinline__ TsLayer ts_layer(double dtau, double ssa, double g, double mu0, double f0, double T0,
+                                            double bt, double bb, bool thermal) {
+  TsLayer r;
+  const double w0 = fmin(ssa, 1.0 - 1e-6);
+  const double f = g * g;
+  const double sc = 1.0 - w0 * f;
+  const double dt = sc * dtau;
+  const double w = (1.0 - f) * w0 * (1.0 / sc);
+  const double gs = g / (1.0 + g);
+  const double g1 = kSqrt3 * (2.0 - w * (1.0 + gs)) / 2.0;
+  const double g2 = kSqrt3 * w * (1.0 - gs) / 2.0;
+  const double g3 = (1.0 - kSqrt3 * gs * mu0) / 2.0;
+  const double g4 = 1.0 - g3;
+  const double lam = sqrt((g1 - g2) * (g1 + g2));
+  r.gam = g2 / (g1 + lam);
+  r.E = exp(-lam * dt);
+  r.g1 = g1;
+  r.g2 = g2;
+  r.T0 = T0;
+  r.T1 = T0 + dt;
+  double m = mu0;
+  const double ml = m * lam;
+  if (fabs(1.0 - ml * ml) < 1e-4) m *= 0.999;
+  const double rm = 1.0 / m;
+  const double rdet = 1.0 / (rm * rm - lam * lam);
+  const double a = f0 * w * (g3 * (rm - g1) - g2 * g4) * rdet;
+  const double b = f0 * w * (-g4 * (g1 + rm) - g2 * g3) * rdet;
+  const double e0 = exp(-T0 * rm), e1 = exp(-r.T1 * rm);
+  r.up0 = a * e0;
+  r.up1 = a * e1;
+  r.dn0 = b * e0;
+  r.dn1 = b * e1;
+  if (thermal) {
+    const double s = (bb - bt) / (dt * (g1 + g2));
+    r.up0 += bt + s;
+    r.up1 += bb + s;
+    r.dn0 += bt - s;
+    r.dn1 += bb - s;
+  }
+  return r;
+}
+
+// Per-column inputs gathered once into the rows of ts (one thread per
+// value): the Planck values of the levels and of the surface (thermal
+// batches), and the asymmetry factors g_l = pmom_l[1] (whose moment rows
+// are 8*pstride bytes apart, so each is a separate cache line of pmom).
+__global__ void k_ts_prep(int64_t n, int L, int P, int thermal, const double* __restrict__ pmom,
+                          const double* __restrict__ temper, const double* __restrict__ scal, double* __restrict__ ts) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nb = 2 * L + 2;  // L + 2 Planck values, L asymmetry factors
+  if (e >= n * nb) return;
+  const int64_t i = e / nb;
+  const int v = (int)(e % nb);
+  double* row = ts + i * ts_row(L) + 2 * (L + 1);
+  if (v < L + 2) {
+    if (!thermal) {
+      row[v] = 0.0;  // no thermal source
+      return;
+    }
+    const double* sc = scal + i * DISORT_NSCAL;
+    const double t = (v <= L) ? temper[i * (L + 1) + v] : sc[DISORT_S_TEMP_BOT];
+    row[v] = dsa::planck_band(sc[DISORT_S_WN_LO], sc[DISORT_S_WN_HI], t);
+  } else {
+    row[v] = pmom[(i * L + (v - L - 2)) * P + 1];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_twostream(int64_t n, int L, int P, const double* __restrict__ dtau,
+                                                   const double* __restrict__ ssa, const double* __restrict__ pmom,
+                                                   const double* __restrict__ temper, const double* __restrict__ scal,
+                                                   int thermal, double* __restrict__ work, double* __restrict__ ts) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* sc = scal + i * DISORT_NSCAL;
+  const double mu0 = sc[DISORT_S_BEAM_MU], f0 = sc[DISORT_S_BEAM_FLUX], al = sc[DISORT_S_ALBEDO];
+  const bool th = thermal != 0;
+  auto W = [&](int k, int l) -> double& { return work[((int64_t)k * L + l) * n + i]; };
+  const double* bl = ts + i * ts_row(L) + 2 * (L + 1);  // level Planck values, then the surface's
+  const double* gl = bl + L + 2;                          // asymmetry factors
+  auto planck_lev = [&](int v) { return kPi * bl[v]; };
+  double bt = th ? planck_lev(0) : 0.0;
+  double bb = th ? planck_lev(1) : 0.0;
+  TsLayer cur = ts_layer(dtau[i * L], ssa[i * L], gl[0], mu0, f0, 0.0, bt, bb, th);
+  // Carried equation c x_l = d; at the top D(0) = 0.
+  double c0 = cur.gam * cur.E, c1 = 1.0, d = -cur.dn0;
+  double Tu = 0.0;  // unscaled depth of the current layer's top
+  for (int l = 0; l + 1 < L; ++l) {
+    bt = bb;
+    bb = th ? planck_lev(l + 2) : 0.0;
+    const TsLayer nx = ts_layer(dtau[i * L + l + 1], ssa[i * L + l + 1], gl[l + 1], mu0, f0, cur.T1, bt, bb, th);
+    // rows [u0, u1, v0, v1, rhs]: carried, continuity of U, continuity of D
+    double R[3][5] = {{c0, c1, 0.0, 0.0, d},
+                      {1.0, cur.gam * cur.E, -nx.E, -nx.gam, nx.up0 - cur.up1},
+                      {cur.gam, cur.E, -nx.gam * nx.E, -1.0, nx.dn0 - cur.dn1}};
+    int p = 0;
+    if (fabs(R[1][0]) > fabs(R[p][0])) p = 1;
+    if (fabs(R[2][0]) > fabs(R[p][0])) p = 2;
+    const int o1 = (p == 0) ? 1 : 0, o2 = (p == 2) ? 1 : 2;
+    double A[5], Bq[5], Cr[5];
+    const double rp = 1.0 / R[p][0];
+    const double fa = R[o1][0] * rp, fb = R[o2][0] * rp;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      A[k] = R[o1][k] - fa * R[p][k];
+      Bq[k] = R[o2][k] - fb * R[p][k];
+    }
+    // second pivot among the two reduced rows (first largest)
+    const bool swap = fabs(Bq[1]) > fabs(A[1]);
+    double Q[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      Q[k] = swap ? Bq[k] : A[k];
+      Cr[k] = swap ? A[k] : Bq[k];
+    }
+    const double rq = 1.0 / Q[1];
+    const double fc = Cr[1] * rq;
+#pragma unroll
+    for (int k = 2; k < 5; ++k) Cr[k] -= fc * Q[k];
+    // x1 = (Q4 - Q2 y0 - Q3 y1) / Q1; x0 = (P4 - P1 x1 - P2 y0 - P3 y1) / P0
+    const double M10 = -Q[2] * rq, M11 = -Q[3] * rq, m1 = Q[4] * rq;
+    const double* Pr = R[p];
+    const double M00 = (-Pr[2] - Pr[1] * M10) * rp, M01 = (-Pr[3] - Pr[1] * M11) * rp;
+    const double m0 = (Pr[4] - Pr[1] * m1) * rp;
+    W(0, l) = M00;
+    W(1, l) = M01;
+    W(2, l) = M10;
+    W(3, l) = M11;
+    W(4, l) = m0;
+    W(5, l) = m1;
+    W(6, l) = cur.gam;
+    W(7, l) = cur.E;
+    W(8, l) = cur.up0;
+    W(9, l) = cur.dn0;
+    W(10, l) = cur.T0;
+    W(11, l) = Tu;
+    Tu += dtau[i * L + l];
+    c0 = Cr[2];
+    c1 = Cr[3];
+    d = Cr[4];
+    cur = nx;
+  }
+  // Last layer: carried equation and the surface.
+  double x0, x1;
+  {
+    const double b0 = 1.0 - al * cur.gam, b1 = (cur.gam - al) * cur.E;
+    double rb = al * mu0 * f0 * exp(-cur.T1 / mu0) - cur.up1 + al * cur.dn1;
+    if (th) rb += (1.0 - al) * kPi * bl[L + 1];
+    double u0 = c0, u1 = c1, r0 = d, v0 = b0, v1 = b1, r1 = rb;
+    if (fabs(v0) > fabs(u0)) {
+      double t;
+      t = u0, u0 = v0, v0 = t;
+      t = u1, u1 = v1, v1 = t;
+      t = r0, r0 = r1, r1 = t;
+    }
+    const double fct = v0 / u0;
+    v1 -= fct * u1;
+    r1 -= fct * r0;
+    x1 = r1 / v1;
+    x0 = (r0 - u1 * x1) / u0;
+  }
+  // Diffuse down as the solver reports it: the scaled total minus the
+  // unscaled direct beam, i.e. + mu0 F0 (e^{-T/mu0} - e^{-Tu/mu0}).
+  const double u = mu0 * f0;
+  auto beam_fix = [&](double T, double Tuv) { return u * (exp(-T / mu0) - exp(-Tuv / mu0)); };
+  const double TuL = Tu + dtau[i * L + L - 1];
+  W(0, L - 1) = x0 + x1 * cur.gam * cur.E + cur.up1;
+  W(1, L - 1) = x0 * cur.gam + x1 * cur.E + cur.dn1 + beam_fix(cur.T1, TuL);
+  W(8, L - 1) = x0 * cur.E + x1 * cur.gam + cur.up0;
+  W(9, L - 1) = x0 * cur.gam * cur.E + x1 + cur.dn0 + beam_fix(cur.T0, Tu);
+  for (int l = L - 2; l >= 0; --l) {
+    const double y0 = x0, y1 = x1;
+    x0 = W(0, l) * y0 + W(1, l) * y1 + W(4, l);
+    x1 = W(2, l) * y0 + W(3, l) * y1 + W(5, l);
+    const double gm = W(6, l), E = W(7, l);
+    const double uv = x0 * E + x1 * gm + W(8, l);
+    const double dv = x0 * gm * E + x1 + W(9, l) + beam_fix(W(10, l), W(11, l));
+    W(8, l) = uv;
+    W(9, l) = dv;
+  }
+}
+
+// Rows of ts from the per-level results the back substitution left in the
+// work array: 32 columns per workgroup, staged through LDS so that the
+// reads ([k][level][column]) and the row writes are both coalesced.
+__global__ __launch_bounds__(256) void k_ts_pack(int64_t n, int L, const double* __restrict__ work,
+                                                 double* __restrict__ ts) {
+  __shared__ double tile[32 * 2 * 64];
+  const int nlev = L + 1, K = 2 * nlev;
+  const int64_t i0 = (int64_t)blockIdx.x * 32;
+  for (int e = threadIdx.x; e < K * 32; e += blockDim.x) {
+    const int k = e / 32, c = e % 32;
+    const int64_t i = i0 + c;
+    if (i >= n) continue;
+    const int q = k / nlev, v = k % nlev;  // q 0: U, 1: D
+    const int slot = (v < L) ? 8 + q : q, l = (v < L) ? v : L - 1;
+    tile[c * K + k] = work[((int64_t)slot * L + l) * n + i];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < K * 32; e += blockDim.x) {
+    const int c = e / K, k = e % K;
+    const int64_t i = i0 + c;
+    if (i < n) ts[i * ts_row(L) + k] = tile[c * K + k];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Jacobian surrogate: the temperature Jacobians of the top-of-atmosphere
 // upward flux and the surface downward flux, dF/dT_k = R_k dB/dT(T_k) for the
 // D = nlyr + 2 degrees of freedom k (level temperatures, then the surface).
@@ -316,7 +586,6 @@ __global__ void k_monitor(int64_t m, const int32_t* idx, const int32_t* flags, i
 //   [k]      R_k / pi of flux_up at the top level
 //   [D + k]  R_k / pi of flux_down at the surface level
 // ---------------------------------------------------------------------------
-constexpr double kPi = 3.14159265358979323846;
 constexpr double kMuBar = 0.6;
 
 __device__ __forceinline__ double dbdt_dof(const double* temper, const double* scal, int64_t i, int L, int k) {
@@ -495,32 +764,33 @@ extern "C" {
 
 int disort_surrogate_features(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlyr, int pstride,
                               const disort_flux_batch* b, const float* shift, const float* scale, const float* lo,
-                              const float* hi, int w0, uint16_t* x, int32_t* ood) {
+                              const float* hi, int w0, uint16_t* x, int32_t* ood, const double* ts) {
   if (!ctx || !b || n < 0 || n_pad < n || nlyr <= 0 || pstride < 2 || w0 < 4 * nlyr + 2) return DISORT_E_INVALID;
   if (!b->temper) return DISORT_E_INVALID;
   if (nlyr + 2 > 64) return DISORT_E_INVALID;  // one lane per level
   hipLaunchKernelGGL(k_features, dim3(blocks(n_pad, 4)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n, n_pad, nlyr,
-                     pstride, b->dtau, b->ssa, b->pmom, b->temper, b->scal, shift, scale, lo, hi, w0, (__bf16*)x, ood);
+                     pstride, b->dtau, b->ssa, b->pmom, b->temper, b->scal, shift, scale, lo, hi, w0, (__bf16*)x, ood,
+                     ts);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;
 }
 
 int disort_surrogate_targets(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlev, const disort_flux_batch* b,
-                             const float* shift, const float* scale, int wL, float* t) {
+                             const float* shift, const float* scale, int wL, float* t, const double* ts) {
   if (!ctx || !b || !b->out || n < 0 || n_pad < n || wL < 3 * nlev) return DISORT_E_INVALID;
   hipLaunchKernelGGL(k_targets, dim3(blocks(n_pad * wL)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n, n_pad,
-                     nlev, b->out, b->total_levels, b->scal, shift, scale, wL, t);
+                     nlev, b->out, b->total_levels, b->scal, shift, scale, wL, t, ts);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;
 }
 
 int disort_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const float* y1, const float* y2,
                             const float* shift, const float* scale, const disort_flux_batch* b, double c_local,
-                            double c_floor, double* flux, float* spread) {
+                            double c_floor, double* flux, float* spread, const double* ts) {
   if (!ctx || !b || !y1 || !flux || !spread || wL < 3 * (nlyr + 1) || !(c_floor > 0.0)) return DISORT_E_INVALID;
   if (nlyr + 1 > 64) return DISORT_E_INVALID;  // one lane per level
   hipLaunchKernelGGL(k_decode, dim3(blocks(n, 4)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n, nlyr, wL, y1, y2,
-                     shift, scale, b->dtau, b->scal, c_local, c_floor, flux, spread);
+                     shift, scale, b->dtau, b->scal, c_local, c_floor, flux, spread, ts);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;
 }
@@ -582,6 +852,22 @@ int disort_surrogate_monitor(disort_ctx* ctx, int64_t m, const int32_t* idx, con
   if (m > 0)
     hipLaunchKernelGGL(k_monitor, dim3(blocks(m)), dim3(256), 0, st, m, idx, flags, nlev, out, total_levels, pred, b->scal,
                        c_local, c_floor, tol, counters);
+  SURR_LAUNCH_CHECK();
+  return DISORT_OK;
+}
+
+int disort_surrogate_twostream(disort_ctx* ctx, int64_t n, int nlyr, int pstride, const disort_flux_batch* b,
+                               double* work, double* ts) {
+  if (!ctx || !b || !work || !ts || n < 0 || nlyr <= 0 || nlyr + 1 > 64 || pstride < 2) return DISORT_E_INVALID;
+  const bool thermal = (b->flags & DISORT_F_THERMAL) != 0;
+  if (thermal && !b->temper) return DISORT_E_INVALID;
+  if (n == 0) return DISORT_OK;
+  hipStream_t st = (hipStream_t)disort_ctx_stream(ctx);
+  hipLaunchKernelGGL(k_ts_prep, dim3(blocks(n * (2 * nlyr + 2))), dim3(256), 0, st, n, nlyr, pstride, thermal ? 1 : 0,
+                     b->pmom, b->temper, b->scal, ts);
+  hipLaunchKernelGGL(k_twostream, dim3(blocks(n)), dim3(256), 0, st, n, nlyr, pstride, b->dtau, b->ssa, b->pmom,
+                     b->temper, b->scal, thermal ? 1 : 0, work, ts);
+  hipLaunchKernelGGL(k_ts_pack, dim3(blocks(n, 32)), dim3(256), 0, st, n, nlyr, work, ts);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;
 }

@@ -48,6 +48,10 @@ NLEV = NLYR + 1
 N_BASE = 4 * NLYR + 2               # 242 per-layer features + mu0 + albedo
 N_LEVEL = N_BASE + 3 * NLEV          # + level optical depth, beam transmission, absorption depth: 425
 N_FEAT = N_LEVEL + NLEV + 1          # + level Planck values and surface emission in beam units: 487
+N_TS = 2 * NLEV                      # two-stream baseline: diffuse net and sum per level
+N_FEAT_TS = N_FEAT + N_TS            # 609: the residual network's inputs
+TS_ROW = 3 * NLEV + 1 + NLYR         # baseline row: up, down, level and surface Planck values, layer g
+WIDTHS_TS = (640, 512, 512, 512, 512, 192)
 N_OUT = 3 * NLEV                     # 183
 N_LOSS = 2 * NLEV                    # the direct-beam outputs are replaced by the closed form
 WIDTHS = (512, 512, 512, 512, 512, 192)
@@ -81,11 +85,22 @@ def c5_columns(n: int, seed: int = C5_SEED) -> FluxBatch:
                      pstride=np.full(n, NSTR + 1, np.int32), dtau=dtau, ssa=ssa, pmom=pmom, scal=sc, temper=temper)
 
 
-def raw_features(b: FluxBatch, n_feat: int = N_FEAT) -> np.ndarray:
+def raw_features(b: FluxBatch, n_feat: int = N_FEAT, ts: np.ndarray | None = None) -> np.ndarray:
     """Host copy of the feature map of the device kernel (statistics only):
-    the first N_BASE features, the level features when n_feat >= N_LEVEL and
-    the thermal-source features when n_feat = N_FEAT."""
+    the first N_BASE features, the level features when n_feat >= N_LEVEL,
+    the thermal-source features when n_feat >= N_FEAT and the two-stream
+    baseline's net and sum in beam units when n_feat = N_FEAT_TS (ts: the
+    baseline [n, 2, nlev] from DeviceColumns.twostream)."""
     n, L = b.n, NLYR
+    if n_feat >= N_FEAT_TS:
+        f = np.empty((n, n_feat))
+        f[:, :N_FEAT] = raw_features(b, N_FEAT)
+        s = b.scal.reshape(n, NSCAL)
+        u = s[:, S_BEAM_MU] * s[:, S_BEAM_FLUX]
+        u = np.where(u > 1e-300, u, 1.0)[:, None]
+        f[:, N_FEAT:N_FEAT + NLEV] = (ts[:, 1] - ts[:, 0]) / u
+        f[:, N_FEAT + NLEV:N_FEAT_TS] = (ts[:, 1] + ts[:, 0]) / u
+        return f
     dtau = b.dtau.reshape(n, L)
     ssa = b.ssa.reshape(n, L)
     f = np.empty((n, n_feat))
@@ -172,10 +187,11 @@ def _mlp_lib():
         l.disort_mlp_train_step.argtypes = [vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float,
                                             C.c_float, C.c_int, vp]
         l.disort_surrogate_features.argtypes = [vp, C.c_int64, C.c_int64, C.c_int, C.c_int, vp, vp, vp, vp, vp,
-                                                C.c_int, vp, vp]
-        l.disort_surrogate_targets.argtypes = [vp, C.c_int64, C.c_int64, C.c_int, vp, vp, vp, C.c_int, vp]
+                                                C.c_int, vp, vp, vp]
+        l.disort_surrogate_targets.argtypes = [vp, C.c_int64, C.c_int64, C.c_int, vp, vp, vp, C.c_int, vp, vp]
         l.disort_surrogate_decode.argtypes = [vp, C.c_int64, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_double,
-                                              C.c_double, vp, vp]
+                                              C.c_double, vp, vp, vp]
+        l.disort_surrogate_twostream.argtypes = [vp, C.c_int64, C.c_int, C.c_int, vp, vp, vp]
         l.disort_surrogate_gather.argtypes = [vp, C.c_int64, vp, C.c_int, C.c_int, vp, vp]
         l.disort_surrogate_scatter.argtypes = [vp, C.c_int64, vp, C.c_int, vp, C.c_int64, vp]
         l.disort_rows_gather.argtypes = [vp, C.c_int64, vp, C.c_int64, C.c_int, vp, vp]
@@ -287,7 +303,10 @@ class Normalizer:
 
     @classmethod
     def fit(cls, feats: np.ndarray, targets: np.ndarray, margin: float = 0.0, w0: int = WIDTHS[0],
-            wl: int = WIDTHS[-1]):
+            wl: int = WIDTHS[-1], t_floor: float = 1e-30):
+        """Feature and target standardisation from samples; t_floor bounds
+        the target standard deviations from below (outputs that are almost
+        always near zero keep an absolute scale instead of being blown up)."""
         nf, nt = feats.shape[1], targets.shape[1]
         fs = np.zeros(w0, np.float32)
         fc = np.ones(w0, np.float32)
@@ -303,7 +322,7 @@ class Normalizer:
         tc = np.ones(wl, np.float32)
         tm, tsd = targets.mean(0), targets.std(0)
         ts[:nt] = tm
-        tc[:nt] = 1.0 / np.where(tsd > 1e-30, tsd, 1.0)
+        tc[:nt] = 1.0 / np.where(tsd > 1e-30, np.maximum(tsd, t_floor), 1.0)
         return cls(fs, fc, lo, hi, ts, tc)
 
     def device(self, ctx):
@@ -316,15 +335,24 @@ def host_fluxes(b: FluxBatch) -> np.ndarray:
     return np.stack([out[2], out[1], out[0]], axis=1)
 
 
-def host_targets(b: FluxBatch) -> np.ndarray:
+def host_targets(b: FluxBatch, ts: np.ndarray | None = None) -> np.ndarray:
     """[n, 183] network targets of a solved host batch (the device kernel's
     output space before standardisation): diffuse net, diffuse sum and
-    direct beam in units of mu0*F0."""
+    direct beam in units of mu0*F0; with the two-stream baseline ts
+    [n, 2, nlev] the first two blocks are the residuals over it."""
     f = host_fluxes(b)
     s = b.scal.reshape(b.n, NSCAL)
     u = s[:, S_BEAM_MU] * s[:, S_BEAM_FLUX]
     u = np.where(u > 1e-300, u, 1.0)[:, None]
-    return np.concatenate([(f[:, 1] - f[:, 0]) / u, (f[:, 1] + f[:, 0]) / u, f[:, 2] / u], axis=1)
+    net, sm = f[:, 1] - f[:, 0], f[:, 1] + f[:, 0]
+    if ts is not None:
+        net, sm = net - (ts[:, 1] - ts[:, 0]), sm - (ts[:, 1] + ts[:, 0])
+    return np.concatenate([net / u, sm / u, f[:, 2] / u], axis=1)
+
+
+def ts_fluxes(ts: "DevArray", n: int) -> np.ndarray:
+    """Host copy [n, 2, nlev] (diffuse up, down) of a device two-stream baseline."""
+    return ts.get(np.float64, (n, TS_ROW))[:, :2 * NLEV].reshape(n, 2, NLEV)
 
 
 class DeviceColumns:
@@ -341,7 +369,23 @@ class DeviceColumns:
     def solve(self):
         self.dev.execute()
 
+    def twostream(self) -> "DevArray":
+        """Enqueue the two-stream baseline of every column; returns the device
+        array [n, TS_ROW] (diffuse up and down at the levels, then the level
+        and surface Planck values).  Computed on every call (it is part of
+        each prediction's work); the buffers are kept."""
+        if getattr(self, "_ts", None) is None:
+            self._work = DevArray(self.ctx, 12 * self.n * NLYR * 8)
+            self._ts = DevArray(self.ctx, self.n * TS_ROW * 8)
+        _ck(self.ctx, _mlp_lib().disort_surrogate_twostream(self.ctx.ptr, self.n, NLYR, NSTR + 1,
+                                                             C.byref(self.cstruct), self._work.ptr, self._ts.ptr))
+        return self._ts
+
     def close(self):
+        for a in (getattr(self, "_work", None), getattr(self, "_ts", None)):
+            if a is not None:
+                a.free()
+        self._work = self._ts = None
         self.dev.close()
 
 
@@ -351,7 +395,9 @@ def build_dataset(ctx, n: int, seed: int, norm: Normalizer | None = None, chunk:
     input width (the level features need w0 >= N_FEAT)."""
     l = _mlp_lib()
     n_pad = _pad64(n)
-    n_feat = N_FEAT if w0 >= N_FEAT else N_LEVEL if w0 >= N_LEVEL else N_BASE
+    n_feat = (N_FEAT_TS if w0 >= N_FEAT_TS else N_FEAT if w0 >= N_FEAT else N_LEVEL if w0 >= N_LEVEL
+              else N_BASE)
+    residual = n_feat == N_FEAT_TS
     x = DevArray(ctx, n_pad * w0 * 2)
     t = DevArray(ctx, n_pad * WIDTHS[-1] * 4)
     dn = None
@@ -362,17 +408,20 @@ def build_dataset(ctx, n: int, seed: int, norm: Normalizer | None = None, chunk:
         hb = c5_columns(m, seed=seed * 1000 + k)
         dc = DeviceColumns(ctx, hb)
         dc.solve()
+        tsd = dc.twostream() if residual else None
         if norm is None:
             out = dc.dev.download()
-            norm = Normalizer.fit(raw_features(hb, n_feat), host_targets(out), margin=0.0, w0=w0)
+            tsh = ts_fluxes(tsd, m) if residual else None
+            norm = Normalizer.fit(raw_features(hb, n_feat, tsh), host_targets(out, tsh), margin=0.0, w0=w0)
         if dn is None:
             dn = norm.device(ctx)
         m_pad = m if done + m < n else n_pad - done
+        tsp = tsd.ptr if residual else None
         _ck(ctx, l.disort_surrogate_features(ctx.ptr, m, m_pad, NLYR, NSTR + 1, C.byref(dc.cstruct),
                                              dn["f_shift"].ptr, dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr,
-                                             w0, x.ptr + done * w0 * 2, None))
+                                             w0, x.ptr + done * w0 * 2, None, tsp))
         _ck(ctx, l.disort_surrogate_targets(ctx.ptr, m, m_pad, NLEV, C.byref(dc.cstruct), dn["t_shift"].ptr,
-                                            dn["t_scale"].ptr, WIDTHS[-1], t.ptr + done * WIDTHS[-1] * 4))
+                                            dn["t_scale"].ptr, WIDTHS[-1], t.ptr + done * WIDTHS[-1] * 4, tsp))
         ctx.sync()
         dc.close()
         done += m
@@ -518,9 +567,10 @@ class GatedSurrogate:
         ys = [DevArray(ctx, n_pad * WIDTHS[-1] * 4) for _ in self.members]
         flux = DevArray(ctx, n * 3 * NLEV * 8)
         spread = DevArray(ctx, n * 4)
+        tsp = dc.twostream().ptr if w0 >= N_FEAT_TS else None
         _ck(ctx, l.disort_surrogate_features(ctx.ptr, n, n_pad, NLYR, NSTR + 1, C.byref(dc.cstruct),
                                              self.dn["f_shift"].ptr, self.dn["f_scale"].ptr, self.dn["f_lo"].ptr,
-                                             self.dn["f_hi"].ptr, w0, x.ptr, ood.ptr))
+                                             self.dn["f_hi"].ptr, w0, x.ptr, ood.ptr, tsp))
         for m, y in zip(self.members, ys):
             for b0 in range(0, n_pad, m.max_batch):
                 bb = min(m.max_batch, n_pad - b0)
@@ -528,7 +578,7 @@ class GatedSurrogate:
         cl, cf = METRICS[self.metric]
         _ck(ctx, l.disort_surrogate_decode(ctx.ptr, n, NLYR, WIDTHS[-1], ys[0].ptr, ys[1].ptr if len(ys) > 1 else None,
                                            self.dn["t_shift"].ptr, self.dn["t_scale"].ptr, C.byref(dc.cstruct),
-                                           cl, cf, flux.ptr, spread.ptr))
+                                           cl, cf, flux.ptr, spread.ptr, tsp))
         for a in [x] + ys:
             a.free()
         return flux, spread, ood

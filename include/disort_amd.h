@@ -455,23 +455,30 @@ int disort_mlp_train_step(disort_ctx* ctx, disort_mlp* mlp, const uint16_t* x, c
 /* Surrogate column kernels on a uniform device batch (n columns of nlyr
  * layers, lay_off[i] = i*nlyr).  Features: 4*nlyr+2 per column (log10 dtau,
  * albedo, g = pmom[1], layer-mean temperature per layer; mu0; surface
- * albedo), standardised as (f - shift) * scale into BF16 rows of width w0
- * (rows n..n_pad-1 zero); ood[i] = 1 when a raw feature leaves [lo, hi]. */
+ * albedo), then with w0 >= 7*nlyr+5 the 3*(nlyr+1) level features, with
+ * w0 >= 8*nlyr+7 the nlyr+2 thermal-source features, and with ts given and
+ * w0 >= 10*nlyr+9 the two-stream baseline's diffuse net and sum flux in
+ * beam units at every level, standardised as (f - shift) * scale into BF16
+ * rows of width w0 (rows n..n_pad-1 zero); ood[i] = 1 when a raw feature
+ * leaves [lo, hi].  ts: the baseline of disort_surrogate_twostream, or NULL. */
 int disort_surrogate_features(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlyr, int pstride,
                               const disort_flux_batch* batch, const float* shift, const float* scale,
-                              const float* lo, const float* hi, int w0, uint16_t* x, int32_t* ood);
+                              const float* lo, const float* hi, int w0, uint16_t* x, int32_t* ood,
+                              const double* ts);
 /* Standardised targets in units of the incident beam mu0*F0 at nlev levels:
  * diffuse net (down - up), diffuse sum (down + up), direct beam; from the
- * solved batch's outputs (batch->out). */
+ * solved batch's outputs (batch->out).  With ts the first two blocks are the
+ * residuals over the two-stream baseline. */
 int disort_surrogate_targets(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlev, const disort_flux_batch* batch,
-                             const float* shift, const float* scale, int wL, float* targets);
+                             const float* shift, const float* scale, int wL, float* targets, const double* ts);
 /* Network outputs of one or two ensemble members -> fluxes [n][3][nlev]
- * (up, down, direct; the direct beam in closed form) and the members'
+ * (up, down, direct; the direct beam in closed form; with ts the outputs
+ * are residuals added to the two-stream baseline) and the members'
  * net-flux spread per column: max over levels of |dFnet| divided by
  * max(c_local |Fnet|, c_floor mu0 F0). */
 int disort_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const float* y1, const float* y2,
                             const float* shift, const float* scale, const disort_flux_batch* batch, double c_local,
-                            double c_floor, double* flux, float* spread);
+                            double c_floor, double* flux, float* spread, const double* ts);
 /* Route columns idx[0..m) of src into the compact batch dst, and the plain
  * solver's outputs for them back into the flux array. */
 int disort_surrogate_gather(disort_ctx* ctx, int64_t m, const int32_t* idx, int nlyr, int pstride,
@@ -500,6 +507,18 @@ int disort_surrogate_pad_index(disort_ctx* ctx, int64_t m, int64_t mpad, int32_t
 int disort_surrogate_monitor(disort_ctx* ctx, int64_t m, const int32_t* idx, const int32_t* flags, int nlev,
                              const double* out, int64_t total_levels, const double* pred, const disort_flux_batch* b,
                              double c_local, double c_floor, double tol, int32_t* counters);
+/* Two-stream baseline of the flux surrogate (no counterpart in the
+ * reference): delta-Eddington scaled, quadrature-coefficient two-stream
+ * fluxes of n uniform columns, one thread per column, with the beam, the
+ * thermal source linear in optical depth (DISORT_F_THERMAL) and a
+ * Lambertian surface; no diffuse flux enters at the top.  Row i of ts
+ * (4*nlyr+4 doubles): diffuse up at the levels, diffuse down without the
+ * unscaled direct beam (DisortFluxSolver's output convention), the band
+ * Planck values of the levels and of the surface temperature (with
+ * DISORT_F_THERMAL), then the layers' g = pmom[1].  work: 12 * n * nlyr doubles of device scratch;
+ * nlyr + 1 <= 64. */
+int disort_surrogate_twostream(disort_ctx* ctx, int64_t n, int nlyr, int pstride, const disort_flux_batch* batch,
+                               double* work, double* ts);
 /* Jacobian surrogate (no counterpart in the reference): the temperature
  * Jacobians of the top-of-atmosphere flux_up and the surface flux_down
  * (rows (0, level 0) and (1, level nlyr) of disort_flux_jacobian's layout)

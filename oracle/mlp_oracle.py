@@ -182,3 +182,90 @@ def emission_features(dtau, ssa, g, albedo, mubar=0.6):
     lev = [np.log10(tau + 1e-3), np.log10(ab + 1e-3), np.exp(-tau / mubar), np.exp(-ab / mubar),
            np.exp(-(tau[:, L:] - tau) / mubar), np.exp(-(ab[:, L:] - ab) / mubar)]
     return np.concatenate([np.log10(dtau), ssa, g, np.asarray(albedo).reshape(n, 1)] + lev, axis=1)
+
+
+SQ3 = 1.7320508075688772  # sqrt(3)
+
+
+def twostream(dtau, ssa, g, mu0, f0, albedo, b_lev=None, b_sfc=None):
+    """Two-stream diffuse fluxes of the surrogate's baseline, in DisORT's
+    output convention: returns (up, down) [n, L+1], down excluding the
+    unscaled direct beam.
+
+    Delta-Eddington scaling (f = g^2), quadrature coefficients
+    gamma1 = sqrt(3)(2 - w(1+g))/2, gamma2 = sqrt(3) w (1-g)/2,
+    gamma3 = (1 - sqrt(3) g mu0)/2, gamma4 = 1 - gamma3, and per layer the
+    exact solution of
+      dU/dt = gamma1 U - gamma2 D - w F0 gamma3 e^{-tau/mu0} - (gamma1-gamma2) pi B(t)
+      dD/dt = gamma2 U - gamma1 D + w F0 gamma4 e^{-tau/mu0} + (gamma1-gamma2) pi B(t)
+    (t the scaled depth in the layer, B linear in t): homogeneous modes
+    e^{lambda (t - dtau)} (1, Gamma) and e^{-lambda t} (Gamma, 1), the
+    exponential beam particular solution (mu0 moved by 0.1% near
+    resonance) and the linear thermal one, pi B(t) +- pi B'/(gamma1+gamma2).
+    Boundary conditions: no diffuse flux from the top; Lambertian surface
+    U = a (D + mu0 F0 e^{-tau_L/mu0}) + (1 - a) pi B_sfc.  The 2L constants
+    are solved with continuity at the interfaces by Gaussian elimination
+    with partial pivoting on the 3 x 2 panel of each layer (a plain dense
+    solve here)."""
+    n, L = dtau.shape
+    w0 = np.minimum(ssa, 1.0 - 1e-6)
+    f = g * g
+    dt = (1.0 - w0 * f) * dtau
+    w = (1.0 - f) * w0 / (1.0 - w0 * f)
+    gs = g / (1.0 + g)
+    g1 = SQ3 * (2.0 - w * (1.0 + gs)) / 2.0
+    g2 = SQ3 * w * (1.0 - gs) / 2.0
+    g3 = (1.0 - SQ3 * gs * mu0[:, None]) / 2.0
+    g4 = 1.0 - g3
+    lam = np.sqrt((g1 - g2) * (g1 + g2))
+    Gam = g2 / (g1 + lam)
+    E = np.exp(-lam * dt)
+    T = np.concatenate([np.zeros((n, 1)), np.cumsum(dt, axis=1)], axis=1)  # scaled level depths
+    Tu = np.concatenate([np.zeros((n, 1)), np.cumsum(dtau, axis=1)], axis=1)  # unscaled
+    thermal = b_lev is not None
+    up_p0, up_p1, dn_p0, dn_p1 = (np.zeros((n, L)) for _ in range(4))
+    for l in range(L):
+        m = mu0.copy()
+        near = np.abs(1.0 - (m * lam[:, l]) ** 2) < 1e-4
+        m[near] *= 0.999
+        det = 1.0 / m ** 2 - lam[:, l] ** 2
+        a = f0 * w[:, l] * (g3[:, l] * (1.0 / m - g1[:, l]) - g2[:, l] * g4[:, l]) / det
+        b = f0 * w[:, l] * (-g4[:, l] * (g1[:, l] + 1.0 / m) - g2[:, l] * g3[:, l]) / det
+        e0, e1 = np.exp(-T[:, l] / m), np.exp(-T[:, l + 1] / m)
+        up_p0[:, l], up_p1[:, l] = a * e0, a * e1
+        dn_p0[:, l], dn_p1[:, l] = b * e0, b * e1
+        if thermal:
+            bt, bb = np.pi * b_lev[:, l], np.pi * b_lev[:, l + 1]
+            s = (bb - bt) / dt[:, l] / (g1[:, l] + g2[:, l])
+            up_p0[:, l] += bt + s
+            up_p1[:, l] += bb + s
+            dn_p0[:, l] += bt - s
+            dn_p1[:, l] += bb - s
+    up = np.empty((n, L + 1))
+    dn = np.empty((n, L + 1))
+    for i in range(n):
+        M = np.zeros((2 * L, 2 * L))
+        r = np.zeros(2 * L)
+        M[0, 0], M[0, 1] = Gam[i, 0] * E[i, 0], 1.0
+        r[0] = -dn_p0[i, 0]
+        for l in range(L - 1):
+            k = 1 + 2 * l
+            M[k, 2 * l:2 * l + 2] = [1.0, Gam[i, l] * E[i, l]]
+            M[k, 2 * l + 2:2 * l + 4] = [-E[i, l + 1], -Gam[i, l + 1]]
+            r[k] = up_p0[i, l + 1] - up_p1[i, l]
+            M[k + 1, 2 * l:2 * l + 2] = [Gam[i, l], E[i, l]]
+            M[k + 1, 2 * l + 2:2 * l + 4] = [-Gam[i, l + 1] * E[i, l + 1], -1.0]
+            r[k + 1] = dn_p0[i, l + 1] - dn_p1[i, l]
+        al = albedo[i]
+        M[-1, -2:] = [1.0 - al * Gam[i, -1], (Gam[i, -1] - al) * E[i, -1]]
+        r[-1] = al * mu0[i] * f0[i] * np.exp(-T[i, L] / mu0[i]) - up_p1[i, -1] + al * dn_p1[i, -1]
+        if thermal:
+            r[-1] += (1.0 - al) * np.pi * b_sfc[i]
+        x = np.linalg.solve(M, r)
+        A, B = x[0::2], x[1::2]
+        up[i, :L] = A * E[i] + B * Gam[i] + up_p0[i]
+        dn[i, :L] = A * Gam[i] * E[i] + B + dn_p0[i]
+        up[i, L] = A[-1] + B[-1] * Gam[i, -1] * E[i, -1] + up_p1[i, -1]
+        dn[i, L] = A[-1] * Gam[i, -1] + B[-1] * E[i, -1] + dn_p1[i, -1]
+    dn = dn + mu0[:, None] * f0[:, None] * (np.exp(-T / mu0[:, None]) - np.exp(-Tu / mu0[:, None]))
+    return up, dn

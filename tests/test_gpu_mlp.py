@@ -129,7 +129,7 @@ def test_column_kernels(ctx):
     x = S.DevArray(ctx, n_pad * 256 * 2)
     ood = S.DevArray(ctx, n * 4)
     assert l.disort_surrogate_features(ctx.ptr, n, n_pad, 60, 17, C.byref(dc.cstruct), dn["f_shift"].ptr,
-                                       dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, 256, x.ptr, ood.ptr) == 0
+                                       dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, 256, x.ptr, ood.ptr, None) == 0
     xs = x.get(np.uint16, (n_pad, 256))
     want = mo.bf16_bits(((feats[:, :242].astype(np.float32) - norm.f_shift[:242]) * norm.f_scale[:242]).astype(np.float32))
     # Same float32 arithmetic up to the log10 / average in double: at most 1 BF16 ulp apart.
@@ -151,7 +151,7 @@ def test_column_kernels(ctx):
     np.testing.assert_allclose(S.raw_features(hb), full, rtol=1e-12, atol=1e-15)
     x2 = S.DevArray(ctx, n_pad * w0 * 2)
     assert l.disort_surrogate_features(ctx.ptr, n, n_pad, 60, 17, C.byref(dc.cstruct), dn["f_shift"].ptr,
-                                       dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, w0, x2.ptr, ood.ptr) == 0
+                                       dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, w0, x2.ptr, ood.ptr, None) == 0
     xs2 = x2.get(np.uint16, (n_pad, w0))
     want2 = mo.bf16_bits(((full.astype(np.float32) - norm.f_shift[:nf]) * norm.f_scale[:nf]).astype(np.float32))
     assert np.abs(xs2[:n, :nf].astype(np.int32) - want2.astype(np.int32)).max() <= 1
@@ -159,13 +159,13 @@ def test_column_kernels(ctx):
     assert (ood.get(np.int32, n) == 0).all()
     t = S.DevArray(ctx, n_pad * 192 * 4)
     assert l.disort_surrogate_targets(ctx.ptr, n, n_pad, 61, C.byref(dc.cstruct), dn["t_shift"].ptr,
-                                      dn["t_scale"].ptr, 192, t.ptr) == 0
+                                      dn["t_scale"].ptr, 192, t.ptr, None) == 0
     ts = t.get(np.float32, (n_pad, 192))
     np.testing.assert_allclose(ts[:n, :183], (tg - norm.t_shift[:183]) * norm.t_scale[:183], rtol=1e-5, atol=1e-5)
     # Decoding the standardised targets returns the solver's fluxes.
     flux, spread = S.DevArray(ctx, n * 3 * 61 * 8), S.DevArray(ctx, n * 4)
     assert l.disort_surrogate_decode(ctx.ptr, n, 60, 192, t.ptr, None, dn["t_shift"].ptr, dn["t_scale"].ptr,
-                                     C.byref(dc.cstruct), 1.0, 1e-3, flux.ptr, spread.ptr) == 0
+                                     C.byref(dc.cstruct), 1.0, 1e-3, flux.ptr, spread.ptr, None) == 0
     f = flux.get(np.float64, (n, 3, 61))
     ref = S.host_fluxes(out)
     np.testing.assert_allclose(f[:, :2], ref[:, :2], rtol=1e-5, atol=1e-6 * np.abs(ref).max())
@@ -250,3 +250,93 @@ def test_end_to_end_gate_small(ctx):
     gs.close()
     for m in members + [mean_net]:
         m.close()
+
+
+@pytest.mark.parametrize("thermal", [True, False])
+def test_twostream_kernel_matches_oracle(ctx, thermal):
+    """The two-stream baseline kernel against the oracle's restatement
+    (mlp_oracle.twostream, pinned by closed forms, exact layer splitting and
+    its accuracy class against DISORT): 1e-10 of the incident beam."""
+    from disort_amd import surrogate as S
+    from disort_amd import _capi
+
+    n, L = 300, 60
+    hb = S.c5_columns(n, seed=41)
+    if not thermal:
+        hb.flags = 0
+    print(f"[twostream] {n} columns thermal={thermal}", flush=True)
+    dc = S.DeviceColumns(ctx, hb)
+    work = S.DevArray(ctx, 12 * n * L * 8)
+    ts = S.DevArray(ctx, n * S.TS_ROW * 8)
+    l = S._mlp_lib()
+    assert l.disort_surrogate_twostream(ctx.ptr, n, L, 17, C.byref(dc.cstruct), work.ptr, ts.ptr) == 0
+    got = S.ts_fluxes(ts, n)
+    sc = hb.scal.reshape(n, 12)
+    bl = bs = None
+    if thermal:
+        t = hb.temper.reshape(n, L + 1)
+        bl = np.stack([_capi.planck_band(1000.0, 1001.0, t[i]) for i in range(n)])
+        bs = _capi.planck_band(1000.0, 1001.0, sc[:, 5])
+    up, dn = mo.twostream(hb.dtau.reshape(n, L), hb.ssa.reshape(n, L), hb.pmom.reshape(n, L, -1)[:, :, 1],
+                          sc[:, 1], sc[:, 0], sc[:, 7], bl, bs)
+    u = (sc[:, 1] * sc[:, 0])[:, None]
+    assert np.abs(got[:, 0] - up).max() / 1.0 <= 1e-10 * u.max()
+    assert (np.abs(got[:, 0] - up) / u).max() <= 1e-10
+    assert (np.abs(got[:, 1] - dn) / u).max() <= 1e-10
+    if thermal:  # the level and surface Planck values shared with the feature kernel
+        pl = ts.get(np.float64, (n, S.TS_ROW))[:, 2 * (L + 1):3 * (L + 1) + 1]
+        np.testing.assert_allclose(pl, np.concatenate([bl, bs[:, None]], axis=1), rtol=1e-13)
+    np.testing.assert_array_equal(ts.get(np.float64, (n, S.TS_ROW))[:, 3 * (L + 1) + 1:],
+                                  hb.pmom.reshape(n, L, -1)[:, :, 1])
+    for a in (work, ts):
+        a.free()
+    dc.close()
+
+
+def test_residual_column_kernels(ctx):
+    """With the two-stream baseline: the appended features are its net and
+    sum in beam units, the targets are the solver's residuals over it, and
+    decoding the standardised targets adds it back (the solver's fluxes)."""
+    from disort_amd import surrogate as S
+
+    l = S._mlp_lib()
+    n, n_pad = 100, 128
+    hb = S.c5_columns(n, seed=12)
+    dc = S.DeviceColumns(ctx, hb)
+    dc.solve()
+    out = dc.dev.download()
+    tsd = dc.twostream()
+    tsh = S.ts_fluxes(tsd, n)
+    feats = S.raw_features(hb, S.N_FEAT_TS, tsh)
+    tg = S.host_targets(out, tsh)
+    w0 = S.WIDTHS_TS[0]
+    norm = S.Normalizer.fit(feats, tg, w0=w0)
+    dn = norm.device(ctx)
+    x = S.DevArray(ctx, n_pad * w0 * 2)
+    ood = S.DevArray(ctx, n * 4)
+    assert l.disort_surrogate_features(ctx.ptr, n, n_pad, 60, 17, C.byref(dc.cstruct), dn["f_shift"].ptr,
+                                       dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, w0, x.ptr, ood.ptr,
+                                       tsd.ptr) == 0
+    xs = x.get(np.uint16, (n_pad, w0))
+    nf = S.N_FEAT_TS
+    want = mo.bf16_bits(((feats.astype(np.float32) - norm.f_shift[:nf]) * norm.f_scale[:nf]).astype(np.float32))
+    assert np.abs(xs[:n, :nf].astype(np.int32) - want.astype(np.int32)).max() <= 1
+    assert (xs[n:] == 0).all() and (xs[:, nf:] == 0).all()
+    assert (ood.get(np.int32, n) == 0).all()
+    t = S.DevArray(ctx, n_pad * 192 * 4)
+    assert l.disort_surrogate_targets(ctx.ptr, n, n_pad, 61, C.byref(dc.cstruct), dn["t_shift"].ptr,
+                                      dn["t_scale"].ptr, 192, t.ptr, tsd.ptr) == 0
+    ts_ = t.get(np.float32, (n_pad, 192))
+    np.testing.assert_allclose(ts_[:n, :183], (tg - norm.t_shift[:183]) * norm.t_scale[:183], rtol=1e-5, atol=1e-5)
+    flux, spread = S.DevArray(ctx, n * 3 * 61 * 8), S.DevArray(ctx, n * 4)
+    assert l.disort_surrogate_decode(ctx.ptr, n, 60, 192, t.ptr, t.ptr, dn["t_shift"].ptr, dn["t_scale"].ptr,
+                                     C.byref(dc.cstruct), 0.0, 1.0, flux.ptr, spread.ptr, tsd.ptr) == 0
+    f = flux.get(np.float64, (n, 3, 61))
+    ref = S.host_fluxes(out)
+    u = (hb.scal.reshape(n, 12)[:, 1] * hb.scal.reshape(n, 12)[:, 0])[:, None]
+    # float32 targets of residuals 10-40x smaller than the fluxes: decoded to ~1e-7 of the beam
+    assert (np.abs(f[:, :2] - ref[:, :2]) / u[:, :, None]).max() < 2e-6
+    assert (spread.get(np.float32, n) == 0).all()
+    for a in (x, ood, t, flux, spread):
+        a.free()
+    dc.close()

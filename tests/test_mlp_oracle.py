@@ -119,3 +119,76 @@ def test_emission_features_closed_forms():
         top, bot = f[:, b + (8 + 4 * q):b + (12 + 4 * q)], f[:, b + (16 + 4 * q):b + (20 + 4 * q)]
         np.testing.assert_allclose(top * bot, top[:, -1:] * np.ones((1, 4)), rtol=1e-12)
     assert (f[:, b + 19] == 1.0).all() and (f[:, b + 23] == 1.0).all() and (f[:, b + 8] == 1.0).all()
+
+
+def _ts_inputs(n, L, seed):
+    r = np.random.default_rng(seed)
+    return (10.0 ** r.uniform(-3, 1, (n, L)), r.uniform(0, 0.99, (n, L)), r.uniform(0, 0.9, (n, L)),
+            r.uniform(0.1, 1, n), np.full(n, np.pi), r.uniform(0, 0.3, n))
+
+
+def test_twostream_closed_forms():
+    # Isothermal absorbing column over a black surface at the same temperature:
+    # U = pi B at every level, no diffuse flux enters at the top.
+    B = np.full((1, 9), 2.0)
+    up, dn = mo.twostream(np.full((1, 8), 0.7), np.zeros((1, 8)), np.zeros((1, 8)), np.array([0.6]),
+                          np.array([0.0]), np.array([0.0]), B, np.array([2.0]))
+    np.testing.assert_allclose(up, np.pi * 2.0, rtol=1e-13)
+    assert dn[0, 0] == 0.0 and np.all(np.diff(dn[0]) > 0) and dn[0, -1] < np.pi * 2.0
+    # Pure absorber lit by the beam over a black surface: no diffuse light at all.
+    dtau, _, g, mu0, f0, _ = _ts_inputs(3, 6, 1)
+    up, dn = mo.twostream(dtau, np.zeros_like(dtau), g, mu0, f0, np.zeros(3))
+    assert np.all(up == 0.0) and np.all(np.abs(dn) < 1e-15)
+    # Conservative scattering (omega clamped at 1 - 1e-6) over a white surface:
+    # the column reflects the incident beam and the net flux vanishes.
+    up, dn = mo.twostream(np.full((1, 8), 2.0), np.ones((1, 8)), np.full((1, 8), 0.5), np.array([0.6]),
+                          np.array([1.0]), np.array([1.0]))
+    direct = 0.6 * np.exp(-np.concatenate([[0.0], np.cumsum(np.full(8, 2.0))]) / 0.6)
+    assert abs(up[0, 0] / 0.6 - 1.0) < 1e-4
+    assert np.all(np.abs(dn[0] + direct - up[0]) < 1e-4)
+
+
+def test_twostream_layer_splitting_is_exact():
+    """The per-layer solution is exact: splitting every layer into two equal
+    halves (the Planck value at the split the linear interpolation) leaves
+    the fluxes at the original levels unchanged, with the beam and the
+    thermal source on."""
+    n, L = 4, 7
+    dtau, ssa, g, mu0, f0, alb = _ts_inputs(n, L, 2)
+    r = np.random.default_rng(3)
+    b = r.uniform(1.0, 3.0, (n, L + 1))
+    bs = r.uniform(1.0, 3.0, n)
+    up, dn = mo.twostream(dtau, ssa, g, mu0, f0, alb, b, bs)
+    rep = lambda a: np.repeat(a, 2, axis=1)  # noqa: E731
+    b2 = np.empty((n, 2 * L + 1))
+    b2[:, 0::2] = b
+    b2[:, 1::2] = 0.5 * (b[:, :-1] + b[:, 1:])
+    up2, dn2 = mo.twostream(rep(dtau) / 2, rep(ssa), rep(g), mu0, f0, alb, b2, bs)
+    np.testing.assert_allclose(up2[:, 0::2], up, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(dn2[:, 0::2], dn, rtol=1e-9, atol=1e-12)
+
+
+def test_twostream_accuracy_class_against_disort():
+    """On the surrogate workload's columns the two-stream net flux is within a
+    few percent of the incident beam of the 16-stream DISORT oracle (a sign
+    or coupling mistake in a source term moves it by tens of percent)."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "tests"))
+    import oracle_lib as ol
+    from disort_amd import _capi
+    from disort_amd import surrogate as S
+
+    n, L = 200, 60
+    hb = S.c5_columns(n, seed=3)
+    ol.flux_batch(hb, nthreads=4)
+    ref = S.host_fluxes(hb)
+    sc = hb.scal.reshape(n, 12)
+    t = hb.temper.reshape(n, L + 1)
+    bl = np.stack([_capi.planck_band(1000.0, 1001.0, t[i]) for i in range(n)])
+    bs = _capi.planck_band(1000.0, 1001.0, sc[:, 5])
+    up, dn = mo.twostream(hb.dtau.reshape(n, L), hb.ssa.reshape(n, L), hb.pmom.reshape(n, L, -1)[:, :, 1],
+                          sc[:, 1], sc[:, 0], sc[:, 7], bl, bs)
+    u = sc[:, 1] * sc[:, 0]
+    err = np.abs((dn - up) - (ref[:, 1] - ref[:, 0])).max(axis=1) / u
+    assert np.median(err) < 0.02 and np.quantile(err, 0.9) < 0.04, np.quantile(err, [0.5, 0.9])
+    errs = np.abs((dn + up) - (ref[:, 1] + ref[:, 0])).max(axis=1) / u
+    assert np.median(errs) < 0.03 and np.quantile(errs, 0.9) < 0.06, np.quantile(errs, [0.5, 0.9])

@@ -261,15 +261,16 @@ class JacGatedSurrogate:
         self.tau, self.alpha, self.tol = tau, alpha, tol
         self.monitor = GateMonitor(alpha)
         self._subs = {}
+        self._bufs = {}
 
     def predict(self, cols: JacColumns):
         """Device Jacobian rows [n, 2, D] (FP64), spread [n] and OOD flags [n]."""
         ctx, l, n = self.ctx, _jlib(), cols.n
         n_pad = _pad64(n)
         w0, wl = JWIDTHS[0], JWIDTHS[-1]
-        x = DevArray(ctx, n_pad * w0 * 2)
+        x = self._scratch("x", n_pad * w0 * 2)
         ood = DevArray(ctx, n * 4)
-        ys = [DevArray(ctx, n_pad * wl * 4) for _ in self.members]
+        ys = [self._scratch(f"y{k}", n_pad * wl * 4) for k in range(len(self.members))]
         rows = DevArray(ctx, n * 2 * D * 8)
         spread = DevArray(ctx, n * 4)
         _ck(ctx, l.disort_jac_surrogate_features(ctx.ptr, n, n_pad, NLYR, NSTR + 1, C.byref(cols.cstruct),
@@ -282,9 +283,16 @@ class JacGatedSurrogate:
         _ck(ctx, l.disort_jac_surrogate_decode(ctx.ptr, n, NLYR, wl, ys[0].ptr, ys[1].ptr if len(ys) > 1 else None,
                                                self.dn["t_shift"].ptr, self.dn["t_scale"].ptr, C.byref(cols.cstruct),
                                                rows.ptr, spread.ptr))
-        for a in [x] + ys:
-            a.free()
         return rows, spread, ood
+
+    def _scratch(self, name: str, nbytes: int) -> DevArray:
+        """Grow-only device scratch kept between calls."""
+        a = self._bufs.get(name)
+        if a is None or a.nbytes < nbytes:
+            if a is not None:
+                a.free()
+            a = self._bufs[name] = DevArray(self.ctx, nbytes)
+        return a
 
     def calibrate(self, cols: JacColumns, ref_rows: np.ndarray):
         """Spread threshold such that at most alpha of the accepted
@@ -372,4 +380,6 @@ class JacGatedSurrogate:
     def close(self):
         for sub in self._subs.values():
             sub.close()
-        self._subs = {}
+        for a in self._bufs.values():
+            a.free()
+        self._subs, self._bufs = {}, {}

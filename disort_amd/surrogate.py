@@ -556,15 +556,16 @@ class GatedSurrogate:
         self.tau, self.alpha, self.tol, self.metric = tau, alpha, tol, metric
         self.monitor = GateMonitor(alpha)
         self._subs = {}
+        self._bufs = {}
 
     def predict(self, dc: DeviceColumns):
         """Device fluxes [n, 3, nlev] (FP64), spread [n] and OOD flags [n] for a resident batch."""
         ctx, l, n = self.ctx, _mlp_lib(), dc.n
         n_pad = _pad64(n)
         w0 = self.members[0].widths[0]
-        x = DevArray(ctx, n_pad * w0 * 2)
+        x = self._scratch("x", n_pad * w0 * 2)
         ood = DevArray(ctx, n * 4)
-        ys = [DevArray(ctx, n_pad * WIDTHS[-1] * 4) for _ in self.members]
+        ys = [self._scratch(f"y{k}", n_pad * WIDTHS[-1] * 4) for k in range(len(self.members))]
         flux = DevArray(ctx, n * 3 * NLEV * 8)
         spread = DevArray(ctx, n * 4)
         tsp = dc.twostream().ptr if w0 >= N_FEAT_TS else None
@@ -579,9 +580,17 @@ class GatedSurrogate:
         _ck(ctx, l.disort_surrogate_decode(ctx.ptr, n, NLYR, WIDTHS[-1], ys[0].ptr, ys[1].ptr if len(ys) > 1 else None,
                                            self.dn["t_shift"].ptr, self.dn["t_scale"].ptr, C.byref(dc.cstruct),
                                            cl, cf, flux.ptr, spread.ptr, tsp))
-        for a in [x] + ys:
-            a.free()
         return flux, spread, ood
+
+    def _scratch(self, name: str, nbytes: int) -> DevArray:
+        """Grow-only device scratch kept between calls (inputs and outputs of
+        the members), so that a serving loop does not allocate per call."""
+        a = self._bufs.get(name)
+        if a is None or a.nbytes < nbytes:
+            if a is not None:
+                a.free()
+            a = self._bufs[name] = DevArray(self.ctx, nbytes)
+        return a
 
     def calibrate(self, dc: DeviceColumns, ref_flux: np.ndarray):
         """Threshold on the spread so that, among accepted calibration columns,
@@ -679,7 +688,9 @@ class GatedSurrogate:
     def close(self):
         for sub in self._subs.values():
             sub.close()
-        self._subs = {}
+        for a in self._bufs.values():
+            a.free()
+        self._subs, self._bufs = {}, {}
 
 
 def _plan_status(dev) -> np.ndarray:

@@ -231,8 +231,8 @@ def bench_c4(args) -> dict:
 
 
 def bench_c5(args) -> dict:
-    """Learned surrogate (C5): train two MLP members (609 features -> 512 x 4
-    -> 183, BF16 MFMA) that predict the residual of the solver's fluxes over
+    """Learned surrogate (C5): train MLP members (609 features -> 512 x 4 ->
+    183, BF16 MFMA; two by default) that predict the residual of the solver's fluxes over
     a two-stream baseline, on solver-generated columns; calibrate the gate,
     then serve fresh columns and compare with the plain FP64 GPU solver on
     the same columns.  Prints the gated throughput at the 1e-3 net-flux
@@ -253,7 +253,7 @@ def bench_c5(args) -> dict:
     # the net flux; the direct beam is the closed form).
     wts = np.concatenate([np.ones(S.NLEV), np.full(S.NLEV, 0.25)])
     restarts = 0
-    for k in range(2):
+    for k in range(args.members):
         m = S.MLP(ctx, W, max_batch=65536)  # training batches of 8192; serving in 65,536-column chunks
         lr = 1.2e-3
         t0 = time.perf_counter()
@@ -275,7 +275,7 @@ def bench_c5(args) -> dict:
     x.free()
     t.free()
     mlp_flops = 2.0 * (S.N_FEAT_TS * 512 + 3 * 512 * 512 + 512 * 183)  # per column, forward, unpadded
-    train_tflops = 3.0 * mlp_flops * n * args.epochs / (sum(train_s) / 2) / 1e12
+    train_tflops = 3.0 * mlp_flops * n * args.epochs / (sum(train_s) / len(train_s)) / 1e12
 
     cal = S.DeviceColumns(ctx, S.c5_columns(32768, seed=77))
     cal.solve()
@@ -388,10 +388,12 @@ def bench_c5(args) -> dict:
         "higher_is_better": True,
         "dtype": "bf16 (MLP, fp32 accumulate) + f64 (plain fallback)",
         "data": "synthetic (seeded Philox draws; C5 column distributions)",
-        "config": {"workload": f"C5 surrogate: train {n} columns x {args.epochs} epochs x 2 members, serve {n_serve} fresh columns",
+        "config": {"workload": f"C5 surrogate: train {n} columns x {args.epochs} epochs x {args.members} members, "
+                               f"serve {n_serve} fresh columns",
                    "mlp": f"{S.N_FEAT_TS} (242 column + 183 level + 62 thermal-source + 122 two-stream features) "
                           "-> 512x4 (GELU) -> 183, predicting the residual over the two-stream baseline",
-                   "gate": "OOD range + 2-member spread + sampled re-solve (online error monitor), routed on the device",
+                   "gate": f"OOD range + {args.members}-member spread + sampled re-solve (online error monitor), "
+                           "routed on the device",
                    "error_metric": "max_levels |dF_net| / (mu0 F0)", "tol": S.TOL,
                    "sample_frac": args.sample_frac},
         "plain_gpu_columns_per_s": plain_rate,
@@ -816,6 +818,7 @@ def main() -> int:
     ap.add_argument("--train-columns", type=int, default=8388608, help="C5 training columns")
     ap.add_argument("--epochs", type=int, default=120, help="C5 training epochs per member")
     ap.add_argument("--serve-columns", type=int, default=1048576, help="C5 fresh columns served")
+X
     ap.add_argument("--sample-frac", type=float, default=0.01, help="C5 sampled re-solve fraction")
     ap.add_argument("--jac-train-columns", type=int, default=2097152, help="Jacobian surrogate training columns")
     ap.add_argument("--jac-epochs", type=int, default=150, help="Jacobian surrogate epochs per member")

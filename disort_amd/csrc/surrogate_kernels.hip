@@ -175,10 +175,16 @@ __global__ void k_targets(int64_t n, int64_t n_pad, int nlev, const double* out,
 // spread = max over levels of |Fnet_1 - Fnet_2| / max(c_local |Fnet|, c_floor mu0 F0)
 // (c_local = 1, c_floor = 1e-3: relative to the local net flux with a floor;
 // c_local = 0, c_floor = 1: relative to the incident beam).
-__global__ __launch_bounds__(256) void k_decode(int64_t n, int L, int wL, const float* y1, const float* y2,
-                                                const float* shift, const float* scale, const double* dtau,
-                                                const double* scal, double c_local, double c_floor, double* flux,
-                                                float* spread, const double* ts) {
+constexpr int kMaxMembers = 8;
+struct Members {
+  const float* y[kMaxMembers];
+  int k;
+};
+
+__global__ __launch_bounds__(256) void k_decode(int64_t n, int L, int wL, Members ms, const float* shift,
+                                                const float* scale, const double* dtau, const double* scal,
+                                                double c_local, double c_floor, double* flux, float* spread,
+                                                const double* ts) {
   // One wavefront per column, lane = level (nlev <= 64).
   const int lane = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -204,17 +210,30 @@ __global__ __launch_bounds__(256) void k_decode(int64_t n, int L, int wL, const 
     const double* tr = ts ? ts + i * ts_row(nlev - 1) : nullptr;
     const double bn = ts ? tr[nlev + lv] - tr[lv] : 0.0;
     const double bs = ts ? tr[nlev + lv] + tr[lv] : 0.0;
-    const double a1 = ((double)y1[i * wL + ka] / scale[ka] + shift[ka]) * u + bn;  // diffuse net
-    const double s1 = ((double)y1[i * wL + ks] / scale[ks] + shift[ks]) * u + bs;  // diffuse sum
-    double a = a1, sm = s1;
-    if (y2) {
-      const double a2 = ((double)y2[i * wL + ka] / scale[ka] + shift[ka]) * u + bn;
-      const double s2 = ((double)y2[i * wL + ks] / scale[ks] + shift[ks]) * u + bs;
-      a = 0.5 * (a1 + a2);
-      sm = 0.5 * (s1 + s2);
+    // Members' diffuse net and sum; their mean is served, and twice the
+    // standard deviation of the net flux (|a_1 - a_2| for two members) is
+    // the spread.
+    double am[kMaxMembers];
+    double asum = 0.0, ssum = 0.0;
+#pragma unroll
+    for (int m = 0; m < kMaxMembers; ++m) {
+      if (m >= ms.k) break;
+      const float* y = ms.y[m];
+      am[m] = ((double)y[i * wL + ka] / scale[ka] + shift[ka]) * u + bn;
+      asum += am[m];
+      ssum += ((double)y[i * wL + ks] / scale[ks] + shift[ks]) * u + bs;
+    }
+    const double a = asum / ms.k, sm = ssum / ms.k;
+    if (ms.k > 1) {
+      double var = 0.0;
+#pragma unroll
+      for (int m = 0; m < kMaxMembers; ++m) {
+        if (m >= ms.k) break;
+        var += (am[m] - a) * (am[m] - a);
+      }
       const double net = c_local * fabs(dir + a);
       const double den = net > floor ? net : floor;
-      worst = fabs(a1 - a2) / den;
+      worst = 2.0 * sqrt(var / ms.k) / den;
       if (!(worst == worst)) worst = __builtin_inf();  // a non-finite output fails the gate
     }
     double* fr = flux + i * 3 * nlev;
@@ -784,12 +803,19 @@ int disort_surrogate_targets(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlev
   return DISORT_OK;
 }
 
-int disort_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const float* y1, const float* y2,
+int disort_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const float* const* ys, int n_members,
                             const float* shift, const float* scale, const disort_flux_batch* b, double c_local,
                             double c_floor, double* flux, float* spread, const double* ts) {
-  if (!ctx || !b || !y1 || !flux || !spread || wL < 3 * (nlyr + 1) || !(c_floor > 0.0)) return DISORT_E_INVALID;
+  if (!ctx || !b || !ys || !flux || !spread || wL < 3 * (nlyr + 1) || !(c_floor > 0.0)) return DISORT_E_INVALID;
+  if (n_members < 1 || n_members > kMaxMembers) return DISORT_E_INVALID;
   if (nlyr + 1 > 64) return DISORT_E_INVALID;  // one lane per level
-  hipLaunchKernelGGL(k_decode, dim3(blocks(n, 4)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n, nlyr, wL, y1, y2,
+  Members ms{};
+  ms.k = n_members;
+  for (int m = 0; m < n_members; ++m) {
+    if (!ys[m]) return DISORT_E_INVALID;
+    ms.y[m] = ys[m];
+  }
+  hipLaunchKernelGGL(k_decode, dim3(blocks(n, 4)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n, nlyr, wL, ms,
                      shift, scale, b->dtau, b->scal, c_local, c_floor, flux, spread, ts);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;

@@ -189,7 +189,7 @@ def _mlp_lib():
         l.disort_surrogate_features.argtypes = [vp, C.c_int64, C.c_int64, C.c_int, C.c_int, vp, vp, vp, vp, vp,
                                                 C.c_int, vp, vp, vp]
         l.disort_surrogate_targets.argtypes = [vp, C.c_int64, C.c_int64, C.c_int, vp, vp, vp, C.c_int, vp, vp]
-        l.disort_surrogate_decode.argtypes = [vp, C.c_int64, C.c_int, C.c_int, vp, vp, vp, vp, vp, C.c_double,
+        l.disort_surrogate_decode.argtypes = [vp, C.c_int64, C.c_int, C.c_int, vp, C.c_int, vp, vp, vp, C.c_double,
                                               C.c_double, vp, vp, vp]
         l.disort_surrogate_twostream.argtypes = [vp, C.c_int64, C.c_int, C.c_int, vp, vp, vp]
         l.disort_surrogate_gather.argtypes = [vp, C.c_int64, vp, C.c_int, C.c_int, vp, vp]
@@ -577,9 +577,10 @@ class GatedSurrogate:
                 bb = min(m.max_batch, n_pad - b0)
                 m.infer(x.ptr + b0 * w0 * 2, bb, y.ptr + b0 * WIDTHS[-1] * 4)
         cl, cf = METRICS[self.metric]
-        _ck(ctx, l.disort_surrogate_decode(ctx.ptr, n, NLYR, WIDTHS[-1], ys[0].ptr, ys[1].ptr if len(ys) > 1 else None,
-                                           self.dn["t_shift"].ptr, self.dn["t_scale"].ptr, C.byref(dc.cstruct),
-                                           cl, cf, flux.ptr, spread.ptr, tsp))
+        yp = (C.c_void_p * len(ys))(*[y.ptr for y in ys])
+        _ck(ctx, l.disort_surrogate_decode(ctx.ptr, n, NLYR, WIDTHS[-1], yp, len(ys), self.dn["t_shift"].ptr,
+                                           self.dn["t_scale"].ptr, C.byref(dc.cstruct), cl, cf, flux.ptr, spread.ptr,
+                                           tsp))
         return flux, spread, ood
 
     def _scratch(self, name: str, nbytes: int) -> DevArray:

@@ -471,12 +471,14 @@ int disort_surrogate_features(disort_ctx* ctx, int64_t n, int64_t n_pad, int nly
  * residuals over the two-stream baseline. */
 int disort_surrogate_targets(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlev, const disort_flux_batch* batch,
                              const float* shift, const float* scale, int wL, float* targets, const double* ts);
-/* Network outputs of one or two ensemble members -> fluxes [n][3][nlev]
- * (up, down, direct; the direct beam in closed form; with ts the outputs
- * are residuals added to the two-stream baseline) and the members'
- * net-flux spread per column: max over levels of |dFnet| divided by
+/* Network outputs of 1..8 ensemble members (ys: host array of n_members
+ * device pointers) -> the members' mean fluxes [n][3][nlev] (up, down,
+ * direct; the direct beam in closed form; with ts the outputs are residuals
+ * added to the two-stream baseline) and their net-flux spread per column:
+ * max over levels of twice the standard deviation of Fnet over the members
+ * (|Fnet_1 - Fnet_2| for two; 0 for one) divided by
  * max(c_local |Fnet|, c_floor mu0 F0). */
-int disort_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const float* y1, const float* y2,
+int disort_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const float* const* ys, int n_members,
                             const float* shift, const float* scale, const disort_flux_batch* batch, double c_local,
                             double c_floor, double* flux, float* spread, const double* ts);
 /* Route columns idx[0..m) of src into the compact batch dst, and the plain

@@ -164,7 +164,8 @@ def test_column_kernels(ctx):
     np.testing.assert_allclose(ts[:n, :183], (tg - norm.t_shift[:183]) * norm.t_scale[:183], rtol=1e-5, atol=1e-5)
     # Decoding the standardised targets returns the solver's fluxes.
     flux, spread = S.DevArray(ctx, n * 3 * 61 * 8), S.DevArray(ctx, n * 4)
-    assert l.disort_surrogate_decode(ctx.ptr, n, 60, 192, t.ptr, None, dn["t_shift"].ptr, dn["t_scale"].ptr,
+    assert l.disort_surrogate_decode(ctx.ptr, n, 60, 192, (C.c_void_p * 1)(t.ptr), 1, dn["t_shift"].ptr,
+                                     dn["t_scale"].ptr,
                                      C.byref(dc.cstruct), 1.0, 1e-3, flux.ptr, spread.ptr, None) == 0
     f = flux.get(np.float64, (n, 3, 61))
     ref = S.host_fluxes(out)
@@ -329,14 +330,29 @@ def test_residual_column_kernels(ctx):
     ts_ = t.get(np.float32, (n_pad, 192))
     np.testing.assert_allclose(ts_[:n, :183], (tg - norm.t_shift[:183]) * norm.t_scale[:183], rtol=1e-5, atol=1e-5)
     flux, spread = S.DevArray(ctx, n * 3 * 61 * 8), S.DevArray(ctx, n * 4)
-    assert l.disort_surrogate_decode(ctx.ptr, n, 60, 192, t.ptr, t.ptr, dn["t_shift"].ptr, dn["t_scale"].ptr,
-                                     C.byref(dc.cstruct), 0.0, 1.0, flux.ptr, spread.ptr, tsd.ptr) == 0
+    assert l.disort_surrogate_decode(ctx.ptr, n, 60, 192, (C.c_void_p * 2)(t.ptr, t.ptr), 2, dn["t_shift"].ptr,
+                                     dn["t_scale"].ptr, C.byref(dc.cstruct), 0.0, 1.0, flux.ptr, spread.ptr,
+                                     tsd.ptr) == 0
     f = flux.get(np.float64, (n, 3, 61))
     ref = S.host_fluxes(out)
     u = (hb.scal.reshape(n, 12)[:, 1] * hb.scal.reshape(n, 12)[:, 0])[:, None]
     # float32 targets of residuals 10-40x smaller than the fluxes: decoded to ~1e-7 of the beam
     assert (np.abs(f[:, :2] - ref[:, :2]) / u[:, :, None]).max() < 2e-6
     assert (spread.get(np.float32, n) == 0).all()
-    for a in (x, ood, t, flux, spread):
+    # Three members t, t + d, t - d on the net output of level 5: the mean is
+    # the single member's, the spread twice their standard deviation.
+    d = 0.3
+    tp, tm = ts_.copy(), ts_.copy()
+    tp[:n, 5] += np.float32(d)
+    tm[:n, 5] -= np.float32(d)
+    dp, dm = S.DevArray.of(ctx, tp), S.DevArray.of(ctx, tm)
+    flux3 = S.DevArray(ctx, n * 3 * 61 * 8)
+    assert l.disort_surrogate_decode(ctx.ptr, n, 60, 192, (C.c_void_p * 3)(t.ptr, dp.ptr, dm.ptr), 3,
+                                     dn["t_shift"].ptr, dn["t_scale"].ptr, C.byref(dc.cstruct), 0.0, 1.0, flux3.ptr,
+                                     spread.ptr, tsd.ptr) == 0
+    # (t + d and t - d are rounded to float32, so their mean is t to float32 rounding)
+    np.testing.assert_allclose(flux3.get(np.float64, (n, 3, 61)), f, rtol=1e-7, atol=1e-9)
+    np.testing.assert_allclose(spread.get(np.float32, n), 2.0 * np.sqrt(2.0 / 3.0) * d / norm.t_scale[5], rtol=1e-4)
+    for a in (x, ood, t, flux, spread, dp, dm, flux3):
         a.free()
     dc.close()

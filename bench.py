@@ -818,7 +818,7 @@ def main() -> int:
     ap.add_argument("--train-columns", type=int, default=8388608, help="C5 training columns")
     ap.add_argument("--epochs", type=int, default=120, help="C5 training epochs per member")
     ap.add_argument("--serve-columns", type=int, default=1048576, help="C5 fresh columns served")
-X
+    ap.add_argument("--members", type=int, default=2, help="C5 ensemble members (the gate's spread is theirs)")
     ap.add_argument("--sample-frac", type=float, default=0.01, help="C5 sampled re-solve fraction")
     ap.add_argument("--jac-train-columns", type=int, default=2097152, help="Jacobian surrogate training columns")
     ap.add_argument("--jac-epochs", type=int, default=150, help="Jacobian surrogate epochs per member")

@@ -480,7 +480,7 @@ def bench_jac(args) -> dict:
     chunk = 65536
     wdc = J.JacColumns(ctx, S.c5_columns(chunk, seed=4999))
     for g_ in [gate] + [g for _, g, _ in sweep]:
-        r_, i_ = g_.serve(wdc, sample_frac=args.sample_frac, seed=4999)
+        r_, i_ = g_.serve(wdc, sample_frac=args.jac_sample_frac, seed=4999)
         r_.free()
         i_["flags"].free()
     wdc.close()
@@ -505,7 +505,7 @@ def bench_jac(args) -> dict:
         mlp_s += dt
         for a in (r_, sp_, od_):
             a.free()
-        (rows, info), dt = S.timed(ctx, lambda: gate.serve(dc, sample_frac=args.sample_frac, seed=k))
+        (rows, info), dt = S.timed(ctx, lambda: gate.serve(dc, sample_frac=args.jac_sample_frac, seed=k))
         gated_s += dt
         got = rows.get(np.float64, (m_, 2, J.D))
         fl = info["flags"].get(np.int32, m_)
@@ -518,7 +518,7 @@ def bench_jac(args) -> dict:
         n_plain += info["n_plain"]
         n_sampled += int(((fl & 2) != 0).sum())
         for j, (tol, g, _) in enumerate(sweep):
-            (r2, i2), dt = S.timed(ctx, lambda: g.serve(dc, sample_frac=args.sample_frac, seed=k))
+            (r2, i2), dt = S.timed(ctx, lambda: g.serve(dc, sample_frac=args.jac_sample_frac, seed=k))
             sweep_s[j] += dt
             got2 = r2.get(np.float64, (m_, 2, J.D))
             acc2 = np.nonzero((i2["flags"].get(np.int32, m_) & 1) == 0)[0]
@@ -566,7 +566,7 @@ def bench_jac(args) -> dict:
                    "mlp": f"{J.N_EFEAT} emission features -> 512x4 (GELU) -> {J.N_JOUT}",
                    "gate": "OOD range + 2-member spread + sampled re-solve (online error monitor), routed on the device",
                    "error_metric": "max_k |dF/dT_k - ref| / (pi dB/dT(T_k)) over both rows", "tol": J.JTOL,
-                   "sample_frac": args.sample_frac},
+                   "sample_frac": args.jac_sample_frac},
         "plain_gpu_columns_per_s": plain_rate,
         "plain_gpu_solves_per_s": plain_rate * J.D,
         "gated_vs_plain": gated_rate / plain_rate,
@@ -824,6 +824,9 @@ def main() -> int:
     ap.add_argument("--jac-epochs", type=int, default=150, help="Jacobian surrogate epochs per member")
     ap.add_argument("--jac-serve-columns", type=int, default=262144, help="Jacobian surrogate columns served")
     ap.add_argument("--jac-cpu-sample", type=int, default=2048, help="columns of the oracle Jacobian timed")
+    ap.add_argument("--jac-sample-frac", type=float, default=0.002,
+                    help="Jacobian gate's sampled re-solve fraction (a plain Jacobian costs nlyr+2 solves; "
+                         "0.2%% of a 65,536-column call still gives the monitor 131 columns)")
     ap.add_argument("--instances", type=int, default=50000, help="C4 batch size")
     ap.add_argument("--cpu-sample", type=int, default=2000, help="C4 instances timed on the CPU")
     ap.add_argument("--no-cpu-baseline", action="store_true")

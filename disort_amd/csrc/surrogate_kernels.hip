@@ -234,8 +234,9 @@ __global__ __launch_bounds__(256) void k_decode(int64_t n, int L, int wL, Member
       const double net = c_local * fabs(dir + a);
       const double den = net > floor ? net : floor;
       worst = 2.0 * sqrt(var / ms.k) / den;
-      if (!(worst == worst)) worst = __builtin_inf();  // a non-finite output fails the gate
     }
+    // A non-finite output fails the gate (one member included).
+    if (!(worst == worst) || !isfinite(a) || !isfinite(sm)) worst = __builtin_inf();
     double* fr = flux + i * 3 * nlev;
     fr[lv] = 0.5 * (sm - a);
     fr[nlev + lv] = 0.5 * (sm + a);

@@ -930,7 +930,8 @@ def main() -> int:
             "thermal": True,
             "path": "plain GPU DISORT (disort_flux_execute, inputs resident in HBM)",
             "schedule": {"streams": tuning.get("streams"), "chunks": tuning.get("chunks"),
-                         "variant": tuning["buckets"][0]["chosen"], "serial_ms": tuning.get("serial_ms"),
+                         "variant": tuning.get("dominant_variant", tuning["buckets"][0]["chosen"]),
+                         "serial_ms": tuning.get("serial_ms"),
                          "tuned_ms": tuning.get("best_ms")},
         },
         "roofline": {

@@ -864,38 +864,61 @@ int disort_flux_autotune(disort_ctx* ctx, disort_flux_plan* plan, const disort_f
     js += buf;
   }
   js += "], \"schedules\": [";
+  // The variant of the bucket with the most work interacts with the
+  // schedule (a variant with fewer registers overlaps differently with the
+  // other chunks' kernels), so the schedule search times every valid
+  // variant of that stream count under every schedule; the other stream
+  // counts keep their isolated choice.
+  int dom = -1;
+  double dom_ms = -1.0;
+  for (auto& kv : plan->tuned_of)
+    if (kv.second > dom_ms) dom = kv.first, dom_ms = kv.second;
+  std::vector<int> dom_variants;
+  for (int v : {DISORT_V_LANE, DISORT_V_LANE_PAIR, DISORT_V_WORKGROUP})
+    if (dom >= 0 && variant_ok(dom, v) && (!(plan->flags & DISORT_F_SPHERICAL) || v == DISORT_V_WORKGROUP))
+      dom_variants.push_back(v);
+  if (dom_variants.empty()) dom_variants.push_back(dom >= 0 ? plan->variant_of[dom] : DISORT_V_DEFAULT);
   struct Cand {
-    int streams, chunks;
+    int streams, chunks, variant;
     double ms;
   };
-  std::vector<Cand> cands = {{1, 1, 0}, {2, 1, 0}, {2, 2, 0}, {3, 1, 0}, {3, 2, 0}, {3, 3, 0}};
-  auto apply = [&](int S, int c) -> int {
+  std::vector<Cand> cands;
+  for (int v : dom_variants)
+    for (auto sc : {std::pair<int, int>{1, 1}, {2, 1}, {2, 2}, {3, 1}, {3, 2}, {3, 3}})
+      cands.push_back({sc.first, sc.second, v, 0.0});
+  auto apply = [&](int S, int c, int v) -> int {
     plan->streams = S;
     plan->concurrent = S > 1;
     plan->chunks.clear();
+    if (dom >= 0) plan->variant_of[dom] = v;
     for (auto& kv : plan->tuned_of)
       if (S > 1 && kv.second >= 0.5 * total_ms / S) plan->chunks[kv.first] = c;
     return build_buckets(ctx, plan);
   };
   size_t best = 0;
+  double serial_ms = -1.0;  // the isolated choice run serially
+  const int iso = dom >= 0 ? plan->variant_of[dom] : DISORT_V_DEFAULT;
   for (size_t q = 0; q < cands.size(); ++q) {
     Cand& cd = cands[q];
-    if ((rc = apply(cd.streams, cd.chunks))) return rc;
+    if ((rc = apply(cd.streams, cd.chunks, cd.variant))) return rc;
     PlanArgs pc;
     if ((rc = prepare(ctx, plan, b, pc))) return rc;
     if ((rc = time_it([&]() { return launch_plan(ctx, plan, pc, false); }, &cd.ms))) return rc;
     if (cd.ms < cands[best].ms) best = q;
-    char buf[160];
-    std::snprintf(buf, sizeof buf, "%s{\"streams\": %d, \"chunks\": %d, \"buckets\": %zu, \"ms\": %.6g}", q ? ", " : "",
-                  cd.streams, cd.chunks, plan->buckets.size(), cd.ms);
+    if (cd.streams == 1 && cd.variant == iso) serial_ms = cd.ms;
+    char buf[200];
+    std::snprintf(buf, sizeof buf,
+                  "%s{\"streams\": %d, \"chunks\": %d, \"variant\": \"%s\", \"buckets\": %zu, \"ms\": %.6g}",
+                  q ? ", " : "", cd.streams, cd.chunks, variant_name(cd.variant), plan->buckets.size(), cd.ms);
     js += buf;
   }
-  if ((rc = apply(cands[best].streams, cands[best].chunks))) return rc;
-  char buf[320];
+  if ((rc = apply(cands[best].streams, cands[best].chunks, cands[best].variant))) return rc;
+  char buf[360];
   std::snprintf(buf, sizeof buf,
-                "], \"serial_ms\": %.6g, \"best_ms\": %.6g, \"streams\": %d, \"chunks\": %d, \"schedule\": \"%s\"}",
-                cands[0].ms, cands[best].ms, cands[best].streams, cands[best].chunks,
-                plan->streams > 1 ? "concurrent" : "serial");
+                "], \"serial_ms\": %.6g, \"best_ms\": %.6g, \"streams\": %d, \"chunks\": %d, \"dominant_nstr\": %d, "
+                "\"dominant_variant\": \"%s\", \"schedule\": \"%s\"}",
+                serial_ms, cands[best].ms, cands[best].streams, cands[best].chunks, dom,
+                variant_name(cands[best].variant), plan->streams > 1 ? "concurrent" : "serial");
   js += buf;
   plan->report = js;
   return DISORT_OK;

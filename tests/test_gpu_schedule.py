@@ -65,8 +65,15 @@ def test_autotune_keeps_results(ctx, ragged):
     codes = {"lane": V_LANE, "lane_pair": V_LANE_PAIR, "workgroup": V_WORKGROUP}
     for n in (8, 16):
         assert got[n]["lane_ms"] > 0 and got[n]["lane_pair_ms"] > 0 and got[n]["workgroup_ms"] > 0
-        assert dev.get_variant(n) == codes[got[n]["chosen"]]
+        # the bucket with the most work takes the variant of the joint
+        # (variant x schedule) search, the others their isolated choice
+        want = rep["dominant_variant"] if n == rep["dominant_nstr"] else got[n]["chosen"]
+        assert dev.get_variant(n) == codes[want]
     assert rep["schedule"] in ("serial", "concurrent")
+    assert rep["dominant_nstr"] == max(got, key=lambda k: min(v for v in (got[k]["lane_ms"], got[k]["lane_pair_ms"],
+                                                                        got[k]["workgroup_ms"]) if v > 0))
+    assert len(rep["schedules"]) == 6 * sum(got[rep["dominant_nstr"]][f"{v}_ms"] > 0
+                                            for v in ("lane", "lane_pair", "workgroup"))
     out = run(dev, "execute_graph")
     assert_parity(out, ref, label="autotuned graph")
     dev.close()

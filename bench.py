@@ -431,12 +431,12 @@ def bench_c5(args) -> dict:
 
 def bench_jac(args) -> dict:
     """Learned temperature Jacobians (the surrogate extension): two members
-    (547 emission features -> 512 x 4 -> 124, BF16 MFMA) learn the responses
+    (671 features -> 512 x 4 -> 124, BF16 MFMA) learn the residual of the responses
     of the top-of-atmosphere upward flux and the surface downward flux to
     every level's Planck value from solver-generated Jacobians; the gate
     (OOD + spread + sampled re-solve, online monitor) routes failing columns
     to the plain GPU Jacobian (disort_jac_execute).  Prints the gated
-    throughput at the 1e-2 bound (error relative to pi dB/dT at each
+    throughput at the 2e-3 bound (error relative to pi dB/dT at each
     temperature) against the plain GPU Jacobian on the same columns, a
     tolerance sweep, and the CPU oracle's Jacobian on a bounded sample."""
     import numpy as np
@@ -451,7 +451,7 @@ def bench_jac(args) -> dict:
     ctx.sync()
     gen_s = time.perf_counter() - t0
     print(f"dataset {n} columns in {gen_s:.1f} s", file=sys.stderr, flush=True)
-    members, train_s, hist = [], [], []
+    members, train_s, hist, restarts = [], [], [], 0
     for k in range(2):
         t0 = time.perf_counter()
         m, h = J.train_member(ctx, x, t, n, args.jac_epochs, seed=300 + k, batch=2048, lr=1e-3, max_batch=65536)
@@ -460,10 +460,23 @@ def bench_jac(args) -> dict:
         hist.append(h[-1])
         members.append(m)
         print(f"member {k}: {train_s[-1]:.1f} s, final loss {h[-1]:.3e}", file=sys.stderr, flush=True)
+    # A member whose loss ends far above the other's has had a training
+    # excursion: train it again at half the rate.
+    worse = int(hist[1] > hist[0])
+    if hist[worse] > 20.0 * hist[1 - worse]:
+        restarts = 1
+        members[worse].close()
+        t0 = time.perf_counter()
+        m, h = J.train_member(ctx, x, t, n, args.jac_epochs, seed=300 + worse, batch=2048, lr=5e-4, max_batch=65536)
+        ctx.sync()
+        train_s[worse] += time.perf_counter() - t0
+        hist[worse] = h[-1]
+        members[worse] = m
+        print(f"member {worse} retrained at lr 5e-4: final loss {h[-1]:.3e}", file=sys.stderr, flush=True)
     x.free()
     t.free()
     W = J.JWIDTHS
-    mlp_flops = 2.0 * (J.N_EFEAT * 512 + 3 * 512 * 512 + 512 * J.N_JOUT)
+    mlp_flops = 2.0 * (J.N_EFEAT_TS * 512 + 3 * 512 * 512 + 512 * J.N_JOUT)
     train_tflops = 3.0 * mlp_flops * n * args.jac_epochs / (sum(train_s) / 2) / 1e12
 
     cal = J.JacColumns(ctx, S.c5_columns(16384, seed=501))
@@ -472,7 +485,7 @@ def bench_jac(args) -> dict:
     gate = J.JacGatedSurrogate(ctx, members, norm, alpha=0.01, tol=J.JTOL)
     cal_info, cal_err = gate.calibrate(cal, cal_ref)
     sweep = []
-    for tol in (3e-2, 1e-2, 5e-3, 1e-3):
+    for tol in (1e-2, 5e-3, 2e-3, 1e-3):
         g = J.JacGatedSurrogate(ctx, members, norm, alpha=0.01, tol=tol)
         sweep.append((tol, g, g.calibrate(cal, cal_ref)[0]))
     cal.close()
@@ -552,7 +565,7 @@ def bench_jac(args) -> dict:
     yb.free()
 
     line = {
-        "metric": "gated Jacobian surrogate vs plain GPU temperature Jacobian, columns/sec at <=1e-2 error "
+        "metric": "gated Jacobian surrogate vs plain GPU temperature Jacobian, columns/sec at <=2e-3 error "
                   "(nlyr=60, nstr=16)",
         "value": gated_rate,
         "unit": "columns/s",
@@ -563,7 +576,8 @@ def bench_jac(args) -> dict:
         "config": {"workload": f"Jacobian surrogate: train {n} columns x {args.jac_epochs} epochs x 2 members, "
                                f"serve {n_serve} fresh columns",
                    "outputs": "dF/dT_k of flux_up at the top and flux_down at the surface, k = 61 levels + surface",
-                   "mlp": f"{J.N_EFEAT} emission features -> 512x4 (GELU) -> {J.N_JOUT}",
+                   "mlp": f"{J.N_EFEAT} emission features + {J.N_JOUT} two-stream responses -> 512x4 (GELU) -> "
+                          f"{J.N_JOUT}, predicting the residual over the two-stream responses",
                    "gate": "OOD range + 2-member spread + sampled re-solve (online error monitor), routed on the device",
                    "error_metric": "max_k |dF/dT_k - ref| / (pi dB/dT(T_k)) over both rows", "tol": J.JTOL,
                    "sample_frac": args.jac_sample_frac},
@@ -590,7 +604,7 @@ def bench_jac(args) -> dict:
                      "achieved": pad_flops / gemm_s / 1e12, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": pad_flops / gemm_s / 1e12 / 2500.0, "traffic": None},
         "training": {"columns": n, "epochs": args.jac_epochs, "seconds_per_member": train_s, "final_loss": hist,
-                     "tflops": train_tflops, "dataset_seconds": gen_s},
+                     "tflops": train_tflops, "dataset_seconds": gen_s, "retrained_members": restarts},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = jac_cpu_baseline(ctx, args.jac_cpu_sample)

@@ -27,6 +27,7 @@
 // (the network's output space: the loss then weighs the net flux directly).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -564,6 +565,174 @@ __global__ __launch_bounds__(256) void k_twostream(int64_t n, int L, int P, cons
   }
 }
 
+// Two-stream responses to a unit Planck value at each degree of freedom k
+// (levels 0..L, then the surface), no beam, one thread per column.  The
+// system matrix does not depend on the source: the layer eliminations of
+// k_twostream are done once, keeping per layer M_l and the 3 x 3 map T_l
+// from the panel's right-hand sides (carried d, U and D continuity) to
+// (m_l, carried d'); the surface solve's 2 x 2 map likewise.  Each k then
+// only pushes its right-hand side -- non-zero next to level k, where the
+// linear thermal source of the two adjacent layers has its particular
+// solution pi B(t) +- pi B'/(gamma1 + gamma2) -- through the maps and the
+// surface solve: resp[i][0][k] = U at the top (through the running
+// products of the M_l, no back substitution), resp[i][1][k] = D at the
+// surface.  work: 18 doubles per (layer, column), [slot][layer][column].
+__global__ __launch_bounds__(256) void k_twostream_resp(int64_t n, int L, const double* __restrict__ dtau,
+                                                        const double* __restrict__ ssa,
+                                                        const double* __restrict__ scal,
+                                                        const double* __restrict__ ts, double* __restrict__ work,
+                                                        double* __restrict__ resp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* sc = scal + i * DISORT_NSCAL;
+  const double mu0 = sc[DISORT_S_BEAM_MU], al = sc[DISORT_S_ALBEDO];
+  const double* gl = ts + i * ts_row(L) + 3 * (L + 1) + 1;  // asymmetry factors
+  auto W = [&](int k, int l) -> double& { return work[((int64_t)k * L + l) * n + i]; };
+  // Phase 1: the eliminations with no source.
+  TsLayer cur = ts_layer(dtau[i * L], ssa[i * L], gl[0], mu0, 0.0, 0.0, 0.0, 0.0, false);
+  double c0 = cur.gam * cur.E, c1 = 1.0;
+  for (int l = 0; l < L; ++l) {
+    const double dtl = cur.T1 - cur.T0;
+    W(13, l) = cur.gam;
+    W(14, l) = cur.E;
+    W(15, l) = 1.0 / (dtl * (cur.g1 + cur.g2));
+    if (l + 1 == L) break;
+    const TsLayer nx = ts_layer(dtau[i * L + l + 1], ssa[i * L + l + 1], gl[l + 1], mu0, 0.0, cur.T1, 0.0, 0.0, false);
+    const double R[3][4] = {{c0, c1, 0.0, 0.0},
+                            {1.0, cur.gam * cur.E, -nx.E, -nx.gam},
+                            {cur.gam, cur.E, -nx.gam * nx.E, -1.0}};
+    int p = 0;
+    if (fabs(R[1][0]) > fabs(R[p][0])) p = 1;
+    if (fabs(R[2][0]) > fabs(R[p][0])) p = 2;
+    const int o1 = (p == 0) ? 1 : 0, o2 = (p == 2) ? 1 : 2;
+    const double rp = 1.0 / R[p][0];
+    const double fa = R[o1][0] * rp, fb = R[o2][0] * rp;
+    double A[4], Bq[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      A[k] = R[o1][k] - fa * R[p][k];
+      Bq[k] = R[o2][k] - fb * R[p][k];
+    }
+    const bool swap = fabs(Bq[1]) > fabs(A[1]);
+    double Q[4], Cr[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      Q[k] = swap ? Bq[k] : A[k];
+      Cr[k] = swap ? A[k] : Bq[k];
+    }
+    const double rq = 1.0 / Q[1];
+    const double fc = Cr[1] * rq;
+    Cr[2] -= fc * Q[2];
+    Cr[3] -= fc * Q[3];
+    const double M10 = -Q[2] * rq, M11 = -Q[3] * rq;
+    const double M00 = (-R[p][2] - R[p][1] * M10) * rp, M01 = (-R[p][3] - R[p][1] * M11) * rp;
+    W(0, l) = M00;
+    W(1, l) = M01;
+    W(2, l) = M10;
+    W(3, l) = M11;
+    // T_l: the same row operations on each unit right-hand side.
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double r[3] = {j == 0 ? 1.0 : 0.0, j == 1 ? 1.0 : 0.0, j == 2 ? 1.0 : 0.0};
+      const double a4 = r[o1] - fa * r[p], b4 = r[o2] - fb * r[p];
+      const double q4 = swap ? b4 : a4;
+      const double c4 = (swap ? a4 : b4) - fc * q4;
+      const double m1 = q4 * rq;
+      const double m0 = (r[p] - R[p][1] * m1) * rp;
+      W(4 + j, l) = m0;
+      W(7 + j, l) = m1;
+      W(10 + j, l) = c4;
+    }
+    c0 = Cr[2];
+    c1 = Cr[3];
+    cur = nx;
+  }
+  // Surface row and the 2 x 2 solve as a map of (d, rb).
+  double S00, S01, S10, S11;
+  {
+    const double b0 = 1.0 - al * cur.gam, b1 = (cur.gam - al) * cur.E;
+    const bool sw = fabs(b0) > fabs(c0);
+    const double u0 = sw ? b0 : c0, u1 = sw ? b1 : c1, v0 = sw ? c0 : b0, v1 = sw ? c1 : b1;
+    const double fct = v0 / u0, v1r = v1 - fct * u1;
+    // rows: r0 = (sw ? rb : d), r1 = (sw ? d : rb) - fct r0; x1 = r1 / v1r; x0 = (r0 - u1 x1) / u0
+    auto solve = [&](double d, double rb, double& x0, double& x1) {
+      const double r0 = sw ? rb : d;
+      const double r1 = (sw ? d : rb) - fct * r0;
+      x1 = r1 / v1r;
+      x0 = (r0 - u1 * x1) / u0;
+    };
+    solve(1.0, 0.0, S00, S10);
+    solve(0.0, 1.0, S01, S11);
+  }
+  // U at the top is [E_0, Gamma_0] x_0 + up_0 with x_0 = sum_l M_0..M_{l-1} m_l
+  // + M_0..M_{L-2} x_{L-1}: keep the row vectors w_l = [E_0, Gamma_0]
+  // M_0..M_{l-1} (slots 16, 17) so that each right-hand side needs no back
+  // substitution.
+  {
+    double w0 = W(14, 0), w1 = W(13, 0);
+    for (int l = 0; l < L; ++l) {
+      W(16, l) = w0;
+      W(17, l) = w1;
+      if (l + 1 == L) break;
+      const double n0 = w0 * W(0, l) + w1 * W(2, l), n1 = w0 * W(1, l) + w1 * W(3, l);
+      w0 = n0;
+      w1 = n1;
+    }
+  }
+  // Phase 2: the right-hand sides of G degrees of freedom at a time, from
+  // the first interface where any of them is non-zero (two above level k).
+  constexpr int G = 4;
+  const int D = L + 2;
+  for (int k0 = 0; k0 < D; k0 += G) {
+    // particular solution of layer l at its top (0) and bottom (1) for B = e_k
+    auto part = [&](int k, int l, double& up0, double& up1, double& dn0, double& dn1) {
+      const double bt = (l == k) ? kPi : 0.0, bb = (l + 1 == k) ? kPi : 0.0;
+      const double s = (bb - bt) * W(15, l);
+      up0 = bt + s;
+      up1 = bb + s;
+      dn0 = bt - s;
+      dn1 = bb - s;
+    };
+    const int l0 = (k0 >= 2) ? k0 - 2 : 0;
+    double d[G], u1a[G], d1a[G], acc[G], up00[G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      double u0a, d0a;
+      part(k0 + q, l0, u0a, u1a[q], d0a, d1a[q]);
+      d[q] = (l0 == 0) ? -d0a : 0.0;
+      up00[q] = (l0 == 0) ? u0a : 0.0;
+      acc[q] = 0.0;
+    }
+    for (int l = l0; l + 1 < L; ++l) {
+      double T[9];
+#pragma unroll
+      for (int t = 0; t < 9; ++t) T[t] = W(4 + t, l);
+      const double wl0 = W(16, l), wl1 = W(17, l);
+#pragma unroll
+      for (int q = 0; q < G; ++q) {
+        double u0b, u1b, d0b, d1b;
+        part(k0 + q, l + 1, u0b, u1b, d0b, d1b);
+        const double r1 = u0b - u1a[q], r2 = d0b - d1a[q];
+        const double m0 = T[0] * d[q] + T[1] * r1 + T[2] * r2;
+        const double m1 = T[3] * d[q] + T[4] * r1 + T[5] * r2;
+        d[q] = T[6] * d[q] + T[7] * r1 + T[8] * r2;
+        acc[q] += wl0 * m0 + wl1 * m1;
+        u1a[q] = u1b;
+        d1a[q] = d1b;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int k = k0 + q;
+      if (k >= D) break;
+      const double rb = -u1a[q] + al * d1a[q] + ((k == L + 1) ? (1.0 - al) * kPi : 0.0);
+      const double x0 = S00 * d[q] + S01 * rb, x1 = S10 * d[q] + S11 * rb;
+      resp[i * 2 * D + D + k] = x0 * W(13, L - 1) + x1 * W(14, L - 1) + d1a[q];
+      resp[i * 2 * D + k] = acc[q] + W(16, L - 1) * x0 + W(17, L - 1) * x1 + up00[q];
+    }
+  }
+}
+
 // Rows of ts from the per-level results the back substitution left in the
 // work array: 32 columns per workgroup, staged through LDS so that the
 // reads ([k][level][column]) and the row writes are both coalesced.
@@ -627,7 +796,8 @@ __global__ __launch_bounds__(256) void k_features_emission(int64_t n, int64_t n_
                                                            const double* dtau, const double* ssa,
                                                            const double* pmom, const double* scal,
                                                            const float* shift, const float* scale, const float* lo,
-                                                           const float* hi, int w0, __bf16* x, int32_t* ood) {
+                                                           const float* hi, int w0, __bf16* x, int32_t* ood,
+                                                           const double* resp) {
   __shared__ double lev[4][2][64];  // [column][tau, absorption tau][level]
   __shared__ double lay[4][2][64];  // [column][dtau, (1 - omega) dtau][layer]
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -638,7 +808,10 @@ __global__ __launch_bounds__(256) void k_features_emission(int64_t n, int64_t n_
     for (int k = lane; k < w0; k += 64) xr[k] = (__bf16)0.0f;
     return;
   }
-  const int nlev = L + 1, base = 3 * L + 1, nf = base + 6 * nlev;
+  const int nlev = L + 1, base = 3 * L + 1, nf0 = base + 6 * nlev, D = L + 2;
+  // With the two-stream responses: their 2D values / pi follow.
+  const bool with_r = resp && w0 >= nf0 + 2 * D;
+  const int nf = nf0 + (with_r ? 2 * D : 0);
   double (*lv)[64] = lev[wv];
   double (*ly)[64] = lay[wv];
   if (lane < L) {
@@ -669,6 +842,7 @@ __global__ __launch_bounds__(256) void k_features_emission(int64_t n, int64_t n_
     else if (k < 2 * L) f = ssa[i * L + (k - L)];
     else if (k < 3 * L) f = pmom[(i * L + (k - 2 * L)) * P + 1];
     else if (k == 3 * L) f = scal[i * DISORT_NSCAL + DISORT_S_ALBEDO];
+    else if (k >= nf0 && k < nf) f = resp[i * 2 * D + (k - nf0)] / kPi;
     else if (k < nf) {
       const int blk = (k - base) / nlev, v = (k - base) % nlev;
       const double tau = lv[0][v], tab = lv[1][v];
@@ -692,7 +866,8 @@ __global__ __launch_bounds__(256) void k_features_emission(int64_t n, int64_t n_
 }
 
 __global__ void k_jac_targets(int64_t n, int64_t n_pad, int L, const double* jac, const double* temper,
-                              const double* scal, const float* shift, const float* scale, int wL, float* t) {
+                              const double* scal, const float* shift, const float* scale, int wL, float* t,
+                              const double* resp) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n_pad * wL) return;
   const int64_t i = e / wL;
@@ -700,7 +875,8 @@ __global__ void k_jac_targets(int64_t n, int64_t n_pad, int L, const double* jac
   float v = 0.f;
   if (i < n && k < 2 * D) {
     const int r = k / D, d = k % D;
-    const double rr = jac[jac_row(i, L, r) + d] / (kPi * dbdt_dof(temper, scal, i, L, d));
+    double rr = jac[jac_row(i, L, r) + d] / (kPi * dbdt_dof(temper, scal, i, L, d));
+    if (resp) rr -= resp[i * 2 * D + k] / kPi;  // residual over the two-stream response
     v = ((float)rr - shift[k]) * scale[k];
   }
   t[e] = v;
@@ -710,7 +886,8 @@ __global__ void k_jac_targets(int64_t n, int64_t n_pad, int L, const double* jac
 // the members' spread max_k |R_1 - R_2| / pi per column (lane = degree of freedom).
 __global__ __launch_bounds__(256) void k_jac_decode(int64_t n, int L, int wL, const float* y1, const float* y2,
                                                     const float* shift, const float* scale, const double* temper,
-                                                    const double* scal, double* jrows, float* spread) {
+                                                    const double* scal, double* jrows, float* spread,
+                                                    const double* resp) {
   const int lane = threadIdx.x & 63;
   const int64_t i = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= n) return;
@@ -720,10 +897,11 @@ __global__ __launch_bounds__(256) void k_jac_decode(int64_t n, int L, int wL, co
     const double db = kPi * dbdt_dof(temper, scal, i, L, lane);
     for (int r = 0; r < 2; ++r) {
       const int k = r * D + lane;
-      const double a1 = (double)y1[i * wL + k] / scale[k] + shift[k];
+      const double base = resp ? resp[i * 2 * D + k] / kPi : 0.0;  // two-stream response (R / pi)
+      const double a1 = (double)y1[i * wL + k] / scale[k] + shift[k] + base;
       double a = a1;
       if (y2) {
-        const double a2 = (double)y2[i * wL + k] / scale[k] + shift[k];
+        const double a2 = (double)y2[i * wL + k] / scale[k] + shift[k] + base;
         a = 0.5 * (a1 + a2);
         worst = fmax(worst, fabs(a1 - a2));
         if (!(a1 - a2 == a1 - a2)) worst = __builtin_inf();
@@ -899,35 +1077,59 @@ int disort_surrogate_twostream(disort_ctx* ctx, int64_t n, int nlyr, int pstride
   return DISORT_OK;
 }
 
+int disort_jac_twostream(disort_ctx* ctx, int64_t n, int nlyr, int pstride, const disort_flux_batch* b, double* work,
+                         int64_t work_cols, double* ts, double* resp) {
+  if (!ctx || !b || !work || !ts || !resp || n < 0 || nlyr <= 0 || nlyr + 2 > 64 || pstride < 2 || work_cols < 1)
+    return DISORT_E_INVALID;
+  hipStream_t st = (hipStream_t)disort_ctx_stream(ctx);
+  const int L = nlyr, D = nlyr + 2, R = ts_row(nlyr);
+  for (int64_t c0 = 0; c0 < n; c0 += work_cols) {
+    const int64_t m = std::min(work_cols, n - c0);
+    const double* dtau = b->dtau + c0 * L;
+    const double* ssa = b->ssa + c0 * L;
+    const double* pmom = b->pmom + c0 * L * pstride;
+    const double* scal = b->scal + c0 * DISORT_NSCAL;
+    double* tsc = ts + c0 * R;
+    // the columns' g (no Planck values: the sources are unit vectors)
+    hipLaunchKernelGGL(k_ts_prep, dim3(blocks(m * (2 * L + 2))), dim3(256), 0, st, m, L, pstride, 0, pmom, nullptr,
+                       scal, tsc);
+    hipLaunchKernelGGL(k_twostream_resp, dim3(blocks(m)), dim3(256), 0, st, m, L, dtau, ssa, scal, tsc, work,
+                       resp + c0 * 2 * D);
+  }
+  SURR_LAUNCH_CHECK();
+  return DISORT_OK;
+}
+
 int disort_jac_surrogate_features(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlyr, int pstride,
                                   const disort_flux_batch* b, const float* shift, const float* scale, const float* lo,
-                                  const float* hi, int w0, uint16_t* x, int32_t* ood) {
+                                  const float* hi, int w0, uint16_t* x, int32_t* ood, const double* resp) {
   if (!ctx || !b || n < 0 || n_pad < n || nlyr <= 0 || pstride < 2 || w0 < 3 * nlyr + 1 + 6 * (nlyr + 1))
     return DISORT_E_INVALID;
   if (nlyr + 1 > 64) return DISORT_E_INVALID;  // one lane per level
   hipLaunchKernelGGL(k_features_emission, dim3(blocks(n_pad, 4)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n,
-                     n_pad, nlyr, pstride, b->dtau, b->ssa, b->pmom, b->scal, shift, scale, lo, hi, w0, (__bf16*)x, ood);
+                     n_pad, nlyr, pstride, b->dtau, b->ssa, b->pmom, b->scal, shift, scale, lo, hi, w0, (__bf16*)x, ood,
+                     resp);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;
 }
 
 int disort_jac_surrogate_targets(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlyr, const double* jac,
                                  const disort_flux_batch* b, const float* shift, const float* scale, int wL,
-                                 float* t) {
+                                 float* t, const double* resp) {
   if (!ctx || !b || !jac || !b->temper || n < 0 || n_pad < n || wL < 2 * (nlyr + 2)) return DISORT_E_INVALID;
   hipLaunchKernelGGL(k_jac_targets, dim3(blocks(n_pad * wL)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n,
-                     n_pad, nlyr, jac, b->temper, b->scal, shift, scale, wL, t);
+                     n_pad, nlyr, jac, b->temper, b->scal, shift, scale, wL, t, resp);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;
 }
 
 int disort_jac_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const float* y1, const float* y2,
                                 const float* shift, const float* scale, const disort_flux_batch* b, double* jrows,
-                                float* spread) {
+                                float* spread, const double* resp) {
   if (!ctx || !b || !b->temper || !y1 || !jrows || !spread || wL < 2 * (nlyr + 2)) return DISORT_E_INVALID;
   if (nlyr + 2 > 64) return DISORT_E_INVALID;  // one lane per degree of freedom
   hipLaunchKernelGGL(k_jac_decode, dim3(blocks(n, 4)), dim3(256), 0, (hipStream_t)disort_ctx_stream(ctx), n, nlyr, wL,
-                     y1, y2, shift, scale, b->temper, b->scal, jrows, spread);
+                     y1, y2, shift, scale, b->temper, b->scal, jrows, spread, resp);
   SURR_LAUNCH_CHECK();
   return DISORT_OK;
 }

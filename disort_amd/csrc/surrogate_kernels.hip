@@ -42,6 +42,11 @@ unsigned blocks(int64_t n, int b = 256) { return (unsigned)((n + b - 1) / b); }
 // levels, the level and surface Planck values, the layer asymmetry factors.
 __host__ __device__ constexpr int ts_row(int L) { return 3 * (L + 1) + 1 + L; }
 
+// Features end up in BF16 (8 significant bits), so their logarithms and
+// exponentials are taken in single precision (about 1e-7 relative).
+__device__ __forceinline__ double flog10(double x) { return (double)log10f((float)x); }
+__device__ __forceinline__ double fexp(double x) { return (double)expf((float)x); }
+
 __device__ __forceinline__ double beam_unit(const double* scal, int64_t i) {
   const double u = scal[i * DISORT_NSCAL + DISORT_S_BEAM_MU] * scal[i * DISORT_NSCAL + DISORT_S_BEAM_FLUX];
   return u > 1e-300 ? u : 1.0;
@@ -96,9 +101,9 @@ __global__ __launch_bounds__(256) void k_features(int64_t n, int64_t n_pad, int 
     const int v = lane;  // v == nlev: the surface (Planck term only)
     const double tau = (lane > 0) ? tau_ex : 0.0, tab = (lane > 0) ? tab_ex : 0.0;
     if (ext > 0 && v < nlev) {
-      lv[0][v] = log10(tau + 1e-3);
-      lv[1][v] = log10(tab + 1e-3);
-      lv[2][v] = exp(-tau / mu0);
+      lv[0][v] = flog10(tau + 1e-3);
+      lv[1][v] = flog10(tab + 1e-3);
+      lv[2][v] = fexp(-tau / mu0);
     }
     if (ext > 3 * nlev && v <= nlev) {
       double bb;
@@ -119,7 +124,7 @@ __global__ __launch_bounds__(256) void k_features(int64_t n, int64_t n_pad, int 
   int bad = 0;
   for (int k = lane; k < w0; k += 64) {
     double f = 0.0;
-    if (k < L) f = log10(dtau[i * L + k]);
+    if (k < L) f = flog10(dtau[i * L + k]);
     else if (k < 2 * L) f = ssa[i * L + (k - L)];
     else if (k < 3 * L) f = with_ts ? tsr[3 * nlev + 1 + (k - 2 * L)] : pmom[(i * L + (k - 2 * L)) * P + 1];
     else if (k < 4 * L) f = 0.5 * (temper[i * nlev + (k - 3 * L)] + temper[i * nlev + (k - 3 * L) + 1]);
@@ -838,7 +843,7 @@ __global__ __launch_bounds__(256) void k_features_emission(int64_t n, int64_t n_
   int bad = 0;
   for (int k = lane; k < w0; k += 64) {
     double f = 0.0;
-    if (k < L) f = log10(dtau[i * L + k]);
+    if (k < L) f = flog10(dtau[i * L + k]);
     else if (k < 2 * L) f = ssa[i * L + (k - L)];
     else if (k < 3 * L) f = pmom[(i * L + (k - 2 * L)) * P + 1];
     else if (k == 3 * L) f = scal[i * DISORT_NSCAL + DISORT_S_ALBEDO];
@@ -846,12 +851,12 @@ __global__ __launch_bounds__(256) void k_features_emission(int64_t n, int64_t n_
     else if (k < nf) {
       const int blk = (k - base) / nlev, v = (k - base) % nlev;
       const double tau = lv[0][v], tab = lv[1][v];
-      f = (blk == 0)   ? log10(tau + 1e-3)
-          : (blk == 1) ? log10(tab + 1e-3)
-          : (blk == 2) ? exp(-tau / kMuBar)
-          : (blk == 3) ? exp(-tab / kMuBar)
-          : (blk == 4) ? exp(-(tauL - tau) / kMuBar)
-                       : exp(-(tabL - tab) / kMuBar);
+      f = (blk == 0)   ? flog10(tau + 1e-3)
+          : (blk == 1) ? flog10(tab + 1e-3)
+          : (blk == 2) ? fexp(-tau / kMuBar)
+          : (blk == 3) ? fexp(-tab / kMuBar)
+          : (blk == 4) ? fexp(-(tauL - tau) / kMuBar)
+                       : fexp(-(tabL - tab) / kMuBar);
     }
     if (k < nf) {
       const float ff = (float)f;

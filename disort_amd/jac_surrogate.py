@@ -243,8 +243,9 @@ def build_jac_dataset(ctx, n: int, seed: int, norm: Normalizer | None = None, ch
         done += m
         k += 1
     if fit:
-        norm.f_lo[:N_EFEAT_TS] = lo
-        norm.f_hi[:N_EFEAT_TS] = hi
+        pad = 1e-5 * np.maximum(1.0, np.maximum(np.abs(lo), np.abs(hi)))  # single-precision logs and exponentials
+        norm.f_lo[:N_EFEAT_TS] = lo - pad
+        norm.f_hi[:N_EFEAT_TS] = hi + pad
     return x, t, n, norm
 
 

@@ -332,8 +332,12 @@ class Normalizer:
         fs[:nf] = m
         fc[:nf] = 1.0 / np.where(s > 0, s, 1.0)
         span = feats.max(0) - feats.min(0)
-        lo[:nf] = feats.min(0) - margin * span
-        hi[:nf] = feats.max(0) + margin * span
+        # The device computes logarithms and exponentials in single
+        # precision: the box also tolerates 1e-5 relative (constant features
+        # such as the top level's depth have no span at all).
+        pad = margin * span + 1e-5 * np.maximum(1.0, np.maximum(np.abs(feats.min(0)), np.abs(feats.max(0))))
+        lo[:nf] = feats.min(0) - pad
+        hi[:nf] = feats.max(0) + pad
         ts = np.zeros(wl, np.float32)
         tc = np.ones(wl, np.float32)
         tm, tsd = targets.mean(0), targets.std(0)

@@ -5,8 +5,8 @@ decode against the plain GPU Jacobian (itself checked against the oracle in
 test_gpu_jacobian_device.py) and the host dB/dT, and a small end-to-end
 train / calibrate / serve run of the gate with its online monitor.
 
-Tolerances: features at most 1 BF16 ulp from the oracle's (the same float32
-standardisation of double features); targets to float32 rounding (rtol
+Tolerances: features within one BF16 step of the oracle's (the kernels'
+logarithms and exponentials are single precision); targets to float32 rounding (rtol
 1e-6); decoded rows of the targets to float32 rounding of R / pi; routed
 columns the plain Jacobian to rounding (1e-12)."""
 import ctypes as C
@@ -19,6 +19,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import mlp_oracle as mo  # noqa: E402
+
+from test_gpu_mlp import assert_features_close  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -65,7 +67,8 @@ def test_jacobian_column_kernels(ctx):
     dbdt = J.dbdt_dofs(hb)
     want_t = ((ref_rows / dbdt[:, None, :] - rh) / np.pi).reshape(n, -1)
     np.testing.assert_allclose(raw, want_t, rtol=1e-5, atol=1e-7 * np.abs(want_t).max())
-    norm = S.Normalizer.fit(feats, raw, w0=J.JWIDTHS[0], wl=J.JWIDTHS[-1])
+    # (the kernel's logs and exponentials are single precision: a tiny margin)
+    norm = S.Normalizer.fit(feats, raw, margin=1e-5, w0=J.JWIDTHS[0], wl=J.JWIDTHS[-1])
     dn = norm.device(ctx)
     n_pad, w0, wl = 128, J.JWIDTHS[0], J.JWIDTHS[-1]
     x = S.DevArray(ctx, n_pad * w0 * 2)
@@ -76,7 +79,7 @@ def test_jacobian_column_kernels(ctx):
     xs = x.get(np.uint16, (n_pad, w0))
     nf = J.N_EFEAT_TS
     want = mo.bf16_bits(((want_f.astype(np.float32) - norm.f_shift[:nf]) * norm.f_scale[:nf]).astype(np.float32))
-    assert np.abs(xs[:n, :nf].astype(np.int32) - want.astype(np.int32)).max() <= 1
+    assert_features_close(xs[:n, :nf], want)
     assert (xs[n:] == 0).all() and (xs[:, nf:] == 0).all()
     assert (ood.get(np.int32, n) == 0).all()  # the columns define the range
     # An out-of-range column is flagged.

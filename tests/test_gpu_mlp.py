@@ -30,6 +30,19 @@ def ctx():
     return Context(0)
 
 
+def bf16_values(bits):
+    """float32 values of BF16 bit patterns."""
+    return (np.asarray(bits, np.uint32) << 16).view(np.float32)
+
+
+def assert_features_close(got_bits, want_bits):
+    """GPU features (BF16 bits) against the oracle's: the feature kernels take
+    logarithms and exponentials in single precision, so a standardised value
+    may land one BF16 step away, or change sign right at zero."""
+    g, w = bf16_values(got_bits), bf16_values(want_bits)
+    assert np.all(np.abs(g - w) <= 2.0 ** -7 * np.abs(w) + 1e-5), float(np.max(np.abs(g - w)))
+
+
 def rel(a, b):
     return float(np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30))
 
@@ -123,7 +136,7 @@ def test_column_kernels(ctx):
     out = dc.dev.download()
     feats = S.raw_features(hb)
     tg = S.host_targets(out)
-    norm = S.Normalizer.fit(feats, tg)
+    norm = S.Normalizer.fit(feats, tg, margin=1e-5)  # the kernel's logs and exps are single precision
     dn = norm.device(ctx)
     n_pad = 128
     x = S.DevArray(ctx, n_pad * 256 * 2)
@@ -132,9 +145,8 @@ def test_column_kernels(ctx):
                                        dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, 256, x.ptr, ood.ptr, None) == 0
     xs = x.get(np.uint16, (n_pad, 256))
     want = mo.bf16_bits(((feats[:, :242].astype(np.float32) - norm.f_shift[:242]) * norm.f_scale[:242]).astype(np.float32))
-    # Same float32 arithmetic up to the log10 / average in double: at most 1 BF16 ulp apart.
-    diff = np.abs(xs[:n, :242].astype(np.int32) - want.astype(np.int32))
-    assert diff.max() <= 1
+    # Same float32 standardisation; the logarithms in single precision.
+    assert_features_close(xs[:n, :242], want)
     assert (xs[n:] == 0).all() and (xs[:, 242:] == 0).all()
     assert (ood.get(np.int32, n) == 0).all()  # the columns define the range
     # With the network's input width the 3 x 61 level features follow (oracle restatement).
@@ -154,7 +166,7 @@ def test_column_kernels(ctx):
                                        dn["f_scale"].ptr, dn["f_lo"].ptr, dn["f_hi"].ptr, w0, x2.ptr, ood.ptr, None) == 0
     xs2 = x2.get(np.uint16, (n_pad, w0))
     want2 = mo.bf16_bits(((full.astype(np.float32) - norm.f_shift[:nf]) * norm.f_scale[:nf]).astype(np.float32))
-    assert np.abs(xs2[:n, :nf].astype(np.int32) - want2.astype(np.int32)).max() <= 1
+    assert_features_close(xs2[:n, :nf], want2)
     assert (xs2[n:] == 0).all() and (xs2[:, nf:] == 0).all()
     assert (ood.get(np.int32, n) == 0).all()
     t = S.DevArray(ctx, n_pad * 192 * 4)
@@ -311,7 +323,7 @@ def test_residual_column_kernels(ctx):
     feats = S.raw_features(hb, S.N_FEAT_TS, tsh)
     tg = S.host_targets(out, tsh)
     w0 = S.WIDTHS_TS[0]
-    norm = S.Normalizer.fit(feats, tg, w0=w0)
+    norm = S.Normalizer.fit(feats, tg, margin=1e-5, w0=w0)  # single-precision logs and exps
     dn = norm.device(ctx)
     x = S.DevArray(ctx, n_pad * w0 * 2)
     ood = S.DevArray(ctx, n * 4)
@@ -321,7 +333,7 @@ def test_residual_column_kernels(ctx):
     xs = x.get(np.uint16, (n_pad, w0))
     nf = S.N_FEAT_TS
     want = mo.bf16_bits(((feats.astype(np.float32) - norm.f_shift[:nf]) * norm.f_scale[:nf]).astype(np.float32))
-    assert np.abs(xs[:n, :nf].astype(np.int32) - want.astype(np.int32)).max() <= 1
+    assert_features_close(xs[:n, :nf], want)
     assert (xs[n:] == 0).all() and (xs[:, nf:] == 0).all()
     assert (ood.get(np.int32, n) == 0).all()
     t = S.DevArray(ctx, n_pad * 192 * 4)

@@ -309,11 +309,16 @@ class DeviceFluxBatch:
             pass
 
 
+JAC_EXPAND = 0x100  # DISORT_JAC_EXPAND: one forward solve per column x degree of freedom
+
+
 class JacPlan:
     """Device-resident temperature Jacobians of n uniform columns
     (disort_jac_plan_create / disort_jac_execute): execute(batch, jac_ptr)
     writes jac [n, 6, nlyr+1, nlyr+2] (FP64, device) for a device batch
-    descriptor, asynchronously on the context stream."""
+    descriptor, asynchronously on the context stream.  Factored form (the
+    unit-source right-hand sides of the boundary system) for nstr <= 16
+    unless flags has JAC_EXPAND."""
 
     def __init__(self, ctx: Context, n: int, nstr: int, nlyr: int, pstride: int, flags: int, max_expanded: int = 0):
         self.ctx, self.n, self.nstr, self.nlyr = ctx, int(n), int(nstr), int(nlyr)

@@ -393,7 +393,8 @@ FluxArgs bucket_args(disort_ctx* ctx, const PlanArgs& pa, const Bucket& bk, size
 }
 
 int launch_bucket(disort_ctx* ctx, const FluxArgs& a, const Bucket& bk, hipStream_t st, hipEvent_t* ev) {
-  const bool ok = bk.variant != DISORT_V_WORKGROUP
+  // The temperature-Jacobian kernels (a.jbase) exist in the register form only.
+  const bool ok = (bk.variant != DISORT_V_WORKGROUP || a.jbase)
                       ? dsa::launch_flux_bucket(bk.nstr, a, st, ev, bk.variant == DISORT_V_LANE_PAIR)
                                               : dsa::launch_flux_bucket_wg(bk.nstr, a, st, ev);
   if (!ok) return fail(ctx, DISORT_E_UNSUPPORTED, "no kernels for num_streams=" + std::to_string(bk.nstr));
@@ -1146,12 +1147,17 @@ struct disort_jac_plan {
   double *dtau = nullptr, *ssa = nullptr, *pmom = nullptr, *temper = nullptr, *scal = nullptr;
   double *ovr = nullptr, *out = nullptr, *dbdt = nullptr;
   int32_t* status = nullptr;  // per column
+  // Factored form: inner plans over chunk columns and over the remainder.
+  bool factored = false;
+  disort_flux_plan* flux_rem = nullptr;
+  double *jbase = nullptr, *jpanel = nullptr, *jcvec = nullptr;
   std::vector<void*> owned;
 };
 
 namespace {
 void free_jac_plan(disort_jac_plan* jp) {
   if (jp->flux) free_plan(jp->flux);
+  if (jp->flux_rem) free_plan(jp->flux_rem);
   for (void* p : jp->owned) (void)hipFree(p);
   delete jp;
 }
@@ -1166,20 +1172,19 @@ int disort_jac_plan_create(disort_ctx* ctx, int64_t n, int32_t nstr, int32_t nly
   if (n <= 0 || nlyr <= 0) return fail(ctx, DISORT_E_INVALID, "need n > 0 columns of nlyr > 0 layers");
   if (!(flags & DISORT_F_THERMAL))
     return fail(ctx, DISORT_E_INVALID, "temperature Jacobians need use_thermal_emission = true");
-  if (flags & ~(DISORT_F_THERMAL | DISORT_F_DELTA_M_PLUS | DISORT_F_DIFFUSION_BC))
+  if (flags & ~(DISORT_F_THERMAL | DISORT_F_DELTA_M_PLUS | DISORT_F_DIFFUSION_BC | DISORT_JAC_EXPAND))
     return fail(ctx, DISORT_E_INVALID, "device Jacobians support the thermal, Delta-M+ and diffusion-BC flags only");
+  const bool factored = !(flags & DISORT_JAC_EXPAND) && nstr <= 16;
+  flags &= ~DISORT_JAC_EXPAND;
   if (!gpu_supported_nstr(nstr)) return fail(ctx, DISORT_E_UNSUPPORTED, "no kernels for num_streams=" + std::to_string(nstr));
   if (pstride < nstr + 1 + ((flags & DISORT_F_DELTA_M_PLUS) ? 1 : 0))
     return fail(ctx, DISORT_E_INVALID, "pstride below the moments the solver reads");
   const int D = nlyr + 2;
-  if (max_expanded <= 0) max_expanded = 131072;
   auto* jp = new disort_jac_plan;
   std::unique_ptr<disort_jac_plan, void (*)(disort_jac_plan*)> guard(jp, free_jac_plan);
   jp->n = n;
   jp->nstr = nstr, jp->nlyr = nlyr, jp->pstride = pstride, jp->flags = flags;
-  jp->chunk = std::max<int64_t>(1, std::min<int64_t>(n, max_expanded / D));
-  jp->ne = jp->chunk * D;
-  const int64_t ne = jp->ne, L = nlyr;
+  jp->factored = factored;
   HIPCHK(ctx, hipSetDevice(ctx->device));
   auto alloc = [&](size_t bytes, void** p) -> int {
     HIPCHK(ctx, hipMalloc(p, std::max<size_t>(bytes, 256)));
@@ -1187,6 +1192,31 @@ int disort_jac_plan_create(disort_ctx* ctx, int64_t n, int32_t nstr, int32_t nly
     return DISORT_OK;
   };
   int rc;
+  if (factored) {
+    // Per column: the layer basis, the panels of each group of kJacRhs
+    // right-hand sides, the mode amplitudes per degree of freedom, dB/dT.
+    const int64_t L = nlyr, N = nstr, NG = (D + dsa::kJacRhs - 1) / dsa::kJacRhs;
+    const int64_t per_col = 8 * (L * N + NG * L * dsa::jac_panel_stride(nstr) + D * L * N + D);
+    int64_t chunk = std::max<int64_t>(1, (int64_t(4) << 30) / per_col);
+    if (max_expanded > 0) chunk = std::min<int64_t>(chunk, std::max<int64_t>(1, max_expanded / D));
+    jp->chunk = std::min<int64_t>(n, chunk);
+    const int64_t c = jp->chunk;
+    if ((rc = alloc(c * L * N * 8, (void**)&jp->jbase)) ||
+        (rc = alloc(c * NG * L * dsa::jac_panel_stride(nstr) * 8, (void**)&jp->jpanel)) ||
+        (rc = alloc(c * D * L * N * 8, (void**)&jp->jcvec)) || (rc = alloc(c * D * 8, (void**)&jp->dbdt)) ||
+        (rc = alloc((size_t)DISORT_NOUT * c * (L + 1) * 8, (void**)&jp->out)) || (rc = alloc(n * 4, (void**)&jp->status)))
+      return rc;
+    std::vector<int32_t> vn(c, nstr), vl(c, nlyr), vp(c, pstride);
+    if ((rc = disort_flux_plan_create(ctx, c, vn.data(), vl.data(), vp.data(), flags, &jp->flux))) return rc;
+    const int64_t rem = n % c;
+    if (rem && (rc = disort_flux_plan_create(ctx, rem, vn.data(), vl.data(), vp.data(), flags, &jp->flux_rem))) return rc;
+    *out = guard.release();
+    return DISORT_OK;
+  }
+  if (max_expanded <= 0) max_expanded = 131072;
+  jp->chunk = std::max<int64_t>(1, std::min<int64_t>(n, max_expanded / D));
+  jp->ne = jp->chunk * D;
+  const int64_t ne = jp->ne, L = nlyr;
   if ((rc = alloc(ne * L * 8, (void**)&jp->dtau)) || (rc = alloc(ne * L * 8, (void**)&jp->ssa)) ||
       (rc = alloc(ne * L * pstride * 8, (void**)&jp->pmom)) || (rc = alloc(ne * (L + 1) * 8, (void**)&jp->temper)) ||
       (rc = alloc(ne * DISORT_NSCAL * 8, (void**)&jp->scal)) ||
@@ -1217,6 +1247,35 @@ int disort_jac_execute(disort_ctx* ctx, disort_jac_plan* jp, const disort_flux_b
   if (!b->dtau || !b->ssa || !b->pmom || !b->temper || !b->scal)
     return fail(ctx, DISORT_E_INVALID, "the batch needs dtau, ssa, pmom, temper and scal");
   const int L = jp->nlyr;
+  if (jp->factored) {
+    for (int64_t c0 = 0; c0 < jp->n; c0 += jp->chunk) {
+      const int64_t cols = std::min(jp->chunk, jp->n - c0);
+      disort_flux_plan* fp = (cols == jp->chunk) ? jp->flux : jp->flux_rem;
+      disort_flux_batch d{};
+      d.n = cols;
+      d.flags = jp->flags;
+      d.dtau = b->dtau + c0 * L;
+      d.ssa = b->ssa + c0 * L;
+      d.pmom = b->pmom + c0 * L * jp->pstride;
+      d.temper = b->temper + c0 * (L + 1);
+      d.scal = b->scal + c0 * DISORT_NSCAL;
+      d.out = jp->out;
+      d.total_levels = cols * (L + 1);
+      PlanArgs pa;
+      int rc;
+      if ((rc = prepare(ctx, fp, &d, pa))) return rc;
+      pa.base.jbase = jp->jbase;
+      pa.base.jpanel = jp->jpanel;
+      pa.base.jcvec = jp->jcvec;
+      pa.base.jdbdt = jp->dbdt;
+      pa.base.jac = jac;
+      pa.base.jstatus = jp->status;
+      pa.base.jac_c0 = c0;
+      pa.base.jac_nlyr = L;
+      if ((rc = launch_plan(ctx, fp, pa, false))) return rc;
+    }
+    return DISORT_OK;
+  }
   const int64_t ne = jp->ne, TV = ne * (L + 1);
   for (int64_t c0 = 0; c0 < jp->n; c0 += jp->chunk) {
     const int64_t cols = std::min(jp->chunk, jp->n - c0);

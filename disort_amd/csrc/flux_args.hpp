@@ -46,6 +46,20 @@ struct FluxArgs {
   // temperature-Jacobian solves): [total_levels] at the levels, then [n][2]
   // the top emission (emissivity_top B(T_top)) and B(T_bottom); null = off.
   const double* planck_ovr;
+  // Temperature Jacobians on the factored boundary system (null = off).
+  // When jbase is set the layer kernels also store the layer's thermal
+  // particular-solution basis ([instance][layer][2 NN]: (1-omega') base,
+  // then d0 for a unit Planck slope), and the bucket launch solves the
+  // (L+2) unit-source right-hand sides instead of the forward problem.
+  double* jbase;
+  double* jpanel;   // [instance][dof group][layer][2 N N + K N]
+  double* jcvec;    // [instance][dof][layer][N] mode amplitudes
+  double* jdbdt;    // [instance][dof] dB/dT
+  double* jac;      // [column][6][L+1][L+2]
+  int32_t* jstatus; // [column]
+  int64_t jac_c0;   // column of batch instance 0
+  int32_t jac_nlyr; // layers per column (uniform)
+  int32_t pad2;
 };
 
 }  // namespace dsa

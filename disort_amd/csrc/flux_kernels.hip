@@ -438,7 +438,7 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
     }
 #pragma unroll
     for (int r = 0; r < NN; ++r) base[r] *= tb[to.sdi() + r];
-    const double c1 = (1.0 - ws) * xr1, c0 = (1.0 - ws) * xr0;
+    const double c1 = (1.0 - ws) * (a.jbase ? 1.0 : xr1), c0 = (1.0 - ws) * xr0;
     double h[NN], dd[NN];
 #pragma unroll
     for (int r = 0; r < NN; ++r) {
@@ -452,6 +452,11 @@ __global__ __launch_bounds__(64) void k_layer(FluxArgs a) {
     for (int r = 0; r < NN; ++r) {
       const double d0 = 2.0 * tb[to.sdi() + r] * dd[r];
       const double s0 = c0 * base[r];
+      if (a.jbase) {
+        double* jb = a.jbase + ((int64_t)i * a.jac_nlyr + lc) * 2 * NN;
+        jb[r] = (1.0 - ws) * base[r];
+        jb[NN + r] = d0;
+      }
       z1[NN + r] = h[r];
       z1[NN - 1 - r] = h[r];
       z0[NN + r] = 0.5 * (s0 + d0);
@@ -951,7 +956,7 @@ __global__ __launch_bounds__(64, (NN <= 4 ? 3 : 2)) void k_layer2(FluxArgs a) {
     }
 #pragma unroll
     for (int r = 0; r < NN; ++r) base[r] *= tb[to.sdi() + r];
-    const double c1 = (1.0 - ws) * xr1, c0 = (1.0 - ws) * xr0;
+    const double c1 = (1.0 - ws) * (a.jbase ? 1.0 : xr1), c0 = (1.0 - ws) * xr0;
     double hh[NN], dd[NN];
 #pragma unroll
     for (int r = 0; r < NN; ++r) {
@@ -964,6 +969,11 @@ __global__ __launch_bounds__(64, (NN <= 4 ? 3 : 2)) void k_layer2(FluxArgs a) {
     for (int r = 0; r < NN; ++r) {
       const double d0 = 2.0 * tb[to.sdi() + r] * dd[r];
       const double s0 = c0 * base[r];
+      if (a.jbase) {
+        double* jb = a.jbase + ((int64_t)i * a.jac_nlyr + lc) * 2 * NN;
+        jb[r] = (1.0 - ws) * base[r];
+        jb[NN + r] = d0;
+      }
       z1[NN + r] = hh[r];
       z1[NN - 1 - r] = hh[r];
       z0[NN + r] = 0.5 * (s0 + d0);
@@ -1267,6 +1277,381 @@ __global__ __launch_bounds__(64, 3) void k_boundary(FluxArgs a) {
   }
 }
 
+// ===========================================================================
+// Temperature Jacobians on the boundary system (src/FluxSolver.cpp:1294-1497
+// computes them by back-substitution on its band LU).  With the optical
+// properties fixed, a unit Planck value at level d changes only the thermal
+// particular solution of the layers above and below level d (and, for the
+// surface degree of freedom d = L+1, the bottom boundary row), so every
+// degree of freedom is one more right-hand side of the same boundary
+// system: k_jac_boundary eliminates K of them alongside one factorisation,
+// k_jac_flux turns the mode amplitudes into the six flux and mean-intensity
+// outputs and scales them by dB/dT.
+// ===========================================================================
+
+// Unit-source particular solution (BC form, row q) of layer lc of instance i
+// at its top or bottom, for the degree of freedom d: the layer's Planck fit
+// B(tau) = xr0 + xr1 tau through (B_top, B_bot) = ([d == lc], [d == lc+1])
+// (src/FluxSolver.cpp:755-776), applied to the stored basis.
+template <int NN>
+__device__ __forceinline__ double jac_zunit(const FluxArgs& a, int i, int64_t lo, int64_t vo, int lc, int d, int q,
+                                            bool bottom, double* pls = nullptr) {
+  if (d != lc && d != lc + 1) {
+    if (pls) *pls = 0.0;
+    return 0.0;
+  }
+  const int64_t TL = a.total_layers, TV = a.total_levels;
+  const double pt = (d == lc) ? 1.0 : 0.0, pb = 1.0 - pt;
+  const double dts = a.lay_d[LD_DTAU_S * TL + lo + lc];
+  const double taut = a.lev_d[VD_TAU_S * TV + vo + lc];
+  const double taub = a.lev_d[VD_TAU_S * TV + vo + lc + 1];
+  const double xr1 = (dts > 1e-4) ? (pb - pt) / dts : 0.0;
+  const double xr0 = pt - xr1 * taut;
+  if (pls) *pls = xr0 + xr1 * taub;
+  const double* jb = a.jbase + ((int64_t)i * a.jac_nlyr + lc) * 2 * NN;
+  const int r = (q >= NN) ? q - NN : NN - 1 - q;
+  const double d0 = xr1 * jb[NN + r];
+  const double z0 = 0.5 * (xr0 * jb[r] + ((q >= NN) ? d0 : -d0));
+  const double z1 = 0.5 * xr1 * jb[r];
+  return z0 + z1 * (bottom ? taub : taut);
+}
+
+// One LPI-lane group per (instance, group of K degrees of freedom): the
+// panel elimination of k_boundary with K right-hand sides carried through
+// every step, then K back substitutions.
+template <int NN, int K>
+__global__ __launch_bounds__(64, 2) void k_jac_boundary(FluxArgs a) {
+  using C = RowCfg<NN>;
+  constexpr int N = C::N, R = C::R, LPI = C::LPI, G = C::G, W = 2 * N + K;
+  constexpr EigOff eo(N);
+  constexpr TabOff to(N);
+  constexpr int PS = 2 * N * N + K * N;
+  constexpr int ES = eo.stride();
+  __shared__ double sRow[G * W];
+  __shared__ double sE[G * 2 * NN];
+  __shared__ double sRS[G * N];
+
+  const int L = a.jac_nlyr, D = L + 2, NG = (D + K - 1) / K;
+  const int g = threadIdx.x / LPI;
+  const int s = threadIdx.x % LPI;
+  const bool slot = s < R;
+  const int64_t nq = (int64_t)a.nb * NG;
+  const int64_t q0 = (int64_t)blockIdx.x * G + g;
+  const bool live = q0 < nq;
+  const int64_t qq = live ? q0 : nq - 1;  // a spare group repeats the last task, stores nothing
+  const int bk = (int)(qq / NG), dg = (int)(qq % NG);
+  const int i = a.b_inst[bk];
+  const int64_t t0 = a.b_task0[bk];
+  const int64_t lo = a.lay_off[i], vo = lo + i;
+  const int64_t TL = a.total_layers;
+  const double* sc = a.scal + (int64_t)i * DISORT_NSCAL;
+  const double* tb = a.tab;
+  const int M = a.inst_i[i * kInstI + II_NCUT];
+  const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
+  const bool diffusion = (a.flags & DISORT_F_DIFFUSION_BC) != 0;
+  const double alb = sc[DISORT_S_ALBEDO];
+  double* const sr = sRow + g * W;
+  double* const se = sE + g * 2 * NN;
+  double* const pbase = a.jpanel + ((int64_t)i * NG + dg) * (int64_t)L * PS;
+
+  if (live && s < K && dg * K + s < D) {
+    const int d = dg * K + s;
+    const double T = (d <= L) ? a.temper[vo + d] : sc[DISORT_S_TEMP_BOT];
+    a.jdbdt[(int64_t)i * D + d] = planck_band_deriv(sc[DISORT_S_WN_LO], sc[DISORT_S_WN_HI], T);
+  }
+  if (live && dg == 0 && s == 0) a.jstatus[a.jac_c0 + i] = a.inst_i[i * kInstI + II_STATUS];
+
+  auto exps = [&](int64_t ta, int64_t tb2) {
+    wave_lds_order();
+    if (s < 2 * NN) {
+      const int64_t t = (s < NN) ? ta : tb2;
+      const int jp = (s < NN) ? s : s - NN;
+      se[s] = exp(-a.eig[eo.at(t, eo.k() + jp)] * a.lay_d[LD_DTAU_S * TL + lo + (t - t0)]);
+    }
+    wave_lds_order();
+  };
+
+  double A0[N], A1[N], rhs[K];
+#pragma unroll
+  for (int j = 0; j < N; ++j) A0[j] = A1[j] = 0.0;
+#pragma unroll
+  for (int kk = 0; kk < K; ++kk) rhs[kk] = 0.0;
+
+  exps(t0, t0);
+  if (s < NN) {
+    g_row<NN>(a.eig + t0 * ES, NN - 1 - s, A1);
+#pragma unroll
+    for (int j = 0; j < NN; ++j) A1[j] *= se[NN - 1 - j];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) rhs[kk] = -jac_zunit<NN>(a, i, lo, vo, 0, dg * K + kk, NN - 1 - s, false);
+  }
+  bool used = slot && s >= NN;
+
+  auto lu_panel = [&](bool excluded, double* pp) {
+    int myc = -1;
+    double myinv = 0.0;
+    static_for<0, N>([&](auto cc) {
+      constexpr int c = decltype(cc)::value;
+      const bool cand = slot && !used && !excluded;
+      const double key = cand ? fabs(A0[c]) : -1.0;
+      const double m = group_max<LPI>(key);
+      const int p = __ffs(group_bits<LPI>(__ballot(key == m), g)) - 1;
+      wave_lds_order();
+      if (s == p) {
+#pragma unroll
+        for (int k = c; k < N; ++k) sr[k] = A0[k];
+#pragma unroll
+        for (int k = 0; k < N; ++k) sr[N + k] = A1[k];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) sr[2 * N + kk] = rhs[kk];
+      }
+      wave_lds_order();
+      const double piv = sr[c];
+      const double inv = (piv != 0.0) ? 1.0 / (piv != 0.0 ? piv : 1.0) : 0.0;
+      if (s == p) {
+        used = true;
+        myc = c;
+        myinv = inv;
+      } else if (cand) {
+        const double l = A0[c] * inv;
+#pragma unroll
+        for (int k = c + 1; k < N; ++k) A0[k] -= l * sr[k];
+#pragma unroll
+        for (int k = 0; k < N; ++k) A1[k] -= l * sr[N + k];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) rhs[kk] -= l * sr[2 * N + kk];
+      }
+    });
+    if (live && myc >= 0) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        pp[myc * N + j] = (j == myc) ? myinv : A0[j];
+        pp[N * N + myc * N + j] = A1[j];
+      }
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) pp[2 * N * N + kk * N + myc] = rhs[kk];
+    }
+  };
+
+  for (int ip = 0; ip < M - 1; ++ip) {
+    const int64_t tt = t0 + ip;
+    exps(tt, tt + 1);
+    const unsigned ub = group_bits<LPI>(__ballot(used), g);
+    if (used) {
+      const int q = __popc(ub & ((1u << s) - 1u));
+      g_row<NN>(a.eig + tt * ES, q, A0);
+      g_row<NN>(a.eig + (tt + 1) * ES, q, A1);
+#pragma unroll
+      for (int j = 0; j < NN; ++j) {
+        A0[NN + j] *= se[j];
+        A1[j] = -A1[j] * se[2 * NN - 1 - j];
+        A1[NN + j] = -A1[NN + j];
+      }
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        const int d = dg * K + kk;
+        rhs[kk] = jac_zunit<NN>(a, i, lo, vo, ip + 1, d, q, false) - jac_zunit<NN>(a, i, lo, vo, ip, d, q, true);
+      }
+      used = false;
+    } else {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        A0[j] = A1[j];
+        A1[j] = 0.0;
+      }
+    }
+    lu_panel(false, pbase + (int64_t)ip * PS);
+  }
+
+  {
+    const int64_t tt = t0 + M - 1;
+    const double* rec = a.eig + tt * ES;
+    exps(tt, tt);
+    const bool refl = !(lyrcut || diffusion);
+    if (s < N) {
+      double rs = 0.0;
+#pragma unroll
+      for (int k = 0; k < NN; ++k) {
+        const double wk = tb[to.cwt() + k] * tb[to.cmu() + k] * alb;
+        rs += wk * g_entry<NN>(a, tt, NN - 1 - k, s);
+      }
+      sRS[g * N + s] = refl ? rs : 0.0;
+    }
+    wave_lds_order();
+    const double dl = a.lay_d[LD_DTAU_S * TL + lo + L - 1];
+    const unsigned ub = group_bits<LPI>(__ballot(used), g);
+    const int q = __popc(ub & ((1u << s) - 1u));
+    const bool bot = used && q < NN;
+    const bool excluded = used && !bot;
+    if (bot) {
+      g_row<NN>(rec, NN + q, A0);
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        A0[j] = (A0[j] - 2.0 * sRS[g * N + j]) * ((j < NN) ? 1.0 : se[j - NN]);
+        A1[j] = 0.0;
+      }
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        const int d = dg * K + kk;
+        double zrefl = 0.0;
+        if (refl) {
+#pragma unroll
+          for (int k = 0; k < NN; ++k)
+            zrefl += tb[to.cwt() + k] * tb[to.cmu() + k] * alb *
+                     jac_zunit<NN>(a, i, lo, vo, M - 1, d, NN - 1 - k, true);
+        }
+        const double plb = (d == L + 1) ? 1.0 : 0.0;
+        double extra = 0.0;
+        if (refl) extra += 2.0 * zrefl;
+        extra += (1.0 - alb) * plb;
+        double plbd = 0.0;
+        if (diffusion && dl > 0.0) plbd = (plb - ((d == L - 1) ? 1.0 : 0.0)) / dl;
+        rhs[kk] = -jac_zunit<NN>(a, i, lo, vo, M - 1, d, NN + q, true) +
+                  (diffusion ? plb + tb[to.cmu() + q] * plbd : extra);
+      }
+      used = false;
+    } else if (!used) {
+#pragma unroll
+      for (int j = 0; j < N; ++j) {
+        A0[j] = A1[j];
+        A1[j] = 0.0;
+      }
+    }
+    lu_panel(excluded, pbase + (int64_t)(M - 1) * PS);
+  }
+
+  // K back substitutions, lane = U row k of each panel.
+  const int k = (s < N) ? s : 0;
+  double xn[K];
+#pragma unroll
+  for (int kk = 0; kk < K; ++kk) xn[kk] = 0.0;
+  for (int ip = M - 1; ip >= 0; --ip) {
+    const double* pp = pbase + (int64_t)ip * PS;
+    double v[K];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) v[kk] = pp[2 * N * N + kk * N + k];
+    if (ip < M - 1) {
+      static_for<0, N>([&](auto cj) {
+        constexpr int c = decltype(cj)::value;
+        const double u = pp[N * N + k * N + c];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) v[kk] -= u * row_bcast<c>(xn[kk]);
+      });
+    }
+    const double rdg = pp[k * N + k];
+    double xk[K];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) xk[kk] = 0.0;
+    static_for<0, N>([&](auto cj) {
+      constexpr int c = N - 1 - decltype(cj)::value;
+      const double u = pp[k * N + c];
+#pragma unroll
+      for (int kk = 0; kk < K; ++kk) {
+        const double xc = row_bcast<c>(v[kk] * rdg);
+        xk[kk] = (k == c) ? xc : xk[kk];
+        v[kk] -= u * xc;
+      }
+    });
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) {
+      xn[kk] = xk[kk];
+      const int d = dg * K + kk;
+      if (live && s < N && d < D) a.jcvec[(((int64_t)i * D + d) * L + ip) * N + k] = xk[kk];
+    }
+  }
+}
+
+// A group of LPL lanes per (level task, degree of freedom), consecutive
+// groups on consecutive degrees of freedom of one level: the k_flux
+// arithmetic with the unit-source amplitudes and particular solution, no
+// beam, times dB/dT, written as [column][6][L+1][L+2].
+template <int NN>
+__global__ __launch_bounds__(64) void k_jac_flux(FluxArgs a) {
+  constexpr int N = 2 * NN;
+  constexpr EigOff eo(N);
+  constexpr TabOff to(N);
+  constexpr int LPL = FluxLanes<NN>::LPL, LPW = FluxLanes<NN>::LPW;
+  __shared__ double sY[LPW * 2 * LPL];
+  const int L = a.jac_nlyr, D = L + 2;
+  const int lv = threadIdx.x / LPL;
+  const int r = threadIdx.x % LPL;
+  const int64_t q = (int64_t)blockIdx.x * LPW + lv;
+  if (q >= a.n_lvtasks * D) return;  // whole groups leave together
+  const int64_t vt = q / D;
+  const int d = (int)(q - vt * D);
+  const int bk = a.v_bk[vt];
+  const int lu = a.v_lu[vt];
+  const int i = a.b_inst[bk];
+  const int64_t t0 = a.b_task0[bk];
+  const int64_t lo = a.lay_off[i], vo = lo + i;
+  const int64_t TL = a.total_layers;
+  const double* tb = a.tab;
+  const int ncut = a.inst_i[i * kInstI + II_NCUT];
+  const bool lyrcut = a.inst_i[i * kInstI + II_LYRCUT] != 0;
+  int lyu = (lu == 0) ? 0 : lu - 1;
+  double o[6];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) o[f] = 0.0;
+  if (!(lyrcut && lyu > ncut - 1)) {
+    if (lyu >= ncut) lyu = ncut - 1;
+    const int64_t tt = t0 + lyu;
+    const double* rec = a.eig + eo.at(tt, 0);
+    const double* cv = a.jcvec + (((int64_t)i * D + d) * L + lyu) * N;
+    const double dts = a.lay_d[LD_DTAU_S * TL + lo + lyu];
+    const bool act = r < NN;
+    const int rr = act ? r : 0;
+    double* y = sY + lv * 2 * LPL;
+    {
+      const double e = exp(-rec[eo.k() + rr] * dts);
+      const double yd = cv[NN - 1 - rr] * ((lu == 0) ? e : 1.0);
+      const double yu = cv[NN + rr] * ((lu == 0) ? 1.0 : e);
+      wave_lds_order();
+      if (act) {
+        y[r] = yd;
+        y[LPL + r] = yu;
+      }
+      wave_lds_order();
+    }
+    double pls = 0.0;
+    double iu = jac_zunit<NN>(a, i, lo, vo, lyu, d, NN + rr, lu != 0, &pls);
+    double id = jac_zunit<NN>(a, i, lo, vo, lyu, d, NN - 1 - rr, lu != 0);
+    if (lu == 0) pls = (d == 0) ? 1.0 : 0.0;
+    constexpr int kUnroll = NN <= 16 ? NN : 8;
+#pragma unroll kUnroll
+    for (int jp = 0; jp < NN; ++jp) {
+      const double yd = y[jp], yu = y[LPL + jp];
+      const double gp = rec[eo.gp() + rr + jp * NN];
+      const double gm = rec[eo.gm() + rr + jp * NN];
+      iu += gp * yu - gm * yd;
+      id += gm * yu - gp * yd;
+    }
+    const double w = act ? tb[to.cwt() + rr] : 0.0, cm = tb[to.cmu() + rr];
+    double mi = group_sum<LPL>(w * id + w * iu);
+    double mid = group_sum<LPL>(w * id);
+    double miu = group_sum<LPL>(w * iu);
+    double fldn = group_sum<LPL>(w * id * cm);
+    double flup = group_sum<LPL>(w * iu * cm);
+    flup *= 2.0 * kPi;
+    fldn *= 2.0 * kPi;
+    mi = (2.0 * kPi * mi) / (4.0 * kPi);
+    mid = (2.0 * kPi * mid) / (4.0 * kPi);
+    miu = (2.0 * kPi * miu) / (4.0 * kPi);
+    o[0] = flup;
+    o[1] = fldn;
+    o[2] = mi;
+    o[3] = mid;
+    o[4] = miu;
+    o[5] = (1.0 - a.ssa[lo + lyu]) * 4.0 * kPi * (mi - pls);
+  }
+  if (r == 0) {
+    const double sdb = a.jdbdt[(int64_t)i * D + d];
+    const int64_t per = (int64_t)(L + 1) * D;
+    double* jc = a.jac + (a.jac_c0 + i) * 6 * per + (int64_t)lu * D + d;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) jc[f * per] = sdb * o[f];
+  }
+}
+
 }  // namespace
 
 // Host-callable launchers, one set per stream count.
@@ -1281,6 +1666,19 @@ static void launch_all(const FluxArgs& a, hipStream_t s, hipEvent_t* ev, bool pa
   else
     hipLaunchKernelGGL(k_layer<NN>, dim3((unsigned)((a.n_tasks + 63) / 64)), dim3(64), 0, s, a);
   if (ev) (void)hipEventRecord(ev[2], s);
+  if (a.jbase) {
+    constexpr int K = kJacRhs;
+    const int64_t NG = (a.jac_nlyr + 2 + K - 1) / K;
+    const int64_t groups = (int64_t)a.nb * NG;
+    hipLaunchKernelGGL((k_jac_boundary<NN, K>), dim3((unsigned)((groups + RowCfg<NN>::G - 1) / RowCfg<NN>::G)), dim3(64),
+                       0, s, a);
+    if (ev) (void)hipEventRecord(ev[3], s);
+    constexpr int LPW = FluxLanes<NN>::LPW;
+    const int64_t ft = a.n_lvtasks * (a.jac_nlyr + 2);
+    hipLaunchKernelGGL(k_jac_flux<NN>, dim3((unsigned)((ft + LPW - 1) / LPW)), dim3(64), 0, s, a);
+    if (ev) (void)hipEventRecord(ev[4], s);
+    return;
+  }
   hipLaunchKernelGGL(k_boundary<NN>, dim3((unsigned)((a.nb + RowCfg<NN>::G - 1) / RowCfg<NN>::G)), dim3(64), 0, s, a);
   if (ev) (void)hipEventRecord(ev[3], s);
   launch_flux_levels<NN>(a, s);

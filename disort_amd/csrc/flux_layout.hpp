@@ -87,4 +87,10 @@ struct EigOff {
 // task*stride + [R (N*N col-major) | U (N*N col-major) | r (N)].
 __host__ __device__ constexpr int panel_stride(int n) { return 2 * n * n + n; }
 
+// Temperature Jacobians: right-hand sides eliminated per factorisation of
+// the boundary system, and the panel record of that solve, U (2 N*N) then
+// the K eliminated right-hand sides (K*N).
+constexpr int kJacRhs = 8;
+__host__ __device__ constexpr int jac_panel_stride(int n) { return 2 * n * n + kJacRhs * n; }
+
 }  // namespace dsa

@@ -178,15 +178,23 @@ int disort_flux_status(disort_ctx* ctx, disort_flux_plan* plan, int32_t* status_
  * arrays.  disort_jac_execute reads the batch's dtau, ssa, pmom, temper and
  * scal (device pointers; values are not validated) and writes the device
  * array jac [n][6][nlyr+1][nlyr+2] in the layout of disort_flux_jacobian,
- * asynchronously on the context stream.  The plan expands up to
- * max_expanded (0: 131072) column x degree-of-freedom instances at a time
- * into buffers it owns and solves them with an inner flux plan
- * (disort_jac_plan_flux: its schedule setters apply); disort_jac_plan_chunk
- * gives the columns per expansion.  flags: DISORT_F_THERMAL (required),
- * DISORT_F_DELTA_M_PLUS, DISORT_F_DIFFUSION_BC.  disort_jac_status copies
- * the per-column DISORT_ST_* codes of the last execute (the first nonzero
- * code among the column's solves) to a host array [n].
+ * asynchronously on the context stream.
+ *
+ * Factored form (nstr <= 16, the default): per column one eigen-system
+ * pass, then the nlyr+2 unit-source right-hand sides of the boundary system
+ * eliminated eight at a time alongside its factorisation, the reference's
+ * back-substitution route.  Expansion form (DISORT_JAC_EXPAND, and nstr >
+ * 16): one forward solve per column x degree of freedom.  Columns are
+ * processed in chunks of disort_jac_plan_chunk (at most max_expanded / (nlyr
+ * + 2) columns when max_expanded > 0; by default 131072 expanded instances,
+ * or a 4 GiB buffer budget in the factored form) with an inner flux plan
+ * (disort_jac_plan_flux: its schedule setters apply).  flags:
+ * DISORT_F_THERMAL (required), DISORT_F_DELTA_M_PLUS, DISORT_F_DIFFUSION_BC,
+ * DISORT_JAC_EXPAND.  disort_jac_status copies the per-column DISORT_ST_*
+ * codes of the last execute (expansion: the first nonzero code among the
+ * column's solves) to a host array [n].
  */
+#define DISORT_JAC_EXPAND 0x100
 typedef struct disort_jac_plan disort_jac_plan;
 int disort_jac_plan_create(disort_ctx* ctx, int64_t n, int32_t nstr, int32_t nlyr, int32_t pstride, int32_t flags,
                            int64_t max_expanded, disort_jac_plan** plan);
